@@ -1,0 +1,200 @@
+/*
+ * gs_hip.h — the C ABI of the MI355X (gfx950) 3D Gaussian Splatting trainer.
+ *
+ * Plain C: opaque handles, POD structs, device pointers as plain pointers,
+ * sizes as integers, int status codes (0 = ok, otherwise GS_HIP_E_*) and a
+ * thread-local message from gs_hip_last_error().  Nothing here throws.
+ *
+ * Layers:
+ *   1. device and memory (gs_hip_init ... gs_hip_synchronize)
+ *   2. the workspace: every device buffer one view-step needs, sized for a
+ *      slot capacity and an image size (the role of the reference's
+ *      VkContext-owned Trainer buffers, engine/src/trainer.cpp:303-386)
+ *   3. the stages of one training step (engine/src/trainer.cpp:665-776)
+ *   4. the trainer handle: the gs::Trainer C++ class behind a C surface, the
+ *      shape a JNI/ctypes binding uses (app/.../NativeTrainer.kt:8-47)
+ *
+ * Threading: a workspace or trainer is used by one thread at a time; calls
+ * on one handle are not thread-safe (engine/src/vk_context.cpp:583-586).
+ */
+#ifndef GS_HIP_H
+#define GS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define GS_HIP_OK 0
+#define GS_HIP_E_HIP 1         /* a HIP runtime call failed */
+#define GS_HIP_E_ARGUMENT 2    /* invalid argument (std::invalid_argument in C++) */
+#define GS_HIP_E_CAPACITY 3    /* more than a buffer or format limit holds */
+#define GS_HIP_E_IO 4          /* file or format error (std::runtime_error in C++) */
+#define GS_HIP_E_STATE 5       /* call out of order (std::logic_error in C++) */
+
+/* ---- constants (engine/shaders/train_common.glsl:59-70) ------------------ */
+#define GS_HIP_TILE 16
+#define GS_HIP_MAX_SH_REST 15     /* degree 3: 15 coefficients beyond DC */
+#define GS_HIP_PARAM_FLOATS 16    /* per slot: [pos, logit | logscale, alive | quat wxyz | dc, -] */
+#define GS_HIP_RECORD_FLOATS 12   /* projected raster record per slot */
+#define GS_HIP_GRAD2D_FLOATS 9    /* dL/d(mean xy, conic abc, opacity, colour rgb), planar */
+
+/* frame flags */
+#define GS_HIP_FLAG_MASKED_TARGET 1u  /* target = mix(bg, photo, alpha) (train_forward.comp:124-127) */
+#define GS_HIP_FLAG_SPARSE_ADAM 2u    /* gaussians that drew nothing keep value and moments (train_step.glsl:223-230) */
+#define GS_HIP_FLAG_DEPTH16 4u        /* order pairs by the reference's 16-bit log depth (train_project.comp:110)
+                                         instead of the float bits of z */
+
+/* Pinhole camera, scene -> camera, OpenCV axes (x right, y down, z forward).
+ * Same fields as gs::Camera (engine/include/gs/camera.h:10-16). */
+typedef struct gs_hip_camera {
+  float rotation[9];    /* row-major */
+  float translation[3];
+  float position[3];    /* camera centre in scene coordinates */
+  float fx, fy, cx, cy;
+  uint32_t width, height;
+} gs_hip_camera;
+
+/* Everything one view-step's kernels read besides buffers: the role of the
+ * reference's FrameData uniform (engine/src/trainer.cpp:47-64, filled by
+ * Trainer::write_frame, 440-547).  Learning rates are final (the position
+ * rate already scaled by the camera extent and decayed); bias1/bias2 are
+ * 1/(1 - beta^t) computed on the host in double. */
+typedef struct gs_hip_frame {
+  gs_hip_camera cam;
+  uint32_t tiles_x, tiles_y;
+  uint32_t slots;        /* slots in use (alive + free) */
+  uint32_t sh_degree;    /* active SH degree, <= the stored degree */
+  uint32_t flags;        /* GS_HIP_FLAG_* */
+  uint32_t seed;         /* split sampling seed (train_densify_spawn.comp:113) */
+  float background[3];
+  float lambda_dssim;    /* loss = (1 - lambda) L1 + lambda (1 - SSIM) */
+  float ssim_c1, ssim_c2;
+  float lr_position, lr_scale, lr_rotation, lr_opacity, lr_sh_dc, lr_sh_rest;
+  float beta1, beta2, epsilon, bias1, bias2;
+  float grad_threshold;   /* densify: mean view-space gradient to clone or split */
+  float clone_max_scale;  /* clone at or below this max scale, split above */
+  float prune_opacity;    /* prune below this opacity */
+  float prune_max_scale;  /* > 0: also prune larger world scales */
+  float prune_max_radius; /* > 0: also prune larger screen radii */
+  float split_divisor;    /* split halves: scale / this (1.6) */
+} gs_hip_frame;
+
+/* ---- 1. device and memory ------------------------------------------------ */
+int gs_hip_init(int device);
+int gs_hip_device_name(char* buffer, size_t size);
+int gs_hip_malloc(void** ptr, size_t bytes);
+int gs_hip_free(void* ptr);
+int gs_hip_upload(void* device_dst, const void* host_src, size_t bytes);
+int gs_hip_download(void* host_dst, const void* device_src, size_t bytes);
+int gs_hip_memset(void* device_dst, int value, size_t bytes);
+int gs_hip_synchronize(void);
+/* Copies the calling thread's last error message (NUL-terminated); returns its status code. */
+int gs_hip_last_error(char* buffer, size_t size);
+
+/* ---- 2. workspace --------------------------------------------------------- */
+typedef struct gs_hip_workspace gs_hip_workspace;
+
+/* Buffers of a workspace, for gs_hip_workspace_buffer.  Layouts:
+ *  PARAMS, M_PARAMS, V_PARAMS   float[capacity][16]
+ *  SH, M_SH, V_SH               float[sh_coeffs][capacity][3]   (coefficient-major planes)
+ *  STATS                        float[capacity][4]  (grad norm sum, views, max radius, -)
+ *  RECORDS                      float[capacity][12] (mean xy, conic abc, opacity, rgb, bbox lo/hi, radius|clamp)
+ *  FOOTPRINT                    float[capacity][4]  (dilated cov2D abc, threshold)
+ *  TILES, DEPTH_KEYS, DEPTH_ORDER, OFFSETS  uint32[capacity]  (OFFSETS in depth order)
+ *  GRAD2D                       float[9][capacity]  (planar)
+ *  PAIR_TILES, PAIR_GAUSSIANS   uint32[pair_capacity] (sorted)
+ *  RANGES                       uint32[tiles + 1][2]
+ *  IMAGE                        float[H*W][4]  (rendered rgb, final transmittance)
+ *  LAST                         uint32[H*W]    (1-based position of the last contributor)
+ *  TARGET                       uint32[H*W]    (RGBA8)
+ *  DLDC                         float[H*W][4]  (dL/dC rgb, -)
+ *  TILE_LOSS                    float[tiles][4] (sum |e|, sum e^2, sum SSIM, -)
+ *  TOTALS                       uint32[4]      ([0] pairs of the last bin/sort) */
+enum gs_hip_buffer {
+  GS_HIP_BUF_PARAMS = 0,
+  GS_HIP_BUF_SH,
+  GS_HIP_BUF_M_PARAMS,
+  GS_HIP_BUF_V_PARAMS,
+  GS_HIP_BUF_M_SH,
+  GS_HIP_BUF_V_SH,
+  GS_HIP_BUF_STATS,
+  GS_HIP_BUF_RECORDS,
+  GS_HIP_BUF_FOOTPRINT,
+  GS_HIP_BUF_TILES,
+  GS_HIP_BUF_DEPTH_KEYS,
+  GS_HIP_BUF_DEPTH_ORDER,
+  GS_HIP_BUF_OFFSETS,
+  GS_HIP_BUF_GRAD2D,
+  GS_HIP_BUF_PAIR_TILES,
+  GS_HIP_BUF_PAIR_GAUSSIANS,
+  GS_HIP_BUF_RANGES,
+  GS_HIP_BUF_IMAGE,
+  GS_HIP_BUF_LAST,
+  GS_HIP_BUF_TARGET,
+  GS_HIP_BUF_DLDC,
+  GS_HIP_BUF_TILE_LOSS,
+  GS_HIP_BUF_TOTALS,
+  GS_HIP_BUF_COUNT
+};
+
+/* sh_degree: the stored SH degree (0..3), fixing the per-slot SH planes. */
+int gs_hip_workspace_create(gs_hip_workspace** ws, uint32_t capacity, int sh_degree, uint32_t width,
+                            uint32_t height);
+int gs_hip_workspace_destroy(gs_hip_workspace* ws);
+int gs_hip_workspace_buffer(gs_hip_workspace* ws, int which, void** device_ptr, size_t* bytes);
+/* Pair buffers for at least `pairs` (tile, gaussian) pairs; grows, never shrinks. */
+int gs_hip_workspace_reserve_pairs(gs_hip_workspace* ws, uint64_t pairs);
+/* Copies `count` slots of parameters (16 floats each) and SH rest coefficients
+ * ([slot][coeff][rgb], `sh_coeffs_in` per slot) in at slot 0; zeroes moments and stats. */
+int gs_hip_workspace_load(gs_hip_workspace* ws, const float* params, const float* sh_rest, uint32_t count,
+                          uint32_t sh_coeffs_in);
+/* Copies slots [0, count) out: params (16 per slot), SH [slot][coeff][rgb]; either may be NULL. */
+int gs_hip_workspace_store(gs_hip_workspace* ws, float* params, float* sh_rest, uint32_t count);
+
+/* ---- 3. stages of one view-step ------------------------------------------ */
+/* EWA projection, SH colour, footprint tile count and depth key per slot;
+ * zeroes GRAD2D (train_project.comp:26-112). */
+int gs_hip_project(gs_hip_workspace* ws, const gs_hip_frame* frame);
+/* Depth sort of the slots, prefix of their tile counts in depth order, pair
+ * emission, stable tile sort of the pairs, tile ranges (trainer.cpp:665-711).
+ * Grows the pair buffers when needed; *pairs = P. */
+int gs_hip_bin(gs_hip_workspace* ws, const gs_hip_frame* frame, uint64_t* pairs);
+/* Front-to-back compositing per 16x16 tile: IMAGE, LAST (train_forward.comp:59-141). */
+int gs_hip_forward(gs_hip_workspace* ws, const gs_hip_frame* frame);
+/* Uploads a RGBA8 target (width*height*4 bytes) into TARGET. */
+int gs_hip_set_target(gs_hip_workspace* ws, const uint8_t* rgba8);
+/* Copies a RGBA8 target already in device memory into TARGET, in stream order. */
+int gs_hip_set_target_device(gs_hip_workspace* ws, const void* device_rgba8);
+/* Loss: SSIM maps and tile sums, then dL/dC of (1-lambda) L1 + lambda (1-SSIM)
+ * into DLDC (train_forward.comp:119-140, train_ssim.comp, train_ssim_grad.comp).
+ * losses (may be NULL): sum |e|, sum e^2, sum SSIM over pixels and channels. */
+int gs_hip_loss(gs_hip_workspace* ws, const gs_hip_frame* frame, double losses[3]);
+/* Backward rasterizer: re-walks each tile back to front from the final
+ * transmittance and accumulates dL/d(2D splat) into GRAD2D (train_backward_serial.glsl). */
+int gs_hip_backward(gs_hip_workspace* ws, const gs_hip_frame* frame);
+/* Projection adjoint, densify statistics, fused Adam on the 16 parameters and
+ * the SH coefficients (train_step.glsl:72-240, train_step_sh.comp:21-44). */
+int gs_hip_adam(gs_hip_workspace* ws, const gs_hip_frame* frame);
+/* Densification: prune / clone / split by stream compaction.  *slots is the
+ * slots in use (updated); counts = {freed, spawned placed, pruned}
+ * (train_densify_mark.comp, train_densify_spawn.comp, trainer.cpp:1083-1119). */
+int gs_hip_densify(gs_hip_workspace* ws, const gs_hip_frame* frame, uint32_t* slots, uint32_t counts[3]);
+/* One full training view-step: project, bin, forward, loss, backward, adam. */
+int gs_hip_train_step(gs_hip_workspace* ws, const gs_hip_frame* frame, uint64_t* pairs, double losses[3]);
+/* Forward only (render): project, bin, forward; image as RGBA8 (clamped rgb, alpha = 1 - T). */
+int gs_hip_render(gs_hip_workspace* ws, const gs_hip_frame* frame, uint8_t* rgba8_out);
+
+/* Kernel timing of the last stage calls on the workspace's stream, in ms
+ * (hipEvents): names[i] / ms[i] for i < *count (at most `capacity`). */
+int gs_hip_workspace_timings(gs_hip_workspace* ws, int enable, const char** names, double* ms, int capacity,
+                             int* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GS_HIP_H */

@@ -1,0 +1,234 @@
+// Loss: (1 - lambda) L1 + lambda (1 - SSIM) between the render
+// x = C + T bg and the target y, per channel, 11x11 Gaussian window
+// (sigma 1.5), zero padding outside the image (engine/shaders/train_ssim.comp,
+// train_ssim_grad.comp, train_forward.comp:119-140).
+//
+// k_ssim: one 16x16 tile per workgroup; stages the 26x26 halo of x and y of
+// one channel in LDS, blurs x, y, x^2, y^2, xy along rows then columns, and
+// writes per pixel and channel the parts of the SSIM gradient
+//   A = dSSIM/dmu_x (through the variances too), B = dSSIM/dE_xx, C = dSSIM/dE_xy
+// plus the tile's sums of |e|, e^2 and SSIM.
+// k_loss_grad: blurs A, B, C back with the same window and forms
+//   dL/dx_q = (1 - lambda) sign(e_q) / (3HW)
+//             - lambda / (3HW) ((G*A)(q) + 2 x_q (G*B)(q) + y_q (G*C)(q)).
+#include "gs_common.h"
+#include "gs_internal.h"
+#include "wave.h"
+
+namespace {
+
+using namespace gsd;
+
+constexpr int kHalo = kTile + 2 * kSsimRadius;  // 26
+
+struct LossArgs {
+  const float4* __restrict__ image;
+  const uint32_t* __restrict__ target;
+  float* __restrict__ maps;  // [9][pixels]
+  float4* __restrict__ dldc;
+  float4* __restrict__ tile_loss;
+};
+
+// Render and target colour of pixel (x, y), channel c; zero outside the image.
+__device__ __forceinline__ void load_xy(const gs_hip_frame& f, const LossArgs& a, int x, int y, float& rx, float& ry,
+                                        float rgb_x[3], float rgb_y[3]) {
+  (void)rx;
+  (void)ry;
+  if (x < 0 || y < 0 || x >= int(f.cam.width) || y >= int(f.cam.height)) {
+    rgb_x[0] = rgb_x[1] = rgb_x[2] = 0.0f;
+    rgb_y[0] = rgb_y[1] = rgb_y[2] = 0.0f;
+    return;
+  }
+  const uint32_t id = uint32_t(y) * f.cam.width + uint32_t(x);
+  const float4 im = a.image[id];
+  rgb_x[0] = im.x + im.w * f.background[0];
+  rgb_x[1] = im.y + im.w * f.background[1];
+  rgb_x[2] = im.z + im.w * f.background[2];
+  const uint32_t t = a.target[id];
+  const float tr = float(t & 0xffu) * (1.0f / 255.0f), tg = float((t >> 8) & 0xffu) * (1.0f / 255.0f);
+  const float tb = float((t >> 16) & 0xffu) * (1.0f / 255.0f), ta = float(t >> 24) * (1.0f / 255.0f);
+  if ((f.flags & GS_HIP_FLAG_MASKED_TARGET) != 0u) {
+    rgb_y[0] = f.background[0] * (1.0f - ta) + tr * ta;
+    rgb_y[1] = f.background[1] * (1.0f - ta) + tg * ta;
+    rgb_y[2] = f.background[2] * (1.0f - ta) + tb * ta;
+  } else {
+    rgb_y[0] = tr, rgb_y[1] = tg, rgb_y[2] = tb;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_ssim(gs_hip_frame f, LossArgs a) {
+  __shared__ float s_x[3][kHalo * kHalo];
+  __shared__ float s_y[3][kHalo * kHalo];
+  __shared__ float s_h[5][kHalo * kTile];
+  __shared__ float s_sum[3][4];
+  const uint32_t tile = blockIdx.x;
+  const int tx = int(tile % f.tiles_x), ty = int(tile / f.tiles_x);
+  const int ox = tx * kTile - kSsimRadius, oy = ty * kTile - kSsimRadius;
+  const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+  const int px = tx * kTile + lx, py = ty * kTile + ly;
+  const bool inside = px < int(f.cam.width) && py < int(f.cam.height);
+  const uint32_t pixels = f.cam.width * f.cam.height;
+  for (int e = threadIdx.x; e < kHalo * kHalo; e += 256) {
+    float rx, ry, vx[3], vy[3];
+    load_xy(f, a, ox + e % kHalo, oy + e / kHalo, rx, ry, vx, vy);
+    for (int c = 0; c < 3; ++c) {
+      s_x[c][e] = vx[c];
+      s_y[c][e] = vy[c];
+    }
+  }
+  __syncthreads();
+  const float c1 = f.ssim_c1, c2 = f.ssim_c2;
+  float abs_e = 0.0f, sq_e = 0.0f, ssim_sum = 0.0f;
+  const uint32_t pix = uint32_t(py) * f.cam.width + uint32_t(px);
+  for (int c = 0; c < 3; ++c) {
+    for (int e = threadIdx.x; e < kHalo * kTile; e += 256) {
+      const int row = e / kTile, col = e % kTile;
+      float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f, m3 = 0.0f, m4 = 0.0f;
+      for (int k = 0; k <= 2 * kSsimRadius; ++k) {
+        const float w = kSsimWindow[k];
+        const float x = s_x[c][row * kHalo + col + k], y = s_y[c][row * kHalo + col + k];
+        m0 += w * x;
+        m1 += w * y;
+        m2 += w * x * x;
+        m3 += w * y * y;
+        m4 += w * x * y;
+      }
+      s_h[0][e] = m0;
+      s_h[1][e] = m1;
+      s_h[2][e] = m2;
+      s_h[3][e] = m3;
+      s_h[4][e] = m4;
+    }
+    __syncthreads();
+    float mx = 0.0f, my = 0.0f, exx = 0.0f, eyy = 0.0f, exy = 0.0f;
+    for (int k = 0; k <= 2 * kSsimRadius; ++k) {
+      const float w = kSsimWindow[k];
+      const int e = (ly + k) * kTile + lx;
+      mx += w * s_h[0][e];
+      my += w * s_h[1][e];
+      exx += w * s_h[2][e];
+      eyy += w * s_h[3][e];
+      exy += w * s_h[4][e];
+    }
+    const float vx = exx - mx * mx, vy = eyy - my * my, cxy = exy - mx * my;
+    const float l = 2.0f * mx * my + c1, cs = 2.0f * cxy + c2;
+    const float m = mx * mx + my * my + c1, v = vx + vy + c2;
+    const float inv = 1.0f / (m * v);
+    const float ssim = l * cs * inv;
+    const float d_cxy = 2.0f * l * inv;  // dSSIM/dE_xy
+    const float d_vx = -ssim / v;        // dSSIM/dE_xx
+    const float A = 2.0f * my * cs * inv - ssim * 2.0f * mx / m - 2.0f * mx * d_vx - my * d_cxy;
+    if (inside) {
+      a.maps[size_t(c) * pixels + pix] = A;
+      a.maps[size_t(3 + c) * pixels + pix] = d_vx;
+      a.maps[size_t(6 + c) * pixels + pix] = d_cxy;
+      const int centre = (ly + kSsimRadius) * kHalo + lx + kSsimRadius;
+      const float err = s_x[c][centre] - s_y[c][centre];
+      abs_e += fabsf(err);
+      sq_e += err * err;
+      ssim_sum += ssim;
+    }
+    __syncthreads();
+  }
+  abs_e = wave_sum(abs_e);
+  sq_e = wave_sum(sq_e);
+  ssim_sum = wave_sum(ssim_sum);
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    s_sum[0][wave] = abs_e;
+    s_sum[1][wave] = sq_e;
+    s_sum[2][wave] = ssim_sum;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float r[3];
+    for (int q = 0; q < 3; ++q) r[q] = s_sum[q][0] + s_sum[q][1] + s_sum[q][2] + s_sum[q][3];
+    a.tile_loss[tile] = make_float4(r[0], r[1], r[2], 0.0f);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_loss_grad(gs_hip_frame f, LossArgs a) {
+  __shared__ float s_m[3][kHalo * kHalo];
+  __shared__ float s_h[3][kHalo * kTile];
+  const uint32_t tile = blockIdx.x;
+  const int tx = int(tile % f.tiles_x), ty = int(tile / f.tiles_x);
+  const int ox = tx * kTile - kSsimRadius, oy = ty * kTile - kSsimRadius;
+  const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+  const int px = tx * kTile + lx, py = ty * kTile + ly;
+  const bool inside = px < int(f.cam.width) && py < int(f.cam.height);
+  const uint32_t pixels = f.cam.width * f.cam.height;
+  const uint32_t pix = uint32_t(py) * f.cam.width + uint32_t(px);
+  float rx, ry, x[3] = {0, 0, 0}, y[3] = {0, 0, 0};
+  if (inside) load_xy(f, a, px, py, rx, ry, x, y);
+  const float scale = 1.0f / (3.0f * float(pixels));
+  const float lambda = f.lambda_dssim;
+  float grad[3];
+  for (int c = 0; c < 3; ++c) {
+    const float e = x[c] - y[c];
+    const float sgn = e > 0.0f ? 1.0f : (e < 0.0f ? -1.0f : 0.0f);
+    grad[c] = sgn * ((1.0f - lambda) * scale);
+  }
+  if (lambda != 0.0f) {
+    for (int c = 0; c < 3; ++c) {
+      for (int e = threadIdx.x; e < kHalo * kHalo; e += 256) {
+        const int gx = ox + e % kHalo, gy = oy + e / kHalo;
+        const bool in_image = gx >= 0 && gy >= 0 && gx < int(f.cam.width) && gy < int(f.cam.height);
+        const uint32_t id = uint32_t(gy) * f.cam.width + uint32_t(gx);
+        for (int q = 0; q < 3; ++q) s_m[q][e] = in_image ? a.maps[size_t(3 * q + c) * pixels + id] : 0.0f;
+      }
+      __syncthreads();
+      for (int e = threadIdx.x; e < kHalo * kTile; e += 256) {
+        const int row = e / kTile, col = e % kTile;
+        for (int q = 0; q < 3; ++q) {
+          float sum = 0.0f;
+          for (int k = 0; k <= 2 * kSsimRadius; ++k) sum += kSsimWindow[k] * s_m[q][row * kHalo + col + k];
+          s_h[q][e] = sum;
+        }
+      }
+      __syncthreads();
+      float g[3] = {0.0f, 0.0f, 0.0f};
+      for (int q = 0; q < 3; ++q) {
+        for (int k = 0; k <= 2 * kSsimRadius; ++k) g[q] += kSsimWindow[k] * s_h[q][(ly + k) * kTile + lx];
+      }
+      grad[c] -= (lambda * scale) * (g[0] + 2.0f * x[c] * g[1] + y[c] * g[2]);
+      __syncthreads();
+    }
+  }
+  if (inside) a.dldc[pix] = make_float4(grad[0], grad[1], grad[2], 0.0f);
+}
+
+}  // namespace
+
+extern "C" {
+
+int gs_hip_loss(gs_hip_workspace* ws, const gs_hip_frame* f, double losses[3]) {
+  if (ws == nullptr || f == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "loss: null argument");
+  const uint32_t tiles = f->tiles_x * f->tiles_y;
+  LossArgs a;
+  a.image = reinterpret_cast<const float4*>(ws->image);
+  a.target = ws->target;
+  a.maps = ws->ssim_maps;
+  a.dldc = reinterpret_cast<float4*>(ws->dldc);
+  a.tile_loss = reinterpret_cast<float4*>(ws->tile_loss);
+  gsh::timer_begin(ws, "ssim");
+  k_ssim<<<tiles, 256, 0, ws->stream>>>(*f, a);
+  GS_CHECK_LAUNCH("k_ssim");
+  gsh::timer_end(ws);
+  gsh::timer_begin(ws, "loss_grad");
+  k_loss_grad<<<tiles, 256, 0, ws->stream>>>(*f, a);
+  GS_CHECK_LAUNCH("k_loss_grad");
+  gsh::timer_end(ws);
+  if (losses != nullptr) {
+    GS_HIP_TRY(hipMemcpyAsync(ws->host_tile_loss, ws->tile_loss, size_t(tiles) * 16, hipMemcpyDeviceToHost,
+                              ws->stream));
+    GS_HIP_TRY(hipStreamSynchronize(ws->stream));
+    double s[3] = {0.0, 0.0, 0.0};
+    for (uint32_t t = 0; t < tiles; ++t) {
+      for (int q = 0; q < 3; ++q) s[q] += ws->host_tile_loss[t * 4 + q];
+    }
+    for (int q = 0; q < 3; ++q) losses[q] = s[q];
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,175 @@
+// EWA projection of every slot: screen mean, dilated 2D covariance and its
+// conic, SH colour, opacity-aware footprint tile count, raster bbox and the
+// depth sort key.  Follows engine/shaders/train_project.comp:26-112.
+//
+// Memory: reads 64 B of parameters + 12 B per SH coefficient (coefficient-major
+// planes, so a wave reads 768 B contiguous per coefficient) and writes a
+// 48 B raster record, a 16 B footprint, tiles, key and order per slot; also
+// clears the slot's nine planar GRAD2D accumulators.  HBM-bound.
+#include "gs_common.h"
+#include "gs_internal.h"
+
+namespace {
+
+using namespace gsd;
+
+struct ProjectArgs {
+  const float4* __restrict__ params;
+  const float* __restrict__ sh;
+  float4* __restrict__ records;
+  float4* __restrict__ footprint;
+  uint32_t* __restrict__ tiles;
+  uint32_t* __restrict__ depth_keys;
+  uint32_t* __restrict__ depth_order;
+  float* __restrict__ grad2d;
+  uint32_t capacity;
+  uint32_t sh_coeffs;  // stored per slot
+};
+
+__global__ void __launch_bounds__(256) k_project(gs_hip_frame f, ProjectArgs a) {
+#pragma clang fp contract(off)
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= f.slots) return;
+  for (int c = 0; c < GS_HIP_GRAD2D_FLOATS; ++c) a.grad2d[size_t(c) * a.capacity + i] = 0.0f;
+
+  const float4 p0 = a.params[i * 4u];
+  const float4 p1 = a.params[i * 4u + 1u];
+  const float* R = f.cam.rotation;
+  // Camera-space position, left to right sums.
+  const float tx = R[0] * p0.x + R[1] * p0.y + R[2] * p0.z + f.cam.translation[0];
+  const float ty = R[3] * p0.x + R[4] * p0.y + R[5] * p0.z + f.cam.translation[1];
+  const float tz = R[6] * p0.x + R[7] * p0.y + R[8] * p0.z + f.cam.translation[2];
+  const float opacity = gs_sigmoid(p0.w);
+
+  uint32_t tiles = 0u;
+  Record rec;
+  rec.a = rec.b = rec.c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float4 foot = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (p1.w > 0.5f && tz > kNear && opacity >= kMinAlpha) {
+    const float fx = f.cam.fx, fy = f.cam.fy, cx = f.cam.cx, cy = f.cam.cy;
+    const float W = float(f.cam.width), H = float(f.cam.height);
+    const float inv_z = 1.0f / tz;
+    const float mx = fx * tx * inv_z + cx;
+    const float my = fy * ty * inv_z + cy;
+    // Jacobian of the perspective map, with the ray clamped to 1.3x the frustum.
+    const float lim_x = 1.3f * fmaxf(cx, W - cx) / fx;
+    const float lim_y = 1.3f * fmaxf(cy, H - cy) / fy;
+    const float rx = fminf(fmaxf(tx * inv_z, -lim_x), lim_x);
+    const float ry = fminf(fmaxf(ty * inv_z, -lim_y), lim_y);
+    const float j00 = fx * inv_z, j02 = -fx * rx * inv_z;
+    const float j11 = fy * inv_z, j12 = -fy * ry * inv_z;
+    // T = J W (2x3), W the world -> camera rotation.
+    const float t00 = j00 * R[0] + j02 * R[6], t01 = j00 * R[1] + j02 * R[7], t02 = j00 * R[2] + j02 * R[8];
+    const float t10 = j11 * R[3] + j12 * R[6], t11 = j11 * R[4] + j12 * R[7], t12 = j11 * R[5] + j12 * R[8];
+    // Rotation of the normalised quaternion (w, x, y, z), columns scaled by exp(log scale): M = R S.
+    const float4 q = a.params[i * 4u + 2u];
+    const float qlen = fmaxf(sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w), 1e-12f);
+    const float w = q.x / qlen, x = q.y / qlen, y = q.z / qlen, z = q.w / qlen;
+    const float sx = gs_expf(p1.x), sy = gs_expf(p1.y), sz = gs_expf(p1.z);
+    const float m00 = (1.0f - 2.0f * (y * y + z * z)) * sx, m01 = (2.0f * (x * y - w * z)) * sy,
+                m02 = (2.0f * (x * z + w * y)) * sz;
+    const float m10 = (2.0f * (x * y + w * z)) * sx, m11 = (1.0f - 2.0f * (x * x + z * z)) * sy,
+                m12 = (2.0f * (y * z - w * x)) * sz;
+    const float m20 = (2.0f * (x * z - w * y)) * sx, m21 = (2.0f * (y * z + w * x)) * sy,
+                m22 = (1.0f - 2.0f * (x * x + y * y)) * sz;
+    // V = T M; cov2D = V V^T (= T M M^T T^T), dilated by 0.3 px^2.
+    const float v00 = t00 * m00 + t01 * m10 + t02 * m20, v01 = t00 * m01 + t01 * m11 + t02 * m21,
+                v02 = t00 * m02 + t01 * m12 + t02 * m22;
+    const float v10 = t10 * m00 + t11 * m10 + t12 * m20, v11 = t10 * m01 + t11 * m11 + t12 * m21,
+                v12 = t10 * m02 + t11 * m12 + t12 * m22;
+    const float ca = v00 * v00 + v01 * v01 + v02 * v02 + kDilation;
+    const float cb = v00 * v10 + v01 * v11 + v02 * v12;
+    const float cc = v10 * v10 + v11 * v11 + v12 * v12 + kDilation;
+    const float det = ca * cc - cb * cb;
+    if (det > 0.0f) {
+      const float mid = 0.5f * (ca + cc);
+      const float lambda = mid + sqrtf(fmaxf(0.1f, mid * mid - det));
+      const float radius = ceilf(3.0f * sqrtf(lambda));
+      const float thr = footprint_threshold(opacity);
+      int row_first, row_last;
+      footprint_rows(my, cc, thr, f.cam.height, row_first, row_last);
+      if (row_first <= row_last) {
+        for (uint32_t tyi = uint32_t(row_first) / kTile; tyi <= uint32_t(row_last) / kTile; ++tyi) {
+          uint32_t x0, x1;
+          footprint_row_tiles(mx, my, ca, cb, cc, thr, row_first, row_last, tyi, f.cam.width, x0, x1);
+          tiles += x1 - x0;
+        }
+      }
+      if (tiles > 0u) {
+        // SH colour in the direction from the camera centre.
+        const float dx = p0.x - f.cam.position[0], dy = p0.y - f.cam.position[1], dz = p0.z - f.cam.position[2];
+        const float dlen = sqrtf(dx * dx + dy * dy + dz * dz);
+        float basis[16];
+        sh_basis(dx / dlen, dy / dlen, dz / dlen, f.sh_degree, basis);
+        const float4 p3 = a.params[i * 4u + 3u];
+        float cr = basis[0] * p3.x, cg = basis[0] * p3.y, cbl = basis[0] * p3.z;
+        const int coeffs = min(sh_rest_coeffs(f.sh_degree), int(a.sh_coeffs));
+        for (int k = 0; k < coeffs; ++k) {
+          const float* s = a.sh + (size_t(k) * a.capacity + i) * 3u;
+          cr = cr + basis[k + 1] * s[0];
+          cg = cg + basis[k + 1] * s[1];
+          cbl = cbl + basis[k + 1] * s[2];
+        }
+        cr = cr + 0.5f;
+        cg = cg + 0.5f;
+        cbl = cbl + 0.5f;
+        const uint32_t clamped = (cr < 0.0f ? 1u : 0u) | (cg < 0.0f ? 2u : 0u) | (cbl < 0.0f ? 4u : 0u);
+        const float inv_det = 1.0f / det;
+        // Pixel bbox of the footprint ellipse: half extents sqrt(t a), sqrt(t c).
+        int bx0, bx1, by0, by1;
+        const float ex = sqrtf(thr * ca);
+        footprint_pixels(mx - 0.5f - ex, mx - 0.5f + ex, f.cam.width, bx0, bx1);
+        by0 = row_first;
+        by1 = row_last;
+        rec.a = make_float4(mx, my, cc * inv_det, -cb * inv_det);
+        rec.b = make_float4(ca * inv_det, opacity, fmaxf(cr, 0.0f), fmaxf(cg, 0.0f));
+        rec.c = make_float4(fmaxf(cbl, 0.0f), __uint_as_float(uint32_t(bx0) | (uint32_t(by0) << 16)),
+                            __uint_as_float(uint32_t(bx1) | (uint32_t(by1) << 16)),
+                            __uint_as_float((uint32_t(fminf(radius, 4.0e8f)) << 3) | clamped));
+        foot = make_float4(ca, cb, cc, thr);
+      }
+    }
+  }
+  float4* r = a.records + size_t(i) * 3u;
+  r[0] = rec.a;
+  r[1] = rec.b;
+  r[2] = rec.c;
+  a.footprint[i] = foot;
+  a.tiles[i] = tiles;
+  uint32_t key = 0xffffffffu;
+  if (tiles > 0u) {
+    if ((f.flags & GS_HIP_FLAG_DEPTH16) != 0u) {
+      // The reference's 16-bit log depth (train_project.comp:110).
+      const float u = gs_logf(tz / kNear) / gs_logf(kFar / kNear);
+      key = uint32_t(fminf(fmaxf(u, 0.0f), 1.0f) * 65535.0f);
+    } else {
+      key = __float_as_uint(tz);  // z > 0: the bits order like the values
+    }
+  }
+  a.depth_keys[i] = key;
+  a.depth_order[i] = i;
+}
+
+}  // namespace
+
+namespace gsh {
+int launch_project(gs_hip_workspace* ws, const gs_hip_frame* f) {
+  if (f->slots == 0) return 0;
+  ProjectArgs a;
+  a.params = reinterpret_cast<const float4*>(ws->params);
+  a.sh = ws->sh;
+  a.records = reinterpret_cast<float4*>(ws->records);
+  a.footprint = reinterpret_cast<float4*>(ws->footprint);
+  a.tiles = ws->tiles;
+  a.depth_keys = ws->depth_keys;
+  a.depth_order = ws->depth_order;
+  a.grad2d = ws->grad2d;
+  a.capacity = ws->capacity;
+  a.sh_coeffs = ws->sh_coeffs;
+  timer_begin(ws, "project");
+  k_project<<<div_up(f->slots, 256), 256, 0, ws->stream>>>(*f, a);
+  GS_CHECK_LAUNCH("k_project");
+  timer_end(ws);
+  return 0;
+}
+}  // namespace gsh

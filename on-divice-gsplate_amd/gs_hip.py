@@ -1,0 +1,387 @@
+"""ctypes binding of libgs_hip.so (the C ABI in include/gs_hip.h).
+
+The Python-side mirror of the trainer's stage interface, used by the tests
+and the benchmark.  It loads the in-tree library built by the Makefile next
+to this file and fails loudly when it is missing: there is no CPU fallback.
+Device memory is owned by the library's workspace; Python moves numpy arrays
+in and out of it.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgs_hip.so")
+
+TILE = 16
+FLAG_MASKED_TARGET = 1
+FLAG_SPARSE_ADAM = 2
+FLAG_DEPTH16 = 4
+
+(BUF_PARAMS, BUF_SH, BUF_M_PARAMS, BUF_V_PARAMS, BUF_M_SH, BUF_V_SH, BUF_STATS, BUF_RECORDS, BUF_FOOTPRINT,
+ BUF_TILES, BUF_DEPTH_KEYS, BUF_DEPTH_ORDER, BUF_OFFSETS, BUF_GRAD2D, BUF_PAIR_TILES, BUF_PAIR_GAUSSIANS,
+ BUF_RANGES, BUF_IMAGE, BUF_LAST, BUF_TARGET, BUF_DLDC, BUF_TILE_LOSS, BUF_TOTALS) = range(23)
+
+EXPORTED = [
+    "gs_hip_init", "gs_hip_device_name", "gs_hip_malloc", "gs_hip_free", "gs_hip_upload", "gs_hip_download",
+    "gs_hip_memset", "gs_hip_synchronize", "gs_hip_last_error", "gs_hip_workspace_create",
+    "gs_hip_workspace_destroy", "gs_hip_workspace_buffer", "gs_hip_workspace_reserve_pairs",
+    "gs_hip_workspace_load", "gs_hip_workspace_store", "gs_hip_project", "gs_hip_bin", "gs_hip_forward",
+    "gs_hip_set_target", "gs_hip_set_target_device", "gs_hip_loss", "gs_hip_backward", "gs_hip_adam", "gs_hip_densify", "gs_hip_train_step",
+    "gs_hip_render", "gs_hip_workspace_timings",
+]
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [("rotation", ctypes.c_float * 9), ("translation", ctypes.c_float * 3),
+                ("position", ctypes.c_float * 3), ("fx", ctypes.c_float), ("fy", ctypes.c_float),
+                ("cx", ctypes.c_float), ("cy", ctypes.c_float), ("width", ctypes.c_uint32),
+                ("height", ctypes.c_uint32)]
+
+
+class Frame(ctypes.Structure):
+    _fields_ = [("cam", Camera), ("tiles_x", ctypes.c_uint32), ("tiles_y", ctypes.c_uint32),
+                ("slots", ctypes.c_uint32), ("sh_degree", ctypes.c_uint32), ("flags", ctypes.c_uint32),
+                ("seed", ctypes.c_uint32), ("background", ctypes.c_float * 3), ("lambda_dssim", ctypes.c_float),
+                ("ssim_c1", ctypes.c_float), ("ssim_c2", ctypes.c_float), ("lr_position", ctypes.c_float),
+                ("lr_scale", ctypes.c_float), ("lr_rotation", ctypes.c_float), ("lr_opacity", ctypes.c_float),
+                ("lr_sh_dc", ctypes.c_float), ("lr_sh_rest", ctypes.c_float), ("beta1", ctypes.c_float),
+                ("beta2", ctypes.c_float), ("epsilon", ctypes.c_float), ("bias1", ctypes.c_float),
+                ("bias2", ctypes.c_float), ("grad_threshold", ctypes.c_float), ("clone_max_scale", ctypes.c_float),
+                ("prune_opacity", ctypes.c_float), ("prune_max_scale", ctypes.c_float),
+                ("prune_max_radius", ctypes.c_float), ("split_divisor", ctypes.c_float)]
+
+
+_lib = None
+
+
+def library() -> ctypes.CDLL:
+    """The loaded libgs_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; "
+                               "g.build()')")
+        lib = ctypes.CDLL(LIB_PATH)
+        vp, u32, u64, i32, sz = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, ctypes.c_size_t
+        fp, dp = ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double)
+        u32p, u64p = ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint64)
+        frame_p = ctypes.POINTER(Frame)
+        sig = {
+            "gs_hip_init": [i32], "gs_hip_device_name": [ctypes.c_char_p, sz],
+            "gs_hip_malloc": [ctypes.POINTER(vp), sz], "gs_hip_free": [vp], "gs_hip_upload": [vp, vp, sz],
+            "gs_hip_download": [vp, vp, sz], "gs_hip_memset": [vp, i32, sz], "gs_hip_synchronize": [],
+            "gs_hip_last_error": [ctypes.c_char_p, sz],
+            "gs_hip_workspace_create": [ctypes.POINTER(vp), u32, i32, u32, u32],
+            "gs_hip_workspace_destroy": [vp],
+            "gs_hip_workspace_buffer": [vp, i32, ctypes.POINTER(vp), ctypes.POINTER(sz)],
+            "gs_hip_workspace_reserve_pairs": [vp, u64],
+            "gs_hip_workspace_load": [vp, fp, fp, u32, u32], "gs_hip_workspace_store": [vp, fp, fp, u32],
+            "gs_hip_project": [vp, frame_p], "gs_hip_bin": [vp, frame_p, u64p], "gs_hip_forward": [vp, frame_p],
+            "gs_hip_set_target": [vp, ctypes.POINTER(ctypes.c_uint8)], "gs_hip_set_target_device": [vp, vp],
+            "gs_hip_loss": [vp, frame_p, dp],
+            "gs_hip_backward": [vp, frame_p], "gs_hip_adam": [vp, frame_p],
+            "gs_hip_densify": [vp, frame_p, u32p, u32p], "gs_hip_train_step": [vp, frame_p, u64p, dp],
+            "gs_hip_render": [vp, frame_p, ctypes.POINTER(ctypes.c_uint8)],
+            "gs_hip_workspace_timings": [vp, i32, ctypes.POINTER(ctypes.c_char_p), dp, i32,
+                                         ctypes.POINTER(ctypes.c_int)],
+        }
+        for name, args in sig.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+        _lib = lib
+    return _lib
+
+
+class GsHipError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"gs_hip error {code}: {message}")
+        self.code = code
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        buf = ctypes.create_string_buffer(1024)
+        library().gs_hip_last_error(buf, 1024)
+        raise GsHipError(rc, buf.value.decode(errors="replace"))
+
+
+def init(device: int = 0) -> str:
+    lib = library()
+    check(lib.gs_hip_init(device))
+    buf = ctypes.create_string_buffer(256)
+    check(lib.gs_hip_device_name(buf, 256))
+    return buf.value.decode()
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+# ---- cameras -----------------------------------------------------------------
+def look_at_camera(eye, target, width: int, height: int, fov_y_deg: float = 60.0, up=(0.0, 1.0, 0.0)) -> Camera:
+    """Pinhole camera at `eye` looking at `target`, scene -> camera in OpenCV
+    axes (x right, y down, z forward), as dataset_camera builds from an OpenGL
+    camera-to-world matrix (engine/src/dataset.cpp:226-250)."""
+    eye = np.asarray(eye, np.float64)
+    f = np.asarray(target, np.float64) - eye
+    f /= np.linalg.norm(f)
+    r = np.cross(f, np.asarray(up, np.float64))
+    r /= np.linalg.norm(r)
+    d = np.cross(f, r)
+    R = np.stack([r, d, f]).astype(np.float32)
+    t = (-(R.astype(np.float64) @ eye)).astype(np.float32)
+    fy = (height / 2.0) / math.tan(math.radians(fov_y_deg) / 2.0)
+    cam = Camera()
+    cam.rotation[:] = [float(v) for v in R.reshape(-1)]
+    cam.translation[:] = [float(v) for v in t]
+    cam.position[:] = [float(v) for v in eye.astype(np.float32)]
+    cam.fx = cam.fy = fy
+    cam.cx = width / 2.0
+    cam.cy = height / 2.0
+    cam.width, cam.height = width, height
+    return cam
+
+
+@dataclass
+class TrainParams:
+    """Per-step knobs (the reference's TrainOptions values for one step)."""
+    lambda_dssim: float = 0.2
+    lr_position: float = 1.6e-4
+    lr_scale: float = 5e-3
+    lr_rotation: float = 5e-3
+    lr_opacity: float = 5e-3
+    lr_sh_dc: float = 5e-3
+    lr_sh_rest: float = 5e-3
+    grad_threshold: float = 2e-4
+    clone_max_scale: float = 0.01
+    prune_opacity: float = 0.005
+    seed: int = 1
+
+
+def make_frame(cam: Camera, slots: int, sh_degree: int, step: int = 0, background=(0.0, 0.0, 0.0), flags: int = 0,
+               tp: TrainParams | None = None) -> Frame:
+    tp = tp or TrainParams()
+    f = Frame()
+    f.cam = cam
+    f.tiles_x = (cam.width + TILE - 1) // TILE
+    f.tiles_y = (cam.height + TILE - 1) // TILE
+    f.slots = slots
+    f.sh_degree = sh_degree
+    f.flags = flags
+    f.seed = (tp.seed * 9781 + step * 6271) & 0xffffffff
+    f.background[:] = [float(b) for b in background]
+    f.lambda_dssim = tp.lambda_dssim
+    f.ssim_c1, f.ssim_c2 = 0.01 ** 2, 0.03 ** 2
+    f.lr_position, f.lr_scale, f.lr_rotation = tp.lr_position, tp.lr_scale, tp.lr_rotation
+    f.lr_opacity, f.lr_sh_dc, f.lr_sh_rest = tp.lr_opacity, tp.lr_sh_dc, tp.lr_sh_rest
+    f.beta1, f.beta2, f.epsilon = 0.9, 0.999, 1e-15
+    n = step + 1.0
+    f.bias1 = 1.0 / (1.0 - 0.9 ** n)
+    f.bias2 = 1.0 / (1.0 - 0.999 ** n)
+    f.grad_threshold, f.clone_max_scale, f.prune_opacity = tp.grad_threshold, tp.clone_max_scale, tp.prune_opacity
+    f.prune_max_scale = 0.0
+    f.prune_max_radius = 0.0
+    f.split_divisor = 1.6
+    return f
+
+
+# ---- workspace ---------------------------------------------------------------
+class Workspace:
+    """Every device buffer of one trainer (gs_hip_workspace)."""
+
+    def __init__(self, capacity: int, sh_degree: int, width: int, height: int):
+        self.lib = library()
+        self.handle = ctypes.c_void_p()
+        check(self.lib.gs_hip_workspace_create(ctypes.byref(self.handle), capacity, sh_degree, width, height))
+        self.capacity, self.sh_degree, self.width, self.height = capacity, sh_degree, width, height
+        self.sh_coeffs = 0 if sh_degree == 0 else (sh_degree + 1) ** 2 - 1
+        self.tiles_x = (width + TILE - 1) // TILE
+        self.tiles_y = (height + TILE - 1) // TILE
+
+    def close(self) -> None:
+        if self.handle:
+            check(self.lib.gs_hip_workspace_destroy(self.handle))
+            self.handle = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def buffer(self, which: int):
+        ptr, size = ctypes.c_void_p(), ctypes.c_size_t()
+        check(self.lib.gs_hip_workspace_buffer(self.handle, which, ctypes.byref(ptr), ctypes.byref(size)))
+        return ptr, size.value
+
+    def download(self, which: int, dtype, count: int | None = None) -> np.ndarray:
+        ptr, size = self.buffer(which)
+        item = np.dtype(dtype).itemsize
+        n = size // item if count is None else count
+        out = np.empty(n, dtype)
+        if n:
+            check(self.lib.gs_hip_download(out.ctypes.data_as(ctypes.c_void_p), ptr, n * item))
+        return out
+
+    def upload(self, which: int, data: np.ndarray) -> None:
+        ptr, size = self.buffer(which)
+        data = np.ascontiguousarray(data)
+        if data.nbytes > size:
+            raise ValueError("upload larger than the buffer")
+        check(self.lib.gs_hip_upload(ptr, data.ctypes.data_as(ctypes.c_void_p), data.nbytes))
+
+    def load(self, params: np.ndarray, sh_rest: np.ndarray | None) -> None:
+        """params [n,16] float32; sh_rest [n, coeffs, 3] float32 (or None)."""
+        params = np.ascontiguousarray(params, np.float32).reshape(-1, 16)
+        n = params.shape[0]
+        if sh_rest is None or sh_rest.size == 0:
+            check(self.lib.gs_hip_workspace_load(self.handle, _fptr(params), None, n, 0))
+        else:
+            sh_rest = np.ascontiguousarray(sh_rest, np.float32).reshape(n, -1, 3)
+            check(self.lib.gs_hip_workspace_load(self.handle, _fptr(params), _fptr(sh_rest), n, sh_rest.shape[1]))
+
+    def store(self, count: int):
+        params = np.empty((count, 16), np.float32)
+        sh = np.empty((count, self.sh_coeffs, 3), np.float32)
+        check(self.lib.gs_hip_workspace_store(self.handle, _fptr(params), _fptr(sh) if self.sh_coeffs else None,
+                                              count))
+        return params, sh
+
+    def project(self, frame: Frame) -> None:
+        check(self.lib.gs_hip_project(self.handle, ctypes.byref(frame)))
+
+    def bin(self, frame: Frame) -> int:
+        pairs = ctypes.c_uint64()
+        check(self.lib.gs_hip_bin(self.handle, ctypes.byref(frame), ctypes.byref(pairs)))
+        return pairs.value
+
+    def forward(self, frame: Frame) -> None:
+        check(self.lib.gs_hip_forward(self.handle, ctypes.byref(frame)))
+
+    def set_target(self, rgba8: np.ndarray) -> None:
+        rgba8 = np.ascontiguousarray(rgba8, np.uint8)
+        assert rgba8.size == self.width * self.height * 4
+        check(self.lib.gs_hip_set_target(self.handle, rgba8.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+
+    def set_target_device(self, device_ptr) -> None:
+        check(self.lib.gs_hip_set_target_device(self.handle, device_ptr))
+
+    def loss(self, frame: Frame, want_losses: bool = True):
+        out = (ctypes.c_double * 3)()
+        check(self.lib.gs_hip_loss(self.handle, ctypes.byref(frame), out if want_losses else None))
+        return tuple(out) if want_losses else None
+
+    def backward(self, frame: Frame) -> None:
+        check(self.lib.gs_hip_backward(self.handle, ctypes.byref(frame)))
+
+    def adam(self, frame: Frame) -> None:
+        check(self.lib.gs_hip_adam(self.handle, ctypes.byref(frame)))
+
+    def densify(self, frame: Frame):
+        slots = ctypes.c_uint32(frame.slots)
+        counts = (ctypes.c_uint32 * 3)()
+        check(self.lib.gs_hip_densify(self.handle, ctypes.byref(frame), ctypes.byref(slots), counts))
+        return slots.value, tuple(counts)
+
+    def train_step(self, frame: Frame, want_losses: bool = True):
+        pairs = ctypes.c_uint64()
+        out = (ctypes.c_double * 3)()
+        check(self.lib.gs_hip_train_step(self.handle, ctypes.byref(frame), ctypes.byref(pairs),
+                                         out if want_losses else None))
+        return pairs.value, (tuple(out) if want_losses else None)
+
+    def render(self, frame: Frame) -> np.ndarray:
+        out = np.empty((self.height, self.width, 4), np.uint8)
+        check(self.lib.gs_hip_render(self.handle, ctypes.byref(frame),
+                                     out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+        return out
+
+    def timings(self, enable: bool = True):
+        names = (ctypes.c_char_p * 64)()
+        ms = (ctypes.c_double * 64)()
+        count = ctypes.c_int()
+        check(self.lib.gs_hip_workspace_timings(self.handle, int(enable), names, ms, 64, ctypes.byref(count)))
+        return [(names[i].decode(), ms[i]) for i in range(count.value)]
+
+
+def synchronize() -> None:
+    check(library().gs_hip_synchronize())
+
+
+# ---- seeded synthetic scenes (the benchmark configurations) -----------------
+def synthetic_scene(n: int, sh_degree: int, seed: int):
+    """Gaussians of the benchmark configurations: means uniform in the unit
+    ball, log-scales N(-4, 0.5), quaternions normalise(N(0,1)^4) (w, x, y, z),
+    opacity logits N(0,1), DC uniform[0,1], SH rest N(0, 0.2).  Returns
+    (params [n,16] float32, sh_rest [n, coeffs, 3] float32)."""
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = rng.uniform(size=(n, 1)) ** (1.0 / 3.0)
+    pos = d * r
+    log_scale = rng.normal(-4.0, 0.5, size=(n, 3))
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    logit = rng.normal(size=(n, 1))
+    dc = rng.uniform(size=(n, 3))
+    coeffs = 0 if sh_degree == 0 else (sh_degree + 1) ** 2 - 1
+    sh = rng.normal(0.0, 0.2, size=(n, coeffs, 3)).astype(np.float32)
+    params = np.zeros((n, 16), np.float32)
+    params[:, 0:3] = pos
+    params[:, 3:4] = logit
+    params[:, 4:7] = log_scale
+    params[:, 7] = 1.0
+    params[:, 8:12] = q
+    params[:, 12:15] = dc
+    return params, sh
+
+
+def orbit_cameras(count: int, width: int, height: int, radius: float = 3.0, height_y: float = 0.5):
+    """`count` cameras evenly spaced on a circle around the origin."""
+    cams = []
+    for k in range(count):
+        th = 2.0 * math.pi * k / count
+        cams.append(look_at_camera((radius * math.sin(th), height_y, radius * math.cos(th)), (0, 0, 0), width,
+                                   height))
+    return cams
+
+
+def jitter_cameras(count: int, width: int, height: int, seed: int):
+    """Cameras at seeded azimuth U[0, 2pi), radius U[2.5, 3.5], height U[-0.5, 1.5]."""
+    rng = np.random.default_rng(seed)
+    cams = []
+    for _ in range(count):
+        th = rng.uniform(0.0, 2.0 * math.pi)
+        r = rng.uniform(2.5, 3.5)
+        h = rng.uniform(-0.5, 1.5)
+        cams.append(look_at_camera((r * math.sin(th), h, r * math.cos(th)), (0, 0, 0), width, height))
+    return cams
+
+
+class DeviceBuffer:
+    """A device allocation (gs_hip_malloc) holding a copy of a numpy array."""
+
+    def __init__(self, data: np.ndarray):
+        data = np.ascontiguousarray(data)
+        self.nbytes = data.nbytes
+        self.ptr = ctypes.c_void_p()
+        check(library().gs_hip_malloc(ctypes.byref(self.ptr), self.nbytes))
+        check(library().gs_hip_upload(self.ptr, data.ctypes.data_as(ctypes.c_void_p), self.nbytes))
+
+    def free(self) -> None:
+        if self.ptr:
+            check(library().gs_hip_free(self.ptr))
+            self.ptr = ctypes.c_void_p()

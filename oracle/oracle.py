@@ -1,0 +1,272 @@
+"""ctypes binding of liboracle.so — TEST INFRASTRUCTURE ONLY.
+
+The CPU restatement of the reference's training-step algorithm (see
+oracle.c's header for what each function follows and what pins it).  Only
+tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it, as
+the checker; nothing in the product imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+class OrcCamera(ctypes.Structure):
+    _fields_ = [("rotation", ctypes.c_float * 9), ("translation", ctypes.c_float * 3),
+                ("position", ctypes.c_float * 3), ("fx", ctypes.c_float), ("fy", ctypes.c_float),
+                ("cx", ctypes.c_float), ("cy", ctypes.c_float), ("width", ctypes.c_uint32),
+                ("height", ctypes.c_uint32)]
+
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def library() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        lib = ctypes.CDLL(LIB_PATH)
+        f32, f64, u32, u64, i32 = ctypes.c_float, ctypes.c_double, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
+        P = ctypes.POINTER
+        cam_p = P(OrcCamera)
+        sig = {
+            "orc_expf": ([f32], f32), "orc_logf": ([f32], f32),
+            "orc_sh_basis64": ([P(f64), i32, P(f64)], None),
+            "orc_footprint_tiles32": ([f32, f32, f32, f32, f32, f32, u32, u32, P(u32), u32], u32),
+            "orc_project32": ([P(f32), P(f32), u32, u32, cam_p, u32, i32, P(f32), P(f32), P(u32), P(u32)], None),
+            "orc_bin32": ([P(f32), P(f32), P(u32), P(u32), u32, u32, u32, P(u32), P(u32), u64, P(u32)], u64),
+            "orc_forward32": ([P(f32), P(u32), P(u32), u32, u32, P(f32), P(u32)], None),
+            "orc_project64": ([P(f64), P(f64), u32, u32, cam_p, i32, P(f64)], None),
+            "orc_render64": ([P(f64), P(f64), u32, u32, cam_p, i32, P(f64), P(f64), P(f64)], None),
+            "orc_ssim64": ([P(f64), P(f64), u32, u32, f64, f64, P(f64)], f64),
+            "orc_loss_image64": ([P(f64), P(f64), u32, u32, f64, P(f64)], f64),
+            "orc_loss64": ([P(f64), P(f64), u32, u32, cam_p, i32, P(f64), P(ctypes.c_uint8), i32, f64], f64),
+            "orc_backward64": ([P(f64), P(f64), u32, u32, cam_p, i32, P(f64), P(ctypes.c_uint8), i32, f64, P(f64),
+                                P(f64), P(f64)], f64),
+            "orc_backward32": ([P(f32), P(u32), P(u32), P(f32), P(u32), P(f32), P(f32), u32, u32, u32, P(f64)],
+                               None),
+            "orc_adjoint64": ([P(f64), P(f64), u32, u32, cam_p, i32, P(f64), P(f64), P(f64)], None),
+            "orc_adam": ([P(f32), P(f32), P(f32), P(f32), P(f32), u64, f32, f32, f32, f32, f32], None),
+            "orc_densify_mark": ([P(f32), P(f32), u32, f32, f32, f32, f32, f32, P(u32), P(u32), P(u32)], None),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = lib
+    return _lib
+
+
+def _p(a: np.ndarray, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def camera(cam) -> OrcCamera:
+    """An OrcCamera from any object with gs::Camera's fields."""
+    c = OrcCamera()
+    c.rotation[:] = list(cam.rotation)
+    c.translation[:] = list(cam.translation)
+    c.position[:] = list(cam.position)
+    c.fx, c.fy, c.cx, c.cy = cam.fx, cam.fy, cam.cx, cam.cy
+    c.width, c.height = cam.width, cam.height
+    return c
+
+
+def expf(x: float) -> float:
+    return library().orc_expf(x)
+
+
+def logf(x: float) -> float:
+    return library().orc_logf(x)
+
+
+def sh_basis64(d, degree: int) -> np.ndarray:
+    d = np.ascontiguousarray(d, np.float64)
+    out = np.zeros(16, np.float64)
+    library().orc_sh_basis64(_p(d, ctypes.c_double), degree, _p(out, ctypes.c_double))
+    return out
+
+
+def footprint_tiles32(mx, my, a, b, c, t, width, height) -> np.ndarray:
+    lib = library()
+    n = lib.orc_footprint_tiles32(mx, my, a, b, c, t, width, height, None, 0)
+    out = np.zeros(max(n, 1), np.uint32)
+    lib.orc_footprint_tiles32(mx, my, a, b, c, t, width, height, _p(out, ctypes.c_uint32), n)
+    return out[:n]
+
+
+def project32(params, sh, cam, sh_degree: int, depth16: bool = False):
+    params = np.ascontiguousarray(params, np.float32).reshape(-1, 16)
+    n = params.shape[0]
+    sh = np.ascontiguousarray(sh if sh is not None else np.zeros((n, 0, 3)), np.float32).reshape(n, -1, 3)
+    coeffs = sh.shape[1]
+    records = np.zeros((n, 12), np.float32)
+    foot = np.zeros((n, 4), np.float32)
+    tiles = np.zeros(n, np.uint32)
+    keys = np.zeros(n, np.uint32)
+    shp = sh if sh.size else np.zeros(1, np.float32)
+    library().orc_project32(_p(params, ctypes.c_float), _p(shp, ctypes.c_float), n, coeffs,
+                            ctypes.byref(camera(cam)), sh_degree, int(depth16), _p(records, ctypes.c_float),
+                            _p(foot, ctypes.c_float), _p(tiles, ctypes.c_uint32), _p(keys, ctypes.c_uint32))
+    return records, foot, tiles, keys
+
+
+def bin32(records, foot, tiles, keys, width: int, height: int):
+    n = records.shape[0]
+    total = int(np.sum(tiles, dtype=np.uint64))
+    pt = np.zeros(max(total, 1), np.uint32)
+    pg = np.zeros(max(total, 1), np.uint32)
+    T = ((width + 15) // 16) * ((height + 15) // 16)
+    ranges = np.zeros((T + 1, 2), np.uint32)
+    got = library().orc_bin32(_p(np.ascontiguousarray(records, np.float32), ctypes.c_float),
+                              _p(np.ascontiguousarray(foot, np.float32), ctypes.c_float),
+                              _p(np.ascontiguousarray(tiles, np.uint32), ctypes.c_uint32),
+                              _p(np.ascontiguousarray(keys, np.uint32), ctypes.c_uint32), n, width, height,
+                              _p(pt, ctypes.c_uint32), _p(pg, ctypes.c_uint32), total, _p(ranges, ctypes.c_uint32))
+    assert got == total
+    return pt[:total], pg[:total], ranges
+
+
+def forward32(records, pair_gaussians, ranges, width: int, height: int):
+    image = np.zeros((height * width, 4), np.float32)
+    last = np.zeros(height * width, np.uint32)
+    pg = np.ascontiguousarray(pair_gaussians, np.uint32)
+    if pg.size == 0:
+        pg = np.zeros(1, np.uint32)
+    library().orc_forward32(_p(np.ascontiguousarray(records, np.float32), ctypes.c_float), _p(pg, ctypes.c_uint32),
+                            _p(np.ascontiguousarray(ranges, np.uint32), ctypes.c_uint32), width, height,
+                            _p(image, ctypes.c_float), _p(last, ctypes.c_uint32))
+    return image, last
+
+
+def _scene64(params, sh):
+    params = np.ascontiguousarray(params, np.float64).reshape(-1, 16)
+    n = params.shape[0]
+    sh = np.ascontiguousarray(sh if sh is not None else np.zeros((n, 0, 3)), np.float64).reshape(n, -1, 3)
+    shp = sh if sh.size else np.zeros(1, np.float64)
+    return params, sh, shp, n, sh.shape[1]
+
+
+def project64(params, sh, cam, degree: int) -> np.ndarray:
+    params, sh, shp, n, coeffs = _scene64(params, sh)
+    out = np.zeros((n, 16), np.float64)
+    library().orc_project64(_p(params, ctypes.c_double), _p(shp, ctypes.c_double), n, coeffs,
+                            ctypes.byref(camera(cam)), degree, _p(out, ctypes.c_double))
+    return out
+
+
+def render64(params, sh, cam, degree: int, bg=(0.0, 0.0, 0.0)):
+    params, sh, shp, n, coeffs = _scene64(params, sh)
+    rgb = np.zeros((cam.height * cam.width, 3), np.float64)
+    T = np.zeros(cam.height * cam.width, np.float64)
+    bgv = np.asarray(bg, np.float64)
+    library().orc_render64(_p(params, ctypes.c_double), _p(shp, ctypes.c_double), n, coeffs,
+                           ctypes.byref(camera(cam)), degree, _p(bgv, ctypes.c_double), _p(rgb, ctypes.c_double),
+                           _p(T, ctypes.c_double))
+    return rgb, T
+
+
+def ssim64(x, y, width: int, height: int, grad: bool = False):
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+    y = np.ascontiguousarray(y, np.float64).reshape(-1, 3)
+    g = np.zeros_like(x) if grad else None
+    s = library().orc_ssim64(_p(x, ctypes.c_double), _p(y, ctypes.c_double), width, height, 1e-4, 9e-4,
+                             _p(g, ctypes.c_double) if grad else None)
+    return (s, g) if grad else s
+
+
+def loss_image64(x, y, width: int, height: int, lam: float, grad: bool = False):
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+    y = np.ascontiguousarray(y, np.float64).reshape(-1, 3)
+    g = np.zeros_like(x) if grad else None
+    L = library().orc_loss_image64(_p(x, ctypes.c_double), _p(y, ctypes.c_double), width, height, lam,
+                                   _p(g, ctypes.c_double) if grad else None)
+    return (L, g) if grad else L
+
+
+def loss64(params, sh, cam, degree: int, target, lam: float, bg=(0.0, 0.0, 0.0), masked: bool = False) -> float:
+    params, sh, shp, n, coeffs = _scene64(params, sh)
+    t = np.ascontiguousarray(target, np.uint8).reshape(-1)
+    bgv = np.asarray(bg, np.float64)
+    return library().orc_loss64(_p(params, ctypes.c_double), _p(shp, ctypes.c_double), n, coeffs,
+                                ctypes.byref(camera(cam)), degree, _p(bgv, ctypes.c_double),
+                                _p(t, ctypes.c_uint8), int(masked), lam)
+
+
+def backward64(params, sh, cam, degree: int, target, lam: float, bg=(0.0, 0.0, 0.0), masked: bool = False):
+    """(loss, grad2d [n,9], gparams [n,16], gsh [n,coeffs,3])."""
+    params, sh, shp, n, coeffs = _scene64(params, sh)
+    t = np.ascontiguousarray(target, np.uint8).reshape(-1)
+    bgv = np.asarray(bg, np.float64)
+    g2 = np.zeros((n, 9), np.float64)
+    gp = np.zeros((n, 16), np.float64)
+    gs = np.zeros((n, max(coeffs, 0), 3), np.float64)
+    gsp = gs if gs.size else np.zeros(1, np.float64)
+    L = library().orc_backward64(_p(params, ctypes.c_double), _p(shp, ctypes.c_double), n, coeffs,
+                                 ctypes.byref(camera(cam)), degree, _p(bgv, ctypes.c_double), _p(t, ctypes.c_uint8),
+                                 int(masked), lam, _p(g2, ctypes.c_double), _p(gp, ctypes.c_double),
+                                 _p(gsp, ctypes.c_double))
+    return L, g2, gp, gs
+
+
+def adam(p, g, m, v, lr, beta1=0.9, beta2=0.999, eps=1e-15, bias1=10.0, bias2=1000.0):
+    p, g, m, v, lr = (np.ascontiguousarray(a, np.float32).copy() for a in (p, g, m, v, lr))
+    library().orc_adam(_p(p, ctypes.c_float), _p(g, ctypes.c_float), _p(m, ctypes.c_float), _p(v, ctypes.c_float),
+                       _p(lr, ctypes.c_float), p.size, beta1, beta2, eps, bias1, bias2)
+    return p, m, v
+
+
+def densify_mark(params, stats, grad_threshold, clone_max_scale, prune_opacity, prune_max_scale=0.0,
+                 prune_max_radius=0.0):
+    params = np.ascontiguousarray(params, np.float32).reshape(-1, 16)
+    stats = np.ascontiguousarray(stats, np.float32).reshape(-1, 4)
+    n = params.shape[0]
+    action = np.zeros(n, np.uint32)
+    release = np.zeros(n, np.uint32)
+    prune = np.zeros(n, np.uint32)
+    library().orc_densify_mark(_p(params, ctypes.c_float), _p(stats, ctypes.c_float), n, grad_threshold,
+                               clone_max_scale, prune_opacity, prune_max_scale, prune_max_radius,
+                               _p(action, ctypes.c_uint32), _p(release, ctypes.c_uint32), _p(prune, ctypes.c_uint32))
+    return action, release, prune
+
+
+def backward32(records, pair_gaussians, ranges, image, last, dldc, bg, width: int, height: int) -> np.ndarray:
+    """grad2d [n, 9] (double sums) of the tile backward on the given forward state."""
+    records = np.ascontiguousarray(records, np.float32).reshape(-1, 12)
+    n = records.shape[0]
+    pg = np.ascontiguousarray(pair_gaussians, np.uint32)
+    if pg.size == 0:
+        pg = np.zeros(1, np.uint32)
+    out = np.zeros((n, 9), np.float64)
+    bgv = np.asarray(bg, np.float32)
+    library().orc_backward32(_p(records, ctypes.c_float), _p(pg, ctypes.c_uint32),
+                             _p(np.ascontiguousarray(ranges, np.uint32), ctypes.c_uint32),
+                             _p(np.ascontiguousarray(image, np.float32), ctypes.c_float),
+                             _p(np.ascontiguousarray(last, np.uint32), ctypes.c_uint32),
+                             _p(np.ascontiguousarray(dldc, np.float32), ctypes.c_float), _p(bgv, ctypes.c_float),
+                             n, width, height, _p(out, ctypes.c_double))
+    return out
+
+
+def adjoint64(params, sh, cam, degree: int, grad2d):
+    """(gparams [n,16], gsh [n,coeffs,3]) of the projection adjoint of grad2d [n,9]."""
+    params, sh, shp, n, coeffs = _scene64(params, sh)
+    g2 = np.ascontiguousarray(grad2d, np.float64).reshape(n, 9)
+    gp = np.zeros((n, 16), np.float64)
+    gs = np.zeros((n, max(coeffs, 0), 3), np.float64)
+    gsp = gs if gs.size else np.zeros(1, np.float64)
+    library().orc_adjoint64(_p(params, ctypes.c_double), _p(shp, ctypes.c_double), n, coeffs,
+                            ctypes.byref(camera(cam)), degree, _p(g2, ctypes.c_double), _p(gp, ctypes.c_double),
+                            _p(gsp, ctypes.c_double))
+    return gp, gs

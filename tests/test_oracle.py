@@ -1,0 +1,354 @@
+"""Pins the CPU oracle against what fixes it independently: libm, scipy's
+spherical harmonics, closed forms, brute force, a second SSIM written with
+scipy.ndimage, and central finite differences.  No GPU."""
+import math
+
+import numpy as np
+import pytest
+from scipy import ndimage, special
+
+from helpers import make_camera, make_scene, noise_target, orc, smooth_target, gs
+
+
+def _ulp_diff(a, b):
+    a = np.asarray(a, np.float32).view(np.int32).astype(np.int64)
+    b = np.asarray(b, np.float32).view(np.int32).astype(np.int64)
+    return np.abs(a - b)
+
+
+def test_expf_within_two_ulp():
+    xs = np.concatenate([np.linspace(-87.0, 88.0, 20001), np.linspace(-1.0, 1.0, 5001),
+                         np.random.default_rng(1).uniform(-30, 30, 5000)]).astype(np.float32)
+    got = np.array([orc.expf(float(x)) for x in xs], np.float32)
+    want = np.exp(xs.astype(np.float64)).astype(np.float32)
+    assert _ulp_diff(got, want).max() <= 2
+
+
+def test_logf_within_two_ulp():
+    rng = np.random.default_rng(2)
+    xs = np.concatenate([np.exp(rng.uniform(-60, 60, 20000)), np.linspace(0.5, 2.0, 5001),
+                         np.linspace(1.0, 255.0, 5001)]).astype(np.float32)
+    got = np.array([orc.logf(float(x)) for x in xs], np.float32)
+    want = np.log(xs.astype(np.float64)).astype(np.float32)
+    # absolute error near log(1) = 0 is what matters there
+    ok = (_ulp_diff(got, want) <= 2) | (np.abs(got - want) <= 2e-7)
+    assert ok.all()
+
+
+def _real_sh(l, m, polar, azimuth):
+    if hasattr(special, "sph_harm_y"):
+        Y = special.sph_harm_y(l, abs(m), polar, azimuth)
+    else:
+        Y = special.sph_harm(abs(m), l, azimuth, polar)
+    if m > 0:
+        return math.sqrt(2.0) * Y.real
+    if m < 0:
+        return math.sqrt(2.0) * Y.imag
+    return Y.real
+
+
+def test_sh_basis_matches_scipy_real_harmonics():
+    """The 3DGS basis is the real SH built from scipy's Condon-Shortley
+    Y_l^m: sqrt(2) Re Y_l^m (m > 0), Y_l^0, sqrt(2) Im Y_l^|m| (m < 0),
+    ordered m = -l..l within each degree."""
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        polar, azimuth = math.acos(d[2]), math.atan2(d[1], d[0])
+        b = orc.sh_basis64(d, 3)
+        k = 0
+        for l in range(4):
+            for m in range(-l, l + 1):
+                assert b[k] == pytest.approx(_real_sh(l, m, polar, azimuth), abs=1e-12), (l, m)
+                k += 1
+
+
+def _single_splat(opacity=0.95, scale=0.1, pos=(0.0, 0.0, 0.0), colour=(1.0, 0.0, 0.0)):
+    p = np.zeros((1, 16), np.float64)
+    p[0, 0:3] = pos
+    p[0, 3] = math.log(opacity / (1 - opacity))
+    p[0, 4:7] = math.log(scale)
+    p[0, 7] = 1.0
+    p[0, 8] = 1.0
+    # DC such that colour = C0 dc + 0.5
+    p[0, 12:15] = (np.asarray(colour) - 0.5) / 0.28209479177387814
+    return p
+
+
+def test_single_splat_falloff_known_answer():
+    """engine/tests/test_splats.cpp:188-210: an isotropic splat of scale 0.1 at
+    distance 3 falls off as o exp(-0.5 d^2 / (sigma_px^2 + 0.3))."""
+    w = h = 64
+    cam = gs.look_at_camera((0, 0, 3), (0, 0, 0), w, h)
+    p = _single_splat()
+    proj = orc.project64(p, None, cam, 0)[0]
+    sigma = cam.fx * 0.1 / 3.0
+    var = sigma * sigma + 0.3
+    assert proj[0] == 1.0
+    assert proj[1] == pytest.approx(32.0) and proj[2] == pytest.approx(32.0)
+    assert proj[5] == pytest.approx(1 / var, rel=1e-6) and proj[7] == pytest.approx(1 / var, rel=1e-6)
+    assert abs(proj[6]) < 1e-9
+    rgb, T = orc.render64(p, None, cam, 0)
+    for dx in (0, 2, 4):
+        px, py = 32 + dx + 0.5 - 32.0, 0.5
+        alpha = min(0.95 * math.exp(-0.5 * (px * px + py * py) / var), 0.99)
+        assert rgb[32 * w + 32 + dx, 0] == pytest.approx(alpha, rel=1e-9)
+        assert rgb[32 * w + 32 + dx, 1] == pytest.approx(0.0, abs=1e-12)
+    assert rgb[2 * w + 2].max() < 1e-12
+
+
+def test_depth_order_known_answer():
+    """engine/tests/test_splats.cpp:212-229: the nearer of two splats on the
+    axis dominates the centre pixel."""
+    w = h = 64
+    cam = gs.look_at_camera((0, 0, 3), (0, 0, 0), w, h)
+    for red_in_front in (True, False):
+        rz = 0.5 if red_in_front else -0.5
+        p = np.concatenate([_single_splat(0.9, 0.1, (0, 0, rz), (1, 0, 0)),
+                            _single_splat(0.9, 0.1, (0, 0, -rz), (0, 0, 1))])
+        rgb, _ = orc.render64(p, None, cam, 0)
+        c = rgb[32 * w + 32]
+        if red_in_front:
+            assert c[0] > 180 / 255 and c[2] < 60 / 255
+        else:
+            assert c[2] > 180 / 255 and c[0] < 60 / 255
+
+
+def test_project32_agrees_with_project64():
+    params, sh = make_scene(400, 3, 11)
+    cam = make_camera(96, 72, 0.4)
+    rec, foot, tiles, keys = orc.project32(params, sh, cam, 3)
+    ref = orc.project64(params, sh, cam, 3)
+    vis = tiles > 0
+    assert vis.sum() > 100
+    assert np.all(ref[vis, 0] == 1.0)
+    np.testing.assert_allclose(rec[vis, 0:2], ref[vis, 1:3], rtol=1e-5, atol=1e-4)
+    con32, con64 = rec[vis][:, [2, 3, 4]], ref[vis][:, 5:8]
+    scale = np.abs(con64).max(axis=1, keepdims=True)
+    assert np.max(np.abs(con32 - con64) / scale) < 1e-4
+    np.testing.assert_allclose(rec[vis, 5], ref[vis, 4], rtol=1e-6)
+    np.testing.assert_allclose(rec[vis, 6:9], ref[vis, 8:11], rtol=1e-5, atol=1e-5)
+    z = keys[vis].view(np.float32)
+    np.testing.assert_allclose(z, ref[vis, 3], rtol=1e-6)
+
+
+def test_depth16_key_matches_reference_formula():
+    params, sh = make_scene(300, 1, 12)
+    cam = make_camera(64, 48, 1.0)
+    _, _, tiles, keys = orc.project32(params, sh, cam, 1, depth16=True)
+    ref = orc.project64(params, sh, cam, 1)
+    vis = tiles > 0
+    want = np.floor(np.clip(np.log2(ref[vis, 3] / 0.02) / np.log2(1000 / 0.02), 0, 1) * 65535)
+    assert np.max(np.abs(keys[vis].astype(np.int64) - want.astype(np.int64))) <= 1
+    assert np.all(keys[~vis] == 0xFFFFFFFF)
+
+
+def test_footprint_against_brute_force():
+    """engine/tests/test_trainer.cpp:235-289: every tile holding a pixel with
+    d^T conic d <= 2 ln(255 o) is emitted, once, and few others."""
+    w, h = 64, 48
+    params, sh = make_scene(200, 1, 13)
+    params[::2, 4] += 1.5  # half of them stretched
+    cam = make_camera(w, h, 0.0)
+    rec, foot, tiles, _ = orc.project32(params, sh, cam, 1)
+    tx = (w + 15) // 16
+    needed = emitted = 0
+    ys, xs = np.mgrid[0:h, 0:w]
+    for i in np.nonzero(tiles)[0]:
+        q = rec[i].astype(np.float64)
+        t = 2.0 * math.log(255.0 * q[5])
+        dx, dy = q[0] - (xs + 0.5), q[1] - (ys + 0.5)
+        inside = q[2] * dx * dx + 2 * q[3] * dx * dy + q[4] * dy * dy <= t
+        want = set(((ys[inside] // 16) * tx + xs[inside] // 16).tolist())
+        got = orc.footprint_tiles32(*[float(v) for v in (q[0], q[1], foot[i, 0], foot[i, 1], foot[i, 2],
+                                                          foot[i, 3])], w, h)
+        assert len(got) == tiles[i]
+        assert len(set(got.tolist())) == len(got), "duplicate tile"
+        assert want <= set(got.tolist()), f"gaussian {i}: missing tiles"
+        needed += len(want)
+        emitted += len(got)
+    assert needed > 100
+    assert emitted <= needed + needed // 20
+
+
+def test_bin_order_is_tile_depth_index():
+    w, h = 64, 48
+    params, sh = make_scene(400, 1, 37)
+    cam = make_camera(w, h, 0.0)
+    rec, foot, tiles, keys = orc.project32(params, sh, cam, 1, depth16=True)
+    pt, pg, ranges = orc.bin32(rec, foot, tiles, keys, w, h)
+    pairs = []
+    for i in np.nonzero(tiles)[0]:
+        for t in orc.footprint_tiles32(*[float(v) for v in (rec[i, 0], rec[i, 1], *foot[i])], w, h):
+            pairs.append((int(t), int(keys[i]), int(i)))
+    want = sorted(pairs)
+    assert len(want) > 500 and len({p[1] for p in want}) > 100
+    assert [p[0] for p in want] == pt.tolist()
+    assert [p[2] for p in want] == pg.tolist()
+    T = ((w + 15) // 16) * ((h + 15) // 16)
+    for t in range(T):
+        s, e = ranges[t]
+        assert np.all(pt[s:e] == t)
+        assert (e - s) == sum(1 for p in want if p[0] == t)
+
+
+def test_forward32_matches_brute_force_render64():
+    w, h = 96, 72
+    params, sh = make_scene(300, 2, 21)
+    cam = make_camera(w, h, 2.0)
+    bg = (0.1, 0.2, 0.3)
+    rec, foot, tiles, keys = orc.project32(params, sh, cam, 2)
+    pt, pg, ranges = orc.bin32(rec, foot, tiles, keys, w, h)
+    img, last = orc.forward32(rec, pg, ranges, w, h)
+    out = img[:, :3] + img[:, 3:4] * np.asarray(bg, np.float32)
+    ref, T = orc.render64(params, sh, cam, 2, bg)
+    err = np.abs(out - ref).max(axis=1)
+    assert np.quantile(err, 0.999) < 1e-4
+    assert err.max() < 2.5 / 255
+
+
+def _ssim_numpy(x, y, w, h):
+    k = np.exp(-np.arange(-5, 6) ** 2 / (2 * 1.5 ** 2))
+    k /= k.sum()
+    blur = lambda a: ndimage.correlate1d(ndimage.correlate1d(a, k, axis=0, mode="constant"), k, axis=1,
+                                         mode="constant")
+    total = 0.0
+    for c in range(3):
+        a, b = x[:, c].reshape(h, w), y[:, c].reshape(h, w)
+        mx, my = blur(a), blur(b)
+        vx, vy, cxy = blur(a * a) - mx * mx, blur(b * b) - my * my, blur(a * b) - mx * my
+        s = (2 * mx * my + 1e-4) * (2 * cxy + 9e-4) / ((mx * mx + my * my + 1e-4) * (vx + vy + 9e-4))
+        total += s.sum()
+    return total
+
+
+def test_ssim64_against_scipy_and_identity():
+    w, h = 40, 30
+    rng = np.random.default_rng(5)
+    x, y = rng.uniform(size=(w * h, 3)), rng.uniform(size=(w * h, 3))
+    assert orc.ssim64(x, y, w, h) == pytest.approx(_ssim_numpy(x, y, w, h), rel=1e-12)
+    s, g = orc.ssim64(x, x, w, h, grad=True)
+    assert s == pytest.approx(3 * w * h, rel=1e-12)
+    assert np.abs(g).max() < 1e-9
+
+
+def test_loss_image_gradient_by_central_differences():
+    w, h = 24, 20
+    rng = np.random.default_rng(6)
+    x, y = rng.uniform(0.2, 0.8, size=(w * h, 3)), rng.uniform(0.2, 0.8, size=(w * h, 3))
+    L, g = orc.loss_image64(x, y, w, h, 0.2, grad=True)
+    eps = 1e-6
+    for idx in rng.choice(w * h * 3, 60, replace=False):
+        if abs(x.flat[idx] - y.flat[idx]) < 1e-3:
+            continue
+        xp, xm = x.copy(), x.copy()
+        xp.flat[idx] += eps
+        xm.flat[idx] -= eps
+        fd = (orc.loss_image64(xp, y, w, h, 0.2) - orc.loss_image64(xm, y, w, h, 0.2)) / (2 * eps)
+        assert g.flat[idx] == pytest.approx(fd, rel=1e-5, abs=1e-10)
+
+
+def test_backward64_by_central_differences():
+    """FP64 central differences of the fp64 loss against the analytic backward
+    (the reference's check_gradients, train_selftest.cpp:356-486, with its
+    per-value rule |g - fd| <= 0.05 |fd| + 2e-6 and at most 5% off per group)."""
+    w, h = 64, 48
+    params, sh = make_scene(10, 2, 44, spread=0.4)
+    cam = make_camera(w, h, 0.3)
+    target = smooth_target(w, h, 7)
+    p64, s64 = params.astype(np.float64), sh.astype(np.float64)
+    L, g2, gp, gsh = orc.backward64(p64, s64, cam, 2, target, 0.2, bg=(0.2, 0.1, 0.0))
+    eps = 1e-5
+    groups = {"position": [0, 1, 2], "opacity": [3], "scale": [4, 5, 6], "rotation": [8, 9, 10, 11],
+              "dc": [12, 13, 14]}
+    for name, cols in groups.items():
+        bad = checked = 0
+        for i in range(params.shape[0]):
+            for c in cols:
+                pp, pm = p64.copy(), p64.copy()
+                pp[i, c] += eps
+                pm[i, c] -= eps
+                fd = (orc.loss64(pp, s64, cam, 2, target, 0.2, bg=(0.2, 0.1, 0.0)) -
+                      orc.loss64(pm, s64, cam, 2, target, 0.2, bg=(0.2, 0.1, 0.0))) / (2 * eps)
+                checked += 1
+                if abs(gp[i, c] - fd) > 0.05 * abs(fd) + 2e-6:
+                    bad += 1
+        assert bad <= checked // 20, f"{name}: {bad}/{checked} off"
+    bad = checked = 0
+    for i in range(params.shape[0]):
+        for k in range(sh.shape[1]):
+            for c in range(3):
+                sp, sm = s64.copy(), s64.copy()
+                sp[i, k, c] += eps
+                sm[i, k, c] -= eps
+                fd = (orc.loss64(p64, sp, cam, 2, target, 0.2, bg=(0.2, 0.1, 0.0)) -
+                      orc.loss64(p64, sm, cam, 2, target, 0.2, bg=(0.2, 0.1, 0.0))) / (2 * eps)
+                checked += 1
+                if abs(gsh[i, k, c] - fd) > 0.05 * abs(fd) + 2e-6:
+                    bad += 1
+    assert bad <= checked // 20, f"sh: {bad}/{checked} off"
+    assert np.abs(gp[:, [0, 3, 4, 12]]).max() > 1e-4  # the check is not vacuous
+
+
+def test_adam_closed_form():
+    rng = np.random.default_rng(8)
+    g = rng.normal(size=100).astype(np.float32)
+    p = rng.normal(size=100).astype(np.float32)
+    zeros = np.zeros(100, np.float32)
+    # lr 0: parameters stay, m / (1 - beta1) is the gradient (trainer.cpp:1802-1811)
+    p1, m, v = orc.adam(p, g, zeros, zeros, zeros, bias1=1 / (1 - 0.9), bias2=1 / (1 - 0.999))
+    np.testing.assert_array_equal(p1, p)
+    np.testing.assert_allclose(m / (1 - 0.9), g, rtol=1e-6)
+    np.testing.assert_allclose(v, g * g * (np.float32(1) - np.float32(0.999)), rtol=1e-6)
+    # first step with bias correction moves each parameter by lr sign(g)
+    lr = np.full(100, 1e-3, np.float32)
+    p1, _, _ = orc.adam(p, g, zeros, zeros, lr, bias1=1 / (1 - 0.9), bias2=1 / (1 - 0.999))
+    np.testing.assert_allclose(p1, p - 1e-3 * np.sign(g), rtol=1e-5, atol=1e-7)
+
+
+def test_densify_mark_predicate():
+    rng = np.random.default_rng(9)
+    n = 2000
+    params = np.zeros((n, 16), np.float32)
+    params[:, 3] = rng.normal(-2, 3, size=n)
+    params[:, 4:7] = rng.normal(-4, 1, size=(n, 3))
+    params[:, 7] = (rng.uniform(size=n) > 0.1).astype(np.float32)
+    stats = np.zeros((n, 4), np.float32)
+    stats[:, 1] = rng.integers(0, 5, size=n)
+    stats[:, 0] = stats[:, 1] * rng.uniform(0, 4e-4, size=n)
+    action, release, prune = orc.densify_mark(params, stats, 2e-4, 0.01, 0.005)
+    alive = params[:, 7] >= 0.5
+    op = 1 / (1 + np.exp(-params[:, 3].astype(np.float64)))
+    want_prune = alive & (op < 0.005)
+    np.testing.assert_array_equal(prune.astype(bool), want_prune)
+    np.testing.assert_array_equal(release.astype(bool), ~alive | want_prune)
+    grad = np.divide(stats[:, 0], stats[:, 1], out=np.zeros(n, np.float32), where=stats[:, 1] > 0)
+    flag = alive & ~want_prune & (stats[:, 1] > 0) & (grad >= 2e-4)
+    big = np.exp(params[:, 4:7].max(axis=1).astype(np.float64)) > 0.01
+    want = np.where(flag, np.where(big, 2, 1), 0)
+    np.testing.assert_array_equal(action, want)
+
+
+def test_backward32_tiles_matches_backward64():
+    """The tile-list fp32 backward on the fp32 forward's state reproduces the
+    brute-force fp64 analytic per-gaussian 2D gradient."""
+    w, h = 64, 48
+    params, sh = make_scene(60, 1, 45, spread=0.5)
+    cam = make_camera(w, h, 0.7)
+    bg = (0.2, 0.1, 0.0)
+    target = smooth_target(w, h, 8)
+    rec, foot, tiles, keys = orc.project32(params, sh, cam, 1)
+    pt, pg, ranges = orc.bin32(rec, foot, tiles, keys, w, h)
+    img, last = orc.forward32(rec, pg, ranges, w, h)
+    x = img[:, :3].astype(np.float64) + img[:, 3:4].astype(np.float64) * np.asarray(bg)
+    y = target.reshape(-1, 4)[:, :3] / 255.0
+    _, dldc = orc.loss_image64(x, y, w, h, 0.2, grad=True)
+    d4 = np.zeros((w * h, 4), np.float32)
+    d4[:, :3] = dldc
+    g32 = orc.backward32(rec, pg, ranges, img, last, d4, bg, w, h)
+    _, g64, _, _ = orc.backward64(params.astype(np.float64), sh.astype(np.float64), cam, 1, target, 0.2, bg=bg)
+    for c in range(9):
+        scale = np.abs(g64[:, c]).max()
+        assert scale > 0
+        assert np.abs(g32[:, c] - g64[:, c]).max() <= 1e-3 * scale, c
