@@ -1,0 +1,58 @@
+// Rendering a splat scene to an RGBA8 image on the GPU (the gs_render path:
+// engine/include/gs/splat_renderer.h:63-114 and offscreen.h:14-44), with
+// the Vulkan render-pass types replaced by a plain render() call.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "gs/camera.h"
+#include "gs/device.h"
+#include "gs/splat_scene.h"
+
+struct gs_hip_workspace;
+
+namespace gs {
+
+// A scene resident on the device at a fixed output size.
+class GpuScene {
+ public:
+  GpuScene(Device& device, const SplatScene& scene, uint32_t width, uint32_t height);
+  ~GpuScene();
+  GpuScene(const GpuScene&) = delete;
+  GpuScene& operator=(const GpuScene&) = delete;
+  uint32_t count() const { return count_; }
+  int sh_degree() const { return sh_degree_; }
+  uint32_t width() const { return width_; }
+  uint32_t height() const { return height_; }
+  gs_hip_workspace* workspace() { return ws_; }
+
+ private:
+  gs_hip_workspace* ws_ = nullptr;
+  uint32_t count_ = 0, width_ = 0, height_ = 0;
+  int sh_degree_ = 0;
+};
+
+std::unique_ptr<GpuScene> upload_scene(Device& device, const SplatScene& scene, uint32_t width, uint32_t height);
+
+class SplatRenderer {
+ public:
+  explicit SplatRenderer(Device& device) : device_(device) {}
+  void attach(GpuScene* scene) { scene_ = scene; }
+  // RGBA8 of `camera` (the scene's output size) front to back over `clear`.
+  std::vector<uint8_t> render(const Camera& camera, const float clear[3]);
+  uint32_t visible_count() const { return visible_; }
+
+ private:
+  Device& device_;
+  GpuScene* scene_ = nullptr;
+  uint32_t visible_ = 0;
+};
+
+// Binary PPM (P6) of an RGBA8 image.
+std::vector<uint8_t> encode_ppm(const std::vector<uint8_t>& rgba, uint32_t width, uint32_t height);
+void write_ppm(const std::string& path, const std::vector<uint8_t>& rgba, uint32_t width, uint32_t height);
+
+}  // namespace gs

@@ -183,6 +183,9 @@ int gs_hip_adam(gs_hip_workspace* ws, const gs_hip_frame* frame);
  * slots in use (updated); counts = {freed, spawned placed, pruned}
  * (train_densify_mark.comp, train_densify_spawn.comp, trainer.cpp:1083-1119). */
 int gs_hip_densify(gs_hip_workspace* ws, const gs_hip_frame* frame, uint32_t* slots, uint32_t counts[3]);
+/* Caps every alive opacity logit at max_logit and clears its Adam moments
+ * (train_reset_opacity.comp:17-23). */
+int gs_hip_reset_opacity(gs_hip_workspace* ws, const gs_hip_frame* frame, float max_logit);
 /* One full training view-step: project, bin, forward, loss, backward, adam. */
 int gs_hip_train_step(gs_hip_workspace* ws, const gs_hip_frame* frame, uint64_t* pairs, double losses[3]);
 /* Forward only (render): project, bin, forward; image as RGBA8 (clamped rgb, alpha = 1 - T). */

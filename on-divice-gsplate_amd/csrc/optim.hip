@@ -385,6 +385,15 @@ __global__ void __launch_bounds__(256) k_densify_spawn(gs_hip_frame f, DensifyAr
   a.params[target * 4u + 3u] = p3;
 }
 
+__global__ void __launch_bounds__(256) k_reset_opacity(uint32_t slots, float max_logit, float4* __restrict__ params,
+                                                       float4* __restrict__ m_params, float4* __restrict__ v_params) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= slots || params[i * 4u + 1u].w < 0.5f) return;
+  params[i * 4u].w = fminf(params[i * 4u].w, max_logit);
+  m_params[i * 4u].w = 0.0f;
+  v_params[i * 4u].w = 0.0f;
+}
+
 __global__ void __launch_bounds__(256) k_zero_stats(float4* __restrict__ stats, uint32_t count) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) stats[i] = make_float4(0, 0, 0, 0);
@@ -472,6 +481,16 @@ int gs_hip_densify(gs_hip_workspace* ws, const gs_hip_frame* f, uint32_t* slots,
     counts[1] = placed;
     counts[2] = pruned;
   }
+  return 0;
+}
+
+int gs_hip_reset_opacity(gs_hip_workspace* ws, const gs_hip_frame* f, float max_logit) {
+  if (ws == nullptr || f == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "reset_opacity: null argument");
+  if (f->slots == 0) return 0;
+  k_reset_opacity<<<gsh::div_up(f->slots, 256), 256, 0, ws->stream>>>(
+      f->slots, max_logit, reinterpret_cast<float4*>(ws->params), reinterpret_cast<float4*>(ws->m_params),
+      reinterpret_cast<float4*>(ws->v_params));
+  GS_CHECK_LAUNCH("k_reset_opacity");
   return 0;
 }
 

@@ -32,7 +32,7 @@ EXPORTED = [
     "gs_hip_memset", "gs_hip_synchronize", "gs_hip_last_error", "gs_hip_workspace_create",
     "gs_hip_workspace_destroy", "gs_hip_workspace_buffer", "gs_hip_workspace_reserve_pairs",
     "gs_hip_workspace_load", "gs_hip_workspace_store", "gs_hip_project", "gs_hip_bin", "gs_hip_forward",
-    "gs_hip_set_target", "gs_hip_set_target_device", "gs_hip_loss", "gs_hip_backward", "gs_hip_adam", "gs_hip_densify", "gs_hip_train_step",
+    "gs_hip_set_target", "gs_hip_set_target_device", "gs_hip_loss", "gs_hip_backward", "gs_hip_adam", "gs_hip_densify", "gs_hip_reset_opacity", "gs_hip_train_step",
     "gs_hip_render", "gs_hip_workspace_timings",
 ]
 
@@ -86,7 +86,8 @@ def library() -> ctypes.CDLL:
             "gs_hip_set_target": [vp, ctypes.POINTER(ctypes.c_uint8)], "gs_hip_set_target_device": [vp, vp],
             "gs_hip_loss": [vp, frame_p, dp],
             "gs_hip_backward": [vp, frame_p], "gs_hip_adam": [vp, frame_p],
-            "gs_hip_densify": [vp, frame_p, u32p, u32p], "gs_hip_train_step": [vp, frame_p, u64p, dp],
+            "gs_hip_densify": [vp, frame_p, u32p, u32p],
+            "gs_hip_reset_opacity": [vp, frame_p, ctypes.c_float], "gs_hip_train_step": [vp, frame_p, u64p, dp],
             "gs_hip_render": [vp, frame_p, ctypes.POINTER(ctypes.c_uint8)],
             "gs_hip_workspace_timings": [vp, i32, ctypes.POINTER(ctypes.c_char_p), dp, i32,
                                          ctypes.POINTER(ctypes.c_int)],

@@ -1,0 +1,94 @@
+// Scene upload and rendering for gs_render (the role of
+// engine/src/splat_renderer.cpp:67-348 and offscreen.cpp): the scene goes
+// into a workspace once; each render is project, bin and front-to-back
+// composite on the GPU, then RGBA8 over the clear colour.
+#include "gs/renderer.h"
+
+#include <algorithm>
+#include <cstring>
+#include <fstream>
+#include <stdexcept>
+#include <string>
+
+#include "gs_hip.h"
+
+namespace gs {
+
+GpuScene::GpuScene(Device& device, const SplatScene& scene, uint32_t width, uint32_t height)
+    : count_(scene.count), width_(width), height_(height), sh_degree_(scene.sh_degree) {
+  (void)device;
+  check_status(gs_hip_workspace_create(&ws_, std::max<uint32_t>(scene.count, 1), scene.sh_degree, width, height));
+  std::vector<float> p(size_t(scene.count) * 16, 0.0f);
+  for (uint32_t i = 0; i < scene.count; ++i) {
+    float* d = &p[size_t(i) * 16];
+    for (int k = 0; k < 3; ++k) {
+      d[k] = scene.positions[i * 3 + k];
+      d[4 + k] = scene.log_scales[i * 3 + k];
+      d[12 + k] = scene.sh_dc[i * 3 + k];
+    }
+    d[3] = scene.opacity_logits[i];
+    d[7] = 1.0f;
+    for (int k = 0; k < 4; ++k) d[8 + k] = scene.rotations[i * 4 + k];
+  }
+  const int coeffs = sh_rest_coeffs(scene.sh_degree);
+  const int rc = gs_hip_workspace_load(ws_, p.data(), coeffs > 0 ? scene.sh_rest.data() : nullptr, scene.count,
+                                       uint32_t(coeffs));
+  if (rc != 0) {
+    char msg[512];
+    gs_hip_last_error(msg, sizeof(msg));
+    gs_hip_workspace_destroy(ws_);
+    throw std::runtime_error(msg);
+  }
+}
+
+GpuScene::~GpuScene() {
+  if (ws_ != nullptr) gs_hip_workspace_destroy(ws_);
+}
+
+std::unique_ptr<GpuScene> upload_scene(Device& device, const SplatScene& scene, uint32_t width, uint32_t height) {
+  return std::make_unique<GpuScene>(device, scene, width, height);
+}
+
+std::vector<uint8_t> SplatRenderer::render(const Camera& camera, const float clear[3]) {
+  if (scene_ == nullptr) throw std::logic_error("SplatRenderer::render: no scene attached");
+  if (camera.width != scene_->width() || camera.height != scene_->height()) {
+    throw std::invalid_argument("SplatRenderer::render: camera size differs from the scene's output size");
+  }
+  gs_hip_frame f;
+  std::memset(&f, 0, sizeof(f));
+  std::memcpy(f.cam.rotation, camera.rotation, sizeof(f.cam.rotation));
+  std::memcpy(f.cam.translation, camera.translation, sizeof(f.cam.translation));
+  std::memcpy(f.cam.position, camera.position, sizeof(f.cam.position));
+  f.cam.fx = camera.fx, f.cam.fy = camera.fy, f.cam.cx = camera.cx, f.cam.cy = camera.cy;
+  f.cam.width = camera.width, f.cam.height = camera.height;
+  f.tiles_x = (camera.width + GS_HIP_TILE - 1) / GS_HIP_TILE;
+  f.tiles_y = (camera.height + GS_HIP_TILE - 1) / GS_HIP_TILE;
+  f.slots = scene_->count();
+  f.sh_degree = uint32_t(scene_->sh_degree());
+  for (int c = 0; c < 3; ++c) f.background[c] = clear[c];
+  std::vector<uint8_t> out(size_t(camera.width) * camera.height * 4);
+  check_status(gs_hip_render(scene_->workspace(), &f, out.data()));
+  void* tiles = nullptr;
+  size_t bytes = 0;
+  check_status(gs_hip_workspace_buffer(scene_->workspace(), GS_HIP_BUF_TILES, &tiles, &bytes));
+  std::vector<uint32_t> t(scene_->count());
+  if (!t.empty()) check_status(gs_hip_download(t.data(), tiles, t.size() * 4));
+  visible_ = 0;
+  for (uint32_t v : t) visible_ += v > 0 ? 1 : 0;
+  return out;
+}
+
+std::vector<uint8_t> encode_ppm(const std::vector<uint8_t>& rgba, uint32_t width, uint32_t height) {
+  if (rgba.size() != size_t(width) * height * 4) throw std::invalid_argument("encode_ppm: size mismatch");
+  const std::string header = "P6\n" + std::to_string(width) + " " + std::to_string(height) + "\n255\n";
+  std::vector<uint8_t> out(header.begin(), header.end());
+  out.reserve(out.size() + size_t(width) * height * 3);
+  for (size_t p = 0; p < size_t(width) * height; ++p) out.insert(out.end(), &rgba[p * 4], &rgba[p * 4 + 3]);
+  return out;
+}
+
+void write_ppm(const std::string& path, const std::vector<uint8_t>& rgba, uint32_t width, uint32_t height) {
+  write_file(path, encode_ppm(rgba, width, height));
+}
+
+}  // namespace gs

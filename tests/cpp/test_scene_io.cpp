@@ -1,0 +1,182 @@
+// Scene I/O and camera conventions of the C++ API, without a GPU:
+// PLY round trip (bit-equal), ASCII PLY rejected, the reference's SPZ v2
+// known answers (engine/tests/test_splats.cpp:34-130), SPZ v3 quaternions,
+// v4 rejected by name, the axis flip and dataset cameras.
+#include <zlib.h>
+
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <stdexcept>
+#include <string>
+
+#include "gs/camera.h"
+#include "gs/dataset.h"
+#include "gs/splat_scene.h"
+#include "test_util.h"
+
+namespace {
+
+std::vector<uint8_t> gzip(const std::vector<uint8_t>& data) {
+  z_stream zs{};
+  deflateInit2(&zs, Z_DEFAULT_COMPRESSION, Z_DEFLATED, 16 + MAX_WBITS, 8, Z_DEFAULT_STRATEGY);
+  std::vector<uint8_t> out(deflateBound(&zs, data.size()) + 64);
+  zs.next_in = const_cast<Bytef*>(data.data());
+  zs.avail_in = uInt(data.size());
+  zs.next_out = out.data();
+  zs.avail_out = uInt(out.size());
+  deflate(&zs, Z_FINISH);
+  out.resize(zs.total_out);
+  deflateEnd(&zs);
+  return out;
+}
+
+gs::SplatScene random_scene(uint32_t n, int degree, uint32_t seed) {
+  gs::SplatScene s;
+  s.resize(n, degree);
+  std::mt19937 rng(seed);
+  std::uniform_real_distribution<float> u(-1.0f, 1.0f);
+  for (auto* v : {&s.positions, &s.log_scales, &s.rotations, &s.opacity_logits, &s.sh_dc, &s.sh_rest})
+    for (float& x : *v) x = u(rng);
+  return s;
+}
+
+void test_ply_roundtrip() {
+  for (int degree = 0; degree <= 3; ++degree) {
+    const gs::SplatScene s = random_scene(257, degree, 3 + degree);
+    const std::vector<uint8_t> bytes = gs::save_ply(s);
+    const gs::SplatScene r = gs::load_ply(bytes.data(), bytes.size());
+    EXPECT(r.count == s.count && r.sh_degree == degree, "ply header: %u %d", r.count, r.sh_degree);
+    EXPECT(r.positions == s.positions && r.log_scales == s.log_scales && r.rotations == s.rotations &&
+               r.opacity_logits == s.opacity_logits && r.sh_dc == s.sh_dc && r.sh_rest == s.sh_rest,
+           "ply round trip differs at degree %d", degree);
+    const gs::SplatScene d = gs::load_splat_file(bytes.data(), bytes.size());
+    EXPECT(d.count == s.count, "load_splat_file detects PLY");
+  }
+  bool threw = false;
+  const std::string ascii = "ply\nformat ascii 1.0\nelement vertex 1\nproperty float x\nend_header\n0\n";
+  try {
+    gs::load_ply(reinterpret_cast<const uint8_t*>(ascii.data()), ascii.size());
+  } catch (const std::runtime_error&) {
+    threw = true;
+  }
+  EXPECT(threw, "ascii PLY must be rejected");
+}
+
+void test_spz_v2() {
+  std::vector<uint8_t> raw;
+  auto put32 = [&](uint32_t v) {
+    for (int i = 0; i < 4; ++i) raw.push_back(uint8_t(v >> (8 * i)));
+  };
+  put32(0x5053474e);
+  put32(2);
+  put32(2);
+  raw.push_back(1);   // SH degree
+  raw.push_back(12);  // fractional bits
+  raw.push_back(0);
+  raw.push_back(0);
+  const float pos[2][3] = {{1.5f, -2.25f, 0.125f}, {-0.5f, 0.0f, 3.0f}};
+  for (const auto& p : pos)
+    for (float v : p) {
+      const int32_t fixed = int32_t(std::lround(v * 4096.0f));
+      for (int i = 0; i < 3; ++i) raw.push_back(uint8_t(fixed >> (8 * i)));
+    }
+  raw.push_back(255);
+  raw.push_back(128);
+  for (int i = 0; i < 6; ++i) raw.push_back(uint8_t(64 + i * 20));
+  for (int i = 0; i < 6; ++i) raw.push_back(uint8_t(100 + i));
+  for (uint8_t b : {128, 128, 128, 255, 128, 128}) raw.push_back(b);
+  for (int i = 0; i < 18; ++i) raw.push_back(uint8_t(128 + i));
+  const std::vector<uint8_t> file = gzip(raw);
+  const gs::SplatScene s = gs::load_spz(file.data(), file.size());
+  EXPECT(s.count == 2 && s.sh_degree == 1, "spz header %u %d", s.count, s.sh_degree);
+  EXPECT(s.convention == gs::AxisConvention::RUB, "spz is RUB");
+  for (int i = 0; i < 2; ++i)
+    for (int k = 0; k < 3; ++k) EXPECT(std::fabs(s.positions[i * 3 + k] - pos[i][k]) < 1e-3f, "spz position");
+  EXPECT(s.opacity_logits[0] > 5.0f, "alpha 255 nearly opaque");
+  EXPECT(std::fabs(s.opacity_logits[1] - std::log(128.0f / 127.0f)) < 1e-3f, "alpha 128");
+  EXPECT(std::fabs(s.sh_dc[0] - (64 / 255.0f - 0.5f) / 0.15f) < 1e-5f, "spz colour");
+  EXPECT(std::fabs(s.log_scales[0] - (100 / 16.0f - 10.0f)) < 1e-5f, "spz scale");
+  EXPECT(std::fabs(s.rotations[0] - 1.0f) < 0.01f, "identity rotation w %f", s.rotations[0]);
+  EXPECT(std::fabs(s.rotations[5] - 1.0f) < 0.01f, "x rotation x %f", s.rotations[5]);
+  EXPECT(std::fabs(s.sh_rest[5] - 5.0f / 128.0f) < 1e-6f, "spz SH layout");
+  bool threw = false;
+  try {
+    const uint8_t v4[8] = {0x4e, 0x47, 0x53, 0x50, 4, 0, 0, 0};
+    gs::load_splat_file(v4, sizeof(v4));
+  } catch (const std::runtime_error& e) {
+    threw = std::string(e.what()).find("v4") != std::string::npos;
+  }
+  EXPECT(threw, "spz v4 rejected by name");
+}
+
+void test_spz_v3_quaternion() {
+  // Smallest-three packing of (w, x, y, z) = (0.8, 0.6, 0, 0): largest is w (index 3 in xyzw).
+  std::vector<uint8_t> raw;
+  auto put32 = [&](uint32_t v) {
+    for (int i = 0; i < 4; ++i) raw.push_back(uint8_t(v >> (8 * i)));
+  };
+  put32(0x5053474e);
+  put32(3);
+  put32(1);
+  raw.push_back(0);
+  raw.push_back(12);
+  raw.push_back(0);
+  raw.push_back(0);
+  for (int i = 0; i < 9; ++i) raw.push_back(0);
+  raw.push_back(200);
+  for (int i = 0; i < 3; ++i) raw.push_back(128);
+  for (int i = 0; i < 3; ++i) raw.push_back(100);
+  // components x (i=0), y (1), z (2) packed from the last down: z first in the low bits.
+  const uint32_t mask = 511;
+  auto mag = [&](float v) { return uint32_t(std::lround(std::fabs(v) / 0.70710678f * mask)); };
+  uint32_t bits = 0;
+  bits |= mag(0.0f);                 // z
+  bits |= mag(0.0f) << 10;           // y
+  bits |= mag(0.6f) << 20;           // x
+  bits |= 3u << 30;                  // largest: w
+  put32(bits);
+  const std::vector<uint8_t> file = gzip(raw);
+  const gs::SplatScene s = gs::load_spz(file.data(), file.size());
+  EXPECT(std::fabs(s.rotations[0] - 0.8f) < 2e-3f && std::fabs(s.rotations[1] - 0.6f) < 2e-3f,
+         "v3 quaternion (%f %f %f %f)", s.rotations[0], s.rotations[1], s.rotations[2], s.rotations[3]);
+}
+
+void test_convention_flip() {
+  gs::SplatScene s = random_scene(50, 3, 9);
+  const gs::SplatScene orig = s;
+  gs::convert_convention(s, gs::AxisConvention::RUB);
+  EXPECT(s.convention == gs::AxisConvention::RUB, "flip sets convention");
+  EXPECT(s.positions[1] == -orig.positions[1] && s.positions[2] == -orig.positions[2], "flip negates y, z");
+  gs::convert_convention(s, gs::AxisConvention::RDF);
+  // Two half turns: positions and SH back; the quaternion is negated (same rotation).
+  EXPECT(s.positions == orig.positions && s.sh_rest == orig.sh_rest, "double flip restores");
+  for (size_t i = 0; i < s.rotations.size(); ++i) EXPECT(s.rotations[i] == -orig.rotations[i], "quat sign");
+}
+
+void test_dataset_camera() {
+  gs::Dataset ds = gs::make_orbit_dataset(8, 64, 48, 3.0f, 0.5f, 60.0f, 4);
+  EXPECT(ds.holdout_count() == 2 && ds.train_count() == 6, "holdout every 4th");
+  EXPECT(std::fabs(ds.camera_extent - 1.1f * 3.0f) < 1e-4f, "extent %f", ds.camera_extent);
+  for (const gs::DatasetFrame& f : ds.frames) {
+    const gs::Camera c = gs::dataset_camera(ds, f);
+    // The origin projects to the image centre, in front of the camera.
+    const float z = c.translation[2];
+    EXPECT(z > 2.9f && z < 3.1f, "depth of the origin %f", z);
+    EXPECT(std::fabs(c.translation[0]) < 1e-4f && std::fabs(c.translation[1]) < 1e-4f, "origin on axis");
+    // Rows orthonormal, y down: scene +Y maps to negative camera y.
+    EXPECT(c.rotation[4] < 0.0f, "y down");
+  }
+}
+
+}  // namespace
+
+int main() {
+  test_ply_roundtrip();
+  test_spz_v2();
+  test_spz_v3_quaternion();
+  test_convention_flip();
+  test_dataset_camera();
+  if (test::failures() == 0) std::printf("test_scene_io: ok\n");
+  return test::failures() == 0 ? 0 : 1;
+}

@@ -1,0 +1,29 @@
+"""Runs the C++ API tests (tests/cpp) built by the package Makefile: scene
+I/O without a GPU, the gs::Trainer on the GPU."""
+import os
+import subprocess
+
+import pytest
+
+from helpers import ROOT
+
+BUILD = os.path.join(ROOT, "on-divice-gsplate_amd", "_build")
+
+
+def _run(name, timeout):
+    exe = os.path.join(BUILD, name)
+    if not os.path.exists(exe):
+        pytest.fail(f"{exe} is not built: run make -C on-divice-gsplate_amd tests")
+    print(f"[case] {name}", flush=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=timeout)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_scene_io():
+    _run("test_scene_io", 120)
+
+
+@pytest.mark.gpu
+def test_trainer_api():
+    _run("test_trainer", 300)
