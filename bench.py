@@ -189,6 +189,8 @@ def main():
         d[1] += 1
     kernels = {k: {"ms_avg": v[0] / v[1], "launches": v[1], "ms_per_step": v[0] / args.steps}
                for k, v in per_kernel.items()}
+    if any(v["launches"] < args.steps for k, v in kernels.items() if k != "densify"):
+        log("warning: fewer timed launches than steps for some kernels")
     tiles_touched = ws.download(gs.BUF_TILES, np.uint32, state["slots"])
     n_visible = int((tiles_touched > 0).sum())
     for k, v in kernels.items():

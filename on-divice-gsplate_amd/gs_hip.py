@@ -311,10 +311,11 @@ class Workspace:
         return out
 
     def timings(self, enable: bool = True):
-        names = (ctypes.c_char_p * 64)()
-        ms = (ctypes.c_double * 64)()
+        cap = 1 << 16
+        names = (ctypes.c_char_p * cap)()
+        ms = (ctypes.c_double * cap)()
         count = ctypes.c_int()
-        check(self.lib.gs_hip_workspace_timings(self.handle, int(enable), names, ms, 64, ctypes.byref(count)))
+        check(self.lib.gs_hip_workspace_timings(self.handle, int(enable), names, ms, cap, ctypes.byref(count)))
         return [(names[i].decode(), ms[i]) for i in range(count.value)]
 
 

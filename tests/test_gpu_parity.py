@@ -201,6 +201,7 @@ def test_adjoint_matches_oracle():
     target = smooth_target(w, h, 51)
     with gs.Workspace(n, deg, w, h) as ws:
         ws.load(params, sh)
+        ws.set_target(target)
         frame = gs.make_frame(cam, n, deg, tp=frozen())
         ws.train_step(frame)
         g2 = ws.download(gs.BUF_GRAD2D, np.float32, 9 * n).reshape(9, n).T.astype(np.float64)
@@ -232,6 +233,7 @@ def test_gradients_against_fp64_reference_and_finite_differences():
     bg = (0.2, 0.1, 0.0)
     with gs.Workspace(n, deg, w, h) as ws:
         ws.load(params, sh)
+        ws.set_target(target)
         frame = gs.make_frame(cam, n, deg, background=bg, tp=frozen(0.2))
         _, losses = ws.train_step(frame)
         img = ws.download(gs.BUF_IMAGE, np.float32).reshape(-1, 4)
@@ -270,6 +272,7 @@ def test_adam_step_matches_oracle():
     target = smooth_target(w, h, 71)
     with gs.Workspace(n, deg, w, h) as ws:
         ws.load(params, sh)
+        ws.set_target(target)
         frame0 = gs.make_frame(cam, n, deg, tp=frozen())
         ws.train_step(frame0)  # step 0, lr 0: m, v hold the first gradient
         g, gsh = _first_step_gradients(ws, n)
@@ -325,7 +328,7 @@ def test_densify_masks_and_slots():
     assert placed == min(len(spawners), len(free_list) + cap - n) and placed > 0
     assert slots == n + max(0, placed - len(free_list))
     assert np.all(st_after == 0)
-    assert np.all(p_after[prune.astype(bool), 3] == -20.0)
+    assert np.all(p_after[:n][prune.astype(bool), 3] == -20.0)
     seed = frame.seed
     for k, i in enumerate(spawners[:placed]):
         target = free_list[k] if k < len(free_list) else n + k - len(free_list)
