@@ -1,11 +1,14 @@
 #!/usr/bin/env python3
 """Training-step throughput of the MI355X 3DGS trainer on one GPU.
 
-Workload: 1M gaussians, SH degree 3, fp32, 1280x720, one view per step
-(fwd + bwd + SSIM + Adam, densification every 100 steps), targets rendered
-from a seeded 1M-gaussian ground-truth scene with 64 orbit cameras, training
-from an independently seeded 1M-gaussian scene.  Synthetic data, inputs
-resident in HBM before the timed region.
+Workload: 1M gaussians, SH degree 3, fp32, 1280x720, fwd + bwd + SSIM + Adam
+with densification every 100 view-iterations; targets rendered from a seeded
+1M-gaussian ground-truth scene at 64 seeded jittered cameras (radius 2.5-3.5,
+height -0.5-1.5), training from an independently seeded 1M-gaussian scene.
+Measured twice on the same views: plain (one view per optimizer step) and
+batched (8 views per optimizer step, each view on its own stream, one Adam on
+the mean gradient).  Synthetic data, inputs resident in HBM before the timed
+region.
 
 Prints one JSON line: the metric, per-kernel HIP-event times over the timed
 region, the roofline figure of the dominant kernel and of the whole step, and
@@ -106,18 +109,71 @@ def cpu_baseline(gs, orc, params, sh, cam, target, degree, budget_s):
     return dt, int(len(pt))
 
 
+def run(gs, ws, cams, target_bufs, order, per_step, views_total, warmup_views, deg, tp, densify_every, state,
+        timed):
+    """Trains for `views_total` view-iterations, `per_step` views per
+    optimizer step (1: the plain single-view step).  Returns elapsed seconds."""
+    steps = views_total // per_step
+    t0 = None
+    for s in range(-(warmup_views // per_step), steps):
+        if s == 0:
+            gs.synchronize()
+            if timed:
+                ws.timings(enable=True)
+            state["pairs"] = 0
+            state["slots_start"] = state["slots"]
+            t0 = time.perf_counter()
+        base = state["views"]
+        vs = [int(order[(base + k) % len(order)]) for k in range(per_step)]
+        frames = [gs.make_frame(cams[v], state["slots"], deg, step=state["step"], tp=tp) for v in vs]
+        if per_step == 1:
+            ws.set_target_device(target_bufs[vs[0]].ptr)
+            pairs, _ = ws.train_step(frames[0], want_losses=False)
+            pairs = [pairs]
+        else:
+            pairs, _ = ws.train_views(frames, [target_bufs[v].ptr for v in vs], want_losses=False)
+        if s >= 0:
+            state["pairs"] += sum(pairs)
+        state["views"] += per_step
+        state["step"] += 1
+        if densify_every > 0 and state["views"] // densify_every != base // densify_every:
+            state["slots"], _ = ws.densify(frames[0])
+            state["densified"] += 1
+        if s >= 0 and (s + 1) % max(1, steps // 4) == 0:
+            log(f"  {per_step}-view steps: {s + 1}/{steps}")
+    gs.synchronize()
+    return time.perf_counter() - t0, steps
+
+
+def kernel_table(events, steps, slots_mean, n_visible, mean_pairs_per_view, per_step, pixels, coeffs):
+    per_kernel = {}
+    for name, ms in events:
+        d = per_kernel.setdefault(name, [0.0, 0])
+        d[0] += ms
+        d[1] += 1
+    kernels = {k: {"ms_avg": v[0] / v[1], "launches": v[1], "ms_per_step": v[0] / steps} for k, v in per_kernel.items()}
+    for k, v in kernels.items():
+        b = kernel_bytes(k, slots_mean, n_visible, mean_pairs_per_view, pixels, coeffs)
+        if b is not None:
+            v["algorithmic_bytes"] = int(b)
+            v["GBps"] = b / (v["ms_avg"] * 1e-3) / 1e9
+    return kernels
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=50, help="timed batched optimizer steps")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed batched optimizer steps")
+    ap.add_argument("--batch", type=int, default=8, help="views per batched optimizer step")
     ap.add_argument("--gaussians", type=int, default=1_000_000)
     ap.add_argument("--width", type=int, default=1280)
     ap.add_argument("--height", type=int, default=720)
     ap.add_argument("--sh-degree", type=int, default=3)
     ap.add_argument("--views", type=int, default=64)
-    ap.add_argument("--densify-interval", type=int, default=100)
+    ap.add_argument("--densify-every", type=int, default=100, help="view-iterations between densifications")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build (multi-GPU "
@@ -129,9 +185,9 @@ def main():
     log(f"device {device}")
     N, W, H, deg = args.gaussians, args.width, args.height, args.sh_degree
     coeffs = SH_COEFFS[deg]
-    cams = gs.orbit_cameras(args.views, W, H)
+    cams = gs.jitter_cameras(args.views, W, H, 3003)
 
-    log(f"ground truth: {N} gaussians, {args.views} views at {W}x{H}")
+    log(f"ground truth: {N} gaussians, {args.views} jittered views at {W}x{H}")
     gt_params, gt_sh = gs.synthetic_scene(N, deg, 1001)
     targets = []
     with gs.Workspace(N, deg, W, H) as gt:
@@ -143,111 +199,107 @@ def main():
     target_bufs = [gs.DeviceBuffer(t) for t in targets]
     del gt_params, gt_sh
 
-    log("training scene")
     params, sh = gs.synthetic_scene(N, deg, 2002)
-    capacity = 2 * N
-    ws = gs.Workspace(capacity, deg, W, H)
-    ws.load(params, sh)
-    tp = gs.TrainParams()
     order = np.random.default_rng(3003).permutation(args.views)
-    state = {"slots": N, "densified": 0, "pairs": 0}
+    tp = gs.TrainParams()
+    views_total = args.steps * args.batch
+    warmup_views = args.warmup * args.batch
+    results = {}
+    for mode, per_step in (("plain", 1), ("batched", args.batch)):
+        if mode == "plain" and args.no_plain:
+            continue
+        log(f"{mode}: {views_total} view-iterations, {per_step} view(s) per optimizer step")
+        ws = gs.Workspace(2 * N, deg, W, H)
+        ws.set_views(per_step)
+        ws.load(params, sh)
+        state = {"slots": N, "densified": 0, "pairs": 0, "views": 0, "step": 0}
+        elapsed, steps = run(gs, ws, cams, target_bufs, order, per_step, views_total, warmup_views, deg, tp,
+                             args.densify_every, state, timed=True)
+        events = ws.timings(enable=False)
+        views_per_s = steps * per_step / elapsed
+        mean_pairs = state["pairs"] / (steps * per_step)
+        slots_mean = 0.5 * (state["slots_start"] + state["slots"])
+        tiles_touched = ws.download(gs.BUF_TILES, np.uint32, state["slots"])
+        n_visible = int((tiles_touched > 0).sum())
+        results[mode] = {
+            "views_per_s": views_per_s, "optimizer_steps_per_s": steps / elapsed, "ms_per_step": 1000 * elapsed / steps,
+            "mean_pairs": mean_pairs, "slots_start": state["slots_start"], "slots_end": state["slots"],
+            "densify_events": state["densified"], "slots_mean": slots_mean, "n_visible": n_visible,
+            "kernels": kernel_table(events, steps, slots_mean, n_visible, mean_pairs, per_step, W * H, coeffs),
+        }
+        log(f"{mode}: {views_per_s:.1f} view-iterations/s ({1000 * elapsed / steps:.3f} ms per optimizer step)")
+        ws.close()
 
-    def step(t):
-        v = int(order[t % args.views])
-        ws.set_target_device(target_bufs[v].ptr)
-        frame = gs.make_frame(cams[v], state["slots"], deg, step=t, tp=tp)
-        pairs, _ = ws.train_step(frame, want_losses=False)
-        state["pairs"] += pairs
-        if args.densify_interval > 0 and (t + 1) % args.densify_interval == 0:
-            state["slots"], _ = ws.densify(frame)
-            state["densified"] += 1
-        return pairs
-
-    for t in range(args.warmup):
-        step(t)
-    gs.synchronize()
-    log(f"warmup done ({args.warmup} steps); timing {args.steps} steps")
-    ws.timings(enable=True)
-    state["pairs"] = 0
-    slots_before = state["slots"]
-    t0 = time.perf_counter()
-    for t in range(args.warmup, args.warmup + args.steps):
-        step(t)
-        if (t - args.warmup + 1) % 50 == 0:
-            log(f"step {t - args.warmup + 1}/{args.steps}")
-    gs.synchronize()
-    elapsed = time.perf_counter() - t0
-    events = ws.timings(enable=False)
-    iters = args.steps / elapsed
-    mean_pairs = state["pairs"] / args.steps
-    slots_mean = 0.5 * (slots_before + state["slots"])
-
-    per_kernel = {}
-    for name, ms in events:
-        d = per_kernel.setdefault(name, [0.0, 0])
-        d[0] += ms
-        d[1] += 1
-    kernels = {k: {"ms_avg": v[0] / v[1], "launches": v[1], "ms_per_step": v[0] / args.steps}
-               for k, v in per_kernel.items()}
-    if any(v["launches"] < args.steps for k, v in kernels.items() if k != "densify"):
-        log("warning: fewer timed launches than steps for some kernels")
-    tiles_touched = ws.download(gs.BUF_TILES, np.uint32, state["slots"])
-    n_visible = int((tiles_touched > 0).sum())
-    for k, v in kernels.items():
-        b = kernel_bytes(k, slots_mean, n_visible, mean_pairs, W * H, coeffs)
-        if b is not None:
-            v["algorithmic_bytes"] = int(b)
-            v["GBps"] = b / (v["ms_avg"] * 1e-3) / 1e9
-    dominant = max(kernels, key=lambda k: kernels[k]["ms_per_step"]) if kernels else None
+    main_mode = "batched"
+    r = results[main_mode]
+    kernels = r["kernels"]
+    per_view = {k: v["ms_per_step"] / args.batch for k, v in kernels.items()}
+    dominant = max(per_view, key=per_view.get) if per_view else None
     roofline = None
     if dominant is not None and "GBps" in kernels[dominant]:
         ach = kernels[dominant]["GBps"]
         roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK / 1e9,
                     "unit": "GB/s", "frac": round(ach * 1e9 / HBM_PEAK, 5), "traffic": None,
                     "ms_avg": round(kernels[dominant]["ms_avg"], 4)}
-    sb = step_bytes(slots_mean, mean_pairs, W * H, deg)
+    # Whole-step bytes per view-iteration; the batched step reads and writes the
+    # optimizer state once per `batch` views.
+    D = 4 * (11 + 3 * (deg + 1) ** 2)
+    adam_bytes = 6 * D + 36
+    sb_plain = step_bytes(r["slots_mean"], r["mean_pairs"], W * H, deg)
+    sb = sb_plain - adam_bytes * r["slots_mean"] * (1.0 - 1.0 / args.batch)
 
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline: the oracle's single-threaded step on one view")
         orc = _load("gs_oracle", "oracle/oracle.py")
-        p0, s0 = gs.synthetic_scene(N, deg, 2002)
-        dt, cpu_pairs = cpu_baseline(gs, orc, p0, s0, cams[int(order[0])], targets[int(order[0])], deg, 30.0)
+        dt, cpu_pairs = cpu_baseline(gs, orc, params.copy(), sh.copy(), cams[int(order[0])], targets[int(order[0])],
+                                     deg, 30.0)
         cpu = {"value": round(1.0 / dt, 5), "unit": "iter/s", "cores": 1, "kind": "port",
                "sample": f"one full training step (1 view, {N} gaussians, SH{deg}, {W}x{H}, {cpu_pairs} pairs) "
                          f"through the oracle's C restatement in {dt:.1f} s on {platform.processor() or 'host'} "
                          f"({os.cpu_count()} cores present, 1 used)"}
 
+    plain = results.get("plain")
     out = {
-        "metric": "train iter/s (fwd+bwd+SSIM+Adam) @1M Gaussians 1280x720",
-        "value": round(iters, 3),
-        "unit": "iter/s",
+        "metric": "train iter/s (fwd+bwd+SSIM+Adam) @1M Gaussians 1280x720; batched vs plain, %HBM",
+        "value": round(r["views_per_s"], 3),
+        "unit": "iter/s (view-iterations)",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1000.0 / iters, 4),
+        "ms_per_step": round(r["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
-        "data": "synthetic (seeded 1M-gaussian GT scene, 64 orbit views rendered by this trainer)",
-        "config": {"workload": "plain single-view training step, 1M gaussians SH3 fp32, 1280x720, 16x16 tiles, "
-                               "0.8 L1 + 0.2 D-SSIM, Adam, densify every 100 steps",
-                   "gaussians_start": N, "gaussians_end": state["slots"], "width": W, "height": H,
-                   "sh_degree": deg, "mean_pairs": int(mean_pairs), "densify_events": state["densified"],
-                   "device": device, "mode": "plain"},
-        "mpix_per_s": round(iters * W * H / 1e6, 2),
-        "step_hbm": {"bytes_per_iter": int(sb), "achieved_GBps": round(sb * iters / 1e9, 2),
-                     "frac": round(sb * iters / HBM_PEAK, 5)},
+        "data": "synthetic (seeded 1M-gaussian GT scene; 64 jittered views rendered by this trainer)",
+        "config": {"workload": f"batched training: {args.batch} views per optimizer step (each view: project, bin, "
+                               "forward, SSIM loss, recompute backward on its own stream; one adjoint + Adam on the "
+                               "mean gradient), 1M gaussians SH3 fp32, 1280x720, 16x16 tiles, 0.8 L1 + 0.2 D-SSIM, "
+                               "densify every 100 view-iterations",
+                   "views_per_step": args.batch, "gaussians_start": N, "gaussians_end": r["slots_end"],
+                   "width": W, "height": H, "sh_degree": deg, "mean_pairs_per_view": int(r["mean_pairs"]),
+                   "densify_events": r["densify_events"], "device": device},
+        "plain": None if plain is None else {
+            "iters_per_s": round(plain["views_per_s"], 3), "ms_per_step": round(plain["ms_per_step"], 4),
+            "mean_pairs": int(plain["mean_pairs"]), "densify_events": plain["densify_events"],
+            "kernels": {k: round(v["ms_avg"], 4) for k, v in plain["kernels"].items()}},
+        "speedup_batched_vs_plain": None if plain is None else round(r["views_per_s"] / plain["views_per_s"], 3),
+        "optimizer_steps_per_s": round(r["optimizer_steps_per_s"], 3),
+        "mpix_per_s": round(r["views_per_s"] * W * H / 1e6, 2),
+        "step_hbm": {"bytes_per_view_iter": int(sb), "achieved_GBps": round(sb * r["views_per_s"] / 1e9, 2),
+                     "frac": round(sb * r["views_per_s"] / HBM_PEAK, 5),
+                     "plain_frac": None if plain is None else round(
+                         step_bytes(plain["slots_mean"], plain["mean_pairs"], W * H, deg) * plain["views_per_s"] /
+                         HBM_PEAK, 5)},
         "roofline": roofline,
         "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in sorted(kernels.items(), key=lambda kv: -kv[1]["ms_per_step"])},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
-    for b in target_bufs:
-        b.free()
-    ws.close()
+    for buf in target_bufs:
+        buf.free()
 
 
 if __name__ == "__main__":

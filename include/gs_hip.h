@@ -41,6 +41,7 @@ extern "C" {
 #define GS_HIP_PARAM_FLOATS 16    /* per slot: [pos, logit | logscale, alive | quat wxyz | dc, -] */
 #define GS_HIP_RECORD_FLOATS 12   /* projected raster record per slot */
 #define GS_HIP_GRAD2D_FLOATS 9    /* dL/d(mean xy, conic abc, opacity, colour rgb), planar */
+#define GS_HIP_MAX_VIEWS 8        /* views of one batched optimizer step */
 
 /* frame flags */
 #define GS_HIP_FLAG_MASKED_TARGET 1u  /* target = mix(bg, photo, alpha) (train_forward.comp:124-127) */
@@ -113,7 +114,9 @@ typedef struct gs_hip_workspace gs_hip_workspace;
  *  TARGET                       uint32[H*W]    (RGBA8)
  *  DLDC                         float[H*W][4]  (dL/dC rgb, -)
  *  TILE_LOSS                    float[tiles][4] (sum |e|, sum e^2, sum SSIM, -)
- *  TOTALS                       uint32[4]      ([0] pairs of the last bin/sort) */
+ *  TOTALS                       uint32[16]     ([0] pairs of the last bin, [1] emit mismatches)
+ *  TILE_ORDER                   uint32[tiles]  (the rasterizers' launch order: longest lists first)
+ * The model buffers (PARAMS .. STATS) are shared; the others belong to a view. */
 enum gs_hip_buffer {
   GS_HIP_BUF_PARAMS = 0,
   GS_HIP_BUF_SH,
@@ -138,6 +141,7 @@ enum gs_hip_buffer {
   GS_HIP_BUF_DLDC,
   GS_HIP_BUF_TILE_LOSS,
   GS_HIP_BUF_TOTALS,
+  GS_HIP_BUF_TILE_ORDER,
   GS_HIP_BUF_COUNT
 };
 
@@ -146,6 +150,12 @@ int gs_hip_workspace_create(gs_hip_workspace** ws, uint32_t capacity, int sh_deg
                             uint32_t height);
 int gs_hip_workspace_destroy(gs_hip_workspace* ws);
 int gs_hip_workspace_buffer(gs_hip_workspace* ws, int which, void** device_ptr, size_t* bytes);
+/* Views: independent sets of per-view buffers, each with its own HIP stream,
+ * over the workspace's one model (view 0 exists from creation).  Grows to
+ * `count` (<= GS_HIP_MAX_VIEWS) views. */
+int gs_hip_workspace_views(gs_hip_workspace* ws, uint32_t count);
+/* A buffer of view `view` (model buffers are the same for every view). */
+int gs_hip_workspace_view_buffer(gs_hip_workspace* ws, uint32_t view, int which, void** device_ptr, size_t* bytes);
 /* Pair buffers for at least `pairs` (tile, gaussian) pairs; grows, never shrinks. */
 int gs_hip_workspace_reserve_pairs(gs_hip_workspace* ws, uint64_t pairs);
 /* Copies `count` slots of parameters (16 floats each) and SH rest coefficients
@@ -188,6 +198,15 @@ int gs_hip_densify(gs_hip_workspace* ws, const gs_hip_frame* frame, uint32_t* sl
 int gs_hip_reset_opacity(gs_hip_workspace* ws, const gs_hip_frame* frame, float max_logit);
 /* One full training view-step: project, bin, forward, loss, backward, adam. */
 int gs_hip_train_step(gs_hip_workspace* ws, const gs_hip_frame* frame, uint64_t* pairs, double losses[3]);
+/* One batched optimizer step over `count` views of the scene (count <= the
+ * workspace's views): each view's project, bin, forward, loss and backward
+ * run on its own stream, concurrently; then one projection adjoint + Adam on
+ * the mean of the views' gradients.  frames[v] differ only in camera (the
+ * optimizer reads frames[0]'s rates and step).  targets[v]: device RGBA8 of
+ * view v, or NULL (targets or an entry) to use what view v's TARGET holds.
+ * pairs[count], losses[3 * count] may be NULL. */
+int gs_hip_train_views(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t count, const void* const* targets,
+                       uint64_t* pairs, double* losses);
 /* Forward only (render): project, bin, forward; image as RGBA8 (clamped rgb, alpha = 1 - T). */
 int gs_hip_render(gs_hip_workspace* ws, const gs_hip_frame* frame, uint8_t* rgba8_out);
 

@@ -30,7 +30,7 @@ void clear_error() {
   g_message.clear();
 }
 
-void timer_begin(gs_hip_workspace* ws, const char* name) {
+void timer_begin(gs_hip_workspace* ws, View& v, const char* name) {
   Timer& t = ws->timer;
   if (!t.enabled) return;
   if (t.used == t.start.size()) {
@@ -41,13 +41,13 @@ void timer_begin(gs_hip_workspace* ws, const char* name) {
     t.names.emplace_back();
   }
   t.names[t.used] = name;
-  hipEventRecord(t.start[t.used], ws->stream);
+  hipEventRecord(t.start[t.used], v.stream);
 }
 
-void timer_end(gs_hip_workspace* ws) {
+void timer_end(gs_hip_workspace* ws, View& v) {
   Timer& t = ws->timer;
   if (!t.enabled || t.used >= t.stop.size()) return;
-  hipEventRecord(t.stop[t.used], ws->stream);
+  hipEventRecord(t.stop[t.used], v.stream);
   ++t.used;
 }
 
@@ -76,17 +76,131 @@ void release(T*& p) {
   p = nullptr;
 }
 
-int zero(gs_hip_workspace* ws, void* p, uint64_t bytes) {
+int zero(gsh::View& v, void* p, uint64_t bytes) {
   if (bytes == 0) return 0;
   const uint64_t words = bytes / 4;
   if (words > 0) {
-    k_fill_u32<<<gsh::div_up(words, 256), 256, 0, ws->stream>>>(static_cast<uint32_t*>(p), words, 0u);
+    k_fill_u32<<<gsh::div_up(words, 256), 256, 0, v.stream>>>(static_cast<uint32_t*>(p), words, 0u);
     GS_CHECK_LAUNCH("k_fill_u32");
   }
   return 0;
 }
 
 }  // namespace
+
+namespace gsh {
+
+int create_view(gs_hip_workspace* ws, View** out) {
+  *out = nullptr;
+  auto* v = new View();
+  const uint64_t cap = std::max<uint32_t>(ws->capacity, 1);
+  const uint64_t pixels = uint64_t(ws->width) * ws->height;
+  const uint64_t tiles = uint64_t(ws->tiles_x) * ws->tiles_y;
+  int rc = 0;
+  auto A = [&](int r) {
+    if (rc == 0) rc = r;
+  };
+  A(hipStreamCreateWithFlags(&v->stream, hipStreamNonBlocking) == hipSuccess
+        ? 0
+        : set_error(GS_HIP_E_HIP, "hipStreamCreate failed"));
+  A(hipEventCreateWithFlags(&v->done, hipEventDisableTiming) == hipSuccess
+        ? 0
+        : set_error(GS_HIP_E_HIP, "hipEventCreate failed"));
+  A(alloc(&v->records, cap * GS_HIP_RECORD_FLOATS));
+  A(alloc(&v->footprint, cap * 4));
+  A(alloc(&v->tiles, cap));
+  A(alloc(&v->depth_keys, cap));
+  A(alloc(&v->depth_order, cap));
+  A(alloc(&v->offsets, cap));
+  A(alloc(&v->grad2d, cap * GS_HIP_GRAD2D_FLOATS));
+  A(alloc(&v->sh_input, cap * 8));
+  A(alloc(&v->flags, cap));
+  A(alloc(&v->scan_tmp, cap + 1));
+  A(alloc(&v->ranges, (tiles + 1) * 2));
+  A(alloc(&v->tile_order, tiles));
+  A(alloc(&v->image, pixels * 4));
+  A(alloc(&v->last, pixels));
+  A(alloc(&v->target, pixels));
+  A(alloc(&v->dldc, pixels * 4));
+  A(alloc(&v->ssim_maps, pixels * 9));
+  A(alloc(&v->tile_loss, tiles * 4));
+  A(alloc(&v->totals, 16));
+  v->block_sums_entries = div_up(cap + 1, 4096) + 1024;
+  A(alloc(&v->block_sums, v->block_sums_entries));
+  if (rc == 0 && hipHostMalloc(reinterpret_cast<void**>(&v->host_totals), 64) != hipSuccess) {
+    rc = set_error(GS_HIP_E_HIP, "hipHostMalloc failed");
+  }
+  if (rc == 0 && hipHostMalloc(reinterpret_cast<void**>(&v->host_tile_loss), tiles * 16) != hipSuccess) {
+    rc = set_error(GS_HIP_E_HIP, "hipHostMalloc failed");
+  }
+  if (rc == 0) rc = reserve_sort(*v, cap);
+  if (rc == 0) rc = zero(*v, v->totals, 64);
+  if (rc == 0) rc = zero(*v, v->target, pixels * 4);
+  if (rc == 0 && hipStreamSynchronize(v->stream) != hipSuccess) rc = set_error(GS_HIP_E_HIP, "view init sync");
+  if (rc != 0) {
+    destroy_view(v);
+    return rc;
+  }
+  *out = v;
+  return 0;
+}
+
+void destroy_view(View* v) {
+  if (v == nullptr) return;
+  if (v->stream != nullptr) hipStreamSynchronize(v->stream);
+  release(v->records);
+  release(v->footprint);
+  release(v->tiles);
+  release(v->depth_keys);
+  release(v->depth_order);
+  release(v->offsets);
+  release(v->grad2d);
+  release(v->sh_input);
+  release(v->flags);
+  release(v->scan_tmp);
+  release(v->pair_tiles);
+  release(v->pair_gaussians);
+  release(v->ranges);
+  release(v->tile_order);
+  release(v->image);
+  release(v->last);
+  release(v->target);
+  release(v->dldc);
+  release(v->ssim_maps);
+  release(v->tile_loss);
+  release(v->totals);
+  release(v->block_sums);
+  release(v->sort.keys_alt);
+  release(v->sort.vals_alt);
+  release(v->sort.hist);
+  release(v->sort.digit_totals);
+  if (v->host_totals != nullptr) hipHostFree(v->host_totals);
+  if (v->host_tile_loss != nullptr) hipHostFree(v->host_tile_loss);
+  if (v->done != nullptr) hipEventDestroy(v->done);
+  if (v->stream != nullptr) hipStreamDestroy(v->stream);
+  delete v;
+}
+
+int reserve_pairs(gs_hip_workspace* ws, View& v, uint64_t pairs) {
+  (void)ws;
+  if (pairs <= v.pair_capacity) return 0;
+  if (pairs >= (uint64_t(1) << 32)) return set_error(GS_HIP_E_CAPACITY, "reserve_pairs: more than 2^32 pairs");
+  // Headroom: views differ in pair count (trainer.cpp:588-603 keeps 1/8 + 1/4).
+  uint64_t n = std::min<uint64_t>(pairs + pairs / 4, (uint64_t(1) << 32) - 1);
+  n = std::max<uint64_t>(n, 1 << 16);
+  GS_HIP_TRY(hipStreamSynchronize(v.stream));
+  release(v.pair_tiles);
+  release(v.pair_gaussians);
+  v.pair_capacity = 0;
+  int rc = alloc(&v.pair_tiles, n);
+  if (rc == 0) rc = alloc(&v.pair_gaussians, n);
+  if (rc == 0) rc = reserve_sort(v, n);
+  if (rc != 0) return rc;
+  v.pair_capacity = n;
+  return 0;
+}
+
+}  // namespace gsh
 
 extern "C" {
 
@@ -176,15 +290,10 @@ int gs_hip_workspace_create(gs_hip_workspace** out, uint32_t capacity, int sh_de
   ws->tiles_y = gsh::div_up(height, GS_HIP_TILE);
   const uint64_t cap = std::max<uint32_t>(capacity, 1);
   const uint64_t shn = cap * ws->sh_coeffs * 3;
-  const uint64_t pixels = uint64_t(width) * height;
-  const uint64_t tiles = uint64_t(ws->tiles_x) * ws->tiles_y;
   int rc = 0;
   auto A = [&](int r) {
     if (rc == 0) rc = r;
   };
-  A(hipStreamCreateWithFlags(&ws->stream, hipStreamNonBlocking) == hipSuccess
-        ? 0
-        : gsh::set_error(GS_HIP_E_HIP, "hipStreamCreate failed"));
   A(alloc(&ws->params, cap * 16));
   A(alloc(&ws->sh, shn));
   A(alloc(&ws->m_params, cap * 16));
@@ -192,38 +301,14 @@ int gs_hip_workspace_create(gs_hip_workspace** out, uint32_t capacity, int sh_de
   A(alloc(&ws->m_sh, shn));
   A(alloc(&ws->v_sh, shn));
   A(alloc(&ws->stats, cap * 4));
-  A(alloc(&ws->records, cap * GS_HIP_RECORD_FLOATS));
-  A(alloc(&ws->footprint, cap * 4));
-  A(alloc(&ws->tiles, cap));
-  A(alloc(&ws->depth_keys, cap));
-  A(alloc(&ws->depth_order, cap));
-  A(alloc(&ws->offsets, cap));
-  A(alloc(&ws->grad2d, cap * GS_HIP_GRAD2D_FLOATS));
-  A(alloc(&ws->flags, cap));
-  A(alloc(&ws->scan_tmp, cap + 1));
-  A(alloc(&ws->ranges, (tiles + 1) * 2));
-  A(alloc(&ws->image, pixels * 4));
-  A(alloc(&ws->last, pixels));
-  A(alloc(&ws->target, pixels));
-  A(alloc(&ws->dldc, pixels * 4));
-  A(alloc(&ws->ssim_maps, pixels * 9));
-  A(alloc(&ws->tile_loss, tiles * 4));
-  A(alloc(&ws->totals, 16));
-  ws->block_sums_entries = gsh::div_up(cap + 1, 4096) + 1024;
-  A(alloc(&ws->block_sums, ws->block_sums_entries));
-  if (rc == 0 && hipHostMalloc(reinterpret_cast<void**>(&ws->host_totals), 64) != hipSuccess) {
-    rc = gsh::set_error(GS_HIP_E_HIP, "hipHostMalloc failed");
-  }
-  if (rc == 0 && hipHostMalloc(reinterpret_cast<void**>(&ws->host_tile_loss), tiles * 16) != hipSuccess) {
-    rc = gsh::set_error(GS_HIP_E_HIP, "hipHostMalloc failed");
-  }
-  if (rc == 0) rc = gsh::reserve_sort(ws, cap);
+  gsh::View* v = nullptr;
+  if (rc == 0) rc = gsh::create_view(ws, &v);
+  if (v != nullptr) ws->views.push_back(v);
   if (rc == 0) {
-    for (void* p : {(void*)ws->params, (void*)ws->m_params, (void*)ws->v_params}) A(zero(ws, p, cap * 64));
-    for (void* p : {(void*)ws->sh, (void*)ws->m_sh, (void*)ws->v_sh}) A(zero(ws, p, shn * 4));
-    A(zero(ws, ws->stats, cap * 16));
-    A(zero(ws, ws->totals, 64));
-    if (hipStreamSynchronize(ws->stream) != hipSuccess) A(gsh::set_error(GS_HIP_E_HIP, "workspace init sync"));
+    for (void* p : {(void*)ws->params, (void*)ws->m_params, (void*)ws->v_params}) A(zero(*v, p, cap * 64));
+    for (void* p : {(void*)ws->sh, (void*)ws->m_sh, (void*)ws->v_sh}) A(zero(*v, p, shn * 4));
+    A(zero(*v, ws->stats, cap * 16));
+    if (rc == 0 && hipStreamSynchronize(v->stream) != hipSuccess) A(gsh::set_error(GS_HIP_E_HIP, "workspace init sync"));
   }
   if (rc != 0) {
     gs_hip_workspace_destroy(ws);
@@ -233,9 +318,25 @@ int gs_hip_workspace_create(gs_hip_workspace** out, uint32_t capacity, int sh_de
   return 0;
 }
 
+int gs_hip_workspace_views(gs_hip_workspace* ws, uint32_t count) {
+  if (ws == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "workspace_views: null workspace");
+  if (count == 0 || count > GS_HIP_MAX_VIEWS) {
+    return gsh::set_error(GS_HIP_E_ARGUMENT, "workspace_views: 1 to " + std::to_string(GS_HIP_MAX_VIEWS) + " views");
+  }
+  while (ws->views.size() < count) {
+    gsh::View* v = nullptr;
+    const int rc = gsh::create_view(ws, &v);
+    if (rc != 0) return rc;
+    ws->views.push_back(v);
+  }
+  return 0;
+}
+
 int gs_hip_workspace_destroy(gs_hip_workspace* ws) {
   if (ws == nullptr) return 0;
-  if (ws->stream != nullptr) hipStreamSynchronize(ws->stream);
+  hipDeviceSynchronize();
+  for (gsh::View* v : ws->views) gsh::destroy_view(v);
+  ws->views.clear();
   release(ws->params);
   release(ws->sh);
   release(ws->m_params);
@@ -243,41 +344,16 @@ int gs_hip_workspace_destroy(gs_hip_workspace* ws) {
   release(ws->m_sh);
   release(ws->v_sh);
   release(ws->stats);
-  release(ws->records);
-  release(ws->footprint);
-  release(ws->tiles);
-  release(ws->depth_keys);
-  release(ws->depth_order);
-  release(ws->offsets);
-  release(ws->grad2d);
-  release(ws->flags);
-  release(ws->scan_tmp);
-  release(ws->pair_tiles);
-  release(ws->pair_gaussians);
-  release(ws->ranges);
-  release(ws->image);
-  release(ws->last);
-  release(ws->target);
-  release(ws->dldc);
-  release(ws->ssim_maps);
-  release(ws->tile_loss);
-  release(ws->totals);
-  release(ws->block_sums);
-  release(ws->sort.keys_alt);
-  release(ws->sort.vals_alt);
-  release(ws->sort.hist);
-  release(ws->sort.digit_totals);
-  if (ws->host_totals != nullptr) hipHostFree(ws->host_totals);
-  if (ws->host_tile_loss != nullptr) hipHostFree(ws->host_tile_loss);
   for (hipEvent_t e : ws->timer.start) hipEventDestroy(e);
   for (hipEvent_t e : ws->timer.stop) hipEventDestroy(e);
-  if (ws->stream != nullptr) hipStreamDestroy(ws->stream);
   delete ws;
   return 0;
 }
 
-int gs_hip_workspace_buffer(gs_hip_workspace* ws, int which, void** ptr, size_t* bytes) {
+int gs_hip_workspace_view_buffer(gs_hip_workspace* ws, uint32_t view, int which, void** ptr, size_t* bytes) {
   if (ws == nullptr || ptr == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "workspace_buffer: null");
+  if (view >= ws->views.size()) return gsh::set_error(GS_HIP_E_ARGUMENT, "workspace_buffer: no such view");
+  const gsh::View& v = *ws->views[view];
   const size_t cap = ws->capacity, pixels = size_t(ws->width) * ws->height;
   const size_t tiles = size_t(ws->tiles_x) * ws->tiles_y, shn = cap * ws->sh_coeffs * 3;
   void* p = nullptr;
@@ -290,22 +366,23 @@ int gs_hip_workspace_buffer(gs_hip_workspace* ws, int which, void** ptr, size_t*
     case GS_HIP_BUF_M_SH: p = ws->m_sh, b = shn * 4; break;
     case GS_HIP_BUF_V_SH: p = ws->v_sh, b = shn * 4; break;
     case GS_HIP_BUF_STATS: p = ws->stats, b = cap * 16; break;
-    case GS_HIP_BUF_RECORDS: p = ws->records, b = cap * GS_HIP_RECORD_FLOATS * 4; break;
-    case GS_HIP_BUF_FOOTPRINT: p = ws->footprint, b = cap * 16; break;
-    case GS_HIP_BUF_TILES: p = ws->tiles, b = cap * 4; break;
-    case GS_HIP_BUF_DEPTH_KEYS: p = ws->depth_keys, b = cap * 4; break;
-    case GS_HIP_BUF_DEPTH_ORDER: p = ws->depth_order, b = cap * 4; break;
-    case GS_HIP_BUF_OFFSETS: p = ws->offsets, b = cap * 4; break;
-    case GS_HIP_BUF_GRAD2D: p = ws->grad2d, b = cap * GS_HIP_GRAD2D_FLOATS * 4; break;
-    case GS_HIP_BUF_PAIR_TILES: p = ws->pair_tiles, b = ws->pair_capacity * 4; break;
-    case GS_HIP_BUF_PAIR_GAUSSIANS: p = ws->pair_gaussians, b = ws->pair_capacity * 4; break;
-    case GS_HIP_BUF_RANGES: p = ws->ranges, b = (tiles + 1) * 8; break;
-    case GS_HIP_BUF_IMAGE: p = ws->image, b = pixels * 16; break;
-    case GS_HIP_BUF_LAST: p = ws->last, b = pixels * 4; break;
-    case GS_HIP_BUF_TARGET: p = ws->target, b = pixels * 4; break;
-    case GS_HIP_BUF_DLDC: p = ws->dldc, b = pixels * 16; break;
-    case GS_HIP_BUF_TILE_LOSS: p = ws->tile_loss, b = tiles * 16; break;
-    case GS_HIP_BUF_TOTALS: p = ws->totals, b = 64; break;
+    case GS_HIP_BUF_RECORDS: p = v.records, b = cap * GS_HIP_RECORD_FLOATS * 4; break;
+    case GS_HIP_BUF_FOOTPRINT: p = v.footprint, b = cap * 16; break;
+    case GS_HIP_BUF_TILES: p = v.tiles, b = cap * 4; break;
+    case GS_HIP_BUF_DEPTH_KEYS: p = v.depth_keys, b = cap * 4; break;
+    case GS_HIP_BUF_DEPTH_ORDER: p = v.depth_order, b = cap * 4; break;
+    case GS_HIP_BUF_OFFSETS: p = v.offsets, b = cap * 4; break;
+    case GS_HIP_BUF_GRAD2D: p = v.grad2d, b = cap * GS_HIP_GRAD2D_FLOATS * 4; break;
+    case GS_HIP_BUF_PAIR_TILES: p = v.pair_tiles, b = v.pair_capacity * 4; break;
+    case GS_HIP_BUF_PAIR_GAUSSIANS: p = v.pair_gaussians, b = v.pair_capacity * 4; break;
+    case GS_HIP_BUF_RANGES: p = v.ranges, b = (tiles + 1) * 8; break;
+    case GS_HIP_BUF_IMAGE: p = v.image, b = pixels * 16; break;
+    case GS_HIP_BUF_LAST: p = v.last, b = pixels * 4; break;
+    case GS_HIP_BUF_TARGET: p = v.target, b = pixels * 4; break;
+    case GS_HIP_BUF_DLDC: p = v.dldc, b = pixels * 16; break;
+    case GS_HIP_BUF_TILE_LOSS: p = v.tile_loss, b = tiles * 16; break;
+    case GS_HIP_BUF_TOTALS: p = v.totals, b = 64; break;
+    case GS_HIP_BUF_TILE_ORDER: p = v.tile_order, b = tiles * 4; break;
     default: return gsh::set_error(GS_HIP_E_ARGUMENT, "workspace_buffer: unknown buffer");
   }
   *ptr = p;
@@ -313,22 +390,16 @@ int gs_hip_workspace_buffer(gs_hip_workspace* ws, int which, void** ptr, size_t*
   return 0;
 }
 
+int gs_hip_workspace_buffer(gs_hip_workspace* ws, int which, void** ptr, size_t* bytes) {
+  return gs_hip_workspace_view_buffer(ws, 0, which, ptr, bytes);
+}
+
 int gs_hip_workspace_reserve_pairs(gs_hip_workspace* ws, uint64_t pairs) {
   if (ws == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "reserve_pairs: null workspace");
-  if (pairs <= ws->pair_capacity) return 0;
-  if (pairs >= (uint64_t(1) << 32)) return gsh::set_error(GS_HIP_E_CAPACITY, "reserve_pairs: more than 2^32 pairs");
-  // Headroom: views differ in pair count (trainer.cpp:588-603 keeps 1/8 + 1/4).
-  uint64_t n = std::min<uint64_t>(pairs + pairs / 4, (uint64_t(1) << 32) - 1);
-  n = std::max<uint64_t>(n, 1 << 16);
-  GS_HIP_TRY(hipStreamSynchronize(ws->stream));
-  release(ws->pair_tiles);
-  release(ws->pair_gaussians);
-  ws->pair_capacity = 0;
-  int rc = alloc(&ws->pair_tiles, n);
-  if (rc == 0) rc = alloc(&ws->pair_gaussians, n);
-  if (rc == 0) rc = gsh::reserve_sort(ws, n);
-  if (rc != 0) return rc;
-  ws->pair_capacity = n;
+  for (gsh::View* v : ws->views) {
+    const int rc = gsh::reserve_pairs(ws, *v, pairs);
+    if (rc != 0) return rc;
+  }
   return 0;
 }
 
@@ -337,16 +408,18 @@ int gs_hip_workspace_load(gs_hip_workspace* ws, const float* params, const float
   if (ws == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "workspace_load: null workspace");
   if (count > ws->capacity) return gsh::set_error(GS_HIP_E_CAPACITY, "workspace_load: more slots than capacity");
   const size_t cap = ws->capacity, shn = cap * ws->sh_coeffs * 3;
+  gsh::View& v0 = gsh::view0(ws);
+  GS_HIP_TRY(hipDeviceSynchronize());
   int rc = 0;
   for (void* p : {(void*)ws->params, (void*)ws->m_params, (void*)ws->v_params}) {
-    if (rc == 0) rc = zero(ws, p, cap * 64);
+    if (rc == 0) rc = zero(v0, p, cap * 64);
   }
   for (void* p : {(void*)ws->sh, (void*)ws->m_sh, (void*)ws->v_sh}) {
-    if (rc == 0) rc = zero(ws, p, shn * 4);
+    if (rc == 0) rc = zero(v0, p, shn * 4);
   }
-  if (rc == 0) rc = zero(ws, ws->stats, cap * 16);
+  if (rc == 0) rc = zero(v0, ws->stats, cap * 16);
   if (rc != 0) return rc;
-  GS_HIP_TRY(hipStreamSynchronize(ws->stream));
+  GS_HIP_TRY(hipStreamSynchronize(v0.stream));
   if (count == 0) return 0;
   GS_HIP_TRY(hipMemcpy(ws->params, params, size_t(count) * 64, hipMemcpyHostToDevice));
   if (ws->sh_coeffs > 0 && sh_rest != nullptr) {
@@ -372,7 +445,7 @@ int gs_hip_workspace_load(gs_hip_workspace* ws, const float* params, const float
 int gs_hip_workspace_store(gs_hip_workspace* ws, float* params, float* sh_rest, uint32_t count) {
   if (ws == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "workspace_store: null workspace");
   if (count > ws->capacity) return gsh::set_error(GS_HIP_E_CAPACITY, "workspace_store: more slots than capacity");
-  GS_HIP_TRY(hipStreamSynchronize(ws->stream));
+  GS_HIP_TRY(hipDeviceSynchronize());
   if (count == 0) return 0;
   if (params != nullptr) GS_HIP_TRY(hipMemcpy(params, ws->params, size_t(count) * 64, hipMemcpyDeviceToHost));
   if (sh_rest != nullptr && ws->sh_coeffs > 0) {
@@ -390,15 +463,17 @@ int gs_hip_workspace_store(gs_hip_workspace* ws, float* params, float* sh_rest, 
 
 int gs_hip_set_target(gs_hip_workspace* ws, const uint8_t* rgba8) {
   if (ws == nullptr || rgba8 == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "set_target: null");
-  GS_HIP_TRY(hipMemcpyAsync(ws->target, rgba8, size_t(ws->width) * ws->height * 4, hipMemcpyHostToDevice,
-                            ws->stream));
+  gsh::View& v = gsh::view0(ws);
+  GS_HIP_TRY(hipMemcpyAsync(v.target, rgba8, size_t(ws->width) * ws->height * 4, hipMemcpyHostToDevice, v.stream));
+  GS_HIP_TRY(hipStreamSynchronize(v.stream));
   return 0;
 }
 
 int gs_hip_set_target_device(gs_hip_workspace* ws, const void* device_rgba8) {
   if (ws == nullptr || device_rgba8 == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "set_target_device: null");
-  GS_HIP_TRY(hipMemcpyAsync(ws->target, device_rgba8, size_t(ws->width) * ws->height * 4, hipMemcpyDeviceToDevice,
-                            ws->stream));
+  gsh::View& v = gsh::view0(ws);
+  GS_HIP_TRY(hipMemcpyAsync(v.target, device_rgba8, size_t(ws->width) * ws->height * 4, hipMemcpyDeviceToDevice,
+                            v.stream));
   return 0;
 }
 
@@ -408,7 +483,7 @@ int gs_hip_workspace_timings(gs_hip_workspace* ws, int enable, const char** name
   gsh::Timer& t = ws->timer;
   int n = 0;
   if (t.enabled && t.used > 0) {
-    GS_HIP_TRY(hipStreamSynchronize(ws->stream));
+    GS_HIP_TRY(hipDeviceSynchronize());
     t.ms.assign(t.used, 0.0);
     for (size_t i = 0; i < t.used; ++i) {
       float v = 0.0f;

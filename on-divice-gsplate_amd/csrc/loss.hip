@@ -199,34 +199,51 @@ __global__ void __launch_bounds__(256) k_loss_grad(gs_hip_frame f, LossArgs a) {
 
 }  // namespace
 
+namespace gsh {
+
+int launch_loss(gs_hip_workspace* ws, View& v, const gs_hip_frame* f, bool copy_losses) {
+  const uint32_t tiles = f->tiles_x * f->tiles_y;
+  LossArgs a;
+  a.image = reinterpret_cast<const float4*>(v.image);
+  a.target = v.target;
+  a.maps = v.ssim_maps;
+  a.dldc = reinterpret_cast<float4*>(v.dldc);
+  a.tile_loss = reinterpret_cast<float4*>(v.tile_loss);
+  timer_begin(ws, v, "ssim");
+  k_ssim<<<tiles, 256, 0, v.stream>>>(*f, a);
+  GS_CHECK_LAUNCH("k_ssim");
+  timer_end(ws, v);
+  timer_begin(ws, v, "loss_grad");
+  k_loss_grad<<<tiles, 256, 0, v.stream>>>(*f, a);
+  GS_CHECK_LAUNCH("k_loss_grad");
+  timer_end(ws, v);
+  if (copy_losses) {
+    GS_HIP_TRY(hipMemcpyAsync(v.host_tile_loss, v.tile_loss, size_t(tiles) * 16, hipMemcpyDeviceToHost, v.stream));
+  }
+  return 0;
+}
+
+void sum_losses(gs_hip_workspace* ws, View& v, double losses[3]) {
+  const uint32_t tiles = ws->tiles_x * ws->tiles_y;
+  double s[3] = {0.0, 0.0, 0.0};
+  for (uint32_t t = 0; t < tiles; ++t) {
+    for (int q = 0; q < 3; ++q) s[q] += v.host_tile_loss[t * 4 + q];
+  }
+  for (int q = 0; q < 3; ++q) losses[q] = s[q];
+}
+
+}  // namespace gsh
+
 extern "C" {
 
 int gs_hip_loss(gs_hip_workspace* ws, const gs_hip_frame* f, double losses[3]) {
   if (ws == nullptr || f == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "loss: null argument");
-  const uint32_t tiles = f->tiles_x * f->tiles_y;
-  LossArgs a;
-  a.image = reinterpret_cast<const float4*>(ws->image);
-  a.target = ws->target;
-  a.maps = ws->ssim_maps;
-  a.dldc = reinterpret_cast<float4*>(ws->dldc);
-  a.tile_loss = reinterpret_cast<float4*>(ws->tile_loss);
-  gsh::timer_begin(ws, "ssim");
-  k_ssim<<<tiles, 256, 0, ws->stream>>>(*f, a);
-  GS_CHECK_LAUNCH("k_ssim");
-  gsh::timer_end(ws);
-  gsh::timer_begin(ws, "loss_grad");
-  k_loss_grad<<<tiles, 256, 0, ws->stream>>>(*f, a);
-  GS_CHECK_LAUNCH("k_loss_grad");
-  gsh::timer_end(ws);
+  gsh::View& v = gsh::view0(ws);
+  const int rc = gsh::launch_loss(ws, v, f, losses != nullptr);
+  if (rc != 0) return rc;
   if (losses != nullptr) {
-    GS_HIP_TRY(hipMemcpyAsync(ws->host_tile_loss, ws->tile_loss, size_t(tiles) * 16, hipMemcpyDeviceToHost,
-                              ws->stream));
-    GS_HIP_TRY(hipStreamSynchronize(ws->stream));
-    double s[3] = {0.0, 0.0, 0.0};
-    for (uint32_t t = 0; t < tiles; ++t) {
-      for (int q = 0; q < 3; ++q) s[q] += ws->host_tile_loss[t * 4 + q];
-    }
-    for (int q = 0; q < 3; ++q) losses[q] = s[q];
+    GS_HIP_TRY(hipStreamSynchronize(v.stream));
+    gsh::sum_losses(ws, v, losses);
   }
   return 0;
 }

@@ -12,12 +12,24 @@
 // ones, and the spawn kernel puts the k-th spawner into the k-th free slot or
 // past the slots in use.  The slot each new gaussian lands in is therefore a
 // function of the flags alone, not of atomic order.
+#include <algorithm>
+
 #include "gs_common.h"
 #include "gs_internal.h"
 
 namespace {
 
 using namespace gsd;
+
+// One view's inputs to the adjoint: its camera and image size, its 2D
+// gradient planes, raster records (radius, clamp bits) and tile counts.
+struct AdamView {
+  gs_hip_camera cam;
+  const float* __restrict__ grad2d;
+  const float4* __restrict__ records;
+  const uint32_t* __restrict__ tiles;
+  float4* __restrict__ sh_input;  // per slot: (dL/dcolour rgb, update flag in view 0), (direction, -)
+};
 
 struct AdamArgs {
   float4* __restrict__ params;
@@ -27,11 +39,10 @@ struct AdamArgs {
   float* __restrict__ m_sh;
   float* __restrict__ v_sh;
   float4* __restrict__ stats;
-  const float* __restrict__ grad2d;
-  const float4* __restrict__ records;
-  const uint32_t* __restrict__ tiles;
   uint32_t capacity;
   uint32_t sh_coeffs;
+  uint32_t count;  // views
+  AdamView views[GS_HIP_MAX_VIEWS];
 };
 
 __device__ __forceinline__ float adam1(float p, float g, float& m, float& v, float lr, const gs_hip_frame& f) {
@@ -52,196 +63,264 @@ __device__ __forceinline__ float4 adam4(float4 p, float4 g, float4* m_ptr, float
   return p;
 }
 
+__device__ __forceinline__ void add4(float4& a, float x, float y, float z, float w) {
+  a.x += x, a.y += y, a.z += z, a.w += w;
+}
+
+// d(colour)/d(direction) summed against dL/dcolour: sum_k (sh_k . dcol)
+// grad basis_k(dir), with each coefficient's term added as soon as its SH
+// triple is loaded (no per-coefficient arrays held in registers).
+__device__ __forceinline__ void sh_direction_gradient(float x, float y, float z, uint32_t degree,
+                                                      const float* __restrict__ sh, uint32_t capacity, uint32_t i,
+                                                      const float dcol[3], float ddir[3]) {
+  auto w = [&](int k) {
+    const float* s = sh + (size_t(k) * capacity + i) * 3u;
+    return s[0] * dcol[0] + s[1] * dcol[1] + s[2] * dcol[2];
+  };
+  auto add = [&](float wk, float gx, float gy, float gz) {
+    ddir[0] += wk * gx;
+    ddir[1] += wk * gy;
+    ddir[2] += wk * gz;
+  };
+  const float C1 = 0.4886025119029199f;
+  add(w(0), 0.0f, -C1, 0.0f);
+  add(w(1), 0.0f, 0.0f, C1);
+  add(w(2), -C1, 0.0f, 0.0f);
+  if (degree < 2u) return;
+  const float xx = x * x, yy = y * y, zz = z * z;
+  const float A = 1.0925484305920792f, B = 0.31539156525252005f, C = 0.5462742152960396f;
+  add(w(3), A * y, A * x, 0.0f);
+  add(w(4), 0.0f, -A * z, -A * y);
+  add(w(5), -2.0f * B * x, -2.0f * B * y, 4.0f * B * z);
+  add(w(6), -A * z, 0.0f, -A * x);
+  add(w(7), 2.0f * C * x, -2.0f * C * y, 0.0f);
+  if (degree < 3u) return;
+  const float D = -0.5900435899266435f, E = 2.890611442640554f, F = -0.4570457994644658f;
+  const float G = 0.3731763325901154f, H = 1.445305721320277f;
+  add(w(8), D * 6.0f * x * y, D * (3.0f * xx - 3.0f * yy), 0.0f);
+  add(w(9), E * y * z, E * x * z, E * x * y);
+  add(w(10), F * (-2.0f * x * y), F * (4.0f * zz - xx - 3.0f * yy), F * 8.0f * y * z);
+  add(w(11), G * (-6.0f * x * z), G * (-6.0f * y * z), G * (6.0f * zz - 3.0f * xx - 3.0f * yy));
+  add(w(12), F * (4.0f * zz - 3.0f * xx - yy), F * (-2.0f * x * y), F * 8.0f * x * z);
+  add(w(13), H * 2.0f * x * z, H * (-2.0f * y * z), H * (xx - yy));
+  add(w(14), D * (3.0f * xx - 3.0f * yy), D * (-6.0f * x * y), 0.0f);
+}
+
+// The projection adjoint of one view (train_step.glsl:102-219): 2D splat
+// gradient g[9] -> dL/d(position, logit, log scale, quaternion, DC) added to
+// the accumulators, and the view's SH colour gradient and direction.
+__device__ __forceinline__ void adjoint_view(const gs_hip_camera& cam, const float g[9], uint32_t clamped, float4 p0,
+                                             float4 p1, float4 q, const float* __restrict__ sh, uint32_t capacity,
+                                             uint32_t i, int coeffs, uint32_t degree, float4& dp0, float4& dp1,
+                                             float4& dq, float4& dp3, float dcol[3], float dir[3]) {
+  const float W = float(cam.width), H = float(cam.height);
+  const float* V = cam.rotation;
+  const float tx = V[0] * p0.x + V[1] * p0.y + V[2] * p0.z + cam.translation[0];
+  const float ty = V[3] * p0.x + V[4] * p0.y + V[5] * p0.z + cam.translation[1];
+  const float tz = V[6] * p0.x + V[7] * p0.y + V[8] * p0.z + cam.translation[2];
+  const float fx = cam.fx, fy = cam.fy, cx = cam.cx, cy = cam.cy;
+  const float inv_z = 1.0f / tz;
+  const float lim_x = 1.3f * fmaxf(cx, W - cx) / fx, lim_y = 1.3f * fmaxf(cy, H - cy) / fy;
+  const float rx = tx * inv_z, ry = ty * inv_z;
+  const bool clamp_x = rx < -lim_x || rx > lim_x, clamp_y = ry < -lim_y || ry > lim_y;
+  const float ux = fminf(fmaxf(rx, -lim_x), lim_x), uy = fminf(fmaxf(ry, -lim_y), lim_y);
+  const float j00 = fx * inv_z, j02 = -fx * ux * inv_z, j11 = fy * inv_z, j12 = -fy * uy * inv_z;
+  float T[2][3];
+  for (int c = 0; c < 3; ++c) {
+    T[0][c] = j00 * V[c] + j02 * V[6 + c];
+    T[1][c] = j11 * V[3 + c] + j12 * V[6 + c];
+  }
+  const float qlen = fmaxf(sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w), 1e-12f);
+  const float qw = q.x / qlen, qx = q.y / qlen, qy = q.z / qlen, qz = q.w / qlen;
+  float R[3][3];
+  R[0][0] = 1.0f - 2.0f * (qy * qy + qz * qz), R[0][1] = 2.0f * (qx * qy - qw * qz), R[0][2] = 2.0f * (qx * qz + qw * qy);
+  R[1][0] = 2.0f * (qx * qy + qw * qz), R[1][1] = 1.0f - 2.0f * (qx * qx + qz * qz), R[1][2] = 2.0f * (qy * qz - qw * qx);
+  R[2][0] = 2.0f * (qx * qz - qw * qy), R[2][1] = 2.0f * (qy * qz + qw * qx), R[2][2] = 1.0f - 2.0f * (qx * qx + qy * qy);
+  const float s[3] = {__expf(p1.x), __expf(p1.y), __expf(p1.z)};
+  float M[3][3], Sig[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) M[r][c] = R[r][c] * s[c];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) Sig[r][c] = M[r][0] * M[c][0] + M[r][1] * M[c][1] + M[r][2] * M[c][2];
+  float U[2][3];
+  for (int r = 0; r < 2; ++r)
+    for (int c = 0; c < 3; ++c) U[r][c] = T[r][0] * Sig[0][c] + T[r][1] * Sig[1][c] + T[r][2] * Sig[2][c];
+  const float A = U[0][0] * T[0][0] + U[0][1] * T[0][1] + U[0][2] * T[0][2] + kDilation;
+  const float B = U[0][0] * T[1][0] + U[0][1] * T[1][1] + U[0][2] * T[1][2];
+  const float C = U[1][0] * T[1][0] + U[1][1] * T[1][1] + U[1][2] * T[1][2] + kDilation;
+  const float D = A * C - B * B;
+
+  // Colour: clamped channels get no gradient; DC and the view direction.
+  const float drx = p0.x - cam.position[0], dry = p0.y - cam.position[1], drz = p0.z - cam.position[2];
+  const float dlen = fmaxf(sqrtf(drx * drx + dry * dry + drz * drz), 1e-12f);
+  dir[0] = drx / dlen, dir[1] = dry / dlen, dir[2] = drz / dlen;
+  dcol[0] = (clamped & 1u) ? 0.0f : g[6];
+  dcol[1] = (clamped & 2u) ? 0.0f : g[7];
+  dcol[2] = (clamped & 4u) ? 0.0f : g[8];
+  const float b0 = 0.28209479177387814f;
+  add4(dp3, b0 * dcol[0], b0 * dcol[1], b0 * dcol[2], 0.0f);
+  float ddir[3] = {0.0f, 0.0f, 0.0f};
+  if (coeffs > 0) sh_direction_gradient(dir[0], dir[1], dir[2], degree, sh, capacity, i, dcol, ddir);
+  const float dd = dir[0] * ddir[0] + dir[1] * ddir[1] + dir[2] * ddir[2];
+  const float dpos_dir[3] = {(ddir[0] - dir[0] * dd) / dlen, (ddir[1] - dir[1] * dd) / dlen,
+                             (ddir[2] - dir[2] * dd) / dlen};
+
+  // Opacity logit.
+  const float op = 1.0f / (1.0f + __expf(-p0.w));
+  const float dlogit = g[5] * op * (1.0f - op);
+
+  // Conic (C/D, -B/D, A/D) -> dilated covariance.
+  const float ga = g[2], gb = g[3], gc = g[4];
+  const float inv_d2 = 1.0f / (D * D);
+  const float gA = (-C * C * ga + B * C * gb - B * B * gc) * inv_d2;
+  const float gB = (2.0f * B * C * ga - (D + 2.0f * B * B) * gb + 2.0f * A * B * gc) * inv_d2;
+  const float gC = (-B * B * ga + A * B * gb - A * A * gc) * inv_d2;
+  const float G2[2][2] = {{gA, 0.5f * gB}, {0.5f * gB, gC}};
+  float G2T[2][3];
+  for (int r = 0; r < 2; ++r)
+    for (int c = 0; c < 3; ++c) G2T[r][c] = G2[r][0] * T[0][c] + G2[r][1] * T[1][c];
+  float dSig[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) dSig[r][c] = T[0][r] * G2T[0][c] + T[1][r] * G2T[1][c];
+  float dT[2][3];
+  for (int r = 0; r < 2; ++r)
+    for (int c = 0; c < 3; ++c)
+      dT[r][c] = 2.0f * (G2T[r][0] * Sig[0][c] + G2T[r][1] * Sig[1][c] + G2T[r][2] * Sig[2][c]);
+  const float dJ00 = dT[0][0] * V[0] + dT[0][1] * V[1] + dT[0][2] * V[2];
+  const float dJ02 = dT[0][0] * V[6] + dT[0][1] * V[7] + dT[0][2] * V[8];
+  const float dJ11 = dT[1][0] * V[3] + dT[1][1] * V[4] + dT[1][2] * V[5];
+  const float dJ12 = dT[1][0] * V[6] + dT[1][1] * V[7] + dT[1][2] * V[8];
+  const float inv_z2 = inv_z * inv_z;
+  float dtx = 0.0f, dty = 0.0f, dtz = -fx * inv_z2 * dJ00 - fy * inv_z2 * dJ11;
+  if (clamp_x) {
+    dtz += fx * ux * inv_z2 * dJ02;
+  } else {
+    dtx += -fx * inv_z2 * dJ02;
+    dtz += 2.0f * fx * tx * inv_z2 * inv_z * dJ02;
+  }
+  if (clamp_y) {
+    dtz += fy * uy * inv_z2 * dJ12;
+  } else {
+    dty += -fy * inv_z2 * dJ12;
+    dtz += 2.0f * fy * ty * inv_z2 * inv_z * dJ12;
+  }
+  dtx += g[0] * fx * inv_z;
+  dty += g[1] * fy * inv_z;
+  dtz += -(g[0] * fx * tx + g[1] * fy * ty) * inv_z2;
+  add4(dp0, V[0] * dtx + V[3] * dty + V[6] * dtz + dpos_dir[0], V[1] * dtx + V[4] * dty + V[7] * dtz + dpos_dir[1],
+       V[2] * dtx + V[5] * dty + V[8] * dtz + dpos_dir[2], dlogit);
+
+  float dM[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c)
+      dM[r][c] = 2.0f * (dSig[r][0] * M[0][c] + dSig[r][1] * M[1][c] + dSig[r][2] * M[2][c]);
+  add4(dp1, (dM[0][0] * R[0][0] + dM[1][0] * R[1][0] + dM[2][0] * R[2][0]) * s[0],
+       (dM[0][1] * R[0][1] + dM[1][1] * R[1][1] + dM[2][1] * R[2][1]) * s[1],
+       (dM[0][2] * R[0][2] + dM[1][2] * R[1][2] + dM[2][2] * R[2][2]) * s[2], 0.0f);
+  float gR[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) gR[r][c] = dM[r][c] * s[c];
+  const float dw = 2.0f * (-qz * gR[0][1] + qy * gR[0][2] + qz * gR[1][0] - qx * gR[1][2] - qy * gR[2][0] +
+                           qx * gR[2][1]);
+  const float dx = 2.0f * (qy * gR[0][1] + qz * gR[0][2] + qy * gR[1][0] - 2.0f * qx * gR[1][1] - qw * gR[1][2] +
+                           qz * gR[2][0] + qw * gR[2][1] - 2.0f * qx * gR[2][2]);
+  const float dy = 2.0f * (-2.0f * qy * gR[0][0] + qx * gR[0][1] + qw * gR[0][2] + qx * gR[1][0] + qz * gR[1][2] -
+                           qw * gR[2][0] + qz * gR[2][1] - 2.0f * qy * gR[2][2]);
+  const float dz = 2.0f * (-2.0f * qz * gR[0][0] - qw * gR[0][1] + qx * gR[0][2] + qw * gR[1][0] -
+                           2.0f * qz * gR[1][1] + qy * gR[1][2] + qx * gR[2][0] + qy * gR[2][1]);
+  const float qd = qw * dw + qx * dx + qy * dy + qz * dz;
+  add4(dq, (dw - qw * qd) / qlen, (dx - qx * qd) / qlen, (dy - qy * qd) / qlen, (dz - qz * qd) / qlen);
+}
+
+// Adjoint of every view in `a.views`, summed and averaged, then one Adam step
+// on the 16 parameters; densify statistics count every view in which the
+// gaussian was visible.  Each view's colour gradient and direction go to its
+// sh_input for k_adam_sh (one thread per SH coefficient, fully coalesced on
+// the coefficient planes, and few registers here).
 __global__ void __launch_bounds__(256) k_adam(gs_hip_frame f, AdamArgs a) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= f.slots) return;
-  float4 p0 = a.params[i * 4u], p1 = a.params[i * 4u + 1u], q = a.params[i * 4u + 2u], p3 = a.params[i * 4u + 3u];
-  if (p1.w < 0.5f) return;  // free slot
-
-  float4 dp0 = make_float4(0, 0, 0, 0), dp1 = dp0, dq = dp0, dp3 = dp0;
-  float dcol[3] = {0.0f, 0.0f, 0.0f};
-  float dir[3] = {0.0f, 0.0f, 0.0f};
-  bool drew = false;
   const uint32_t degree = f.sh_degree;
   const int coeffs = min(sh_rest_coeffs(degree), int(a.sh_coeffs));
-
-  if (a.tiles[i] > 0u) {
+  float4 p0 = a.params[i * 4u], p1 = a.params[i * 4u + 1u], q = a.params[i * 4u + 2u], p3 = a.params[i * 4u + 3u];
+  if (p1.w < 0.5f) {  // free slot
+    if (coeffs > 0) a.views[0].sh_input[size_t(i) * 2u] = make_float4(0, 0, 0, 0);
+    return;
+  }
+  float4 dp0 = make_float4(0, 0, 0, 0), dp1 = dp0, dq = dp0, dp3 = dp0;
+  bool drew = false, seen = false;
+  float4 st = a.stats[i];
+  for (uint32_t v = 0; v < a.count; ++v) {
+    const AdamView& view = a.views[v];
+    if (view.tiles[i] == 0u) {
+      if (coeffs > 0) {
+        view.sh_input[size_t(i) * 2u] = make_float4(0, 0, 0, 0);
+        view.sh_input[size_t(i) * 2u + 1u] = make_float4(0, 0, 0, 0);
+      }
+      continue;
+    }
+    seen = true;
     float g[9];
     for (int c = 0; c < 9; ++c) {
-      g[c] = a.grad2d[size_t(c) * a.capacity + i];
+      g[c] = view.grad2d[size_t(c) * a.capacity + i];
       drew = drew || g[c] != 0.0f;
     }
-    const float W = float(f.cam.width), H = float(f.cam.height);
     // Densification statistics: view-space positional gradient in NDC units.
-    {
-      float4 st = a.stats[i];
-      const float gx = g[0] * (0.5f * W), gy = g[1] * (0.5f * H);
-      st.x += sqrtf(gx * gx + gy * gy);
-      st.y += 1.0f;
-      st.z = fmaxf(st.z, float(__float_as_uint(a.records[size_t(i) * 3u + 2u].w) >> 3));
-      a.stats[i] = st;
-    }
-    const float* V = f.cam.rotation;
-    const float tx = V[0] * p0.x + V[1] * p0.y + V[2] * p0.z + f.cam.translation[0];
-    const float ty = V[3] * p0.x + V[4] * p0.y + V[5] * p0.z + f.cam.translation[1];
-    const float tz = V[6] * p0.x + V[7] * p0.y + V[8] * p0.z + f.cam.translation[2];
-    const float fx = f.cam.fx, fy = f.cam.fy, cx = f.cam.cx, cy = f.cam.cy;
-    const float inv_z = 1.0f / tz;
-    const float lim_x = 1.3f * fmaxf(cx, W - cx) / fx, lim_y = 1.3f * fmaxf(cy, H - cy) / fy;
-    const float rx = tx * inv_z, ry = ty * inv_z;
-    const bool clamp_x = rx < -lim_x || rx > lim_x, clamp_y = ry < -lim_y || ry > lim_y;
-    const float ux = fminf(fmaxf(rx, -lim_x), lim_x), uy = fminf(fmaxf(ry, -lim_y), lim_y);
-    const float j00 = fx * inv_z, j02 = -fx * ux * inv_z, j11 = fy * inv_z, j12 = -fy * uy * inv_z;
-    // T = J V (2x3).
-    float T[2][3];
-    for (int c = 0; c < 3; ++c) {
-      T[0][c] = j00 * V[c] + j02 * V[6 + c];
-      T[1][c] = j11 * V[3 + c] + j12 * V[6 + c];
-    }
-    const float qlen = fmaxf(sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w), 1e-12f);
-    const float qw = q.x / qlen, qx = q.y / qlen, qy = q.z / qlen, qz = q.w / qlen;
-    float R[3][3];
-    R[0][0] = 1.0f - 2.0f * (qy * qy + qz * qz), R[0][1] = 2.0f * (qx * qy - qw * qz), R[0][2] = 2.0f * (qx * qz + qw * qy);
-    R[1][0] = 2.0f * (qx * qy + qw * qz), R[1][1] = 1.0f - 2.0f * (qx * qx + qz * qz), R[1][2] = 2.0f * (qy * qz - qw * qx);
-    R[2][0] = 2.0f * (qx * qz - qw * qy), R[2][1] = 2.0f * (qy * qz + qw * qx), R[2][2] = 1.0f - 2.0f * (qx * qx + qy * qy);
-    const float s[3] = {__expf(p1.x), __expf(p1.y), __expf(p1.z)};
-    float M[3][3], Sig[3][3];
-    for (int r = 0; r < 3; ++r)
-      for (int c = 0; c < 3; ++c) M[r][c] = R[r][c] * s[c];
-    for (int r = 0; r < 3; ++r)
-      for (int c = 0; c < 3; ++c) Sig[r][c] = M[r][0] * M[c][0] + M[r][1] * M[c][1] + M[r][2] * M[c][2];
-    // U = T Sigma (2x3); cov = U T^T.
-    float U[2][3];
-    for (int r = 0; r < 2; ++r)
-      for (int c = 0; c < 3; ++c) U[r][c] = T[r][0] * Sig[0][c] + T[r][1] * Sig[1][c] + T[r][2] * Sig[2][c];
-    const float A = U[0][0] * T[0][0] + U[0][1] * T[0][1] + U[0][2] * T[0][2] + kDilation;
-    const float B = U[0][0] * T[1][0] + U[0][1] * T[1][1] + U[0][2] * T[1][2];
-    const float C = U[1][0] * T[1][0] + U[1][1] * T[1][1] + U[1][2] * T[1][2] + kDilation;
-    const float D = A * C - B * B;
-
-    // Colour: clamped channels get no gradient; DC and the view direction.
-    const float drx = p0.x - f.cam.position[0], dry = p0.y - f.cam.position[1], drz = p0.z - f.cam.position[2];
-    const float dlen = fmaxf(sqrtf(drx * drx + dry * dry + drz * drz), 1e-12f);
-    dir[0] = drx / dlen, dir[1] = dry / dlen, dir[2] = drz / dlen;
-    const uint32_t clamped = __float_as_uint(a.records[size_t(i) * 3u + 2u].w) & 7u;
-    dcol[0] = (clamped & 1u) ? 0.0f : g[6];
-    dcol[1] = (clamped & 2u) ? 0.0f : g[7];
-    dcol[2] = (clamped & 4u) ? 0.0f : g[8];
-    const float b0 = 0.28209479177387814f;
-    dp3.x = b0 * dcol[0], dp3.y = b0 * dcol[1], dp3.z = b0 * dcol[2];
-    float ddir[3] = {0.0f, 0.0f, 0.0f};
+    const float gx = g[0] * (0.5f * float(view.cam.width)), gy = g[1] * (0.5f * float(view.cam.height));
+    const uint32_t misc = __float_as_uint(view.records[size_t(i) * 3u + 2u].w);
+    st.x += sqrtf(gx * gx + gy * gy);
+    st.y += 1.0f;
+    st.z = fmaxf(st.z, float(misc >> 3));
+    float dcol[3], dir[3];
+    adjoint_view(view.cam, g, misc & 7u, p0, p1, q, a.sh, a.capacity, i, coeffs, degree, dp0, dp1, dq, dp3, dcol,
+                 dir);
     if (coeffs > 0) {
-      float gx[16], gy[16], gz[16];
-      sh_basis_grad(dir[0], dir[1], dir[2], degree, gx, gy, gz);
-      for (int k = 0; k < coeffs; ++k) {
-        const float* sk = a.sh + (size_t(k) * a.capacity + i) * 3u;
-        const float w = sk[0] * dcol[0] + sk[1] * dcol[1] + sk[2] * dcol[2];
-        ddir[0] += w * gx[k + 1];
-        ddir[1] += w * gy[k + 1];
-        ddir[2] += w * gz[k + 1];
-      }
+      view.sh_input[size_t(i) * 2u] = make_float4(dcol[0], dcol[1], dcol[2], 0.0f);
+      view.sh_input[size_t(i) * 2u + 1u] = make_float4(dir[0], dir[1], dir[2], 0.0f);
     }
-    const float dd = dir[0] * ddir[0] + dir[1] * ddir[1] + dir[2] * ddir[2];
-    const float dpos_dir[3] = {(ddir[0] - dir[0] * dd) / dlen, (ddir[1] - dir[1] * dd) / dlen,
-                               (ddir[2] - dir[2] * dd) / dlen};
-
-    // Opacity logit.
-    const float op = 1.0f / (1.0f + __expf(-p0.w));
-    dp0.w = g[5] * op * (1.0f - op);
-
-    // Conic (C/D, -B/D, A/D) -> dilated covariance.
-    const float ga = g[2], gb = g[3], gc = g[4];
-    const float inv_d2 = 1.0f / (D * D);
-    const float gA = (-C * C * ga + B * C * gb - B * B * gc) * inv_d2;
-    const float gB = (2.0f * B * C * ga - (D + 2.0f * B * B) * gb + 2.0f * A * B * gc) * inv_d2;
-    const float gC = (-B * B * ga + A * B * gb - A * A * gc) * inv_d2;
-    const float G2[2][2] = {{gA, 0.5f * gB}, {0.5f * gB, gC}};
-    // dSigma = T^T G2 T; dT = 2 G2 T Sigma.
-    float G2T[2][3];
-    for (int r = 0; r < 2; ++r)
-      for (int c = 0; c < 3; ++c) G2T[r][c] = G2[r][0] * T[0][c] + G2[r][1] * T[1][c];
-    float dSig[3][3];
-    for (int r = 0; r < 3; ++r)
-      for (int c = 0; c < 3; ++c) dSig[r][c] = T[0][r] * G2T[0][c] + T[1][r] * G2T[1][c];
-    float dT[2][3];
-    for (int r = 0; r < 2; ++r)
-      for (int c = 0; c < 3; ++c)
-        dT[r][c] = 2.0f * (G2T[r][0] * Sig[0][c] + G2T[r][1] * Sig[1][c] + G2T[r][2] * Sig[2][c]);
-    // dJ = dT V^T: only J00, J02, J11, J12 depend on t.
-    const float dJ00 = dT[0][0] * V[0] + dT[0][1] * V[1] + dT[0][2] * V[2];
-    const float dJ02 = dT[0][0] * V[6] + dT[0][1] * V[7] + dT[0][2] * V[8];
-    const float dJ11 = dT[1][0] * V[3] + dT[1][1] * V[4] + dT[1][2] * V[5];
-    const float dJ12 = dT[1][0] * V[6] + dT[1][1] * V[7] + dT[1][2] * V[8];
-    const float inv_z2 = inv_z * inv_z;
-    float dtx = 0.0f, dty = 0.0f, dtz = -fx * inv_z2 * dJ00 - fy * inv_z2 * dJ11;
-    if (clamp_x) {
-      dtz += fx * ux * inv_z2 * dJ02;
-    } else {
-      dtx += -fx * inv_z2 * dJ02;
-      dtz += 2.0f * fx * tx * inv_z2 * inv_z * dJ02;
-    }
-    if (clamp_y) {
-      dtz += fy * uy * inv_z2 * dJ12;
-    } else {
-      dty += -fy * inv_z2 * dJ12;
-      dtz += 2.0f * fy * ty * inv_z2 * inv_z * dJ12;
-    }
-    // Screen mean.
-    dtx += g[0] * fx * inv_z;
-    dty += g[1] * fy * inv_z;
-    dtz += -(g[0] * fx * tx + g[1] * fy * ty) * inv_z2;
-    dp0.x = V[0] * dtx + V[3] * dty + V[6] * dtz + dpos_dir[0];
-    dp0.y = V[1] * dtx + V[4] * dty + V[7] * dtz + dpos_dir[1];
-    dp0.z = V[2] * dtx + V[5] * dty + V[8] * dtz + dpos_dir[2];
-
-    // Sigma = M M^T -> dM = 2 dSigma M; M = R diag(s).
-    float dM[3][3];
-    for (int r = 0; r < 3; ++r)
-      for (int c = 0; c < 3; ++c)
-        dM[r][c] = 2.0f * (dSig[r][0] * M[0][c] + dSig[r][1] * M[1][c] + dSig[r][2] * M[2][c]);
-    dp1.x = (dM[0][0] * R[0][0] + dM[1][0] * R[1][0] + dM[2][0] * R[2][0]) * s[0];
-    dp1.y = (dM[0][1] * R[0][1] + dM[1][1] * R[1][1] + dM[2][1] * R[2][1]) * s[1];
-    dp1.z = (dM[0][2] * R[0][2] + dM[1][2] * R[1][2] + dM[2][2] * R[2][2]) * s[2];
-    float gR[3][3];
-    for (int r = 0; r < 3; ++r)
-      for (int c = 0; c < 3; ++c) gR[r][c] = dM[r][c] * s[c];
-    const float dw = 2.0f * (-qz * gR[0][1] + qy * gR[0][2] + qz * gR[1][0] - qx * gR[1][2] - qy * gR[2][0] +
-                             qx * gR[2][1]);
-    const float dx = 2.0f * (qy * gR[0][1] + qz * gR[0][2] + qy * gR[1][0] - 2.0f * qx * gR[1][1] -
-                             qw * gR[1][2] + qz * gR[2][0] + qw * gR[2][1] - 2.0f * qx * gR[2][2]);
-    const float dy = 2.0f * (-2.0f * qy * gR[0][0] + qx * gR[0][1] + qw * gR[0][2] + qx * gR[1][0] +
-                             qz * gR[1][2] - qw * gR[2][0] + qz * gR[2][1] - 2.0f * qy * gR[2][2]);
-    const float dz = 2.0f * (-2.0f * qz * gR[0][0] - qw * gR[0][1] + qx * gR[0][2] + qw * gR[1][0] -
-                             2.0f * qz * gR[1][1] + qy * gR[1][2] + qx * gR[2][0] + qy * gR[2][1]);
-    const float qd = qw * dw + qx * dx + qy * dy + qz * dz;
-    dq.x = (dw - qw * qd) / qlen;
-    dq.y = (dx - qx * qd) / qlen;
-    dq.z = (dy - qy * qd) / qlen;
-    dq.w = (dz - qz * qd) / qlen;
   }
-
-  if (!drew && (f.flags & GS_HIP_FLAG_SPARSE_ADAM) != 0u) return;
+  if (seen) a.stats[i] = st;
+  const bool update = drew || (f.flags & GS_HIP_FLAG_SPARSE_ADAM) == 0u;
+  if (coeffs > 0) a.views[0].sh_input[size_t(i) * 2u].w = update ? 1.0f : 0.0f;
+  if (!update) return;
+  const float inv = 1.0f / float(a.count);  // the mean over the views
   const float4 lr0 = make_float4(f.lr_position, f.lr_position, f.lr_position, f.lr_opacity);
   const float4 lr1 = make_float4(f.lr_scale, f.lr_scale, f.lr_scale, 0.0f);
   const float4 lrq = make_float4(f.lr_rotation, f.lr_rotation, f.lr_rotation, f.lr_rotation);
   const float4 lr3 = make_float4(f.lr_sh_dc, f.lr_sh_dc, f.lr_sh_dc, 0.0f);
-  a.params[i * 4u] = adam4(p0, dp0, a.m_params + i * 4u, a.v_params + i * 4u, lr0, f);
-  a.params[i * 4u + 1u] = adam4(p1, dp1, a.m_params + i * 4u + 1u, a.v_params + i * 4u + 1u, lr1, f);
-  a.params[i * 4u + 2u] = adam4(q, dq, a.m_params + i * 4u + 2u, a.v_params + i * 4u + 2u, lrq, f);
-  a.params[i * 4u + 3u] = adam4(p3, dp3, a.m_params + i * 4u + 3u, a.v_params + i * 4u + 3u, lr3, f);
+  auto scale4 = [inv](float4 v) { return make_float4(v.x * inv, v.y * inv, v.z * inv, v.w * inv); };
+  a.params[i * 4u] = adam4(p0, scale4(dp0), a.m_params + i * 4u, a.v_params + i * 4u, lr0, f);
+  a.params[i * 4u + 1u] = adam4(p1, scale4(dp1), a.m_params + i * 4u + 1u, a.v_params + i * 4u + 1u, lr1, f);
+  a.params[i * 4u + 2u] = adam4(q, scale4(dq), a.m_params + i * 4u + 2u, a.v_params + i * 4u + 2u, lrq, f);
+  a.params[i * 4u + 3u] = adam4(p3, scale4(dp3), a.m_params + i * 4u + 3u, a.v_params + i * 4u + 3u, lr3, f);
+}
 
-  if (coeffs > 0) {
+// Adam on the SH coefficients beyond DC, one thread per (slot, coefficient)
+// (train_step_sh.comp:21-44): g = mean over the views of basis_k(dir) dL/dcolour.
+__global__ void __launch_bounds__(256) k_adam_sh(gs_hip_frame f, AdamArgs a) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t k = blockIdx.y;
+  if (i >= f.slots) return;
+  if (a.views[0].sh_input[size_t(i) * 2u].w == 0.0f) return;  // free slot, or kept by sparse Adam
+  float g[3] = {0.0f, 0.0f, 0.0f};
+  for (uint32_t v = 0; v < a.count; ++v) {
+    const float4 c = a.views[v].sh_input[size_t(i) * 2u];
+    if (c.x == 0.0f && c.y == 0.0f && c.z == 0.0f) continue;
+    const float4 d = a.views[v].sh_input[size_t(i) * 2u + 1u];
     float basis[16];
-    sh_basis(dir[0], dir[1], dir[2], degree, basis);
-    for (int k = 0; k < coeffs; ++k) {
-      const size_t o = (size_t(k) * a.capacity + i) * 3u;
-      for (int c = 0; c < 3; ++c) {
-        float m = a.m_sh[o + c], v = a.v_sh[o + c];
-        a.sh[o + c] = adam1(a.sh[o + c], basis[k + 1] * dcol[c], m, v, f.lr_sh_rest, f);
-        a.m_sh[o + c] = m;
-        a.v_sh[o + c] = v;
-      }
-    }
+    sh_basis(d.x, d.y, d.z, f.sh_degree, basis);
+    const float b = basis[k + 1];
+    g[0] += b * c.x;
+    g[1] += b * c.y;
+    g[2] += b * c.z;
+  }
+  const float inv = 1.0f / float(a.count);
+  const size_t o = (size_t(k) * a.capacity + i) * 3u;
+  for (int c = 0; c < 3; ++c) {
+    float m = a.m_sh[o + c], v = a.v_sh[o + c];
+    a.sh[o + c] = adam1(a.sh[o + c], g[c] * inv, m, v, f.lr_sh_rest, f);
+    a.m_sh[o + c] = m;
+    a.v_sh[o + c] = v;
   }
 }
 
@@ -405,11 +484,14 @@ __global__ void k_zero_counters(uint32_t* __restrict__ totals) {
 
 }  // namespace
 
-extern "C" {
+namespace gsh {
 
-int gs_hip_adam(gs_hip_workspace* ws, const gs_hip_frame* f) {
-  if (ws == nullptr || f == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "adam: null argument");
-  if (f->slots == 0) return 0;
+int launch_adam(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t count) {
+  if (count == 0 || count > ws->views.size() || count > GS_HIP_MAX_VIEWS) {
+    return set_error(GS_HIP_E_ARGUMENT, "adam: view count out of range");
+  }
+  const gs_hip_frame& f = frames[0];
+  if (f.slots == 0) return 0;
   AdamArgs a;
   a.params = reinterpret_cast<float4*>(ws->params);
   a.m_params = reinterpret_cast<float4*>(ws->m_params);
@@ -418,16 +500,37 @@ int gs_hip_adam(gs_hip_workspace* ws, const gs_hip_frame* f) {
   a.m_sh = ws->m_sh;
   a.v_sh = ws->v_sh;
   a.stats = reinterpret_cast<float4*>(ws->stats);
-  a.grad2d = ws->grad2d;
-  a.records = reinterpret_cast<const float4*>(ws->records);
-  a.tiles = ws->tiles;
   a.capacity = ws->capacity;
   a.sh_coeffs = ws->sh_coeffs;
-  gsh::timer_begin(ws, "adam");
-  k_adam<<<gsh::div_up(f->slots, 256), 256, 0, ws->stream>>>(*f, a);
+  a.count = count;
+  for (uint32_t v = 0; v < count; ++v) {
+    View& view = *ws->views[v];
+    a.views[v].cam = frames[v].cam;
+    a.views[v].grad2d = view.grad2d;
+    a.views[v].records = reinterpret_cast<const float4*>(view.records);
+    a.views[v].tiles = view.tiles;
+    a.views[v].sh_input = reinterpret_cast<float4*>(view.sh_input);
+  }
+  View& v0 = view0(ws);
+  timer_begin(ws, v0, "adam");
+  k_adam<<<div_up(f.slots, 256), 256, 0, v0.stream>>>(f, a);
   GS_CHECK_LAUNCH("k_adam");
-  gsh::timer_end(ws);
+  const int coeffs = std::min(f.sh_degree == 0 ? 0 : int((f.sh_degree + 1) * (f.sh_degree + 1)) - 1, int(ws->sh_coeffs));
+  if (coeffs > 0) {
+    k_adam_sh<<<dim3(div_up(f.slots, 256), coeffs), 256, 0, v0.stream>>>(f, a);
+    GS_CHECK_LAUNCH("k_adam_sh");
+  }
+  timer_end(ws, v0);
   return 0;
+}
+
+}  // namespace gsh
+
+extern "C" {
+
+int gs_hip_adam(gs_hip_workspace* ws, const gs_hip_frame* f) {
+  if (ws == nullptr || f == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "adam: null argument");
+  return gsh::launch_adam(ws, f, 1);
 }
 
 int gs_hip_densify(gs_hip_workspace* ws, const gs_hip_frame* f, uint32_t* slots, uint32_t counts[3]) {
@@ -436,6 +539,9 @@ int gs_hip_densify(gs_hip_workspace* ws, const gs_hip_frame* f, uint32_t* slots,
   if (counts != nullptr) counts[0] = counts[1] = counts[2] = 0;
   const uint32_t n = *slots;
   if (n == 0) return 0;
+  gsh::View& v0 = gsh::view0(ws);
+  // The optimizer (view 0's stream) must be done with every view.
+  GS_HIP_TRY(hipDeviceSynchronize());
   DensifyArgs a;
   a.params = reinterpret_cast<float4*>(ws->params);
   a.m_params = reinterpret_cast<float4*>(ws->m_params);
@@ -444,35 +550,35 @@ int gs_hip_densify(gs_hip_workspace* ws, const gs_hip_frame* f, uint32_t* slots,
   a.m_sh = ws->m_sh;
   a.v_sh = ws->v_sh;
   a.stats = reinterpret_cast<float4*>(ws->stats);
-  // Per-step scratch reused: the next step's projection rewrites all of these.
-  a.action = ws->flags;
-  a.release = ws->depth_keys;
-  a.spawn = ws->depth_order;
-  a.free_rank = ws->offsets;
-  a.spawn_rank = ws->scan_tmp;
-  a.free_list = ws->tiles;
-  a.totals = ws->totals;
+  // View 0's per-step scratch: the next projection rewrites all of it.
+  a.action = v0.flags;
+  a.release = v0.depth_keys;
+  a.spawn = v0.depth_order;
+  a.free_rank = v0.offsets;
+  a.spawn_rank = v0.scan_tmp;
+  a.free_list = v0.tiles;
+  a.totals = v0.totals;
   a.capacity = ws->capacity;
   a.sh_coeffs = ws->sh_coeffs;
   const uint32_t blocks = gsh::div_up(n, 256);
-  gsh::timer_begin(ws, "densify");
-  k_zero_counters<<<1, 64, 0, ws->stream>>>(ws->totals);
+  gsh::timer_begin(ws, v0, "densify");
+  k_zero_counters<<<1, 64, 0, v0.stream>>>(v0.totals);
   GS_CHECK_LAUNCH("k_zero_counters");
-  k_densify_mark<<<blocks, 256, 0, ws->stream>>>(*f, a);
+  k_densify_mark<<<blocks, 256, 0, v0.stream>>>(*f, a);
   GS_CHECK_LAUNCH("k_densify_mark");
-  int rc = gsh::exclusive_scan(ws, a.release, nullptr, a.free_rank, n, ws->totals + 6);
-  if (rc == 0) rc = gsh::exclusive_scan(ws, a.spawn, nullptr, a.spawn_rank, n, ws->totals + 7);
+  int rc = gsh::exclusive_scan(v0, a.release, nullptr, a.free_rank, n, v0.totals + 6);
+  if (rc == 0) rc = gsh::exclusive_scan(v0, a.spawn, nullptr, a.spawn_rank, n, v0.totals + 7);
   if (rc != 0) return rc;
-  k_free_list<<<blocks, 256, 0, ws->stream>>>(n, a);
+  k_free_list<<<blocks, 256, 0, v0.stream>>>(n, a);
   GS_CHECK_LAUNCH("k_free_list");
-  k_densify_spawn<<<blocks, 256, 0, ws->stream>>>(*f, a);
+  k_densify_spawn<<<blocks, 256, 0, v0.stream>>>(*f, a);
   GS_CHECK_LAUNCH("k_densify_spawn");
-  k_zero_stats<<<gsh::div_up(ws->capacity, 256), 256, 0, ws->stream>>>(a.stats, ws->capacity);
+  k_zero_stats<<<gsh::div_up(ws->capacity, 256), 256, 0, v0.stream>>>(a.stats, ws->capacity);
   GS_CHECK_LAUNCH("k_zero_stats");
-  GS_HIP_TRY(hipMemcpyAsync(ws->host_totals, ws->totals, 32, hipMemcpyDeviceToHost, ws->stream));
-  GS_HIP_TRY(hipStreamSynchronize(ws->stream));
-  gsh::timer_end(ws);
-  const uint32_t pruned = ws->host_totals[5], free_slots = ws->host_totals[6], spawned = ws->host_totals[7];
+  GS_HIP_TRY(hipMemcpyAsync(v0.host_totals, v0.totals, 32, hipMemcpyDeviceToHost, v0.stream));
+  GS_HIP_TRY(hipStreamSynchronize(v0.stream));
+  gsh::timer_end(ws, v0);
+  const uint32_t pruned = v0.host_totals[5], free_slots = v0.host_totals[6], spawned = v0.host_totals[7];
   const uint32_t room = free_slots + (ws->capacity - n);
   const uint32_t placed = spawned < room ? spawned : room;
   if (placed > free_slots) *slots = n + (placed - free_slots);
@@ -487,7 +593,8 @@ int gs_hip_densify(gs_hip_workspace* ws, const gs_hip_frame* f, uint32_t* slots,
 int gs_hip_reset_opacity(gs_hip_workspace* ws, const gs_hip_frame* f, float max_logit) {
   if (ws == nullptr || f == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "reset_opacity: null argument");
   if (f->slots == 0) return 0;
-  k_reset_opacity<<<gsh::div_up(f->slots, 256), 256, 0, ws->stream>>>(
+  gsh::View& v0 = gsh::view0(ws);
+  k_reset_opacity<<<gsh::div_up(f->slots, 256), 256, 0, v0.stream>>>(
       f->slots, max_logit, reinterpret_cast<float4*>(ws->params), reinterpret_cast<float4*>(ws->m_params),
       reinterpret_cast<float4*>(ws->v_params));
   GS_CHECK_LAUNCH("k_reset_opacity");
@@ -495,13 +602,80 @@ int gs_hip_reset_opacity(gs_hip_workspace* ws, const gs_hip_frame* f, float max_
 }
 
 int gs_hip_train_step(gs_hip_workspace* ws, const gs_hip_frame* f, uint64_t* pairs, double losses[3]) {
+  if (ws == nullptr || f == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "train_step: null argument");
+  gsh::View& v = gsh::view0(ws);
   int rc = gs_hip_project(ws, f);
   if (rc == 0) rc = gs_hip_bin(ws, f, pairs);
-  if (rc == 0) rc = gs_hip_forward(ws, f);
-  if (rc == 0) rc = gs_hip_loss(ws, f, losses);
-  if (rc == 0) rc = gs_hip_backward(ws, f);
-  if (rc == 0) rc = gs_hip_adam(ws, f);
+  if (rc == 0) rc = gsh::launch_forward(ws, v, f);
+  if (rc == 0) rc = gsh::launch_loss(ws, v, f, losses != nullptr);
+  if (rc == 0) rc = gsh::launch_backward(ws, v, f);
+  if (rc == 0) rc = gsh::launch_adam(ws, f, 1);
+  if (rc == 0 && losses != nullptr) {
+    GS_HIP_TRY(hipStreamSynchronize(v.stream));
+    gsh::sum_losses(ws, v, losses);
+  }
   return rc;
+}
+
+int gs_hip_train_views(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t count,
+                       const void* const* targets, uint64_t* pairs, double* losses) {
+  if (ws == nullptr || frames == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "train_views: null argument");
+  if (count == 0 || count > ws->views.size()) {
+    return gsh::set_error(GS_HIP_E_ARGUMENT, "train_views: count exceeds the workspace's views");
+  }
+  for (uint32_t v = 0; v < count; ++v) {
+    const gs_hip_frame& f = frames[v];
+    if (f.slots > ws->capacity || f.slots != frames[0].slots) {
+      return gsh::set_error(GS_HIP_E_ARGUMENT, "train_views: slots differ or exceed capacity");
+    }
+    if (f.cam.width != ws->width || f.cam.height != ws->height) {
+      return gsh::set_error(GS_HIP_E_ARGUMENT, "train_views: camera size differs from the workspace");
+    }
+  }
+  const size_t image_bytes = size_t(ws->width) * ws->height * 4;
+  int rc = 0;
+  // Projection, depth sort and pair count of every view, each on its stream.
+  for (uint32_t v = 0; v < count && rc == 0; ++v) {
+    gsh::View& view = *ws->views[v];
+    if (targets != nullptr && targets[v] != nullptr) {
+      GS_HIP_TRY(hipMemcpyAsync(view.target, targets[v], image_bytes, hipMemcpyDeviceToDevice, view.stream));
+    }
+    rc = gsh::launch_project(ws, view, &frames[v]);
+    if (rc == 0) rc = gsh::bin_count(ws, view, &frames[v]);
+  }
+  if (rc != 0) return rc;
+  std::vector<uint64_t> p(count);
+  for (uint32_t v = 0; v < count; ++v) {
+    gsh::View& view = *ws->views[v];
+    GS_HIP_TRY(hipStreamSynchronize(view.stream));
+    p[v] = view.host_totals[0];
+    rc = gsh::reserve_pairs(ws, view, p[v]);
+    if (rc != 0) return rc;
+    if (pairs != nullptr) pairs[v] = p[v];
+  }
+  // The rest of every view, stage by stage across the streams.
+  for (uint32_t v = 0; v < count && rc == 0; ++v) rc = gsh::bin_pairs(ws, *ws->views[v], &frames[v], p[v]);
+  for (uint32_t v = 0; v < count && rc == 0; ++v) rc = gsh::launch_forward(ws, *ws->views[v], &frames[v]);
+  for (uint32_t v = 0; v < count && rc == 0; ++v)
+    rc = gsh::launch_loss(ws, *ws->views[v], &frames[v], losses != nullptr);
+  for (uint32_t v = 0; v < count && rc == 0; ++v) rc = gsh::launch_backward(ws, *ws->views[v], &frames[v]);
+  if (rc != 0) return rc;
+  gsh::View& v0 = gsh::view0(ws);
+  for (uint32_t v = 1; v < count; ++v) {
+    gsh::View& view = *ws->views[v];
+    GS_HIP_TRY(hipEventRecord(view.done, view.stream));
+    GS_HIP_TRY(hipStreamWaitEvent(v0.stream, view.done, 0));
+  }
+  rc = gsh::launch_adam(ws, frames, count);
+  if (rc != 0) return rc;
+  if (losses != nullptr) {
+    for (uint32_t v = 0; v < count; ++v) {
+      gsh::View& view = *ws->views[v];
+      GS_HIP_TRY(hipStreamSynchronize(view.stream));
+      gsh::sum_losses(ws, view, losses + 3 * v);
+    }
+  }
+  return 0;
 }
 
 int gs_hip_render(gs_hip_workspace* ws, const gs_hip_frame* f, uint8_t* rgba8_out) {
@@ -510,18 +684,19 @@ int gs_hip_render(gs_hip_workspace* ws, const gs_hip_frame* f, uint8_t* rgba8_ou
   if (rc == 0) rc = gs_hip_forward(ws, f);
   if (rc != 0) return rc;
   if (rgba8_out != nullptr) {
+    gsh::View& v = gsh::view0(ws);
     const size_t pixels = size_t(ws->width) * ws->height;
     std::vector<float> im(pixels * 4);
-    GS_HIP_TRY(hipMemcpyAsync(im.data(), ws->image, pixels * 16, hipMemcpyDeviceToHost, ws->stream));
-    GS_HIP_TRY(hipStreamSynchronize(ws->stream));
-    auto to8 = [](float v) {
-      v = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
-      return uint8_t(v * 255.0f + 0.5f);
+    GS_HIP_TRY(hipMemcpyAsync(im.data(), v.image, pixels * 16, hipMemcpyDeviceToHost, v.stream));
+    GS_HIP_TRY(hipStreamSynchronize(v.stream));
+    auto to8 = [](float x) {
+      x = x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x);
+      return uint8_t(x * 255.0f + 0.5f);
     };
-    for (size_t p = 0; p < pixels; ++p) {
-      const float T = im[p * 4 + 3];
-      for (int c = 0; c < 3; ++c) rgba8_out[p * 4 + c] = to8(im[p * 4 + c] + T * f->background[c]);
-      rgba8_out[p * 4 + 3] = to8(1.0f - T);
+    for (size_t q = 0; q < pixels; ++q) {
+      const float T = im[q * 4 + 3];
+      for (int c = 0; c < 3; ++c) rgba8_out[q * 4 + c] = to8(im[q * 4 + c] + T * f->background[c]);
+      rgba8_out[q * 4 + 3] = to8(1.0f - T);
     }
   }
   return 0;

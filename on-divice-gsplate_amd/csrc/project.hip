@@ -153,23 +153,23 @@ __global__ void __launch_bounds__(256) k_project(gs_hip_frame f, ProjectArgs a) 
 }  // namespace
 
 namespace gsh {
-int launch_project(gs_hip_workspace* ws, const gs_hip_frame* f) {
+int launch_project(gs_hip_workspace* ws, View& v, const gs_hip_frame* f) {
   if (f->slots == 0) return 0;
   ProjectArgs a;
   a.params = reinterpret_cast<const float4*>(ws->params);
   a.sh = ws->sh;
-  a.records = reinterpret_cast<float4*>(ws->records);
-  a.footprint = reinterpret_cast<float4*>(ws->footprint);
-  a.tiles = ws->tiles;
-  a.depth_keys = ws->depth_keys;
-  a.depth_order = ws->depth_order;
-  a.grad2d = ws->grad2d;
+  a.records = reinterpret_cast<float4*>(v.records);
+  a.footprint = reinterpret_cast<float4*>(v.footprint);
+  a.tiles = v.tiles;
+  a.depth_keys = v.depth_keys;
+  a.depth_order = v.depth_order;
+  a.grad2d = v.grad2d;
   a.capacity = ws->capacity;
   a.sh_coeffs = ws->sh_coeffs;
-  timer_begin(ws, "project");
-  k_project<<<div_up(f->slots, 256), 256, 0, ws->stream>>>(*f, a);
+  timer_begin(ws, v, "project");
+  k_project<<<div_up(f->slots, 256), 256, 0, v.stream>>>(*f, a);
   GS_CHECK_LAUNCH("k_project");
-  timer_end(ws);
+  timer_end(ws, v);
   return 0;
 }
 }  // namespace gsh

@@ -35,6 +35,7 @@ constexpr int kBatch = 256;
 
 struct RasterArgs {
   const uint2* __restrict__ ranges;
+  const uint32_t* __restrict__ tile_order;  // launch order: longest tile lists first
   const uint32_t* __restrict__ pair_gaussians;
   const float4* __restrict__ records;
   float4* __restrict__ image;
@@ -52,9 +53,9 @@ struct PixelCoord {
   int wave, lane;
 };
 
-__device__ __forceinline__ PixelCoord pixel_coord(const gs_hip_frame& f) {
+__device__ __forceinline__ PixelCoord pixel_coord(const gs_hip_frame& f, uint32_t tile) {
   PixelCoord c;
-  c.tile = blockIdx.x;
+  c.tile = tile;
   const int tx = int(c.tile % f.tiles_x), ty = int(c.tile / f.tiles_x);
   c.wave = threadIdx.x >> 6;
   c.lane = threadIdx.x & 63;
@@ -86,7 +87,7 @@ __device__ __forceinline__ SplatEval eval_splat(float4 sa, float4 sb, float cx, 
 __global__ void __launch_bounds__(kThreads) k_forward(gs_hip_frame f, RasterArgs a) {
   __shared__ float4 s_a[kBatch], s_b[kBatch], s_c[kBatch];
   __shared__ uint64_t s_mask[4][kBatch / 64];
-  const PixelCoord pc = pixel_coord(f);
+  const PixelCoord pc = pixel_coord(f, a.tile_order[blockIdx.x]);
   const uint2 range = a.ranges[pc.tile];
   const float cx = float(pc.px) + 0.5f, cy = float(pc.py) + 0.5f;
   bool done = !pc.inside;
@@ -165,7 +166,7 @@ __global__ void __launch_bounds__(kThreads) k_backward(gs_hip_frame f, RasterArg
   __shared__ float s_grad[GS_HIP_GRAD2D_FLOATS][kBatch];
   __shared__ uint64_t s_mask[4][kBatch / 64];
   __shared__ uint32_t s_wave_last[4];
-  const PixelCoord pc = pixel_coord(f);
+  const PixelCoord pc = pixel_coord(f, a.tile_order[blockIdx.x]);
   const uint2 range = a.ranges[pc.tile];
   const float cx = float(pc.px) + 0.5f, cy = float(pc.py) + 0.5f;
   const int tx0 = (pc.tile % f.tiles_x) * kTile, ty0 = (pc.tile / f.tiles_x) * kTile;
@@ -292,41 +293,54 @@ __global__ void __launch_bounds__(kThreads) k_backward(gs_hip_frame f, RasterArg
   }
 }
 
-RasterArgs make_args(gs_hip_workspace* ws) {
+RasterArgs make_args(gs_hip_workspace* ws, gsh::View& v) {
   RasterArgs a;
-  a.ranges = reinterpret_cast<const uint2*>(ws->ranges);
-  a.pair_gaussians = ws->pair_gaussians;
-  a.records = reinterpret_cast<const float4*>(ws->records);
-  a.image = reinterpret_cast<float4*>(ws->image);
-  a.last = ws->last;
-  a.dldc = reinterpret_cast<const float4*>(ws->dldc);
-  a.grad2d = ws->grad2d;
+  a.ranges = reinterpret_cast<const uint2*>(v.ranges);
+  a.tile_order = v.tile_order;
+  a.pair_gaussians = v.pair_gaussians;
+  a.records = reinterpret_cast<const float4*>(v.records);
+  a.image = reinterpret_cast<float4*>(v.image);
+  a.last = v.last;
+  a.dldc = reinterpret_cast<const float4*>(v.dldc);
+  a.grad2d = v.grad2d;
   a.capacity = ws->capacity;
   return a;
 }
 
 }  // namespace
 
+namespace gsh {
+
+int launch_forward(gs_hip_workspace* ws, View& v, const gs_hip_frame* f) {
+  const uint32_t tiles = f->tiles_x * f->tiles_y;
+  timer_begin(ws, v, "forward");
+  k_forward<<<tiles, kThreads, 0, v.stream>>>(*f, make_args(ws, v));
+  GS_CHECK_LAUNCH("k_forward");
+  timer_end(ws, v);
+  return 0;
+}
+
+int launch_backward(gs_hip_workspace* ws, View& v, const gs_hip_frame* f) {
+  const uint32_t tiles = f->tiles_x * f->tiles_y;
+  timer_begin(ws, v, "backward");
+  k_backward<<<tiles, kThreads, 0, v.stream>>>(*f, make_args(ws, v));
+  GS_CHECK_LAUNCH("k_backward");
+  timer_end(ws, v);
+  return 0;
+}
+
+}  // namespace gsh
+
 extern "C" {
 
 int gs_hip_forward(gs_hip_workspace* ws, const gs_hip_frame* f) {
   if (ws == nullptr || f == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "forward: null argument");
-  const uint32_t tiles = f->tiles_x * f->tiles_y;
-  gsh::timer_begin(ws, "forward");
-  k_forward<<<tiles, kThreads, 0, ws->stream>>>(*f, make_args(ws));
-  GS_CHECK_LAUNCH("k_forward");
-  gsh::timer_end(ws);
-  return 0;
+  return gsh::launch_forward(ws, gsh::view0(ws), f);
 }
 
 int gs_hip_backward(gs_hip_workspace* ws, const gs_hip_frame* f) {
   if (ws == nullptr || f == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "backward: null argument");
-  const uint32_t tiles = f->tiles_x * f->tiles_y;
-  gsh::timer_begin(ws, "backward");
-  k_backward<<<tiles, kThreads, 0, ws->stream>>>(*f, make_args(ws));
-  GS_CHECK_LAUNCH("k_backward");
-  gsh::timer_end(ws);
-  return 0;
+  return gsh::launch_backward(ws, gsh::view0(ws), f);
 }
 
 }  // extern "C"

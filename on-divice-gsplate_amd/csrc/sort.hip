@@ -304,6 +304,34 @@ __global__ void __launch_bounds__(256) k_ranges(const uint32_t* __restrict__ key
   if (s == count - 1u) ranges[tile].y = count;
 }
 
+// Launch order of the rasterizers: tiles bucketed by floor(log2(list
+// length)), longest first, empty tiles last (longest-processing-time-first
+// scheduling within a factor of two).  One workgroup.
+__global__ void __launch_bounds__(1024) k_tile_order(const uint2* __restrict__ ranges, uint32_t tiles,
+                                                     uint32_t* __restrict__ order) {
+  __shared__ uint32_t s_count[33];
+  if (threadIdx.x < 33) s_count[threadIdx.x] = 0u;
+  __syncthreads();
+  for (uint32_t t = threadIdx.x; t < tiles; t += blockDim.x) {
+    const uint2 r = ranges[t];
+    atomicAdd(&s_count[__clz(r.y - r.x)], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t sum = 0;
+    for (int b = 0; b < 33; ++b) {
+      const uint32_t c = s_count[b];
+      s_count[b] = sum;
+      sum += c;
+    }
+  }
+  __syncthreads();
+  for (uint32_t t = threadIdx.x; t < tiles; t += blockDim.x) {
+    const uint2 r = ranges[t];
+    order[atomicAdd(&s_count[__clz(r.y - r.x)], 1u)] = t;
+  }
+}
+
 int bits_for(uint32_t n) {
   int bits = 0;
   while (bits < 32 && (uint64_t(1) << bits) < n) ++bits;
@@ -314,10 +342,8 @@ int bits_for(uint32_t n) {
 
 namespace gsh {
 
-int launch_project(gs_hip_workspace* ws, const gs_hip_frame* f);
-
-int reserve_sort(gs_hip_workspace* ws, uint64_t items) {
-  SortScratch& s = ws->sort;
+int reserve_sort(View& v, uint64_t items) {
+  SortScratch& s = v.sort;
   if (items <= s.items) return 0;
   if (s.keys_alt) hipFree(s.keys_alt);
   if (s.vals_alt) hipFree(s.vals_alt);
@@ -330,91 +356,94 @@ int reserve_sort(gs_hip_workspace* ws, uint64_t items) {
   GS_HIP_TRY(hipMalloc(reinterpret_cast<void**>(&s.hist), std::max<uint64_t>(blocks, 1) * 256 * 4));
   GS_HIP_TRY(hipMalloc(reinterpret_cast<void**>(&s.digit_totals), 256 * 4));
   s.items = items;
-  s.hist_entries = uint32_t(std::max<uint64_t>(blocks, 1) * 256);
   return 0;
 }
 
-int radix_sort(gs_hip_workspace* ws, uint32_t* keys, uint32_t* vals, uint32_t n, int bits) {
+int radix_sort(View& v, uint32_t* keys, uint32_t* vals, uint32_t n, int bits) {
   if (n <= 1 || bits <= 0) return 0;
-  if (n > ws->sort.items) return set_error(GS_HIP_E_CAPACITY, "radix_sort: scratch too small");
+  if (n > v.sort.items) return set_error(GS_HIP_E_CAPACITY, "radix_sort: scratch too small");
   const uint32_t blocks = div_up(n, kSortItems);
   const int passes = (bits + 7) / 8;
-  uint32_t *ka = keys, *va = vals, *kb = ws->sort.keys_alt, *vb = ws->sort.vals_alt;
+  uint32_t *ka = keys, *va = vals, *kb = v.sort.keys_alt, *vb = v.sort.vals_alt;
   for (int pass = 0; pass < passes; ++pass) {
     const int shift = pass * 8;
     const int nbits = std::min(8, bits - shift);
-    k_radix_upsweep<<<blocks, kSortThreads, 0, ws->stream>>>(ka, n, shift, nbits, ws->sort.hist, blocks);
+    k_radix_upsweep<<<blocks, kSortThreads, 0, v.stream>>>(ka, n, shift, nbits, v.sort.hist, blocks);
     GS_CHECK_LAUNCH("k_radix_upsweep");
-    k_radix_scan<<<1u << nbits, 256, 0, ws->stream>>>(ws->sort.hist, blocks, ws->sort.digit_totals);
+    k_radix_scan<<<1u << nbits, 256, 0, v.stream>>>(v.sort.hist, blocks, v.sort.digit_totals);
     GS_CHECK_LAUNCH("k_radix_scan");
-    k_radix_scatter<<<blocks, kSortThreads, 0, ws->stream>>>(ka, va, kb, vb, n, shift, nbits, ws->sort.hist,
-                                                             ws->sort.digit_totals, blocks);
+    k_radix_scatter<<<blocks, kSortThreads, 0, v.stream>>>(ka, va, kb, vb, n, shift, nbits, v.sort.hist,
+                                                           v.sort.digit_totals, blocks);
     GS_CHECK_LAUNCH("k_radix_scatter");
     std::swap(ka, kb);
     std::swap(va, vb);
   }
   if (ka != keys) {
-    GS_HIP_TRY(hipMemcpyAsync(keys, ka, size_t(n) * 4, hipMemcpyDeviceToDevice, ws->stream));
-    GS_HIP_TRY(hipMemcpyAsync(vals, va, size_t(n) * 4, hipMemcpyDeviceToDevice, ws->stream));
+    GS_HIP_TRY(hipMemcpyAsync(keys, ka, size_t(n) * 4, hipMemcpyDeviceToDevice, v.stream));
+    GS_HIP_TRY(hipMemcpyAsync(vals, va, size_t(n) * 4, hipMemcpyDeviceToDevice, v.stream));
   }
   return 0;
 }
 
-int exclusive_scan(gs_hip_workspace* ws, const uint32_t* values, const uint32_t* gather, uint32_t* out,
-                   uint32_t count, uint32_t* total_dev) {
+int exclusive_scan(View& v, const uint32_t* values, const uint32_t* gather, uint32_t* out, uint32_t count,
+                   uint32_t* total_dev) {
   const uint32_t blocks = std::max<uint32_t>(div_up(count, kScanItems), 1);
-  if (blocks > ws->block_sums_entries) return set_error(GS_HIP_E_CAPACITY, "exclusive_scan: too many blocks");
-  k_scan_reduce<<<blocks, kScanThreads, 0, ws->stream>>>(values, gather, count, ws->block_sums);
+  if (blocks > v.block_sums_entries) return set_error(GS_HIP_E_CAPACITY, "exclusive_scan: too many blocks");
+  k_scan_reduce<<<blocks, kScanThreads, 0, v.stream>>>(values, gather, count, v.block_sums);
   GS_CHECK_LAUNCH("k_scan_reduce");
-  k_scan_blocks<<<1, 1024, 0, ws->stream>>>(ws->block_sums, blocks, total_dev);
+  k_scan_blocks<<<1, 1024, 0, v.stream>>>(v.block_sums, blocks, total_dev);
   GS_CHECK_LAUNCH("k_scan_blocks");
-  k_scan_down<<<blocks, kScanThreads, 0, ws->stream>>>(values, gather, count, ws->block_sums, out);
+  k_scan_down<<<blocks, kScanThreads, 0, v.stream>>>(values, gather, count, v.block_sums, out);
   GS_CHECK_LAUNCH("k_scan_down");
   return 0;
 }
 
-// Depth sort, offsets, pair count to the host.
-int bin_count(gs_hip_workspace* ws, const gs_hip_frame* f, uint64_t* pairs) {
-  *pairs = 0;
-  if (f->slots == 0) return 0;
+// Depth sort and offsets; the pair count is copied to v.host_totals[0]
+// asynchronously (valid once v.stream has reached this point).
+int bin_count(gs_hip_workspace* ws, View& v, const gs_hip_frame* f) {
+  if (f->slots == 0) {
+    v.host_totals[0] = 0;
+    return 0;
+  }
   const int depth_bits = (f->flags & GS_HIP_FLAG_DEPTH16) != 0u ? 16 : 32;
-  timer_begin(ws, "depth_sort");
-  int rc = radix_sort(ws, ws->depth_keys, ws->depth_order, f->slots, depth_bits);
-  timer_end(ws);
+  timer_begin(ws, v, "depth_sort");
+  int rc = radix_sort(v, v.depth_keys, v.depth_order, f->slots, depth_bits);
+  timer_end(ws, v);
   if (rc != 0) return rc;
-  timer_begin(ws, "scan");
-  rc = exclusive_scan(ws, ws->tiles, ws->depth_order, ws->offsets, f->slots, ws->totals);
-  timer_end(ws);
+  timer_begin(ws, v, "scan");
+  rc = exclusive_scan(v, v.tiles, v.depth_order, v.offsets, f->slots, v.totals);
+  timer_end(ws, v);
   if (rc != 0) return rc;
-  GS_HIP_TRY(hipMemcpyAsync(ws->host_totals, ws->totals, 8, hipMemcpyDeviceToHost, ws->stream));
-  GS_HIP_TRY(hipStreamSynchronize(ws->stream));
-  *pairs = ws->host_totals[0];
+  GS_HIP_TRY(hipMemcpyAsync(v.host_totals, v.totals, 8, hipMemcpyDeviceToHost, v.stream));
   return 0;
 }
 
-int bin_pairs(gs_hip_workspace* ws, const gs_hip_frame* f, uint64_t pairs) {
+int bin_pairs(gs_hip_workspace* ws, View& v, const gs_hip_frame* f, uint64_t pairs) {
   const uint32_t tiles = f->tiles_x * f->tiles_y;
-  timer_begin(ws, "emit");
-  k_fill_ranges<<<div_up((tiles + 1) * 2, 256), 256, 0, ws->stream>>>(ws->ranges, (tiles + 1) * 2);
+  v.pairs = pairs;
+  timer_begin(ws, v, "emit");
+  k_fill_ranges<<<div_up((tiles + 1) * 2, 256), 256, 0, v.stream>>>(v.ranges, (tiles + 1) * 2);
   GS_CHECK_LAUNCH("k_fill_ranges");
-  if (pairs == 0) {
-    timer_end(ws);
-    return 0;
+  if (pairs > 0) {
+    k_emit<<<div_up(f->slots, 256), 256, 0, v.stream>>>(
+        *f, reinterpret_cast<const float4*>(v.records), reinterpret_cast<const float4*>(v.footprint), v.tiles,
+        v.depth_order, v.offsets, v.pair_tiles, v.pair_gaussians, v.pair_capacity, v.totals);
+    GS_CHECK_LAUNCH("k_emit");
   }
-  k_emit<<<div_up(f->slots, 256), 256, 0, ws->stream>>>(
-      *f, reinterpret_cast<const float4*>(ws->records), reinterpret_cast<const float4*>(ws->footprint), ws->tiles,
-      ws->depth_order, ws->offsets, ws->pair_tiles, ws->pair_gaussians, ws->pair_capacity, ws->totals);
-  GS_CHECK_LAUNCH("k_emit");
-  timer_end(ws);
-  timer_begin(ws, "tile_sort");
-  int rc = radix_sort(ws, ws->pair_tiles, ws->pair_gaussians, uint32_t(pairs), bits_for(tiles + 1));
-  timer_end(ws);
-  if (rc != 0) return rc;
-  timer_begin(ws, "ranges");
-  k_ranges<<<div_up(pairs, 256), 256, 0, ws->stream>>>(ws->pair_tiles, uint32_t(pairs),
-                                                          reinterpret_cast<uint2*>(ws->ranges));
-  GS_CHECK_LAUNCH("k_ranges");
-  timer_end(ws);
+  timer_end(ws, v);
+  if (pairs > 0) {
+    timer_begin(ws, v, "tile_sort");
+    const int rc = radix_sort(v, v.pair_tiles, v.pair_gaussians, uint32_t(pairs), bits_for(tiles + 1));
+    timer_end(ws, v);
+    if (rc != 0) return rc;
+    timer_begin(ws, v, "ranges");
+    k_ranges<<<div_up(pairs, 256), 256, 0, v.stream>>>(v.pair_tiles, uint32_t(pairs),
+                                                        reinterpret_cast<uint2*>(v.ranges));
+    GS_CHECK_LAUNCH("k_ranges");
+    timer_end(ws, v);
+  }
+  k_tile_order<<<1, 1024, 0, v.stream>>>(reinterpret_cast<const uint2*>(v.ranges), tiles, v.tile_order);
+  GS_CHECK_LAUNCH("k_tile_order");
   return 0;
 }
 
@@ -428,17 +457,19 @@ int gs_hip_project(gs_hip_workspace* ws, const gs_hip_frame* f) {
   if (f->cam.width != ws->width || f->cam.height != ws->height) {
     return gsh::set_error(GS_HIP_E_ARGUMENT, "project: camera size differs from the workspace");
   }
-  return gsh::launch_project(ws, f);
+  return gsh::launch_project(ws, gsh::view0(ws), f);
 }
 
 int gs_hip_bin(gs_hip_workspace* ws, const gs_hip_frame* f, uint64_t* pairs) {
   if (ws == nullptr || f == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "bin: null argument");
-  uint64_t p = 0;
-  int rc = gsh::bin_count(ws, f, &p);
+  gsh::View& v = gsh::view0(ws);
+  int rc = gsh::bin_count(ws, v, f);
   if (rc != 0) return rc;
-  rc = gs_hip_workspace_reserve_pairs(ws, p);
+  GS_HIP_TRY(hipStreamSynchronize(v.stream));
+  const uint64_t p = v.host_totals[0];
+  rc = gsh::reserve_pairs(ws, v, p);
   if (rc != 0) return rc;
-  rc = gsh::bin_pairs(ws, f, p);
+  rc = gsh::bin_pairs(ws, v, f, p);
   if (pairs != nullptr) *pairs = p;
   return rc;
 }

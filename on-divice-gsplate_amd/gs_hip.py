@@ -25,14 +25,16 @@ FLAG_DEPTH16 = 4
 
 (BUF_PARAMS, BUF_SH, BUF_M_PARAMS, BUF_V_PARAMS, BUF_M_SH, BUF_V_SH, BUF_STATS, BUF_RECORDS, BUF_FOOTPRINT,
  BUF_TILES, BUF_DEPTH_KEYS, BUF_DEPTH_ORDER, BUF_OFFSETS, BUF_GRAD2D, BUF_PAIR_TILES, BUF_PAIR_GAUSSIANS,
- BUF_RANGES, BUF_IMAGE, BUF_LAST, BUF_TARGET, BUF_DLDC, BUF_TILE_LOSS, BUF_TOTALS) = range(23)
+ BUF_RANGES, BUF_IMAGE, BUF_LAST, BUF_TARGET, BUF_DLDC, BUF_TILE_LOSS, BUF_TOTALS, BUF_TILE_ORDER) = range(24)
+MAX_VIEWS = 8
 
 EXPORTED = [
     "gs_hip_init", "gs_hip_device_name", "gs_hip_malloc", "gs_hip_free", "gs_hip_upload", "gs_hip_download",
     "gs_hip_memset", "gs_hip_synchronize", "gs_hip_last_error", "gs_hip_workspace_create",
-    "gs_hip_workspace_destroy", "gs_hip_workspace_buffer", "gs_hip_workspace_reserve_pairs",
+    "gs_hip_workspace_destroy", "gs_hip_workspace_buffer", "gs_hip_workspace_views",
+    "gs_hip_workspace_view_buffer", "gs_hip_workspace_reserve_pairs",
     "gs_hip_workspace_load", "gs_hip_workspace_store", "gs_hip_project", "gs_hip_bin", "gs_hip_forward",
-    "gs_hip_set_target", "gs_hip_set_target_device", "gs_hip_loss", "gs_hip_backward", "gs_hip_adam", "gs_hip_densify", "gs_hip_reset_opacity", "gs_hip_train_step",
+    "gs_hip_set_target", "gs_hip_set_target_device", "gs_hip_loss", "gs_hip_backward", "gs_hip_adam", "gs_hip_densify", "gs_hip_reset_opacity", "gs_hip_train_step", "gs_hip_train_views",
     "gs_hip_render", "gs_hip_workspace_timings",
 ]
 
@@ -80,7 +82,9 @@ def library() -> ctypes.CDLL:
             "gs_hip_workspace_create": [ctypes.POINTER(vp), u32, i32, u32, u32],
             "gs_hip_workspace_destroy": [vp],
             "gs_hip_workspace_buffer": [vp, i32, ctypes.POINTER(vp), ctypes.POINTER(sz)],
-            "gs_hip_workspace_reserve_pairs": [vp, u64],
+            "gs_hip_workspace_reserve_pairs": [vp, u64], "gs_hip_workspace_views": [vp, u32],
+            "gs_hip_workspace_view_buffer": [vp, u32, i32, ctypes.POINTER(vp), ctypes.POINTER(sz)],
+            "gs_hip_train_views": [vp, frame_p, u32, ctypes.POINTER(vp), u64p, dp],
             "gs_hip_workspace_load": [vp, fp, fp, u32, u32], "gs_hip_workspace_store": [vp, fp, fp, u32],
             "gs_hip_project": [vp, frame_p], "gs_hip_bin": [vp, frame_p, u64p], "gs_hip_forward": [vp, frame_p],
             "gs_hip_set_target": [vp, ctypes.POINTER(ctypes.c_uint8)], "gs_hip_set_target_device": [vp, vp],
@@ -223,13 +227,31 @@ class Workspace:
         except Exception:
             pass
 
-    def buffer(self, which: int):
+    def buffer(self, which: int, view: int = 0):
         ptr, size = ctypes.c_void_p(), ctypes.c_size_t()
-        check(self.lib.gs_hip_workspace_buffer(self.handle, which, ctypes.byref(ptr), ctypes.byref(size)))
+        check(self.lib.gs_hip_workspace_view_buffer(self.handle, view, which, ctypes.byref(ptr),
+                                                    ctypes.byref(size)))
         return ptr, size.value
 
-    def download(self, which: int, dtype, count: int | None = None) -> np.ndarray:
-        ptr, size = self.buffer(which)
+    def set_views(self, count: int) -> None:
+        check(self.lib.gs_hip_workspace_views(self.handle, count))
+
+    def train_views(self, frames, targets=None, want_losses: bool = True):
+        """One batched optimizer step over len(frames) views; targets: device
+        pointers (or None).  Returns (pairs per view, [3 losses per view])."""
+        n = len(frames)
+        arr = (Frame * n)(*frames)
+        tarr = None
+        if targets is not None:
+            tarr = (ctypes.c_void_p * n)(*[t if isinstance(t, int) else (t.value if t else None) for t in targets])
+        pairs = (ctypes.c_uint64 * n)()
+        losses = (ctypes.c_double * (3 * n))()
+        check(self.lib.gs_hip_train_views(self.handle, arr, n, tarr, pairs, losses if want_losses else None))
+        ls = [tuple(losses[3 * v:3 * v + 3]) for v in range(n)] if want_losses else None
+        return list(pairs), ls
+
+    def download(self, which: int, dtype, count: int | None = None, view: int = 0) -> np.ndarray:
+        ptr, size = self.buffer(which, view)
         item = np.dtype(dtype).itemsize
         n = size // item if count is None else count
         out = np.empty(n, dtype)

@@ -328,7 +328,10 @@ def test_densify_masks_and_slots():
     assert placed == min(len(spawners), len(free_list) + cap - n) and placed > 0
     assert slots == n + max(0, placed - len(free_list))
     assert np.all(st_after == 0)
-    assert np.all(p_after[:n][prune.astype(bool), 3] == -20.0)
+    targets = [free_list[k] if k < len(free_list) else n + k - len(free_list) for k in range(placed)]
+    reused = np.zeros(n, bool)
+    reused[[t for t in targets if t < n]] = True
+    assert np.all(p_after[:n][prune.astype(bool) & ~reused, 3] == -20.0)  # pruned and not refilled
     seed = frame.seed
     for k, i in enumerate(spawners[:placed]):
         target = free_list[k] if k < len(free_list) else n + k - len(free_list)
@@ -360,3 +363,51 @@ def test_densify_masks_and_slots():
             s = np.exp(params[i, 4:7].astype(np.float64))
             np.testing.assert_allclose(p_after[i, 0:3], params[i, 0:3] + R @ (na * s), atol=2e-5)
             np.testing.assert_allclose(p_after[target, 0:3], params[i, 0:3] + R @ (nb * s), atol=2e-5)
+
+
+def test_train_views_matches_single_view_steps():
+    """A batched step over four views, each on its own stream: every view's
+    2D gradient equals that view's plain step from the same state, and the
+    optimizer sees their mean (Adam's first moment over 1 - beta1)."""
+    case("batched views vs single-view steps")
+    w, h, n, deg, V = 96, 72, 500, 2, 4
+    params, sh = make_scene(n, deg, 91)
+    cams = [make_camera(w, h, 0.5 + 1.3 * v) for v in range(V)]
+    targets = [smooth_target(w, h, 91 + v) for v in range(V)]
+    s_g2, s_gp, s_gsh, s_loss, s_vis = [], [], [], [], []
+    for v in range(V):
+        with gs.Workspace(n, deg, w, h) as ws:
+            ws.load(params, sh)
+            ws.set_target(targets[v])
+            _, losses = ws.train_step(gs.make_frame(cams[v], n, deg, tp=frozen()))
+            s_g2.append(ws.download(gs.BUF_GRAD2D, np.float32, 9 * n).reshape(9, n).T)
+            gp, gsh = _first_step_gradients(ws, n)
+            s_gp.append(gp)
+            s_gsh.append(gsh)
+            s_loss.append(losses)
+            s_vis.append(ws.download(gs.BUF_TILES, np.uint32, n) > 0)
+    bufs = [gs.DeviceBuffer(t) for t in targets]
+    try:
+        with gs.Workspace(n, deg, w, h) as ws:
+            ws.set_views(V)
+            ws.load(params, sh)
+            frames = [gs.make_frame(c, n, deg, tp=frozen()) for c in cams]
+            pairs, losses = ws.train_views(frames, [b.ptr for b in bufs])
+            for v in range(V):
+                g2 = ws.download(gs.BUF_GRAD2D, np.float32, 9 * n, view=v).reshape(9, n).T
+                scale = np.abs(s_g2[v]).max(axis=0) + 1e-30
+                assert (np.abs(g2 - s_g2[v]).max(axis=0) / scale).max() < 1e-5, v
+                assert losses[v] == pytest.approx(s_loss[v], rel=1e-6)
+            gp, gsh = _first_step_gradients(ws, n)
+            stats = ws.download(gs.BUF_STATS, np.float32, n * 4).reshape(n, 4)
+            after = ws.store(n)[0]
+    finally:
+        for b in bufs:
+            b.free()
+    np.testing.assert_array_equal(after, params)
+    mean_gp, mean_gsh = np.mean(s_gp, axis=0), np.mean(s_gsh, axis=0)
+    for cols in ([0, 1, 2], [3], [4, 5, 6], [8, 9, 10, 11], [12, 13, 14]):
+        scale = np.abs(mean_gp[:, cols]).max()
+        assert np.abs(gp[:, cols] - mean_gp[:, cols]).max() <= 1e-5 * scale, cols
+    assert np.abs(gsh - mean_gsh).max() <= 1e-5 * np.abs(mean_gsh).max()
+    np.testing.assert_array_equal(stats[:, 1], np.sum(s_vis, axis=0).astype(np.float32))
