@@ -109,6 +109,7 @@ int create_view(gs_hip_workspace* ws, View** out) {
   A(alloc(&v->records, cap * GS_HIP_RECORD_FLOATS));
   A(alloc(&v->footprint, cap * 4));
   A(alloc(&v->tiles, cap));
+  A(alloc(&v->misc, cap));
   A(alloc(&v->depth_keys, cap));
   A(alloc(&v->depth_order, cap));
   A(alloc(&v->offsets, cap));
@@ -151,6 +152,7 @@ void destroy_view(View* v) {
   release(v->records);
   release(v->footprint);
   release(v->tiles);
+  release(v->misc);
   release(v->depth_keys);
   release(v->depth_order);
   release(v->offsets);

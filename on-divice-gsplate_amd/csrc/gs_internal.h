@@ -51,6 +51,7 @@ struct View {
   float* records = nullptr;
   float* footprint = nullptr;
   uint32_t* tiles = nullptr;
+  uint32_t* misc = nullptr;  // per slot: radius << 3 | colour clamp bits (the adjoint's view of the record)
   uint32_t* depth_keys = nullptr;
   uint32_t* depth_order = nullptr;
   uint32_t* offsets = nullptr;

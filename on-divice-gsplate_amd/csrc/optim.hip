@@ -22,11 +22,11 @@ namespace {
 using namespace gsd;
 
 // One view's inputs to the adjoint: its camera and image size, its 2D
-// gradient planes, raster records (radius, clamp bits) and tile counts.
+// gradient planes, per-slot radius and clamp bits, and tile counts.
 struct AdamView {
   gs_hip_camera cam;
   const float* __restrict__ grad2d;
-  const float4* __restrict__ records;
+  const uint32_t* __restrict__ misc;
   const uint32_t* __restrict__ tiles;
   float4* __restrict__ sh_input;  // per slot: (dL/dcolour rgb, update flag in view 0), (direction, -)
 };
@@ -267,7 +267,7 @@ __global__ void __launch_bounds__(256) k_adam(gs_hip_frame f, AdamArgs a) {
     }
     // Densification statistics: view-space positional gradient in NDC units.
     const float gx = g[0] * (0.5f * float(view.cam.width)), gy = g[1] * (0.5f * float(view.cam.height));
-    const uint32_t misc = __float_as_uint(view.records[size_t(i) * 3u + 2u].w);
+    const uint32_t misc = view.misc[i];
     st.x += sqrtf(gx * gx + gy * gy);
     st.y += 1.0f;
     st.z = fmaxf(st.z, float(misc >> 3));
@@ -295,32 +295,70 @@ __global__ void __launch_bounds__(256) k_adam(gs_hip_frame f, AdamArgs a) {
   a.params[i * 4u + 3u] = adam4(p3, scale4(dp3), a.m_params + i * 4u + 3u, a.v_params + i * 4u + 3u, lr3, f);
 }
 
-// Adam on the SH coefficients beyond DC, one thread per (slot, coefficient)
-// (train_step_sh.comp:21-44): g = mean over the views of basis_k(dir) dL/dcolour.
+// Adam on coefficients [k0, k0 + kShGroup) of one slot: all loads of the
+// group are issued before any store (the planes are distinct arrays, which
+// the restrict parameters tell the compiler), so a wave has 3 x kShGroup
+// coalesced 768 B loads in flight per array.
+constexpr int kShGroup = 5;
+
+__device__ __forceinline__ void adam_sh_group(float* __restrict__ sh, float* __restrict__ m_sh,
+                                              float* __restrict__ v_sh, const float (&g)[GS_HIP_MAX_SH_REST][3],
+                                              int k0, int coeffs, uint32_t capacity, uint32_t i, float inv,
+                                              const gs_hip_frame& f) {
+  float p[kShGroup][3], m[kShGroup][3], v[kShGroup][3];
+  for (int d = 0; d < kShGroup; ++d) {
+    if (k0 + d >= coeffs) break;
+    const size_t o = (size_t(k0 + d) * capacity + i) * 3u;
+    for (int c = 0; c < 3; ++c) p[d][c] = sh[o + c], m[d][c] = m_sh[o + c], v[d][c] = v_sh[o + c];
+  }
+  for (int d = 0; d < kShGroup; ++d) {
+    if (k0 + d >= coeffs) break;
+    const size_t o = (size_t(k0 + d) * capacity + i) * 3u;
+    for (int c = 0; c < 3; ++c) {
+      sh[o + c] = adam1(p[d][c], g[k0 + d][c] * inv, m[d][c], v[d][c], f.lr_sh_rest, f);
+      m_sh[o + c] = m[d][c];
+      v_sh[o + c] = v[d][c];
+    }
+  }
+}
+
+// Adam on the SH coefficients beyond DC (train_step_sh.comp:21-44), one
+// thread per slot: g_k = mean over the views of basis_k(dir) dL/dcolour,
+// with each view's (dL/dcolour, dir) read once and the basis evaluated once
+// per view; then Adam on each coefficient plane (coalesced across the wave).
 __global__ void __launch_bounds__(256) k_adam_sh(gs_hip_frame f, AdamArgs a) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t k = blockIdx.y;
   if (i >= f.slots) return;
   if (a.views[0].sh_input[size_t(i) * 2u].w == 0.0f) return;  // free slot, or kept by sparse Adam
-  float g[3] = {0.0f, 0.0f, 0.0f};
-  for (uint32_t v = 0; v < a.count; ++v) {
-    const float4 c = a.views[v].sh_input[size_t(i) * 2u];
+  const int coeffs = min(sh_rest_coeffs(f.sh_degree), int(a.sh_coeffs));
+  float g[GS_HIP_MAX_SH_REST][3];
+  for (int k = 0; k < GS_HIP_MAX_SH_REST; ++k) g[k][0] = g[k][1] = g[k][2] = 0.0f;
+  // Every view's (dL/dcolour, direction) is loaded before any is used.
+  float4 cv[GS_HIP_MAX_VIEWS], dv[GS_HIP_MAX_VIEWS];
+#pragma unroll
+  for (uint32_t v = 0; v < GS_HIP_MAX_VIEWS; ++v) {
+    if (v >= a.count) break;
+    cv[v] = a.views[v].sh_input[size_t(i) * 2u];
+    dv[v] = a.views[v].sh_input[size_t(i) * 2u + 1u];
+  }
+#pragma unroll
+  for (uint32_t v = 0; v < GS_HIP_MAX_VIEWS; ++v) {
+    if (v >= a.count) break;
+    const float4 c = cv[v], d = dv[v];
     if (c.x == 0.0f && c.y == 0.0f && c.z == 0.0f) continue;
-    const float4 d = a.views[v].sh_input[size_t(i) * 2u + 1u];
     float basis[16];
     sh_basis(d.x, d.y, d.z, f.sh_degree, basis);
-    const float b = basis[k + 1];
-    g[0] += b * c.x;
-    g[1] += b * c.y;
-    g[2] += b * c.z;
+    for (int k = 0; k < GS_HIP_MAX_SH_REST; ++k) {
+      g[k][0] += basis[k + 1] * c.x;
+      g[k][1] += basis[k + 1] * c.y;
+      g[k][2] += basis[k + 1] * c.z;
+    }
   }
   const float inv = 1.0f / float(a.count);
-  const size_t o = (size_t(k) * a.capacity + i) * 3u;
-  for (int c = 0; c < 3; ++c) {
-    float m = a.m_sh[o + c], v = a.v_sh[o + c];
-    a.sh[o + c] = adam1(a.sh[o + c], g[c] * inv, m, v, f.lr_sh_rest, f);
-    a.m_sh[o + c] = m;
-    a.v_sh[o + c] = v;
+#pragma unroll
+  for (int k0 = 0; k0 < GS_HIP_MAX_SH_REST; k0 += kShGroup) {
+    if (k0 >= coeffs) break;
+    adam_sh_group(a.sh, a.m_sh, a.v_sh, g, k0, coeffs, a.capacity, i, inv, f);
   }
 }
 
@@ -507,7 +545,7 @@ int launch_adam(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t count
     View& view = *ws->views[v];
     a.views[v].cam = frames[v].cam;
     a.views[v].grad2d = view.grad2d;
-    a.views[v].records = reinterpret_cast<const float4*>(view.records);
+    a.views[v].misc = view.misc;
     a.views[v].tiles = view.tiles;
     a.views[v].sh_input = reinterpret_cast<float4*>(view.sh_input);
   }
@@ -517,7 +555,7 @@ int launch_adam(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t count
   GS_CHECK_LAUNCH("k_adam");
   const int coeffs = std::min(f.sh_degree == 0 ? 0 : int((f.sh_degree + 1) * (f.sh_degree + 1)) - 1, int(ws->sh_coeffs));
   if (coeffs > 0) {
-    k_adam_sh<<<dim3(div_up(f.slots, 256), coeffs), 256, 0, v0.stream>>>(f, a);
+    k_adam_sh<<<div_up(f.slots, 256), 256, 0, v0.stream>>>(f, a);
     GS_CHECK_LAUNCH("k_adam_sh");
   }
   timer_end(ws, v0);

@@ -19,6 +19,7 @@ struct ProjectArgs {
   float4* __restrict__ records;
   float4* __restrict__ footprint;
   uint32_t* __restrict__ tiles;
+  uint32_t* __restrict__ misc;
   uint32_t* __restrict__ depth_keys;
   uint32_t* __restrict__ depth_order;
   float* __restrict__ grad2d;
@@ -136,6 +137,7 @@ __global__ void __launch_bounds__(256) k_project(gs_hip_frame f, ProjectArgs a) 
   r[2] = rec.c;
   a.footprint[i] = foot;
   a.tiles[i] = tiles;
+  a.misc[i] = __float_as_uint(rec.c.w);
   uint32_t key = 0xffffffffu;
   if (tiles > 0u) {
     if ((f.flags & GS_HIP_FLAG_DEPTH16) != 0u) {
@@ -161,6 +163,7 @@ int launch_project(gs_hip_workspace* ws, View& v, const gs_hip_frame* f) {
   a.records = reinterpret_cast<float4*>(v.records);
   a.footprint = reinterpret_cast<float4*>(v.footprint);
   a.tiles = v.tiles;
+  a.misc = v.misc;
   a.depth_keys = v.depth_keys;
   a.depth_order = v.depth_order;
   a.grad2d = v.grad2d;

@@ -3,11 +3,14 @@
 // 256 threads.  Each wave owns an 8x8 quadrant of the tile.
 //
 // Both kernels stage the tile's splats through LDS in batches of 256 (one
-// coalesced 48 B record load per thread), and while loading each thread tests
-// its splat's footprint bbox against the four quadrants: a ballot per quadrant
-// gives each wave a bitmask of the batch's splats that can reach its pixels,
-// and the wave walks only those bits.  Splat parameters are then LDS broadcast
-// reads, and the per-pixel work is pure VALU.
+// coalesced 48 B record load per thread).  The staging thread rewrites the
+// record into evaluation form, (mean, -a/2, -b), (-c/2, opacity, r, g), b, so
+// the per-pixel exponent is power = dx (A dx + B dy) + C dy^2 (the same
+// quadratic form as the reference's -(a dx^2 + c dy^2)/2 - b dx dy), and
+// tests the footprint bbox against the four quadrants: a ballot per quadrant
+// gives each wave a 64-bit mask of the batch's splats that can reach its
+// pixels.  The mask is moved to scalar registers, so the walk over its bits
+// is scalar control flow and the vector unit only evaluates splats.
 //
 // Forward (engine/shaders/train_forward.comp:59-141): alpha = min(0.99,
 // o exp(power)), skipped below 1/255 or for power > 0; a pixel stops before
@@ -19,9 +22,9 @@
 // its tile's list back to front from its last contributor, recovering
 // T_i = T_{i+1} / (1 - alpha_i) and the colour behind each splat; nothing
 // per pixel is stored by the forward besides T, LAST and dL/dC.  The nine
-// per-splat gradients are summed over each wave with DPP/permlane, over the
-// four waves in LDS (ds_add_f32), and added to the slot's planar GRAD2D with
-// one global float atomic each.
+// per-splat gradients are summed over each wave by a transposed reduction
+// (one quad ends with each value), over the four waves in LDS (ds_add_f32),
+// and added to the slot's planar GRAD2D with one global float atomic each.
 #include "gs_common.h"
 #include "gs_internal.h"
 #include "wave.h"
@@ -32,6 +35,7 @@ using namespace gsd;
 
 constexpr int kThreads = 256;
 constexpr int kBatch = 256;
+constexpr int kGrad = GS_HIP_GRAD2D_FLOATS;
 
 struct RasterArgs {
   const uint2* __restrict__ ranges;
@@ -46,38 +50,72 @@ struct RasterArgs {
 };
 
 struct PixelCoord {
-  int px, py;      // pixel
-  int rx0, ry0;    // the wave's quadrant origin
+  int px, py;  // pixel
+  int tx0, ty0;  // tile origin
   bool inside;
   uint32_t tile;
-  int wave, lane;
+  int wave, lane;  // wave is scalar
 };
 
 __device__ __forceinline__ PixelCoord pixel_coord(const gs_hip_frame& f, uint32_t tile) {
   PixelCoord c;
   c.tile = tile;
-  const int tx = int(c.tile % f.tiles_x), ty = int(c.tile / f.tiles_x);
-  c.wave = threadIdx.x >> 6;
-  c.lane = threadIdx.x & 63;
-  c.rx0 = tx * kTile + (c.wave & 1) * 8;
-  c.ry0 = ty * kTile + (c.wave >> 1) * 8;
-  c.px = c.rx0 + (c.lane & 7);
-  c.py = c.ry0 + (c.lane >> 3);
+  c.tx0 = int(tile % f.tiles_x) * kTile;
+  c.ty0 = int(tile / f.tiles_x) * kTile;
+  c.wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+  c.lane = int(threadIdx.x & 63u);
+  c.px = c.tx0 + (c.wave & 1) * 8 + (c.lane & 7);
+  c.py = c.ty0 + (c.wave >> 1) * 8 + (c.lane >> 3);
   c.inside = c.px < int(f.cam.width) && c.py < int(f.cam.height);
   return c;
+}
+
+// One batch of splats in LDS, in evaluation form.
+struct Staging {
+  float4 a[kBatch];  // mean x, mean y, A = -a/2, B = -b
+  float4 b[kBatch];  // C = -c/2, opacity, r, g
+  float blue[kBatch];
+  uint64_t mask[4][kBatch / 64];  // [quadrant][loading wave]
+};
+
+// Loads pair k's record into slot t and returns which quadrants its bbox
+// reaches (bit q).
+__device__ __forceinline__ uint32_t stage(Staging& s, const RasterArgs& a, uint32_t g, int t, int tx0, int ty0) {
+  const float4* r = a.records + size_t(g) * 3u;
+  const float4 ra = r[0], rb = r[1], rc = r[2];
+  s.a[t] = make_float4(ra.x, ra.y, -0.5f * ra.z, -ra.w);
+  s.b[t] = make_float4(-0.5f * rb.x, rb.y, rb.z, rb.w);
+  s.blue[t] = rc.x;
+  const uint32_t lo = __float_as_uint(rc.y), hi = __float_as_uint(rc.z);
+  uint32_t hits = 0u;
+  for (int q = 0; q < 4; ++q) {
+    const int qx = tx0 + (q & 1) * 8, qy = ty0 + (q >> 1) * 8;
+    hits |= bbox_hits(lo, hi, qx, qy, qx + 7, qy + 7) ? (1u << q) : 0u;
+  }
+  return hits;
+}
+
+__device__ __forceinline__ void publish_masks(Staging& s, uint32_t hits, int wave, int lane) {
+  for (int q = 0; q < 4; ++q) {
+    const uint64_t m = __ballot((hits >> q) & 1u);
+    if (lane == 0) s.mask[q][wave] = m;
+  }
 }
 
 // Power and alpha of a splat at a pixel centre, shared by both kernels so
 // they make the same skip decisions on the same bits.
 struct SplatEval {
-  float dx, dy, power, G, raw, alpha;
+  float dx, dy, adx, cdy, power, G, raw, alpha;
 };
 
 __device__ __forceinline__ SplatEval eval_splat(float4 sa, float4 sb, float cx, float cy) {
   SplatEval e;
   e.dx = sa.x - cx;
   e.dy = sa.y - cy;
-  e.power = -0.5f * (sa.z * e.dx * e.dx + sb.x * e.dy * e.dy) - sa.w * e.dx * e.dy;
+  e.adx = sa.z * e.dx;
+  e.cdy = sb.x * e.dy;
+  const float u = fmaf(sa.w, e.dy, e.adx);  // A dx + B dy
+  e.power = fmaf(e.dx, u, e.cdy * e.dy);
   e.G = __expf(e.power);
   e.raw = sb.y * e.G;
   e.alpha = fminf(kMaxAlpha, e.raw);
@@ -85,70 +123,50 @@ __device__ __forceinline__ SplatEval eval_splat(float4 sa, float4 sb, float cx, 
 }
 
 __global__ void __launch_bounds__(kThreads) k_forward(gs_hip_frame f, RasterArgs a) {
-  __shared__ float4 s_a[kBatch], s_b[kBatch], s_c[kBatch];
-  __shared__ uint64_t s_mask[4][kBatch / 64];
+  __shared__ Staging s;
   const PixelCoord pc = pixel_coord(f, a.tile_order[blockIdx.x]);
   const uint2 range = a.ranges[pc.tile];
   const float cx = float(pc.px) + 0.5f, cy = float(pc.py) + 0.5f;
   bool done = !pc.inside;
   float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
   uint32_t last = 0u;
-  const int tx0 = (pc.tile % f.tiles_x) * kTile, ty0 = (pc.tile / f.tiles_x) * kTile;
 
   for (uint32_t start = range.x; start < range.y; start += kBatch) {
     if (__syncthreads_and(done)) break;
     const uint32_t n = min(uint32_t(kBatch), range.y - start);
-    bool hit[4] = {false, false, false, false};
-    if (threadIdx.x < n) {
-      const uint32_t g = a.pair_gaussians[start + threadIdx.x];
-      const float4* r = a.records + size_t(g) * 3u;
-      const float4 ra = r[0], rb = r[1], rc = r[2];
-      s_a[threadIdx.x] = ra;
-      s_b[threadIdx.x] = rb;
-      s_c[threadIdx.x] = rc;
-      const uint32_t lo = __float_as_uint(rc.y), hi = __float_as_uint(rc.z);
-      for (int q = 0; q < 4; ++q) {
-        const int qx = tx0 + (q & 1) * 8, qy = ty0 + (q >> 1) * 8;
-        hit[q] = bbox_hits(lo, hi, qx, qy, qx + 7, qy + 7);
-      }
-    }
-    for (int q = 0; q < 4; ++q) {
-      const uint64_t m = __ballot(hit[q]);
-      if (pc.lane == 0) s_mask[q][pc.wave] = m;
-    }
+    uint32_t hits = 0u;
+    if (threadIdx.x < n) hits = stage(s, a, a.pair_gaussians[start + threadIdx.x], threadIdx.x, pc.tx0, pc.ty0);
+    publish_masks(s, hits, pc.wave, pc.lane);
     __syncthreads();
-    bool wave_live = __ballot(!done) != 0ull;
-    for (int chunk = 0; chunk < kBatch / 64 && wave_live; ++chunk) {
-      uint64_t m = s_mask[pc.wave][chunk];
-      while (m != 0ull) {
-        const int bit = __builtin_ctzll(m);
-        m &= m - 1ull;
-        const int j = chunk * 64 + bit;
-        if (!done) {
-          const SplatEval e = eval_splat(s_a[j], s_b[j], cx, cy);
-          if (e.power <= 0.0f) {
-            const float alpha = e.alpha;
-            if (alpha >= kMinAlpha) {
-              const float next_T = T * (1.0f - alpha);
-              if (next_T < kMinTransmittance) {
-                done = true;
-              } else {
-                const float4 sc = s_c[j];
-                const float w = alpha * T;
-                const float4 sb = s_b[j];
-                cr += sb.z * w;
-                cg += sb.w * w;
-                cb += sc.x * w;
-                T = next_T;
-                last = start + uint32_t(j) - range.x + 1u;
-              }
-            }
+    if (__ballot(!done) != 0ull) {
+      const uint32_t base = start - range.x + 1u;  // LAST of splat j is base + j
+      for (int chunk = 0; chunk < kBatch / 64; ++chunk) {
+        uint64_t m = uniform_u64(s.mask[pc.wave][chunk]);
+        bool live = true;
+        while (m != 0ull) {
+          const int j = chunk * 64 + __builtin_ctzll(m);
+          m &= m - 1ull;
+          // Branch-free: lanes that are done or skip this splat add zero.
+          const float4 sb = s.b[j];
+          const float blue = s.blue[j];
+          const SplatEval e = eval_splat(s.a[j], sb, cx, cy);
+          const float next_T = T * (1.0f - e.alpha);
+          const bool ok = !done && e.power <= 0.0f && e.alpha >= kMinAlpha;
+          const bool stop = ok && next_T < kMinTransmittance;
+          const bool take = ok && !stop;
+          const float w = take ? e.alpha * T : 0.0f;
+          cr = fmaf(sb.z, w, cr);
+          cg = fmaf(sb.w, w, cg);
+          cb = fmaf(blue, w, cb);
+          T = take ? next_T : T;
+          last = take ? base + uint32_t(j) : last;
+          done = done || stop;
+          if (__ballot(!done) == 0ull) {
+            live = false;
+            break;
           }
         }
-        if (__ballot(!done) == 0ull) {
-          wave_live = false;
-          break;
-        }
+        if (!live) break;
       }
     }
     __syncthreads();
@@ -161,15 +179,15 @@ __global__ void __launch_bounds__(kThreads) k_forward(gs_hip_frame f, RasterArgs
 }
 
 __global__ void __launch_bounds__(kThreads) k_backward(gs_hip_frame f, RasterArgs a) {
-  __shared__ float4 s_a[kBatch], s_b[kBatch], s_c[kBatch];
+  __shared__ Staging s;
   __shared__ uint32_t s_g[kBatch];
-  __shared__ float s_grad[GS_HIP_GRAD2D_FLOATS][kBatch];
-  __shared__ uint64_t s_mask[4][kBatch / 64];
+  __shared__ float s_grad[kBatch * kGrad];  // [splat][9]: the nine lanes of one reduction hit nine banks
   __shared__ uint32_t s_wave_last[4];
   const PixelCoord pc = pixel_coord(f, a.tile_order[blockIdx.x]);
   const uint2 range = a.ranges[pc.tile];
   const float cx = float(pc.px) + 0.5f, cy = float(pc.py) + 0.5f;
-  const int tx0 = (pc.tile % f.tiles_x) * kTile, ty0 = (pc.tile / f.tiles_x) * kTile;
+  const Reduce9 red;
+  const int slot = red.slot();
 
   float T_final = 0.0f, gr = 0.0f, gg = 0.0f, gb = 0.0f;
   uint32_t last = 0u;
@@ -180,7 +198,8 @@ __global__ void __launch_bounds__(kThreads) k_backward(gs_hip_frame f, RasterArg
     const float4 d = a.dldc[pix];
     gr = d.x, gg = d.y, gb = d.z;
   }
-  const float bg_dot = f.background[0] * gr + f.background[1] * gg + f.background[2] * gb;
+  // T_final/(1 - alpha) times bg . dL/dC: the background seen through the splat.
+  const float bg_term = T_final * (f.background[0] * gr + f.background[1] * gg + f.background[2] * gb);
   // Back-to-front state: T in front of the current splat, colour behind it.
   float T = T_final, acc_r = 0.0f, acc_g = 0.0f, acc_b = 0.0f;
   float last_alpha = 0.0f, last_r = 0.0f, last_g = 0.0f, last_b = 0.0f;
@@ -194,97 +213,70 @@ __global__ void __launch_bounds__(kThreads) k_backward(gs_hip_frame f, RasterArg
   for (uint32_t bend = end; bend > range.x;) {
     const uint32_t bstart = bend > range.x + kBatch ? bend - kBatch : range.x;
     const uint32_t n = bend - bstart;
-    bool hit[4] = {false, false, false, false};
+    uint32_t hits = 0u;
     if (threadIdx.x < n) {
       const uint32_t k = bstart + threadIdx.x;
       const uint32_t g = a.pair_gaussians[k];
-      const float4* r = a.records + size_t(g) * 3u;
-      const float4 ra = r[0], rb = r[1], rc = r[2];
-      s_a[threadIdx.x] = ra;
-      s_b[threadIdx.x] = rb;
-      s_c[threadIdx.x] = rc;
       s_g[threadIdx.x] = g;
-      for (int c = 0; c < GS_HIP_GRAD2D_FLOATS; ++c) s_grad[c][threadIdx.x] = 0.0f;
-      const uint32_t lo = __float_as_uint(rc.y), hi = __float_as_uint(rc.z);
+      for (int c = 0; c < kGrad; ++c) s_grad[threadIdx.x * kGrad + c] = 0.0f;
+      hits = stage(s, a, g, threadIdx.x, pc.tx0, pc.ty0);
       const uint32_t pos = k - range.x;
-      for (int q = 0; q < 4; ++q) {
-        const int qx = tx0 + (q & 1) * 8, qy = ty0 + (q >> 1) * 8;
-        hit[q] = pos < s_wave_last[q] && bbox_hits(lo, hi, qx, qy, qx + 7, qy + 7);
-      }
+      for (int q = 0; q < 4; ++q)
+        if (pos >= s_wave_last[q]) hits &= ~(1u << q);
     }
-    for (int q = 0; q < 4; ++q) {
-      const uint64_t m = __ballot(hit[q]);
-      if (pc.lane == 0) s_mask[q][pc.wave] = m;
-    }
+    publish_masks(s, hits, pc.wave, pc.lane);
     __syncthreads();
+    const uint32_t base = bstart - range.x;  // position of splat j is base + j
     for (int chunk = kBatch / 64 - 1; chunk >= 0; --chunk) {
-      uint64_t m = s_mask[pc.wave][chunk];
+      uint64_t m = uniform_u64(s.mask[pc.wave][chunk]);
       while (m != 0ull) {
         const int bit = 63 - __builtin_clzll(m);
-        m &= ~(1ull << bit);
+        m ^= 1ull << bit;
         const int j = chunk * 64 + bit;
-        const uint32_t pos = bstart + uint32_t(j) - range.x;
-        float g0 = 0.0f, g1 = 0.0f, g2 = 0.0f, g3 = 0.0f, g4 = 0.0f, g5 = 0.0f, g6 = 0.0f, g7 = 0.0f, g8 = 0.0f;
-        bool contributes = false;
-        if (pos < last) {
-          const float4 sa = s_a[j];
-          const float4 sb = s_b[j];
-          const SplatEval e = eval_splat(sa, sb, cx, cy);
-          const float dx = e.dx, dy = e.dy, G = e.G, raw = e.raw, alpha = e.alpha;
-          if (e.power <= 0.0f && alpha >= kMinAlpha) {
-            contributes = true;
-            const float4 sc = s_c[j];
-            const float one_minus = 1.0f - alpha;
-            T = T / one_minus;
-            acc_r = last_alpha * last_r + (1.0f - last_alpha) * acc_r;
-            acc_g = last_alpha * last_g + (1.0f - last_alpha) * acc_g;
-            acc_b = last_alpha * last_b + (1.0f - last_alpha) * acc_b;
-            last_r = sb.z, last_g = sb.w, last_b = sc.x;
-            float dL_dalpha = ((sb.z - acc_r) * gr + (sb.w - acc_g) * gg + (sc.x - acc_b) * gb) * T;
-            last_alpha = alpha;
-            dL_dalpha += (-T_final / one_minus) * bg_dot;
-            const float wT = alpha * T;
-            g6 = wT * gr, g7 = wT * gg, g8 = wT * gb;
-            if (raw < kMaxAlpha) {
-              g5 = G * dL_dalpha;
-              const float dpow = raw * dL_dalpha;
-              g0 = dpow * -(sa.z * dx + sa.w * dy);
-              g1 = dpow * -(sb.x * dy + sa.w * dx);
-              g2 = dpow * (-0.5f * dx * dx);
-              g3 = dpow * (-dx * dy);
-              g4 = dpow * (-0.5f * dy * dy);
-            }
-          }
-        }
-        if (__ballot(contributes) != 0ull) {
-          g0 = wave_sum(g0);
-          g1 = wave_sum(g1);
-          g2 = wave_sum(g2);
-          g3 = wave_sum(g3);
-          g4 = wave_sum(g4);
-          g5 = wave_sum(g5);
-          g6 = wave_sum(g6);
-          g7 = wave_sum(g7);
-          g8 = wave_sum(g8);
-          if (pc.lane == 0) {
-            atomicAdd(&s_grad[0][j], g0);
-            atomicAdd(&s_grad[1][j], g1);
-            atomicAdd(&s_grad[2][j], g2);
-            atomicAdd(&s_grad[3][j], g3);
-            atomicAdd(&s_grad[4][j], g4);
-            atomicAdd(&s_grad[5][j], g5);
-            atomicAdd(&s_grad[6][j], g6);
-            atomicAdd(&s_grad[7][j], g7);
-            atomicAdd(&s_grad[8][j], g8);
-          }
-        }
+        const float4 sa = s.a[j];
+        const float4 sb = s.b[j];
+        const float blue = s.blue[j];
+        const SplatEval e = eval_splat(sa, sb, cx, cy);
+        const float alpha = e.alpha;
+        const bool contributes = base + uint32_t(j) < last && e.power <= 0.0f && alpha >= kMinAlpha;
+        if (__ballot(contributes) == 0ull) continue;
+        // Branch-free from here: lanes that do not contribute keep their
+        // state and add zeros (selects, so no 0 * inf can leak in).
+        const float inv = __builtin_amdgcn_rcpf(1.0f - alpha);
+        const float T_in = T * inv;  // transmittance in front of this splat
+        const float ar = fmaf(last_alpha, last_r - acc_r, acc_r);
+        const float ag = fmaf(last_alpha, last_g - acc_g, acc_g);
+        const float ab = fmaf(last_alpha, last_b - acc_b, acc_b);
+        const float dot = (sb.z - ar) * gr + (sb.w - ag) * gg + (blue - ab) * gb;
+        const float dL_dalpha = dot * T_in - bg_term * inv;
+        const bool unclamped = contributes && e.raw < kMaxAlpha;
+        const float wT = contributes ? alpha * T_in : 0.0f;
+        const float dpow = unclamped ? e.raw * dL_dalpha : 0.0f;
+        const float g5 = unclamped ? e.G * dL_dalpha : 0.0f;
+        const float dxp = dpow * e.dx;
+        const float g0 = dpow * (fmaf(sa.w, e.dy, e.adx) + e.adx);  // dpow (2A dx + B dy)
+        const float g1 = dpow * fmaf(sa.w, e.dx, 2.0f * e.cdy);     // dpow (B dx + 2C dy)
+        const float g2 = -0.5f * dxp * e.dx;
+        const float g3 = -dxp * e.dy;
+        const float g4 = -0.5f * dpow * e.dy * e.dy;
+        const float g6 = wT * gr, g7 = wT * gg, g8 = wT * gb;
+        T = contributes ? T_in : T;
+        acc_r = contributes ? ar : acc_r;
+        acc_g = contributes ? ag : acc_g;
+        acc_b = contributes ? ab : acc_b;
+        last_r = contributes ? sb.z : last_r;
+        last_g = contributes ? sb.w : last_g;
+        last_b = contributes ? blue : last_b;
+        last_alpha = contributes ? alpha : last_alpha;
+        const float z = red.reduce(g0, g1, g2, g3, g4, g5, g6, g7, g8);
+        if (slot >= 0) atomicAdd(&s_grad[j * kGrad + slot], z);
       }
     }
     __syncthreads();
     if (threadIdx.x < n) {
       const uint32_t g = s_g[threadIdx.x];
-      for (int c = 0; c < GS_HIP_GRAD2D_FLOATS; ++c) {
-        const float v = s_grad[c][threadIdx.x];
+      for (int c = 0; c < kGrad; ++c) {
+        const float v = s_grad[threadIdx.x * kGrad + c];
         if (v != 0.0f) unsafeAtomicAdd(&a.grad2d[size_t(c) * a.capacity + g], v);
       }
     }

@@ -415,10 +415,22 @@ uint64_t orc_bin32(const float* records, const float* footprint, const uint32_t*
 }
 
 /* ===================== fp32 tile compositor (train_forward.comp:77-117) ===== */
+/* A splat evaluation whose skip / clamp / stop decision sits within 1e-5
+ * relative of its threshold (power > 0, alpha < 1/255, raw alpha >= 0.99):
+ * another fp32 evaluation order or exp (the GPU's, the Vulkan driver's) may
+ * decide it the other way, so a comparison at the bit level is meaningless
+ * there.  Pixels that evaluate such a splat are reported as ambiguous. */
+static int near_threshold(float power, float raw) {
+  const float a = 1.0f / 255.0f;
+  return power > -1e-6f || fabsf(raw - a) <= 1e-5f * a || fabsf(raw - 0.99f) <= 1e-5f * 0.99f;
+}
+
 /* image [H*W][4] = (C rgb, final T); last [H*W] = 1-based position of the
- * last composited splat in its tile's range. */
+ * last composited splat in its tile's range; ambiguous [H*W] (may be null):
+ * 1 where the pixel met a splat at a decision threshold (near_threshold) or
+ * stopped within 1e-5 relative of T = 1e-4. */
 void orc_forward32(const float* records, const uint32_t* pair_gaussians, const uint32_t* ranges, uint32_t width,
-                   uint32_t height, float* image, uint32_t* last) {
+                   uint32_t height, float* image, uint32_t* last, uint8_t* ambiguous) {
   const uint32_t tiles_x = (width + TILE - 1) / TILE;
   for (uint32_t py = 0; py < height; ++py) {
     for (uint32_t px = 0; px < width; ++px) {
@@ -427,14 +439,18 @@ void orc_forward32(const float* records, const uint32_t* pair_gaussians, const u
       const float cx = (float)px + 0.5f, cy = (float)py + 0.5f;
       float T = 1.0f, C[3] = {0.0f, 0.0f, 0.0f};
       uint32_t lst = 0;
+      int amb = 0;
       for (uint32_t k = start; k < end; ++k) {
         const float* r = records + (size_t)pair_gaussians[k] * 12;
         const float dx = r[0] - cx, dy = r[1] - cy;
         const float power = -0.5f * (r[2] * dx * dx + r[4] * dy * dy) - r[3] * dx * dy;
+        const float raw = r[5] * expf(power);
+        amb |= near_threshold(power, raw);
         if (power > 0.0f) continue;
-        const float alpha = fminf(0.99f, r[5] * expf(power));
+        const float alpha = fminf(0.99f, raw);
         if (alpha < 1.0f / 255.0f) continue;
         const float next_T = T * (1.0f - alpha);
+        amb |= fabsf(next_T - 1e-4f) <= 1e-5f * 1e-4f;
         if (next_T < 1e-4f) break;
         for (int c = 0; c < 3; ++c) C[c] += r[6 + c] * alpha * T;
         T = next_T;
@@ -446,6 +462,7 @@ void orc_forward32(const float* records, const uint32_t* pair_gaussians, const u
       image[pix * 4 + 2] = C[2];
       image[pix * 4 + 3] = T;
       last[pix] = lst;
+      if (ambiguous) ambiguous[pix] = (uint8_t)amb;
     }
   }
 }
@@ -454,10 +471,13 @@ void orc_forward32(const float* records, const uint32_t* pair_gaussians, const u
 /* Per pixel, back to front over its tile's list from its last contributor,
  * with the forward's final T (image[..][3]) and dL/dC (dldc [H*W][4]); the
  * per-pair terms of splat_pixel_grad (train_common.glsl:228-253) summed per
- * gaussian in double into grad2d [n][9]. */
+ * gaussian in double into grad2d [n][9].  tainted [n] (may be null): 1 for
+ * every splat met at a decision threshold (near_threshold) and for every
+ * gaussian that contributes at a pixel whose walk meets one, where another
+ * evaluation may skip or clamp differently and so shift that pixel's terms. */
 void orc_backward32(const float* records, const uint32_t* pair_gaussians, const uint32_t* ranges, const float* image,
                     const uint32_t* last, const float* dldc, const float bg[3], uint32_t n, uint32_t width,
-                    uint32_t height, double* grad2d) {
+                    uint32_t height, double* grad2d, uint8_t* tainted) {
   const uint32_t tiles_x = (width + TILE - 1) / TILE;
   memset(grad2d, 0, sizeof(double) * n * 9);
   for (uint32_t py = 0; py < height; ++py) {
@@ -470,6 +490,18 @@ void orc_backward32(const float* records, const uint32_t* pair_gaussians, const 
       const float* dC = dldc + pix * 4;
       const float bg_dot = bg[0] * dC[0] + bg[1] * dC[1] + bg[2] * dC[2];
       float T = T_final, acc[3] = {0, 0, 0}, last_alpha = 0.0f, last_color[3] = {0, 0, 0};
+      int amb = 0;
+      if (tainted) {
+        for (uint32_t k = start + last[pix]; k-- > start;) {
+          const float* r = records + (size_t)pair_gaussians[k] * 12;
+          const float dx = r[0] - cx, dy = r[1] - cy;
+          const float power = -0.5f * (r[2] * dx * dx + r[4] * dy * dy) - r[3] * dx * dy;
+          if (near_threshold(power, r[5] * expf(power))) {
+            amb = 1;
+            tainted[pair_gaussians[k]] = 1; /* in or out of the sum depending on the evaluation */
+          }
+        }
+      }
       for (uint32_t k = start + last[pix]; k-- > start;) {
         const uint32_t i = pair_gaussians[k];
         const float* r = records + (size_t)i * 12;
@@ -490,6 +522,7 @@ void orc_backward32(const float* records, const uint32_t* pair_gaussians, const 
         last_alpha = alpha;
         dL_dalpha += (-T_final / (1.0f - alpha)) * bg_dot;
         double* o = grad2d + (size_t)i * 9;
+        if (amb) tainted[i] = 1;
         for (int c = 0; c < 3; ++c) o[6 + c] += (double)(alpha * T * dC[c]);
         if (raw < 0.99f) {
           const float dpow = raw * dL_dalpha;

@@ -46,7 +46,7 @@ def library() -> ctypes.CDLL:
             "orc_footprint_tiles32": ([f32, f32, f32, f32, f32, f32, u32, u32, P(u32), u32], u32),
             "orc_project32": ([P(f32), P(f32), u32, u32, cam_p, u32, i32, P(f32), P(f32), P(u32), P(u32)], None),
             "orc_bin32": ([P(f32), P(f32), P(u32), P(u32), u32, u32, u32, P(u32), P(u32), u64, P(u32)], u64),
-            "orc_forward32": ([P(f32), P(u32), P(u32), u32, u32, P(f32), P(u32)], None),
+            "orc_forward32": ([P(f32), P(u32), P(u32), u32, u32, P(f32), P(u32), P(ctypes.c_uint8)], None),
             "orc_project64": ([P(f64), P(f64), u32, u32, cam_p, i32, P(f64)], None),
             "orc_render64": ([P(f64), P(f64), u32, u32, cam_p, i32, P(f64), P(f64), P(f64)], None),
             "orc_ssim64": ([P(f64), P(f64), u32, u32, f64, f64, P(f64)], f64),
@@ -54,8 +54,8 @@ def library() -> ctypes.CDLL:
             "orc_loss64": ([P(f64), P(f64), u32, u32, cam_p, i32, P(f64), P(ctypes.c_uint8), i32, f64], f64),
             "orc_backward64": ([P(f64), P(f64), u32, u32, cam_p, i32, P(f64), P(ctypes.c_uint8), i32, f64, P(f64),
                                 P(f64), P(f64)], f64),
-            "orc_backward32": ([P(f32), P(u32), P(u32), P(f32), P(u32), P(f32), P(f32), u32, u32, u32, P(f64)],
-                               None),
+            "orc_backward32": ([P(f32), P(u32), P(u32), P(f32), P(u32), P(f32), P(f32), u32, u32, u32, P(f64),
+                                P(ctypes.c_uint8)], None),
             "orc_adjoint64": ([P(f64), P(f64), u32, u32, cam_p, i32, P(f64), P(f64), P(f64)], None),
             "orc_adam": ([P(f32), P(f32), P(f32), P(f32), P(f32), u64, f32, f32, f32, f32, f32], None),
             "orc_densify_mark": ([P(f32), P(f32), u32, f32, f32, f32, f32, f32, P(u32), P(u32), P(u32)], None),
@@ -138,15 +138,20 @@ def bin32(records, foot, tiles, keys, width: int, height: int):
     return pt[:total], pg[:total], ranges
 
 
-def forward32(records, pair_gaussians, ranges, width: int, height: int):
+def forward32(records, pair_gaussians, ranges, width: int, height: int, flags: bool = False):
+    """(image [HW,4], last [HW]) of the fp32 tile compositor; with flags=True also
+    ambiguous [HW] (bool): pixels that met a splat at a skip/clamp/stop threshold."""
     image = np.zeros((height * width, 4), np.float32)
     last = np.zeros(height * width, np.uint32)
+    amb = np.zeros(height * width, np.uint8)
     pg = np.ascontiguousarray(pair_gaussians, np.uint32)
     if pg.size == 0:
         pg = np.zeros(1, np.uint32)
     library().orc_forward32(_p(np.ascontiguousarray(records, np.float32), ctypes.c_float), _p(pg, ctypes.c_uint32),
                             _p(np.ascontiguousarray(ranges, np.uint32), ctypes.c_uint32), width, height,
-                            _p(image, ctypes.c_float), _p(last, ctypes.c_uint32))
+                            _p(image, ctypes.c_float), _p(last, ctypes.c_uint32), _p(amb, ctypes.c_uint8))
+    if flags:
+        return image, last, amb.astype(bool)
     return image, last
 
 
@@ -241,21 +246,28 @@ def densify_mark(params, stats, grad_threshold, clone_max_scale, prune_opacity, 
     return action, release, prune
 
 
-def backward32(records, pair_gaussians, ranges, image, last, dldc, bg, width: int, height: int) -> np.ndarray:
-    """grad2d [n, 9] (double sums) of the tile backward on the given forward state."""
+def backward32(records, pair_gaussians, ranges, image, last, dldc, bg, width: int, height: int,
+               flags: bool = False):
+    """grad2d [n, 9] (double sums) of the tile backward on the given forward state;
+    with flags=True also tainted [n] (bool): gaussians that contribute at a pixel
+    whose walk meets a splat at a skip/clamp threshold."""
     records = np.ascontiguousarray(records, np.float32).reshape(-1, 12)
     n = records.shape[0]
     pg = np.ascontiguousarray(pair_gaussians, np.uint32)
     if pg.size == 0:
         pg = np.zeros(1, np.uint32)
     out = np.zeros((n, 9), np.float64)
+    tainted = np.zeros(max(n, 1), np.uint8)
     bgv = np.asarray(bg, np.float32)
     library().orc_backward32(_p(records, ctypes.c_float), _p(pg, ctypes.c_uint32),
                              _p(np.ascontiguousarray(ranges, np.uint32), ctypes.c_uint32),
                              _p(np.ascontiguousarray(image, np.float32), ctypes.c_float),
                              _p(np.ascontiguousarray(last, np.uint32), ctypes.c_uint32),
                              _p(np.ascontiguousarray(dldc, np.float32), ctypes.c_float), _p(bgv, ctypes.c_float),
-                             n, width, height, _p(out, ctypes.c_double))
+                             n, width, height, _p(out, ctypes.c_double),
+                             _p(tainted, ctypes.c_uint8) if flags else None)
+    if flags:
+        return out, tainted[:n].astype(bool)
     return out
 
 

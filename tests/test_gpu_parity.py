@@ -158,10 +158,20 @@ def test_forward_loss_backward_match_oracle(name, n, deg, w, h, seed):
         ws.backward(frame)
         g2 = ws.download(gs.BUF_GRAD2D, np.float32, 9 * n).reshape(9, n).T
     # forward: the same records and pairs through the oracle's compositor
-    o_img, o_last = orc.forward32(rec, pg, ranges, w, h)
+    # Pixels that meet a splat at a skip/clamp/stop threshold (within 1e-5
+    # relative) may take the other branch under another fp32 evaluation
+    # order: they are counted and bounded by one splat's share, the rest must
+    # agree to 2e-5.
+    o_img, o_last, amb = orc.forward32(rec, pg, ranges, w, h, flags=True)
     same = last == o_last
     assert same.mean() > 0.999, f"last contributor differs on {(~same).sum()} pixels"
-    assert np.abs(img[same] - o_img[same]).max() < 2e-5
+    assert amb.mean() < 0.01, f"{amb.sum()} threshold-ambiguous pixels"
+    perr = np.abs(img - o_img).max(axis=1)
+    clear = same & ~amb
+    print(f"[info] {name}: {amb.sum()} ambiguous pixels, {(~same).sum()} with another last contributor; "
+          f"max error clear {perr[clear].max():.3g}, ambiguous {perr[same & amb].max(initial=0):.3g}", flush=True)
+    assert perr[clear].max() < 2e-5
+    assert perr[same].max() <= 2.0 / 255.0 * max(1.0, float(np.abs(rec[:, 6:9]).max()))
     # loss: sums and dL/dC against the fp64 loss of the GPU's own render
     x = img[:, :3].astype(np.float64) + img[:, 3:4].astype(np.float64) * np.asarray(bg)
     y = target.reshape(-1, 4)[:, :3] / 255.0
@@ -177,11 +187,19 @@ def test_forward_loss_backward_match_oracle(name, n, deg, w, h, seed):
     dd = np.abs(dldc[:, :3] - d_ref)
     assert dd[sign_ok].max() <= 1e-4 * np.abs(d_ref).max()
     # backward: float-atomic sums against the oracle's tile backward on the same state
-    o_g2 = orc.backward32(rec, pg, ranges, img, last, dldc, bg, w, h)
+    # Gaussians contributing at a pixel whose walk meets a threshold-ambiguous
+    # splat are held to a looser bound; every other one to 1e-4 of the largest.
+    o_g2, tainted = orc.backward32(rec, pg, ranges, img, last, dldc, bg, w, h, flags=True)
+    # (tainted = the gaussians drawn at the few ambiguous pixels, so a sanity
+    # bound only: the strict check must still cover most of them)
+    visible = tiles > 0
+    assert tainted.sum() <= 0.25 * max(1, visible.sum()), f"{tainted.sum()} tainted of {visible.sum()}"
     for c in range(9):
         scale = np.abs(o_g2[:, c]).max()
-        err = np.abs(g2[:, c] - o_g2[:, c]).max()
-        assert err <= 1e-4 * scale, f"grad2d[{c}]: err {err:.3g} vs max {scale:.3g}"
+        err = np.abs(g2[:, c] - o_g2[:, c])
+        assert err[~tainted].max() <= 1e-4 * scale, f"grad2d[{c}]: err {err[~tainted].max():.3g} vs max {scale:.3g}"
+        assert err.max() <= 5e-2 * scale, f"grad2d[{c}] (tainted): err {err.max():.3g} vs max {scale:.3g}"
+    print(f"[info] {name}: {tainted.sum()} of {visible.sum()} visible gaussians tainted", flush=True)
     print(f"[ok] {name}: pairs {pairs}, loss {L_gpu:.6f}", flush=True)
 
 

@@ -208,6 +208,26 @@ def test_forward32_matches_brute_force_render64():
     assert err.max() < 2.5 / 255
 
 
+def test_threshold_flags_mark_only_decisions_at_the_edge():
+    """The ambiguity flags of forward32/backward32 (used to bound comparisons
+    with other fp32 evaluation orders) fire exactly where a splat's alpha
+    sits on the 1/255 cut, and nowhere else."""
+    w = h = 32
+    d2 = 2 * 15.75 ** 2  # pixel (0, 0) seen from a splat at (16.25, 16.25)
+    o = np.float32((1.0 / 255.0) / math.exp(-0.5 * 0.01 * d2))
+    rec = np.zeros((2, 12), np.float32)
+    rec[0, :9] = [16.25, 16.25, 0.01, 0.0, 0.01, o, 0.9, 0.5, 0.1]
+    rec[1, :9] = [28.25, 28.25, 0.5, 0.0, 0.5, 0.8, 0.2, 0.3, 0.4]
+    pg = np.array([0, 0, 0, 0, 1], np.uint32)  # tiles 0..2: splat 0; tile 3: splats 0, 1
+    ranges = np.array([0, 1, 1, 2, 2, 3, 3, 5, 0, 0], np.uint32)
+    img, last, amb = orc.forward32(rec, pg, ranges, w, h, flags=True)
+    assert amb.sum() == 1 and amb[0]
+    dldc = np.ones((w * h, 4), np.float32)
+    g, tainted = orc.backward32(rec, pg, ranges, img, last, dldc, (0.0, 0.0, 0.0), w, h, flags=True)
+    assert tainted.tolist() == [True, False]
+    assert np.abs(g[1]).max() > 0
+
+
 def _ssim_numpy(x, y, w, h):
     k = np.exp(-np.arange(-5, 6) ** 2 / (2 * 1.5 ** 2))
     k /= k.sum()
