@@ -6,9 +6,10 @@ with densification every 100 view-iterations; targets rendered from a seeded
 1M-gaussian ground-truth scene at 64 seeded jittered cameras (radius 2.5-3.5,
 height -0.5-1.5), training from an independently seeded 1M-gaussian scene.
 Measured twice on the same views: plain (one view per optimizer step) and
-batched (8 views per optimizer step, each view on its own stream, one Adam on
-the mean gradient).  Synthetic data, inputs resident in HBM before the timed
-region.
+batched (8 views per optimizer step: every stage of all 8 views in one launch,
+one Adam on the mean gradient).  Each mode's launch shapes are picked first by
+the library's autotuner on that mode's first views.  Synthetic data, inputs
+resident in HBM before the timed region.
 
 Prints one JSON line: the metric, per-kernel HIP-event times over the timed
 region, the roofline figure of the dominant kernel and of the whole step, and
@@ -46,32 +47,33 @@ HBM_PEAK = 8.0e12
 SH_COEFFS = {0: 0, 1: 3, 2: 8, 3: 15}
 
 
-def kernel_bytes(name, n_slots, n_visible, pairs, pixels, coeffs):
-    """Algorithmic HBM bytes of one launch of each kernel family (fp32), as
-    counted in the design document's roofline table."""
+def kernel_bytes(name, n_slots, n_visible, pairs, pixels, coeffs, views):
+    """Algorithmic HBM bytes of one launch of each kernel family (fp32) over
+    `views` views, as counted in the design document's roofline table;
+    `pairs` and `n_visible` are per view."""
     sh = 12 * coeffs
-    if name == "project":
-        return n_slots * (64 + 112) + n_visible * sh
+    if name == "project":  # parameters + SH once, per-view record/footprint/tiles/misc/key/order/grad2d
+        return n_slots * 64 + n_visible * sh + views * n_slots * 116
     if name == "forward":
-        return pairs * 52 + pixels * 20
+        return views * (pairs * 52 + pixels * 20)
     if name == "backward":
-        return pairs * 52 + pixels * 20 + n_visible * 36
+        return views * (pairs * 52 + pixels * 20 + n_visible * 36)
     if name == "ssim":
-        return pixels * (16 + 4 + 36)
+        return views * pixels * (16 + 4 + 36)
     if name == "loss_grad":
-        return pixels * (36 + 16 + 4 + 12)
-    if name == "adam":
-        return n_slots * (3 * 64 * 2 + 3 * sh * 2 + 36 + 4 + 32)
+        return views * pixels * (36 + 16 + 4 + 12)
+    if name == "adam":  # state r/w once; per view: grad2d, tiles, misc, SH input w + r
+        return n_slots * (3 * 64 * 2 + 3 * sh * 2 + sh + 32) + views * n_slots * (36 + 4 + 4 + 64)
     if name == "emit":
-        return n_visible * (48 + 16 + 8) + pairs * 8
+        return views * (n_visible * (48 + 16 + 8) + pairs * 8)
     if name in ("tile_sort",):
-        return pairs * 8 * 2
+        return views * pairs * 8 * 2
     if name == "depth_sort":
-        return n_slots * 8 * 2
+        return views * n_slots * 8 * 2
     if name == "scan":
-        return n_slots * 12
+        return views * n_slots * 12
     if name == "ranges":
-        return pairs * 8
+        return views * pairs * 8
     return None
 
 
@@ -153,7 +155,7 @@ def kernel_table(events, steps, slots_mean, n_visible, mean_pairs_per_view, per_
         d[1] += 1
     kernels = {k: {"ms_avg": v[0] / v[1], "launches": v[1], "ms_per_step": v[0] / steps} for k, v in per_kernel.items()}
     for k, v in kernels.items():
-        b = kernel_bytes(k, slots_mean, n_visible, mean_pairs_per_view, pixels, coeffs)
+        b = kernel_bytes(k, slots_mean, n_visible, mean_pairs_per_view, pixels, coeffs, per_step)
         if b is not None:
             v["algorithmic_bytes"] = int(b)
             v["GBps"] = b / (v["ms_avg"] * 1e-3) / 1e9
@@ -174,6 +176,7 @@ def main():
     ap.add_argument("--densify-every", type=int, default=100, help="view-iterations between densifications")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="default launch shapes instead of the autotuner's")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build (multi-GPU "
@@ -212,9 +215,27 @@ def main():
         ws = gs.Workspace(2 * N, deg, W, H)
         ws.set_views(per_step)
         ws.load(params, sh)
+        tuning, tune_ms = None, None
+        if not args.no_autotune:
+            vs = [int(order[k % len(order)]) for k in range(per_step)]
+            frames = [gs.make_frame(cams[v], N, deg, step=0, tp=tp) for v in vs]
+            best, tune_ms = ws.autotune(frames, [target_bufs[v].ptr for v in vs], repeats=3)
+            tuning = best.as_dict()
+            log(f"{mode}: autotuned launch shape {tuning} ({tune_ms:.3f} ms per step while tuning)")
+        graph = bool(tuning and tuning["graph"])
         state = {"slots": N, "densified": 0, "pairs": 0, "views": 0, "step": 0}
+        # Per-kernel HIP events need eager launches: with graph replay chosen,
+        # the timed run replays the graph and a short eager pass after it
+        # times the kernels.
         elapsed, steps = run(gs, ws, cams, target_bufs, order, per_step, views_total, warmup_views, deg, tp,
-                             args.densify_every, state, timed=True)
+                             args.densify_every, state, timed=not graph)
+        if graph:
+            probe = dict(state)
+            probe_views = max(per_step, (views_total // 5 // per_step) * per_step)
+            run(gs, ws, cams, target_bufs, order, per_step, probe_views, 0, deg, tp, 0, probe, timed=True)
+            kernel_steps = probe_views // per_step
+        else:
+            kernel_steps = steps
         events = ws.timings(enable=False)
         views_per_s = steps * per_step / elapsed
         mean_pairs = state["pairs"] / (steps * per_step)
@@ -225,7 +246,9 @@ def main():
             "views_per_s": views_per_s, "optimizer_steps_per_s": steps / elapsed, "ms_per_step": 1000 * elapsed / steps,
             "mean_pairs": mean_pairs, "slots_start": state["slots_start"], "slots_end": state["slots"],
             "densify_events": state["densified"], "slots_mean": slots_mean, "n_visible": n_visible,
-            "kernels": kernel_table(events, steps, slots_mean, n_visible, mean_pairs, per_step, W * H, coeffs),
+            "kernels": kernel_table(events, kernel_steps, slots_mean, n_visible, mean_pairs, per_step, W * H, coeffs),
+            "tuning": tuning, "tune_ms": tune_ms, "overflow_steps": ws.overflows(),
+            "kernel_timing": "eager pass after the timed run" if graph else "the timed run",
         }
         log(f"{mode}: {views_per_s:.1f} view-iterations/s ({1000 * elapsed / steps:.3f} ms per optimizer step)")
         ws.close()
@@ -273,16 +296,18 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic (seeded 1M-gaussian GT scene; 64 jittered views rendered by this trainer)",
-        "config": {"workload": f"batched training: {args.batch} views per optimizer step (each view: project, bin, "
-                               "forward, SSIM loss, recompute backward on its own stream; one adjoint + Adam on the "
-                               "mean gradient), 1M gaussians SH3 fp32, 1280x720, 16x16 tiles, 0.8 L1 + 0.2 D-SSIM, "
-                               "densify every 100 view-iterations",
+        "config": {"workload": f"batched training: {args.batch} views per optimizer step (project, bin, forward, "
+                               "SSIM loss and recompute backward of all views, one launch per stage; one adjoint + "
+                               "Adam on the mean gradient), 1M gaussians SH3 fp32, 1280x720, 16x16 tiles, "
+                               "0.8 L1 + 0.2 D-SSIM, densify every 100 view-iterations",
                    "views_per_step": args.batch, "gaussians_start": N, "gaussians_end": r["slots_end"],
                    "width": W, "height": H, "sh_degree": deg, "mean_pairs_per_view": int(r["mean_pairs"]),
-                   "densify_events": r["densify_events"], "device": device},
+                   "densify_events": r["densify_events"], "device": device, "tuning": r["tuning"],
+                   "kernel_timing": r["kernel_timing"], "overflow_steps": r["overflow_steps"]},
         "plain": None if plain is None else {
             "iters_per_s": round(plain["views_per_s"], 3), "ms_per_step": round(plain["ms_per_step"], 4),
             "mean_pairs": int(plain["mean_pairs"]), "densify_events": plain["densify_events"],
+            "tuning": plain["tuning"],
             "kernels": {k: round(v["ms_avg"], 4) for k, v in plain["kernels"].items()}},
         "speedup_batched_vs_plain": None if plain is None else round(r["views_per_s"] / plain["views_per_s"], 3),
         "optimizer_steps_per_s": round(r["optimizer_steps_per_s"], 3),

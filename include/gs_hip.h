@@ -48,6 +48,7 @@ extern "C" {
 #define GS_HIP_FLAG_SPARSE_ADAM 2u    /* gaussians that drew nothing keep value and moments (train_step.glsl:223-230) */
 #define GS_HIP_FLAG_DEPTH16 4u        /* order pairs by the reference's 16-bit log depth (train_project.comp:110)
                                          instead of the float bits of z */
+#define GS_HIP_FLAG_NO_UPDATE 8u      /* run the step but leave parameters, moments and statistics alone */
 
 /* Pinhole camera, scene -> camera, OpenCV axes (x right, y down, z forward).
  * Same fields as gs::Camera (engine/include/gs/camera.h:10-16). */
@@ -150,9 +151,9 @@ int gs_hip_workspace_create(gs_hip_workspace** ws, uint32_t capacity, int sh_deg
                             uint32_t height);
 int gs_hip_workspace_destroy(gs_hip_workspace* ws);
 int gs_hip_workspace_buffer(gs_hip_workspace* ws, int which, void** device_ptr, size_t* bytes);
-/* Views: independent sets of per-view buffers, each with its own HIP stream,
- * over the workspace's one model (view 0 exists from creation).  Grows to
- * `count` (<= GS_HIP_MAX_VIEWS) views. */
+/* Views: independent sets of per-view buffers over the workspace's one model
+ * (view 0 exists from creation); a batched step runs every stage of all its
+ * views in one launch.  Grows to `count` (<= GS_HIP_MAX_VIEWS) views. */
 int gs_hip_workspace_views(gs_hip_workspace* ws, uint32_t count);
 /* A buffer of view `view` (model buffers are the same for every view). */
 int gs_hip_workspace_view_buffer(gs_hip_workspace* ws, uint32_t view, int which, void** device_ptr, size_t* bytes);
@@ -199,9 +200,11 @@ int gs_hip_reset_opacity(gs_hip_workspace* ws, const gs_hip_frame* frame, float 
 /* One full training view-step: project, bin, forward, loss, backward, adam. */
 int gs_hip_train_step(gs_hip_workspace* ws, const gs_hip_frame* frame, uint64_t* pairs, double losses[3]);
 /* One batched optimizer step over `count` views of the scene (count <= the
- * workspace's views): each view's project, bin, forward, loss and backward
- * run on its own stream, concurrently; then one projection adjoint + Adam on
- * the mean of the views' gradients.  frames[v] differ only in camera (the
+ * workspace's views): project, bin, forward, loss and backward of all views,
+ * one launch per stage; then one projection adjoint + Adam on the mean of the
+ * views' gradients.  Pair counts stay on the device; a step whose pairs
+ * overflow the buffers leaves the model alone and runs again with larger
+ * buffers (trainer.cpp:819-836).  frames[v] differ only in camera (the
  * optimizer reads frames[0]'s rates and step).  targets[v]: device RGBA8 of
  * view v, or NULL (targets or an entry) to use what view v's TARGET holds.
  * pairs[count], losses[3 * count] may be NULL. */
@@ -209,6 +212,30 @@ int gs_hip_train_views(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_
                        uint64_t* pairs, double* losses);
 /* Forward only (render): project, bin, forward; image as RGBA8 (clamped rgb, alpha = 1 - T). */
 int gs_hip_render(gs_hip_workspace* ws, const gs_hip_frame* frame, uint8_t* rgba8_out);
+
+/* ---- 4. launch shapes ------------------------------------------------------ */
+/* Knobs of the batched step (no counterpart in the reference, whose shaders
+ * fix them: trainer.h:351 kTileSize, radix_sort.cpp 4-bit digits).  The tile
+ * stays 16x16 (GS_HIP_TILE), the reference's size, so pair order and tile
+ * ranges stay comparable with it. */
+typedef struct gs_hip_tuning {
+  uint32_t raster_groups_per_cu; /* persistent rasterizers: workgroups per CU pulling (view, tile) work;
+                                    0 = one workgroup per work item */
+  uint32_t depth_digit_bits;     /* radix digit width of the depth sort, 4..8 */
+  uint32_t tile_digit_bits;      /* radix digit width of the tile sort, 4..8 */
+  uint32_t graph;                /* 1: capture the batched step once and replay it as a HIP graph */
+} gs_hip_tuning;
+int gs_hip_workspace_get_tuning(gs_hip_workspace* ws, gs_hip_tuning* out);
+int gs_hip_workspace_set_tuning(gs_hip_workspace* ws, const gs_hip_tuning* tuning);
+/* Times `repeats` batched steps of the given views (GS_HIP_FLAG_NO_UPDATE
+ * added, so the model is left alone) for every candidate shape, keeps the
+ * fastest in the workspace and returns it (and its ms per step). */
+int gs_hip_autotune(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t count, const void* const* targets,
+                    uint32_t repeats, gs_hip_tuning* best, double* best_ms);
+/* Pair-buffer overflows so far: steps that found more (tile, gaussian) pairs
+ * than the buffers held, left the model alone and ran again with larger
+ * buffers (trainer.cpp:819-836). */
+int gs_hip_workspace_overflows(gs_hip_workspace* ws, uint64_t* steps);
 
 /* Kernel timing of the last stage calls on the workspace's stream, in ms
  * (hipEvents): names[i] / ms[i] for i < *count (at most `capacity`). */

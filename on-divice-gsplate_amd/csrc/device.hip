@@ -1,7 +1,9 @@
-// Device, memory and workspace entry points of the C ABI.
+// Device, memory and workspace entry points of the C ABI; the device table
+// the batched kernels read.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cstring>
 #include <string>
 
@@ -30,7 +32,7 @@ void clear_error() {
   g_message.clear();
 }
 
-void timer_begin(gs_hip_workspace* ws, View& v, const char* name) {
+void timer_begin(gs_hip_workspace* ws, const char* name) {
   Timer& t = ws->timer;
   if (!t.enabled) return;
   if (t.used == t.start.size()) {
@@ -41,14 +43,24 @@ void timer_begin(gs_hip_workspace* ws, View& v, const char* name) {
     t.names.emplace_back();
   }
   t.names[t.used] = name;
-  hipEventRecord(t.start[t.used], v.stream);
+  hipEventRecord(t.start[t.used], ws->stream);
 }
 
-void timer_end(gs_hip_workspace* ws, View& v) {
+void timer_end(gs_hip_workspace* ws) {
   Timer& t = ws->timer;
   if (!t.enabled || t.used >= t.stop.size()) return;
-  hipEventRecord(t.stop[t.used], v.stream);
+  hipEventRecord(t.stop[t.used], ws->stream);
   ++t.used;
+}
+
+void drop_graph(gs_hip_workspace* ws) {
+  if (ws->graph_exec != nullptr) hipGraphExecDestroy(ws->graph_exec);
+  if (ws->graph != nullptr) hipGraphDestroy(ws->graph);
+  ws->graph_exec = nullptr;
+  ws->graph = nullptr;
+  ws->graph_views = 0;
+  ws->graph_pair_capacity = 0;
+  ws->graph_signature = 0;
 }
 
 }  // namespace gsh
@@ -76,36 +88,27 @@ void release(T*& p) {
   p = nullptr;
 }
 
-int zero(gsh::View& v, void* p, uint64_t bytes) {
-  if (bytes == 0) return 0;
-  const uint64_t words = bytes / 4;
-  if (words > 0) {
-    k_fill_u32<<<gsh::div_up(words, 256), 256, 0, v.stream>>>(static_cast<uint32_t*>(p), words, 0u);
-    GS_CHECK_LAUNCH("k_fill_u32");
-  }
-  return 0;
-}
-
 }  // namespace
 
 namespace gsh {
 
-int create_view(gs_hip_workspace* ws, View** out) {
+int fill_u32(gs_hip_workspace* ws, void* p, uint64_t words, uint32_t value) {
+  if (words == 0) return 0;
+  k_fill_u32<<<div_up(words, 256), 256, 0, ws->stream>>>(static_cast<uint32_t*>(p), words, value);
+  GS_CHECK_LAUNCH("k_fill_u32");
+  return 0;
+}
+
+int create_view(gs_hip_workspace* ws, uint32_t index, View** out) {
   *out = nullptr;
   auto* v = new View();
   const uint64_t cap = std::max<uint32_t>(ws->capacity, 1);
   const uint64_t pixels = uint64_t(ws->width) * ws->height;
-  const uint64_t tiles = uint64_t(ws->tiles_x) * ws->tiles_y;
+  const uint64_t tiles = tile_count(ws);
   int rc = 0;
   auto A = [&](int r) {
     if (rc == 0) rc = r;
   };
-  A(hipStreamCreateWithFlags(&v->stream, hipStreamNonBlocking) == hipSuccess
-        ? 0
-        : set_error(GS_HIP_E_HIP, "hipStreamCreate failed"));
-  A(hipEventCreateWithFlags(&v->done, hipEventDisableTiming) == hipSuccess
-        ? 0
-        : set_error(GS_HIP_E_HIP, "hipEventCreate failed"));
   A(alloc(&v->records, cap * GS_HIP_RECORD_FLOATS));
   A(alloc(&v->footprint, cap * 4));
   A(alloc(&v->tiles, cap));
@@ -118,26 +121,26 @@ int create_view(gs_hip_workspace* ws, View** out) {
   A(alloc(&v->flags, cap));
   A(alloc(&v->scan_tmp, cap + 1));
   A(alloc(&v->ranges, (tiles + 1) * 2));
-  A(alloc(&v->tile_order, tiles));
   A(alloc(&v->image, pixels * 4));
   A(alloc(&v->last, pixels));
   A(alloc(&v->target, pixels));
   A(alloc(&v->dldc, pixels * 4));
   A(alloc(&v->ssim_maps, pixels * 9));
-  A(alloc(&v->tile_loss, tiles * 4));
-  A(alloc(&v->totals, 16));
   v->block_sums_entries = div_up(cap + 1, 4096) + 1024;
   A(alloc(&v->block_sums, v->block_sums_entries));
-  if (rc == 0 && hipHostMalloc(reinterpret_cast<void**>(&v->host_totals), 64) != hipSuccess) {
-    rc = set_error(GS_HIP_E_HIP, "hipHostMalloc failed");
+  v->totals = ws->totals_all + size_t(index) * 16;
+  v->tile_loss = ws->tile_loss_all + size_t(index) * tiles * 4;
+  if (rc == 0) rc = reserve_sort(*v, std::max<uint64_t>(cap, ws->views.empty() ? 0 : view0(ws).pair_capacity));
+  if (rc == 0 && !ws->views.empty() && view0(ws).pair_capacity > 0) {
+    // Same pair capacity as the other views.
+    const uint64_t n = view0(ws).pair_capacity;
+    A(alloc(&v->pair_tiles, n));
+    A(alloc(&v->pair_gaussians, n));
+    if (rc == 0) v->pair_capacity = n;
   }
-  if (rc == 0 && hipHostMalloc(reinterpret_cast<void**>(&v->host_tile_loss), tiles * 16) != hipSuccess) {
-    rc = set_error(GS_HIP_E_HIP, "hipHostMalloc failed");
-  }
-  if (rc == 0) rc = reserve_sort(*v, cap);
-  if (rc == 0) rc = zero(*v, v->totals, 64);
-  if (rc == 0) rc = zero(*v, v->target, pixels * 4);
-  if (rc == 0 && hipStreamSynchronize(v->stream) != hipSuccess) rc = set_error(GS_HIP_E_HIP, "view init sync");
+  if (rc == 0) rc = fill_u32(ws, v->target, pixels, 0u);
+  if (rc == 0) rc = fill_u32(ws, v->totals, 16, 0u);
+  if (rc == 0 && hipStreamSynchronize(ws->stream) != hipSuccess) rc = set_error(GS_HIP_E_HIP, "view init sync");
   if (rc != 0) {
     destroy_view(v);
     return rc;
@@ -148,7 +151,6 @@ int create_view(gs_hip_workspace* ws, View** out) {
 
 void destroy_view(View* v) {
   if (v == nullptr) return;
-  if (v->stream != nullptr) hipStreamSynchronize(v->stream);
   release(v->records);
   release(v->footprint);
   release(v->tiles);
@@ -163,42 +165,131 @@ void destroy_view(View* v) {
   release(v->pair_tiles);
   release(v->pair_gaussians);
   release(v->ranges);
-  release(v->tile_order);
   release(v->image);
   release(v->last);
   release(v->target);
   release(v->dldc);
   release(v->ssim_maps);
-  release(v->tile_loss);
-  release(v->totals);
   release(v->block_sums);
   release(v->sort.keys_alt);
   release(v->sort.vals_alt);
   release(v->sort.hist);
   release(v->sort.digit_totals);
-  if (v->host_totals != nullptr) hipHostFree(v->host_totals);
-  if (v->host_tile_loss != nullptr) hipHostFree(v->host_tile_loss);
-  if (v->done != nullptr) hipEventDestroy(v->done);
-  if (v->stream != nullptr) hipStreamDestroy(v->stream);
   delete v;
 }
 
-int reserve_pairs(gs_hip_workspace* ws, View& v, uint64_t pairs) {
-  (void)ws;
-  if (pairs <= v.pair_capacity) return 0;
+int reserve_sort(View& v, uint64_t items) {
+  SortScratch& s = v.sort;
+  if (items <= s.items) return 0;
+  release(s.keys_alt);
+  release(s.vals_alt);
+  release(s.hist);
+  release(s.digit_totals);
+  s = SortScratch{};
+  const uint64_t blocks = (items + 4095) / 4096;
+  int rc = alloc(&s.keys_alt, items);
+  if (rc == 0) rc = alloc(&s.vals_alt, items);
+  if (rc == 0) rc = alloc(&s.hist, std::max<uint64_t>(blocks, 1) * 256);
+  if (rc == 0) rc = alloc(&s.digit_totals, 256);
+  if (rc != 0) return rc;
+  s.items = items;
+  return 0;
+}
+
+int reserve_pairs(gs_hip_workspace* ws, uint64_t pairs) {
+  if (pairs <= view0(ws).pair_capacity) return 0;
   if (pairs >= (uint64_t(1) << 32)) return set_error(GS_HIP_E_CAPACITY, "reserve_pairs: more than 2^32 pairs");
   // Headroom: views differ in pair count (trainer.cpp:588-603 keeps 1/8 + 1/4).
   uint64_t n = std::min<uint64_t>(pairs + pairs / 4, (uint64_t(1) << 32) - 1);
   n = std::max<uint64_t>(n, 1 << 16);
-  GS_HIP_TRY(hipStreamSynchronize(v.stream));
-  release(v.pair_tiles);
-  release(v.pair_gaussians);
-  v.pair_capacity = 0;
-  int rc = alloc(&v.pair_tiles, n);
-  if (rc == 0) rc = alloc(&v.pair_gaussians, n);
-  if (rc == 0) rc = reserve_sort(v, n);
-  if (rc != 0) return rc;
-  v.pair_capacity = n;
+  GS_HIP_TRY(hipStreamSynchronize(ws->stream));
+  drop_graph(ws);
+  for (View* v : ws->views) {
+    release(v->pair_tiles);
+    release(v->pair_gaussians);
+    v->pair_capacity = 0;
+    int rc = alloc(&v->pair_tiles, n);
+    if (rc == 0) rc = alloc(&v->pair_gaussians, n);
+    if (rc == 0) rc = reserve_sort(*v, std::max<uint64_t>(n, ws->capacity));
+    if (rc != 0) return rc;
+    v->pair_capacity = n;
+  }
+  ws->table_pair_capacity = 0;  // the segments point at the old buffers
+  return 0;
+}
+
+namespace {
+
+ViewDev make_view_dev(gs_hip_workspace* ws, View& v, const gs_hip_frame& f) {
+  ViewDev d;
+  std::memset(&d, 0, sizeof(d));
+  d.f = f;
+  d.records = reinterpret_cast<float4*>(v.records);
+  d.footprint = reinterpret_cast<float4*>(v.footprint);
+  d.tiles = v.tiles;
+  d.misc = v.misc;
+  d.depth_keys = v.depth_keys;
+  d.depth_order = v.depth_order;
+  d.offsets = v.offsets;
+  d.grad2d = v.grad2d;
+  d.sh_input = reinterpret_cast<float4*>(v.sh_input);
+  d.pair_tiles = v.pair_tiles;
+  d.pair_gaussians = v.pair_gaussians;
+  d.ranges = reinterpret_cast<uint2*>(v.ranges);
+  d.image = reinterpret_cast<float4*>(v.image);
+  d.last = v.last;
+  d.target = v.target;
+  d.dldc = reinterpret_cast<float4*>(v.dldc);
+  d.maps = v.ssim_maps;
+  d.tile_loss = reinterpret_cast<float4*>(v.tile_loss);
+  d.totals = v.totals;
+  d.pair_capacity = v.pair_capacity;
+  (void)ws;
+  return d;
+}
+
+// Sort and scan segments of every view (they change only when buffers move).
+int upload_segments(gs_hip_workspace* ws) {
+  SortSeg depth[GS_HIP_MAX_VIEWS], tile[GS_HIP_MAX_VIEWS];
+  ScanSeg scan[GS_HIP_MAX_VIEWS];
+  std::memset(depth, 0, sizeof(depth));
+  std::memset(tile, 0, sizeof(tile));
+  std::memset(scan, 0, sizeof(scan));
+  const size_t slots_offset = offsetof(ViewDev, f) + offsetof(gs_hip_frame, slots);
+  for (size_t i = 0; i < ws->views.size(); ++i) {
+    View& v = *ws->views[i];
+    const uint32_t* slots = reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(ws->dev_views + i) +
+                                                              slots_offset);
+    depth[i] = SortSeg{v.depth_keys, v.depth_order, v.sort.keys_alt, v.sort.vals_alt, v.sort.hist,
+                       v.sort.digit_totals, slots, ws->capacity, 0u};
+    tile[i] = SortSeg{v.pair_tiles, v.pair_gaussians, v.sort.keys_alt, v.sort.vals_alt, v.sort.hist,
+                      v.sort.digit_totals, v.totals, uint32_t(std::min<uint64_t>(v.pair_capacity, 0xffffffffu)), 0u};
+    scan[i] = ScanSeg{v.tiles, v.depth_order, v.offsets, v.block_sums, v.totals, slots, ws->capacity, 0u};
+  }
+  GS_HIP_TRY(hipStreamSynchronize(ws->stream));
+  GS_HIP_TRY(hipMemcpy(ws->dev_depth_sort, depth, sizeof(depth), hipMemcpyHostToDevice));
+  GS_HIP_TRY(hipMemcpy(ws->dev_tile_sort, tile, sizeof(tile), hipMemcpyHostToDevice));
+  GS_HIP_TRY(hipMemcpy(ws->dev_tile_scan, scan, sizeof(scan), hipMemcpyHostToDevice));
+  ws->table_pair_capacity = view0(ws).pair_capacity;
+  return 0;
+}
+
+}  // namespace
+
+int upload_table(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t count) {
+  if (count == 0 || count > ws->views.size()) return set_error(GS_HIP_E_ARGUMENT, "view count out of range");
+  if (ws->table_pair_capacity != view0(ws).pair_capacity || ws->table_pair_capacity == 0) {
+    const int rc = upload_segments(ws);
+    if (rc != 0) return rc;
+  }
+  // A staging slot is reused only after its previous copy has run.
+  const uint32_t s = ws->staging_next;
+  ws->staging_next = (s + 1) % kStagingSlots;
+  GS_HIP_TRY(hipEventSynchronize(ws->staging_done[s]));
+  ViewDev* host = ws->host_views + size_t(s) * GS_HIP_MAX_VIEWS;
+  for (uint32_t v = 0; v < count; ++v) host[v] = make_view_dev(ws, *ws->views[v], frames[v]);
+  GS_HIP_TRY(hipMemcpyAsync(ws->dev_views, host, sizeof(ViewDev) * count, hipMemcpyHostToDevice, ws->stream));
+  GS_HIP_TRY(hipEventRecord(ws->staging_done[s], ws->stream));
   return 0;
 }
 
@@ -290,12 +381,26 @@ int gs_hip_workspace_create(gs_hip_workspace** out, uint32_t capacity, int sh_de
   ws->height = height;
   ws->tiles_x = gsh::div_up(width, GS_HIP_TILE);
   ws->tiles_y = gsh::div_up(height, GS_HIP_TILE);
+  ws->tuning.raster_groups_per_cu = 0;
+  ws->tuning.depth_digit_bits = 8;
+  ws->tuning.tile_digit_bits = 8;
+  ws->tuning.graph = 0;
   const uint64_t cap = std::max<uint32_t>(capacity, 1);
   const uint64_t shn = cap * ws->sh_coeffs * 3;
+  const uint64_t tiles = gsh::tile_count(ws);
   int rc = 0;
   auto A = [&](int r) {
     if (rc == 0) rc = r;
   };
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+    ws->cus = uint32_t(prop.multiProcessorCount);
+  }
+  if (ws->cus == 0) ws->cus = 256;
+  A(hipStreamCreateWithFlags(&ws->stream, hipStreamNonBlocking) == hipSuccess
+        ? 0
+        : gsh::set_error(GS_HIP_E_HIP, "hipStreamCreate failed"));
   A(alloc(&ws->params, cap * 16));
   A(alloc(&ws->sh, shn));
   A(alloc(&ws->m_params, cap * 16));
@@ -303,15 +408,47 @@ int gs_hip_workspace_create(gs_hip_workspace** out, uint32_t capacity, int sh_de
   A(alloc(&ws->m_sh, shn));
   A(alloc(&ws->v_sh, shn));
   A(alloc(&ws->stats, cap * 4));
-  gsh::View* v = nullptr;
-  if (rc == 0) rc = gsh::create_view(ws, &v);
-  if (v != nullptr) ws->views.push_back(v);
-  if (rc == 0) {
-    for (void* p : {(void*)ws->params, (void*)ws->m_params, (void*)ws->v_params}) A(zero(*v, p, cap * 64));
-    for (void* p : {(void*)ws->sh, (void*)ws->m_sh, (void*)ws->v_sh}) A(zero(*v, p, shn * 4));
-    A(zero(*v, ws->stats, cap * 16));
-    if (rc == 0 && hipStreamSynchronize(v->stream) != hipSuccess) A(gsh::set_error(GS_HIP_E_HIP, "workspace init sync"));
+  A(alloc(&ws->dev_views, GS_HIP_MAX_VIEWS));
+  A(alloc(&ws->dev_depth_sort, GS_HIP_MAX_VIEWS));
+  A(alloc(&ws->dev_tile_sort, GS_HIP_MAX_VIEWS));
+  A(alloc(&ws->dev_tile_scan, GS_HIP_MAX_VIEWS));
+  A(alloc(&ws->dev_scan_tmp, 1));
+  A(alloc(&ws->work, tiles * GS_HIP_MAX_VIEWS));
+  A(alloc(&ws->work_counters, 16));
+  A(alloc(&ws->totals_all, 16 * GS_HIP_MAX_VIEWS));
+  A(alloc(&ws->tile_loss_all, tiles * 4 * GS_HIP_MAX_VIEWS));
+  if (rc == 0 && hipHostMalloc(reinterpret_cast<void**>(&ws->host_views),
+                               sizeof(gsh::ViewDev) * GS_HIP_MAX_VIEWS * gsh::kStagingSlots) != hipSuccess) {
+    rc = gsh::set_error(GS_HIP_E_HIP, "hipHostMalloc failed");
   }
+  if (rc == 0 && hipHostMalloc(reinterpret_cast<void**>(&ws->host_totals), 64 * GS_HIP_MAX_VIEWS) != hipSuccess) {
+    rc = gsh::set_error(GS_HIP_E_HIP, "hipHostMalloc failed");
+  }
+  if (rc == 0 && hipHostMalloc(reinterpret_cast<void**>(&ws->host_tile_loss), tiles * 16 * GS_HIP_MAX_VIEWS) !=
+                     hipSuccess) {
+    rc = gsh::set_error(GS_HIP_E_HIP, "hipHostMalloc failed");
+  }
+  for (uint32_t s = 0; s < gsh::kStagingSlots && rc == 0; ++s) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      rc = gsh::set_error(GS_HIP_E_HIP, "hipEventCreate failed");
+    } else {
+      ws->staging_done.push_back(e);
+    }
+  }
+  if (rc == 0) {
+    for (void* p : {(void*)ws->params, (void*)ws->m_params, (void*)ws->v_params}) A(gsh::fill_u32(ws, p, cap * 16, 0));
+    for (void* p : {(void*)ws->sh, (void*)ws->m_sh, (void*)ws->v_sh}) A(gsh::fill_u32(ws, p, shn, 0));
+    A(gsh::fill_u32(ws, ws->stats, cap * 4, 0));
+    A(gsh::fill_u32(ws, ws->totals_all, 16 * GS_HIP_MAX_VIEWS, 0));
+    A(gsh::fill_u32(ws, ws->work_counters, 16, 0));
+  }
+  gsh::View* v = nullptr;
+  if (rc == 0) rc = gsh::create_view(ws, 0, &v);
+  if (v != nullptr) ws->views.push_back(v);
+  // First guess before any count: a few tiles per gaussian (trainer.cpp:816-818).
+  if (rc == 0) rc = gsh::reserve_pairs(ws, std::max<uint64_t>(uint64_t(capacity) * 4, 1 << 16));
+  if (rc == 0 && hipStreamSynchronize(ws->stream) != hipSuccess) A(gsh::set_error(GS_HIP_E_HIP, "workspace init sync"));
   if (rc != 0) {
     gs_hip_workspace_destroy(ws);
     return rc;
@@ -327,9 +464,11 @@ int gs_hip_workspace_views(gs_hip_workspace* ws, uint32_t count) {
   }
   while (ws->views.size() < count) {
     gsh::View* v = nullptr;
-    const int rc = gsh::create_view(ws, &v);
+    const int rc = gsh::create_view(ws, uint32_t(ws->views.size()), &v);
     if (rc != 0) return rc;
     ws->views.push_back(v);
+    ws->table_pair_capacity = 0;
+    gsh::drop_graph(ws);
   }
   return 0;
 }
@@ -337,6 +476,7 @@ int gs_hip_workspace_views(gs_hip_workspace* ws, uint32_t count) {
 int gs_hip_workspace_destroy(gs_hip_workspace* ws) {
   if (ws == nullptr) return 0;
   hipDeviceSynchronize();
+  gsh::drop_graph(ws);
   for (gsh::View* v : ws->views) gsh::destroy_view(v);
   ws->views.clear();
   release(ws->params);
@@ -346,8 +486,22 @@ int gs_hip_workspace_destroy(gs_hip_workspace* ws) {
   release(ws->m_sh);
   release(ws->v_sh);
   release(ws->stats);
+  release(ws->dev_views);
+  release(ws->dev_depth_sort);
+  release(ws->dev_tile_sort);
+  release(ws->dev_tile_scan);
+  release(ws->dev_scan_tmp);
+  release(ws->work);
+  release(ws->work_counters);
+  release(ws->totals_all);
+  release(ws->tile_loss_all);
+  if (ws->host_views != nullptr) hipHostFree(ws->host_views);
+  if (ws->host_totals != nullptr) hipHostFree(ws->host_totals);
+  if (ws->host_tile_loss != nullptr) hipHostFree(ws->host_tile_loss);
+  for (hipEvent_t e : ws->staging_done) hipEventDestroy(e);
   for (hipEvent_t e : ws->timer.start) hipEventDestroy(e);
   for (hipEvent_t e : ws->timer.stop) hipEventDestroy(e);
+  if (ws->stream != nullptr) hipStreamDestroy(ws->stream);
   delete ws;
   return 0;
 }
@@ -357,7 +511,7 @@ int gs_hip_workspace_view_buffer(gs_hip_workspace* ws, uint32_t view, int which,
   if (view >= ws->views.size()) return gsh::set_error(GS_HIP_E_ARGUMENT, "workspace_buffer: no such view");
   const gsh::View& v = *ws->views[view];
   const size_t cap = ws->capacity, pixels = size_t(ws->width) * ws->height;
-  const size_t tiles = size_t(ws->tiles_x) * ws->tiles_y, shn = cap * ws->sh_coeffs * 3;
+  const size_t tiles = gsh::tile_count(ws), shn = cap * ws->sh_coeffs * 3;
   void* p = nullptr;
   size_t b = 0;
   switch (which) {
@@ -384,7 +538,7 @@ int gs_hip_workspace_view_buffer(gs_hip_workspace* ws, uint32_t view, int which,
     case GS_HIP_BUF_DLDC: p = v.dldc, b = pixels * 16; break;
     case GS_HIP_BUF_TILE_LOSS: p = v.tile_loss, b = tiles * 16; break;
     case GS_HIP_BUF_TOTALS: p = v.totals, b = 64; break;
-    case GS_HIP_BUF_TILE_ORDER: p = v.tile_order, b = tiles * 4; break;
+    case GS_HIP_BUF_TILE_ORDER: p = ws->work, b = tiles * 4 * ws->views.size(); break;
     default: return gsh::set_error(GS_HIP_E_ARGUMENT, "workspace_buffer: unknown buffer");
   }
   *ptr = p;
@@ -398,11 +552,7 @@ int gs_hip_workspace_buffer(gs_hip_workspace* ws, int which, void** ptr, size_t*
 
 int gs_hip_workspace_reserve_pairs(gs_hip_workspace* ws, uint64_t pairs) {
   if (ws == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "reserve_pairs: null workspace");
-  for (gsh::View* v : ws->views) {
-    const int rc = gsh::reserve_pairs(ws, *v, pairs);
-    if (rc != 0) return rc;
-  }
-  return 0;
+  return gsh::reserve_pairs(ws, pairs);
 }
 
 int gs_hip_workspace_load(gs_hip_workspace* ws, const float* params, const float* sh_rest, uint32_t count,
@@ -410,18 +560,17 @@ int gs_hip_workspace_load(gs_hip_workspace* ws, const float* params, const float
   if (ws == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "workspace_load: null workspace");
   if (count > ws->capacity) return gsh::set_error(GS_HIP_E_CAPACITY, "workspace_load: more slots than capacity");
   const size_t cap = ws->capacity, shn = cap * ws->sh_coeffs * 3;
-  gsh::View& v0 = gsh::view0(ws);
   GS_HIP_TRY(hipDeviceSynchronize());
   int rc = 0;
   for (void* p : {(void*)ws->params, (void*)ws->m_params, (void*)ws->v_params}) {
-    if (rc == 0) rc = zero(v0, p, cap * 64);
+    if (rc == 0) rc = gsh::fill_u32(ws, p, cap * 16, 0);
   }
   for (void* p : {(void*)ws->sh, (void*)ws->m_sh, (void*)ws->v_sh}) {
-    if (rc == 0) rc = zero(v0, p, shn * 4);
+    if (rc == 0) rc = gsh::fill_u32(ws, p, shn, 0);
   }
-  if (rc == 0) rc = zero(v0, ws->stats, cap * 16);
+  if (rc == 0) rc = gsh::fill_u32(ws, ws->stats, cap * 4, 0);
   if (rc != 0) return rc;
-  GS_HIP_TRY(hipStreamSynchronize(v0.stream));
+  GS_HIP_TRY(hipStreamSynchronize(ws->stream));
   if (count == 0) return 0;
   GS_HIP_TRY(hipMemcpy(ws->params, params, size_t(count) * 64, hipMemcpyHostToDevice));
   if (ws->sh_coeffs > 0 && sh_rest != nullptr) {
@@ -466,8 +615,9 @@ int gs_hip_workspace_store(gs_hip_workspace* ws, float* params, float* sh_rest, 
 int gs_hip_set_target(gs_hip_workspace* ws, const uint8_t* rgba8) {
   if (ws == nullptr || rgba8 == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "set_target: null");
   gsh::View& v = gsh::view0(ws);
-  GS_HIP_TRY(hipMemcpyAsync(v.target, rgba8, size_t(ws->width) * ws->height * 4, hipMemcpyHostToDevice, v.stream));
-  GS_HIP_TRY(hipStreamSynchronize(v.stream));
+  GS_HIP_TRY(hipMemcpyAsync(v.target, rgba8, size_t(ws->width) * ws->height * 4, hipMemcpyHostToDevice,
+                            ws->stream));
+  GS_HIP_TRY(hipStreamSynchronize(ws->stream));
   return 0;
 }
 
@@ -475,7 +625,31 @@ int gs_hip_set_target_device(gs_hip_workspace* ws, const void* device_rgba8) {
   if (ws == nullptr || device_rgba8 == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "set_target_device: null");
   gsh::View& v = gsh::view0(ws);
   GS_HIP_TRY(hipMemcpyAsync(v.target, device_rgba8, size_t(ws->width) * ws->height * 4, hipMemcpyDeviceToDevice,
-                            v.stream));
+                            ws->stream));
+  return 0;
+}
+
+int gs_hip_workspace_get_tuning(gs_hip_workspace* ws, gs_hip_tuning* out) {
+  if (ws == nullptr || out == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "get_tuning: null");
+  *out = ws->tuning;
+  return 0;
+}
+
+int gs_hip_workspace_set_tuning(gs_hip_workspace* ws, const gs_hip_tuning* t) {
+  if (ws == nullptr || t == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "set_tuning: null");
+  if (t->depth_digit_bits < 4 || t->depth_digit_bits > 8 || t->tile_digit_bits < 4 || t->tile_digit_bits > 8) {
+    return gsh::set_error(GS_HIP_E_ARGUMENT, "set_tuning: digit widths are 4..8 bits");
+  }
+  if (t->raster_groups_per_cu > 64) return gsh::set_error(GS_HIP_E_ARGUMENT, "set_tuning: at most 64 groups per CU");
+  const bool changed = std::memcmp(&ws->tuning, t, sizeof(*t)) != 0;
+  ws->tuning = *t;
+  if (changed) gsh::drop_graph(ws);
+  return 0;
+}
+
+int gs_hip_workspace_overflows(gs_hip_workspace* ws, uint64_t* steps) {
+  if (ws == nullptr || steps == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "overflows: null");
+  *steps = ws->overflow_steps;
   return 0;
 }
 

@@ -1,6 +1,13 @@
 // Host-side internals shared by the C ABI translation units: the workspace
-// (the model's per-slot state plus one or more views' transient buffers),
-// error reporting, launch helpers.
+// (the model's per-slot state plus up to GS_HIP_MAX_VIEWS views' transient
+// buffers), the device table the batched kernels read, error reporting and
+// launch helpers.
+//
+// Every stage is one launch over all views of a step (the view is blockIdx.y,
+// or part of the work item for the rasterizers), on the workspace's one
+// stream.  Kernels find their view's frame and buffers in a device-resident
+// table (ViewDev), so the step's launch sequence does not depend on the
+// frames and can be captured once in a HIP graph and replayed.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -33,51 +40,99 @@ void clear_error();
 
 inline uint32_t div_up(uint64_t a, uint64_t b) { return uint32_t((a + b - 1) / b); }
 
-// Radix sort scratch for up to `items` (key, value) pairs.
+// One segment of a batched radix sort: (keys, vals)[0, n) with
+// n = min(*count, count_max) (count_max when count is null), sorted in place
+// through the alternates.
+struct SortSeg {
+  uint32_t* keys;
+  uint32_t* vals;
+  uint32_t* keys_alt;
+  uint32_t* vals_alt;
+  uint32_t* hist;          // [256][blocks]
+  uint32_t* digit_totals;  // [256]
+  const uint32_t* count;
+  uint32_t count_max;
+  uint32_t pad;
+};
+
+// One segment of a batched exclusive scan: out[i] = sum of values[gather[j]]
+// (or values[j]) for j < i, i < n = min(*count, count_max) (count_max when
+// count is null); the total goes to *total.
+struct ScanSeg {
+  const uint32_t* values;
+  const uint32_t* gather;
+  uint32_t* out;
+  uint32_t* block_sums;
+  uint32_t* total;
+  const uint32_t* count;
+  uint32_t count_max;
+  uint32_t pad;
+};
+
+// A view's frame and buffers as the kernels see them (device memory).
+struct ViewDev {
+  gs_hip_frame f;
+  // per slot
+  float4* records;
+  float4* footprint;
+  uint32_t* tiles;
+  uint32_t* misc;  // radius << 3 | colour clamp bits: what the adjoint needs of the record
+  uint32_t* depth_keys;
+  uint32_t* depth_order;
+  uint32_t* offsets;
+  float* grad2d;     // [9][capacity]
+  float4* sh_input;  // [capacity][2]: (dL/dcolour, update flag in view 0), (direction, -)
+  // per pair
+  uint32_t* pair_tiles;
+  uint32_t* pair_gaussians;
+  // per tile / pixel
+  uint2* ranges;
+  float4* image;
+  uint32_t* last;
+  const uint32_t* target;
+  float4* dldc;
+  float* maps;  // [9][pixels]: SSIM gradient maps A, B, C per channel
+  float4* tile_loss;
+  uint32_t* totals;  // [0] pairs, [1] emit mismatches, [5..7] densify counters
+  uint64_t pair_capacity;
+};
+
+// Radix sort scratch of one view, for up to `items` (key, value) pairs.
 struct SortScratch {
   uint32_t* keys_alt = nullptr;
   uint32_t* vals_alt = nullptr;
-  uint32_t* hist = nullptr;          // [digits][blocks]
-  uint32_t* digit_totals = nullptr;  // [digits]
+  uint32_t* hist = nullptr;          // [256][blocks]
+  uint32_t* digit_totals = nullptr;  // [256]
   uint64_t items = 0;
 };
 
-// One view's transient buffers and its stream: everything a view-step
-// computes before the optimizer, so several views can be in flight at once.
+// One view's transient buffers (host handles of what ViewDev points at).
 struct View {
-  hipStream_t stream = nullptr;
-  hipEvent_t done = nullptr;
-  // per slot
   float* records = nullptr;
   float* footprint = nullptr;
   uint32_t* tiles = nullptr;
-  uint32_t* misc = nullptr;  // per slot: radius << 3 | colour clamp bits (the adjoint's view of the record)
+  uint32_t* misc = nullptr;
   uint32_t* depth_keys = nullptr;
   uint32_t* depth_order = nullptr;
   uint32_t* offsets = nullptr;
   float* grad2d = nullptr;
-  float* sh_input = nullptr;     // per slot: 8 floats, the view's SH colour gradient and direction
+  float* sh_input = nullptr;
   uint32_t* flags = nullptr;     // densify: per-slot actions
   uint32_t* scan_tmp = nullptr;  // per-slot scan output
-  // per pair
   uint64_t pair_capacity = 0;
   uint32_t* pair_tiles = nullptr;
   uint32_t* pair_gaussians = nullptr;
-  uint64_t pairs = 0;  // of the last bin
-  // per tile / pixel
+  uint64_t pairs = 0;  // of the last step (host copy)
   uint32_t* ranges = nullptr;
-  uint32_t* tile_order = nullptr;  // tiles, longest list first (launch order of the rasterizers)
   float* image = nullptr;
   uint32_t* last = nullptr;
   uint32_t* target = nullptr;
   float* dldc = nullptr;
-  float* ssim_maps = nullptr;  // float[9][pixels]: A, B, C per channel
-  float* tile_loss = nullptr;
-  uint32_t* totals = nullptr;  // [0] pairs, [1] emit mismatches, [5..7] densify counters
+  float* ssim_maps = nullptr;
+  float* tile_loss = nullptr;  // slice of the workspace's tile_loss_all
+  uint32_t* totals = nullptr;  // slice of the workspace's totals_all
   uint32_t* block_sums = nullptr;
   uint32_t block_sums_entries = 0;
-  uint32_t* host_totals = nullptr;  // pinned
-  float* host_tile_loss = nullptr;  // pinned
   SortScratch sort;
 };
 
@@ -96,6 +151,8 @@ struct gs_hip_workspace {
   int sh_degree = 0;
   uint32_t sh_coeffs = 0;  // rest coefficients stored per slot
   uint32_t width = 0, height = 0, tiles_x = 0, tiles_y = 0;
+  uint32_t cus = 0;  // compute units of the device
+  hipStream_t stream = nullptr;
   // the model: per-slot parameters, Adam moments, densify statistics
   float* params = nullptr;
   float* sh = nullptr;
@@ -104,36 +161,79 @@ struct gs_hip_workspace {
   float* m_sh = nullptr;
   float* v_sh = nullptr;
   float* stats = nullptr;
-  std::vector<gsh::View*> views;  // views[0] always exists; its stream runs the optimizer
+  std::vector<gsh::View*> views;  // views[0] always exists
+  // device table: ViewDev[MAX_VIEWS] and the batched sort / scan segments
+  gsh::ViewDev* dev_views = nullptr;
+  gsh::SortSeg* dev_depth_sort = nullptr;  // [MAX_VIEWS]
+  gsh::SortSeg* dev_tile_sort = nullptr;   // [MAX_VIEWS]
+  gsh::ScanSeg* dev_tile_scan = nullptr;   // [MAX_VIEWS]
+  gsh::ScanSeg* dev_scan_tmp = nullptr;    // [1]: host-count scans (densify)
+  uint32_t* work = nullptr;           // rasterizer work items (view * tiles + tile), longest lists first
+  uint32_t* work_counters = nullptr;  // [0] forward, [1] backward (persistent rasterizers)
+  uint32_t* totals_all = nullptr;     // [MAX_VIEWS][16]
+  float* tile_loss_all = nullptr;     // [MAX_VIEWS][tiles][4]
+  // host side of the table and readbacks (pinned)
+  gsh::ViewDev* host_views = nullptr;  // staging ring: kStagingSlots copies of ViewDev[MAX_VIEWS]
+  std::vector<hipEvent_t> staging_done;
+  uint32_t staging_next = 0;
+  uint32_t* host_totals = nullptr;  // [MAX_VIEWS][16]
+  float* host_tile_loss = nullptr;  // [MAX_VIEWS][tiles][4]
+  uint64_t table_pair_capacity = 0;  // pair capacity the segments were built for
+  uint64_t max_pairs_seen = 0;
+  uint64_t overflow_steps = 0;
+  gs_hip_tuning tuning;
+  // captured batched step
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  uint32_t graph_views = 0;  // view count the graph was captured for
+  uint64_t graph_pair_capacity = 0;
+  uint64_t graph_signature = 0;  // view count and step kind it was captured for
   gsh::Timer timer;
 };
 
 namespace gsh {
+constexpr uint32_t kStagingSlots = 64;
+
 inline View& view0(gs_hip_workspace* ws) { return *ws->views[0]; }
-// Kernel timing on a view's stream (no-ops unless enabled).
-void timer_begin(gs_hip_workspace* ws, View& v, const char* name);
-void timer_end(gs_hip_workspace* ws, View& v);
+inline uint32_t tile_count(const gs_hip_workspace* ws) { return ws->tiles_x * ws->tiles_y; }
+// Kernel timing on the workspace stream (no-ops unless enabled).
+void timer_begin(gs_hip_workspace* ws, const char* name);
+void timer_end(gs_hip_workspace* ws);
 // Allocates / frees one view's buffers.
-int create_view(gs_hip_workspace* ws, View** out);
+int create_view(gs_hip_workspace* ws, uint32_t index, View** out);
 void destroy_view(View* v);
-int reserve_pairs(gs_hip_workspace* ws, View& v, uint64_t pairs);
-// Grows v.sort for `items`.
+// Grows every view's pair buffers (and sort scratch) to hold `pairs` with
+// headroom; drops the captured graph when buffers move.
+int reserve_pairs(gs_hip_workspace* ws, uint64_t pairs);
 int reserve_sort(View& v, uint64_t items);
-// Stable LSD radix sort of (keys, vals)[0, n) by key bits [0, bits) on v.stream.
-int radix_sort(View& v, uint32_t* keys, uint32_t* vals, uint32_t n, int bits);
-// Exclusive scan of `count` u32 values (values[gather[k]] when gather is not
-// null) into out; the total goes to *total_dev (device).
-int exclusive_scan(View& v, const uint32_t* values, const uint32_t* gather, uint32_t* out, uint32_t count,
-                   uint32_t* total_dev);
-// Stages on one view.
-int launch_project(gs_hip_workspace* ws, View& v, const gs_hip_frame* f);
-int bin_count(gs_hip_workspace* ws, View& v, const gs_hip_frame* f);          // async; P lands in v.host_totals
-int bin_pairs(gs_hip_workspace* ws, View& v, const gs_hip_frame* f, uint64_t pairs);
-int launch_forward(gs_hip_workspace* ws, View& v, const gs_hip_frame* f);
-int launch_loss(gs_hip_workspace* ws, View& v, const gs_hip_frame* f, bool copy_losses);
-int launch_backward(gs_hip_workspace* ws, View& v, const gs_hip_frame* f);
-// Adjoint + Adam over the mean gradient of views[0, count) on view 0's stream.
-int launch_adam(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t count);
-// Sums a view's tile losses (after its stream has finished).
-void sum_losses(gs_hip_workspace* ws, View& v, double losses[3]);
+// Writes frames[0, count) into the device table for views [0, count)
+// (stream-ordered copy from a pinned staging slot) with every view's buffer
+// pointers, and the sort / scan segments when the buffers moved.
+int upload_table(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t count);
+// Batched stages over views [0, count) of the uploaded table.
+int launch_project(gs_hip_workspace* ws, uint32_t count);
+int launch_depth_sort_and_scan(gs_hip_workspace* ws, uint32_t count, bool depth16);
+int launch_emit_and_tile_sort(gs_hip_workspace* ws, uint32_t count);
+int launch_forward(gs_hip_workspace* ws, uint32_t count);
+int launch_loss(gs_hip_workspace* ws, uint32_t count);
+int launch_backward(gs_hip_workspace* ws, uint32_t count);
+// Adjoint + Adam over the mean gradient of views [0, count); skipped on the
+// device when any view's pairs overflowed its buffers.
+int launch_adam(gs_hip_workspace* ws, uint32_t count);
+// One batched step over views [0, count) (train: with backward and Adam;
+// losses: tile loss sums read back), rerun with larger pair buffers when a
+// view overflowed; pairs[count] and loss_sums[3 * count] may be null.
+int run_step(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t count, const void* const* targets, bool train,
+             bool losses, uint64_t* pairs, double* loss_sums);
+// Copies the views' totals (and tile losses) to the pinned host copies.
+int launch_readback(gs_hip_workspace* ws, uint32_t count, bool losses);
+// Exclusive scan of `count` u32 on the stream (one segment, host count).
+int exclusive_scan(gs_hip_workspace* ws, View& v, const uint32_t* values, const uint32_t* gather, uint32_t* out,
+                   uint32_t count, uint32_t* total_dev);
+// Sums view v's tile losses (host copy, after the stream finished).
+void sum_losses(gs_hip_workspace* ws, uint32_t v, double losses[3]);
+// Fills `words` u32 with `value` by a kernel (graph-safe, unlike hipMemsetAsync).
+int fill_u32(gs_hip_workspace* ws, void* p, uint64_t words, uint32_t value);
+// Drops the captured graph (buffers or view count changed).
+void drop_graph(gs_hip_workspace* ws);
 }  // namespace gsh

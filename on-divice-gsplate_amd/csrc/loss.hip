@@ -56,7 +56,21 @@ __device__ __forceinline__ void load_xy(const gs_hip_frame& f, const LossArgs& a
   }
 }
 
-__global__ void __launch_bounds__(256) k_ssim(gs_hip_frame f, LossArgs a) {
+__device__ __forceinline__ LossArgs loss_args(const gsh::ViewDev& v) {
+  LossArgs a;
+  a.image = v.image;
+  a.target = v.target;
+  a.maps = v.maps;
+  a.dldc = v.dldc;
+  a.tile_loss = v.tile_loss;
+  return a;
+}
+
+// One tile (blockIdx.x) of one view (blockIdx.y).
+__global__ void __launch_bounds__(256) k_ssim(const gsh::ViewDev* __restrict__ views) {
+  const gsh::ViewDev& view = views[blockIdx.y];
+  const gs_hip_frame& f = view.f;
+  const LossArgs a = loss_args(view);
   __shared__ float s_x[3][kHalo * kHalo];
   __shared__ float s_y[3][kHalo * kHalo];
   __shared__ float s_h[5][kHalo * kTile];
@@ -147,7 +161,10 @@ __global__ void __launch_bounds__(256) k_ssim(gs_hip_frame f, LossArgs a) {
   }
 }
 
-__global__ void __launch_bounds__(256) k_loss_grad(gs_hip_frame f, LossArgs a) {
+__global__ void __launch_bounds__(256) k_loss_grad(const gsh::ViewDev* __restrict__ views) {
+  const gsh::ViewDev& view = views[blockIdx.y];
+  const gs_hip_frame& f = view.f;
+  const LossArgs a = loss_args(view);
   __shared__ float s_m[3][kHalo * kHalo];
   __shared__ float s_h[3][kHalo * kTile];
   const uint32_t tile = blockIdx.x;
@@ -201,33 +218,35 @@ __global__ void __launch_bounds__(256) k_loss_grad(gs_hip_frame f, LossArgs a) {
 
 namespace gsh {
 
-int launch_loss(gs_hip_workspace* ws, View& v, const gs_hip_frame* f, bool copy_losses) {
-  const uint32_t tiles = f->tiles_x * f->tiles_y;
-  LossArgs a;
-  a.image = reinterpret_cast<const float4*>(v.image);
-  a.target = v.target;
-  a.maps = v.ssim_maps;
-  a.dldc = reinterpret_cast<float4*>(v.dldc);
-  a.tile_loss = reinterpret_cast<float4*>(v.tile_loss);
-  timer_begin(ws, v, "ssim");
-  k_ssim<<<tiles, 256, 0, v.stream>>>(*f, a);
+int launch_loss(gs_hip_workspace* ws, uint32_t count) {
+  const dim3 grid(tile_count(ws), count);
+  timer_begin(ws, "ssim");
+  k_ssim<<<grid, 256, 0, ws->stream>>>(ws->dev_views);
   GS_CHECK_LAUNCH("k_ssim");
-  timer_end(ws, v);
-  timer_begin(ws, v, "loss_grad");
-  k_loss_grad<<<tiles, 256, 0, v.stream>>>(*f, a);
+  timer_end(ws);
+  timer_begin(ws, "loss_grad");
+  k_loss_grad<<<grid, 256, 0, ws->stream>>>(ws->dev_views);
   GS_CHECK_LAUNCH("k_loss_grad");
-  timer_end(ws, v);
-  if (copy_losses) {
-    GS_HIP_TRY(hipMemcpyAsync(v.host_tile_loss, v.tile_loss, size_t(tiles) * 16, hipMemcpyDeviceToHost, v.stream));
+  timer_end(ws);
+  return 0;
+}
+
+int launch_readback(gs_hip_workspace* ws, uint32_t count, bool losses) {
+  GS_HIP_TRY(hipMemcpyAsync(ws->host_totals, ws->totals_all, size_t(count) * 64, hipMemcpyDeviceToHost,
+                            ws->stream));
+  if (losses) {
+    GS_HIP_TRY(hipMemcpyAsync(ws->host_tile_loss, ws->tile_loss_all, size_t(count) * tile_count(ws) * 16,
+                              hipMemcpyDeviceToHost, ws->stream));
   }
   return 0;
 }
 
-void sum_losses(gs_hip_workspace* ws, View& v, double losses[3]) {
-  const uint32_t tiles = ws->tiles_x * ws->tiles_y;
+void sum_losses(gs_hip_workspace* ws, uint32_t v, double losses[3]) {
+  const uint32_t tiles = tile_count(ws);
+  const float* t = ws->host_tile_loss + size_t(v) * tiles * 4;
   double s[3] = {0.0, 0.0, 0.0};
-  for (uint32_t t = 0; t < tiles; ++t) {
-    for (int q = 0; q < 3; ++q) s[q] += v.host_tile_loss[t * 4 + q];
+  for (uint32_t i = 0; i < tiles; ++i) {
+    for (int q = 0; q < 3; ++q) s[q] += t[i * 4 + q];
   }
   for (int q = 0; q < 3; ++q) losses[q] = s[q];
 }
@@ -238,12 +257,14 @@ extern "C" {
 
 int gs_hip_loss(gs_hip_workspace* ws, const gs_hip_frame* f, double losses[3]) {
   if (ws == nullptr || f == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "loss: null argument");
-  gsh::View& v = gsh::view0(ws);
-  const int rc = gsh::launch_loss(ws, v, f, losses != nullptr);
+  int rc = gsh::upload_table(ws, f, 1);
+  if (rc == 0) rc = gsh::launch_loss(ws, 1);
   if (rc != 0) return rc;
   if (losses != nullptr) {
-    GS_HIP_TRY(hipStreamSynchronize(v.stream));
-    gsh::sum_losses(ws, v, losses);
+    rc = gsh::launch_readback(ws, 1, true);
+    if (rc != 0) return rc;
+    GS_HIP_TRY(hipStreamSynchronize(ws->stream));
+    gsh::sum_losses(ws, 0, losses);
   }
   return 0;
 }

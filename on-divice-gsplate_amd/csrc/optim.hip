@@ -13,6 +13,7 @@
 // past the slots in use.  The slot each new gaussian lands in is therefore a
 // function of the flags alone, not of atomic order.
 #include <algorithm>
+#include <vector>
 
 #include "gs_common.h"
 #include "gs_internal.h"
@@ -20,16 +21,6 @@
 namespace {
 
 using namespace gsd;
-
-// One view's inputs to the adjoint: its camera and image size, its 2D
-// gradient planes, per-slot radius and clamp bits, and tile counts.
-struct AdamView {
-  gs_hip_camera cam;
-  const float* __restrict__ grad2d;
-  const uint32_t* __restrict__ misc;
-  const uint32_t* __restrict__ tiles;
-  float4* __restrict__ sh_input;  // per slot: (dL/dcolour rgb, update flag in view 0), (direction, -)
-};
 
 struct AdamArgs {
   float4* __restrict__ params;
@@ -39,11 +30,22 @@ struct AdamArgs {
   float* __restrict__ m_sh;
   float* __restrict__ v_sh;
   float4* __restrict__ stats;
+  const gsh::ViewDev* __restrict__ views;  // the step's views; views[0].f holds the optimizer's rates
   uint32_t capacity;
   uint32_t sh_coeffs;
   uint32_t count;  // views
-  AdamView views[GS_HIP_MAX_VIEWS];
 };
+
+// The step leaves the model alone when asked to, or when any view found more
+// pairs than its buffers hold (its render was cut short; the host runs the
+// step again with larger buffers, trainer.cpp:819-836).
+__device__ __forceinline__ bool skip_update(const AdamArgs& a) {
+  if ((a.views[0].f.flags & GS_HIP_FLAG_NO_UPDATE) != 0u) return true;
+  for (uint32_t v = 0; v < a.count; ++v) {
+    if (uint64_t(a.views[v].totals[0]) > a.views[v].pair_capacity) return true;
+  }
+  return false;
+}
 
 __device__ __forceinline__ float adam1(float p, float g, float& m, float& v, float lr, const gs_hip_frame& f) {
   m = g * (1.0f - f.beta1) + m * f.beta1;
@@ -237,9 +239,10 @@ __device__ __forceinline__ void adjoint_view(const gs_hip_camera& cam, const flo
 // gaussian was visible.  Each view's colour gradient and direction go to its
 // sh_input for k_adam_sh (one thread per SH coefficient, fully coalesced on
 // the coefficient planes, and few registers here).
-__global__ void __launch_bounds__(256) k_adam(gs_hip_frame f, AdamArgs a) {
+__global__ void __launch_bounds__(256) k_adam(AdamArgs a) {
+  const gs_hip_frame& f = a.views[0].f;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= f.slots) return;
+  if (i >= f.slots || skip_update(a)) return;
   const uint32_t degree = f.sh_degree;
   const int coeffs = min(sh_rest_coeffs(degree), int(a.sh_coeffs));
   float4 p0 = a.params[i * 4u], p1 = a.params[i * 4u + 1u], q = a.params[i * 4u + 2u], p3 = a.params[i * 4u + 3u];
@@ -251,7 +254,7 @@ __global__ void __launch_bounds__(256) k_adam(gs_hip_frame f, AdamArgs a) {
   bool drew = false, seen = false;
   float4 st = a.stats[i];
   for (uint32_t v = 0; v < a.count; ++v) {
-    const AdamView& view = a.views[v];
+    const gsh::ViewDev& view = a.views[v];
     if (view.tiles[i] == 0u) {
       if (coeffs > 0) {
         view.sh_input[size_t(i) * 2u] = make_float4(0, 0, 0, 0);
@@ -266,13 +269,13 @@ __global__ void __launch_bounds__(256) k_adam(gs_hip_frame f, AdamArgs a) {
       drew = drew || g[c] != 0.0f;
     }
     // Densification statistics: view-space positional gradient in NDC units.
-    const float gx = g[0] * (0.5f * float(view.cam.width)), gy = g[1] * (0.5f * float(view.cam.height));
+    const float gx = g[0] * (0.5f * float(view.f.cam.width)), gy = g[1] * (0.5f * float(view.f.cam.height));
     const uint32_t misc = view.misc[i];
     st.x += sqrtf(gx * gx + gy * gy);
     st.y += 1.0f;
     st.z = fmaxf(st.z, float(misc >> 3));
     float dcol[3], dir[3];
-    adjoint_view(view.cam, g, misc & 7u, p0, p1, q, a.sh, a.capacity, i, coeffs, degree, dp0, dp1, dq, dp3, dcol,
+    adjoint_view(view.f.cam, g, misc & 7u, p0, p1, q, a.sh, a.capacity, i, coeffs, degree, dp0, dp1, dq, dp3, dcol,
                  dir);
     if (coeffs > 0) {
       view.sh_input[size_t(i) * 2u] = make_float4(dcol[0], dcol[1], dcol[2], 0.0f);
@@ -326,9 +329,10 @@ __device__ __forceinline__ void adam_sh_group(float* __restrict__ sh, float* __r
 // thread per slot: g_k = mean over the views of basis_k(dir) dL/dcolour,
 // with each view's (dL/dcolour, dir) read once and the basis evaluated once
 // per view; then Adam on each coefficient plane (coalesced across the wave).
-__global__ void __launch_bounds__(256) k_adam_sh(gs_hip_frame f, AdamArgs a) {
+__global__ void __launch_bounds__(256) k_adam_sh(AdamArgs a) {
+  const gs_hip_frame& f = a.views[0].f;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= f.slots) return;
+  if (i >= f.slots || skip_update(a)) return;
   if (a.views[0].sh_input[size_t(i) * 2u].w == 0.0f) return;  // free slot, or kept by sparse Adam
   const int coeffs = min(sh_rest_coeffs(f.sh_degree), int(a.sh_coeffs));
   float g[GS_HIP_MAX_SH_REST][3];
@@ -524,12 +528,11 @@ __global__ void k_zero_counters(uint32_t* __restrict__ totals) {
 
 namespace gsh {
 
-int launch_adam(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t count) {
+int launch_adam(gs_hip_workspace* ws, uint32_t count) {
   if (count == 0 || count > ws->views.size() || count > GS_HIP_MAX_VIEWS) {
     return set_error(GS_HIP_E_ARGUMENT, "adam: view count out of range");
   }
-  const gs_hip_frame& f = frames[0];
-  if (f.slots == 0) return 0;
+  if (ws->capacity == 0) return 0;
   AdamArgs a;
   a.params = reinterpret_cast<float4*>(ws->params);
   a.m_params = reinterpret_cast<float4*>(ws->m_params);
@@ -538,27 +541,134 @@ int launch_adam(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t count
   a.m_sh = ws->m_sh;
   a.v_sh = ws->v_sh;
   a.stats = reinterpret_cast<float4*>(ws->stats);
+  a.views = ws->dev_views;
   a.capacity = ws->capacity;
   a.sh_coeffs = ws->sh_coeffs;
   a.count = count;
-  for (uint32_t v = 0; v < count; ++v) {
-    View& view = *ws->views[v];
-    a.views[v].cam = frames[v].cam;
-    a.views[v].grad2d = view.grad2d;
-    a.views[v].misc = view.misc;
-    a.views[v].tiles = view.tiles;
-    a.views[v].sh_input = reinterpret_cast<float4*>(view.sh_input);
-  }
-  View& v0 = view0(ws);
-  timer_begin(ws, v0, "adam");
-  k_adam<<<div_up(f.slots, 256), 256, 0, v0.stream>>>(f, a);
+  const uint32_t blocks = div_up(ws->capacity, 256);
+  timer_begin(ws, "adam");
+  k_adam<<<blocks, 256, 0, ws->stream>>>(a);
   GS_CHECK_LAUNCH("k_adam");
-  const int coeffs = std::min(f.sh_degree == 0 ? 0 : int((f.sh_degree + 1) * (f.sh_degree + 1)) - 1, int(ws->sh_coeffs));
-  if (coeffs > 0) {
-    k_adam_sh<<<div_up(f.slots, 256), 256, 0, v0.stream>>>(f, a);
+  if (ws->sh_coeffs > 0) {
+    k_adam_sh<<<blocks, 256, 0, ws->stream>>>(a);
     GS_CHECK_LAUNCH("k_adam_sh");
   }
-  timer_end(ws, v0);
+  timer_end(ws);
+  return 0;
+}
+
+namespace {
+
+// What a captured graph was recorded for.
+uint64_t step_signature(uint32_t count, bool depth16, bool train, bool losses) {
+  return uint64_t(count) | (uint64_t(depth16) << 8) | (uint64_t(train) << 9) | (uint64_t(losses) << 10);
+}
+
+// The stages of one batched step over views [0, count), in stream order.
+int launch_stages(gs_hip_workspace* ws, uint32_t count, bool depth16, bool train, bool losses) {
+  int rc = launch_project(ws, count);
+  if (rc == 0) rc = launch_depth_sort_and_scan(ws, count, depth16);
+  if (rc == 0) rc = launch_emit_and_tile_sort(ws, count);
+  if (rc == 0) rc = launch_forward(ws, count);
+  if (rc == 0 && (train || losses)) rc = launch_loss(ws, count);
+  if (rc == 0 && train) rc = launch_backward(ws, count);
+  if (rc == 0 && train) rc = launch_adam(ws, count);
+  if (rc == 0) rc = launch_readback(ws, count, losses);
+  return rc;
+}
+
+// Replays the captured step (capturing it first when the views, buffers,
+// launch shapes or step kind changed).
+int launch_graph(gs_hip_workspace* ws, uint32_t count, bool depth16, bool train, bool losses) {
+  const uint64_t sig = step_signature(count, depth16, train, losses);
+  if (ws->graph_exec == nullptr || ws->graph_views != count || ws->graph_pair_capacity != view0(ws).pair_capacity ||
+      ws->graph_signature != sig) {
+    drop_graph(ws);
+    GS_HIP_TRY(hipStreamBeginCapture(ws->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = launch_stages(ws, count, depth16, train, losses);
+    hipGraph_t graph = nullptr;
+    const hipError_t end = hipStreamEndCapture(ws->stream, &graph);
+    if (rc != 0) {
+      if (graph != nullptr) hipGraphDestroy(graph);
+      return rc;
+    }
+    if (end != hipSuccess) return hip_error(end, "hipStreamEndCapture");
+    ws->graph = graph;
+    GS_HIP_TRY(hipGraphInstantiate(&ws->graph_exec, ws->graph, nullptr, nullptr, 0));
+    ws->graph_views = count;
+    ws->graph_pair_capacity = view0(ws).pair_capacity;
+    ws->graph_signature = sig;
+  }
+  GS_HIP_TRY(hipGraphLaunch(ws->graph_exec, ws->stream));
+  return 0;
+}
+
+int check_frames(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t count) {
+  if (count == 0 || count > ws->views.size()) {
+    return set_error(GS_HIP_E_ARGUMENT, "step: view count exceeds the workspace's views");
+  }
+  for (uint32_t v = 0; v < count; ++v) {
+    const gs_hip_frame& f = frames[v];
+    if (f.slots > ws->capacity || f.slots != frames[0].slots) {
+      return set_error(GS_HIP_E_ARGUMENT, "step: slots differ between views or exceed capacity");
+    }
+    if (f.cam.width != ws->width || f.cam.height != ws->height) {
+      return set_error(GS_HIP_E_ARGUMENT, "step: camera size differs from the workspace");
+    }
+    if (f.tiles_x != ws->tiles_x || f.tiles_y != ws->tiles_y) {
+      return set_error(GS_HIP_E_ARGUMENT, "step: tile grid differs from the workspace");
+    }
+  }
+  return 0;
+}
+
+}  // namespace
+
+// One batched step: every stage of every view, pair counts on the device;
+// when a view's pairs overflow the buffers the optimizer skipped the update,
+// so grow the buffers and run it again (trainer.cpp:812-836).
+int run_step(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t count, const void* const* targets, bool train,
+             bool losses, uint64_t* pairs, double* loss_sums) {
+  int rc = check_frames(ws, frames, count);
+  if (rc != 0) return rc;
+  const bool depth16 = (frames[0].flags & GS_HIP_FLAG_DEPTH16) != 0u;
+  const size_t image_bytes = size_t(ws->width) * ws->height * 4;
+  if (targets != nullptr) {
+    for (uint32_t v = 0; v < count; ++v) {
+      if (targets[v] == nullptr) continue;
+      GS_HIP_TRY(hipMemcpyAsync(ws->views[v]->target, targets[v], image_bytes, hipMemcpyDeviceToDevice, ws->stream));
+    }
+  }
+  // Headroom over the largest count seen (trainer.cpp:588-592).
+  const uint64_t wanted = ws->max_pairs_seen + ws->max_pairs_seen / 8;
+  if (wanted > view0(ws).pair_capacity) {
+    rc = reserve_pairs(ws, wanted);
+    if (rc != 0) return rc;
+  }
+  for (int attempt = 0;; ++attempt) {
+    rc = upload_table(ws, frames, count);
+    if (rc != 0) return rc;
+    const bool graph = ws->tuning.graph != 0u && !ws->timer.enabled;
+    rc = graph ? launch_graph(ws, count, depth16, train, losses) : launch_stages(ws, count, depth16, train, losses);
+    if (rc != 0) return rc;
+    GS_HIP_TRY(hipStreamSynchronize(ws->stream));
+    uint64_t most = 0;
+    for (uint32_t v = 0; v < count; ++v) {
+      const uint64_t p = ws->host_totals[v * 16];
+      ws->views[v]->pairs = p;
+      most = std::max(most, p);
+    }
+    ws->max_pairs_seen = std::max(ws->max_pairs_seen, most);
+    if (most <= view0(ws).pair_capacity) break;
+    ++ws->overflow_steps;
+    if (attempt > 4) return set_error(GS_HIP_E_CAPACITY, "step: pair buffers keep overflowing");
+    rc = reserve_pairs(ws, most);
+    if (rc != 0) return rc;
+  }
+  for (uint32_t v = 0; v < count; ++v) {
+    if (pairs != nullptr) pairs[v] = ws->views[v]->pairs;
+    if (losses && loss_sums != nullptr) sum_losses(ws, v, loss_sums + 3 * v);
+  }
   return 0;
 }
 
@@ -568,7 +678,8 @@ extern "C" {
 
 int gs_hip_adam(gs_hip_workspace* ws, const gs_hip_frame* f) {
   if (ws == nullptr || f == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "adam: null argument");
-  return gsh::launch_adam(ws, f, 1);
+  const int rc = gsh::upload_table(ws, f, 1);
+  return rc != 0 ? rc : gsh::launch_adam(ws, 1);
 }
 
 int gs_hip_densify(gs_hip_workspace* ws, const gs_hip_frame* f, uint32_t* slots, uint32_t counts[3]) {
@@ -578,8 +689,7 @@ int gs_hip_densify(gs_hip_workspace* ws, const gs_hip_frame* f, uint32_t* slots,
   const uint32_t n = *slots;
   if (n == 0) return 0;
   gsh::View& v0 = gsh::view0(ws);
-  // The optimizer (view 0's stream) must be done with every view.
-  GS_HIP_TRY(hipDeviceSynchronize());
+  GS_HIP_TRY(hipStreamSynchronize(ws->stream));
   DensifyArgs a;
   a.params = reinterpret_cast<float4*>(ws->params);
   a.m_params = reinterpret_cast<float4*>(ws->m_params);
@@ -599,24 +709,26 @@ int gs_hip_densify(gs_hip_workspace* ws, const gs_hip_frame* f, uint32_t* slots,
   a.capacity = ws->capacity;
   a.sh_coeffs = ws->sh_coeffs;
   const uint32_t blocks = gsh::div_up(n, 256);
-  gsh::timer_begin(ws, v0, "densify");
-  k_zero_counters<<<1, 64, 0, v0.stream>>>(v0.totals);
+  gsh::timer_begin(ws, "densify");
+  k_zero_counters<<<1, 64, 0, ws->stream>>>(v0.totals);
   GS_CHECK_LAUNCH("k_zero_counters");
-  k_densify_mark<<<blocks, 256, 0, v0.stream>>>(*f, a);
+  k_densify_mark<<<blocks, 256, 0, ws->stream>>>(*f, a);
   GS_CHECK_LAUNCH("k_densify_mark");
-  int rc = gsh::exclusive_scan(v0, a.release, nullptr, a.free_rank, n, v0.totals + 6);
-  if (rc == 0) rc = gsh::exclusive_scan(v0, a.spawn, nullptr, a.spawn_rank, n, v0.totals + 7);
+  int rc = gsh::exclusive_scan(ws, v0, a.release, nullptr, a.free_rank, n, v0.totals + 6);
   if (rc != 0) return rc;
-  k_free_list<<<blocks, 256, 0, v0.stream>>>(n, a);
+  GS_HIP_TRY(hipStreamSynchronize(ws->stream));  // the scan segment is reused below
+  rc = gsh::exclusive_scan(ws, v0, a.spawn, nullptr, a.spawn_rank, n, v0.totals + 7);
+  if (rc != 0) return rc;
+  k_free_list<<<blocks, 256, 0, ws->stream>>>(n, a);
   GS_CHECK_LAUNCH("k_free_list");
-  k_densify_spawn<<<blocks, 256, 0, v0.stream>>>(*f, a);
+  k_densify_spawn<<<blocks, 256, 0, ws->stream>>>(*f, a);
   GS_CHECK_LAUNCH("k_densify_spawn");
-  k_zero_stats<<<gsh::div_up(ws->capacity, 256), 256, 0, v0.stream>>>(a.stats, ws->capacity);
+  k_zero_stats<<<gsh::div_up(ws->capacity, 256), 256, 0, ws->stream>>>(a.stats, ws->capacity);
   GS_CHECK_LAUNCH("k_zero_stats");
-  GS_HIP_TRY(hipMemcpyAsync(v0.host_totals, v0.totals, 32, hipMemcpyDeviceToHost, v0.stream));
-  GS_HIP_TRY(hipStreamSynchronize(v0.stream));
-  gsh::timer_end(ws, v0);
-  const uint32_t pruned = v0.host_totals[5], free_slots = v0.host_totals[6], spawned = v0.host_totals[7];
+  GS_HIP_TRY(hipMemcpyAsync(ws->host_totals, v0.totals, 32, hipMemcpyDeviceToHost, ws->stream));
+  GS_HIP_TRY(hipStreamSynchronize(ws->stream));
+  gsh::timer_end(ws);
+  const uint32_t pruned = ws->host_totals[5], free_slots = ws->host_totals[6], spawned = ws->host_totals[7];
   const uint32_t room = free_slots + (ws->capacity - n);
   const uint32_t placed = spawned < room ? spawned : room;
   if (placed > free_slots) *slots = n + (placed - free_slots);
@@ -631,8 +743,7 @@ int gs_hip_densify(gs_hip_workspace* ws, const gs_hip_frame* f, uint32_t* slots,
 int gs_hip_reset_opacity(gs_hip_workspace* ws, const gs_hip_frame* f, float max_logit) {
   if (ws == nullptr || f == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "reset_opacity: null argument");
   if (f->slots == 0) return 0;
-  gsh::View& v0 = gsh::view0(ws);
-  k_reset_opacity<<<gsh::div_up(f->slots, 256), 256, 0, v0.stream>>>(
+  k_reset_opacity<<<gsh::div_up(f->slots, 256), 256, 0, ws->stream>>>(
       f->slots, max_logit, reinterpret_cast<float4*>(ws->params), reinterpret_cast<float4*>(ws->m_params),
       reinterpret_cast<float4*>(ws->v_params));
   GS_CHECK_LAUNCH("k_reset_opacity");
@@ -641,92 +752,25 @@ int gs_hip_reset_opacity(gs_hip_workspace* ws, const gs_hip_frame* f, float max_
 
 int gs_hip_train_step(gs_hip_workspace* ws, const gs_hip_frame* f, uint64_t* pairs, double losses[3]) {
   if (ws == nullptr || f == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "train_step: null argument");
-  gsh::View& v = gsh::view0(ws);
-  int rc = gs_hip_project(ws, f);
-  if (rc == 0) rc = gs_hip_bin(ws, f, pairs);
-  if (rc == 0) rc = gsh::launch_forward(ws, v, f);
-  if (rc == 0) rc = gsh::launch_loss(ws, v, f, losses != nullptr);
-  if (rc == 0) rc = gsh::launch_backward(ws, v, f);
-  if (rc == 0) rc = gsh::launch_adam(ws, f, 1);
-  if (rc == 0 && losses != nullptr) {
-    GS_HIP_TRY(hipStreamSynchronize(v.stream));
-    gsh::sum_losses(ws, v, losses);
-  }
-  return rc;
+  return gsh::run_step(ws, f, 1, nullptr, true, losses != nullptr, pairs, losses);
 }
 
 int gs_hip_train_views(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t count,
                        const void* const* targets, uint64_t* pairs, double* losses) {
   if (ws == nullptr || frames == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "train_views: null argument");
-  if (count == 0 || count > ws->views.size()) {
-    return gsh::set_error(GS_HIP_E_ARGUMENT, "train_views: count exceeds the workspace's views");
-  }
-  for (uint32_t v = 0; v < count; ++v) {
-    const gs_hip_frame& f = frames[v];
-    if (f.slots > ws->capacity || f.slots != frames[0].slots) {
-      return gsh::set_error(GS_HIP_E_ARGUMENT, "train_views: slots differ or exceed capacity");
-    }
-    if (f.cam.width != ws->width || f.cam.height != ws->height) {
-      return gsh::set_error(GS_HIP_E_ARGUMENT, "train_views: camera size differs from the workspace");
-    }
-  }
-  const size_t image_bytes = size_t(ws->width) * ws->height * 4;
-  int rc = 0;
-  // Projection, depth sort and pair count of every view, each on its stream.
-  for (uint32_t v = 0; v < count && rc == 0; ++v) {
-    gsh::View& view = *ws->views[v];
-    if (targets != nullptr && targets[v] != nullptr) {
-      GS_HIP_TRY(hipMemcpyAsync(view.target, targets[v], image_bytes, hipMemcpyDeviceToDevice, view.stream));
-    }
-    rc = gsh::launch_project(ws, view, &frames[v]);
-    if (rc == 0) rc = gsh::bin_count(ws, view, &frames[v]);
-  }
-  if (rc != 0) return rc;
-  std::vector<uint64_t> p(count);
-  for (uint32_t v = 0; v < count; ++v) {
-    gsh::View& view = *ws->views[v];
-    GS_HIP_TRY(hipStreamSynchronize(view.stream));
-    p[v] = view.host_totals[0];
-    rc = gsh::reserve_pairs(ws, view, p[v]);
-    if (rc != 0) return rc;
-    if (pairs != nullptr) pairs[v] = p[v];
-  }
-  // The rest of every view, stage by stage across the streams.
-  for (uint32_t v = 0; v < count && rc == 0; ++v) rc = gsh::bin_pairs(ws, *ws->views[v], &frames[v], p[v]);
-  for (uint32_t v = 0; v < count && rc == 0; ++v) rc = gsh::launch_forward(ws, *ws->views[v], &frames[v]);
-  for (uint32_t v = 0; v < count && rc == 0; ++v)
-    rc = gsh::launch_loss(ws, *ws->views[v], &frames[v], losses != nullptr);
-  for (uint32_t v = 0; v < count && rc == 0; ++v) rc = gsh::launch_backward(ws, *ws->views[v], &frames[v]);
-  if (rc != 0) return rc;
-  gsh::View& v0 = gsh::view0(ws);
-  for (uint32_t v = 1; v < count; ++v) {
-    gsh::View& view = *ws->views[v];
-    GS_HIP_TRY(hipEventRecord(view.done, view.stream));
-    GS_HIP_TRY(hipStreamWaitEvent(v0.stream, view.done, 0));
-  }
-  rc = gsh::launch_adam(ws, frames, count);
-  if (rc != 0) return rc;
-  if (losses != nullptr) {
-    for (uint32_t v = 0; v < count; ++v) {
-      gsh::View& view = *ws->views[v];
-      GS_HIP_TRY(hipStreamSynchronize(view.stream));
-      gsh::sum_losses(ws, view, losses + 3 * v);
-    }
-  }
-  return 0;
+  return gsh::run_step(ws, frames, count, targets, true, losses != nullptr, pairs, losses);
 }
 
 int gs_hip_render(gs_hip_workspace* ws, const gs_hip_frame* f, uint8_t* rgba8_out) {
-  int rc = gs_hip_project(ws, f);
-  if (rc == 0) rc = gs_hip_bin(ws, f, nullptr);
-  if (rc == 0) rc = gs_hip_forward(ws, f);
+  if (ws == nullptr || f == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "render: null argument");
+  const int rc = gsh::run_step(ws, f, 1, nullptr, false, false, nullptr, nullptr);
   if (rc != 0) return rc;
   if (rgba8_out != nullptr) {
     gsh::View& v = gsh::view0(ws);
     const size_t pixels = size_t(ws->width) * ws->height;
     std::vector<float> im(pixels * 4);
-    GS_HIP_TRY(hipMemcpyAsync(im.data(), v.image, pixels * 16, hipMemcpyDeviceToHost, v.stream));
-    GS_HIP_TRY(hipStreamSynchronize(v.stream));
+    GS_HIP_TRY(hipMemcpyAsync(im.data(), v.image, pixels * 16, hipMemcpyDeviceToHost, ws->stream));
+    GS_HIP_TRY(hipStreamSynchronize(ws->stream));
     auto to8 = [](float x) {
       x = x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x);
       return uint8_t(x * 255.0f + 0.5f);

@@ -1,11 +1,14 @@
-// EWA projection of every slot: screen mean, dilated 2D covariance and its
-// conic, SH colour, opacity-aware footprint tile count, raster bbox and the
-// depth sort key.  Follows engine/shaders/train_project.comp:26-112.
+// EWA projection of every slot for every view of a step: screen mean,
+// dilated 2D covariance and its conic, SH colour, opacity-aware footprint
+// tile count, raster bbox and the depth sort key.  Follows
+// engine/shaders/train_project.comp:26-112.
 //
-// Memory: reads 64 B of parameters + 12 B per SH coefficient (coefficient-major
-// planes, so a wave reads 768 B contiguous per coefficient) and writes a
-// 48 B raster record, a 16 B footprint, tiles, key and order per slot; also
-// clears the slot's nine planar GRAD2D accumulators.  HBM-bound.
+// One thread per slot handles all the step's views, so the 64 B of
+// parameters and the 12 B per SH coefficient (coefficient-major planes: a
+// wave reads 768 B contiguous per coefficient) are read once per step, not
+// once per view.  Per view it writes a 48 B raster record, a 16 B footprint,
+// tiles, misc, key and order, and clears the nine planar GRAD2D accumulators.
+// HBM-bound.
 #include "gs_common.h"
 #include "gs_internal.h"
 
@@ -16,25 +19,21 @@ using namespace gsd;
 struct ProjectArgs {
   const float4* __restrict__ params;
   const float* __restrict__ sh;
-  float4* __restrict__ records;
-  float4* __restrict__ footprint;
-  uint32_t* __restrict__ tiles;
-  uint32_t* __restrict__ misc;
-  uint32_t* __restrict__ depth_keys;
-  uint32_t* __restrict__ depth_order;
-  float* __restrict__ grad2d;
+  const gsh::ViewDev* __restrict__ views;
   uint32_t capacity;
   uint32_t sh_coeffs;  // stored per slot
+  uint32_t count;      // views
 };
 
-__global__ void __launch_bounds__(256) k_project(gs_hip_frame f, ProjectArgs a) {
+// One view's projection of slot i (the arithmetic of the single-view
+// kernel, term for term, with contraction off: records, tile counts and keys
+// are bit-exact with the oracle's fp32 restatement).
+__device__ __forceinline__ void project_view(const gsh::ViewDev& vd, const ProjectArgs& a, uint32_t i, float4 p0,
+                                             float4 p1, float4 q, float4 p3,
+                                             const float (&rest)[GS_HIP_MAX_SH_REST][3]) {
 #pragma clang fp contract(off)
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= f.slots) return;
-  for (int c = 0; c < GS_HIP_GRAD2D_FLOATS; ++c) a.grad2d[size_t(c) * a.capacity + i] = 0.0f;
-
-  const float4 p0 = a.params[i * 4u];
-  const float4 p1 = a.params[i * 4u + 1u];
+  const gs_hip_frame& f = vd.f;
+  for (int c = 0; c < GS_HIP_GRAD2D_FLOATS; ++c) vd.grad2d[size_t(c) * a.capacity + i] = 0.0f;
   const float* R = f.cam.rotation;
   // Camera-space position, left to right sums.
   const float tx = R[0] * p0.x + R[1] * p0.y + R[2] * p0.z + f.cam.translation[0];
@@ -63,7 +62,6 @@ __global__ void __launch_bounds__(256) k_project(gs_hip_frame f, ProjectArgs a) 
     const float t00 = j00 * R[0] + j02 * R[6], t01 = j00 * R[1] + j02 * R[7], t02 = j00 * R[2] + j02 * R[8];
     const float t10 = j11 * R[3] + j12 * R[6], t11 = j11 * R[4] + j12 * R[7], t12 = j11 * R[5] + j12 * R[8];
     // Rotation of the normalised quaternion (w, x, y, z), columns scaled by exp(log scale): M = R S.
-    const float4 q = a.params[i * 4u + 2u];
     const float qlen = fmaxf(sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w), 1e-12f);
     const float w = q.x / qlen, x = q.y / qlen, y = q.z / qlen, z = q.w / qlen;
     const float sx = gs_expf(p1.x), sy = gs_expf(p1.y), sz = gs_expf(p1.z);
@@ -102,14 +100,13 @@ __global__ void __launch_bounds__(256) k_project(gs_hip_frame f, ProjectArgs a) 
         const float dlen = sqrtf(dx * dx + dy * dy + dz * dz);
         float basis[16];
         sh_basis(dx / dlen, dy / dlen, dz / dlen, f.sh_degree, basis);
-        const float4 p3 = a.params[i * 4u + 3u];
         float cr = basis[0] * p3.x, cg = basis[0] * p3.y, cbl = basis[0] * p3.z;
         const int coeffs = min(sh_rest_coeffs(f.sh_degree), int(a.sh_coeffs));
-        for (int k = 0; k < coeffs; ++k) {
-          const float* s = a.sh + (size_t(k) * a.capacity + i) * 3u;
-          cr = cr + basis[k + 1] * s[0];
-          cg = cg + basis[k + 1] * s[1];
-          cbl = cbl + basis[k + 1] * s[2];
+        for (int k = 0; k < GS_HIP_MAX_SH_REST; ++k) {
+          if (k >= coeffs) break;
+          cr = cr + basis[k + 1] * rest[k][0];
+          cg = cg + basis[k + 1] * rest[k][1];
+          cbl = cbl + basis[k + 1] * rest[k][2];
         }
         cr = cr + 0.5f;
         cg = cg + 0.5f;
@@ -131,13 +128,13 @@ __global__ void __launch_bounds__(256) k_project(gs_hip_frame f, ProjectArgs a) 
       }
     }
   }
-  float4* r = a.records + size_t(i) * 3u;
+  float4* r = vd.records + size_t(i) * 3u;
   r[0] = rec.a;
   r[1] = rec.b;
   r[2] = rec.c;
-  a.footprint[i] = foot;
-  a.tiles[i] = tiles;
-  a.misc[i] = __float_as_uint(rec.c.w);
+  vd.footprint[i] = foot;
+  vd.tiles[i] = tiles;
+  vd.misc[i] = __float_as_uint(rec.c.w);
   uint32_t key = 0xffffffffu;
   if (tiles > 0u) {
     if ((f.flags & GS_HIP_FLAG_DEPTH16) != 0u) {
@@ -148,31 +145,46 @@ __global__ void __launch_bounds__(256) k_project(gs_hip_frame f, ProjectArgs a) 
       key = __float_as_uint(tz);  // z > 0: the bits order like the values
     }
   }
-  a.depth_keys[i] = key;
-  a.depth_order[i] = i;
+  vd.depth_keys[i] = key;
+  vd.depth_order[i] = i;
+}
+
+__global__ void __launch_bounds__(256) k_project(ProjectArgs a) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.views[0].f.slots) return;
+  const float4 p0 = a.params[i * 4u], p1 = a.params[i * 4u + 1u];
+  const float4 q = a.params[i * 4u + 2u], p3 = a.params[i * 4u + 3u];
+  // SH rest of a live slot, once for all views.
+  float rest[GS_HIP_MAX_SH_REST][3];
+  for (int k = 0; k < GS_HIP_MAX_SH_REST; ++k) rest[k][0] = rest[k][1] = rest[k][2] = 0.0f;
+  if (p1.w > 0.5f) {
+    for (int k = 0; k < GS_HIP_MAX_SH_REST; ++k) {
+      if (k >= int(a.sh_coeffs)) break;
+      const float* s = a.sh + (size_t(k) * a.capacity + i) * 3u;
+      rest[k][0] = s[0];
+      rest[k][1] = s[1];
+      rest[k][2] = s[2];
+    }
+  }
+  for (uint32_t v = 0; v < a.count; ++v) project_view(a.views[v], a, i, p0, p1, q, p3, rest);
 }
 
 }  // namespace
 
 namespace gsh {
-int launch_project(gs_hip_workspace* ws, View& v, const gs_hip_frame* f) {
-  if (f->slots == 0) return 0;
+int launch_project(gs_hip_workspace* ws, uint32_t count) {
+  if (ws->capacity == 0) return 0;
   ProjectArgs a;
   a.params = reinterpret_cast<const float4*>(ws->params);
   a.sh = ws->sh;
-  a.records = reinterpret_cast<float4*>(v.records);
-  a.footprint = reinterpret_cast<float4*>(v.footprint);
-  a.tiles = v.tiles;
-  a.misc = v.misc;
-  a.depth_keys = v.depth_keys;
-  a.depth_order = v.depth_order;
-  a.grad2d = v.grad2d;
+  a.views = ws->dev_views;
   a.capacity = ws->capacity;
   a.sh_coeffs = ws->sh_coeffs;
-  timer_begin(ws, v, "project");
-  k_project<<<div_up(f->slots, 256), 256, 0, v.stream>>>(*f, a);
+  a.count = count;
+  timer_begin(ws, "project");
+  k_project<<<div_up(ws->capacity, 256), 256, 0, ws->stream>>>(a);
   GS_CHECK_LAUNCH("k_project");
-  timer_end(ws, v);
+  timer_end(ws);
   return 0;
 }
 }  // namespace gsh

@@ -25,6 +25,13 @@
 // per-splat gradients are summed over each wave by a transposed reduction
 // (one quad ends with each value), over the four waves in LDS (ds_add_f32),
 // and added to the slot's planar GRAD2D with one global float atomic each.
+//
+// Work: the (view, tile) items of all views of the step in longest-list-first
+// order.  Either one workgroup per item (the hardware dispatches them in
+// order), or a persistent grid of a few workgroups per CU that take items
+// from a device counter until none are left.
+#include <algorithm>
+
 #include "gs_common.h"
 #include "gs_internal.h"
 #include "wave.h"
@@ -38,15 +45,13 @@ constexpr int kBatch = 256;
 constexpr int kGrad = GS_HIP_GRAD2D_FLOATS;
 
 struct RasterArgs {
-  const uint2* __restrict__ ranges;
-  const uint32_t* __restrict__ tile_order;  // launch order: longest tile lists first
-  const uint32_t* __restrict__ pair_gaussians;
-  const float4* __restrict__ records;
-  float4* __restrict__ image;
-  uint32_t* __restrict__ last;
-  const float4* __restrict__ dldc;
-  float* __restrict__ grad2d;
+  const gsh::ViewDev* __restrict__ views;
+  const uint32_t* __restrict__ work;  // view * tiles + tile, longest lists first
+  uint32_t* __restrict__ counter;     // persistent mode: next work item
+  uint32_t items;                     // views * tiles
+  uint32_t tiles;
   uint32_t capacity;
+  uint32_t persistent;
 };
 
 struct PixelCoord {
@@ -80,8 +85,9 @@ struct Staging {
 
 // Loads pair k's record into slot t and returns which quadrants its bbox
 // reaches (bit q).
-__device__ __forceinline__ uint32_t stage(Staging& s, const RasterArgs& a, uint32_t g, int t, int tx0, int ty0) {
-  const float4* r = a.records + size_t(g) * 3u;
+__device__ __forceinline__ uint32_t stage(Staging& s, const float4* __restrict__ records, uint32_t g, int t, int tx0,
+                                          int ty0) {
+  const float4* r = records + size_t(g) * 3u;
   const float4 ra = r[0], rb = r[1], rc = r[2];
   s.a[t] = make_float4(ra.x, ra.y, -0.5f * ra.z, -ra.w);
   s.b[t] = make_float4(-0.5f * rb.x, rb.y, rb.z, rb.w);
@@ -122,9 +128,10 @@ __device__ __forceinline__ SplatEval eval_splat(float4 sa, float4 sb, float cx, 
   return e;
 }
 
-__global__ void __launch_bounds__(kThreads) k_forward(gs_hip_frame f, RasterArgs a) {
-  __shared__ Staging s;
-  const PixelCoord pc = pixel_coord(f, a.tile_order[blockIdx.x]);
+// Front-to-back compositing of one tile of one view.
+__device__ __forceinline__ void forward_tile(const gsh::ViewDev& a, uint32_t tile, Staging& s) {
+  const gs_hip_frame& f = a.f;
+  const PixelCoord pc = pixel_coord(f, tile);
   const uint2 range = a.ranges[pc.tile];
   const float cx = float(pc.px) + 0.5f, cy = float(pc.py) + 0.5f;
   bool done = !pc.inside;
@@ -135,7 +142,9 @@ __global__ void __launch_bounds__(kThreads) k_forward(gs_hip_frame f, RasterArgs
     if (__syncthreads_and(done)) break;
     const uint32_t n = min(uint32_t(kBatch), range.y - start);
     uint32_t hits = 0u;
-    if (threadIdx.x < n) hits = stage(s, a, a.pair_gaussians[start + threadIdx.x], threadIdx.x, pc.tx0, pc.ty0);
+    if (threadIdx.x < n) {
+      hits = stage(s, a.records, a.pair_gaussians[start + threadIdx.x], threadIdx.x, pc.tx0, pc.ty0);
+    }
     publish_masks(s, hits, pc.wave, pc.lane);
     __syncthreads();
     if (__ballot(!done) != 0ull) {
@@ -178,16 +187,24 @@ __global__ void __launch_bounds__(kThreads) k_forward(gs_hip_frame f, RasterArgs
   }
 }
 
-__global__ void __launch_bounds__(kThreads) k_backward(gs_hip_frame f, RasterArgs a) {
-  __shared__ Staging s;
-  __shared__ uint32_t s_g[kBatch];
-  __shared__ float s_grad[kBatch * kGrad];  // [splat][9]: the nine lanes of one reduction hit nine banks
-  __shared__ uint32_t s_wave_last[4];
-  const PixelCoord pc = pixel_coord(f, a.tile_order[blockIdx.x]);
+struct BackwardShared {
+  Staging s;
+  uint32_t g[kBatch];
+  float grad[kBatch * kGrad];  // [splat][9]: the nine lanes of one reduction hit nine banks
+  uint32_t wave_last[4];
+};
+
+// Recompute-transmittance adjoint of one tile of one view.
+__device__ __forceinline__ void backward_tile(const gsh::ViewDev& a, uint32_t tile, BackwardShared& sh,
+                                              uint32_t capacity, const Reduce9& red, int slot) {
+  const gs_hip_frame& f = a.f;
+  Staging& s = sh.s;
+  uint32_t* s_g = sh.g;
+  float* s_grad = sh.grad;
+  uint32_t* s_wave_last = sh.wave_last;
+  const PixelCoord pc = pixel_coord(f, tile);
   const uint2 range = a.ranges[pc.tile];
   const float cx = float(pc.px) + 0.5f, cy = float(pc.py) + 0.5f;
-  const Reduce9 red;
-  const int slot = red.slot();
 
   float T_final = 0.0f, gr = 0.0f, gg = 0.0f, gb = 0.0f;
   uint32_t last = 0u;
@@ -219,7 +236,7 @@ __global__ void __launch_bounds__(kThreads) k_backward(gs_hip_frame f, RasterArg
       const uint32_t g = a.pair_gaussians[k];
       s_g[threadIdx.x] = g;
       for (int c = 0; c < kGrad; ++c) s_grad[threadIdx.x * kGrad + c] = 0.0f;
-      hits = stage(s, a, g, threadIdx.x, pc.tx0, pc.ty0);
+      hits = stage(s, a.records, g, threadIdx.x, pc.tx0, pc.ty0);
       const uint32_t pos = k - range.x;
       for (int q = 0; q < 4; ++q)
         if (pos >= s_wave_last[q]) hits &= ~(1u << q);
@@ -277,7 +294,7 @@ __global__ void __launch_bounds__(kThreads) k_backward(gs_hip_frame f, RasterArg
       const uint32_t g = s_g[threadIdx.x];
       for (int c = 0; c < kGrad; ++c) {
         const float v = s_grad[threadIdx.x * kGrad + c];
-        if (v != 0.0f) unsafeAtomicAdd(&a.grad2d[size_t(c) * a.capacity + g], v);
+        if (v != 0.0f) unsafeAtomicAdd(&a.grad2d[size_t(c) * capacity + g], v);
       }
     }
     __syncthreads();
@@ -285,17 +302,55 @@ __global__ void __launch_bounds__(kThreads) k_backward(gs_hip_frame f, RasterArg
   }
 }
 
-RasterArgs make_args(gs_hip_workspace* ws, gsh::View& v) {
+// Both kernels: one workgroup per work item, or (persistent) a grid that
+// takes items from the counter until they run out; the exit test reads the
+// shared item, so the whole workgroup leaves together.
+__device__ __forceinline__ uint32_t next_item(const RasterArgs& a, uint32_t* s_item) {
+  if (threadIdx.x == 0) *s_item = atomicAdd(a.counter, 1u);
+  __syncthreads();
+  const uint32_t item = __builtin_amdgcn_readfirstlane(*s_item);  // uniform: scalar loads from here on
+  __syncthreads();
+  return item;
+}
+
+__global__ void __launch_bounds__(kThreads) k_forward(RasterArgs a) {
+  __shared__ Staging s;
+  __shared__ uint32_t s_item;
+  uint32_t item = a.persistent != 0u ? next_item(a, &s_item) : blockIdx.x;
+  while (item < a.items) {
+    const uint32_t w = a.work[item];
+    forward_tile(a.views[w / a.tiles], w % a.tiles, s);
+    if (a.persistent == 0u) break;
+    __syncthreads();
+    item = next_item(a, &s_item);
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_backward(RasterArgs a) {
+  __shared__ BackwardShared sh;
+  __shared__ uint32_t s_item;
+  const Reduce9 red;
+  const int slot = red.slot();
+  uint32_t item = a.persistent != 0u ? next_item(a, &s_item) : blockIdx.x;
+  while (item < a.items) {
+    const uint32_t w = a.work[item];
+    backward_tile(a.views[w / a.tiles], w % a.tiles, sh, a.capacity, red, slot);
+    if (a.persistent == 0u) break;
+    __syncthreads();
+    item = next_item(a, &s_item);
+  }
+}
+
+RasterArgs make_args(gs_hip_workspace* ws, uint32_t count, int which, uint32_t& grid) {
   RasterArgs a;
-  a.ranges = reinterpret_cast<const uint2*>(v.ranges);
-  a.tile_order = v.tile_order;
-  a.pair_gaussians = v.pair_gaussians;
-  a.records = reinterpret_cast<const float4*>(v.records);
-  a.image = reinterpret_cast<float4*>(v.image);
-  a.last = v.last;
-  a.dldc = reinterpret_cast<const float4*>(v.dldc);
-  a.grad2d = v.grad2d;
+  a.views = ws->dev_views;
+  a.work = ws->work;
+  a.counter = ws->work_counters + which;
+  a.tiles = gsh::tile_count(ws);
+  a.items = count * a.tiles;
   a.capacity = ws->capacity;
+  a.persistent = ws->tuning.raster_groups_per_cu > 0 ? 1u : 0u;
+  grid = a.persistent ? std::min<uint32_t>(a.items, ws->cus * ws->tuning.raster_groups_per_cu) : a.items;
   return a;
 }
 
@@ -303,21 +358,33 @@ RasterArgs make_args(gs_hip_workspace* ws, gsh::View& v) {
 
 namespace gsh {
 
-int launch_forward(gs_hip_workspace* ws, View& v, const gs_hip_frame* f) {
-  const uint32_t tiles = f->tiles_x * f->tiles_y;
-  timer_begin(ws, v, "forward");
-  k_forward<<<tiles, kThreads, 0, v.stream>>>(*f, make_args(ws, v));
+int launch_forward(gs_hip_workspace* ws, uint32_t count) {
+  uint32_t grid = 0;
+  const RasterArgs a = make_args(ws, count, 0, grid);
+  if (grid == 0) return 0;
+  if (a.persistent != 0u) {
+    const int rc = fill_u32(ws, a.counter, 1, 0u);
+    if (rc != 0) return rc;
+  }
+  timer_begin(ws, "forward");
+  k_forward<<<grid, kThreads, 0, ws->stream>>>(a);
   GS_CHECK_LAUNCH("k_forward");
-  timer_end(ws, v);
+  timer_end(ws);
   return 0;
 }
 
-int launch_backward(gs_hip_workspace* ws, View& v, const gs_hip_frame* f) {
-  const uint32_t tiles = f->tiles_x * f->tiles_y;
-  timer_begin(ws, v, "backward");
-  k_backward<<<tiles, kThreads, 0, v.stream>>>(*f, make_args(ws, v));
+int launch_backward(gs_hip_workspace* ws, uint32_t count) {
+  uint32_t grid = 0;
+  const RasterArgs a = make_args(ws, count, 1, grid);
+  if (grid == 0) return 0;
+  if (a.persistent != 0u) {
+    const int rc = fill_u32(ws, a.counter, 1, 0u);
+    if (rc != 0) return rc;
+  }
+  timer_begin(ws, "backward");
+  k_backward<<<grid, kThreads, 0, ws->stream>>>(a);
   GS_CHECK_LAUNCH("k_backward");
-  timer_end(ws, v);
+  timer_end(ws);
   return 0;
 }
 
@@ -327,12 +394,14 @@ extern "C" {
 
 int gs_hip_forward(gs_hip_workspace* ws, const gs_hip_frame* f) {
   if (ws == nullptr || f == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "forward: null argument");
-  return gsh::launch_forward(ws, gsh::view0(ws), f);
+  const int rc = gsh::upload_table(ws, f, 1);
+  return rc != 0 ? rc : gsh::launch_forward(ws, 1);
 }
 
 int gs_hip_backward(gs_hip_workspace* ws, const gs_hip_frame* f) {
   if (ws == nullptr || f == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "backward: null argument");
-  return gsh::launch_backward(ws, gsh::view0(ws), f);
+  const int rc = gsh::upload_table(ws, f, 1);
+  return rc != 0 ? rc : gsh::launch_backward(ws, 1);
 }
 
 }  // extern "C"

@@ -1,17 +1,25 @@
-// Binning: depth sort of the slots, exclusive scan of their tile counts in
-// depth order, pair emission, stable tile sort of the pairs and per-tile
-// ranges.  The order this produces is that of sorting 64-bit keys
+// Binning of every view of a step, one launch per stage: depth sort of the
+// slots, exclusive scan of their tile counts in depth order, pair emission,
+// stable tile sort of the pairs, per-tile ranges, and the rasterizers' work
+// order.  The order this produces is that of sorting 64-bit keys
 // (tile << 32) | depth_key with ties broken by the gaussian index: the slots
 // are sorted stably by depth key, each emits its pairs at its offset in that
 // order, and a stable sort by tile alone then leaves each tile's pairs front
 // to back (engine/src/trainer.cpp:665-711, train_emit.comp:3-9).
 //
-// The radix sort is LSD with up to 8-bit digits.  Per pass: an upsweep kernel
-// counts digits per block of 4096 items (per-wave LDS histograms), a scan
-// kernel turns the [digit][block] counts into offsets, and a scatter kernel
-// ranks every item stably inside its block with wave64 ballots (the lanes
-// holding the same digit, and the rank as the popcount of those below) plus
-// per-wave digit counts in LDS.
+// Counts stay on the device: a sort or scan segment reads its length from
+// device memory (the frame's slot count, the pair total) and its launch is
+// sized for the buffer's capacity, so no step waits for the host between
+// stages.  Pairs beyond the pair buffers are dropped and the step is flagged
+// as overflowed (the optimizer then leaves the model alone and the host runs
+// the step again with larger buffers, as trainer.cpp:819-836 does).
+//
+// The radix sort is LSD with 4..8-bit digits; a block owns 4096 items.  Per
+// pass: an upsweep counts the block's digits (per-wave LDS histograms), a
+// scan turns the [digit][block] counts into offsets, and a scatter ranks the
+// block's items stably (wave64 ballots: the lanes holding the same digit,
+// rank = popcount of those below; per-wave digit counts in LDS), places them
+// in LDS in digit order, and writes each digit's run out contiguously.
 #include <algorithm>
 
 #include "gs_common.h"
@@ -20,11 +28,19 @@
 namespace {
 
 using namespace gsd;
+using gsh::ScanSeg;
+using gsh::SortSeg;
+using gsh::ViewDev;
+using gsh::div_up;
 
 constexpr int kSortThreads = 256;
 constexpr int kSortRounds = 16;
 constexpr int kSortItems = kSortThreads * kSortRounds;  // 4096 per block
 constexpr int kWaves = kSortThreads / 64;
+
+__device__ __forceinline__ uint32_t seg_count(const uint32_t* count, uint32_t count_max) {
+  return count != nullptr ? min(*count, count_max) : count_max;
+}
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
   const uint32_t lane = __lane_id();
@@ -42,10 +58,14 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t digit, int nbits, uint6
   return peers;
 }
 
-__global__ void __launch_bounds__(kSortThreads) k_radix_upsweep(const uint32_t* __restrict__ keys, uint32_t n,
-                                                                int shift, int nbits, uint32_t* __restrict__ hist,
-                                                                uint32_t blocks) {
+// Histogram of block (blockIdx.x) of segment blockIdx.y; blocks past the
+// segment's length write zeros.
+__global__ void __launch_bounds__(kSortThreads) k_radix_upsweep(const SortSeg* __restrict__ segs, int shift,
+                                                                int nbits, int from_alt, uint32_t blocks) {
   __shared__ uint32_t h[kWaves][256];
+  const SortSeg& g = segs[blockIdx.y];
+  const uint32_t n = seg_count(g.count, g.count_max);
+  const uint32_t* __restrict__ keys = from_alt ? g.keys_alt : g.keys;
   const uint32_t radix = 1u << nbits, mask = radix - 1u;
   const int wave = threadIdx.x / 64;
   for (int d = threadIdx.x; d < kWaves * 256; d += kSortThreads) (&h[0][0])[d] = 0u;
@@ -59,17 +79,17 @@ __global__ void __launch_bounds__(kSortThreads) k_radix_upsweep(const uint32_t* 
   for (uint32_t d = threadIdx.x; d < radix; d += kSortThreads) {
     uint32_t s = 0;
     for (int w = 0; w < kWaves; ++w) s += h[w][d];
-    hist[size_t(d) * blocks + blockIdx.x] = s;
+    g.hist[size_t(d) * blocks + blockIdx.x] = s;
   }
 }
 
-// One block per digit: exclusive scan of that digit's per-block counts, and
-// the digit's total.
-__global__ void __launch_bounds__(256) k_radix_scan(uint32_t* __restrict__ hist, uint32_t blocks,
-                                                    uint32_t* __restrict__ digit_totals) {
+// One block per (digit, segment): exclusive scan of that digit's per-block
+// counts, and the digit's total.
+__global__ void __launch_bounds__(256) k_radix_scan(const SortSeg* __restrict__ segs, uint32_t blocks) {
   __shared__ uint32_t s_wave[4];
   __shared__ uint32_t s_carry;
-  uint32_t* row = hist + size_t(blockIdx.x) * blocks;
+  const SortSeg& g = segs[blockIdx.y];
+  uint32_t* row = g.hist + size_t(blockIdx.x) * blocks;
   if (threadIdx.x == 0) s_carry = 0u;
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -90,55 +110,79 @@ __global__ void __launch_bounds__(256) k_radix_scan(uint32_t* __restrict__ hist,
     if (threadIdx.x == 0) s_carry += s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
     __syncthreads();
   }
-  if (threadIdx.x == 0) digit_totals[blockIdx.x] = s_carry;
+  if (threadIdx.x == 0) g.digit_totals[blockIdx.x] = s_carry;
 }
 
-__global__ void __launch_bounds__(kSortThreads) k_radix_scatter(const uint32_t* __restrict__ keys_in,
-                                                                const uint32_t* __restrict__ vals_in,
-                                                                uint32_t* __restrict__ keys_out,
-                                                                uint32_t* __restrict__ vals_out, uint32_t n, int shift,
-                                                                int nbits, const uint32_t* __restrict__ hist,
-                                                                const uint32_t* __restrict__ digit_totals,
-                                                                uint32_t blocks) {
-  __shared__ uint32_t s_running[256];
-  __shared__ uint32_t s_counts[kWaves][256];
-  const uint32_t radix = 1u << nbits, mask = radix - 1u;
-  const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
-  // Start of each digit: the digits below it (a block scan of the totals),
-  // plus this digit's items in the blocks before this one.
-  {
-    const uint32_t v = threadIdx.x < radix ? digit_totals[threadIdx.x] : 0u;
-    uint32_t incl = v;
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t t = __shfl_up(incl, o, 64);
-      if (lane >= o) incl += t;
-    }
-    if (lane == 63) s_counts[0][wave] = incl;
-    __syncthreads();
-    uint32_t below = incl - v;
-    for (int w = 0; w < wave; ++w) below += s_counts[0][w];
-    s_running[threadIdx.x] = threadIdx.x < radix ? below + hist[size_t(threadIdx.x) * blocks + blockIdx.x] : 0u;
-    __syncthreads();
+// Exclusive scan of one value per thread of a 256-thread block into out[thread].
+__device__ __forceinline__ void block_scan_256(uint32_t v, uint32_t* s_tmp, uint32_t* out) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t incl = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += t;
   }
+  if (lane == 63) s_tmp[wave] = incl;
+  __syncthreads();
+  uint32_t below = incl - v;
+  for (int w = 0; w < wave; ++w) below += s_tmp[w];
+  out[threadIdx.x] = below;
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(kSortThreads) k_radix_scatter(const SortSeg* __restrict__ segs, int shift,
+                                                                int nbits, int from_alt, uint32_t blocks) {
+  __shared__ uint32_t s_keys[kSortItems];
+  __shared__ uint32_t s_vals[kSortItems];
+  __shared__ uint32_t s_counts[kWaves][256];
+  __shared__ uint32_t s_local[256];    // block-local start of each digit
+  __shared__ uint32_t s_running[256];  // items of each digit placed so far
+  __shared__ uint32_t s_global[256];   // where the block's run of each digit starts in the output
+  __shared__ uint32_t s_tmp[kWaves];
+  const SortSeg& g = segs[blockIdx.y];
+  const uint32_t n = seg_count(g.count, g.count_max);
+  const uint32_t base = blockIdx.x * kSortItems;
+  if (base >= n) return;
+  const uint32_t* __restrict__ keys_in = from_alt ? g.keys_alt : g.keys;
+  const uint32_t* __restrict__ vals_in = from_alt ? g.vals_alt : g.vals;
+  uint32_t* __restrict__ keys_out = from_alt ? g.keys : g.keys_alt;
+  uint32_t* __restrict__ vals_out = from_alt ? g.vals : g.vals_alt;
+  const uint32_t radix = 1u << nbits, mask = radix - 1u;
+  const int wave = threadIdx.x / 64;
+  // Output start of each digit: the digits below it over the whole segment
+  // (a block scan of the totals) plus this digit's items in earlier blocks.
+  block_scan_256(threadIdx.x < radix ? g.digit_totals[threadIdx.x] : 0u, s_tmp, s_global);
+  if (threadIdx.x < radix) s_global[threadIdx.x] += g.hist[size_t(threadIdx.x) * blocks + blockIdx.x];
+  // The block's items, in registers; its digit histogram in LDS.
+  uint32_t key[kSortRounds], val[kSortRounds];
   for (int d = threadIdx.x; d < kWaves * 256; d += kSortThreads) (&s_counts[0][0])[d] = 0u;
   __syncthreads();
-  const uint32_t base = blockIdx.x * kSortItems;
+  for (int r = 0; r < kSortRounds; ++r) {
+    const uint32_t idx = base + r * kSortThreads + threadIdx.x;
+    key[r] = idx < n ? keys_in[idx] : 0u;
+    val[r] = idx < n ? vals_in[idx] : 0u;
+    if (idx < n) atomicAdd(&s_counts[0][(key[r] >> shift) & mask], 1u);
+  }
+  __syncthreads();
+  const uint32_t block_count = threadIdx.x < radix ? s_counts[0][threadIdx.x] : 0u;
+  block_scan_256(block_count, s_tmp, s_local);
+  s_running[threadIdx.x] = 0u;
+  s_counts[0][threadIdx.x] = 0u;
+  __syncthreads();
+  // Stable rank of every item inside the block -> its place in LDS.
   const uint64_t below_me = lanemask_lt();
   for (int r = 0; r < kSortRounds; ++r) {
     const uint32_t idx = base + r * kSortThreads + threadIdx.x;
     const bool valid = idx < n;
-    const uint32_t key = valid ? keys_in[idx] : 0u;
-    const uint32_t val = valid ? vals_in[idx] : 0u;
-    const uint32_t digit = (key >> shift) & mask;
+    const uint32_t digit = (key[r] >> shift) & mask;
     const uint64_t peers = match_digit(digit, nbits, __ballot(valid));
     const uint32_t rank = __popcll(peers & below_me);
     if (valid && rank == 0u) s_counts[wave][digit] = __popcll(peers);
     __syncthreads();
     if (valid) {
-      uint32_t pos = s_running[digit] + rank;
+      uint32_t pos = s_local[digit] + s_running[digit] + rank;
       for (int w = 0; w < wave; ++w) pos += s_counts[w][digit];
-      keys_out[pos] = key;
-      vals_out[pos] = val;
+      s_keys[pos] = key[r];
+      s_vals[pos] = val[r];
     }
     __syncthreads();
     if (threadIdx.x < radix) {
@@ -151,6 +195,25 @@ __global__ void __launch_bounds__(kSortThreads) k_radix_scatter(const uint32_t* 
     }
     __syncthreads();
   }
+  // Each digit's run leaves contiguously: consecutive threads, consecutive addresses.
+  const uint32_t items = min(uint32_t(kSortItems), n - base);
+  for (uint32_t i = threadIdx.x; i < items; i += kSortThreads) {
+    const uint32_t k = s_keys[i];
+    const uint32_t digit = (k >> shift) & mask;
+    const uint32_t pos = s_global[digit] + (i - s_local[digit]);
+    keys_out[pos] = k;
+    vals_out[pos] = s_vals[i];
+  }
+}
+
+// After an odd number of passes: the result back into (keys, vals).
+__global__ void __launch_bounds__(256) k_radix_copy_back(const SortSeg* __restrict__ segs) {
+  const SortSeg& g = segs[blockIdx.y];
+  const uint32_t n = seg_count(g.count, g.count_max);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    g.keys[i] = g.keys_alt[i];
+    g.vals[i] = g.vals_alt[i];
+  }
 }
 
 // ---- exclusive scan ----------------------------------------------------------
@@ -158,31 +221,31 @@ constexpr int kScanThreads = 256;
 constexpr int kScanRounds = 16;
 constexpr int kScanItems = kScanThreads * kScanRounds;
 
-__device__ __forceinline__ uint32_t load_value(const uint32_t* __restrict__ values, const uint32_t* __restrict__ gather,
-                                               uint32_t idx) {
-  return gather != nullptr ? values[gather[idx]] : values[idx];
+__device__ __forceinline__ uint32_t load_value(const ScanSeg& s, uint32_t idx) {
+  return s.gather != nullptr ? s.values[s.gather[idx]] : s.values[idx];
 }
 
-__global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const uint32_t* __restrict__ values,
-                                                              const uint32_t* __restrict__ gather, uint32_t count,
-                                                              uint32_t* __restrict__ block_sums) {
+__global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const ScanSeg* __restrict__ segs) {
   __shared__ uint32_t s[kScanThreads / 64];
+  const ScanSeg& g = segs[blockIdx.y];
+  const uint32_t count = seg_count(g.count, g.count_max);
   uint32_t sum = 0;
   const uint32_t base = blockIdx.x * kScanItems;
   for (int r = 0; r < kScanRounds; ++r) {
     const uint32_t idx = base + r * kScanThreads + threadIdx.x;
-    if (idx < count) sum += load_value(values, gather, idx);
+    if (idx < count) sum += load_value(g, idx);
   }
   for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
   if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = sum;
   __syncthreads();
-  if (threadIdx.x == 0) block_sums[blockIdx.x] = s[0] + s[1] + s[2] + s[3];
+  if (threadIdx.x == 0) g.block_sums[blockIdx.x] = s[0] + s[1] + s[2] + s[3];
 }
 
-__global__ void __launch_bounds__(1024) k_scan_blocks(uint32_t* __restrict__ block_sums, uint32_t blocks,
-                                                      uint32_t* __restrict__ total) {
+__global__ void __launch_bounds__(1024) k_scan_blocks(const ScanSeg* __restrict__ segs, uint32_t blocks) {
   __shared__ uint32_t s_wave[16];
   __shared__ uint32_t s_carry;
+  const ScanSeg& g = segs[blockIdx.y];
+  uint32_t* block_sums = g.block_sums;
   if (threadIdx.x == 0) s_carry = 0u;
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -207,22 +270,22 @@ __global__ void __launch_bounds__(1024) k_scan_blocks(uint32_t* __restrict__ blo
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) *total = s_carry;
+  if (threadIdx.x == 0) *g.total = s_carry;
 }
 
-__global__ void __launch_bounds__(kScanThreads) k_scan_down(const uint32_t* __restrict__ values,
-                                                            const uint32_t* __restrict__ gather, uint32_t count,
-                                                            const uint32_t* __restrict__ block_sums,
-                                                            uint32_t* __restrict__ out) {
+__global__ void __launch_bounds__(kScanThreads) k_scan_down(const ScanSeg* __restrict__ segs) {
   __shared__ uint32_t s_wave[kScanThreads / 64];
   __shared__ uint32_t s_carry;
-  if (threadIdx.x == 0) s_carry = block_sums[blockIdx.x];
+  const ScanSeg& g = segs[blockIdx.y];
+  const uint32_t count = seg_count(g.count, g.count_max);
+  const uint32_t base = blockIdx.x * kScanItems;
+  if (base >= count) return;
+  if (threadIdx.x == 0) s_carry = g.block_sums[blockIdx.x];
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t base = blockIdx.x * kScanItems;
   for (int r = 0; r < kScanRounds; ++r) {
     const uint32_t idx = base + r * kScanThreads + threadIdx.x;
-    const uint32_t v = idx < count ? load_value(values, gather, idx) : 0u;
+    const uint32_t v = idx < count ? load_value(g, idx) : 0u;
     uint32_t incl = v;
     for (int o = 1; o < 64; o <<= 1) {
       const uint32_t t = __shfl_up(incl, o, 64);
@@ -232,7 +295,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_down(const uint32_t* __re
     __syncthreads();
     uint32_t before = s_carry;
     for (int w = 0; w < wave; ++w) before += s_wave[w];
-    if (idx < count) out[idx] = before + incl - v;
+    if (idx < count) g.out[idx] = before + incl - v;
     __syncthreads();
     if (threadIdx.x == 0) s_carry += s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
     __syncthreads();
@@ -242,24 +305,21 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_down(const uint32_t* __re
 // ---- pair emission and ranges ------------------------------------------------
 // One thread per slot in depth order: writes the slot's pairs (tile, slot) at
 // its offset, row by row over the same footprint the projection counted
-// (train_emit.comp:26-64).  A count that differs (it cannot: same code, same
+// (train_emit.comp:26-64).  Pairs past the buffer are dropped (the step is
+// then an overflow); a count that differs (it cannot: same code, same
 // inputs) is padded with the unused tile `tiles` and counted in totals[1].
-__global__ void __launch_bounds__(256) k_emit(gs_hip_frame f, const float4* __restrict__ records,
-                                              const float4* __restrict__ footprint,
-                                              const uint32_t* __restrict__ tiles,
-                                              const uint32_t* __restrict__ order,
-                                              const uint32_t* __restrict__ offsets, uint32_t* __restrict__ pair_tiles,
-                                              uint32_t* __restrict__ pair_gaussians, uint64_t pair_capacity,
-                                              uint32_t* __restrict__ totals) {
+__global__ void __launch_bounds__(256) k_emit(const ViewDev* __restrict__ views) {
+  const ViewDev& v = views[blockIdx.y];
+  const gs_hip_frame& f = v.f;
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= f.slots) return;
-  const uint32_t i = order[k];
-  const uint32_t count = tiles[i];
+  const uint32_t i = v.depth_order[k];
+  const uint32_t count = v.tiles[i];
   if (count == 0u) return;
-  const float4 r0 = records[size_t(i) * 3u];
-  const float4 fp = footprint[i];
-  uint64_t p = offsets[k];
-  const uint64_t end = std::min<uint64_t>(p + count, pair_capacity);
+  const float4 r0 = v.records[size_t(i) * 3u];
+  const float4 fp = v.footprint[i];
+  uint64_t p = v.offsets[k];
+  const uint64_t end = std::min<uint64_t>(p + count, v.pair_capacity);
   uint32_t wanted = 0;
   int row_first, row_last;
   footprint_rows(r0.y, fp.z, fp.w, f.cam.height, row_first, row_last);
@@ -269,51 +329,59 @@ __global__ void __launch_bounds__(256) k_emit(gs_hip_frame f, const float4* __re
       footprint_row_tiles(r0.x, r0.y, fp.x, fp.y, fp.z, fp.w, row_first, row_last, ty, f.cam.width, x0, x1);
       wanted += x1 - x0;
       for (uint32_t x = x0; x < x1 && p < end; ++x, ++p) {
-        pair_tiles[p] = ty * f.tiles_x + x;
-        pair_gaussians[p] = i;
+        v.pair_tiles[p] = ty * f.tiles_x + x;
+        v.pair_gaussians[p] = i;
       }
     }
   }
   if (wanted == count) return;
-  atomicAdd(&totals[1], 1u);
+  atomicAdd(&v.totals[1], 1u);
   for (; p < end; ++p) {
-    pair_tiles[p] = f.tiles_x * f.tiles_y;
-    pair_gaussians[p] = i;
+    v.pair_tiles[p] = f.tiles_x * f.tiles_y;
+    v.pair_gaussians[p] = i;
   }
 }
 
-__global__ void __launch_bounds__(256) k_fill_ranges(uint32_t* __restrict__ ranges, uint32_t entries) {
+__global__ void __launch_bounds__(256) k_clear_ranges(const ViewDev* __restrict__ views, uint32_t entries) {
+  const ViewDev& v = views[blockIdx.y];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < entries) ranges[i] = 0u;
+  if (i < entries) v.ranges[i] = make_uint2(0u, 0u);
+  if (i == 0u) v.totals[1] = 0u;
 }
 
-__global__ void __launch_bounds__(256) k_ranges(const uint32_t* __restrict__ keys, uint32_t count,
-                                                uint2* __restrict__ ranges) {
+__global__ void __launch_bounds__(256) k_ranges(const ViewDev* __restrict__ views) {
+  const ViewDev& v = views[blockIdx.y];
+  const uint32_t count = uint32_t(std::min<uint64_t>(v.totals[0], v.pair_capacity));
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= count) return;
-  const uint32_t tile = keys[s];
+  const uint32_t tile = v.pair_tiles[s];
   if (s == 0u) {
-    ranges[tile].x = 0u;
+    v.ranges[tile].x = 0u;
   } else {
-    const uint32_t prev = keys[s - 1u];
+    const uint32_t prev = v.pair_tiles[s - 1u];
     if (prev != tile) {
-      ranges[prev].y = s;
-      ranges[tile].x = s;
+      v.ranges[prev].y = s;
+      v.ranges[tile].x = s;
     }
   }
-  if (s == count - 1u) ranges[tile].y = count;
+  if (s == count - 1u) v.ranges[tile].y = count;
 }
 
-// Launch order of the rasterizers: tiles bucketed by floor(log2(list
-// length)), longest first, empty tiles last (longest-processing-time-first
-// scheduling within a factor of two).  One workgroup.
-__global__ void __launch_bounds__(1024) k_tile_order(const uint2* __restrict__ ranges, uint32_t tiles,
-                                                     uint32_t* __restrict__ order) {
+// The rasterizers' work: every (view, tile) of the step, bucketed by
+// floor(log2(list length)), longest first, empty tiles last
+// (longest-processing-time-first scheduling within a factor of two, across
+// all views).  One workgroup; also resets the persistent rasterizers' work
+// counters.
+__global__ void __launch_bounds__(1024) k_work_order(const ViewDev* __restrict__ views, uint32_t count,
+                                                     uint32_t tiles, uint32_t* __restrict__ work,
+                                                     uint32_t* __restrict__ counters) {
   __shared__ uint32_t s_count[33];
   if (threadIdx.x < 33) s_count[threadIdx.x] = 0u;
+  if (threadIdx.x < 2) counters[threadIdx.x] = 0u;
   __syncthreads();
-  for (uint32_t t = threadIdx.x; t < tiles; t += blockDim.x) {
-    const uint2 r = ranges[t];
+  const uint32_t items = count * tiles;
+  for (uint32_t t = threadIdx.x; t < items; t += blockDim.x) {
+    const uint2 r = views[t / tiles].ranges[t % tiles];
     atomicAdd(&s_count[__clz(r.y - r.x)], 1u);
   }
   __syncthreads();
@@ -326,9 +394,9 @@ __global__ void __launch_bounds__(1024) k_tile_order(const uint2* __restrict__ r
     }
   }
   __syncthreads();
-  for (uint32_t t = threadIdx.x; t < tiles; t += blockDim.x) {
-    const uint2 r = ranges[t];
-    order[atomicAdd(&s_count[__clz(r.y - r.x)], 1u)] = t;
+  for (uint32_t t = threadIdx.x; t < items; t += blockDim.x) {
+    const uint2 r = views[t / tiles].ranges[t % tiles];
+    work[atomicAdd(&s_count[__clz(r.y - r.x)], 1u)] = t;
   }
 }
 
@@ -338,112 +406,89 @@ int bits_for(uint32_t n) {
   return bits;
 }
 
-}  // namespace
-
-namespace gsh {
-
-int reserve_sort(View& v, uint64_t items) {
-  SortScratch& s = v.sort;
-  if (items <= s.items) return 0;
-  if (s.keys_alt) hipFree(s.keys_alt);
-  if (s.vals_alt) hipFree(s.vals_alt);
-  if (s.hist) hipFree(s.hist);
-  if (s.digit_totals) hipFree(s.digit_totals);
-  s = SortScratch{};
-  const uint64_t blocks = (items + kSortItems - 1) / kSortItems;
-  GS_HIP_TRY(hipMalloc(reinterpret_cast<void**>(&s.keys_alt), items * 4));
-  GS_HIP_TRY(hipMalloc(reinterpret_cast<void**>(&s.vals_alt), items * 4));
-  GS_HIP_TRY(hipMalloc(reinterpret_cast<void**>(&s.hist), std::max<uint64_t>(blocks, 1) * 256 * 4));
-  GS_HIP_TRY(hipMalloc(reinterpret_cast<void**>(&s.digit_totals), 256 * 4));
-  s.items = items;
-  return 0;
-}
-
-int radix_sort(View& v, uint32_t* keys, uint32_t* vals, uint32_t n, int bits) {
-  if (n <= 1 || bits <= 0) return 0;
-  if (n > v.sort.items) return set_error(GS_HIP_E_CAPACITY, "radix_sort: scratch too small");
-  const uint32_t blocks = div_up(n, kSortItems);
-  const int passes = (bits + 7) / 8;
-  uint32_t *ka = keys, *va = vals, *kb = v.sort.keys_alt, *vb = v.sort.vals_alt;
+// Batched LSD sort of `count` segments by key bits [0, bits), `digit` bits per pass.
+int radix_sort(gs_hip_workspace* ws, const SortSeg* segs, uint32_t count, uint64_t items_max, int bits, int digit) {
+  if (bits <= 0 || count == 0 || items_max == 0) return 0;
+  const uint32_t blocks = div_up(items_max, kSortItems);
+  const int passes = (bits + digit - 1) / digit;
+  const dim3 grid(blocks, count);
   for (int pass = 0; pass < passes; ++pass) {
-    const int shift = pass * 8;
-    const int nbits = std::min(8, bits - shift);
-    k_radix_upsweep<<<blocks, kSortThreads, 0, v.stream>>>(ka, n, shift, nbits, v.sort.hist, blocks);
+    const int shift = pass * digit;
+    const int nbits = std::min(digit, bits - shift);
+    const int from_alt = pass & 1;
+    k_radix_upsweep<<<grid, kSortThreads, 0, ws->stream>>>(segs, shift, nbits, from_alt, blocks);
     GS_CHECK_LAUNCH("k_radix_upsweep");
-    k_radix_scan<<<1u << nbits, 256, 0, v.stream>>>(v.sort.hist, blocks, v.sort.digit_totals);
+    k_radix_scan<<<dim3(1u << nbits, count), 256, 0, ws->stream>>>(segs, blocks);
     GS_CHECK_LAUNCH("k_radix_scan");
-    k_radix_scatter<<<blocks, kSortThreads, 0, v.stream>>>(ka, va, kb, vb, n, shift, nbits, v.sort.hist,
-                                                           v.sort.digit_totals, blocks);
+    k_radix_scatter<<<grid, kSortThreads, 0, ws->stream>>>(segs, shift, nbits, from_alt, blocks);
     GS_CHECK_LAUNCH("k_radix_scatter");
-    std::swap(ka, kb);
-    std::swap(va, vb);
   }
-  if (ka != keys) {
-    GS_HIP_TRY(hipMemcpyAsync(keys, ka, size_t(n) * 4, hipMemcpyDeviceToDevice, v.stream));
-    GS_HIP_TRY(hipMemcpyAsync(vals, va, size_t(n) * 4, hipMemcpyDeviceToDevice, v.stream));
+  if (passes & 1) {
+    k_radix_copy_back<<<dim3(std::min<uint32_t>(div_up(items_max, 256), 4096), count), 256, 0, ws->stream>>>(segs);
+    GS_CHECK_LAUNCH("k_radix_copy_back");
   }
   return 0;
 }
 
-int exclusive_scan(View& v, const uint32_t* values, const uint32_t* gather, uint32_t* out, uint32_t count,
-                   uint32_t* total_dev) {
-  const uint32_t blocks = std::max<uint32_t>(div_up(count, kScanItems), 1);
-  if (blocks > v.block_sums_entries) return set_error(GS_HIP_E_CAPACITY, "exclusive_scan: too many blocks");
-  k_scan_reduce<<<blocks, kScanThreads, 0, v.stream>>>(values, gather, count, v.block_sums);
+int scan_segments(gs_hip_workspace* ws, const ScanSeg* segs, uint32_t count, uint64_t items_max) {
+  const uint32_t blocks = std::max<uint32_t>(div_up(items_max, kScanItems), 1);
+  k_scan_reduce<<<dim3(blocks, count), kScanThreads, 0, ws->stream>>>(segs);
   GS_CHECK_LAUNCH("k_scan_reduce");
-  k_scan_blocks<<<1, 1024, 0, v.stream>>>(v.block_sums, blocks, total_dev);
+  k_scan_blocks<<<dim3(1, count), 1024, 0, ws->stream>>>(segs, blocks);
   GS_CHECK_LAUNCH("k_scan_blocks");
-  k_scan_down<<<blocks, kScanThreads, 0, v.stream>>>(values, gather, count, v.block_sums, out);
+  k_scan_down<<<dim3(blocks, count), kScanThreads, 0, ws->stream>>>(segs);
   GS_CHECK_LAUNCH("k_scan_down");
   return 0;
 }
 
-// Depth sort and offsets; the pair count is copied to v.host_totals[0]
-// asynchronously (valid once v.stream has reached this point).
-int bin_count(gs_hip_workspace* ws, View& v, const gs_hip_frame* f) {
-  if (f->slots == 0) {
-    v.host_totals[0] = 0;
-    return 0;
-  }
-  const int depth_bits = (f->flags & GS_HIP_FLAG_DEPTH16) != 0u ? 16 : 32;
-  timer_begin(ws, v, "depth_sort");
-  int rc = radix_sort(v, v.depth_keys, v.depth_order, f->slots, depth_bits);
-  timer_end(ws, v);
-  if (rc != 0) return rc;
-  timer_begin(ws, v, "scan");
-  rc = exclusive_scan(v, v.tiles, v.depth_order, v.offsets, f->slots, v.totals);
-  timer_end(ws, v);
-  if (rc != 0) return rc;
-  GS_HIP_TRY(hipMemcpyAsync(v.host_totals, v.totals, 8, hipMemcpyDeviceToHost, v.stream));
-  return 0;
+}  // namespace
+
+namespace gsh {
+
+int exclusive_scan(gs_hip_workspace* ws, View& v, const uint32_t* values, const uint32_t* gather, uint32_t* out,
+                   uint32_t count, uint32_t* total_dev) {
+  const uint32_t blocks = std::max<uint32_t>(div_up(count, kScanItems), 1);
+  if (blocks > v.block_sums_entries) return set_error(GS_HIP_E_CAPACITY, "exclusive_scan: too many blocks");
+  const ScanSeg seg{values, gather, out, v.block_sums, total_dev, nullptr, count, 0u};
+  GS_HIP_TRY(hipStreamSynchronize(ws->stream));
+  GS_HIP_TRY(hipMemcpy(ws->dev_scan_tmp, &seg, sizeof(seg), hipMemcpyHostToDevice));
+  return scan_segments(ws, ws->dev_scan_tmp, 1, count);
 }
 
-int bin_pairs(gs_hip_workspace* ws, View& v, const gs_hip_frame* f, uint64_t pairs) {
-  const uint32_t tiles = f->tiles_x * f->tiles_y;
-  v.pairs = pairs;
-  timer_begin(ws, v, "emit");
-  k_fill_ranges<<<div_up((tiles + 1) * 2, 256), 256, 0, v.stream>>>(v.ranges, (tiles + 1) * 2);
-  GS_CHECK_LAUNCH("k_fill_ranges");
-  if (pairs > 0) {
-    k_emit<<<div_up(f->slots, 256), 256, 0, v.stream>>>(
-        *f, reinterpret_cast<const float4*>(v.records), reinterpret_cast<const float4*>(v.footprint), v.tiles,
-        v.depth_order, v.offsets, v.pair_tiles, v.pair_gaussians, v.pair_capacity, v.totals);
+int launch_depth_sort_and_scan(gs_hip_workspace* ws, uint32_t count, bool depth16) {
+  if (ws->capacity == 0) return 0;
+  timer_begin(ws, "depth_sort");
+  int rc = radix_sort(ws, ws->dev_depth_sort, count, ws->capacity, depth16 ? 16 : 32,
+                      int(ws->tuning.depth_digit_bits));
+  timer_end(ws);
+  if (rc != 0) return rc;
+  timer_begin(ws, "scan");
+  rc = scan_segments(ws, ws->dev_tile_scan, count, ws->capacity);
+  timer_end(ws);
+  return rc;
+}
+
+int launch_emit_and_tile_sort(gs_hip_workspace* ws, uint32_t count) {
+  const uint32_t tiles = tile_count(ws);
+  const uint64_t cap = view0(ws).pair_capacity;
+  timer_begin(ws, "emit");
+  k_clear_ranges<<<dim3(div_up(tiles + 1, 256), count), 256, 0, ws->stream>>>(ws->dev_views, tiles + 1);
+  GS_CHECK_LAUNCH("k_clear_ranges");
+  if (ws->capacity > 0) {
+    k_emit<<<dim3(div_up(ws->capacity, 256), count), 256, 0, ws->stream>>>(ws->dev_views);
     GS_CHECK_LAUNCH("k_emit");
   }
-  timer_end(ws, v);
-  if (pairs > 0) {
-    timer_begin(ws, v, "tile_sort");
-    const int rc = radix_sort(v, v.pair_tiles, v.pair_gaussians, uint32_t(pairs), bits_for(tiles + 1));
-    timer_end(ws, v);
-    if (rc != 0) return rc;
-    timer_begin(ws, v, "ranges");
-    k_ranges<<<div_up(pairs, 256), 256, 0, v.stream>>>(v.pair_tiles, uint32_t(pairs),
-                                                        reinterpret_cast<uint2*>(v.ranges));
-    GS_CHECK_LAUNCH("k_ranges");
-    timer_end(ws, v);
-  }
-  k_tile_order<<<1, 1024, 0, v.stream>>>(reinterpret_cast<const uint2*>(v.ranges), tiles, v.tile_order);
-  GS_CHECK_LAUNCH("k_tile_order");
+  timer_end(ws);
+  timer_begin(ws, "tile_sort");
+  int rc = radix_sort(ws, ws->dev_tile_sort, count, cap, bits_for(tiles + 1), int(ws->tuning.tile_digit_bits));
+  timer_end(ws);
+  if (rc != 0) return rc;
+  timer_begin(ws, "ranges");
+  k_ranges<<<dim3(div_up(cap, 256), count), 256, 0, ws->stream>>>(ws->dev_views);
+  GS_CHECK_LAUNCH("k_ranges");
+  k_work_order<<<1, 1024, 0, ws->stream>>>(ws->dev_views, count, tiles, ws->work, ws->work_counters);
+  GS_CHECK_LAUNCH("k_work_order");
+  timer_end(ws);
   return 0;
 }
 
@@ -457,19 +502,27 @@ int gs_hip_project(gs_hip_workspace* ws, const gs_hip_frame* f) {
   if (f->cam.width != ws->width || f->cam.height != ws->height) {
     return gsh::set_error(GS_HIP_E_ARGUMENT, "project: camera size differs from the workspace");
   }
-  return gsh::launch_project(ws, gsh::view0(ws), f);
+  const int rc = gsh::upload_table(ws, f, 1);
+  return rc != 0 ? rc : gsh::launch_project(ws, 1);
 }
 
 int gs_hip_bin(gs_hip_workspace* ws, const gs_hip_frame* f, uint64_t* pairs) {
   if (ws == nullptr || f == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "bin: null argument");
+  int rc = gsh::upload_table(ws, f, 1);
+  if (rc == 0) rc = gsh::launch_depth_sort_and_scan(ws, 1, (f->flags & GS_HIP_FLAG_DEPTH16) != 0u);
+  if (rc != 0) return rc;
   gsh::View& v = gsh::view0(ws);
-  int rc = gsh::bin_count(ws, v, f);
-  if (rc != 0) return rc;
-  GS_HIP_TRY(hipStreamSynchronize(v.stream));
-  const uint64_t p = v.host_totals[0];
-  rc = gsh::reserve_pairs(ws, v, p);
-  if (rc != 0) return rc;
-  rc = gsh::bin_pairs(ws, v, f, p);
+  uint32_t p = 0;
+  GS_HIP_TRY(hipMemcpyAsync(&p, v.totals, 4, hipMemcpyDeviceToHost, ws->stream));
+  GS_HIP_TRY(hipStreamSynchronize(ws->stream));
+  if (p > v.pair_capacity) {
+    rc = gsh::reserve_pairs(ws, p);
+    if (rc == 0) rc = gsh::upload_table(ws, f, 1);
+    if (rc != 0) return rc;
+  }
+  v.pairs = p;
+  ws->max_pairs_seen = std::max<uint64_t>(ws->max_pairs_seen, p);
+  rc = gsh::launch_emit_and_tile_sort(ws, 1);
   if (pairs != nullptr) *pairs = p;
   return rc;
 }

@@ -22,6 +22,7 @@ TILE = 16
 FLAG_MASKED_TARGET = 1
 FLAG_SPARSE_ADAM = 2
 FLAG_DEPTH16 = 4
+FLAG_NO_UPDATE = 8
 
 (BUF_PARAMS, BUF_SH, BUF_M_PARAMS, BUF_V_PARAMS, BUF_M_SH, BUF_V_SH, BUF_STATS, BUF_RECORDS, BUF_FOOTPRINT,
  BUF_TILES, BUF_DEPTH_KEYS, BUF_DEPTH_ORDER, BUF_OFFSETS, BUF_GRAD2D, BUF_PAIR_TILES, BUF_PAIR_GAUSSIANS,
@@ -35,7 +36,8 @@ EXPORTED = [
     "gs_hip_workspace_view_buffer", "gs_hip_workspace_reserve_pairs",
     "gs_hip_workspace_load", "gs_hip_workspace_store", "gs_hip_project", "gs_hip_bin", "gs_hip_forward",
     "gs_hip_set_target", "gs_hip_set_target_device", "gs_hip_loss", "gs_hip_backward", "gs_hip_adam", "gs_hip_densify", "gs_hip_reset_opacity", "gs_hip_train_step", "gs_hip_train_views",
-    "gs_hip_render", "gs_hip_workspace_timings",
+    "gs_hip_render", "gs_hip_workspace_get_tuning", "gs_hip_workspace_set_tuning", "gs_hip_autotune",
+    "gs_hip_workspace_overflows", "gs_hip_workspace_timings",
 ]
 
 
@@ -44,6 +46,14 @@ class Camera(ctypes.Structure):
                 ("position", ctypes.c_float * 3), ("fx", ctypes.c_float), ("fy", ctypes.c_float),
                 ("cx", ctypes.c_float), ("cy", ctypes.c_float), ("width", ctypes.c_uint32),
                 ("height", ctypes.c_uint32)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("raster_groups_per_cu", ctypes.c_uint32), ("depth_digit_bits", ctypes.c_uint32),
+                ("tile_digit_bits", ctypes.c_uint32), ("graph", ctypes.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {name: int(getattr(self, name)) for name, _ in self._fields_}
 
 
 class Frame(ctypes.Structure):
@@ -93,6 +103,10 @@ def library() -> ctypes.CDLL:
             "gs_hip_densify": [vp, frame_p, u32p, u32p],
             "gs_hip_reset_opacity": [vp, frame_p, ctypes.c_float], "gs_hip_train_step": [vp, frame_p, u64p, dp],
             "gs_hip_render": [vp, frame_p, ctypes.POINTER(ctypes.c_uint8)],
+            "gs_hip_workspace_get_tuning": [vp, ctypes.POINTER(Tuning)],
+            "gs_hip_workspace_set_tuning": [vp, ctypes.POINTER(Tuning)],
+            "gs_hip_autotune": [vp, frame_p, u32, ctypes.POINTER(vp), u32, ctypes.POINTER(Tuning), dp],
+            "gs_hip_workspace_overflows": [vp, u64p],
             "gs_hip_workspace_timings": [vp, i32, ctypes.POINTER(ctypes.c_char_p), dp, i32,
                                          ctypes.POINTER(ctypes.c_int)],
         }
@@ -233,6 +247,9 @@ class Workspace:
                                                     ctypes.byref(size)))
         return ptr, size.value
 
+    def reserve_pairs(self, pairs: int) -> None:
+        check(self.lib.gs_hip_workspace_reserve_pairs(self.handle, pairs))
+
     def set_views(self, count: int) -> None:
         check(self.lib.gs_hip_workspace_views(self.handle, count))
 
@@ -331,6 +348,39 @@ class Workspace:
         check(self.lib.gs_hip_render(self.handle, ctypes.byref(frame),
                                      out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
         return out
+
+    def tuning(self) -> Tuning:
+        t = Tuning()
+        check(self.lib.gs_hip_workspace_get_tuning(self.handle, ctypes.byref(t)))
+        return t
+
+    def set_tuning(self, raster_groups_per_cu=None, depth_digit_bits=None, tile_digit_bits=None, graph=None) -> Tuning:
+        t = self.tuning()
+        for name, value in (("raster_groups_per_cu", raster_groups_per_cu), ("depth_digit_bits", depth_digit_bits),
+                            ("tile_digit_bits", tile_digit_bits), ("graph", graph)):
+            if value is not None:
+                setattr(t, name, int(value))
+        check(self.lib.gs_hip_workspace_set_tuning(self.handle, ctypes.byref(t)))
+        return t
+
+    def autotune(self, frames, targets=None, repeats: int = 3):
+        """Times the candidate launch shapes on these views (model untouched);
+        keeps and returns (the fastest Tuning, its ms per step)."""
+        count = len(frames)
+        arr = (Frame * count)(*frames)
+        tp = None
+        if targets is not None:
+            tp = (ctypes.c_void_p * count)(*[t if isinstance(t, int) else (t.value if t else None)
+                                             for t in targets])
+        best = Tuning()
+        ms = ctypes.c_double()
+        check(self.lib.gs_hip_autotune(self.handle, arr, count, tp, repeats, ctypes.byref(best), ctypes.byref(ms)))
+        return best, ms.value
+
+    def overflows(self) -> int:
+        n = ctypes.c_uint64()
+        check(self.lib.gs_hip_workspace_overflows(self.handle, ctypes.byref(n)))
+        return n.value
 
     def timings(self, enable: bool = True):
         cap = 1 << 16
