@@ -177,6 +177,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="default launch shapes instead of the autotuner's")
+    ap.add_argument("--shape", type=str, default=None, metavar="G,D,T,GRAPH",
+                    help="fixed launch shape instead of autotuning: raster groups per CU (0 = one per tile), depth "
+                         "and tile radix digit bits, graph replay 0/1")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="after the bench, copy MIB MiB device to device twice (a known-size kernel that scales "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE counts to bytes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build (multi-GPU "
@@ -216,7 +222,11 @@ def main():
         ws.set_views(per_step)
         ws.load(params, sh)
         tuning, tune_ms = None, None
-        if not args.no_autotune:
+        if args.shape:
+            g, d, t, gr = (int(x) for x in args.shape.split(","))
+            tuning = ws.set_tuning(raster_groups_per_cu=g, depth_digit_bits=d, tile_digit_bits=t, graph=gr).as_dict()
+            log(f"{mode}: launch shape {tuning} (given)")
+        elif not args.no_autotune:
             vs = [int(order[k % len(order)]) for k in range(per_step)]
             frames = [gs.make_frame(cams[v], N, deg, step=0, tp=tp) for v in vs]
             best, tune_ms = ws.autotune(frames, [target_bufs[v].ptr for v in vs], repeats=3)
@@ -325,6 +335,14 @@ def main():
     print(json.dumps(out), flush=True)
     for buf in target_bufs:
         buf.free()
+    if args.calibrate_copy > 0:
+        nbytes = args.calibrate_copy << 20
+        src, dst = gs.device_alloc(nbytes), gs.device_alloc(nbytes)
+        for _ in range(2):
+            gs.device_copy(dst, src, nbytes)
+        gs.device_free(src)
+        gs.device_free(dst)
+        log(f"copy calibration: k_copy16 read {args.calibrate_copy} MiB + wrote {args.calibrate_copy} MiB, twice")
 
 
 if __name__ == "__main__":

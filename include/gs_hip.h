@@ -93,6 +93,9 @@ int gs_hip_free(void* ptr);
 int gs_hip_upload(void* device_dst, const void* host_src, size_t bytes);
 int gs_hip_download(void* host_dst, const void* device_src, size_t bytes);
 int gs_hip_memset(void* device_dst, int value, size_t bytes);
+/* Device-to-device copy by the library's own streaming kernel (16-byte
+ * vectors when both pointers and the size allow); waits for completion. */
+int gs_hip_copy(void* device_dst, const void* device_src, size_t bytes);
 int gs_hip_synchronize(void);
 /* Copies the calling thread's last error message (NUL-terminated); returns its status code. */
 int gs_hip_last_error(char* buffer, size_t size);
@@ -105,7 +108,8 @@ typedef struct gs_hip_workspace gs_hip_workspace;
  *  SH, M_SH, V_SH               float[sh_coeffs][capacity][3]   (coefficient-major planes)
  *  STATS                        float[capacity][4]  (grad norm sum, views, max radius, -)
  *  RECORDS                      float[capacity][12] (mean xy, conic abc, opacity, rgb, bbox lo/hi, radius|clamp)
- *  FOOTPRINT                    float[capacity][4]  (dilated cov2D abc, threshold)
+ *  FOOTPRINT                    float[capacity][8]  (dilated cov2D abc, threshold | mean xy, tiles as u32 bits, 0):
+ *                               everything the pair emitter reads of a slot, in one 32 B line
  *  TILES, DEPTH_KEYS, DEPTH_ORDER, OFFSETS  uint32[capacity]  (OFFSETS in depth order)
  *  GRAD2D                       float[9][capacity]  (planar)
  *  PAIR_TILES, PAIR_GAUSSIANS   uint32[pair_capacity] (sorted)

@@ -74,6 +74,13 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t count, uint32_t va
   if (i < count) p[i] = value;
 }
 
+// Streaming copy, 16 B per thread and iteration.
+__global__ void __launch_bounds__(256) k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
+    dst[i] = src[i];
+  }
+}
+
 template <typename T>
 int alloc(T** p, uint64_t count) {
   *p = nullptr;
@@ -110,7 +117,7 @@ int create_view(gs_hip_workspace* ws, uint32_t index, View** out) {
     if (rc == 0) rc = r;
   };
   A(alloc(&v->records, cap * GS_HIP_RECORD_FLOATS));
-  A(alloc(&v->footprint, cap * 4));
+  A(alloc(&v->footprint, cap * 8));
   A(alloc(&v->tiles, cap));
   A(alloc(&v->misc, cap));
   A(alloc(&v->depth_keys, cap));
@@ -353,6 +360,24 @@ int gs_hip_memset(void* dst, int value, size_t bytes) {
   return 0;
 }
 
+int gs_hip_copy(void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return 0;
+  if (dst == nullptr || src == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "gs_hip_copy: null pointer");
+  GS_HIP_TRY(hipDeviceSynchronize());
+  const bool vec = (reinterpret_cast<uintptr_t>(dst) % 16 == 0) && (reinterpret_cast<uintptr_t>(src) % 16 == 0) &&
+                   bytes % 16 == 0;
+  if (vec) {
+    const uint64_t n = bytes / 16;
+    const uint32_t blocks = uint32_t(std::min<uint64_t>(gsh::div_up(n, 256), 65536));
+    k_copy16<<<blocks, 256>>>(static_cast<const uint4*>(src), static_cast<uint4*>(dst), n);
+    GS_CHECK_LAUNCH("k_copy16");
+  } else {
+    GS_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  }
+  GS_HIP_TRY(hipDeviceSynchronize());
+  return 0;
+}
+
 int gs_hip_synchronize(void) {
   GS_HIP_TRY(hipDeviceSynchronize());
   return 0;
@@ -523,7 +548,7 @@ int gs_hip_workspace_view_buffer(gs_hip_workspace* ws, uint32_t view, int which,
     case GS_HIP_BUF_V_SH: p = ws->v_sh, b = shn * 4; break;
     case GS_HIP_BUF_STATS: p = ws->stats, b = cap * 16; break;
     case GS_HIP_BUF_RECORDS: p = v.records, b = cap * GS_HIP_RECORD_FLOATS * 4; break;
-    case GS_HIP_BUF_FOOTPRINT: p = v.footprint, b = cap * 16; break;
+    case GS_HIP_BUF_FOOTPRINT: p = v.footprint, b = cap * 32; break;
     case GS_HIP_BUF_TILES: p = v.tiles, b = cap * 4; break;
     case GS_HIP_BUF_DEPTH_KEYS: p = v.depth_keys, b = cap * 4; break;
     case GS_HIP_BUF_DEPTH_ORDER: p = v.depth_order, b = cap * 4; break;

@@ -74,7 +74,7 @@ struct ViewDev {
   gs_hip_frame f;
   // per slot
   float4* records;
-  float4* footprint;
+  float4* footprint;  // [capacity][2]: (cov abc, threshold), (mean xy, tiles bits, 0)
   uint32_t* tiles;
   uint32_t* misc;  // radius << 3 | colour clamp bits: what the adjoint needs of the record
   uint32_t* depth_keys;

@@ -70,15 +70,11 @@ __device__ __forceinline__ void add4(float4& a, float x, float y, float z, float
 }
 
 // d(colour)/d(direction) summed against dL/dcolour: sum_k (sh_k . dcol)
-// grad basis_k(dir), with each coefficient's term added as soon as its SH
-// triple is loaded (no per-coefficient arrays held in registers).
+// grad basis_k(dir), over the slot's SH rest coefficients (in registers).
 __device__ __forceinline__ void sh_direction_gradient(float x, float y, float z, uint32_t degree,
-                                                      const float* __restrict__ sh, uint32_t capacity, uint32_t i,
-                                                      const float dcol[3], float ddir[3]) {
-  auto w = [&](int k) {
-    const float* s = sh + (size_t(k) * capacity + i) * 3u;
-    return s[0] * dcol[0] + s[1] * dcol[1] + s[2] * dcol[2];
-  };
+                                                      const float (&rest)[GS_HIP_MAX_SH_REST][3], const float dcol[3],
+                                                      float ddir[3]) {
+  auto w = [&](int k) { return rest[k][0] * dcol[0] + rest[k][1] * dcol[1] + rest[k][2] * dcol[2]; };
   auto add = [&](float wk, float gx, float gy, float gz) {
     ddir[0] += wk * gx;
     ddir[1] += wk * gy;
@@ -112,9 +108,9 @@ __device__ __forceinline__ void sh_direction_gradient(float x, float y, float z,
 // gradient g[9] -> dL/d(position, logit, log scale, quaternion, DC) added to
 // the accumulators, and the view's SH colour gradient and direction.
 __device__ __forceinline__ void adjoint_view(const gs_hip_camera& cam, const float g[9], uint32_t clamped, float4 p0,
-                                             float4 p1, float4 q, const float* __restrict__ sh, uint32_t capacity,
-                                             uint32_t i, int coeffs, uint32_t degree, float4& dp0, float4& dp1,
-                                             float4& dq, float4& dp3, float dcol[3], float dir[3]) {
+                                             float4 p1, float4 q, const float (&rest)[GS_HIP_MAX_SH_REST][3],
+                                             int coeffs, uint32_t degree, float4& dp0, float4& dp1, float4& dq,
+                                             float4& dp3, float dcol[3], float dir[3]) {
   const float W = float(cam.width), H = float(cam.height);
   const float* V = cam.rotation;
   const float tx = V[0] * p0.x + V[1] * p0.y + V[2] * p0.z + cam.translation[0];
@@ -162,7 +158,7 @@ __device__ __forceinline__ void adjoint_view(const gs_hip_camera& cam, const flo
   const float b0 = 0.28209479177387814f;
   add4(dp3, b0 * dcol[0], b0 * dcol[1], b0 * dcol[2], 0.0f);
   float ddir[3] = {0.0f, 0.0f, 0.0f};
-  if (coeffs > 0) sh_direction_gradient(dir[0], dir[1], dir[2], degree, sh, capacity, i, dcol, ddir);
+  if (coeffs > 0) sh_direction_gradient(dir[0], dir[1], dir[2], degree, rest, dcol, ddir);
   const float dd = dir[0] * ddir[0] + dir[1] * ddir[1] + dir[2] * ddir[2];
   const float dpos_dir[3] = {(ddir[0] - dir[0] * dd) / dlen, (ddir[1] - dir[1] * dd) / dlen,
                              (ddir[2] - dir[2] * dd) / dlen};
@@ -250,6 +246,18 @@ __global__ void __launch_bounds__(256) k_adam(AdamArgs a) {
     if (coeffs > 0) a.views[0].sh_input[size_t(i) * 2u] = make_float4(0, 0, 0, 0);
     return;
   }
+  // The slot's SH rest coefficients, once for every view's direction gradient.
+  float rest[GS_HIP_MAX_SH_REST][3];
+#pragma unroll
+  for (int k = 0; k < GS_HIP_MAX_SH_REST; ++k) {
+    rest[k][0] = rest[k][1] = rest[k][2] = 0.0f;
+    if (k < coeffs) {
+      const float* sk = a.sh + (size_t(k) * a.capacity + i) * 3u;
+      rest[k][0] = sk[0];
+      rest[k][1] = sk[1];
+      rest[k][2] = sk[2];
+    }
+  }
   float4 dp0 = make_float4(0, 0, 0, 0), dp1 = dp0, dq = dp0, dp3 = dp0;
   bool drew = false, seen = false;
   float4 st = a.stats[i];
@@ -275,8 +283,7 @@ __global__ void __launch_bounds__(256) k_adam(AdamArgs a) {
     st.y += 1.0f;
     st.z = fmaxf(st.z, float(misc >> 3));
     float dcol[3], dir[3];
-    adjoint_view(view.f.cam, g, misc & 7u, p0, p1, q, a.sh, a.capacity, i, coeffs, degree, dp0, dp1, dq, dp3, dcol,
-                 dir);
+    adjoint_view(view.f.cam, g, misc & 7u, p0, p1, q, rest, coeffs, degree, dp0, dp1, dq, dp3, dcol, dir);
     if (coeffs > 0) {
       view.sh_input[size_t(i) * 2u] = make_float4(dcol[0], dcol[1], dcol[2], 0.0f);
       view.sh_input[size_t(i) * 2u + 1u] = make_float4(dir[0], dir[1], dir[2], 0.0f);

@@ -6,8 +6,9 @@
 // One thread per slot handles all the step's views, so the 64 B of
 // parameters and the 12 B per SH coefficient (coefficient-major planes: a
 // wave reads 768 B contiguous per coefficient) are read once per step, not
-// once per view.  Per view it writes a 48 B raster record, a 16 B footprint,
-// tiles, misc, key and order, and clears the nine planar GRAD2D accumulators.
+// once per view.  Per view it writes a 48 B raster record, a 32 B footprint
+// (all the pair emitter reads of a slot), tiles, misc, key and order, and
+// clears the nine planar GRAD2D accumulators.
 // HBM-bound.
 #include "gs_common.h"
 #include "gs_internal.h"
@@ -102,11 +103,13 @@ __device__ __forceinline__ void project_view(const gsh::ViewDev& vd, const Proje
         sh_basis(dx / dlen, dy / dlen, dz / dlen, f.sh_degree, basis);
         float cr = basis[0] * p3.x, cg = basis[0] * p3.y, cbl = basis[0] * p3.z;
         const int coeffs = min(sh_rest_coeffs(f.sh_degree), int(a.sh_coeffs));
-        for (int k = 0; k < GS_HIP_MAX_SH_REST; ++k) {
-          if (k >= coeffs) break;
-          cr = cr + basis[k + 1] * rest[k][0];
-          cg = cg + basis[k + 1] * rest[k][1];
-          cbl = cbl + basis[k + 1] * rest[k][2];
+#pragma unroll
+        for (int k = 0; k < GS_HIP_MAX_SH_REST; ++k) {  // unrolled: rest[] stays in registers
+          if (k < coeffs) {
+            cr = cr + basis[k + 1] * rest[k][0];
+            cg = cg + basis[k + 1] * rest[k][1];
+            cbl = cbl + basis[k + 1] * rest[k][2];
+          }
         }
         cr = cr + 0.5f;
         cg = cg + 0.5f;
@@ -132,7 +135,8 @@ __device__ __forceinline__ void project_view(const gsh::ViewDev& vd, const Proje
   r[0] = rec.a;
   r[1] = rec.b;
   r[2] = rec.c;
-  vd.footprint[i] = foot;
+  vd.footprint[size_t(i) * 2u] = foot;
+  vd.footprint[size_t(i) * 2u + 1u] = make_float4(rec.a.x, rec.a.y, __uint_as_float(tiles), 0.0f);
   vd.tiles[i] = tiles;
   vd.misc[i] = __float_as_uint(rec.c.w);
   uint32_t key = 0xffffffffu;
@@ -156,14 +160,17 @@ __global__ void __launch_bounds__(256) k_project(ProjectArgs a) {
   const float4 q = a.params[i * 4u + 2u], p3 = a.params[i * 4u + 3u];
   // SH rest of a live slot, once for all views.
   float rest[GS_HIP_MAX_SH_REST][3];
+#pragma unroll
   for (int k = 0; k < GS_HIP_MAX_SH_REST; ++k) rest[k][0] = rest[k][1] = rest[k][2] = 0.0f;
   if (p1.w > 0.5f) {
+#pragma unroll
     for (int k = 0; k < GS_HIP_MAX_SH_REST; ++k) {
-      if (k >= int(a.sh_coeffs)) break;
-      const float* s = a.sh + (size_t(k) * a.capacity + i) * 3u;
-      rest[k][0] = s[0];
-      rest[k][1] = s[1];
-      rest[k][2] = s[2];
+      if (k < int(a.sh_coeffs)) {
+        const float* s = a.sh + (size_t(k) * a.capacity + i) * 3u;
+        rest[k][0] = s[0];
+        rest[k][1] = s[1];
+        rest[k][2] = s[2];
+      }
     }
   }
   for (uint32_t v = 0; v < a.count; ++v) project_view(a.views[v], a, i, p0, p1, q, p3, rest);

@@ -314,10 +314,11 @@ __global__ void __launch_bounds__(256) k_emit(const ViewDev* __restrict__ views)
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= f.slots) return;
   const uint32_t i = v.depth_order[k];
-  const uint32_t count = v.tiles[i];
+  // One 32 B line per slot: the depth order makes these reads random.
+  const float4 fp = v.footprint[size_t(i) * 2u];
+  const float4 r0 = v.footprint[size_t(i) * 2u + 1u];  // mean x, mean y, tiles
+  const uint32_t count = __float_as_uint(r0.z);
   if (count == 0u) return;
-  const float4 r0 = v.records[size_t(i) * 3u];
-  const float4 fp = v.footprint[i];
   uint64_t p = v.offsets[k];
   const uint64_t end = std::min<uint64_t>(p + count, v.pair_capacity);
   uint32_t wanted = 0;

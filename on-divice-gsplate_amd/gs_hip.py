@@ -31,7 +31,7 @@ MAX_VIEWS = 8
 
 EXPORTED = [
     "gs_hip_init", "gs_hip_device_name", "gs_hip_malloc", "gs_hip_free", "gs_hip_upload", "gs_hip_download",
-    "gs_hip_memset", "gs_hip_synchronize", "gs_hip_last_error", "gs_hip_workspace_create",
+    "gs_hip_memset", "gs_hip_copy", "gs_hip_synchronize", "gs_hip_last_error", "gs_hip_workspace_create",
     "gs_hip_workspace_destroy", "gs_hip_workspace_buffer", "gs_hip_workspace_views",
     "gs_hip_workspace_view_buffer", "gs_hip_workspace_reserve_pairs",
     "gs_hip_workspace_load", "gs_hip_workspace_store", "gs_hip_project", "gs_hip_bin", "gs_hip_forward",
@@ -87,7 +87,8 @@ def library() -> ctypes.CDLL:
         sig = {
             "gs_hip_init": [i32], "gs_hip_device_name": [ctypes.c_char_p, sz],
             "gs_hip_malloc": [ctypes.POINTER(vp), sz], "gs_hip_free": [vp], "gs_hip_upload": [vp, vp, sz],
-            "gs_hip_download": [vp, vp, sz], "gs_hip_memset": [vp, i32, sz], "gs_hip_synchronize": [],
+            "gs_hip_download": [vp, vp, sz], "gs_hip_memset": [vp, i32, sz], "gs_hip_copy": [vp, vp, sz],
+            "gs_hip_synchronize": [],
             "gs_hip_last_error": [ctypes.c_char_p, sz],
             "gs_hip_workspace_create": [ctypes.POINTER(vp), u32, i32, u32, u32],
             "gs_hip_workspace_destroy": [vp],
@@ -393,6 +394,21 @@ class Workspace:
 
 def synchronize() -> None:
     check(library().gs_hip_synchronize())
+
+
+def device_alloc(nbytes: int) -> int:
+    ptr = ctypes.c_void_p()
+    check(library().gs_hip_malloc(ctypes.byref(ptr), nbytes))
+    return ptr.value
+
+
+def device_free(ptr: int) -> None:
+    check(library().gs_hip_free(ptr))
+
+
+def device_copy(dst: int, src: int, nbytes: int) -> None:
+    """Device-to-device copy by the library's streaming kernel (k_copy16)."""
+    check(library().gs_hip_copy(dst, src, nbytes))
 
 
 # ---- seeded synthetic scenes (the benchmark configurations) -----------------

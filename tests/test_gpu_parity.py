@@ -40,7 +40,11 @@ def project_and_bin(ws, frame):
 
 def gpu_state(ws, n, pairs):
     rec = ws.download(gs.BUF_RECORDS, np.float32, n * 12).reshape(n, 12)
-    foot = ws.download(gs.BUF_FOOTPRINT, np.float32, n * 4).reshape(n, 4)
+    foot8 = ws.download(gs.BUF_FOOTPRINT, np.float32, n * 8).reshape(n, 8)
+    foot = foot8[:, :4]
+    rec_ = ws.download(gs.BUF_RECORDS, np.float32, n * 12).reshape(n, 12)
+    np.testing.assert_array_equal(foot8[:, 4:6].view(np.uint32), rec_[:, 0:2].view(np.uint32))
+    np.testing.assert_array_equal(foot8[:, 6].view(np.uint32), ws.download(gs.BUF_TILES, np.uint32, n))
     tiles = ws.download(gs.BUF_TILES, np.uint32, n)
     keys = ws.download(gs.BUF_DEPTH_KEYS, np.uint32, n)
     order = ws.download(gs.BUF_DEPTH_ORDER, np.uint32, n)

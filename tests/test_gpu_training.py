@@ -1,0 +1,124 @@
+"""Training trajectories: N optimizer steps of the HIP trainer against the
+oracle's CPU restatement of the same steps (fp32 projection, binning,
+compositing and tile backward; fp64 loss, SSIM and projection adjoint; fp32
+Adam), from the same seeded start on the same views in the same order.
+
+The bar is the one set for training parity: the PSNR of the two trained
+models (each rendered by the oracle's fp64 brute-force renderer) agrees
+within 0.01 dB after N iterations.  The trajectories are not bit-identical
+(fp32 vs fp64 adjoint, float-atomic sums, hardware exp), so the parameters
+are compared within a stated tolerance too.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import gs, make_camera, make_scene, orc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    name = gs.init(0)
+    print(f"\n[gpu] {name}", flush=True)
+    yield name
+    gs.synchronize()
+
+
+def gt_targets(n, deg, cams, seed):
+    """Targets: a seeded ground-truth scene rendered by the oracle (fp64)."""
+    gt_p, gt_sh = make_scene(n, deg, seed, spread=0.6)
+    out = []
+    for cam in cams:
+        rgb, _ = orc.render64(gt_p, gt_sh, cam, deg)
+        t = np.zeros((cam.height, cam.width, 4), np.uint8)
+        t[..., :3] = np.clip(np.rint(rgb.reshape(cam.height, cam.width, 3) * 255.0), 0, 255).astype(np.uint8)
+        t[..., 3] = 255
+        out.append(t)
+    return out
+
+
+def lr_columns(tp, n):
+    lr = np.zeros(16, np.float32)
+    lr[0:3] = tp.lr_position
+    lr[3] = tp.lr_opacity
+    lr[4:7] = tp.lr_scale
+    lr[8:12] = tp.lr_rotation
+    lr[12:15] = tp.lr_sh_dc
+    return np.tile(lr, (n, 1))
+
+
+def oracle_train(params, sh, deg, cams, targets, steps, tp):
+    p, s = params.copy(), sh.copy()
+    n = p.shape[0]
+    m, v = np.zeros_like(p), np.zeros_like(p)
+    msh, vsh = np.zeros_like(s), np.zeros_like(s)
+    lr = lr_columns(tp, n)
+    for k in range(steps):
+        cam, target = cams[k % len(cams)], targets[k % len(cams)]
+        w, h = cam.width, cam.height
+        rec, foot, tiles, keys = orc.project32(p, s, cam, deg)
+        pt, pg, ranges = orc.bin32(rec, foot, tiles, keys, w, h)
+        img, last = orc.forward32(rec, pg, ranges, w, h)
+        x = img[:, :3].astype(np.float64)
+        y = target.reshape(-1, 4)[:, :3] / 255.0
+        _, dldc = orc.loss_image64(x, y, w, h, tp.lambda_dssim, grad=True)
+        d4 = np.zeros((w * h, 4), np.float32)
+        d4[:, :3] = dldc
+        g2 = orc.backward32(rec, pg, ranges, img, last, d4, (0.0, 0.0, 0.0), w, h)
+        gp, gsh = orc.adjoint64(p, s, cam, deg, g2)
+        b1, b2 = 1.0 / (1.0 - 0.9 ** (k + 1)), 1.0 / (1.0 - 0.999 ** (k + 1))
+        p, m, v = orc.adam(p, gp.astype(np.float32), m, v, lr, bias1=b1, bias2=b2)
+        if s.size:
+            s, msh, vsh = orc.adam(s, gsh.astype(np.float32), msh, vsh, np.full(s.shape, tp.lr_sh_rest, np.float32),
+                                   bias1=b1, bias2=b2)
+    return p, s
+
+
+def gpu_train(params, sh, deg, cams, targets, steps, tp):
+    n = params.shape[0]
+    cam0 = cams[0]
+    with gs.Workspace(n, deg, cam0.width, cam0.height) as ws:
+        ws.load(params, sh)
+        for k in range(steps):
+            ws.set_target(targets[k % len(cams)])
+            ws.train_step(gs.make_frame(cams[k % len(cams)], n, deg, step=k, tp=tp), want_losses=False)
+        return ws.store(n)
+
+
+def psnr(params, sh, deg, cams, targets):
+    out = []
+    for cam, t in zip(cams, targets):
+        rgb, _ = orc.render64(params, sh, cam, deg)
+        y = t.reshape(-1, 4)[:, :3] / 255.0
+        mse = np.mean((rgb - y) ** 2)
+        out.append(-10.0 * math.log10(max(mse, 1e-12)))
+    return float(np.mean(out))
+
+
+TRAJ_CASES = [
+    ("sh1-400-4views-40steps", 400, 1, 96, 72, 4, 40, 81),
+    ("sh3-600-3views-30steps", 600, 3, 80, 64, 3, 30, 82),
+]
+
+
+@pytest.mark.parametrize("name,n,deg,w,h,views,steps,seed", TRAJ_CASES, ids=[c[0] for c in TRAJ_CASES])
+def test_training_psnr_matches_oracle_after_n_steps(name, n, deg, w, h, views, steps, seed):
+    print(f"[case] trajectory {name}", flush=True)
+    cams = [make_camera(w, h, 0.3 + 2.0 * math.pi * v / views) for v in range(views)]
+    targets = gt_targets(n, deg, cams, seed + 1000)
+    params, sh = make_scene(n, deg, seed)
+    tp = gs.TrainParams(lambda_dssim=0.2)
+    g_p, g_sh = gpu_train(params, sh, deg, cams, targets, steps, tp)
+    o_p, o_sh = oracle_train(params, sh, deg, cams, targets, steps, tp)
+    start = psnr(params, sh, deg, cams, targets)
+    g_psnr = psnr(g_p, g_sh, deg, cams, targets)
+    o_psnr = psnr(o_p, o_sh, deg, cams, targets)
+    dp = np.abs(g_p - o_p).max()
+    print(f"[info] {name}: PSNR start {start:.4f} dB, GPU {g_psnr:.4f} dB, oracle {o_psnr:.4f} dB, "
+          f"max |dparam| {dp:.3g}", flush=True)
+    assert o_psnr > start + 0.5, "the oracle's training does not improve the fit"
+    assert abs(g_psnr - o_psnr) <= 0.01
+    assert dp <= 1e-3 * max(1.0, np.abs(o_p).max())
