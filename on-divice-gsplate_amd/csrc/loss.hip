@@ -20,6 +20,10 @@ namespace {
 using namespace gsd;
 
 constexpr int kHalo = kTile + 2 * kSsimRadius;  // 26
+// LDS row stride of the halo: 48 = 16 mod 32, so the two 16-pixel rows each
+// half-wave reads in the horizontal blur fall on disjoint banks (with 26 they
+// overlapped: 1.1-1.7 bank-conflict cycles per LDS instruction).
+constexpr int kStride = 48;
 
 struct LossArgs {
   const float4* __restrict__ image;
@@ -71,8 +75,8 @@ __global__ void __launch_bounds__(256) k_ssim(const gsh::ViewDev* __restrict__ v
   const gsh::ViewDev& view = views[blockIdx.y];
   const gs_hip_frame& f = view.f;
   const LossArgs a = loss_args(view);
-  __shared__ float s_x[3][kHalo * kHalo];
-  __shared__ float s_y[3][kHalo * kHalo];
+  __shared__ float s_x[3][kHalo * kStride];
+  __shared__ float s_y[3][kHalo * kStride];
   __shared__ float s_h[5][kHalo * kTile];
   __shared__ float s_sum[3][4];
   const uint32_t tile = blockIdx.x;
@@ -85,9 +89,10 @@ __global__ void __launch_bounds__(256) k_ssim(const gsh::ViewDev* __restrict__ v
   for (int e = threadIdx.x; e < kHalo * kHalo; e += 256) {
     float rx, ry, vx[3], vy[3];
     load_xy(f, a, ox + e % kHalo, oy + e / kHalo, rx, ry, vx, vy);
+    const int at = (e / kHalo) * kStride + e % kHalo;
     for (int c = 0; c < 3; ++c) {
-      s_x[c][e] = vx[c];
-      s_y[c][e] = vy[c];
+      s_x[c][at] = vx[c];
+      s_y[c][at] = vy[c];
     }
   }
   __syncthreads();
@@ -100,7 +105,7 @@ __global__ void __launch_bounds__(256) k_ssim(const gsh::ViewDev* __restrict__ v
       float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f, m3 = 0.0f, m4 = 0.0f;
       for (int k = 0; k <= 2 * kSsimRadius; ++k) {
         const float w = kSsimWindow[k];
-        const float x = s_x[c][row * kHalo + col + k], y = s_y[c][row * kHalo + col + k];
+        const float x = s_x[c][row * kStride + col + k], y = s_y[c][row * kStride + col + k];
         m0 += w * x;
         m1 += w * y;
         m2 += w * x * x;
@@ -136,7 +141,7 @@ __global__ void __launch_bounds__(256) k_ssim(const gsh::ViewDev* __restrict__ v
       a.maps[size_t(c) * pixels + pix] = A;
       a.maps[size_t(3 + c) * pixels + pix] = d_vx;
       a.maps[size_t(6 + c) * pixels + pix] = d_cxy;
-      const int centre = (ly + kSsimRadius) * kHalo + lx + kSsimRadius;
+      const int centre = (ly + kSsimRadius) * kStride + lx + kSsimRadius;
       const float err = s_x[c][centre] - s_y[c][centre];
       abs_e += fabsf(err);
       sq_e += err * err;
@@ -165,7 +170,7 @@ __global__ void __launch_bounds__(256) k_loss_grad(const gsh::ViewDev* __restric
   const gsh::ViewDev& view = views[blockIdx.y];
   const gs_hip_frame& f = view.f;
   const LossArgs a = loss_args(view);
-  __shared__ float s_m[3][kHalo * kHalo];
+  __shared__ float s_m[3][kHalo * kStride];
   __shared__ float s_h[3][kHalo * kTile];
   const uint32_t tile = blockIdx.x;
   const int tx = int(tile % f.tiles_x), ty = int(tile / f.tiles_x);
@@ -191,14 +196,15 @@ __global__ void __launch_bounds__(256) k_loss_grad(const gsh::ViewDev* __restric
         const int gx = ox + e % kHalo, gy = oy + e / kHalo;
         const bool in_image = gx >= 0 && gy >= 0 && gx < int(f.cam.width) && gy < int(f.cam.height);
         const uint32_t id = uint32_t(gy) * f.cam.width + uint32_t(gx);
-        for (int q = 0; q < 3; ++q) s_m[q][e] = in_image ? a.maps[size_t(3 * q + c) * pixels + id] : 0.0f;
+        const int at = (e / kHalo) * kStride + e % kHalo;
+        for (int q = 0; q < 3; ++q) s_m[q][at] = in_image ? a.maps[size_t(3 * q + c) * pixels + id] : 0.0f;
       }
       __syncthreads();
       for (int e = threadIdx.x; e < kHalo * kTile; e += 256) {
         const int row = e / kTile, col = e % kTile;
         for (int q = 0; q < 3; ++q) {
           float sum = 0.0f;
-          for (int k = 0; k <= 2 * kSsimRadius; ++k) sum += kSsimWindow[k] * s_m[q][row * kHalo + col + k];
+          for (int k = 0; k <= 2 * kSsimRadius; ++k) sum += kSsimWindow[k] * s_m[q][row * kStride + col + k];
           s_h[q][e] = sum;
         }
       }

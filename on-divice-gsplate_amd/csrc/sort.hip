@@ -14,12 +14,14 @@
 // as overflowed (the optimizer then leaves the model alone and the host runs
 // the step again with larger buffers, as trainer.cpp:819-836 does).
 //
-// The radix sort is LSD with 4..8-bit digits; a block owns 4096 items.  Per
-// pass: an upsweep counts the block's digits (per-wave LDS histograms), a
-// scan turns the [digit][block] counts into offsets, and a scatter ranks the
-// block's items stably (wave64 ballots: the lanes holding the same digit,
-// rank = popcount of those below; per-wave digit counts in LDS), places them
-// in LDS in digit order, and writes each digit's run out contiguously.
+// The radix sort is LSD with 4..8-bit digits; a block owns 4096 items, each
+// wave a contiguous 1024 of them.  Per pass: an upsweep counts the block's
+// digits (per-wave LDS histograms, one add per group of lanes sharing a
+// digit), a scan turns the [digit][block] counts into offsets, and a scatter
+// ranks the items stably (wave64 ballots find the lanes holding the same
+// digit, rank = popcount of those below plus the wave's running count of
+// that digit), places them in LDS in digit order, and writes each digit's
+// run out contiguously.
 #include <algorithm>
 
 #include "gs_common.h"
@@ -59,7 +61,9 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t digit, int nbits, uint6
 }
 
 // Histogram of block (blockIdx.x) of segment blockIdx.y; blocks past the
-// segment's length write zeros.
+// segment's length write zeros.  Each wave counts a contiguous 1024 items;
+// the lanes holding the same digit add their count once (no LDS atomics
+// piling on one counter when neighbouring keys share a digit).
 __global__ void __launch_bounds__(kSortThreads) k_radix_upsweep(const SortSeg* __restrict__ segs, int shift,
                                                                 int nbits, int from_alt, uint32_t blocks) {
   __shared__ uint32_t h[kWaves][256];
@@ -67,13 +71,17 @@ __global__ void __launch_bounds__(kSortThreads) k_radix_upsweep(const SortSeg* _
   const uint32_t n = seg_count(g.count, g.count_max);
   const uint32_t* __restrict__ keys = from_alt ? g.keys_alt : g.keys;
   const uint32_t radix = 1u << nbits, mask = radix - 1u;
-  const int wave = threadIdx.x / 64;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
   for (int d = threadIdx.x; d < kWaves * 256; d += kSortThreads) (&h[0][0])[d] = 0u;
   __syncthreads();
-  const uint32_t base = blockIdx.x * kSortItems;
+  const uint32_t wbase = blockIdx.x * kSortItems + wave * (kSortItems / kWaves);
+  const uint64_t below_me = lanemask_lt();
   for (int r = 0; r < kSortRounds; ++r) {
-    const uint32_t idx = base + r * kSortThreads + threadIdx.x;
-    if (idx < n) atomicAdd(&h[wave][(keys[idx] >> shift) & mask], 1u);
+    const uint32_t idx = wbase + r * 64 + lane;
+    const bool valid = idx < n;
+    const uint32_t digit = valid ? (keys[idx] >> shift) & mask : 0u;
+    const uint64_t peers = match_digit(digit, nbits, __ballot(valid));
+    if (valid && (peers & below_me) == 0ull) h[wave][digit] += __popcll(peers);
   }
   __syncthreads();
   for (uint32_t d = threadIdx.x; d < radix; d += kSortThreads) {
@@ -129,14 +137,18 @@ __device__ __forceinline__ void block_scan_256(uint32_t v, uint32_t* s_tmp, uint
   __syncthreads();
 }
 
+// Stable scatter of one block: wave w ranks its contiguous 1024 items
+// (items base + 1024 w + 64 r + lane, so (w, r, lane) is the input order)
+// against a running per-wave digit count in LDS; one block-level combine
+// turns (wave, digit) counts into offsets; items go to LDS in digit order and
+// each digit's run leaves contiguously.
 __global__ void __launch_bounds__(kSortThreads) k_radix_scatter(const SortSeg* __restrict__ segs, int shift,
                                                                 int nbits, int from_alt, uint32_t blocks) {
   __shared__ uint32_t s_keys[kSortItems];
   __shared__ uint32_t s_vals[kSortItems];
-  __shared__ uint32_t s_counts[kWaves][256];
-  __shared__ uint32_t s_local[256];    // block-local start of each digit
-  __shared__ uint32_t s_running[256];  // items of each digit placed so far
-  __shared__ uint32_t s_global[256];   // where the block's run of each digit starts in the output
+  __shared__ uint32_t s_wave[kWaves][256];  // per-wave digit counts, then each wave's offset inside the digit
+  __shared__ uint32_t s_local[256];         // block-local start of each digit
+  __shared__ uint32_t s_global[256];        // where the block's run of each digit starts in the output
   __shared__ uint32_t s_tmp[kWaves];
   const SortSeg& g = segs[blockIdx.y];
   const uint32_t n = seg_count(g.count, g.count_max);
@@ -147,54 +159,50 @@ __global__ void __launch_bounds__(kSortThreads) k_radix_scatter(const SortSeg* _
   uint32_t* __restrict__ keys_out = from_alt ? g.keys : g.keys_alt;
   uint32_t* __restrict__ vals_out = from_alt ? g.vals : g.vals_alt;
   const uint32_t radix = 1u << nbits, mask = radix - 1u;
-  const int wave = threadIdx.x / 64;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
   // Output start of each digit: the digits below it over the whole segment
   // (a block scan of the totals) plus this digit's items in earlier blocks.
   block_scan_256(threadIdx.x < radix ? g.digit_totals[threadIdx.x] : 0u, s_tmp, s_global);
   if (threadIdx.x < radix) s_global[threadIdx.x] += g.hist[size_t(threadIdx.x) * blocks + blockIdx.x];
-  // The block's items, in registers; its digit histogram in LDS.
-  uint32_t key[kSortRounds], val[kSortRounds];
-  for (int d = threadIdx.x; d < kWaves * 256; d += kSortThreads) (&s_counts[0][0])[d] = 0u;
+  for (int d = threadIdx.x; d < kWaves * 256; d += kSortThreads) (&s_wave[0][0])[d] = 0u;
   __syncthreads();
+  const uint32_t wbase = base + wave * (kSortItems / kWaves);
+  uint32_t key[kSortRounds], val[kSortRounds], wpos[kSortRounds];
   for (int r = 0; r < kSortRounds; ++r) {
-    const uint32_t idx = base + r * kSortThreads + threadIdx.x;
+    const uint32_t idx = wbase + r * 64 + lane;
     key[r] = idx < n ? keys_in[idx] : 0u;
     val[r] = idx < n ? vals_in[idx] : 0u;
-    if (idx < n) atomicAdd(&s_counts[0][(key[r] >> shift) & mask], 1u);
   }
-  __syncthreads();
-  const uint32_t block_count = threadIdx.x < radix ? s_counts[0][threadIdx.x] : 0u;
-  block_scan_256(block_count, s_tmp, s_local);
-  s_running[threadIdx.x] = 0u;
-  s_counts[0][threadIdx.x] = 0u;
-  __syncthreads();
-  // Stable rank of every item inside the block -> its place in LDS.
+  // Wave-local stable ranks: earlier items of the same digit in this wave's chunk.
   const uint64_t below_me = lanemask_lt();
   for (int r = 0; r < kSortRounds; ++r) {
-    const uint32_t idx = base + r * kSortThreads + threadIdx.x;
-    const bool valid = idx < n;
+    const bool valid = wbase + r * 64 + lane < n;
     const uint32_t digit = (key[r] >> shift) & mask;
     const uint64_t peers = match_digit(digit, nbits, __ballot(valid));
-    const uint32_t rank = __popcll(peers & below_me);
-    if (valid && rank == 0u) s_counts[wave][digit] = __popcll(peers);
-    __syncthreads();
-    if (valid) {
-      uint32_t pos = s_local[digit] + s_running[digit] + rank;
-      for (int w = 0; w < wave; ++w) pos += s_counts[w][digit];
+    const uint32_t before = valid ? s_wave[wave][digit] : 0u;  // every peer reads, then the first one writes
+    if (valid && (peers & below_me) == 0ull) s_wave[wave][digit] = before + __popcll(peers);
+    wpos[r] = before + __popcll(peers & below_me);
+  }
+  __syncthreads();
+  // Per digit: the waves' counts -> each wave's offset inside the digit, and the block's count.
+  uint32_t total = 0u;
+  if (threadIdx.x < radix) {
+    for (int w = 0; w < kWaves; ++w) {
+      const uint32_t c = s_wave[w][threadIdx.x];
+      s_wave[w][threadIdx.x] = total;
+      total += c;
+    }
+  }
+  block_scan_256(total, s_tmp, s_local);
+  for (int r = 0; r < kSortRounds; ++r) {
+    if (wbase + r * 64 + lane < n) {
+      const uint32_t digit = (key[r] >> shift) & mask;
+      const uint32_t pos = s_local[digit] + s_wave[wave][digit] + wpos[r];
       s_keys[pos] = key[r];
       s_vals[pos] = val[r];
     }
-    __syncthreads();
-    if (threadIdx.x < radix) {
-      uint32_t s = 0;
-      for (int w = 0; w < kWaves; ++w) {
-        s += s_counts[w][threadIdx.x];
-        s_counts[w][threadIdx.x] = 0u;
-      }
-      s_running[threadIdx.x] += s;
-    }
-    __syncthreads();
   }
+  __syncthreads();
   // Each digit's run leaves contiguously: consecutive threads, consecutive addresses.
   const uint32_t items = min(uint32_t(kSortItems), n - base);
   for (uint32_t i = threadIdx.x; i < items; i += kSortThreads) {
