@@ -25,6 +25,7 @@ class GpuScene {
   GpuScene& operator=(const GpuScene&) = delete;
   uint32_t count() const { return count_; }
   int sh_degree() const { return sh_degree_; }
+  bool antialiased() const { return antialiased_; }
   uint32_t width() const { return width_; }
   uint32_t height() const { return height_; }
   gs_hip_workspace* workspace() { return ws_; }
@@ -33,6 +34,7 @@ class GpuScene {
   gs_hip_workspace* ws_ = nullptr;
   uint32_t count_ = 0, width_ = 0, height_ = 0;
   int sh_degree_ = 0;
+  bool antialiased_ = false;
 };
 
 std::unique_ptr<GpuScene> upload_scene(Device& device, const SplatScene& scene, uint32_t width, uint32_t height);
@@ -41,7 +43,8 @@ class SplatRenderer {
  public:
   explicit SplatRenderer(Device& device) : device_(device) {}
   void attach(GpuScene* scene) { scene_ = scene; }
-  // RGBA8 of `camera` (the scene's output size) front to back over `clear`.
+  // RGBA8 of `camera` (the scene's output size): splats back to front over
+  // the opaque clear colour; visible_count() = splats drawn.
   std::vector<uint8_t> render(const Camera& camera, const float clear[3]);
   uint32_t visible_count() const { return visible_; }
 
@@ -50,6 +53,10 @@ class SplatRenderer {
   GpuScene* scene_ = nullptr;
   uint32_t visible_ = 0;
 };
+
+// The nearest fp16 value to `f` (ties to even) as a float: the precision at
+// which the viewer stores SH rest coefficients (splat_renderer.cpp:103-116).
+float round_to_half(float f);
 
 // Binary PPM (P6) of an RGBA8 image.
 std::vector<uint8_t> encode_ppm(const std::vector<uint8_t>& rgba, uint32_t width, uint32_t height);

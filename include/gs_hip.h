@@ -217,6 +217,21 @@ int gs_hip_train_views(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_
 /* Forward only (render): project, bin, forward; image as RGBA8 (clamped rgb, alpha = 1 - T). */
 int gs_hip_render(gs_hip_workspace* ws, const gs_hip_frame* frame, uint8_t* rgba8_out);
 
+/* The viewer's image contract (gs_render, the app viewer:
+ * engine/shaders/splat_preprocess.comp:159-291, splat.vert, splat.frag,
+ * engine/src/splat_renderer.cpp:243-348): quads at +-2 sqrt(2 lambda) along
+ * the 2D covariance's axes, alpha = min(o exp(-|local|^2), 0.99) dropped
+ * below 1/255, colour and opacity at fp16 precision, a 16-bit log-depth key,
+ * drawn back to front with premultiplied "over" into an RGBA8 target cleared
+ * to `clear` (rgba in 0..1).  The SH rest coefficients are used as stored:
+ * load them rounded to fp16 for the reference's upload (gs::upload_scene
+ * does).  Writes width*height*4 bytes; *visible = splats drawn (on screen).
+ * Works in view 0's per-view buffers (a training step rewrites them) and
+ * grows the pair buffers when a view needs more. */
+#define GS_HIP_VIEW_ANTIALIASED 1u /* mip-splat opacity compensation sqrt(det_raw / det) */
+int gs_hip_view_render(gs_hip_workspace* ws, const gs_hip_frame* frame, const float clear[4], uint32_t flags,
+                       uint8_t* rgba8_out, uint32_t* visible);
+
 /* ---- 4. launch shapes ------------------------------------------------------ */
 /* Knobs of the batched step (no counterpart in the reference, whose shaders
  * fix them: trainer.h:351 kTileSize, radix_sort.cpp 4-bit digits).  The tile

@@ -214,6 +214,10 @@ int upload_table(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t coun
 int launch_project(gs_hip_workspace* ws, uint32_t count);
 int launch_depth_sort_and_scan(gs_hip_workspace* ws, uint32_t count, bool depth16);
 int launch_emit_and_tile_sort(gs_hip_workspace* ws, uint32_t count);
+// Pieces of the above for other emitters: ranges cleared (and the mismatch
+// counter), then the stable tile sort, ranges and the rasterizers' work order.
+int launch_clear_ranges(gs_hip_workspace* ws, uint32_t count);
+int launch_tile_sort_and_ranges(gs_hip_workspace* ws, uint32_t count);
 int launch_forward(gs_hip_workspace* ws, uint32_t count);
 int launch_loss(gs_hip_workspace* ws, uint32_t count);
 int launch_backward(gs_hip_workspace* ws, uint32_t count);

@@ -477,17 +477,16 @@ int launch_depth_sort_and_scan(gs_hip_workspace* ws, uint32_t count, bool depth1
   return rc;
 }
 
-int launch_emit_and_tile_sort(gs_hip_workspace* ws, uint32_t count) {
+int launch_clear_ranges(gs_hip_workspace* ws, uint32_t count) {
   const uint32_t tiles = tile_count(ws);
-  const uint64_t cap = view0(ws).pair_capacity;
-  timer_begin(ws, "emit");
   k_clear_ranges<<<dim3(div_up(tiles + 1, 256), count), 256, 0, ws->stream>>>(ws->dev_views, tiles + 1);
   GS_CHECK_LAUNCH("k_clear_ranges");
-  if (ws->capacity > 0) {
-    k_emit<<<dim3(div_up(ws->capacity, 256), count), 256, 0, ws->stream>>>(ws->dev_views);
-    GS_CHECK_LAUNCH("k_emit");
-  }
-  timer_end(ws);
+  return 0;
+}
+
+int launch_tile_sort_and_ranges(gs_hip_workspace* ws, uint32_t count) {
+  const uint32_t tiles = tile_count(ws);
+  const uint64_t cap = view0(ws).pair_capacity;
   timer_begin(ws, "tile_sort");
   int rc = radix_sort(ws, ws->dev_tile_sort, count, cap, bits_for(tiles + 1), int(ws->tuning.tile_digit_bits));
   timer_end(ws);
@@ -499,6 +498,17 @@ int launch_emit_and_tile_sort(gs_hip_workspace* ws, uint32_t count) {
   GS_CHECK_LAUNCH("k_work_order");
   timer_end(ws);
   return 0;
+}
+
+int launch_emit_and_tile_sort(gs_hip_workspace* ws, uint32_t count) {
+  timer_begin(ws, "emit");
+  int rc = launch_clear_ranges(ws, count);
+  if (rc == 0 && ws->capacity > 0) {
+    k_emit<<<dim3(div_up(ws->capacity, 256), count), 256, 0, ws->stream>>>(ws->dev_views);
+    GS_CHECK_LAUNCH("k_emit");
+  }
+  timer_end(ws);
+  return rc != 0 ? rc : launch_tile_sort_and_ranges(ws, count);
 }
 
 }  // namespace gsh

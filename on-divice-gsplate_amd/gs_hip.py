@@ -23,6 +23,7 @@ FLAG_MASKED_TARGET = 1
 FLAG_SPARSE_ADAM = 2
 FLAG_DEPTH16 = 4
 FLAG_NO_UPDATE = 8
+VIEW_ANTIALIASED = 1
 
 (BUF_PARAMS, BUF_SH, BUF_M_PARAMS, BUF_V_PARAMS, BUF_M_SH, BUF_V_SH, BUF_STATS, BUF_RECORDS, BUF_FOOTPRINT,
  BUF_TILES, BUF_DEPTH_KEYS, BUF_DEPTH_ORDER, BUF_OFFSETS, BUF_GRAD2D, BUF_PAIR_TILES, BUF_PAIR_GAUSSIANS,
@@ -37,7 +38,7 @@ EXPORTED = [
     "gs_hip_workspace_load", "gs_hip_workspace_store", "gs_hip_project", "gs_hip_bin", "gs_hip_forward",
     "gs_hip_set_target", "gs_hip_set_target_device", "gs_hip_loss", "gs_hip_backward", "gs_hip_adam", "gs_hip_densify", "gs_hip_reset_opacity", "gs_hip_train_step", "gs_hip_train_views",
     "gs_hip_render", "gs_hip_workspace_get_tuning", "gs_hip_workspace_set_tuning", "gs_hip_autotune",
-    "gs_hip_workspace_overflows", "gs_hip_workspace_timings",
+    "gs_hip_workspace_overflows", "gs_hip_workspace_timings", "gs_hip_view_render",
 ]
 
 
@@ -104,6 +105,7 @@ def library() -> ctypes.CDLL:
             "gs_hip_densify": [vp, frame_p, u32p, u32p],
             "gs_hip_reset_opacity": [vp, frame_p, ctypes.c_float], "gs_hip_train_step": [vp, frame_p, u64p, dp],
             "gs_hip_render": [vp, frame_p, ctypes.POINTER(ctypes.c_uint8)],
+            "gs_hip_view_render": [vp, frame_p, fp, u32, ctypes.POINTER(ctypes.c_uint8), u32p],
             "gs_hip_workspace_get_tuning": [vp, ctypes.POINTER(Tuning)],
             "gs_hip_workspace_set_tuning": [vp, ctypes.POINTER(Tuning)],
             "gs_hip_autotune": [vp, frame_p, u32, ctypes.POINTER(vp), u32, ctypes.POINTER(Tuning), dp],
@@ -349,6 +351,16 @@ class Workspace:
         check(self.lib.gs_hip_render(self.handle, ctypes.byref(frame),
                                      out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
         return out
+
+    def view_render(self, frame: Frame, clear=(0.0, 0.0, 0.0, 0.0), antialiased: bool = False):
+        """(RGBA8 [H, W, 4], splats drawn): the viewer renderer's image of the
+        loaded model (back-to-front quads into an 8-bit target)."""
+        out = np.empty((self.height, self.width, 4), np.uint8)
+        cl = (ctypes.c_float * 4)(*[float(c) for c in clear])
+        vis = ctypes.c_uint32()
+        check(self.lib.gs_hip_view_render(self.handle, ctypes.byref(frame), cl, VIEW_ANTIALIASED if antialiased else 0,
+                                          out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), ctypes.byref(vis)))
+        return out, vis.value
 
     def tuning(self) -> Tuning:
         t = Tuning()

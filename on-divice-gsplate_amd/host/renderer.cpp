@@ -1,10 +1,13 @@
 // Scene upload and rendering for gs_render (the role of
 // engine/src/splat_renderer.cpp:67-348 and offscreen.cpp): the scene goes
-// into a workspace once; each render is project, bin and front-to-back
-// composite on the GPU, then RGBA8 over the clear colour.
+// into a workspace once, its SH rest rounded to fp16 as the viewer stores it
+// (splat_renderer.cpp:103-116); each render is the viewer's image contract
+// (quads back to front into an 8-bit target over the clear colour) through
+// gs_hip_view_render.
 #include "gs/renderer.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <fstream>
 #include <stdexcept>
@@ -14,8 +17,20 @@
 
 namespace gs {
 
+float round_to_half(float f) {
+  if (!std::isfinite(f)) return f;
+  const float a = std::fabs(f);
+  if (a >= 65520.0f) return std::copysign(INFINITY, f);
+  int e;
+  std::frexp(a, &e);                         // a = m 2^e, m in [0.5, 1)
+  const int step = std::max(e - 11, -24);   // 11 significant bits; subnormal spacing 2^-24
+  const double q = std::nearbyint(std::ldexp(double(a), -step));  // default rounding: ties to even
+  return std::copysign(float(std::ldexp(q, step)), f);
+}
+
 GpuScene::GpuScene(Device& device, const SplatScene& scene, uint32_t width, uint32_t height)
-    : count_(scene.count), width_(width), height_(height), sh_degree_(scene.sh_degree) {
+    : count_(scene.count), width_(width), height_(height), sh_degree_(scene.sh_degree),
+      antialiased_(scene.antialiased) {
   (void)device;
   check_status(gs_hip_workspace_create(&ws_, std::max<uint32_t>(scene.count, 1), scene.sh_degree, width, height));
   std::vector<float> p(size_t(scene.count) * 16, 0.0f);
@@ -31,7 +46,9 @@ GpuScene::GpuScene(Device& device, const SplatScene& scene, uint32_t width, uint
     for (int k = 0; k < 4; ++k) d[8 + k] = scene.rotations[i * 4 + k];
   }
   const int coeffs = sh_rest_coeffs(scene.sh_degree);
-  const int rc = gs_hip_workspace_load(ws_, p.data(), coeffs > 0 ? scene.sh_rest.data() : nullptr, scene.count,
+  std::vector<float> rest(scene.sh_rest.begin(), scene.sh_rest.end());
+  for (float& r : rest) r = round_to_half(r);
+  const int rc = gs_hip_workspace_load(ws_, p.data(), coeffs > 0 ? rest.data() : nullptr, scene.count,
                                        uint32_t(coeffs));
   if (rc != 0) {
     char msg[512];
@@ -65,16 +82,10 @@ std::vector<uint8_t> SplatRenderer::render(const Camera& camera, const float cle
   f.tiles_y = (camera.height + GS_HIP_TILE - 1) / GS_HIP_TILE;
   f.slots = scene_->count();
   f.sh_degree = uint32_t(scene_->sh_degree());
-  for (int c = 0; c < 3; ++c) f.background[c] = clear[c];
+  const float rgba[4] = {clear[0], clear[1], clear[2], 1.0f};
   std::vector<uint8_t> out(size_t(camera.width) * camera.height * 4);
-  check_status(gs_hip_render(scene_->workspace(), &f, out.data()));
-  void* tiles = nullptr;
-  size_t bytes = 0;
-  check_status(gs_hip_workspace_buffer(scene_->workspace(), GS_HIP_BUF_TILES, &tiles, &bytes));
-  std::vector<uint32_t> t(scene_->count());
-  if (!t.empty()) check_status(gs_hip_download(t.data(), tiles, t.size() * 4));
-  visible_ = 0;
-  for (uint32_t v : t) visible_ += v > 0 ? 1 : 0;
+  check_status(gs_hip_view_render(scene_->workspace(), &f, rgba, scene_->antialiased() ? GS_HIP_VIEW_ANTIALIASED : 0u,
+                                  out.data(), &visible_));
   return out;
 }
 

@@ -43,6 +43,10 @@
  *   orc_backward64            central FD of orc_loss64 (h = 1e-4 .. 1e-6)
  *   orc_adam                  closed form; lr 0 leaves m/(1-b1) = g
  *   orc_densify_mark          the predicate of train_densify_mark.comp:28-53
+ *   orc_half                  exhaustive: every fp16 value and the midpoints between
+ *                             neighbours (ties to even)
+ *   orc_view_render32         the reference viewer tests' closed forms (single-splat
+ *                             falloff, depth order, axis-flip equivalence)
  */
 #include <math.h>
 #include <stdint.h>
@@ -536,6 +540,176 @@ void orc_backward32(const float* records, const uint32_t* pair_gaussians, const 
       }
     }
   }
+}
+
+/* ===================== viewer renderer (splat_preprocess.comp, splat.vert/.frag) === */
+/* Round to the nearest fp16 value, ties to even, back to float: the
+ * precision at which the viewer stores colour and opacity (packHalf2x16). */
+float orc_half(float f) {
+  uint32_t x = f2i(f);
+  const uint32_t sign = x & 0x80000000u;
+  const uint32_t ax = x & 0x7fffffffu;
+  if (ax >= 0x7f800000u) return f; /* inf / nan */
+  /* fp16 step at this magnitude: 2^(e-10) for normals, 2^-24 below 2^-14 */
+  int e = (int)(ax >> 23) - 127;
+  if (e > 15) return i2f(sign | 0x7f800000u);
+  const int step_exp = e < -14 ? -24 : e - 10;
+  const double step = ldexp(1.0, step_exp);
+  const double v = (double)i2f(ax) / step;
+  double r = floor(v);
+  const double frac = v - r;
+  if (frac > 0.5 || (frac == 0.5 && fmod(r, 2.0) != 0.0)) r += 1.0;
+  double out = r * step;
+  if (out >= 65520.0) return i2f(sign | 0x7f800000u);
+  return i2f(sign | f2i((float)out));
+}
+
+typedef struct {
+  float mx, my, u1x, u1y, u2x, u2y, op, col[3];
+  uint32_t lo, hi, key;
+  int drawn;
+} view_splat;
+
+static void view_project(const float* p, const float* rest, int coeffs, const orc_camera* cam, uint32_t degree,
+                         int antialiased, view_splat* o) {
+  const float* R = cam->rotation;
+  memset(o, 0, sizeof(*o));
+  o->key = 0xffffu;
+  const float tx = R[0] * p[0] + R[1] * p[1] + R[2] * p[2] + cam->translation[0];
+  const float ty = R[3] * p[0] + R[4] * p[1] + R[5] * p[2] + cam->translation[1];
+  const float tz = R[6] * p[0] + R[7] * p[1] + R[8] * p[2] + cam->translation[2];
+  float opacity = 1.0f / (1.0f + orc_expf(-p[3]));
+  if (!(p[7] > 0.5f && tz > 0.02f && opacity >= 1.0f / 255.0f)) return;
+  const float fx = cam->fx, fy = cam->fy, cx = cam->cx, cy = cam->cy;
+  const float W = (float)cam->width, H = (float)cam->height;
+  const float inv_z = 1.0f / tz;
+  const float mx = fx * tx * inv_z + cx, my = fy * ty * inv_z + cy;
+  const float lim_x = 1.3f * fmaxf(cx, W - cx) / fx, lim_y = 1.3f * fmaxf(cy, H - cy) / fy;
+  const float rx = fminf(fmaxf(tx * inv_z, -lim_x), lim_x), ry = fminf(fmaxf(ty * inv_z, -lim_y), lim_y);
+  const float j00 = fx * inv_z, j02 = -fx * rx * inv_z, j11 = fy * inv_z, j12 = -fy * ry * inv_z;
+  const float t00 = j00 * R[0] + j02 * R[6], t01 = j00 * R[1] + j02 * R[7], t02 = j00 * R[2] + j02 * R[8];
+  const float t10 = j11 * R[3] + j12 * R[6], t11 = j11 * R[4] + j12 * R[7], t12 = j11 * R[5] + j12 * R[8];
+  const float* q = p + 8;
+  const float qlen = fmaxf(sqrtf(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]), 1e-12f);
+  const float w = q[0] / qlen, x = q[1] / qlen, y = q[2] / qlen, z = q[3] / qlen;
+  const float sx = orc_expf(p[4]), sy = orc_expf(p[5]), sz = orc_expf(p[6]);
+  const float m00 = (1.0f - 2.0f * (y * y + z * z)) * sx, m01 = (2.0f * (x * y - w * z)) * sy,
+              m02 = (2.0f * (x * z + w * y)) * sz;
+  const float m10 = (2.0f * (x * y + w * z)) * sx, m11 = (1.0f - 2.0f * (x * x + z * z)) * sy,
+              m12 = (2.0f * (y * z - w * x)) * sz;
+  const float m20 = (2.0f * (x * z - w * y)) * sx, m21 = (2.0f * (y * z + w * x)) * sy,
+              m22 = (1.0f - 2.0f * (x * x + y * y)) * sz;
+  const float v00 = t00 * m00 + t01 * m10 + t02 * m20, v01 = t00 * m01 + t01 * m11 + t02 * m21,
+              v02 = t00 * m02 + t01 * m12 + t02 * m22;
+  const float v10 = t10 * m00 + t11 * m10 + t12 * m20, v11 = t10 * m01 + t11 * m11 + t12 * m21,
+              v12 = t10 * m02 + t11 * m12 + t12 * m22;
+  const float ra = v00 * v00 + v01 * v01 + v02 * v02;
+  const float rb = v00 * v10 + v01 * v11 + v02 * v12;
+  const float rc = v10 * v10 + v11 * v11 + v12 * v12;
+  const float det_raw = ra * rc - rb * rb;
+  const float ca = ra + 0.3f, cc = rc + 0.3f;
+  const float det = ca * cc - rb * rb;
+  if (antialiased) opacity = opacity * sqrtf(fmaxf(det_raw, 0.0f) / fmaxf(det, 1e-12f));
+  if (!(det > 0.0f && opacity >= 1.0f / 255.0f)) return;
+  const float mid = 0.5f * (ca + cc);
+  const float hx = 0.5f * (ca - cc);
+  const float spread = sqrtf(hx * hx + rb * rb);
+  const float l1 = mid + spread, l2 = fmaxf(mid - spread, 0.1f);
+  float e1x = rb, e1y = l1 - ca;
+  const float elen = sqrtf(e1x * e1x + e1y * e1y);
+  if (elen > 1e-12f) {
+    e1x = e1x / elen, e1y = e1y / elen;
+  } else {
+    e1x = 1.0f, e1y = 0.0f;
+  }
+  const float s1 = fminf(sqrtf(2.0f * l1), 2048.0f), s2 = fminf(sqrtf(2.0f * l2), 2048.0f);
+  const float a1x = e1x * s1, a1y = e1y * s1, a2x = -e1y * s2, a2y = e1x * s2;
+  const float extent = 2.0f * sqrtf(a1x * a1x + a1y * a1y);
+  if (!(mx + extent > 0.0f && mx - extent < W && my + extent > 0.0f && my - extent < H)) return;
+  const float dx = p[0] - cam->position[0], dy = p[1] - cam->position[1], dz = p[2] - cam->position[2];
+  const float dlen = sqrtf(dx * dx + dy * dy + dz * dz);
+  float basis[16];
+  sh_basis32(dx / dlen, dy / dlen, dz / dlen, degree, basis);
+  float col[3] = {basis[0] * p[12], basis[0] * p[13], basis[0] * p[14]};
+  int use = rest_coeffs((int)degree);
+  if (use > coeffs) use = coeffs;
+  for (int k = 0; k < use; ++k) {
+    for (int c = 0; c < 3; ++c) col[c] = col[c] + basis[k + 1] * rest[k * 3 + c];
+  }
+  for (int c = 0; c < 3; ++c) col[c] = fmaxf(col[c] + 0.5f, 0.0f);
+  const float n1 = a1x * a1x + a1y * a1y, n2 = a2x * a2x + a2y * a2y;
+  o->mx = mx, o->my = my;
+  o->u1x = a1x / n1, o->u1y = a1y / n1, o->u2x = a2x / n2, o->u2y = a2y / n2;
+  o->op = orc_half(opacity);
+  for (int c = 0; c < 3; ++c) o->col[c] = orc_half(col[c]);
+  const float ex = 2.0f * (fabsf(a1x) + fabsf(a2x)), ey = 2.0f * (fabsf(a1y) + fabsf(a2y));
+  const int x0 = (int)fmaxf(ceilf(mx - ex - 0.5f), 0.0f), x1 = (int)fminf(floorf(mx + ex - 0.5f), W - 1.0f);
+  const int y0 = (int)fmaxf(ceilf(my - ey - 0.5f), 0.0f), y1 = (int)fminf(floorf(my + ey - 0.5f), H - 1.0f);
+  if (x0 <= x1 && y0 <= y1) {
+    o->lo = (uint32_t)x0 | ((uint32_t)y0 << 16);
+    o->hi = (uint32_t)x1 | ((uint32_t)y1 << 16);
+  } else {
+    o->lo = 1u, o->hi = 0u;
+  }
+  const float u = orc_logf(tz / 0.02f) / orc_logf(1000.0f / 0.02f);
+  o->key = 65534u - (uint32_t)(fminf(fmaxf(u, 0.0f), 1.0f) * 65534.0f);
+  o->drawn = 1;
+}
+
+static int bbox_overlaps(uint32_t lo, uint32_t hi, int rx0, int ry0, int rx1, int ry1) {
+  const int bx0 = (int)(lo & 0xffffu), by0 = (int)(lo >> 16), bx1 = (int)(hi & 0xffffu), by1 = (int)(hi >> 16);
+  return bx0 <= rx1 && bx1 >= rx0 && by0 <= ry1 && by1 >= ry0;
+}
+
+static const view_splat* g_view_splats;
+static int cmp_view(const void* a, const void* b) {
+  const uint32_t ia = *(const uint32_t*)a, ib = *(const uint32_t*)b;
+  const uint32_t ka = g_view_splats[ia].key, kb = g_view_splats[ib].key;
+  if (ka != kb) return ka < kb ? -1 : 1;
+  return ia < ib ? -1 : (ia > ib ? 1 : 0);
+}
+
+/* The viewer image: params [n][16], rest [n][coeffs][3] (used as given: pass
+ * them rounded to fp16 for the reference's storage), RGBA8 out, *visible =
+ * splats drawn.  Per pixel, every drawn splat in key order (ties by index)
+ * whose bbox reaches the pixel's 8x8 quadrant and whose quad holds the
+ * pixel centre is blended into an 8-bit target: colour = round(min(c a, 1)
+ * 255 + (1 - a) colour), the same for alpha with a. */
+void orc_view_render32(const float* params, const float* rest, uint32_t n, uint32_t coeffs, const orc_camera* cam,
+                       uint32_t degree, int antialiased, const float clear[4], uint8_t* rgba, uint32_t* visible) {
+  view_splat* v = (view_splat*)calloc(n ? n : 1, sizeof(view_splat));
+  uint32_t* order = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+  uint32_t drawn = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    view_project(params + (size_t)i * 16, rest + (size_t)i * coeffs * 3, (int)coeffs, cam, degree, antialiased, &v[i]);
+    if (v[i].drawn) order[drawn++] = i;
+  }
+  g_view_splats = v;
+  qsort(order, drawn, sizeof(uint32_t), cmp_view);
+  for (uint32_t py = 0; py < cam->height; ++py) {
+    for (uint32_t px = 0; px < cam->width; ++px) {
+      const int qx = (int)(px / 8) * 8, qy = (int)(py / 8) * 8;
+      const float cx = (float)px + 0.5f, cy = (float)py + 0.5f;
+      float c[4];
+      for (int k = 0; k < 4; ++k) c[k] = rintf(fminf(fmaxf(clear[k], 0.0f), 1.0f) * 255.0f);
+      for (uint32_t t = 0; t < drawn; ++t) {
+        const view_splat* s = &v[order[t]];
+        if (!bbox_overlaps(s->lo, s->hi, qx, qy, qx + 7, qy + 7)) continue;
+        const float dx = cx - s->mx, dy = cy - s->my;
+        const float lx = dx * s->u1x + dy * s->u1y, ly = dx * s->u2x + dy * s->u2y;
+        const float alpha = fminf(s->op * orc_expf(-(lx * lx + ly * ly)), 0.99f);
+        if (!(fabsf(lx) <= 2.0f && fabsf(ly) <= 2.0f && alpha >= 1.0f / 255.0f)) continue;
+        const float keep = 1.0f - alpha;
+        for (int k = 0; k < 3; ++k) c[k] = rintf(fminf(fminf(s->col[k] * alpha, 1.0f) * 255.0f + keep * c[k], 255.0f));
+        c[3] = rintf(fminf(alpha * 255.0f + keep * c[3], 255.0f));
+      }
+      uint8_t* o = rgba + ((size_t)py * cam->width + px) * 4;
+      for (int k = 0; k < 4; ++k) o[k] = (uint8_t)c[k];
+    }
+  }
+  *visible = drawn;
+  free(order);
+  free(v);
 }
 
 /* ===================== fp64 reference ====================================== */

@@ -59,6 +59,9 @@ def library() -> ctypes.CDLL:
             "orc_adjoint64": ([P(f64), P(f64), u32, u32, cam_p, i32, P(f64), P(f64), P(f64)], None),
             "orc_adam": ([P(f32), P(f32), P(f32), P(f32), P(f32), u64, f32, f32, f32, f32, f32], None),
             "orc_densify_mark": ([P(f32), P(f32), u32, f32, f32, f32, f32, f32, P(u32), P(u32), P(u32)], None),
+            "orc_half": ([f32], f32),
+            "orc_view_render32": ([P(f32), P(f32), u32, u32, cam_p, u32, i32, P(f32), P(ctypes.c_uint8), P(u32)],
+                                  None),
         }
         for name, (args, res) in sig.items():
             fn = getattr(lib, name)
@@ -282,3 +285,27 @@ def adjoint64(params, sh, cam, degree: int, grad2d):
                             ctypes.byref(camera(cam)), degree, _p(g2, ctypes.c_double), _p(gp, ctypes.c_double),
                             _p(gsp, ctypes.c_double))
     return gp, gs
+
+
+def half(x) -> np.ndarray:
+    """Round to the nearest fp16 value (ties to even), as float32."""
+    f = library().orc_half
+    a = np.asarray(x, np.float32)
+    return np.array([f(float(v)) for v in a.reshape(-1)], np.float32).reshape(a.shape)
+
+
+def view_render32(params, sh, cam, degree: int, antialiased: bool = False, clear=(0.0, 0.0, 0.0, 0.0)):
+    """(rgba [H, W, 4] uint8, visible) of the viewer renderer; `sh` is used as
+    given (round it to fp16 first for the viewer's storage precision)."""
+    params = np.ascontiguousarray(params, np.float32).reshape(-1, 16)
+    n = params.shape[0]
+    sh = np.ascontiguousarray(sh if sh is not None else np.zeros((n, 0, 3)), np.float32).reshape(n, -1, 3)
+    coeffs = sh.shape[1]
+    shp = sh if sh.size else np.zeros(1, np.float32)
+    out = np.zeros((cam.height, cam.width, 4), np.uint8)
+    cl = np.ascontiguousarray(clear, np.float32)
+    vis = ctypes.c_uint32(0)
+    library().orc_view_render32(_p(params, ctypes.c_float), _p(shp, ctypes.c_float), n, coeffs,
+                                ctypes.byref(camera(cam)), degree, int(antialiased), _p(cl, ctypes.c_float),
+                                _p(out, ctypes.c_uint8), ctypes.byref(vis))
+    return out, int(vis.value)

@@ -1,7 +1,9 @@
 // Scene I/O and camera conventions of the C++ API, without a GPU:
 // PLY round trip (bit-equal), ASCII PLY rejected, the reference's SPZ v2
 // known answers (engine/tests/test_splats.cpp:34-130), SPZ v3 quaternions,
-// v4 rejected by name, the axis flip and dataset cameras.
+// v4 rejected by name, the axis flip and dataset cameras; the viewer's fp16
+// rounding of SH rest against the compiler's _Float16 conversion, over every
+// fp16 value and the midpoints between neighbours.
 #include <zlib.h>
 
 #include <cmath>
@@ -12,6 +14,7 @@
 
 #include "gs/camera.h"
 #include "gs/dataset.h"
+#include "gs/renderer.h"
 #include "gs/splat_scene.h"
 #include "test_util.h"
 
@@ -169,6 +172,27 @@ void test_dataset_camera() {
   }
 }
 
+void test_round_to_half() {
+  int bad = 0;
+  float prev = -1.0f;
+  for (uint32_t b = 0; b < 0x7c00u; ++b) {  // every finite non-negative fp16
+    _Float16 h;
+    const uint16_t bits = uint16_t(b);
+    std::memcpy(&h, &bits, 2);
+    const float v = float(h);
+    for (float x : {v, -v}) bad += gs::round_to_half(x) != float(_Float16(x));
+    if (prev >= 0.0f) {
+      const float mid = float((double(prev) + double(v)) / 2.0);  // exact in fp32
+      for (float x : {mid, std::nextafter(mid, 0.0f), std::nextafter(mid, 1e30f), -mid}) {
+        bad += gs::round_to_half(x) != float(_Float16(x));
+      }
+    }
+    prev = v;
+  }
+  for (float x : {65519.0f, 65520.0f, 1e6f, 1e-9f, -65520.0f}) bad += gs::round_to_half(x) != float(_Float16(x));
+  EXPECT(bad == 0, "round_to_half: %d values differ from _Float16", bad);
+}
+
 }  // namespace
 
 int main() {
@@ -177,6 +201,7 @@ int main() {
   test_spz_v3_quaternion();
   test_convention_flip();
   test_dataset_camera();
+  test_round_to_half();
   if (test::failures() == 0) std::printf("test_scene_io: ok\n");
   return test::failures() == 0 ? 0 : 1;
 }

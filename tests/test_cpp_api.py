@@ -1,5 +1,5 @@
 """Runs the C++ API tests (tests/cpp) built by the package Makefile: scene
-I/O without a GPU, the gs::Trainer on the GPU."""
+I/O without a GPU, the gs::Trainer and gs::SplatRenderer on the GPU."""
 import os
 import subprocess
 
@@ -27,3 +27,8 @@ def test_scene_io():
 @pytest.mark.gpu
 def test_trainer_api():
     _run("test_trainer", 300)
+
+
+@pytest.mark.gpu
+def test_renderer_api():
+    _run("test_renderer", 120)
