@@ -1,7 +1,8 @@
 // Posed images a trainer fits (engine/include/gs/dataset.h:12-85): frames with
-// an OpenGL camera-to-world pose and an RGBA8 image at training resolution.
-// The transforms.json loader is not part of this build; datasets come from
-// code (make_orbit_dataset) or from the caller.
+// an OpenGL camera-to-world pose and an RGBA8 image at training resolution,
+// from a nerfstudio transforms.json folder (load_dataset), from code
+// (make_orbit_dataset) or from the caller; and the initial gaussians built
+// from a dataset's depth maps, sparse points or cameras.
 #pragma once
 
 #include <cstdint>
@@ -9,6 +10,7 @@
 #include <vector>
 
 #include "gs/camera.h"
+#include "gs/splat_scene.h"
 
 namespace gs {
 
@@ -16,21 +18,38 @@ struct DatasetFrame {
   std::string name;
   float c2w[16];              // camera-to-world, OpenGL convention, row-major
   std::vector<uint8_t> rgba;  // training resolution; alpha = subject mask (255 without masks)
-  bool holdout = false;       // reserved for evaluation
+  uint32_t depth_width = 0, depth_height = 0;
+  std::vector<float> depth;                // metres, 0 = missing (low resolution, as phones give it)
+  std::vector<uint8_t> depth_confidence;   // 0..255 per depth sample, or empty
+  bool holdout = false;                    // reserved for evaluation
 };
 
 struct Dataset {
   std::string name;
-  uint32_t width = 0, height = 0;
+  uint32_t width = 0, height = 0;  // training resolution
   float fx = 0, fy = 0, cx = 0, cy = 0;
   bool has_masks = false;
+  bool has_depth = false;
   std::vector<DatasetFrame> frames;
+  std::vector<float> points;          // optional sparse point cloud, xyz
+  std::vector<uint8_t> point_colors;  // rgb per point
   float camera_center[3] = {0, 0, 0};
-  float camera_extent = 1.0f;  // 1.1 x the largest camera distance from camera_center
+  float camera_extent = 1.0f;      // 1.1 x the largest camera distance from camera_center
+  float world_up[3] = {0, 1, 0};   // gsplat_lab.world_up when the capture gives it
+  std::vector<std::string> skipped;  // frames left out because their files could not be read
 
   size_t train_count() const;
   size_t holdout_count() const;
 };
+
+// Loads `directory`/transforms.json (nerfstudio layout: w, h, fl_x[, fl_y, cx,
+// cy], frames with file_path and transform_matrix, optional mask_path,
+// depth_file_path, depth_confidence_path, depth_unit_scale_factor,
+// ply_file_path).  Images are area-resampled so their short side is
+// `target_width` (never enlarged); every 8th frame is held out.  A frame
+// whose files cannot be read is skipped and listed in `skipped`; fewer than
+// two readable frames, or fewer than half, throws std::runtime_error.
+Dataset load_dataset(const std::string& directory, uint32_t target_width);
 
 // Camera of a frame: the OpenGL c2w turned into scene -> camera OpenCV axes.
 Camera dataset_camera(const Dataset& dataset, const DatasetFrame& frame);
@@ -40,5 +59,28 @@ void compute_camera_bounds(Dataset& dataset);
 // looking at it, vertical field of view `fov_y_deg`; images left black.
 Dataset make_orbit_dataset(uint32_t views, uint32_t width, uint32_t height, float radius, float height_y,
                            float fov_y_deg, uint32_t holdout_every = 0);
+
+struct InitOptions {
+  uint32_t max_gaussians = 150000;
+  bool subject_only = true;     // with masks: keep only depth points inside the subject
+  float min_confidence = 0.3f;  // depth confidence threshold, 0..1
+  float initial_opacity = 0.2f;
+  uint32_t seed = 1;
+};
+
+struct InitStats {
+  std::string source;  // "depth", "points" or "random"
+  size_t raw_points = 0;
+  float voxel_size = 0;  // 0 when no merging was needed
+};
+
+// Initial gaussians (SH degree 0, RUB) from the training frames' depth maps,
+// else the sparse points, else random points in a ball of half the camera
+// extent around the cameras' centre.  Over budget, points are merged per
+// voxel (the cell size whose occupancy falls just under max_gaussians);
+// gaussians are laid out in Morton order; each one's scale is the mean
+// distance to its three nearest neighbours, clamped to [1e-4, 2e-2] of the
+// points' extent.
+SplatScene initialize_gaussians(const Dataset& dataset, const InitOptions& options, InitStats* stats = nullptr);
 
 }  // namespace gs

@@ -1,0 +1,31 @@
+// A JSON document as a tree of values (the role of engine/include/gs/json_parse.h
+// for transforms.json): objects keep their keys in file order.
+#pragma once
+
+#include <string>
+#include <string_view>
+#include <utility>
+#include <vector>
+
+namespace gs {
+
+struct JsonValue {
+  enum class Type { Null, Bool, Number, String, Array, Object };
+  Type type = Type::Null;
+  bool boolean = false;
+  double number = 0.0;
+  std::string string;
+  std::vector<JsonValue> array;
+  std::vector<std::pair<std::string, JsonValue>> object;
+
+  // The member `key` of an object (the last one if repeated), else nullptr.
+  const JsonValue* find(const std::string& key) const;
+  double number_or(const std::string& key, double fallback) const;
+  std::string string_or(const std::string& key, const std::string& fallback) const;
+};
+
+// Parses RFC 8259 JSON; throws std::runtime_error naming the byte offset of
+// the first error.
+JsonValue parse_json(std::string_view text);
+
+}  // namespace gs

@@ -55,6 +55,14 @@ void convert_convention(SplatScene& scene, AxisConvention target);
 // A torus knot of anisotropic splats with degree-1 SH, in RUB convention.
 SplatScene make_synthetic_scene(uint32_t count, uint32_t seed = 1);
 
+// xyz (and rgb, 128 where absent) of the vertices of a binary little-endian
+// PLY point cloud.
+struct PointCloud {
+  std::vector<float> xyz;
+  std::vector<uint8_t> rgb;
+};
+PointCloud load_ply_points(const uint8_t* data, size_t size);
+
 // Reads a whole file; throws std::runtime_error if it cannot.
 std::vector<uint8_t> read_file(const std::string& path);
 void write_file(const std::string& path, const std::vector<uint8_t>& data);

@@ -1,12 +1,20 @@
-// gs_train <out.ply> [options]: trains gaussians against posed views and
-// writes the result as a 3DGS PLY (the command surface of
-// engine/cli/gs_train.cpp, with the views synthesised on an orbit: the
-// transforms.json dataset loader is not part of this build).
+// Trains gaussians and writes a 3DGS .ply (the command surface of
+// engine/cli/gs_train.cpp:1-345).
 //
-//   --gt <scene.ply|scene.spz>  ground-truth scene the target views are rendered from
-//   --gt-random N               or N seeded random gaussians (default 200000)
-//   --init-random N             initial gaussians (default: as many as the ground truth)
-//   --views V --size WxH --iters I --sh D --seed S --lambda L --no-densify --eval-every N
+//   gs_train <dataset_dir> <out.ply> [--width 360] [--iters 3000] [--max-gaussians N] [--sh 3]
+//            [--init-gaussians 150000] [--opacity-reset N (0 = off)] [--grad-threshold 2e-4]
+//            [--dssim 0.2] [--dssim-end L] [--no-masks] [--sparse-adam] [--preview <dir>] [--eval-every 1000]
+//            [--views-per-step V] [--raster-groups G] [--depth-bits B] [--tile-bits B] [--graph 0|1] [--autotune]
+//
+// The dataset is a nerfstudio transforms.json folder (gs::load_dataset); the
+// initial gaussians come from its depth maps, sparse points or cameras.  The
+// reference's --reduction / --alternatives (its backward variants) become the
+// launch-shape flags.  With --synthetic instead of a dataset, the views are
+// rendered on an orbit from a ground-truth scene:
+//
+//   gs_train --synthetic <out.ply> [--gt scene.ply|scene.spz | --gt-random N] [--init-random N]
+//            [--views V] [--size WxH] [--iters I] [--sh D] [--seed S] [--dssim L] [--no-densify] [...]
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -50,77 +58,165 @@ gs::SplatScene random_scene(uint32_t n, int degree, uint32_t seed) {
   return s;
 }
 
+struct Args {
+  bool synthetic = false;
+  std::string dataset, out, gt_path, preview;
+  uint32_t width = 360, iters = 3000, init_gaussians = 150000, eval_every = 1000;
+  int reset_override = -1;
+  float grad_threshold = -1.0f;
+  gs::TrainOptions o;
+  // synthetic views
+  uint32_t gt_random = 200000, init_random = 0, views = 24, size_w = 640, size_h = 360, seed = 1;
+  bool densify = true;
+};
+
+void usage(const char* argv0) {
+  std::fprintf(stderr,
+               "usage: %s <dataset_dir> <out.ply> [--width 360] [--iters 3000] [--max-gaussians N] [--sh 3]\n"
+               "         [--init-gaussians N] [--opacity-reset N] [--grad-threshold G] [--dssim L] [--dssim-end L]\n"
+               "         [--no-masks] [--sparse-adam] [--preview <dir>] [--eval-every N] [--views-per-step V]\n"
+               "         [--raster-groups G] [--depth-bits B] [--tile-bits B] [--graph 0|1] [--autotune]\n"
+               "       %s --synthetic <out.ply> [--gt scene | --gt-random N] [--init-random N] [--views V]\n"
+               "         [--size WxH] [--iters I] [--sh D] [--seed S] [--dssim L] [--no-densify] [...]\n",
+               argv0, argv0);
+}
+
+Args parse(int argc, char** argv) {
+  Args a;
+  int i = 1;
+  if (argc >= 3 && std::string(argv[1]) == "--synthetic") {
+    a.synthetic = true;
+    a.out = argv[2];
+    i = 3;
+    a.iters = 300;
+    a.eval_every = 0;
+    a.o.masked = false;
+  } else if (argc >= 3 && argv[1][0] != '-') {
+    a.dataset = argv[1];
+    a.out = argv[2];
+    i = 3;
+  } else {
+    throw std::invalid_argument("missing <dataset_dir> <out.ply>");
+  }
+  a.o.max_gaussians = 300000;
+  for (; i < argc; ++i) {
+    const std::string s = argv[i];
+    auto next = [&]() -> const char* {
+      if (i + 1 >= argc) throw std::invalid_argument(s + " needs a value");
+      return argv[++i];
+    };
+    if (s == "--width") a.width = uint32_t(std::atoi(next()));
+    else if (s == "--iters") a.iters = uint32_t(std::atoi(next()));
+    else if (s == "--max-gaussians") a.o.max_gaussians = uint32_t(std::atoi(next()));
+    else if (s == "--sh") a.o.sh_degree = std::atoi(next());
+    else if (s == "--init-gaussians") a.init_gaussians = uint32_t(std::atoi(next()));
+    else if (s == "--opacity-reset") a.reset_override = std::atoi(next());
+    else if (s == "--grad-threshold") a.grad_threshold = float(std::atof(next()));
+    else if (s == "--dssim" || s == "--lambda") a.o.lambda_dssim = float(std::atof(next()));
+    else if (s == "--dssim-end") a.o.lambda_dssim_end = float(std::atof(next()));
+    else if (s == "--no-masks") a.o.masked = false;
+    else if (s == "--sparse-adam") a.o.sparse_adam = true;
+    else if (s == "--preview") a.preview = next();
+    else if (s == "--eval-every") a.eval_every = uint32_t(std::atoi(next()));
+    else if (s == "--views-per-step") a.o.views_per_step = uint32_t(std::atoi(next()));
+    else if (s == "--raster-groups") a.o.raster_groups_per_cu = std::atoi(next());
+    else if (s == "--depth-bits") a.o.depth_digit_bits = std::atoi(next());
+    else if (s == "--tile-bits") a.o.tile_digit_bits = std::atoi(next());
+    else if (s == "--graph") a.o.graph = std::atoi(next());
+    else if (s == "--autotune") a.o.autotune = true;
+    else if (a.synthetic && s == "--gt") a.gt_path = next();
+    else if (a.synthetic && s == "--gt-random") a.gt_random = uint32_t(std::atoi(next()));
+    else if (a.synthetic && s == "--init-random") a.init_random = uint32_t(std::atoi(next()));
+    else if (a.synthetic && s == "--views") a.views = uint32_t(std::atoi(next()));
+    else if (a.synthetic && s == "--size") {
+      if (std::sscanf(next(), "%ux%u", &a.size_w, &a.size_h) != 2) throw std::invalid_argument("--size WxH");
+    } else if (a.synthetic && s == "--seed") a.seed = uint32_t(std::atoi(next()));
+    else if (a.synthetic && s == "--no-densify") a.densify = false;
+    else throw std::invalid_argument("unknown option " + s);
+  }
+  return a;
+}
+
+// Keeps the schedule proportional for shorter or longer runs (the reference's scaled defaults).
+void scale_schedule(gs::TrainOptions& o, uint32_t iters) {
+  const gs::TrainOptions d = gs::scaled_options(iters);
+  o.iterations = d.iterations;
+  o.densify_from = d.densify_from;
+  o.densify_until = d.densify_until;
+  o.opacity_reset_interval = d.opacity_reset_interval;
+  o.sh_degree_interval = d.sh_degree_interval;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
-  if (argc < 2 || argv[1][0] == '-') {
-    std::fprintf(stderr,
-                 "usage: %s <out.ply> [--gt scene.ply|--gt-random N] [--init-random N] [--views V] [--size WxH] "
-                 "[--iters I] [--sh D] [--seed S] [--lambda L] [--no-densify] [--eval-every N]\n",
-                 argv[0]);
+  Args a;
+  try {
+    a = parse(argc, argv);
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "gs_train: %s\n", e.what());
+    usage(argv[0]);
     return 2;
   }
   try {
-    std::string out = argv[1], gt_path;
-    uint32_t gt_random = 200000, init_random = 0, views = 24, width = 640, height = 360, iters = 300, seed = 1;
-    uint32_t eval_every = 0;
-    int sh = 3;
-    float lambda = 0.2f;
-    bool densify = true;
-    for (int i = 2; i < argc; ++i) {
-      const std::string a = argv[i];
-      auto next = [&]() -> const char* {
-        if (i + 1 >= argc) throw std::invalid_argument("missing value after " + a);
-        return argv[++i];
-      };
-      if (a == "--gt") gt_path = next();
-      else if (a == "--gt-random") gt_random = uint32_t(std::atoi(next()));
-      else if (a == "--init-random") init_random = uint32_t(std::atoi(next()));
-      else if (a == "--views") views = uint32_t(std::atoi(next()));
-      else if (a == "--size") {
-        if (std::sscanf(next(), "%ux%u", &width, &height) != 2) throw std::invalid_argument("--size WxH");
-      } else if (a == "--iters") iters = uint32_t(std::atoi(next()));
-      else if (a == "--sh") sh = std::atoi(next());
-      else if (a == "--seed") seed = uint32_t(std::atoi(next()));
-      else if (a == "--lambda") lambda = float(std::atof(next()));
-      else if (a == "--no-densify") densify = false;
-      else if (a == "--eval-every") eval_every = uint32_t(std::atoi(next()));
-      else throw std::invalid_argument("unknown option " + a);
-    }
+    using clock = std::chrono::steady_clock;
     gs::Device device(0);
     std::printf("device: %s\n", device.name().c_str());
-    gs::SplatScene gt;
-    if (!gt_path.empty()) {
-      const auto bytes = gs::read_file(gt_path);
-      gt = gs::load_splat_file(bytes.data(), bytes.size());
-      gs::convert_convention(gt, gs::AxisConvention::RUB);
+    gs::Dataset ds;
+    gs::SplatScene init;
+    if (!a.synthetic) {
+      const auto t0 = clock::now();
+      ds = gs::load_dataset(a.dataset, a.width);
+      std::printf("dataset %s: %zu frames (%zu held out, %zu skipped), %ux%u, masks %d, depth %d, %zu points (%.1f s)\n",
+                  ds.name.c_str(), ds.frames.size(), ds.holdout_count(), ds.skipped.size(), ds.width, ds.height,
+                  int(ds.has_masks), int(ds.has_depth), ds.points.size() / 3,
+                  std::chrono::duration<double>(clock::now() - t0).count());
+      for (const std::string& s : ds.skipped) std::printf("  skipped %s\n", s.c_str());
+      gs::InitOptions io;
+      io.subject_only = a.o.masked;
+      io.max_gaussians = a.init_gaussians;
+      gs::InitStats st;
+      init = gs::initialize_gaussians(ds, io, &st);
+      std::printf("init from %s: %zu points -> %u gaussians (voxel %.4f)\n", st.source.c_str(), st.raw_points,
+                  init.count, st.voxel_size);
+      scale_schedule(a.o, a.iters);
     } else {
-      gt = random_scene(gt_random, sh, 1001);
-    }
-    gs::Dataset ds = gs::make_orbit_dataset(views, width, height, 3.0f, 0.5f, 60.0f, 8);
-    {
-      auto scene = gs::upload_scene(device, gt, width, height);
-      gs::SplatRenderer r(device);
-      r.attach(scene.get());
-      const float black[3] = {0, 0, 0};
-      for (gs::DatasetFrame& f : ds.frames) {
-        f.rgba = r.render(gs::dataset_camera(ds, f), black);
-        for (size_t p = 0; p < size_t(width) * height; ++p) f.rgba[p * 4 + 3] = 255;
+      gs::SplatScene gt;
+      if (!a.gt_path.empty()) {
+        const auto bytes = gs::read_file(a.gt_path);
+        gt = gs::load_splat_file(bytes.data(), bytes.size());
+        gs::convert_convention(gt, gs::AxisConvention::RUB);
+      } else {
+        gt = random_scene(a.gt_random, a.o.sh_degree, 1001);
       }
+      ds = gs::make_orbit_dataset(a.views, a.size_w, a.size_h, 3.0f, 0.5f, 60.0f, 8);
+      {
+        auto scene = gs::upload_scene(device, gt, a.size_w, a.size_h);
+        gs::SplatRenderer r(device);
+        r.attach(scene.get());
+        const float black[3] = {0, 0, 0};
+        for (gs::DatasetFrame& f : ds.frames) {
+          f.rgba = r.render(gs::dataset_camera(ds, f), black);
+          for (size_t p = 0; p < size_t(a.size_w) * a.size_h; ++p) f.rgba[p * 4 + 3] = 255;
+        }
+      }
+      init = random_scene(a.init_random ? a.init_random : gt.count, a.o.sh_degree, 2002 + a.seed);
+      scale_schedule(a.o, a.iters);
+      a.o.sh_degree_interval = 0;
+      a.o.lambda_dssim_end = -1.0f;
+      a.o.random_background = false;
+      a.o.max_gaussians = std::max(a.o.max_gaussians, 2 * init.count);
+      a.o.seed = a.seed;
+      if (!a.densify) a.o.densify_from = a.o.densify_until = ~0u;
     }
-    const gs::SplatScene init = random_scene(init_random ? init_random : gt.count, sh, 2002 + seed);
-    gs::TrainOptions o = gs::scaled_options(iters);
-    o.sh_degree = sh;
-    o.sh_degree_interval = 0;
-    o.lambda_dssim = lambda;
-    o.lambda_dssim_end = -1.0f;
-    o.random_background = false;
-    o.max_gaussians = 2 * init.count;
-    o.seed = seed;
-    if (!densify) o.densify_from = o.densify_until = ~0u;
-    gs::Trainer trainer(device, std::move(ds), init, o);
+    if (a.reset_override >= 0) a.o.opacity_reset_interval = uint32_t(a.reset_override);
+    if (a.grad_threshold >= 0.0f) a.o.densify_grad_threshold = a.grad_threshold;
+    const uint32_t width = ds.width, height = ds.height;
+    gs::Trainer trainer(device, std::move(ds), init, a.o);
+    const auto t_start = clock::now();
+    uint32_t last_print = 0;
     while (!trainer.finished()) {
-      const bool profile = trainer.stats().step % 100 == 50;
+      const bool profile = trainer.stats().optimizer_steps % 100 == 50;
       trainer.step(profile);
       const gs::TrainStats& st = trainer.stats();
       if (profile) {
@@ -128,19 +224,34 @@ int main(int argc, char** argv) {
         std::printf("  pass ms: project %.3f sort %.3f raster %.3f ssim %.3f backward %.3f optimize %.3f\n",
                     p.project, p.sort, p.rasterize, p.ssim, p.backward, p.optimize);
       }
-      if (st.step % 100 == 0 || trainer.finished()) {
+      if (st.step / 100 != last_print / 100 || trainer.finished()) {
         std::printf("step %u: loss %.5f psnr %.2f ssim %.4f gaussians %u pairs %llu  %.2f ms/step\n", st.step,
                     st.loss, st.psnr, st.ssim, st.gaussians, (unsigned long long)st.pairs, st.step_ms);
       }
-      if (eval_every > 0 && st.step % eval_every == 0) {
+      if (a.eval_every > 0 && (st.step / a.eval_every != last_print / a.eval_every || trainer.finished())) {
         const gs::EvalResult e = trainer.evaluate();
         std::printf("  eval: %u views psnr %.2f ssim %.4f\n", e.views, e.psnr, e.ssim);
+        if (!a.preview.empty()) {
+          const gs::Dataset& d = trainer.dataset();
+          const float black[3] = {0, 0, 0};
+          for (size_t f = 0; f < d.frames.size(); f += std::max<size_t>(1, d.frames.size() / 4)) {
+            char name[64];
+            std::snprintf(name, sizeof(name), "/step%05u_view%03zu.ppm", st.step, f);
+            gs::write_ppm(a.preview + name, trainer.render(trainer.camera_for(f), black), width, height);
+          }
+        }
       }
+      last_print = st.step;
     }
+    const gs::TrainStats& st = trainer.stats();
+    const double secs = std::chrono::duration<double>(clock::now() - t_start).count();
+    std::printf("trained %u views in %u optimizer steps, %.1f s (%.1f views/s); %u densify events, %u overflow reruns\n",
+                st.step, st.optimizer_steps, secs, st.step / std::max(secs, 1e-9), st.densify_events,
+                st.overflow_steps);
     gs::SplatScene result = trainer.export_scene();
     gs::convert_convention(result, gs::AxisConvention::RDF);
-    gs::write_file(out, gs::save_ply(result));
-    std::printf("wrote %s: %u gaussians\n", out.c_str(), result.count);
+    gs::write_file(a.out, gs::save_ply(result));
+    std::printf("wrote %s: %u gaussians\n", a.out.c_str(), result.count);
     return 0;
   } catch (const std::exception& e) {
     std::fprintf(stderr, "gs_train: %s\n", e.what());

@@ -130,7 +130,7 @@ void compute_camera_bounds(Dataset& ds) {
     far = std::max(far, std::sqrt(d));
   }
   for (int k = 0; k < 3; ++k) ds.camera_center[k] = float(mean[k]);
-  ds.camera_extent = float(1.1 * far);
+  ds.camera_extent = std::max(float(1.1 * far), 1e-3f);
 }
 
 Dataset make_orbit_dataset(uint32_t views, uint32_t width, uint32_t height, float radius, float height_y,

@@ -1,7 +1,8 @@
 // gs::Trainer on the GPU, restating the reference's trainer tests
 // (engine/tests/test_trainer.cpp): pair order (197-233), frozen-rate
 // gradients and parameters, render size check, densification at capacity
-// (866-888) and fitting a known scene (890-897; train_selftest.cpp:692-739).
+// (866-888) and fitting a known scene (890-897; train_selftest.cpp:692-739),
+// one view per optimizer step and four (batched, autotuned).
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -159,7 +160,10 @@ void test_densify_at_capacity(gs::Device& dev) {
   EXPECT(t.stats().densify_events >= 1, "densified");
 }
 
-void test_training_converges(gs::Device& dev) {
+// views_per_step > 1: the batched step (mean gradient of the views, one launch
+// per stage), run for as many optimizer steps as the plain run, with the
+// launch shape autotuned on the first views.
+void test_training_converges(gs::Device& dev, uint32_t views_per_step) {
   const int degree = 1;
   gs::Dataset ds = test_dataset(96, 72, 24, true, 9);
   const gs::SplatScene truth = test_scene(60, degree, 21);
@@ -188,13 +192,20 @@ void test_training_converges(gs::Device& dev) {
   o.sh_degree_interval = 200;
   o.densify_from = o.densify_until = 1000000;
   o.max_gaussians = 2000;
+  o.views_per_step = views_per_step;
+  o.autotune = views_per_step > 1;
+  o.iterations *= views_per_step;
+  o.sh_degree_interval *= views_per_step;
   gs::Trainer t(dev, ds, init, o);
   const double before = t.evaluate().psnr;
   while (!t.finished()) t.step();
   const gs::EvalResult after = t.evaluate();
-  std::printf("  fit known scene: held-out PSNR %.2f -> %.2f dB (%u views), %.2f ms/step\n", before, after.psnr,
-              after.views, t.stats().step_ms);
+  std::printf("  fit known scene, %u view(s) per step: held-out PSNR %.2f -> %.2f dB (%u views), %u optimizer steps, "
+              "%.2f ms/step\n", views_per_step, before, after.psnr, after.views, t.stats().optimizer_steps,
+              t.stats().step_ms);
   EXPECT(after.psnr > 35.0 && after.psnr > before + 8.0, "PSNR %.2f -> %.2f", before, after.psnr);
+  EXPECT(t.stats().step == o.iterations && t.stats().optimizer_steps == 600, "steps %u, optimizer steps %u",
+         t.stats().step, t.stats().optimizer_steps);
 }
 
 }  // namespace
@@ -207,7 +218,8 @@ int main() {
     test_pair_order(dev);
     test_frozen_step(dev);
     test_densify_at_capacity(dev);
-    test_training_converges(dev);
+    test_training_converges(dev, 1);
+    test_training_converges(dev, 4);
   } catch (const std::exception& e) {
     std::fprintf(stderr, "exception: %s\n", e.what());
     return 2;

@@ -1,5 +1,6 @@
 """Runs the C++ API tests (tests/cpp) built by the package Makefile: scene
-I/O without a GPU, the gs::Trainer and gs::SplatRenderer on the GPU."""
+I/O, image files, JSON and the dataset loader without a GPU; the gs::Trainer
+and gs::SplatRenderer on the GPU."""
 import os
 import subprocess
 
@@ -22,6 +23,14 @@ def _run(name, timeout):
 
 def test_scene_io():
     _run("test_scene_io", 120)
+
+
+def test_image_io():
+    _run("test_image_io", 120)
+
+
+def test_dataset_loader():
+    _run("test_dataset", 120)
 
 
 @pytest.mark.gpu
