@@ -261,6 +261,33 @@ int gs_hip_workspace_overflows(gs_hip_workspace* ws, uint64_t* steps);
 int gs_hip_workspace_timings(gs_hip_workspace* ws, int enable, const char** names, double* ms, int capacity,
                              int* count);
 
+/* ---- 5. batches of scenes ---------------------------------------------------- */
+/* Several independent scenes (workspaces of any capacity, SH degree and image
+ * size) trained together, one launch per stage over every view of every
+ * scene: grids are sized for the largest scene, pair buffer and tile grid,
+ * and each block finds its scene and view in device tables.  No counterpart
+ * in the reference, which trains one scene at a time.  The batch borrows the
+ * scenes' models and views (it owns only its tables, stream and work lists),
+ * so a scene stays usable on its own (densify, render, store) between batch
+ * steps.  At most 32 scenes and 64 views per step; tiles per view below 2^16. */
+typedef struct gs_hip_batch gs_hip_batch;
+int gs_hip_batch_create(gs_hip_batch** out, gs_hip_workspace* const* scenes, uint32_t count);
+int gs_hip_batch_destroy(gs_hip_batch* batch);
+/* One optimizer step of every scene: views[s] frames of scene s (1 .. that
+ * workspace's views), all frames concatenated in scene order; targets[i] is
+ * the device RGBA8 of frame i, read in place (NULL, or a NULL entry: that
+ * view's TARGET buffer).  Each scene's Adam applies the mean gradient of its
+ * own views (its frames[first].rates and step); a step any view of which
+ * overflows its pair buffers leaves every model alone, grows the buffers
+ * and runs again (each overflowing scene's overflow count goes up).
+ * pairs[n] and losses[3 n] (n = sum of views[]) may be NULL. */
+int gs_hip_batch_train(gs_hip_batch* batch, const gs_hip_frame* frames, const uint32_t* views,
+                       const void* const* targets, uint64_t* pairs, double* losses);
+int gs_hip_batch_get_tuning(gs_hip_batch* batch, gs_hip_tuning* out);
+int gs_hip_batch_set_tuning(gs_hip_batch* batch, const gs_hip_tuning* tuning);
+/* Kernel timing of the last batch steps (as gs_hip_workspace_timings). */
+int gs_hip_batch_timings(gs_hip_batch* batch, int enable, const char** names, double* ms, int capacity, int* count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -58,8 +58,6 @@ void drop_graph(gs_hip_workspace* ws) {
   if (ws->graph != nullptr) hipGraphDestroy(ws->graph);
   ws->graph_exec = nullptr;
   ws->graph = nullptr;
-  ws->graph_views = 0;
-  ws->graph_pair_capacity = 0;
   ws->graph_signature = 0;
 }
 
@@ -137,6 +135,8 @@ int create_view(gs_hip_workspace* ws, uint32_t index, View** out) {
   A(alloc(&v->block_sums, v->block_sums_entries));
   v->totals = ws->totals_all + size_t(index) * 16;
   v->tile_loss = ws->tile_loss_all + size_t(index) * tiles * 4;
+  v->capacity = ws->capacity;
+  v->tile_grid = uint32_t(tiles);
   if (rc == 0) rc = reserve_sort(*v, std::max<uint64_t>(cap, ws->views.empty() ? 0 : view0(ws).pair_capacity));
   if (rc == 0 && !ws->views.empty() && view0(ws).pair_capacity > 0) {
     // Same pair capacity as the other views.
@@ -221,13 +221,13 @@ int reserve_pairs(gs_hip_workspace* ws, uint64_t pairs) {
     if (rc != 0) return rc;
     v->pair_capacity = n;
   }
-  ws->table_pair_capacity = 0;  // the segments point at the old buffers
+  ws->table_signature = 0;  // the segments point at the old buffers
   return 0;
 }
 
 namespace {
 
-ViewDev make_view_dev(gs_hip_workspace* ws, View& v, const gs_hip_frame& f) {
+ViewDev make_view_dev(const View& v, const gs_hip_frame& f, uint32_t* totals, float* tile_loss, const void* target) {
   ViewDev d;
   std::memset(&d, 0, sizeof(d));
   d.f = f;
@@ -245,59 +245,196 @@ ViewDev make_view_dev(gs_hip_workspace* ws, View& v, const gs_hip_frame& f) {
   d.ranges = reinterpret_cast<uint2*>(v.ranges);
   d.image = reinterpret_cast<float4*>(v.image);
   d.last = v.last;
-  d.target = v.target;
+  d.target = target != nullptr ? static_cast<const uint32_t*>(target) : v.target;
   d.dldc = reinterpret_cast<float4*>(v.dldc);
   d.maps = v.ssim_maps;
-  d.tile_loss = reinterpret_cast<float4*>(v.tile_loss);
-  d.totals = v.totals;
+  d.tile_loss = reinterpret_cast<float4*>(tile_loss);
+  d.totals = totals;
   d.pair_capacity = v.pair_capacity;
-  (void)ws;
+  d.capacity = v.capacity;
+  d.tile_count = v.tile_grid;
   return d;
 }
 
-// Sort and scan segments of every view (they change only when buffers move).
-int upload_segments(gs_hip_workspace* ws) {
-  SortSeg depth[GS_HIP_MAX_VIEWS], tile[GS_HIP_MAX_VIEWS];
-  ScanSeg scan[GS_HIP_MAX_VIEWS];
-  std::memset(depth, 0, sizeof(depth));
-  std::memset(tile, 0, sizeof(tile));
-  std::memset(scan, 0, sizeof(scan));
+uint64_t mix64(uint64_t h, uint64_t x) {
+  h ^= x + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
+  return h;
+}
+
+// Sort and scan segments of the entries (rebuilt only when their buffers move).
+int upload_segments(gs_hip_workspace* ws, const std::vector<TableEntry>& entries) {
+  const size_t n = entries.size();
+  std::vector<SortSeg> depth(n), tile(n);
+  std::vector<ScanSeg> scan(n);
   const size_t slots_offset = offsetof(ViewDev, f) + offsetof(gs_hip_frame, slots);
-  for (size_t i = 0; i < ws->views.size(); ++i) {
-    View& v = *ws->views[i];
+  for (size_t i = 0; i < n; ++i) {
+    View& v = *entries[i].view;
     const uint32_t* slots = reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(ws->dev_views + i) +
                                                               slots_offset);
+    uint32_t* totals = ws->totals_all + i * 16;
+    const uint32_t blocks = std::max<uint32_t>(div_up(v.sort.items, 4096), 1);
     depth[i] = SortSeg{v.depth_keys, v.depth_order, v.sort.keys_alt, v.sort.vals_alt, v.sort.hist,
-                       v.sort.digit_totals, slots, ws->capacity, 0u};
+                       v.sort.digit_totals, slots, v.capacity, blocks};
     tile[i] = SortSeg{v.pair_tiles, v.pair_gaussians, v.sort.keys_alt, v.sort.vals_alt, v.sort.hist,
-                      v.sort.digit_totals, v.totals, uint32_t(std::min<uint64_t>(v.pair_capacity, 0xffffffffu)), 0u};
-    scan[i] = ScanSeg{v.tiles, v.depth_order, v.offsets, v.block_sums, v.totals, slots, ws->capacity, 0u};
+                      v.sort.digit_totals, totals, uint32_t(std::min<uint64_t>(v.pair_capacity, 0xffffffffu)), blocks};
+    scan[i] = ScanSeg{v.tiles, v.depth_order, v.offsets, v.block_sums, totals, slots, v.capacity,
+                      v.block_sums_entries};
   }
   GS_HIP_TRY(hipStreamSynchronize(ws->stream));
-  GS_HIP_TRY(hipMemcpy(ws->dev_depth_sort, depth, sizeof(depth), hipMemcpyHostToDevice));
-  GS_HIP_TRY(hipMemcpy(ws->dev_tile_sort, tile, sizeof(tile), hipMemcpyHostToDevice));
-  GS_HIP_TRY(hipMemcpy(ws->dev_tile_scan, scan, sizeof(scan), hipMemcpyHostToDevice));
-  ws->table_pair_capacity = view0(ws).pair_capacity;
+  GS_HIP_TRY(hipMemcpy(ws->dev_depth_sort, depth.data(), sizeof(SortSeg) * n, hipMemcpyHostToDevice));
+  GS_HIP_TRY(hipMemcpy(ws->dev_tile_sort, tile.data(), sizeof(SortSeg) * n, hipMemcpyHostToDevice));
+  GS_HIP_TRY(hipMemcpy(ws->dev_tile_scan, scan.data(), sizeof(ScanSeg) * n, hipMemcpyHostToDevice));
   return 0;
+}
+
+int bits_for(uint32_t n) {
+  int bits = 0;
+  while (bits < 32 && (uint64_t(1) << bits) < n) ++bits;
+  return bits;
 }
 
 }  // namespace
 
-int upload_table(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t count) {
-  if (count == 0 || count > ws->views.size()) return set_error(GS_HIP_E_ARGUMENT, "view count out of range");
-  if (ws->table_pair_capacity != view0(ws).pair_capacity || ws->table_pair_capacity == 0) {
-    const int rc = upload_segments(ws);
-    if (rc != 0) return rc;
+int upload_entries(gs_hip_workspace* ws, const std::vector<TableEntry>& entries) {
+  const uint32_t count = uint32_t(entries.size());
+  if (count == 0 || count > ws->table_views) return set_error(GS_HIP_E_ARGUMENT, "view count out of range");
+  // Shape, scenes (runs of entries of one scene) and each view's first tile in the per-tile arrays.
+  StepShape shape;
+  shape.views = count;
+  std::vector<SceneDev> scenes;
+  ws->tile_base.assign(count, 0);
+  uint64_t sig = count;
+  for (uint32_t i = 0; i < count; ++i) {
+    const TableEntry& e = entries[i];
+    const View& v = *e.view;
+    if (i == 0 || entries[i - 1].scene != e.scene) {
+      gs_hip_workspace* m = e.scene;
+      SceneDev d;
+      d.params = reinterpret_cast<float4*>(m->params);
+      d.sh = m->sh;
+      d.m_params = reinterpret_cast<float4*>(m->m_params);
+      d.v_params = reinterpret_cast<float4*>(m->v_params);
+      d.m_sh = m->m_sh;
+      d.v_sh = m->v_sh;
+      d.stats = reinterpret_cast<float4*>(m->stats);
+      d.capacity = m->capacity;
+      d.sh_coeffs = m->sh_coeffs;
+      d.first_view = i;
+      d.views = 0;
+      scenes.push_back(d);
+      shape.max_capacity = std::max(shape.max_capacity, m->capacity);
+      shape.any_sh = shape.any_sh || m->sh_coeffs > 0;
+    }
+    ++scenes.back().views;
+    if (scenes.back().views > GS_HIP_MAX_VIEWS) return set_error(GS_HIP_E_ARGUMENT, "more views of one scene than a step holds");
+    ws->tile_base[i] = shape.total_tiles;
+    shape.total_tiles += v.tile_grid;
+    shape.max_tiles = std::max(shape.max_tiles, v.tile_grid);
+    shape.max_pair_capacity = std::max(shape.max_pair_capacity, v.pair_capacity);
+    sig = mix64(mix64(mix64(sig, reinterpret_cast<uintptr_t>(&v)), v.pair_capacity), v.sort.items);
   }
+  shape.scenes = uint32_t(scenes.size());
+  shape.tile_key_bits = uint32_t(bits_for(shape.max_tiles + 1));
+  if (shape.scenes > ws->table_scenes) return set_error(GS_HIP_E_ARGUMENT, "too many scenes in one step");
+  if (shape.total_tiles > ws->work_entries || shape.total_tiles > ws->tile_loss_entries) {
+    return set_error(GS_HIP_E_ARGUMENT, "the step's tiles exceed the launch context's tables");
+  }
+  if (sig != ws->table_signature || ws->table_signature == 0) {
+    const int rc = upload_segments(ws, entries);
+    if (rc != 0) return rc;
+    ws->table_signature = sig == 0 ? 1 : sig;
+    drop_graph(ws);
+  }
+  ws->shape = shape;
   // A staging slot is reused only after its previous copy has run.
   const uint32_t s = ws->staging_next;
   ws->staging_next = (s + 1) % kStagingSlots;
   GS_HIP_TRY(hipEventSynchronize(ws->staging_done[s]));
-  ViewDev* host = ws->host_views + size_t(s) * GS_HIP_MAX_VIEWS;
-  for (uint32_t v = 0; v < count; ++v) host[v] = make_view_dev(ws, *ws->views[v], frames[v]);
+  ViewDev* host = ws->host_views + size_t(s) * ws->table_views;
+  for (uint32_t i = 0; i < count; ++i) {
+    const TableEntry& e = entries[i];
+    host[i] = make_view_dev(*e.view, *e.frame, ws->totals_all + size_t(i) * 16,
+                            ws->tile_loss_all + size_t(ws->tile_base[i]) * 4, e.target);
+  }
+  SceneDev* hs = ws->host_scenes + size_t(s) * ws->table_scenes;
+  std::copy(scenes.begin(), scenes.end(), hs);
   GS_HIP_TRY(hipMemcpyAsync(ws->dev_views, host, sizeof(ViewDev) * count, hipMemcpyHostToDevice, ws->stream));
+  GS_HIP_TRY(hipMemcpyAsync(ws->dev_scenes, hs, sizeof(SceneDev) * scenes.size(), hipMemcpyHostToDevice, ws->stream));
   GS_HIP_TRY(hipEventRecord(ws->staging_done[s], ws->stream));
   return 0;
+}
+
+int upload_table(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t count) {
+  if (count == 0 || count > ws->views.size()) return set_error(GS_HIP_E_ARGUMENT, "view count out of range");
+  std::vector<TableEntry> entries(count);
+  for (uint32_t v = 0; v < count; ++v) entries[v] = TableEntry{ws->views[v], &frames[v], ws, nullptr};
+  return upload_entries(ws, entries);
+}
+
+int alloc_tables(gs_hip_workspace* ws, uint32_t views, uint32_t scenes, uint32_t tile_entries) {
+  ws->table_views = views;
+  ws->table_scenes = scenes;
+  ws->work_entries = tile_entries;
+  ws->tile_loss_entries = tile_entries;
+  int rc = 0;
+  auto A = [&](int r) {
+    if (rc == 0) rc = r;
+  };
+  A(alloc(&ws->dev_views, views));
+  A(alloc(&ws->dev_scenes, scenes));
+  A(alloc(&ws->dev_depth_sort, views));
+  A(alloc(&ws->dev_tile_sort, views));
+  A(alloc(&ws->dev_tile_scan, views));
+  A(alloc(&ws->dev_scan_tmp, 1));
+  A(alloc(&ws->work, std::max<uint32_t>(tile_entries, 1)));
+  A(alloc(&ws->work_counters, 16));
+  A(alloc(&ws->totals_all, size_t(16) * views));
+  A(alloc(&ws->tile_loss_all, size_t(std::max<uint32_t>(tile_entries, 1)) * 4));
+  auto pinned = [&](void** p, size_t bytes) {
+    if (rc == 0 && hipHostMalloc(p, bytes) != hipSuccess) rc = set_error(GS_HIP_E_HIP, "hipHostMalloc failed");
+  };
+  pinned(reinterpret_cast<void**>(&ws->host_views), sizeof(ViewDev) * views * kStagingSlots);
+  pinned(reinterpret_cast<void**>(&ws->host_scenes), sizeof(SceneDev) * scenes * kStagingSlots);
+  pinned(reinterpret_cast<void**>(&ws->host_totals), size_t(64) * views);
+  pinned(reinterpret_cast<void**>(&ws->host_tile_loss), size_t(std::max<uint32_t>(tile_entries, 1)) * 16);
+  for (uint32_t s = 0; s < kStagingSlots && rc == 0; ++s) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      rc = set_error(GS_HIP_E_HIP, "hipEventCreate failed");
+    } else {
+      ws->staging_done.push_back(e);
+    }
+  }
+  if (rc == 0) rc = fill_u32(ws, ws->totals_all, size_t(16) * views, 0);
+  if (rc == 0) rc = fill_u32(ws, ws->work_counters, 16, 0);
+  return rc;
+}
+
+void release_tables(gs_hip_workspace* ws) {
+  release(ws->dev_views);
+  release(ws->dev_scenes);
+  release(ws->dev_depth_sort);
+  release(ws->dev_tile_sort);
+  release(ws->dev_tile_scan);
+  release(ws->dev_scan_tmp);
+  release(ws->work);
+  release(ws->work_counters);
+  release(ws->totals_all);
+  release(ws->tile_loss_all);
+  if (ws->host_views != nullptr) hipHostFree(ws->host_views);
+  if (ws->host_scenes != nullptr) hipHostFree(ws->host_scenes);
+  if (ws->host_totals != nullptr) hipHostFree(ws->host_totals);
+  if (ws->host_tile_loss != nullptr) hipHostFree(ws->host_tile_loss);
+  ws->host_views = nullptr;
+  ws->host_scenes = nullptr;
+  ws->host_totals = nullptr;
+  ws->host_tile_loss = nullptr;
+  for (hipEvent_t e : ws->staging_done) hipEventDestroy(e);
+  ws->staging_done.clear();
+  for (hipEvent_t e : ws->timer.start) hipEventDestroy(e);
+  for (hipEvent_t e : ws->timer.stop) hipEventDestroy(e);
+  ws->timer.start.clear();
+  ws->timer.stop.clear();
 }
 
 }  // namespace gsh
@@ -433,40 +570,12 @@ int gs_hip_workspace_create(gs_hip_workspace** out, uint32_t capacity, int sh_de
   A(alloc(&ws->m_sh, shn));
   A(alloc(&ws->v_sh, shn));
   A(alloc(&ws->stats, cap * 4));
-  A(alloc(&ws->dev_views, GS_HIP_MAX_VIEWS));
-  A(alloc(&ws->dev_depth_sort, GS_HIP_MAX_VIEWS));
-  A(alloc(&ws->dev_tile_sort, GS_HIP_MAX_VIEWS));
-  A(alloc(&ws->dev_tile_scan, GS_HIP_MAX_VIEWS));
-  A(alloc(&ws->dev_scan_tmp, 1));
-  A(alloc(&ws->work, tiles * GS_HIP_MAX_VIEWS));
-  A(alloc(&ws->work_counters, 16));
-  A(alloc(&ws->totals_all, 16 * GS_HIP_MAX_VIEWS));
-  A(alloc(&ws->tile_loss_all, tiles * 4 * GS_HIP_MAX_VIEWS));
-  if (rc == 0 && hipHostMalloc(reinterpret_cast<void**>(&ws->host_views),
-                               sizeof(gsh::ViewDev) * GS_HIP_MAX_VIEWS * gsh::kStagingSlots) != hipSuccess) {
-    rc = gsh::set_error(GS_HIP_E_HIP, "hipHostMalloc failed");
-  }
-  if (rc == 0 && hipHostMalloc(reinterpret_cast<void**>(&ws->host_totals), 64 * GS_HIP_MAX_VIEWS) != hipSuccess) {
-    rc = gsh::set_error(GS_HIP_E_HIP, "hipHostMalloc failed");
-  }
-  if (rc == 0 && hipHostMalloc(reinterpret_cast<void**>(&ws->host_tile_loss), tiles * 16 * GS_HIP_MAX_VIEWS) !=
-                     hipSuccess) {
-    rc = gsh::set_error(GS_HIP_E_HIP, "hipHostMalloc failed");
-  }
-  for (uint32_t s = 0; s < gsh::kStagingSlots && rc == 0; ++s) {
-    hipEvent_t e = nullptr;
-    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
-      rc = gsh::set_error(GS_HIP_E_HIP, "hipEventCreate failed");
-    } else {
-      ws->staging_done.push_back(e);
-    }
-  }
+  if (tiles > gsh::kMaxTilesPerView) A(gsh::set_error(GS_HIP_E_ARGUMENT, "workspace_create: more than 65535 tiles"));
+  A(gsh::alloc_tables(ws, GS_HIP_MAX_VIEWS, 1, uint32_t(tiles) * GS_HIP_MAX_VIEWS));
   if (rc == 0) {
     for (void* p : {(void*)ws->params, (void*)ws->m_params, (void*)ws->v_params}) A(gsh::fill_u32(ws, p, cap * 16, 0));
     for (void* p : {(void*)ws->sh, (void*)ws->m_sh, (void*)ws->v_sh}) A(gsh::fill_u32(ws, p, shn, 0));
     A(gsh::fill_u32(ws, ws->stats, cap * 4, 0));
-    A(gsh::fill_u32(ws, ws->totals_all, 16 * GS_HIP_MAX_VIEWS, 0));
-    A(gsh::fill_u32(ws, ws->work_counters, 16, 0));
   }
   gsh::View* v = nullptr;
   if (rc == 0) rc = gsh::create_view(ws, 0, &v);
@@ -492,7 +601,7 @@ int gs_hip_workspace_views(gs_hip_workspace* ws, uint32_t count) {
     const int rc = gsh::create_view(ws, uint32_t(ws->views.size()), &v);
     if (rc != 0) return rc;
     ws->views.push_back(v);
-    ws->table_pair_capacity = 0;
+    ws->table_signature = 0;
     gsh::drop_graph(ws);
   }
   return 0;
@@ -502,7 +611,8 @@ int gs_hip_workspace_destroy(gs_hip_workspace* ws) {
   if (ws == nullptr) return 0;
   hipDeviceSynchronize();
   gsh::drop_graph(ws);
-  for (gsh::View* v : ws->views) gsh::destroy_view(v);
+  if (!ws->borrowed_views)
+    for (gsh::View* v : ws->views) gsh::destroy_view(v);
   ws->views.clear();
   release(ws->params);
   release(ws->sh);
@@ -511,21 +621,7 @@ int gs_hip_workspace_destroy(gs_hip_workspace* ws) {
   release(ws->m_sh);
   release(ws->v_sh);
   release(ws->stats);
-  release(ws->dev_views);
-  release(ws->dev_depth_sort);
-  release(ws->dev_tile_sort);
-  release(ws->dev_tile_scan);
-  release(ws->dev_scan_tmp);
-  release(ws->work);
-  release(ws->work_counters);
-  release(ws->totals_all);
-  release(ws->tile_loss_all);
-  if (ws->host_views != nullptr) hipHostFree(ws->host_views);
-  if (ws->host_totals != nullptr) hipHostFree(ws->host_totals);
-  if (ws->host_tile_loss != nullptr) hipHostFree(ws->host_tile_loss);
-  for (hipEvent_t e : ws->staging_done) hipEventDestroy(e);
-  for (hipEvent_t e : ws->timer.start) hipEventDestroy(e);
-  for (hipEvent_t e : ws->timer.stop) hipEventDestroy(e);
+  gsh::release_tables(ws);
   if (ws->stream != nullptr) hipStreamDestroy(ws->stream);
   delete ws;
   return 0;

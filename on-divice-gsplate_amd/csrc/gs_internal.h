@@ -1,13 +1,17 @@
 // Host-side internals shared by the C ABI translation units: the workspace
 // (the model's per-slot state plus up to GS_HIP_MAX_VIEWS views' transient
-// buffers), the device table the batched kernels read, error reporting and
+// buffers), the device tables the batched kernels read, error reporting and
 // launch helpers.
 //
 // Every stage is one launch over all views of a step (the view is blockIdx.y,
-// or part of the work item for the rasterizers), on the workspace's one
-// stream.  Kernels find their view's frame and buffers in a device-resident
-// table (ViewDev), so the step's launch sequence does not depend on the
-// frames and can be captured once in a HIP graph and replayed.
+// or part of the work item for the rasterizers), on one stream.  Kernels find
+// their view's frame and buffers in a device-resident table (ViewDev) and
+// their scene's model in another (SceneDev), so the step's launch sequence
+// does not depend on the frames and can be captured once in a HIP graph and
+// replayed.  A step's views may belong to one scene (a workspace) or to
+// several (a batch, whose launch context borrows the scenes' views): grids
+// are sized for the largest scene, pair buffer and tile grid of the step
+// (StepShape) and each block finds its own extent in the tables.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -52,7 +56,7 @@ struct SortSeg {
   uint32_t* digit_totals;  // [256]
   const uint32_t* count;
   uint32_t count_max;
-  uint32_t pad;
+  uint32_t blocks;  // 4096-item blocks the scratch holds (the hist row length); a launch's extra blocks idle
 };
 
 // One segment of a batched exclusive scan: out[i] = sum of values[gather[j]]
@@ -66,7 +70,23 @@ struct ScanSeg {
   uint32_t* total;
   const uint32_t* count;
   uint32_t count_max;
-  uint32_t pad;
+  uint32_t blocks;  // block_sums entries; a launch's extra blocks idle
+};
+
+// A scene's model as the per-slot kernels see it, and its views in the step's
+// view table: views [first_view, first_view + views).
+struct SceneDev {
+  float4* params;
+  float* sh;
+  float4* m_params;
+  float4* v_params;
+  float* m_sh;
+  float* v_sh;
+  float4* stats;
+  uint32_t capacity;
+  uint32_t sh_coeffs;
+  uint32_t first_view;
+  uint32_t views;
 };
 
 // A view's frame and buffers as the kernels see them (device memory).
@@ -93,8 +113,29 @@ struct ViewDev {
   float4* dldc;
   float* maps;  // [9][pixels]: SSIM gradient maps A, B, C per channel
   float4* tile_loss;
-  uint32_t* totals;  // [0] pairs, [1] emit mismatches, [5..7] densify counters
+  uint32_t* totals;  // [0] pairs, [1] emit mismatches, [5..7] densify counters, [8] visible splats (viewer)
   uint64_t pair_capacity;
+  uint32_t capacity;    // the scene's slot capacity: plane stride of grad2d and sh_input
+  uint32_t tile_count;  // tiles_x * tiles_y
+};
+
+// Rasterizer work items name (view, tile): tiles per view stay below 2^16.
+constexpr uint32_t kTileBits = 16;
+constexpr uint32_t kMaxTilesPerView = (1u << kTileBits) - 1u;
+// A launch context's table sizes: a workspace's own views, or a batch's.
+constexpr uint32_t kMaxBatchViews = 64;
+constexpr uint32_t kMaxBatchScenes = 32;
+
+// The extent of the current step's table, set when it is uploaded.
+struct StepShape {
+  uint32_t views = 0;
+  uint32_t scenes = 0;
+  uint32_t max_capacity = 0;       // the largest scene's slot capacity (per-slot grids)
+  uint64_t max_pair_capacity = 0;  // the largest pair buffer (per-pair grids)
+  uint32_t max_tiles = 0;          // the largest tile grid (per-tile grids)
+  uint32_t total_tiles = 0;        // every view's tiles (rasterizer work items)
+  uint32_t tile_key_bits = 0;      // tile sort key width
+  bool any_sh = false;             // some scene stores SH rest coefficients
 };
 
 // Radix sort scratch of one view, for up to `items` (key, value) pairs.
@@ -134,6 +175,8 @@ struct View {
   uint32_t* block_sums = nullptr;
   uint32_t block_sums_entries = 0;
   SortScratch sort;
+  uint32_t capacity = 0;  // of the scene the view belongs to
+  uint32_t tile_grid = 0;  // tiles_x * tiles_y of the view's image
 };
 
 struct Timer {
@@ -161,33 +204,39 @@ struct gs_hip_workspace {
   float* m_sh = nullptr;
   float* v_sh = nullptr;
   float* stats = nullptr;
-  std::vector<gsh::View*> views;  // views[0] always exists
-  // device table: ViewDev[MAX_VIEWS] and the batched sort / scan segments
+  std::vector<gsh::View*> views;  // views[0] always exists (a batch's launch context borrows its scenes' views)
+  bool borrowed_views = false;
+  // device tables: ViewDev[table_views], SceneDev[table_scenes] and the batched sort / scan segments
+  uint32_t table_views = GS_HIP_MAX_VIEWS, table_scenes = 1;
   gsh::ViewDev* dev_views = nullptr;
-  gsh::SortSeg* dev_depth_sort = nullptr;  // [MAX_VIEWS]
-  gsh::SortSeg* dev_tile_sort = nullptr;   // [MAX_VIEWS]
-  gsh::ScanSeg* dev_tile_scan = nullptr;   // [MAX_VIEWS]
+  gsh::SceneDev* dev_scenes = nullptr;
+  gsh::SortSeg* dev_depth_sort = nullptr;  // [table_views]
+  gsh::SortSeg* dev_tile_sort = nullptr;   // [table_views]
+  gsh::ScanSeg* dev_tile_scan = nullptr;   // [table_views]
   gsh::ScanSeg* dev_scan_tmp = nullptr;    // [1]: host-count scans (densify)
-  uint32_t* work = nullptr;           // rasterizer work items (view * tiles + tile), longest lists first
+  uint32_t* work = nullptr;           // rasterizer work items (view << 16 | tile), longest lists first
+  uint32_t work_entries = 0;
   uint32_t* work_counters = nullptr;  // [0] forward, [1] backward (persistent rasterizers)
-  uint32_t* totals_all = nullptr;     // [MAX_VIEWS][16]
-  float* tile_loss_all = nullptr;     // [MAX_VIEWS][tiles][4]
-  // host side of the table and readbacks (pinned)
-  gsh::ViewDev* host_views = nullptr;  // staging ring: kStagingSlots copies of ViewDev[MAX_VIEWS]
+  uint32_t* totals_all = nullptr;     // [table_views][16]
+  float* tile_loss_all = nullptr;     // [tile_loss_entries][4], view v's tiles from tile_base[v]
+  uint32_t tile_loss_entries = 0;
+  std::vector<uint32_t> tile_base;    // per view of the current table
+  gsh::StepShape shape;               // of the current table
+  // host side of the tables and readbacks (pinned)
+  gsh::ViewDev* host_views = nullptr;    // staging ring: kStagingSlots copies of ViewDev[table_views]
+  gsh::SceneDev* host_scenes = nullptr;  // and of SceneDev[table_scenes]
   std::vector<hipEvent_t> staging_done;
   uint32_t staging_next = 0;
-  uint32_t* host_totals = nullptr;  // [MAX_VIEWS][16]
-  float* host_tile_loss = nullptr;  // [MAX_VIEWS][tiles][4]
-  uint64_t table_pair_capacity = 0;  // pair capacity the segments were built for
+  uint32_t* host_totals = nullptr;  // [table_views][16]
+  float* host_tile_loss = nullptr;  // [tile_loss_entries][4]
+  uint64_t table_signature = 0;     // buffers the segments were built for (0: rebuild)
   uint64_t max_pairs_seen = 0;
   uint64_t overflow_steps = 0;
   gs_hip_tuning tuning;
   // captured batched step
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
-  uint32_t graph_views = 0;  // view count the graph was captured for
-  uint64_t graph_pair_capacity = 0;
-  uint64_t graph_signature = 0;  // view count and step kind it was captured for
+  uint64_t graph_signature = 0;  // table, view count and step kind it was captured for
   gsh::Timer timer;
 };
 
@@ -196,6 +245,14 @@ constexpr uint32_t kStagingSlots = 64;
 
 inline View& view0(gs_hip_workspace* ws) { return *ws->views[0]; }
 inline uint32_t tile_count(const gs_hip_workspace* ws) { return ws->tiles_x * ws->tiles_y; }
+
+// One view of a step's table: whose buffers, which frame, which scene's model.
+struct TableEntry {
+  View* view;
+  const gs_hip_frame* frame;
+  gs_hip_workspace* scene;
+  const void* target;  // device RGBA8 read in place, or null for the view's TARGET buffer
+};
 // Kernel timing on the workspace stream (no-ops unless enabled).
 void timer_begin(gs_hip_workspace* ws, const char* name);
 void timer_end(gs_hip_workspace* ws);
@@ -210,6 +267,9 @@ int reserve_sort(View& v, uint64_t items);
 // (stream-ordered copy from a pinned staging slot) with every view's buffer
 // pointers, and the sort / scan segments when the buffers moved.
 int upload_table(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t count);
+// The same for any entries (a batch's views of several scenes, grouped by
+// scene): sets ws->shape and ws->tile_base.
+int upload_entries(gs_hip_workspace* ws, const std::vector<TableEntry>& entries);
 // Batched stages over views [0, count) of the uploaded table.
 int launch_project(gs_hip_workspace* ws, uint32_t count);
 int launch_depth_sort_and_scan(gs_hip_workspace* ws, uint32_t count, bool depth16);
@@ -224,6 +284,13 @@ int launch_backward(gs_hip_workspace* ws, uint32_t count);
 // Adjoint + Adam over the mean gradient of views [0, count); skipped on the
 // device when any view's pairs overflowed its buffers.
 int launch_adam(gs_hip_workspace* ws, uint32_t count);
+// Every stage of one step over the uploaded table, eagerly or by replaying
+// the captured graph (tuning.graph), ending with the totals' readback.
+int launch_step(gs_hip_workspace* ws, uint32_t count, bool depth16, bool train, bool losses);
+// Device tables of a launch context for `views` views of up to `scenes`
+// scenes and `tile_entries` tiles in all; and their release.
+int alloc_tables(gs_hip_workspace* ws, uint32_t views, uint32_t scenes, uint32_t tile_entries);
+void release_tables(gs_hip_workspace* ws);
 // One batched step over views [0, count) (train: with backward and Adam;
 // losses: tile loss sums read back), rerun with larger pair buffers when a
 // view overflowed; pairs[count] and loss_sums[3 * count] may be null.

@@ -73,6 +73,7 @@ __device__ __forceinline__ LossArgs loss_args(const gsh::ViewDev& v) {
 // One tile (blockIdx.x) of one view (blockIdx.y).
 __global__ void __launch_bounds__(256) k_ssim(const gsh::ViewDev* __restrict__ views) {
   const gsh::ViewDev& view = views[blockIdx.y];
+  if (blockIdx.x >= view.tile_count) return;  // a smaller view of the launch: the whole block leaves
   const gs_hip_frame& f = view.f;
   const LossArgs a = loss_args(view);
   __shared__ float s_x[3][kHalo * kStride];
@@ -168,6 +169,7 @@ __global__ void __launch_bounds__(256) k_ssim(const gsh::ViewDev* __restrict__ v
 
 __global__ void __launch_bounds__(256) k_loss_grad(const gsh::ViewDev* __restrict__ views) {
   const gsh::ViewDev& view = views[blockIdx.y];
+  if (blockIdx.x >= view.tile_count) return;  // a smaller view of the launch: the whole block leaves
   const gs_hip_frame& f = view.f;
   const LossArgs a = loss_args(view);
   __shared__ float s_m[3][kHalo * kStride];
@@ -225,7 +227,7 @@ __global__ void __launch_bounds__(256) k_loss_grad(const gsh::ViewDev* __restric
 namespace gsh {
 
 int launch_loss(gs_hip_workspace* ws, uint32_t count) {
-  const dim3 grid(tile_count(ws), count);
+  const dim3 grid(ws->shape.max_tiles, count);
   timer_begin(ws, "ssim");
   k_ssim<<<grid, 256, 0, ws->stream>>>(ws->dev_views);
   GS_CHECK_LAUNCH("k_ssim");
@@ -241,15 +243,16 @@ int launch_readback(gs_hip_workspace* ws, uint32_t count, bool losses) {
   GS_HIP_TRY(hipMemcpyAsync(ws->host_totals, ws->totals_all, size_t(count) * 64, hipMemcpyDeviceToHost,
                             ws->stream));
   if (losses) {
-    GS_HIP_TRY(hipMemcpyAsync(ws->host_tile_loss, ws->tile_loss_all, size_t(count) * tile_count(ws) * 16,
+    GS_HIP_TRY(hipMemcpyAsync(ws->host_tile_loss, ws->tile_loss_all, size_t(ws->shape.total_tiles) * 16,
                               hipMemcpyDeviceToHost, ws->stream));
   }
   return 0;
 }
 
 void sum_losses(gs_hip_workspace* ws, uint32_t v, double losses[3]) {
-  const uint32_t tiles = tile_count(ws);
-  const float* t = ws->host_tile_loss + size_t(v) * tiles * 4;
+  const uint32_t tiles = v + 1 < ws->tile_base.size() ? ws->tile_base[v + 1] - ws->tile_base[v]
+                                                       : ws->shape.total_tiles - ws->tile_base[v];
+  const float* t = ws->host_tile_loss + size_t(ws->tile_base[v]) * 4;
   double s[3] = {0.0, 0.0, 0.0};
   for (uint32_t i = 0; i < tiles; ++i) {
     for (int q = 0; q < 3; ++q) s[q] += t[i * 4 + q];

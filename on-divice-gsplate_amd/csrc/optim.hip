@@ -30,19 +30,39 @@ struct AdamArgs {
   float* __restrict__ m_sh;
   float* __restrict__ v_sh;
   float4* __restrict__ stats;
-  const gsh::ViewDev* __restrict__ views;  // the step's views; views[0].f holds the optimizer's rates
+  const gsh::ViewDev* __restrict__ views;  // the scene's views; views[0].f holds the optimizer's rates
   uint32_t capacity;
   uint32_t sh_coeffs;
-  uint32_t count;  // views
+  uint32_t count;                               // the scene's views
+  const gsh::ViewDev* __restrict__ step_views;  // every view of the step (all scenes)
+  uint32_t step_count;
 };
 
-// The step leaves the model alone when asked to, or when any view found more
-// pairs than its buffers hold (its render was cut short; the host runs the
-// step again with larger buffers, trainer.cpp:819-836).
+__device__ __forceinline__ AdamArgs scene_args(const gsh::SceneDev& sc, const gsh::ViewDev* views, uint32_t count) {
+  AdamArgs a;
+  a.params = sc.params;
+  a.m_params = sc.m_params;
+  a.v_params = sc.v_params;
+  a.sh = sc.sh;
+  a.m_sh = sc.m_sh;
+  a.v_sh = sc.v_sh;
+  a.stats = sc.stats;
+  a.views = views + sc.first_view;
+  a.capacity = sc.capacity;
+  a.sh_coeffs = sc.sh_coeffs;
+  a.count = sc.views;
+  a.step_views = views;
+  a.step_count = count;
+  return a;
+}
+
+// The step leaves the model alone when asked to, or when any view of the step
+// found more pairs than its buffers hold (its render was cut short; the host
+// runs the step again with larger buffers, trainer.cpp:819-836).
 __device__ __forceinline__ bool skip_update(const AdamArgs& a) {
   if ((a.views[0].f.flags & GS_HIP_FLAG_NO_UPDATE) != 0u) return true;
-  for (uint32_t v = 0; v < a.count; ++v) {
-    if (uint64_t(a.views[v].totals[0]) > a.views[v].pair_capacity) return true;
+  for (uint32_t v = 0; v < a.step_count; ++v) {
+    if (uint64_t(a.step_views[v].totals[0]) > a.step_views[v].pair_capacity) return true;
   }
   return false;
 }
@@ -235,7 +255,9 @@ __device__ __forceinline__ void adjoint_view(const gs_hip_camera& cam, const flo
 // gaussian was visible.  Each view's colour gradient and direction go to its
 // sh_input for k_adam_sh (one thread per SH coefficient, fully coalesced on
 // the coefficient planes, and few registers here).
-__global__ void __launch_bounds__(256) k_adam(AdamArgs a) {
+__global__ void __launch_bounds__(256) k_adam(const gsh::SceneDev* __restrict__ scenes,
+                                              const gsh::ViewDev* __restrict__ views, uint32_t count) {
+  const AdamArgs a = scene_args(scenes[blockIdx.y], views, count);
   const gs_hip_frame& f = a.views[0].f;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= f.slots || skip_update(a)) return;
@@ -336,10 +358,12 @@ __device__ __forceinline__ void adam_sh_group(float* __restrict__ sh, float* __r
 // thread per slot: g_k = mean over the views of basis_k(dir) dL/dcolour,
 // with each view's (dL/dcolour, dir) read once and the basis evaluated once
 // per view; then Adam on each coefficient plane (coalesced across the wave).
-__global__ void __launch_bounds__(256) k_adam_sh(AdamArgs a) {
+__global__ void __launch_bounds__(256) k_adam_sh(const gsh::SceneDev* __restrict__ scenes,
+                                                 const gsh::ViewDev* __restrict__ views, uint32_t count) {
+  const AdamArgs a = scene_args(scenes[blockIdx.y], views, count);
   const gs_hip_frame& f = a.views[0].f;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= f.slots || skip_update(a)) return;
+  if (a.sh_coeffs == 0u || i >= f.slots || skip_update(a)) return;
   if (a.views[0].sh_input[size_t(i) * 2u].w == 0.0f) return;  // free slot, or kept by sparse Adam
   const int coeffs = min(sh_rest_coeffs(f.sh_degree), int(a.sh_coeffs));
   float g[GS_HIP_MAX_SH_REST][3];
@@ -536,28 +560,14 @@ __global__ void k_zero_counters(uint32_t* __restrict__ totals) {
 namespace gsh {
 
 int launch_adam(gs_hip_workspace* ws, uint32_t count) {
-  if (count == 0 || count > ws->views.size() || count > GS_HIP_MAX_VIEWS) {
-    return set_error(GS_HIP_E_ARGUMENT, "adam: view count out of range");
-  }
-  if (ws->capacity == 0) return 0;
-  AdamArgs a;
-  a.params = reinterpret_cast<float4*>(ws->params);
-  a.m_params = reinterpret_cast<float4*>(ws->m_params);
-  a.v_params = reinterpret_cast<float4*>(ws->v_params);
-  a.sh = ws->sh;
-  a.m_sh = ws->m_sh;
-  a.v_sh = ws->v_sh;
-  a.stats = reinterpret_cast<float4*>(ws->stats);
-  a.views = ws->dev_views;
-  a.capacity = ws->capacity;
-  a.sh_coeffs = ws->sh_coeffs;
-  a.count = count;
-  const uint32_t blocks = div_up(ws->capacity, 256);
+  if (count == 0 || count != ws->shape.views) return set_error(GS_HIP_E_ARGUMENT, "adam: view count out of range");
+  if (ws->shape.max_capacity == 0) return 0;
+  const dim3 grid(div_up(ws->shape.max_capacity, 256), ws->shape.scenes);
   timer_begin(ws, "adam");
-  k_adam<<<blocks, 256, 0, ws->stream>>>(a);
+  k_adam<<<grid, 256, 0, ws->stream>>>(ws->dev_scenes, ws->dev_views, count);
   GS_CHECK_LAUNCH("k_adam");
-  if (ws->sh_coeffs > 0) {
-    k_adam_sh<<<blocks, 256, 0, ws->stream>>>(a);
+  if (ws->shape.any_sh) {
+    k_adam_sh<<<grid, 256, 0, ws->stream>>>(ws->dev_scenes, ws->dev_views, count);
     GS_CHECK_LAUNCH("k_adam_sh");
   }
   timer_end(ws);
@@ -585,11 +595,10 @@ int launch_stages(gs_hip_workspace* ws, uint32_t count, bool depth16, bool train
 }
 
 // Replays the captured step (capturing it first when the views, buffers,
-// launch shapes or step kind changed).
+// launch shapes or step kind changed: the table signature covers the buffers).
 int launch_graph(gs_hip_workspace* ws, uint32_t count, bool depth16, bool train, bool losses) {
-  const uint64_t sig = step_signature(count, depth16, train, losses);
-  if (ws->graph_exec == nullptr || ws->graph_views != count || ws->graph_pair_capacity != view0(ws).pair_capacity ||
-      ws->graph_signature != sig) {
+  const uint64_t sig = step_signature(count, depth16, train, losses) ^ (ws->table_signature * 0x100000001b3ull);
+  if (ws->graph_exec == nullptr || ws->graph_signature != sig) {
     drop_graph(ws);
     GS_HIP_TRY(hipStreamBeginCapture(ws->stream, hipStreamCaptureModeThreadLocal));
     const int rc = launch_stages(ws, count, depth16, train, losses);
@@ -602,13 +611,20 @@ int launch_graph(gs_hip_workspace* ws, uint32_t count, bool depth16, bool train,
     if (end != hipSuccess) return hip_error(end, "hipStreamEndCapture");
     ws->graph = graph;
     GS_HIP_TRY(hipGraphInstantiate(&ws->graph_exec, ws->graph, nullptr, nullptr, 0));
-    ws->graph_views = count;
-    ws->graph_pair_capacity = view0(ws).pair_capacity;
     ws->graph_signature = sig;
   }
   GS_HIP_TRY(hipGraphLaunch(ws->graph_exec, ws->stream));
   return 0;
 }
+
+}  // namespace
+
+int launch_step(gs_hip_workspace* ws, uint32_t count, bool depth16, bool train, bool losses) {
+  const bool graph = ws->tuning.graph != 0u && !ws->timer.enabled;
+  return graph ? launch_graph(ws, count, depth16, train, losses) : launch_stages(ws, count, depth16, train, losses);
+}
+
+namespace {
 
 int check_frames(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t count) {
   if (count == 0 || count > ws->views.size()) {
@@ -655,8 +671,7 @@ int run_step(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t count, c
   for (int attempt = 0;; ++attempt) {
     rc = upload_table(ws, frames, count);
     if (rc != 0) return rc;
-    const bool graph = ws->tuning.graph != 0u && !ws->timer.enabled;
-    rc = graph ? launch_graph(ws, count, depth16, train, losses) : launch_stages(ws, count, depth16, train, losses);
+    rc = launch_step(ws, count, depth16, train, losses);
     if (rc != 0) return rc;
     GS_HIP_TRY(hipStreamSynchronize(ws->stream));
     uint64_t most = 0;

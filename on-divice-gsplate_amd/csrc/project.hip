@@ -153,7 +153,17 @@ __device__ __forceinline__ void project_view(const gsh::ViewDev& vd, const Proje
   vd.depth_order[i] = i;
 }
 
-__global__ void __launch_bounds__(256) k_project(ProjectArgs a) {
+// One thread per slot of scene blockIdx.y (its model and views from the tables).
+__global__ void __launch_bounds__(256) k_project(const gsh::SceneDev* __restrict__ scenes,
+                                                 const gsh::ViewDev* __restrict__ views) {
+  const gsh::SceneDev& sc = scenes[blockIdx.y];
+  ProjectArgs a;
+  a.params = sc.params;
+  a.sh = sc.sh;
+  a.views = views + sc.first_view;
+  a.capacity = sc.capacity;
+  a.sh_coeffs = sc.sh_coeffs;
+  a.count = sc.views;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.views[0].f.slots) return;
   const float4 p0 = a.params[i * 4u], p1 = a.params[i * 4u + 1u];
@@ -180,16 +190,11 @@ __global__ void __launch_bounds__(256) k_project(ProjectArgs a) {
 
 namespace gsh {
 int launch_project(gs_hip_workspace* ws, uint32_t count) {
-  if (ws->capacity == 0) return 0;
-  ProjectArgs a;
-  a.params = reinterpret_cast<const float4*>(ws->params);
-  a.sh = ws->sh;
-  a.views = ws->dev_views;
-  a.capacity = ws->capacity;
-  a.sh_coeffs = ws->sh_coeffs;
-  a.count = count;
+  (void)count;
+  if (ws->shape.max_capacity == 0) return 0;
   timer_begin(ws, "project");
-  k_project<<<div_up(ws->capacity, 256), 256, 0, ws->stream>>>(a);
+  k_project<<<dim3(div_up(ws->shape.max_capacity, 256), ws->shape.scenes), 256, 0, ws->stream>>>(ws->dev_scenes,
+                                                                                                  ws->dev_views);
   GS_CHECK_LAUNCH("k_project");
   timer_end(ws);
   return 0;

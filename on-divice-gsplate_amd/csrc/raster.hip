@@ -46,11 +46,9 @@ constexpr int kGrad = GS_HIP_GRAD2D_FLOATS;
 
 struct RasterArgs {
   const gsh::ViewDev* __restrict__ views;
-  const uint32_t* __restrict__ work;  // view * tiles + tile, longest lists first
+  const uint32_t* __restrict__ work;  // view << 16 | tile, longest lists first
   uint32_t* __restrict__ counter;     // persistent mode: next work item
-  uint32_t items;                     // views * tiles
-  uint32_t tiles;
-  uint32_t capacity;
+  uint32_t items;                     // every view's tiles
   uint32_t persistent;
 };
 
@@ -319,7 +317,7 @@ __global__ void __launch_bounds__(kThreads) k_forward(RasterArgs a) {
   uint32_t item = a.persistent != 0u ? next_item(a, &s_item) : blockIdx.x;
   while (item < a.items) {
     const uint32_t w = a.work[item];
-    forward_tile(a.views[w / a.tiles], w % a.tiles, s);
+    forward_tile(a.views[w >> gsh::kTileBits], w & gsh::kMaxTilesPerView, s);
     if (a.persistent == 0u) break;
     __syncthreads();
     item = next_item(a, &s_item);
@@ -334,7 +332,8 @@ __global__ void __launch_bounds__(kThreads) k_backward(RasterArgs a) {
   uint32_t item = a.persistent != 0u ? next_item(a, &s_item) : blockIdx.x;
   while (item < a.items) {
     const uint32_t w = a.work[item];
-    backward_tile(a.views[w / a.tiles], w % a.tiles, sh, a.capacity, red, slot);
+    const gsh::ViewDev& view = a.views[w >> gsh::kTileBits];
+    backward_tile(view, w & gsh::kMaxTilesPerView, sh, view.capacity, red, slot);
     if (a.persistent == 0u) break;
     __syncthreads();
     item = next_item(a, &s_item);
@@ -346,9 +345,8 @@ RasterArgs make_args(gs_hip_workspace* ws, uint32_t count, int which, uint32_t& 
   a.views = ws->dev_views;
   a.work = ws->work;
   a.counter = ws->work_counters + which;
-  a.tiles = gsh::tile_count(ws);
-  a.items = count * a.tiles;
-  a.capacity = ws->capacity;
+  a.items = ws->shape.total_tiles;
+  (void)count;
   a.persistent = ws->tuning.raster_groups_per_cu > 0 ? 1u : 0u;
   grid = a.persistent ? std::min<uint32_t>(a.items, ws->cus * ws->tuning.raster_groups_per_cu) : a.items;
   return a;

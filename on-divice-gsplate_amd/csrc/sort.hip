@@ -65,9 +65,10 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t digit, int nbits, uint6
 // the lanes holding the same digit add their count once (no LDS atomics
 // piling on one counter when neighbouring keys share a digit).
 __global__ void __launch_bounds__(kSortThreads) k_radix_upsweep(const SortSeg* __restrict__ segs, int shift,
-                                                                int nbits, int from_alt, uint32_t blocks) {
+                                                                int nbits, int from_alt) {
   __shared__ uint32_t h[kWaves][256];
   const SortSeg& g = segs[blockIdx.y];
+  if (blockIdx.x >= g.blocks) return;  // a smaller segment of the launch
   const uint32_t n = seg_count(g.count, g.count_max);
   const uint32_t* __restrict__ keys = from_alt ? g.keys_alt : g.keys;
   const uint32_t radix = 1u << nbits, mask = radix - 1u;
@@ -87,17 +88,19 @@ __global__ void __launch_bounds__(kSortThreads) k_radix_upsweep(const SortSeg* _
   for (uint32_t d = threadIdx.x; d < radix; d += kSortThreads) {
     uint32_t s = 0;
     for (int w = 0; w < kWaves; ++w) s += h[w][d];
-    g.hist[size_t(d) * blocks + blockIdx.x] = s;
+    g.hist[size_t(d) * g.blocks + blockIdx.x] = s;
   }
 }
 
 // One block per (digit, segment): exclusive scan of that digit's per-block
-// counts, and the digit's total.
-__global__ void __launch_bounds__(256) k_radix_scan(const SortSeg* __restrict__ segs, uint32_t blocks) {
+// counts (the blocks the pass launched, up to the segment's scratch), and the
+// digit's total.
+__global__ void __launch_bounds__(256) k_radix_scan(const SortSeg* __restrict__ segs, uint32_t launched) {
   __shared__ uint32_t s_wave[4];
   __shared__ uint32_t s_carry;
   const SortSeg& g = segs[blockIdx.y];
-  uint32_t* row = g.hist + size_t(blockIdx.x) * blocks;
+  const uint32_t blocks = min(launched, g.blocks);
+  uint32_t* row = g.hist + size_t(blockIdx.x) * g.blocks;
   if (threadIdx.x == 0) s_carry = 0u;
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -143,7 +146,7 @@ __device__ __forceinline__ void block_scan_256(uint32_t v, uint32_t* s_tmp, uint
 // turns (wave, digit) counts into offsets; items go to LDS in digit order and
 // each digit's run leaves contiguously.
 __global__ void __launch_bounds__(kSortThreads) k_radix_scatter(const SortSeg* __restrict__ segs, int shift,
-                                                                int nbits, int from_alt, uint32_t blocks) {
+                                                                int nbits, int from_alt) {
   __shared__ uint32_t s_keys[kSortItems];
   __shared__ uint32_t s_vals[kSortItems];
   __shared__ uint32_t s_wave[kWaves][256];  // per-wave digit counts, then each wave's offset inside the digit
@@ -153,7 +156,7 @@ __global__ void __launch_bounds__(kSortThreads) k_radix_scatter(const SortSeg* _
   const SortSeg& g = segs[blockIdx.y];
   const uint32_t n = seg_count(g.count, g.count_max);
   const uint32_t base = blockIdx.x * kSortItems;
-  if (base >= n) return;
+  if (base >= n || blockIdx.x >= g.blocks) return;
   const uint32_t* __restrict__ keys_in = from_alt ? g.keys_alt : g.keys;
   const uint32_t* __restrict__ vals_in = from_alt ? g.vals_alt : g.vals;
   uint32_t* __restrict__ keys_out = from_alt ? g.keys : g.keys_alt;
@@ -163,7 +166,7 @@ __global__ void __launch_bounds__(kSortThreads) k_radix_scatter(const SortSeg* _
   // Output start of each digit: the digits below it over the whole segment
   // (a block scan of the totals) plus this digit's items in earlier blocks.
   block_scan_256(threadIdx.x < radix ? g.digit_totals[threadIdx.x] : 0u, s_tmp, s_global);
-  if (threadIdx.x < radix) s_global[threadIdx.x] += g.hist[size_t(threadIdx.x) * blocks + blockIdx.x];
+  if (threadIdx.x < radix) s_global[threadIdx.x] += g.hist[size_t(threadIdx.x) * g.blocks + blockIdx.x];
   for (int d = threadIdx.x; d < kWaves * 256; d += kSortThreads) (&s_wave[0][0])[d] = 0u;
   __syncthreads();
   const uint32_t wbase = base + wave * (kSortItems / kWaves);
@@ -236,6 +239,7 @@ __device__ __forceinline__ uint32_t load_value(const ScanSeg& s, uint32_t idx) {
 __global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const ScanSeg* __restrict__ segs) {
   __shared__ uint32_t s[kScanThreads / 64];
   const ScanSeg& g = segs[blockIdx.y];
+  if (blockIdx.x >= g.blocks) return;  // a smaller segment of the launch
   const uint32_t count = seg_count(g.count, g.count_max);
   uint32_t sum = 0;
   const uint32_t base = blockIdx.x * kScanItems;
@@ -254,6 +258,7 @@ __global__ void __launch_bounds__(1024) k_scan_blocks(const ScanSeg* __restrict_
   __shared__ uint32_t s_carry;
   const ScanSeg& g = segs[blockIdx.y];
   uint32_t* block_sums = g.block_sums;
+  blocks = min(blocks, g.blocks);
   if (threadIdx.x == 0) s_carry = 0u;
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -287,7 +292,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_down(const ScanSeg* __res
   const ScanSeg& g = segs[blockIdx.y];
   const uint32_t count = seg_count(g.count, g.count_max);
   const uint32_t base = blockIdx.x * kScanItems;
-  if (base >= count) return;
+  if (base >= count || blockIdx.x >= g.blocks) return;
   if (threadIdx.x == 0) s_carry = g.block_sums[blockIdx.x];
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -351,10 +356,10 @@ __global__ void __launch_bounds__(256) k_emit(const ViewDev* __restrict__ views)
   }
 }
 
-__global__ void __launch_bounds__(256) k_clear_ranges(const ViewDev* __restrict__ views, uint32_t entries) {
+__global__ void __launch_bounds__(256) k_clear_ranges(const ViewDev* __restrict__ views) {
   const ViewDev& v = views[blockIdx.y];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < entries) v.ranges[i] = make_uint2(0u, 0u);
+  if (i <= v.tile_count) v.ranges[i] = make_uint2(0u, 0u);
   if (i == 0u) v.totals[1] = 0u;
 }
 
@@ -376,22 +381,23 @@ __global__ void __launch_bounds__(256) k_ranges(const ViewDev* __restrict__ view
   if (s == count - 1u) v.ranges[tile].y = count;
 }
 
-// The rasterizers' work: every (view, tile) of the step, bucketed by
-// floor(log2(list length)), longest first, empty tiles last
+// The rasterizers' work: every (view, tile) of the step, as view << 16 | tile,
+// bucketed by floor(log2(list length)), longest first, empty tiles last
 // (longest-processing-time-first scheduling within a factor of two, across
 // all views).  One workgroup; also resets the persistent rasterizers' work
 // counters.
 __global__ void __launch_bounds__(1024) k_work_order(const ViewDev* __restrict__ views, uint32_t count,
-                                                     uint32_t tiles, uint32_t* __restrict__ work,
-                                                     uint32_t* __restrict__ counters) {
+                                                     uint32_t* __restrict__ work, uint32_t* __restrict__ counters) {
   __shared__ uint32_t s_count[33];
   if (threadIdx.x < 33) s_count[threadIdx.x] = 0u;
   if (threadIdx.x < 2) counters[threadIdx.x] = 0u;
   __syncthreads();
-  const uint32_t items = count * tiles;
-  for (uint32_t t = threadIdx.x; t < items; t += blockDim.x) {
-    const uint2 r = views[t / tiles].ranges[t % tiles];
-    atomicAdd(&s_count[__clz(r.y - r.x)], 1u);
+  for (uint32_t v = 0; v < count; ++v) {
+    const ViewDev& view = views[v];
+    for (uint32_t t = threadIdx.x; t < view.tile_count; t += blockDim.x) {
+      const uint2 r = view.ranges[t];
+      atomicAdd(&s_count[__clz(r.y - r.x)], 1u);
+    }
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -403,16 +409,13 @@ __global__ void __launch_bounds__(1024) k_work_order(const ViewDev* __restrict__
     }
   }
   __syncthreads();
-  for (uint32_t t = threadIdx.x; t < items; t += blockDim.x) {
-    const uint2 r = views[t / tiles].ranges[t % tiles];
-    work[atomicAdd(&s_count[__clz(r.y - r.x)], 1u)] = t;
+  for (uint32_t v = 0; v < count; ++v) {
+    const ViewDev& view = views[v];
+    for (uint32_t t = threadIdx.x; t < view.tile_count; t += blockDim.x) {
+      const uint2 r = view.ranges[t];
+      work[atomicAdd(&s_count[__clz(r.y - r.x)], 1u)] = (v << gsh::kTileBits) | t;
+    }
   }
-}
-
-int bits_for(uint32_t n) {
-  int bits = 0;
-  while (bits < 32 && (uint64_t(1) << bits) < n) ++bits;
-  return bits;
 }
 
 // Batched LSD sort of `count` segments by key bits [0, bits), `digit` bits per pass.
@@ -425,11 +428,11 @@ int radix_sort(gs_hip_workspace* ws, const SortSeg* segs, uint32_t count, uint64
     const int shift = pass * digit;
     const int nbits = std::min(digit, bits - shift);
     const int from_alt = pass & 1;
-    k_radix_upsweep<<<grid, kSortThreads, 0, ws->stream>>>(segs, shift, nbits, from_alt, blocks);
+    k_radix_upsweep<<<grid, kSortThreads, 0, ws->stream>>>(segs, shift, nbits, from_alt);
     GS_CHECK_LAUNCH("k_radix_upsweep");
     k_radix_scan<<<dim3(1u << nbits, count), 256, 0, ws->stream>>>(segs, blocks);
     GS_CHECK_LAUNCH("k_radix_scan");
-    k_radix_scatter<<<grid, kSortThreads, 0, ws->stream>>>(segs, shift, nbits, from_alt, blocks);
+    k_radix_scatter<<<grid, kSortThreads, 0, ws->stream>>>(segs, shift, nbits, from_alt);
     GS_CHECK_LAUNCH("k_radix_scatter");
   }
   if (passes & 1) {
@@ -458,43 +461,41 @@ int exclusive_scan(gs_hip_workspace* ws, View& v, const uint32_t* values, const 
                    uint32_t count, uint32_t* total_dev) {
   const uint32_t blocks = std::max<uint32_t>(div_up(count, kScanItems), 1);
   if (blocks > v.block_sums_entries) return set_error(GS_HIP_E_CAPACITY, "exclusive_scan: too many blocks");
-  const ScanSeg seg{values, gather, out, v.block_sums, total_dev, nullptr, count, 0u};
+  const ScanSeg seg{values, gather, out, v.block_sums, total_dev, nullptr, count, v.block_sums_entries};
   GS_HIP_TRY(hipStreamSynchronize(ws->stream));
   GS_HIP_TRY(hipMemcpy(ws->dev_scan_tmp, &seg, sizeof(seg), hipMemcpyHostToDevice));
   return scan_segments(ws, ws->dev_scan_tmp, 1, count);
 }
 
 int launch_depth_sort_and_scan(gs_hip_workspace* ws, uint32_t count, bool depth16) {
-  if (ws->capacity == 0) return 0;
+  const uint32_t items = ws->shape.max_capacity;
+  if (items == 0) return 0;
   timer_begin(ws, "depth_sort");
-  int rc = radix_sort(ws, ws->dev_depth_sort, count, ws->capacity, depth16 ? 16 : 32,
-                      int(ws->tuning.depth_digit_bits));
+  int rc = radix_sort(ws, ws->dev_depth_sort, count, items, depth16 ? 16 : 32, int(ws->tuning.depth_digit_bits));
   timer_end(ws);
   if (rc != 0) return rc;
   timer_begin(ws, "scan");
-  rc = scan_segments(ws, ws->dev_tile_scan, count, ws->capacity);
+  rc = scan_segments(ws, ws->dev_tile_scan, count, items);
   timer_end(ws);
   return rc;
 }
 
 int launch_clear_ranges(gs_hip_workspace* ws, uint32_t count) {
-  const uint32_t tiles = tile_count(ws);
-  k_clear_ranges<<<dim3(div_up(tiles + 1, 256), count), 256, 0, ws->stream>>>(ws->dev_views, tiles + 1);
+  k_clear_ranges<<<dim3(div_up(ws->shape.max_tiles + 1, 256), count), 256, 0, ws->stream>>>(ws->dev_views);
   GS_CHECK_LAUNCH("k_clear_ranges");
   return 0;
 }
 
 int launch_tile_sort_and_ranges(gs_hip_workspace* ws, uint32_t count) {
-  const uint32_t tiles = tile_count(ws);
-  const uint64_t cap = view0(ws).pair_capacity;
+  const uint64_t cap = ws->shape.max_pair_capacity;
   timer_begin(ws, "tile_sort");
-  int rc = radix_sort(ws, ws->dev_tile_sort, count, cap, bits_for(tiles + 1), int(ws->tuning.tile_digit_bits));
+  int rc = radix_sort(ws, ws->dev_tile_sort, count, cap, int(ws->shape.tile_key_bits), int(ws->tuning.tile_digit_bits));
   timer_end(ws);
   if (rc != 0) return rc;
   timer_begin(ws, "ranges");
   k_ranges<<<dim3(div_up(cap, 256), count), 256, 0, ws->stream>>>(ws->dev_views);
   GS_CHECK_LAUNCH("k_ranges");
-  k_work_order<<<1, 1024, 0, ws->stream>>>(ws->dev_views, count, tiles, ws->work, ws->work_counters);
+  k_work_order<<<1, 1024, 0, ws->stream>>>(ws->dev_views, count, ws->work, ws->work_counters);
   GS_CHECK_LAUNCH("k_work_order");
   timer_end(ws);
   return 0;
@@ -503,8 +504,8 @@ int launch_tile_sort_and_ranges(gs_hip_workspace* ws, uint32_t count) {
 int launch_emit_and_tile_sort(gs_hip_workspace* ws, uint32_t count) {
   timer_begin(ws, "emit");
   int rc = launch_clear_ranges(ws, count);
-  if (rc == 0 && ws->capacity > 0) {
-    k_emit<<<dim3(div_up(ws->capacity, 256), count), 256, 0, ws->stream>>>(ws->dev_views);
+  if (rc == 0 && ws->shape.max_capacity > 0) {
+    k_emit<<<dim3(div_up(ws->shape.max_capacity, 256), count), 256, 0, ws->stream>>>(ws->dev_views);
     GS_CHECK_LAUNCH("k_emit");
   }
   timer_end(ws);

@@ -39,6 +39,8 @@ EXPORTED = [
     "gs_hip_set_target", "gs_hip_set_target_device", "gs_hip_loss", "gs_hip_backward", "gs_hip_adam", "gs_hip_densify", "gs_hip_reset_opacity", "gs_hip_train_step", "gs_hip_train_views",
     "gs_hip_render", "gs_hip_workspace_get_tuning", "gs_hip_workspace_set_tuning", "gs_hip_autotune",
     "gs_hip_workspace_overflows", "gs_hip_workspace_timings", "gs_hip_view_render",
+    "gs_hip_batch_create", "gs_hip_batch_destroy", "gs_hip_batch_train", "gs_hip_batch_get_tuning",
+    "gs_hip_batch_set_tuning", "gs_hip_batch_timings",
 ]
 
 
@@ -112,6 +114,12 @@ def library() -> ctypes.CDLL:
             "gs_hip_workspace_overflows": [vp, u64p],
             "gs_hip_workspace_timings": [vp, i32, ctypes.POINTER(ctypes.c_char_p), dp, i32,
                                          ctypes.POINTER(ctypes.c_int)],
+            "gs_hip_batch_create": [ctypes.POINTER(vp), ctypes.POINTER(vp), u32],
+            "gs_hip_batch_destroy": [vp],
+            "gs_hip_batch_train": [vp, frame_p, u32p, ctypes.POINTER(vp), u64p, dp],
+            "gs_hip_batch_get_tuning": [vp, ctypes.POINTER(Tuning)],
+            "gs_hip_batch_set_tuning": [vp, ctypes.POINTER(Tuning)],
+            "gs_hip_batch_timings": [vp, i32, ctypes.POINTER(ctypes.c_char_p), dp, i32, ctypes.POINTER(ctypes.c_int)],
         }
         for name, args in sig.items():
             fn = getattr(lib, name)
@@ -401,6 +409,76 @@ class Workspace:
         ms = (ctypes.c_double * cap)()
         count = ctypes.c_int()
         check(self.lib.gs_hip_workspace_timings(self.handle, int(enable), names, ms, cap, ctypes.byref(count)))
+        return [(names[i].decode(), ms[i]) for i in range(count.value)]
+
+
+class Batch:
+    """Several workspaces trained by one launch per stage (gs_hip_batch); the
+    workspaces must outlive the batch."""
+
+    def __init__(self, workspaces):
+        self.lib = library()
+        self.scenes = list(workspaces)
+        self.handle = ctypes.c_void_p()
+        arr = (ctypes.c_void_p * len(self.scenes))(*[w.handle.value for w in self.scenes])
+        check(self.lib.gs_hip_batch_create(ctypes.byref(self.handle), arr, len(self.scenes)))
+
+    def close(self) -> None:
+        if self.handle:
+            check(self.lib.gs_hip_batch_destroy(self.handle))
+            self.handle = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def train(self, frames_per_scene, targets_per_scene=None, want_losses: bool = True):
+        """One optimizer step of every scene: frames_per_scene[s] is the list of
+        scene s's frames, targets_per_scene[s] their device RGBA8 pointers (or
+        None).  Returns (pairs per view, [3 losses per view]), views in order."""
+        frames = [f for fs in frames_per_scene for f in fs]
+        n = len(frames)
+        counts = (ctypes.c_uint32 * len(frames_per_scene))(*[len(fs) for fs in frames_per_scene])
+        arr = (Frame * n)(*frames)
+        tarr = None
+        if targets_per_scene is not None:
+            flat = []
+            for fs, ts in zip(frames_per_scene, targets_per_scene):
+                flat += list(ts) if ts is not None else [None] * len(fs)
+            tarr = (ctypes.c_void_p * n)(*[t if isinstance(t, int) else (t.value if t else None) for t in flat])
+        pairs = (ctypes.c_uint64 * n)()
+        losses = (ctypes.c_double * (3 * n))()
+        check(self.lib.gs_hip_batch_train(self.handle, arr, counts, tarr, pairs, losses if want_losses else None))
+        ls = [tuple(losses[3 * v:3 * v + 3]) for v in range(n)] if want_losses else None
+        return list(pairs), ls
+
+    def tuning(self) -> Tuning:
+        t = Tuning()
+        check(self.lib.gs_hip_batch_get_tuning(self.handle, ctypes.byref(t)))
+        return t
+
+    def set_tuning(self, **kw) -> Tuning:
+        t = self.tuning()
+        for name, value in kw.items():
+            if value is not None:
+                setattr(t, name, int(value))
+        check(self.lib.gs_hip_batch_set_tuning(self.handle, ctypes.byref(t)))
+        return t
+
+    def timings(self, enable: bool = True):
+        cap = 1 << 16
+        names = (ctypes.c_char_p * cap)()
+        ms = (ctypes.c_double * cap)()
+        count = ctypes.c_int()
+        check(self.lib.gs_hip_batch_timings(self.handle, int(enable), names, ms, cap, ctypes.byref(count)))
         return [(names[i].decode(), ms[i]) for i in range(count.value)]
 
 

@@ -1,0 +1,183 @@
+"""Batches of scenes (gs_hip_batch_train) against each scene's own batched
+step (gs_hip_train_views) on the same frames and targets.
+
+Scenes differ in capacity, SH degree, image size and views per step, so
+every per-slot, per-pair and per-tile launch runs blocks past a smaller
+scene's extent.  After the first step, per view: records, pairs, tile
+ranges, LAST and the image are bit-exact (the same arithmetic per view),
+GRAD2D within 1e-5 of each plane's largest value and the parameters within
+1e-5 (float-atomic order), loss sums within 1e-6; later steps within 1e-4.
+Also: launch shapes (persistent rasterizers, digit widths, graph replay) and
+the pair-overflow retry inside a batch, and a scene used on its own between
+batch steps.
+"""
+import numpy as np
+import pytest
+
+from helpers import gs, make_camera, make_scene, smooth_target
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    name = gs.init(0)
+    print(f"\n[gpu] {name}", flush=True)
+    yield name
+    gs.synchronize()
+
+
+SCENES = [
+    # n, degree, width, height, views, seed, log-scale offset
+    (1500, 3, 160, 120, 2, 81, 0.0),
+    (700, 0, 96, 72, 1, 82, 0.0),
+    (2500, 1, 200, 150, 3, 83, 0.3),
+]
+
+
+def scene_inputs(spec):
+    n, deg, w, h, views, seed, grow = spec
+    params, sh = make_scene(n, deg, seed)
+    params[:, 4:7] += grow
+    cams = [make_camera(w, h, 0.3 + 0.9 * v + 0.1 * seed) for v in range(views)]
+    targets = [smooth_target(w, h, seed + 10 * v) for v in range(views)]
+    return params, sh, cams, targets
+
+
+def frames_for(spec, cams, step):
+    n, deg = spec[0], spec[1]
+    return [gs.make_frame(c, n, deg, tp=gs.TrainParams(lambda_dssim=0.2), step=step) for c in cams]
+
+
+def view_state(ws, n, views):
+    out = []
+    for v in range(views):
+        pairs = int(ws.download(gs.BUF_TOTALS, np.uint32, 1, view=v)[0])
+        T = ws.tiles_x * ws.tiles_y
+        out.append({
+            "records": ws.download(gs.BUF_RECORDS, np.float32, n * 12, view=v).view(np.uint32),
+            "pairs": ws.download(gs.BUF_PAIR_GAUSSIANS, np.uint32, pairs, view=v),
+            "ranges": ws.download(gs.BUF_RANGES, np.uint32, (T + 1) * 2, view=v),
+            "last": ws.download(gs.BUF_LAST, np.uint32, view=v),
+            "image": ws.download(gs.BUF_IMAGE, np.float32, view=v).view(np.uint32),
+            "grad2d": ws.download(gs.BUF_GRAD2D, np.float32, 9 * n, view=v).reshape(9, n),
+        })
+    return out
+
+
+def run(batched, steps, tuning=None):
+    """steps optimizer steps of every scene; returns per scene (state after
+    step 1, params after the last step, losses per step)."""
+    inputs = [scene_inputs(s) for s in SCENES]
+    spaces, bufs = [], []
+    try:
+        for spec, (params, sh, cams, targets) in zip(SCENES, inputs):
+            ws = gs.Workspace(spec[0], spec[1], spec[2], spec[3])
+            ws.set_views(spec[4])
+            ws.load(params, sh)
+            spaces.append(ws)
+            bufs.append([gs.DeviceBuffer(t) for t in targets])
+        batch = gs.Batch(spaces) if batched else None
+        if batch is not None and tuning:
+            batch.set_tuning(**tuning)
+        firsts, losses = [None] * len(SCENES), [[] for _ in SCENES]
+        for step in range(1, steps + 1):
+            fps = [frames_for(spec, inp[2], step) for spec, inp in zip(SCENES, inputs)]
+            if batched:
+                _, ls = batch.train(fps, [[b.ptr for b in bs] for bs in bufs])
+                k = 0
+                for s, spec in enumerate(SCENES):
+                    losses[s].append(ls[k:k + spec[4]])
+                    k += spec[4]
+            else:
+                for s, (ws, fs, bs) in enumerate(zip(spaces, fps, bufs)):
+                    _, ls = ws.train_views(fs, [b.ptr for b in bs])
+                    losses[s].append(ls)
+            if step == 1:
+                firsts = [view_state(ws, spec[0], spec[4]) for ws, spec in zip(spaces, SCENES)]
+        params = [ws.store(spec[0])[0] for ws, spec in zip(spaces, SCENES)]
+        overflows = [ws.overflows() for ws in spaces]
+        if batch is not None:
+            batch.close()
+        return firsts, params, losses, overflows
+    finally:
+        for ws in spaces:
+            ws.close()
+        for bs in bufs:
+            for b in bs:
+                b.free()
+
+
+def compare(ref, got):
+    (fa, pa, la, _), (fb, pb, lb, _) = ref, got
+    for s in range(len(SCENES)):
+        for va, vb in zip(fa[s], fb[s]):
+            for key in ("records", "pairs", "ranges", "last", "image"):
+                np.testing.assert_array_equal(va[key], vb[key], err_msg=f"scene {s} {key}")
+            scale = np.abs(va["grad2d"]).max(axis=1, keepdims=True) + 1e-30
+            assert (np.abs(va["grad2d"] - vb["grad2d"]) / scale).max() <= 1e-5, f"scene {s} grad2d"
+        np.testing.assert_allclose(np.asarray(la[s][0]), np.asarray(lb[s][0]), rtol=1e-6)
+        np.testing.assert_allclose(np.asarray(la[s]), np.asarray(lb[s]), rtol=1e-4)
+        assert np.abs(pa[s] - pb[s]).max() <= 1e-4 * max(1.0, np.abs(pa[s]).max()), f"scene {s} params"
+
+
+def test_batch_step_equals_each_scenes_own_step():
+    print("[case] 3 scenes (SH3 160x120 x2, SH0 96x72 x1, SH1 200x150 x3): batch vs own steps, 3 steps", flush=True)
+    ref = run(False, 3)
+    got = run(True, 3)
+    compare(ref, got)
+
+
+@pytest.mark.parametrize("name,tuning", [
+    ("persistent-digits", dict(raster_groups_per_cu=2, depth_digit_bits=6, tile_digit_bits=4)),
+    ("graph", dict(graph=1)),
+], ids=["persistent-digits", "graph"])
+def test_batch_launch_shapes(name, tuning):
+    print(f"[case] batch launch shape {name}", flush=True)
+    ref = run(True, 3)
+    got = run(True, 3, tuning=tuning)
+    compare(ref, got)
+
+
+def test_batch_overflow_retry_and_scene_on_its_own():
+    """Large splats in one scene overflow its first pair buffers inside the
+    batch: the step reruns and equals a batch whose buffers had room; the
+    scenes then step on their own and render."""
+    print("[case] batch pair overflow", flush=True)
+    specs = [(3000, 0, 256, 192, 1, 91, 1.6), (800, 2, 96, 72, 2, 92, 0.0)]
+    inputs = [scene_inputs(s) for s in specs]
+    results = []
+    for reserve in (False, True):
+        spaces, bufs = [], []
+        try:
+            for spec, (params, sh, cams, targets) in zip(specs, inputs):
+                ws = gs.Workspace(spec[0], spec[1], spec[2], spec[3])
+                ws.set_views(spec[4])
+                if reserve:
+                    ws.reserve_pairs(3_000_000)
+                ws.load(params, sh)
+                spaces.append(ws)
+                bufs.append([gs.DeviceBuffer(t) for t in targets])
+            with gs.Batch(spaces) as batch:
+                fps = [frames_for(spec, inp[2], 1) for spec, inp in zip(specs, inputs)]
+                pairs, losses = batch.train(fps, [[b.ptr for b in bs] for bs in bufs])
+            overflows = [ws.overflows() for ws in spaces]
+            params = [ws.store(spec[0])[0] for ws, spec in zip(spaces, specs)]
+            # each scene on its own afterwards
+            own = spaces[1].train_views(frames_for(specs[1], inputs[1][2], 2), [b.ptr for b in bufs[1]])
+            img = spaces[0].render(frames_for(specs[0], inputs[0][2], 2)[0])
+            results.append((pairs, losses, overflows, params, own, img))
+        finally:
+            for ws in spaces:
+                ws.close()
+            for bs in bufs:
+                for b in bs:
+                    b.free()
+    (p0, l0, o0, a0, w0, i0), (p1, l1, o1, a1, w1, i1) = results
+    print(f"[info] pairs {p0}, overflow steps {o0} / {o1}", flush=True)
+    assert p0 == p1 and o0[0] >= 1 and o1 == [0, 0]
+    np.testing.assert_allclose(np.asarray(l0), np.asarray(l1), rtol=1e-6)
+    for x, y in zip(a0, a1):
+        assert np.abs(x - y).max() <= 1e-5 * max(1.0, np.abs(y).max())
+    assert w0[0] == w1[0]
+    np.testing.assert_array_equal(i0, i1)
