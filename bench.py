@@ -147,6 +147,162 @@ def run(gs, ws, cams, target_bufs, order, per_step, views_total, warmup_views, d
     return time.perf_counter() - t0, steps
 
 
+RESOLUTIONS = ((640, 360), (1280, 720), (1920, 1080))
+
+
+def multi_scene_workload(seed_base=5000, total=6_000_000, tol=0.10):
+    """16 scenes, gaussian counts floor(exp(U[ln 5e4, ln 3e6])) and a
+    resolution from 640x360 / 1280x720 / 1920x1080 each, drawn from the first
+    seed >= seed_base whose counts total `total` within `tol`."""
+    for seed in range(seed_base, seed_base + 10000):
+        rng = np.random.default_rng(seed)
+        counts = np.floor(np.exp(rng.uniform(math.log(5e4), math.log(3e6), 16))).astype(int)
+        res = [RESOLUTIONS[int(i)] for i in rng.integers(0, 3, 16)]
+        if abs(counts.sum() - total) <= tol * total:
+            return seed, [int(c) for c in counts], res
+    raise RuntimeError("no seed gives the multi-scene total")
+
+
+def device_used_bytes():
+    """Bytes of device memory in use (hipMemGetInfo: total - free)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    free, tot = ctypes.c_size_t(), ctypes.c_size_t()
+    if hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(tot)) != 0:
+        return None
+    return tot.value - free.value
+
+
+def multi_scene(gs, args, device):
+    """16 independent scenes (about 6M gaussians, SH3 fp32, mixed resolutions),
+    2 views per scene per optimizer step: every stage of all 32 views as one
+    launch through a gs_hip_batch, against the same scenes stepped one after
+    another (each scene's own 2-view step).  Prints one JSON line."""
+    deg = 3
+    seed, counts, res = multi_scene_workload()
+    log(f"multi-scene: seed {seed}, {sum(counts)} gaussians in 16 scenes, counts {counts}")
+    used0 = device_used_bytes()
+    tp = gs.TrainParams()
+    scenes, targets, cams = [], [], []
+    for s, (n, (w, h)) in enumerate(zip(counts, res)):
+        cs = gs.jitter_cameras(8, w, h, 3003 + s)
+        gt_p, gt_sh = gs.synthetic_scene(n, deg, 1001 + 100 * s)
+        with gs.Workspace(n, deg, w, h) as gt:
+            gt.load(gt_p, gt_sh)
+            imgs = []
+            for cam in cs:
+                img = gt.render(gs.make_frame(cam, n, deg))
+                img[..., 3] = 255
+                imgs.append(gs.DeviceBuffer(img))
+        del gt_p, gt_sh
+        p, sh = gs.synthetic_scene(n, deg, 2002 + 100 * s)
+        ws = gs.Workspace(n, deg, w, h)
+        ws.set_views(2)
+        ws.load(p, sh)
+        scenes.append(ws)
+        targets.append(imgs)
+        cams.append(cs)
+    gs.synchronize()
+
+    def frames_of(step, flags=0):
+        fps, tps = [], []
+        for s, (n, cs, ts) in enumerate(zip(counts, cams, targets)):
+            vs = [(2 * step) % 8, (2 * step + 1) % 8]
+            fps.append([gs.make_frame(cs[v], n, deg, step=step, tp=tp, flags=flags) for v in vs])
+            tps.append([ts[v].ptr for v in vs])
+        return fps, tps
+
+    batch = gs.Batch(scenes)
+    # Launch shape: the candidates timed on two untimed (NO_UPDATE) batch steps each.
+    best, best_ms = None, None
+    for shape in ((0, 8, 8, 0), (0, 8, 6, 1), (0, 6, 6, 1), (2, 8, 6, 1)):
+        batch.set_tuning(raster_groups_per_cu=shape[0], depth_digit_bits=shape[1], tile_digit_bits=shape[2],
+                         graph=shape[3])
+        fps, tps = frames_of(0, flags=gs.FLAG_NO_UPDATE)
+        batch.train(fps, tps, want_losses=False)
+        gs.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(2):
+            batch.train(fps, tps, want_losses=False)
+        gs.synchronize()
+        ms = 1000 * (time.perf_counter() - t0) / 2
+        if best_ms is None or ms < best_ms:
+            best, best_ms = shape, ms
+    batch.set_tuning(raster_groups_per_cu=best[0], depth_digit_bits=best[1], tile_digit_bits=best[2], graph=best[3])
+    log(f"multi-scene: launch shape {best} ({best_ms:.2f} ms per step while choosing)")
+    steps, warmup = args.steps, args.warmup
+    pixels = sum(2 * w * h for (w, h) in res)
+    pairs_total = 0
+    for s in range(-warmup, steps):
+        if s == 0:
+            gs.synchronize()
+            batch.timings(enable=True)
+            t0 = time.perf_counter()
+        fps, tps = frames_of(s + warmup)
+        pairs, _ = batch.train(fps, tps, want_losses=False)
+        if s >= 0:
+            pairs_total += sum(pairs)
+        if s >= 0 and (s + 1) % max(1, steps // 4) == 0:
+            log(f"  batch steps: {s + 1}/{steps}")
+    gs.synchronize()
+    batch_s = (time.perf_counter() - t0) / steps
+    events = batch.timings(enable=False)
+    peak = device_used_bytes()
+    # The stored-list alternative keeps every pixel's contributors (gaussian index + alpha, 8 B each) from
+    # the forward for the backward; the recompute backward keeps LAST (4 B) and T (in the image) per pixel.
+    stored_list, recompute_state = 0, 0
+    for ws, n in zip(scenes, counts):
+        for v in range(2):
+            last = ws.download(gs.BUF_LAST, np.uint32, view=v)
+            stored_list += 8 * int(last.astype(np.int64).sum())
+            recompute_state += 8 * last.size
+    batch.close()
+    # The same scenes, one scene's 2-view step after another (each scene's own launches).
+    for ws in scenes:
+        ws.set_tuning(raster_groups_per_cu=best[0], depth_digit_bits=best[1], tile_digit_bits=best[2], graph=best[3])
+    for s in range(-warmup, steps):
+        if s == 0:
+            gs.synchronize()
+            t0 = time.perf_counter()
+        fps, tps = frames_of(steps + warmup + s + warmup)
+        for ws, f, t in zip(scenes, fps, tps):
+            ws.train_views(f, t, want_losses=False)
+    gs.synchronize()
+    seq_s = (time.perf_counter() - t0) / steps
+    kern = {}
+    for name, ms in events:
+        kern[name] = kern.get(name, 0.0) + ms / steps
+    views = 32
+    out = {
+        "metric": "multi-scene train iter/s (fwd+bwd+SSIM+Adam): 16 scenes, ~6M gaussians, 2 views per scene per step",
+        "value": round(views / batch_s, 3),
+        "unit": "iter/s (view-iterations)",
+        "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": round(1000 * batch_s, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": f"synthetic (16 seeded scenes, workload seed {seed}; 8 jittered views each rendered by this trainer)",
+        "config": {"workload": "16 independent scenes trained concurrently: one launch per stage over all 32 views "
+                               "(gs_hip_batch), each scene's Adam on the mean of its 2 views, recompute backward",
+                   "scenes": 16, "gaussians": counts, "gaussians_total": sum(counts),
+                   "resolutions": [f"{w}x{h}" for (w, h) in res], "views_per_scene": 2, "sh_degree": deg,
+                   "tuning": {"raster_groups_per_cu": best[0], "depth_digit_bits": best[1],
+                              "tile_digit_bits": best[2], "graph": best[3]}, "device": device},
+        "sequential_scenes": {"iters_per_s": round(views / seq_s, 3), "ms_per_step": round(1000 * seq_s, 4)},
+        "speedup_batch_vs_sequential": round(seq_s / batch_s, 3),
+        "mpix_per_s": round(pixels / batch_s / 1e6, 2),
+        "mean_pairs_per_view": int(pairs_total / (steps * views)),
+        "memory": {"device_bytes_in_use": peak, "device_bytes_before": used0,
+                   "recompute_per_pixel_state_bytes": recompute_state,
+                   "stored_list_per_pixel_bytes": stored_list},
+        "kernels_ms_per_step": {k: round(v, 4) for k, v in sorted(kern.items(), key=lambda kv: -kv[1])},
+    }
+    print(json.dumps(out), flush=True)
+    for ws in scenes:
+        ws.close()
+    for ts in targets:
+        for t in ts:
+            t.free()
+
+
 def kernel_table(events, steps, slots_mean, n_visible, mean_pairs_per_view, per_step, pixels, coeffs):
     per_kernel = {}
     for name, ms in events:
@@ -180,6 +336,9 @@ def main():
     ap.add_argument("--shape", type=str, default=None, metavar="G,D,T,GRAPH",
                     help="fixed launch shape instead of autotuning: raster groups per CU (0 = one per tile), depth "
                          "and tile radix digit bits, graph replay 0/1")
+    ap.add_argument("--multi-scene", action="store_true",
+                    help="instead: 16 scenes (~6M gaussians, mixed resolutions), 2 views each per step, as one "
+                         "batch against scene after scene")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="after the bench, copy MIB MiB device to device twice (a known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counts to bytes)")
@@ -192,6 +351,9 @@ def main():
     gs = _load("gs_hip", "on-divice-gsplate_amd/gs_hip.py")
     device = gs.init(0)
     log(f"device {device}")
+    if args.multi_scene:
+        multi_scene(gs, args, device)
+        return
     N, W, H, deg = args.gaussians, args.width, args.height, args.sh_degree
     coeffs = SH_COEFFS[deg]
     cams = gs.jitter_cameras(args.views, W, H, 3003)
