@@ -127,7 +127,8 @@ def run(gs, ws, cams, target_bufs, order, per_step, views_total, warmup_views, d
             t0 = time.perf_counter()
         base = state["views"]
         vs = [int(order[(base + k) % len(order)]) for k in range(per_step)]
-        frames = [gs.make_frame(cams[v], state["slots"], deg, step=state["step"], tp=tp) for v in vs]
+        frames = [gs.make_frame(cams[v], state["slots"], deg, step=state["step"], tp=tp, flags=state["flags"])
+                  for v in vs]
         if per_step == 1:
             ws.set_target_device(target_bufs[vs[0]].ptr)
             pairs, _ = ws.train_step(frames[0], want_losses=False)
@@ -208,7 +209,7 @@ def multi_scene(gs, args, device):
         fps, tps = [], []
         for s, (n, cs, ts) in enumerate(zip(counts, cams, targets)):
             vs = [(2 * step) % 8, (2 * step + 1) % 8]
-            fps.append([gs.make_frame(cs[v], n, deg, step=step, tp=tp, flags=flags) for v in vs])
+            fps.append([gs.make_frame(cs[v], n, deg, step=step, tp=tp, flags=flags | gs.FLAG_DEPTH16) for v in vs])
             tps.append([ts[v].ptr for v in vs])
         return fps, tps
 
@@ -336,6 +337,9 @@ def main():
     ap.add_argument("--shape", type=str, default=None, metavar="G,D,T,GRAPH",
                     help="fixed launch shape instead of autotuning: raster groups per CU (0 = one per tile), depth "
                          "and tile radix digit bits, graph replay 0/1")
+    ap.add_argument("--depth-key", type=int, default=16, choices=(16, 32),
+                    help="pair order within a tile: the reference's 16-bit log-depth key (train_project.comp:110) "
+                         "or the float bits of the depth")
     ap.add_argument("--multi-scene", action="store_true",
                     help="instead: 16 scenes (~6M gaussians, mixed resolutions), 2 views each per step, as one "
                          "batch against scene after scene")
@@ -372,6 +376,7 @@ def main():
 
     params, sh = gs.synthetic_scene(N, deg, 2002)
     order = np.random.default_rng(3003).permutation(args.views)
+    key_flags = gs.FLAG_DEPTH16 if args.depth_key == 16 else 0
     tp = gs.TrainParams()
     views_total = args.steps * args.batch
     warmup_views = args.warmup * args.batch
@@ -390,12 +395,12 @@ def main():
             log(f"{mode}: launch shape {tuning} (given)")
         elif not args.no_autotune:
             vs = [int(order[k % len(order)]) for k in range(per_step)]
-            frames = [gs.make_frame(cams[v], N, deg, step=0, tp=tp) for v in vs]
+            frames = [gs.make_frame(cams[v], N, deg, step=0, tp=tp, flags=key_flags) for v in vs]
             best, tune_ms = ws.autotune(frames, [target_bufs[v].ptr for v in vs], repeats=3)
             tuning = best.as_dict()
             log(f"{mode}: autotuned launch shape {tuning} ({tune_ms:.3f} ms per step while tuning)")
         graph = bool(tuning and tuning["graph"])
-        state = {"slots": N, "densified": 0, "pairs": 0, "views": 0, "step": 0}
+        state = {"slots": N, "densified": 0, "pairs": 0, "views": 0, "step": 0, "flags": key_flags}
         # Per-kernel HIP events need eager launches: with graph replay chosen,
         # the timed run replays the graph and a short eager pass after it
         # times the kernels.
@@ -471,9 +476,11 @@ def main():
         "config": {"workload": f"batched training: {args.batch} views per optimizer step (project, bin, forward, "
                                "SSIM loss and recompute backward of all views, one launch per stage; one adjoint + "
                                "Adam on the mean gradient), 1M gaussians SH3 fp32, 1280x720, 16x16 tiles, "
-                               "0.8 L1 + 0.2 D-SSIM, densify every 100 view-iterations",
+                               f"{args.depth_key}-bit depth key, 0.8 L1 + 0.2 D-SSIM, densify every 100 "
+                               "view-iterations",
                    "views_per_step": args.batch, "gaussians_start": N, "gaussians_end": r["slots_end"],
                    "width": W, "height": H, "sh_degree": deg, "mean_pairs_per_view": int(r["mean_pairs"]),
+                   "depth_key_bits": args.depth_key,
                    "densify_events": r["densify_events"], "device": device, "tuning": r["tuning"],
                    "kernel_timing": r["kernel_timing"], "overflow_steps": r["overflow_steps"]},
         "plain": None if plain is None else {

@@ -70,8 +70,12 @@ __global__ void __launch_bounds__(kSortThreads) k_radix_upsweep(const SortSeg* _
   const SortSeg& g = segs[blockIdx.y];
   if (blockIdx.x >= g.blocks) return;  // a smaller segment of the launch
   const uint32_t n = seg_count(g.count, g.count_max);
-  const uint32_t* __restrict__ keys = from_alt ? g.keys_alt : g.keys;
   const uint32_t radix = 1u << nbits, mask = radix - 1u;
+  if (blockIdx.x * kSortItems >= n) {  // past the segment's items: zero counts
+    for (uint32_t d = threadIdx.x; d < radix; d += kSortThreads) g.hist[size_t(d) * g.blocks + blockIdx.x] = 0u;
+    return;
+  }
+  const uint32_t* __restrict__ keys = from_alt ? g.keys_alt : g.keys;
   const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
   for (int d = threadIdx.x; d < kWaves * 256; d += kSortThreads) (&h[0][0])[d] = 0u;
   __syncthreads();
