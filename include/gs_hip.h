@@ -110,7 +110,9 @@ typedef struct gs_hip_workspace gs_hip_workspace;
  *  RECORDS                      float[capacity][12] (mean xy, conic abc, opacity, rgb, bbox lo/hi, radius|clamp)
  *  FOOTPRINT                    float[capacity][8]  (dilated cov2D abc, threshold | mean xy, tiles as u32 bits, 0):
  *                               everything the pair emitter reads of a slot, in one 32 B line
- *  TILES, DEPTH_KEYS, DEPTH_ORDER, OFFSETS  uint32[capacity]  (OFFSETS in depth order)
+ *  TILES, DEPTH_KEYS, DEPTH_ORDER, OFFSETS  uint32[capacity]  (OFFSETS in depth order; with
+ *                               GS_HIP_FLAG_DEPTH16 a key is the 16-bit depth | tiles << 16, sorted by
+ *                               its lower half, 0xffff for a slot that touches no tile)
  *  GRAD2D                       float[9][capacity]  (planar)
  *  PAIR_TILES, PAIR_GAUSSIANS   uint32[pair_capacity] (sorted)
  *  RANGES                       uint32[tiles + 1][2]

@@ -277,8 +277,15 @@ int upload_segments(gs_hip_workspace* ws, const std::vector<TableEntry>& entries
                        v.sort.digit_totals, slots, v.capacity, blocks};
     tile[i] = SortSeg{v.pair_tiles, v.pair_gaussians, v.sort.keys_alt, v.sort.vals_alt, v.sort.hist,
                       v.sort.digit_totals, totals, uint32_t(std::min<uint64_t>(v.pair_capacity, 0xffffffffu)), blocks};
-    scan[i] = ScanSeg{v.tiles, v.depth_order, v.offsets, v.block_sums, totals, slots, v.capacity,
-                      v.block_sums_entries};
+    // 16-bit depth keys carry the slot's tile count in their upper half (the sort ranks the
+    // lower half only), so the scan reads the sorted keys in order instead of gathering.
+    if ((entries[i].frame->flags & GS_HIP_FLAG_DEPTH16) != 0u) {
+      scan[i] = ScanSeg{v.depth_keys, nullptr, v.offsets, v.block_sums, totals, slots, v.capacity,
+                        v.block_sums_entries, 16u};
+    } else {
+      scan[i] = ScanSeg{v.tiles, v.depth_order, v.offsets, v.block_sums, totals, slots, v.capacity,
+                        v.block_sums_entries, 0u};
+    }
   }
   GS_HIP_TRY(hipStreamSynchronize(ws->stream));
   GS_HIP_TRY(hipMemcpy(ws->dev_depth_sort, depth.data(), sizeof(SortSeg) * n, hipMemcpyHostToDevice));
@@ -332,6 +339,7 @@ int upload_entries(gs_hip_workspace* ws, const std::vector<TableEntry>& entries)
     shape.max_tiles = std::max(shape.max_tiles, v.tile_grid);
     shape.max_pair_capacity = std::max(shape.max_pair_capacity, v.pair_capacity);
     sig = mix64(mix64(mix64(sig, reinterpret_cast<uintptr_t>(&v)), v.pair_capacity), v.sort.items);
+    sig = mix64(sig, e.frame->flags & GS_HIP_FLAG_DEPTH16);
   }
   shape.scenes = uint32_t(scenes.size());
   shape.tile_key_bits = uint32_t(bits_for(shape.max_tiles + 1));

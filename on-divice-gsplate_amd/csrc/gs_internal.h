@@ -60,8 +60,8 @@ struct SortSeg {
 };
 
 // One segment of a batched exclusive scan: out[i] = sum of values[gather[j]]
-// (or values[j]) for j < i, i < n = min(*count, count_max) (count_max when
-// count is null); the total goes to *total.
+// (or values[j] >> shift) for j < i, i < n = min(*count, count_max)
+// (count_max when count is null); the total goes to *total.
 struct ScanSeg {
   const uint32_t* values;
   const uint32_t* gather;
@@ -71,6 +71,7 @@ struct ScanSeg {
   const uint32_t* count;
   uint32_t count_max;
   uint32_t blocks;  // block_sums entries; a launch's extra blocks idle
+  uint32_t shift;
 };
 
 // A scene's model as the per-slot kernels see it, and its views in the step's

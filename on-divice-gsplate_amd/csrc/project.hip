@@ -139,12 +139,14 @@ __device__ __forceinline__ void project_view(const gsh::ViewDev& vd, const Proje
   vd.footprint[size_t(i) * 2u + 1u] = make_float4(rec.a.x, rec.a.y, __uint_as_float(tiles), 0.0f);
   vd.tiles[i] = tiles;
   vd.misc[i] = __float_as_uint(rec.c.w);
-  uint32_t key = 0xffffffffu;
+  const bool depth16 = (f.flags & GS_HIP_FLAG_DEPTH16) != 0u;
+  uint32_t key = depth16 ? 0xffffu : 0xffffffffu;
   if (tiles > 0u) {
-    if ((f.flags & GS_HIP_FLAG_DEPTH16) != 0u) {
-      // The reference's 16-bit log depth (train_project.comp:110).
+    if (depth16) {
+      // The reference's 16-bit log depth (train_project.comp:110) in the lower half; the sort
+      // ranks those 16 bits only, and the tile count (< 2^16) rides in the upper half.
       const float u = gs_logf(tz / kNear) / gs_logf(kFar / kNear);
-      key = uint32_t(fminf(fmaxf(u, 0.0f), 1.0f) * 65535.0f);
+      key = uint32_t(fminf(fmaxf(u, 0.0f), 1.0f) * 65535.0f) | (tiles << 16);
     } else {
       key = __float_as_uint(tz);  // z > 0: the bits order like the values
     }

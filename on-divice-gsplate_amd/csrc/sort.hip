@@ -237,7 +237,7 @@ constexpr int kScanRounds = 16;
 constexpr int kScanItems = kScanThreads * kScanRounds;
 
 __device__ __forceinline__ uint32_t load_value(const ScanSeg& s, uint32_t idx) {
-  return s.gather != nullptr ? s.values[s.gather[idx]] : s.values[idx];
+  return s.gather != nullptr ? s.values[s.gather[idx]] : s.values[idx] >> s.shift;
 }
 
 __global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const ScanSeg* __restrict__ segs) {

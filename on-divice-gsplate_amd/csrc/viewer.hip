@@ -161,7 +161,7 @@ __global__ void __launch_bounds__(256) k_view_project(ViewArgs a) {
         }
         rec.c = make_float4(to_half(col[1]), __uint_as_float(lo), __uint_as_float(hi), to_half(col[2]));
         const float u = gs_logf(tz / kNear) / gs_logf(kFar / kNear);
-        key = 65534u - uint32_t(fminf(fmaxf(u, 0.0f), 1.0f) * 65534.0f);
+        key = (65534u - uint32_t(fminf(fmaxf(u, 0.0f), 1.0f) * 65534.0f)) | (tiles << 16);  // tile count rides above
         atomicAdd(a.visible, 1u);
       }
     }

@@ -79,8 +79,13 @@ def test_project_and_bin_bit_exact(name, n, deg, w, h, seed, depth16):
         rec, foot, tiles, keys, order, pt, pg, ranges = gpu_state(ws, n, pairs)
     r_rec, r_foot, r_tiles, r_keys = orc.project32(params, sh, cam, deg, depth16=depth16)
     np.testing.assert_array_equal(tiles, r_tiles)
-    # keys are sorted in place by the bin step: compare through the order
-    np.testing.assert_array_equal(keys, r_keys[order])
+    # keys are sorted in place by the bin step: compare through the order; 16-bit keys carry the
+    # slot's tile count in their upper half
+    if depth16:
+        np.testing.assert_array_equal(keys & 0xFFFF, r_keys[order] & 0xFFFF)
+        np.testing.assert_array_equal(keys >> 16, tiles[order])
+    else:
+        np.testing.assert_array_equal(keys, r_keys[order])
     np.testing.assert_array_equal(order, np.argsort(r_keys, kind="stable").astype(np.uint32))
     np.testing.assert_array_equal(rec.view(np.uint32), r_rec.view(np.uint32))
     np.testing.assert_array_equal(foot.view(np.uint32), r_foot.view(np.uint32))
