@@ -21,6 +21,7 @@ import json
 import math
 import os
 import platform
+import re
 import sys
 import time
 
@@ -146,6 +147,20 @@ def run(gs, ws, cams, target_bufs, order, per_step, views_total, warmup_views, d
             log(f"  {per_step}-view steps: {s + 1}/{steps}")
     gs.synchronize()
     return time.perf_counter() - t0, steps
+
+
+TRAFFIC_PROFILE = os.path.join("profiles", "hbm_traffic_8views_1M_1280x720_pmc.md")
+
+
+def profiled_traffic(kernel):
+    """HBM bytes of one 8-view launch of `kernel` (calibrated FETCH_SIZE + WRITE_SIZE) from the
+    rocprofv3 summary of this workload, or None."""
+    try:
+        text = open(os.path.join(ROOT, TRAFFIC_PROFILE)).read()
+    except OSError:
+        return None
+    m = re.search(r"^\| k_%s \| \d+ \| ([0-9.]+) \| ([0-9.]+) \| ([0-9.]+) \|" % re.escape(kernel), text, re.M)
+    return int(float(m.group(3)) * 1e6) if m else None
 
 
 RESOLUTIONS = ((640, 360), (1280, 720), (1920, 1080))
@@ -438,8 +453,13 @@ def main():
     roofline = None
     if dominant is not None and "GBps" in kernels[dominant]:
         ach = kernels[dominant]["GBps"]
+        default_workload = (args.batch == 8 and N == 1_000_000 and (W, H) == (1280, 720) and deg == 3 and
+                            args.depth_key == 16)
+        traffic = profiled_traffic(dominant) if default_workload else None
         roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK / 1e9,
-                    "unit": "GB/s", "frac": round(ach * 1e9 / HBM_PEAK, 5), "traffic": None,
+                    "unit": "GB/s", "frac": round(ach * 1e9 / HBM_PEAK, 5), "traffic": traffic,
+                    "traffic_source": TRAFFIC_PROFILE if traffic is not None else None,
+                    "algorithmic_bytes": kernels[dominant].get("algorithmic_bytes"),
                     "ms_avg": round(kernels[dominant]["ms_avg"], 4)}
     # Whole-step bytes per view-iteration; the batched step reads and writes the
     # optimizer state once per `batch` views.
