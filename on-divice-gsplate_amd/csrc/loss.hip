@@ -60,6 +60,14 @@ __device__ __forceinline__ void load_xy(const gs_hip_frame& f, const LossArgs& a
   }
 }
 
+// Tile of block b among n: consecutive blocks go to the 8 XCDs in turn, so XCD x
+// gets the contiguous run of tiles [x q + min(x, r), ...) (q = n / 8, r = n % 8)
+// and neighbouring tiles' halos meet in one L2.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
+  const uint32_t x = b & 7u, idx = b >> 3, q = n >> 3, r = n & 7u;
+  return x * q + min(x, r) + idx;
+}
+
 __device__ __forceinline__ LossArgs loss_args(const gsh::ViewDev& v) {
   LossArgs a;
   a.image = v.image;
@@ -73,14 +81,14 @@ __device__ __forceinline__ LossArgs loss_args(const gsh::ViewDev& v) {
 // One tile (blockIdx.x) of one view (blockIdx.y).
 __global__ void __launch_bounds__(256) k_ssim(const gsh::ViewDev* __restrict__ views) {
   const gsh::ViewDev& view = views[blockIdx.y];
-  if (blockIdx.x >= view.tile_count) return;  // a smaller view of the launch: the whole block leaves
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  if (tile >= view.tile_count) return;  // a smaller view of the launch: the whole block leaves
   const gs_hip_frame& f = view.f;
   const LossArgs a = loss_args(view);
   __shared__ float s_x[3][kHalo * kStride];
   __shared__ float s_y[3][kHalo * kStride];
   __shared__ float s_h[5][kHalo * kTile];
   __shared__ float s_sum[3][4];
-  const uint32_t tile = blockIdx.x;
   const int tx = int(tile % f.tiles_x), ty = int(tile / f.tiles_x);
   const int ox = tx * kTile - kSsimRadius, oy = ty * kTile - kSsimRadius;
   const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
@@ -169,12 +177,12 @@ __global__ void __launch_bounds__(256) k_ssim(const gsh::ViewDev* __restrict__ v
 
 __global__ void __launch_bounds__(256) k_loss_grad(const gsh::ViewDev* __restrict__ views) {
   const gsh::ViewDev& view = views[blockIdx.y];
-  if (blockIdx.x >= view.tile_count) return;  // a smaller view of the launch: the whole block leaves
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  if (tile >= view.tile_count) return;  // a smaller view of the launch: the whole block leaves
   const gs_hip_frame& f = view.f;
   const LossArgs a = loss_args(view);
   __shared__ float s_m[3][kHalo * kStride];
   __shared__ float s_h[3][kHalo * kTile];
-  const uint32_t tile = blockIdx.x;
   const int tx = int(tile % f.tiles_x), ty = int(tile / f.tiles_x);
   const int ox = tx * kTile - kSsimRadius, oy = ty * kTile - kSsimRadius;
   const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;

@@ -322,41 +322,94 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_down(const ScanSeg* __res
 // ---- pair emission and ranges ------------------------------------------------
 // One thread per slot in depth order: writes the slot's pairs (tile, slot) at
 // its offset, row by row over the same footprint the projection counted
-// (train_emit.comp:26-64).  Pairs past the buffer are dropped (the step is
-// then an overflow); a count that differs (it cannot: same code, same
-// inputs) is padded with the unused tile `tiles` and counted in totals[1].
+// (train_emit.comp:26-64).  A slot with more than kWaveEmit tiles is written
+// by its whole wave instead: the wave walks its tile rows together and the
+// lanes write one row's tiles side by side, so one large splat neither holds
+// its wave for hundreds of iterations nor scatters single-lane stores.  Pairs
+// past the buffer are dropped (the step is then an overflow); a count that
+// differs (it cannot: same code, same inputs) is padded with the unused tile
+// `tiles` and counted in totals[1].
+constexpr uint32_t kWaveEmit = 32;
+
 __global__ void __launch_bounds__(256) k_emit(const ViewDev* __restrict__ views) {
   const ViewDev& v = views[blockIdx.y];
   const gs_hip_frame& f = v.f;
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= f.slots) return;
-  const uint32_t i = v.depth_order[k];
-  // One 32 B line per slot: the depth order makes these reads random.
-  const float4 fp = v.footprint[size_t(i) * 2u];
-  const float4 r0 = v.footprint[size_t(i) * 2u + 1u];  // mean x, mean y, tiles
-  const uint32_t count = __float_as_uint(r0.z);
-  if (count == 0u) return;
-  uint64_t p = v.offsets[k];
-  const uint64_t end = std::min<uint64_t>(p + count, v.pair_capacity);
-  uint32_t wanted = 0;
-  int row_first, row_last;
-  footprint_rows(r0.y, fp.z, fp.w, f.cam.height, row_first, row_last);
-  if (row_first <= row_last) {
-    for (uint32_t ty = uint32_t(row_first) / kTile; ty <= uint32_t(row_last) / kTile; ++ty) {
-      uint32_t x0, x1;
-      footprint_row_tiles(r0.x, r0.y, fp.x, fp.y, fp.z, fp.w, row_first, row_last, ty, f.cam.width, x0, x1);
-      wanted += x1 - x0;
-      for (uint32_t x = x0; x < x1 && p < end; ++x, ++p) {
-        v.pair_tiles[p] = ty * f.tiles_x + x;
+  const int lane = int(threadIdx.x & 63u);
+  uint32_t i = 0u, count = 0u;
+  float4 fp = make_float4(0, 0, 0, 0), r0 = fp;
+  if (k < f.slots) {
+    i = v.depth_order[k];
+    // One 32 B line per slot: the depth order makes these reads random.
+    fp = v.footprint[size_t(i) * 2u];
+    r0 = v.footprint[size_t(i) * 2u + 1u];  // mean x, mean y, tiles
+    count = __float_as_uint(r0.z);
+  }
+  const uint64_t p0 = count > 0u ? uint64_t(v.offsets[k]) : 0ull;
+  const uint64_t cap = v.pair_capacity;
+  const uint32_t pad_tile = f.tiles_x * f.tiles_y;
+  if (count > 0u && count <= kWaveEmit) {
+    uint64_t p = p0;
+    const uint64_t end = min(p + count, cap);
+    uint32_t wanted = 0;
+    int row_first, row_last;
+    footprint_rows(r0.y, fp.z, fp.w, f.cam.height, row_first, row_last);
+    if (row_first <= row_last) {
+      for (uint32_t ty = uint32_t(row_first) / kTile; ty <= uint32_t(row_last) / kTile; ++ty) {
+        uint32_t x0, x1;
+        footprint_row_tiles(r0.x, r0.y, fp.x, fp.y, fp.z, fp.w, row_first, row_last, ty, f.cam.width, x0, x1);
+        wanted += x1 - x0;
+        for (uint32_t x = x0; x < x1 && p < end; ++x, ++p) {
+          v.pair_tiles[p] = ty * f.tiles_x + x;
+          v.pair_gaussians[p] = i;
+        }
+      }
+    }
+    if (wanted != count) {
+      atomicAdd(&v.totals[1], 1u);
+      for (; p < end; ++p) {
+        v.pair_tiles[p] = pad_tile;
         v.pair_gaussians[p] = i;
       }
     }
   }
-  if (wanted == count) return;
-  atomicAdd(&v.totals[1], 1u);
-  for (; p < end; ++p) {
-    v.pair_tiles[p] = f.tiles_x * f.tiles_y;
-    v.pair_gaussians[p] = i;
+  // Large splats, one at a time per wave: every lane takes part.
+  uint64_t big = __ballot(count > kWaveEmit);
+  while (big != 0ull) {
+    const int src = __builtin_ctzll(big);
+    big &= big - 1ull;
+    const uint32_t bi = __shfl(i, src, 64), bcount = __shfl(count, src, 64);
+    const float4 bfp = make_float4(__shfl(fp.x, src, 64), __shfl(fp.y, src, 64), __shfl(fp.z, src, 64),
+                                   __shfl(fp.w, src, 64));
+    const float bmx = __shfl(r0.x, src, 64), bmy = __shfl(r0.y, src, 64);
+    const uint64_t start = (uint64_t(__shfl(uint32_t(p0 >> 32), src, 64)) << 32) | __shfl(uint32_t(p0), src, 64);
+    const uint64_t end = min(start + bcount, cap);
+    uint64_t p = start;
+    uint32_t wanted = 0;
+    int row_first, row_last;
+    footprint_rows(bmy, bfp.z, bfp.w, f.cam.height, row_first, row_last);
+    if (row_first <= row_last) {
+      for (uint32_t ty = uint32_t(row_first) / kTile; ty <= uint32_t(row_last) / kTile; ++ty) {
+        uint32_t x0, x1;
+        footprint_row_tiles(bmx, bmy, bfp.x, bfp.y, bfp.z, bfp.w, row_first, row_last, ty, f.cam.width, x0, x1);
+        wanted += x1 - x0;
+        for (uint32_t x = x0 + uint32_t(lane); x < x1; x += 64u) {
+          const uint64_t q = p + (x - x0);
+          if (q < end) {
+            v.pair_tiles[q] = ty * f.tiles_x + x;
+            v.pair_gaussians[q] = bi;
+          }
+        }
+        p += x1 - x0;
+      }
+    }
+    if (wanted != bcount) {
+      if (lane == 0) atomicAdd(&v.totals[1], 1u);
+      for (uint64_t q = min(p, end) + uint64_t(lane); q < end; q += 64u) {
+        v.pair_tiles[q] = pad_tile;
+        v.pair_gaussians[q] = bi;
+      }
+    }
   }
 }
 
@@ -367,22 +420,24 @@ __global__ void __launch_bounds__(256) k_clear_ranges(const ViewDev* __restrict_
   if (i == 0u) v.totals[1] = 0u;
 }
 
+// Tile boundaries of the sorted pairs (grid-stride: the grid is bounded, the
+// count is on the device).
 __global__ void __launch_bounds__(256) k_ranges(const ViewDev* __restrict__ views) {
   const ViewDev& v = views[blockIdx.y];
   const uint32_t count = uint32_t(std::min<uint64_t>(v.totals[0], v.pair_capacity));
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= count) return;
-  const uint32_t tile = v.pair_tiles[s];
-  if (s == 0u) {
-    v.ranges[tile].x = 0u;
-  } else {
-    const uint32_t prev = v.pair_tiles[s - 1u];
-    if (prev != tile) {
-      v.ranges[prev].y = s;
-      v.ranges[tile].x = s;
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < count; s += gridDim.x * blockDim.x) {
+    const uint32_t tile = v.pair_tiles[s];
+    if (s == 0u) {
+      v.ranges[tile].x = 0u;
+    } else {
+      const uint32_t prev = v.pair_tiles[s - 1u];
+      if (prev != tile) {
+        v.ranges[prev].y = s;
+        v.ranges[tile].x = s;
+      }
     }
+    if (s == count - 1u) v.ranges[tile].y = count;
   }
-  if (s == count - 1u) v.ranges[tile].y = count;
 }
 
 // The rasterizers' work: every (view, tile) of the step, as view << 16 | tile,
@@ -497,7 +552,7 @@ int launch_tile_sort_and_ranges(gs_hip_workspace* ws, uint32_t count) {
   timer_end(ws);
   if (rc != 0) return rc;
   timer_begin(ws, "ranges");
-  k_ranges<<<dim3(div_up(cap, 256), count), 256, 0, ws->stream>>>(ws->dev_views);
+  k_ranges<<<dim3(std::min<uint32_t>(div_up(cap, 256), 2048), count), 256, 0, ws->stream>>>(ws->dev_views);
   GS_CHECK_LAUNCH("k_ranges");
   k_work_order<<<1, 1024, 0, ws->stream>>>(ws->dev_views, count, ws->work, ws->work_counters);
   GS_CHECK_LAUNCH("k_work_order");
