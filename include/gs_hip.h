@@ -49,6 +49,10 @@ extern "C" {
 #define GS_HIP_FLAG_DEPTH16 4u        /* order pairs by the reference's 16-bit log depth (train_project.comp:110)
                                          instead of the float bits of z */
 #define GS_HIP_FLAG_NO_UPDATE 8u      /* run the step but leave parameters, moments and statistics alone */
+#define GS_HIP_FLAG_POSE_GRAD 16u     /* this view's camera pose gradient (train_step.glsl:196-201, 253-269), read
+                                         with gs_hip_pose_gradient; with NO_UPDATE the reference's kFlagPoseOnly */
+#define GS_HIP_FLAG_EXPOSURE 32u      /* photo rgb times exposure[] (train_forward.comp:123) and the view's gain
+                                         fit sums (142-151), read with gs_hip_exposure_sums */
 
 /* Pinhole camera, scene -> camera, OpenCV axes (x right, y down, z forward).
  * Same fields as gs::Camera (engine/include/gs/camera.h:10-16). */
@@ -83,6 +87,8 @@ typedef struct gs_hip_frame {
   float prune_max_scale;  /* > 0: also prune larger world scales */
   float prune_max_radius; /* > 0: also prune larger screen radii */
   float split_divisor;    /* split halves: scale / this (1.6) */
+  float exposure[3];      /* with GS_HIP_FLAG_EXPOSURE: the photo's rgb is multiplied by these (1 / the view's
+                             gain: the photo at the model's brightness, trainer.cpp:442-444) */
 } gs_hip_frame;
 
 /* ---- 1. device and memory ------------------------------------------------ */
@@ -123,6 +129,10 @@ typedef struct gs_hip_workspace gs_hip_workspace;
  *  TILE_LOSS                    float[tiles][4] (sum |e|, sum e^2, sum SSIM, -)
  *  TOTALS                       uint32[16]     ([0] pairs of the last bin, [1] emit mismatches)
  *  TILE_ORDER                   uint32[tiles]  (the rasterizers' launch order: longest lists first)
+ *  TILE_FIT                     float[tiles][8] (with GS_HIP_FLAG_EXPOSURE: sum C^2 rgb, fitted pixels,
+ *                               sum C photo rgb, subject pixels; C the model's colour before the background)
+ *  POSE_GRAD                    float[ceil(capacity / 256)][8] (with GS_HIP_FLAG_POSE_GRAD, per 256 slots:
+ *                               dL/dd xyz, 0, dL/dw xyz, 0)
  * The model buffers (PARAMS .. STATS) are shared; the others belong to a view. */
 enum gs_hip_buffer {
   GS_HIP_BUF_PARAMS = 0,
@@ -149,6 +159,8 @@ enum gs_hip_buffer {
   GS_HIP_BUF_TILE_LOSS,
   GS_HIP_BUF_TOTALS,
   GS_HIP_BUF_TILE_ORDER,
+  GS_HIP_BUF_TILE_FIT,
+  GS_HIP_BUF_POSE_GRAD,
   GS_HIP_BUF_COUNT
 };
 
@@ -216,6 +228,18 @@ int gs_hip_train_step(gs_hip_workspace* ws, const gs_hip_frame* frame, uint64_t*
  * pairs[count], losses[3 * count] may be NULL. */
 int gs_hip_train_views(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t count, const void* const* targets,
                        uint64_t* pairs, double* losses);
+/* Camera pose gradient of view `view`'s latest step that carried GS_HIP_FLAG_POSE_GRAD: the loss's
+ * derivative by a shift d of the camera ([R | t] -> [R | t + d]) and by a turn w about its centre
+ * ([R | t] -> [E R | E t], E = exp([w]x)), in camera axes: grad = {dL/dd xyz, dL/dw xyz}, the
+ * per-256-slot partials of POSE_GRAD summed in double (Trainer::pose_gradient, trainer.cpp:1319-1330).
+ * Colour's dependence on the view direction is left out, as in the reference.  Waits for the stream. */
+int gs_hip_pose_gradient(gs_hip_workspace* ws, uint32_t view, double grad[6]);
+/* Exposure fit sums of view `view`'s latest step that carried GS_HIP_FLAG_EXPOSURE, over pixels whose
+ * photo is opaque (alpha > 0.95) and render nearly so (T < 0.05): sums[0..2] = sum C^2, sums[3] = those
+ * pixels, sums[4..6] = sum C photo (the photo as stored, before exposure[]), sums[7] = pixels with
+ * alpha > 0; the least-squares gain per channel is sums[4 + c] / sums[c] (trainer.cpp:786-792,
+ * 1697-1711).  Summed in double over TILE_FIT.  Waits for the stream. */
+int gs_hip_exposure_sums(gs_hip_workspace* ws, uint32_t view, double sums[8]);
 /* Forward only (render): project, bin, forward; image as RGBA8 (clamped rgb, alpha = 1 - T). */
 int gs_hip_render(gs_hip_workspace* ws, const gs_hip_frame* frame, uint8_t* rgba8_out);
 

@@ -131,6 +131,8 @@ int create_view(gs_hip_workspace* ws, uint32_t index, View** out) {
   A(alloc(&v->target, pixels));
   A(alloc(&v->dldc, pixels * 4));
   A(alloc(&v->ssim_maps, pixels * 9));
+  A(alloc(&v->tile_fit, tiles * 8));
+  A(alloc(&v->pose_grad, div_up(cap, 256) * 8));
   v->block_sums_entries = div_up(cap + 1, 4096) + 1024;
   A(alloc(&v->block_sums, v->block_sums_entries));
   v->totals = ws->totals_all + size_t(index) * 16;
@@ -177,6 +179,8 @@ void destroy_view(View* v) {
   release(v->target);
   release(v->dldc);
   release(v->ssim_maps);
+  release(v->tile_fit);
+  release(v->pose_grad);
   release(v->block_sums);
   release(v->sort.keys_alt);
   release(v->sort.vals_alt);
@@ -249,6 +253,8 @@ ViewDev make_view_dev(const View& v, const gs_hip_frame& f, uint32_t* totals, fl
   d.dldc = reinterpret_cast<float4*>(v.dldc);
   d.maps = v.ssim_maps;
   d.tile_loss = reinterpret_cast<float4*>(tile_loss);
+  d.tile_fit = reinterpret_cast<float4*>(v.tile_fit);
+  d.pose_grad = reinterpret_cast<float4*>(v.pose_grad);
   d.totals = totals;
   d.pair_capacity = v.pair_capacity;
   d.capacity = v.capacity;
@@ -339,7 +345,9 @@ int upload_entries(gs_hip_workspace* ws, const std::vector<TableEntry>& entries)
     shape.max_tiles = std::max(shape.max_tiles, v.tile_grid);
     shape.max_pair_capacity = std::max(shape.max_pair_capacity, v.pair_capacity);
     sig = mix64(mix64(mix64(sig, reinterpret_cast<uintptr_t>(&v)), v.pair_capacity), v.sort.items);
-    sig = mix64(sig, e.frame->flags & GS_HIP_FLAG_DEPTH16);
+    // The flags that change the launch sequence (a captured graph holds one).
+    sig = mix64(sig, e.frame->flags & (GS_HIP_FLAG_DEPTH16 | GS_HIP_FLAG_POSE_GRAD));
+    shape.any_pose = shape.any_pose || (e.frame->flags & GS_HIP_FLAG_POSE_GRAD) != 0u;
   }
   shape.scenes = uint32_t(scenes.size());
   shape.tile_key_bits = uint32_t(bits_for(shape.max_tiles + 1));
@@ -668,6 +676,8 @@ int gs_hip_workspace_view_buffer(gs_hip_workspace* ws, uint32_t view, int which,
     case GS_HIP_BUF_TILE_LOSS: p = v.tile_loss, b = tiles * 16; break;
     case GS_HIP_BUF_TOTALS: p = v.totals, b = 64; break;
     case GS_HIP_BUF_TILE_ORDER: p = ws->work, b = tiles * 4 * ws->views.size(); break;
+    case GS_HIP_BUF_TILE_FIT: p = v.tile_fit, b = tiles * 32; break;
+    case GS_HIP_BUF_POSE_GRAD: p = v.pose_grad, b = gsh::div_up(uint32_t(std::max<size_t>(cap, 1)), 256) * 32; break;
     default: return gsh::set_error(GS_HIP_E_ARGUMENT, "workspace_buffer: unknown buffer");
   }
   *ptr = p;

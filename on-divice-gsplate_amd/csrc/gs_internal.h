@@ -114,6 +114,8 @@ struct ViewDev {
   float4* dldc;
   float* maps;  // [9][pixels]: SSIM gradient maps A, B, C per channel
   float4* tile_loss;
+  float4* tile_fit;   // [tiles][2]: exposure fit sums (GS_HIP_FLAG_EXPOSURE)
+  float4* pose_grad;  // [capacity / 256][2]: pose gradient partials (GS_HIP_FLAG_POSE_GRAD)
   uint32_t* totals;  // [0] pairs, [1] emit mismatches, [5..7] densify counters, [8] visible splats (viewer)
   uint64_t pair_capacity;
   uint32_t capacity;    // the scene's slot capacity: plane stride of grad2d and sh_input
@@ -137,6 +139,7 @@ struct StepShape {
   uint32_t total_tiles = 0;        // every view's tiles (rasterizer work items)
   uint32_t tile_key_bits = 0;      // tile sort key width
   bool any_sh = false;             // some scene stores SH rest coefficients
+  bool any_pose = false;           // some view asks for its pose gradient
 };
 
 // Radix sort scratch of one view, for up to `items` (key, value) pairs.
@@ -172,6 +175,8 @@ struct View {
   float* dldc = nullptr;
   float* ssim_maps = nullptr;
   float* tile_loss = nullptr;  // slice of the workspace's tile_loss_all
+  float* tile_fit = nullptr;   // [tiles][8]
+  float* pose_grad = nullptr;  // [div_up(capacity, 256)][8]
   uint32_t* totals = nullptr;  // slice of the workspace's totals_all
   uint32_t* block_sums = nullptr;
   uint32_t block_sums_entries = 0;

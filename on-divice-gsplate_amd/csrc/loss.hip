@@ -11,6 +11,8 @@
 // k_loss_grad: blurs A, B, C back with the same window and forms
 //   dL/dx_q = (1 - lambda) sign(e_q) / (3HW)
 //             - lambda / (3HW) ((G*A)(q) + 2 x_q (G*B)(q) + y_q (G*C)(q)).
+#include <vector>
+
 #include "gs_common.h"
 #include "gs_internal.h"
 #include "wave.h"
@@ -49,8 +51,14 @@ __device__ __forceinline__ void load_xy(const gs_hip_frame& f, const LossArgs& a
   rgb_x[1] = im.y + im.w * f.background[1];
   rgb_x[2] = im.z + im.w * f.background[2];
   const uint32_t t = a.target[id];
-  const float tr = float(t & 0xffu) * (1.0f / 255.0f), tg = float((t >> 8) & 0xffu) * (1.0f / 255.0f);
-  const float tb = float((t >> 16) & 0xffu) * (1.0f / 255.0f), ta = float(t >> 24) * (1.0f / 255.0f);
+  float tr = float(t & 0xffu) * (1.0f / 255.0f), tg = float((t >> 8) & 0xffu) * (1.0f / 255.0f);
+  float tb = float((t >> 16) & 0xffu) * (1.0f / 255.0f);
+  const float ta = float(t >> 24) * (1.0f / 255.0f);
+  if ((f.flags & GS_HIP_FLAG_EXPOSURE) != 0u) {  // the photo at the model's brightness
+    tr *= f.exposure[0];
+    tg *= f.exposure[1];
+    tb *= f.exposure[2];
+  }
   if ((f.flags & GS_HIP_FLAG_MASKED_TARGET) != 0u) {
     rgb_y[0] = f.background[0] * (1.0f - ta) + tr * ta;
     rgb_y[1] = f.background[1] * (1.0f - ta) + tg * ta;
@@ -76,6 +84,46 @@ __device__ __forceinline__ LossArgs loss_args(const gsh::ViewDev& v) {
   a.dldc = v.dldc;
   a.tile_loss = v.tile_loss;
   return a;
+}
+
+// The exposure gain's least-squares sums over one tile (train_forward.comp:142-151):
+// where the photo is opaque (alpha > 0.95) and the render nearly so (T < 0.05),
+// C^2 and C times the photo as stored (C the model's colour before the
+// background), those pixels, and the pixels with any alpha.  pix < 0: outside
+// the image.  Every thread of the block calls it.
+__device__ void exposure_fit(const gsh::ViewDev& view, uint32_t tile, int pix) {
+  __shared__ float s_fit[8][4];
+  float v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (pix >= 0) {
+    const float4 im = view.image[pix];
+    const uint32_t t = view.target[pix];
+    const float ta = float(t >> 24) * (1.0f / 255.0f);
+    if (ta > 0.95f && im.w < 0.05f) {
+      const float photo[3] = {float(t & 0xffu) * (1.0f / 255.0f), float((t >> 8) & 0xffu) * (1.0f / 255.0f),
+                              float((t >> 16) & 0xffu) * (1.0f / 255.0f)};
+      const float C[3] = {im.x, im.y, im.z};
+      for (int c = 0; c < 3; ++c) {
+        v[c] = C[c] * C[c];
+        v[4 + c] = C[c] * photo[c];
+      }
+      v[3] = 1.0f;
+    }
+    v[7] = ta > 0.0f ? 1.0f : 0.0f;
+  }
+  const int wave = threadIdx.x >> 6;
+  for (int k = 0; k < 8; ++k) {
+    const float w = wave_sum(v[k]);
+    if ((threadIdx.x & 63) == 0) s_fit[k][wave] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    float r[4];
+    for (int k = 0; k < 4; ++k) {
+      const int q = int(threadIdx.x) * 4 + k;
+      r[k] = s_fit[q][0] + s_fit[q][1] + s_fit[q][2] + s_fit[q][3];
+    }
+    view.tile_fit[size_t(tile) * 2u + threadIdx.x] = make_float4(r[0], r[1], r[2], r[3]);
+  }
 }
 
 // One tile (blockIdx.x) of one view (blockIdx.y).
@@ -173,6 +221,7 @@ __global__ void __launch_bounds__(256) k_ssim(const gsh::ViewDev* __restrict__ v
     for (int q = 0; q < 3; ++q) r[q] = s_sum[q][0] + s_sum[q][1] + s_sum[q][2] + s_sum[q][3];
     a.tile_loss[tile] = make_float4(r[0], r[1], r[2], 0.0f);
   }
+  if ((f.flags & GS_HIP_FLAG_EXPOSURE) != 0u) exposure_fit(view, tile, inside ? int(pix) : -1);
 }
 
 __global__ void __launch_bounds__(256) k_loss_grad(const gsh::ViewDev* __restrict__ views) {
@@ -282,6 +331,20 @@ int gs_hip_loss(gs_hip_workspace* ws, const gs_hip_frame* f, double losses[3]) {
     if (rc != 0) return rc;
     GS_HIP_TRY(hipStreamSynchronize(ws->stream));
     gsh::sum_losses(ws, 0, losses);
+  }
+  return 0;
+}
+
+int gs_hip_exposure_sums(gs_hip_workspace* ws, uint32_t view, double sums[8]) {
+  if (ws == nullptr || sums == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "exposure_sums: null argument");
+  if (view >= ws->views.size()) return gsh::set_error(GS_HIP_E_ARGUMENT, "exposure_sums: no such view");
+  const gsh::View& v = *ws->views[view];
+  std::vector<float> fit(size_t(v.tile_grid) * 8);
+  GS_HIP_TRY(hipStreamSynchronize(ws->stream));
+  GS_HIP_TRY(hipMemcpy(fit.data(), v.tile_fit, fit.size() * sizeof(float), hipMemcpyDeviceToHost));
+  for (int k = 0; k < 8; ++k) sums[k] = 0.0;
+  for (uint32_t t = 0; t < v.tile_grid; ++t) {
+    for (int k = 0; k < 8; ++k) sums[k] += fit[size_t(t) * 8 + k];
   }
   return 0;
 }

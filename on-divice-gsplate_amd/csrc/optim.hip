@@ -17,6 +17,7 @@
 
 #include "gs_common.h"
 #include "gs_internal.h"
+#include "wave.h"
 
 namespace {
 
@@ -126,11 +127,14 @@ __device__ __forceinline__ void sh_direction_gradient(float x, float y, float z,
 
 // The projection adjoint of one view (train_step.glsl:102-219): 2D splat
 // gradient g[9] -> dL/d(position, logit, log scale, quaternion, DC) added to
-// the accumulators, and the view's SH colour gradient and direction.
+// the accumulators, and the view's SH colour gradient and direction.  kPose:
+// also the slot's share of the camera pose gradient, pose[6] = dL/dd, dL/dw
+// (train_step.glsl:196-201).
+template <bool kPose = false>
 __device__ __forceinline__ void adjoint_view(const gs_hip_camera& cam, const float g[9], uint32_t clamped, float4 p0,
                                              float4 p1, float4 q, const float (&rest)[GS_HIP_MAX_SH_REST][3],
                                              int coeffs, uint32_t degree, float4& dp0, float4& dp1, float4& dq,
-                                             float4& dp3, float dcol[3], float dir[3]) {
+                                             float4& dp3, float dcol[3], float dir[3], float* pose = nullptr) {
   const float W = float(cam.width), H = float(cam.height);
   const float* V = cam.rotation;
   const float tx = V[0] * p0.x + V[1] * p0.y + V[2] * p0.z + cam.translation[0];
@@ -225,6 +229,24 @@ __device__ __forceinline__ void adjoint_view(const gs_hip_camera& cam, const flo
   dtx += g[0] * fx * inv_z;
   dty += g[1] * fy * inv_z;
   dtz += -(g[0] * fx * tx + g[1] * fy * ty) * inv_z2;
+  if constexpr (kPose) {
+    // A shift d of the camera moves t by d; a turn w moves t by w x t and the
+    // rotation in T = J W by [w]x W, whose share of dL/dw is the antisymmetric
+    // part of GW = J^T dT W^T.
+    float P[3][3];
+    for (int c = 0; c < 3; ++c) {
+      P[0][c] = j00 * dT[0][c];
+      P[1][c] = j11 * dT[1][c];
+      P[2][c] = j02 * dT[0][c] + j12 * dT[1][c];
+    }
+    auto GW = [&](int r, int c) { return P[r][0] * V[c * 3] + P[r][1] * V[c * 3 + 1] + P[r][2] * V[c * 3 + 2]; };
+    pose[0] = dtx;
+    pose[1] = dty;
+    pose[2] = dtz;
+    pose[3] = ty * dtz - tz * dty + (GW(2, 1) - GW(1, 2));
+    pose[4] = tz * dtx - tx * dtz + (GW(0, 2) - GW(2, 0));
+    pose[5] = tx * dty - ty * dtx + (GW(1, 0) - GW(0, 1));
+  }
   add4(dp0, V[0] * dtx + V[3] * dty + V[6] * dtz + dpos_dir[0], V[1] * dtx + V[4] * dty + V[7] * dtz + dpos_dir[1],
        V[2] * dtx + V[5] * dty + V[8] * dtz + dpos_dir[2], dlogit);
 
@@ -325,6 +347,62 @@ __global__ void __launch_bounds__(256) k_adam(const gsh::SceneDev* __restrict__ 
   a.params[i * 4u + 1u] = adam4(p1, scale4(dp1), a.m_params + i * 4u + 1u, a.v_params + i * 4u + 1u, lr1, f);
   a.params[i * 4u + 2u] = adam4(q, scale4(dq), a.m_params + i * 4u + 2u, a.v_params + i * 4u + 2u, lrq, f);
   a.params[i * 4u + 3u] = adam4(p3, scale4(dp3), a.m_params + i * 4u + 3u, a.v_params + i * 4u + 3u, lr3, f);
+}
+
+// Camera pose gradient of each view that asks for it (GS_HIP_FLAG_POSE_GRAD,
+// train_step.glsl:196-201, 253-269): every visible slot's share from the
+// adjoint's camera-space terms, summed per block of 256 slots into the view's
+// pose_grad[block] = (dL/dd xyz, 0), (dL/dw xyz, 0); the host adds the blocks
+// in double.  Runs before k_adam, on the parameters the step rendered with:
+// 64 B of parameters and 36 B of GRAD2D per visible slot and view.  A step
+// whose pairs overflowed writes zeros (its gradients are incomplete; it runs
+// again).  Each view's gradient is that of its own loss.
+__global__ void __launch_bounds__(256) k_pose_grad(const gsh::SceneDev* __restrict__ scenes,
+                                                   const gsh::ViewDev* __restrict__ views, uint32_t count) {
+  const AdamArgs a = scene_args(scenes[blockIdx.y], views, count);
+  if (blockIdx.x * 256u >= a.capacity) return;  // past this scene's slots: the whole block
+  bool overflow = false;
+  for (uint32_t v = 0; v < a.step_count; ++v) {
+    overflow = overflow || uint64_t(a.step_views[v].totals[0]) > a.step_views[v].pair_capacity;
+  }
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  bool live = !overflow && i < a.views[0].f.slots;
+  float4 p0 = make_float4(0, 0, 0, 0), p1 = p0, q = p0;
+  if (live) {
+    p0 = a.params[i * 4u];
+    p1 = a.params[i * 4u + 1u];
+    q = a.params[i * 4u + 2u];
+    live = p1.w >= 0.5f;  // not a free slot
+  }
+  __shared__ float s_part[6][4];
+  const float rest[GS_HIP_MAX_SH_REST][3] = {};
+  const int wave = threadIdx.x >> 6;
+  for (uint32_t v = 0; v < a.count; ++v) {
+    const gsh::ViewDev& view = a.views[v];
+    if ((view.f.flags & GS_HIP_FLAG_POSE_GRAD) == 0u) continue;  // the same for the whole block
+    float pose[6] = {0, 0, 0, 0, 0, 0};
+    if (live && view.tiles[i] != 0u) {
+      float g[9];
+      for (int c = 0; c < 9; ++c) g[c] = view.grad2d[size_t(c) * a.capacity + i];
+      float4 dp0 = make_float4(0, 0, 0, 0), dp1 = dp0, dq = dp0, dp3 = dp0;
+      float dcol[3], dir[3];
+      adjoint_view<true>(view.f.cam, g, 0u, p0, p1, q, rest, 0, 0u, dp0, dp1, dq, dp3, dcol, dir, pose);
+    }
+    for (int k = 0; k < 6; ++k) {
+      const float w = wave_sum(pose[k]);
+      if ((threadIdx.x & 63) == 0) s_part[k][wave] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+      float r[3];
+      for (int k = 0; k < 3; ++k) {
+        const int c = int(threadIdx.x) * 3 + k;
+        r[k] = s_part[c][0] + s_part[c][1] + s_part[c][2] + s_part[c][3];
+      }
+      view.pose_grad[size_t(blockIdx.x) * 2u + threadIdx.x] = make_float4(r[0], r[1], r[2], 0.0f);
+    }
+    __syncthreads();
+  }
 }
 
 // Adam on coefficients [k0, k0 + kShGroup) of one slot: all loads of the
@@ -574,6 +652,16 @@ int launch_adam(gs_hip_workspace* ws, uint32_t count) {
   return 0;
 }
 
+int launch_pose(gs_hip_workspace* ws, uint32_t count) {
+  if (ws->shape.max_capacity == 0) return 0;
+  const dim3 grid(div_up(ws->shape.max_capacity, 256), ws->shape.scenes);
+  timer_begin(ws, "pose");
+  k_pose_grad<<<grid, 256, 0, ws->stream>>>(ws->dev_scenes, ws->dev_views, count);
+  GS_CHECK_LAUNCH("k_pose_grad");
+  timer_end(ws);
+  return 0;
+}
+
 namespace {
 
 // What a captured graph was recorded for.
@@ -589,6 +677,7 @@ int launch_stages(gs_hip_workspace* ws, uint32_t count, bool depth16, bool train
   if (rc == 0) rc = launch_forward(ws, count);
   if (rc == 0 && (train || losses)) rc = launch_loss(ws, count);
   if (rc == 0 && train) rc = launch_backward(ws, count);
+  if (rc == 0 && train && ws->shape.any_pose) rc = launch_pose(ws, count);
   if (rc == 0 && train) rc = launch_adam(ws, count);
   if (rc == 0) rc = launch_readback(ws, count, losses);
   return rc;
@@ -700,8 +789,27 @@ extern "C" {
 
 int gs_hip_adam(gs_hip_workspace* ws, const gs_hip_frame* f) {
   if (ws == nullptr || f == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "adam: null argument");
-  const int rc = gsh::upload_table(ws, f, 1);
+  int rc = gsh::upload_table(ws, f, 1);
+  if (rc == 0 && ws->shape.any_pose) rc = gsh::launch_pose(ws, 1);
   return rc != 0 ? rc : gsh::launch_adam(ws, 1);
+}
+
+int gs_hip_pose_gradient(gs_hip_workspace* ws, uint32_t view, double grad[6]) {
+  if (ws == nullptr || grad == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "pose_gradient: null argument");
+  if (view >= ws->views.size()) return gsh::set_error(GS_HIP_E_ARGUMENT, "pose_gradient: no such view");
+  const gsh::View& v = *ws->views[view];
+  const uint32_t blocks = gsh::div_up(std::max<uint32_t>(ws->capacity, 1), 256);
+  std::vector<float> part(size_t(blocks) * 8);
+  GS_HIP_TRY(hipStreamSynchronize(ws->stream));
+  GS_HIP_TRY(hipMemcpy(part.data(), v.pose_grad, part.size() * sizeof(float), hipMemcpyDeviceToHost));
+  for (int c = 0; c < 6; ++c) grad[c] = 0.0;
+  for (uint32_t b = 0; b < blocks; ++b) {
+    for (int c = 0; c < 3; ++c) {
+      grad[c] += part[size_t(b) * 8 + c];
+      grad[3 + c] += part[size_t(b) * 8 + 4 + c];
+    }
+  }
+  return 0;
 }
 
 int gs_hip_densify(gs_hip_workspace* ws, const gs_hip_frame* f, uint32_t* slots, uint32_t counts[3]) {

@@ -23,11 +23,14 @@ FLAG_MASKED_TARGET = 1
 FLAG_SPARSE_ADAM = 2
 FLAG_DEPTH16 = 4
 FLAG_NO_UPDATE = 8
+FLAG_POSE_GRAD = 16
+FLAG_EXPOSURE = 32
 VIEW_ANTIALIASED = 1
 
 (BUF_PARAMS, BUF_SH, BUF_M_PARAMS, BUF_V_PARAMS, BUF_M_SH, BUF_V_SH, BUF_STATS, BUF_RECORDS, BUF_FOOTPRINT,
  BUF_TILES, BUF_DEPTH_KEYS, BUF_DEPTH_ORDER, BUF_OFFSETS, BUF_GRAD2D, BUF_PAIR_TILES, BUF_PAIR_GAUSSIANS,
- BUF_RANGES, BUF_IMAGE, BUF_LAST, BUF_TARGET, BUF_DLDC, BUF_TILE_LOSS, BUF_TOTALS, BUF_TILE_ORDER) = range(24)
+ BUF_RANGES, BUF_IMAGE, BUF_LAST, BUF_TARGET, BUF_DLDC, BUF_TILE_LOSS, BUF_TOTALS, BUF_TILE_ORDER, BUF_TILE_FIT,
+ BUF_POSE_GRAD) = range(26)
 MAX_VIEWS = 8
 
 EXPORTED = [
@@ -40,7 +43,7 @@ EXPORTED = [
     "gs_hip_render", "gs_hip_workspace_get_tuning", "gs_hip_workspace_set_tuning", "gs_hip_autotune",
     "gs_hip_workspace_overflows", "gs_hip_workspace_timings", "gs_hip_view_render",
     "gs_hip_batch_create", "gs_hip_batch_destroy", "gs_hip_batch_train", "gs_hip_batch_get_tuning",
-    "gs_hip_batch_set_tuning", "gs_hip_batch_timings",
+    "gs_hip_batch_set_tuning", "gs_hip_batch_timings", "gs_hip_pose_gradient", "gs_hip_exposure_sums",
 ]
 
 
@@ -69,7 +72,8 @@ class Frame(ctypes.Structure):
                 ("beta2", ctypes.c_float), ("epsilon", ctypes.c_float), ("bias1", ctypes.c_float),
                 ("bias2", ctypes.c_float), ("grad_threshold", ctypes.c_float), ("clone_max_scale", ctypes.c_float),
                 ("prune_opacity", ctypes.c_float), ("prune_max_scale", ctypes.c_float),
-                ("prune_max_radius", ctypes.c_float), ("split_divisor", ctypes.c_float)]
+                ("prune_max_radius", ctypes.c_float), ("split_divisor", ctypes.c_float),
+                ("exposure", ctypes.c_float * 3)]
 
 
 _lib = None
@@ -219,6 +223,7 @@ def make_frame(cam: Camera, slots: int, sh_degree: int, step: int = 0, backgroun
     f.prune_max_scale = 0.0
     f.prune_max_radius = 0.0
     f.split_divisor = 1.6
+    f.exposure[:] = [1.0, 1.0, 1.0]
     return f
 
 
@@ -353,6 +358,19 @@ class Workspace:
         check(self.lib.gs_hip_train_step(self.handle, ctypes.byref(frame), ctypes.byref(pairs),
                                          out if want_losses else None))
         return pairs.value, (tuple(out) if want_losses else None)
+
+    def pose_gradient(self, view: int = 0) -> np.ndarray:
+        """dL/dd xyz, dL/dw xyz of the view's latest step with FLAG_POSE_GRAD."""
+        out = (ctypes.c_double * 6)()
+        check(self.lib.gs_hip_pose_gradient(self.handle, view, out))
+        return np.array(out[:], np.float64)
+
+    def exposure_sums(self, view: int = 0) -> np.ndarray:
+        """The gain fit sums of the view's latest step with FLAG_EXPOSURE:
+        sum C^2 rgb, fitted pixels, sum C photo rgb, subject pixels."""
+        out = (ctypes.c_double * 8)()
+        check(self.lib.gs_hip_exposure_sums(self.handle, view, out))
+        return np.array(out[:], np.float64)
 
     def render(self, frame: Frame) -> np.ndarray:
         out = np.empty((self.height, self.width, 4), np.uint8)

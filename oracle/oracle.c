@@ -47,6 +47,9 @@
  *                             neighbours (ties to even)
  *   orc_view_render32         the reference viewer tests' closed forms (single-splat
  *                             falloff, depth order, axis-flip equivalence)
+ *   orc_pose_adjoint64        central FD of orc_loss64 under camera shifts and turns
+ *                             (SH degree 0), and the reference's identity with the
+ *                             per-gaussian gradients (test_trainer.cpp:404-464)
  */
 #include <math.h>
 #include <stdint.h>
@@ -1033,8 +1036,9 @@ double orc_loss64(const double* params, const double* sh, uint32_t n, uint32_t c
  * grad2d [n][9] -> dL/d(16 params) gparams and dL/d(SH rest) gsh. */
 static void adjoint_scene64(const scene64_t* sc, const double* params, const double* sh, uint32_t n,
                             uint32_t coeffs_in, const orc_camera* cam, int degree, const double* grad2d,
-                            double* gparams, double* gsh) {
+                            double* gparams, double* gsh, double* pose) {
   memset(gparams, 0, sizeof(double) * n * 16);
+  if (pose != NULL) memset(pose, 0, sizeof(double) * n * 6);
   if (coeffs_in > 0) memset(gsh, 0, sizeof(double) * n * coeffs_in * 3);
   const double* Rc = NULL;
   double V[3][3];
@@ -1166,6 +1170,22 @@ static void adjoint_scene64(const scene64_t* sc, const double* params, const dou
     dt[1] += o[1] * fy / z;
     dt[2] += -(o[0] * fx * t[0] + o[1] * fy * t[1]) * iz2;
     for (int k = 0; k < 3; ++k) gp[k] = V[0][k] * dt[0] + V[1][k] * dt[1] + V[2][k] * dt[2] + dpos[k];
+    if (pose != NULL) {
+      /* Camera pose (train_step.glsl:196-201): a shift d moves t by d; a turn
+         w moves t by w x t and turns W in T = J W by [w]x, whose share is the
+         antisymmetric part of GW = J^T dT W^T.  The view direction's share
+         (colour through SH) is left out, as in the reference. */
+      double P[3][3], GW[3][3];
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) P[r][c] = J[0][r] * dT[0][c] + J[1][r] * dT[1][c];
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) GW[r][c] = P[r][0] * V[c][0] + P[r][1] * V[c][1] + P[r][2] * V[c][2];
+      double* ps = pose + (size_t)i * 6;
+      for (int k = 0; k < 3; ++k) ps[k] = dt[k];
+      ps[3] = t[1] * dt[2] - t[2] * dt[1] + GW[2][1] - GW[1][2];
+      ps[4] = t[2] * dt[0] - t[0] * dt[2] + GW[0][2] - GW[2][0];
+      ps[5] = t[0] * dt[1] - t[1] * dt[0] + GW[1][0] - GW[0][1];
+    }
     /* scale and rotation */
     double dM[3][3], gR[3][3];
     for (int a = 0; a < 3; ++a)
@@ -1190,7 +1210,21 @@ static void adjoint_scene64(const scene64_t* sc, const double* params, const dou
 void orc_adjoint64(const double* params, const double* sh, uint32_t n, uint32_t coeffs_in, const orc_camera* cam,
                    int degree, const double* grad2d, double* gparams, double* gsh) {
   scene64_t s = build_scene64(params, sh, n, coeffs_in, cam, degree);
-  adjoint_scene64(&s, params, sh, n, coeffs_in, cam, degree, grad2d, gparams, gsh);
+  adjoint_scene64(&s, params, sh, n, coeffs_in, cam, degree, grad2d, gparams, gsh, NULL);
+  free_scene64(&s);
+}
+
+/* Camera pose gradient per gaussian (train_step.glsl:196-201): pose [n][6] =
+ * its share of dL/dd xyz, dL/dw xyz (camera axes) from grad2d [n][9]; the
+ * view's gradient is the sum over the gaussians (trainer.cpp:1319-1330). */
+void orc_pose_adjoint64(const double* params, const double* sh, uint32_t n, uint32_t coeffs_in, const orc_camera* cam,
+                        int degree, const double* grad2d, double* pose) {
+  scene64_t s = build_scene64(params, sh, n, coeffs_in, cam, degree);
+  double* gp = (double*)malloc(sizeof(double) * (n > 0 ? n : 1) * 16);
+  double* gs = (double*)malloc(sizeof(double) * (n > 0 ? n : 1) * (coeffs_in > 0 ? coeffs_in : 1) * 3);
+  adjoint_scene64(&s, params, sh, n, coeffs_in, cam, degree, grad2d, gp, gs, pose);
+  free(gp);
+  free(gs);
   free_scene64(&s);
 }
 
@@ -1270,7 +1304,7 @@ double orc_backward64(const double* params, const double* sh, uint32_t n, uint32
     }
   }
   free(contrib);
-  adjoint_scene64(&s, params, sh, n, coeffs_in, cam, degree, grad2d, gparams, gsh);
+  adjoint_scene64(&s, params, sh, n, coeffs_in, cam, degree, grad2d, gparams, gsh, NULL);
   free(rgb);
   free(Tf);
   free(tgt);

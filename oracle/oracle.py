@@ -57,6 +57,7 @@ def library() -> ctypes.CDLL:
             "orc_backward32": ([P(f32), P(u32), P(u32), P(f32), P(u32), P(f32), P(f32), u32, u32, u32, P(f64),
                                 P(ctypes.c_uint8)], None),
             "orc_adjoint64": ([P(f64), P(f64), u32, u32, cam_p, i32, P(f64), P(f64), P(f64)], None),
+            "orc_pose_adjoint64": ([P(f64), P(f64), u32, u32, cam_p, i32, P(f64), P(f64)], None),
             "orc_adam": ([P(f32), P(f32), P(f32), P(f32), P(f32), u64, f32, f32, f32, f32, f32], None),
             "orc_densify_mark": ([P(f32), P(f32), u32, f32, f32, f32, f32, f32, P(u32), P(u32), P(u32)], None),
             "orc_half": ([f32], f32),
@@ -285,6 +286,42 @@ def adjoint64(params, sh, cam, degree: int, grad2d):
                             ctypes.byref(camera(cam)), degree, _p(g2, ctypes.c_double), _p(gp, ctypes.c_double),
                             _p(gsp, ctypes.c_double))
     return gp, gs
+
+
+def pose_adjoint64(params, sh, cam, degree: int, grad2d):
+    """[n, 6]: each gaussian's share of the camera pose gradient (dL/dd xyz,
+    dL/dw xyz in camera axes) from grad2d [n, 9] (train_step.glsl:196-201);
+    the view's gradient is the column sum (trainer.cpp:1319-1330)."""
+    params, sh, shp, n, coeffs = _scene64(params, sh)
+    g2 = np.ascontiguousarray(grad2d, np.float64).reshape(n, 9)
+    out = np.zeros((n, 6), np.float64)
+    library().orc_pose_adjoint64(_p(params, ctypes.c_double), _p(shp, ctypes.c_double), n, coeffs,
+                                 ctypes.byref(camera(cam)), degree, _p(g2, ctypes.c_double), _p(out, ctypes.c_double))
+    return out
+
+
+def target_colour64(target, bg=(0.0, 0.0, 0.0), masked: bool = False, exposure=None) -> np.ndarray:
+    """[pixels, 3]: the colour the render is compared with (train_forward.comp:
+    122-127): the photo times the view's exposure (1 / its gain), over the
+    background by the photo's alpha when masked."""
+    t = np.asarray(target, np.uint8).reshape(-1, 4).astype(np.float64) / 255.0
+    photo = t[:, :3] * (np.asarray(exposure, np.float64) if exposure is not None else 1.0)
+    if masked:
+        return np.asarray(bg, np.float64) * (1.0 - t[:, 3:4]) + photo * t[:, 3:4]
+    return photo
+
+
+def exposure_sums64(image, target) -> np.ndarray:
+    """The exposure gain's least-squares sums of one view (train_forward.comp:
+    142-151, summed as trainer.cpp:786-792): over pixels whose photo alpha is
+    above 0.95 and whose final transmittance is below 0.05, sum C^2 per
+    channel, the pixel count, sum C x photo per channel; then the pixels with
+    any alpha.  image [pixels, 4] = (C rgb before the background, T)."""
+    im = np.asarray(image, np.float64).reshape(-1, 4)
+    t = np.asarray(target, np.uint8).reshape(-1, 4).astype(np.float64) / 255.0
+    fit = (t[:, 3] > 0.95) & (im[:, 3] < 0.05)
+    C, P = im[fit, :3], t[fit, :3]
+    return np.concatenate([(C * C).sum(0), [fit.sum()], (C * P).sum(0), [(t[:, 3] > 0).sum()]])
 
 
 def half(x) -> np.ndarray:

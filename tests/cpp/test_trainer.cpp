@@ -1,6 +1,8 @@
 // gs::Trainer on the GPU, restating the reference's trainer tests
 // (engine/tests/test_trainer.cpp): pair order (197-233), frozen-rate
-// gradients and parameters, render size check, densification at capacity
+// gradients and parameters, render size check, the pose gradient against the
+// per-gaussian gradients (404-464), pose recovery (509-536), exposure
+// recovery (700-742), held-out alignment, densification at capacity
 // (866-888) and fitting a known scene (890-897; train_selftest.cpp:692-739),
 // one view per optimizer step and four (batched, autotuned).
 #include <algorithm>
@@ -208,6 +210,199 @@ void test_training_converges(gs::Device& dev, uint32_t views_per_step) {
          t.stats().step, t.stats().optimizer_steps);
 }
 
+// Targets rendered from `scene` at the dataset's own cameras, alpha 255.
+void render_targets(gs::Device& dev, gs::Dataset& ds, const gs::SplatScene& scene) {
+  gs::Trainer renderer(dev, ds, scene, frozen(scene.sh_degree));
+  const float black[3] = {0, 0, 0};
+  for (gs::DatasetFrame& f : ds.frames) {
+    f.rgba = renderer.render(gs::dataset_camera(ds, f), black);
+    for (size_t p = 0; p < f.rgba.size() / 4; ++p) f.rgba[p * 4 + 3] = 255;
+  }
+}
+
+// Moves frame f's camera as the pose correction does: world -> camera [W | t]
+// becomes [E W | E t + d], E the rotation by w; stored back as camera-to-world
+// in OpenGL axes.
+void nudge_camera(gs::Dataset& ds, size_t f, const double d[3], const double w[3]) {
+  const gs::Camera cam = gs::dataset_camera(ds, ds.frames[f]);
+  const double theta = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+  const double k[3] = {w[0] / theta, w[1] / theta, w[2] / theta};
+  const double K[9] = {0, -k[2], k[1], k[2], 0, -k[0], -k[1], k[0], 0};
+  double E[9], W[9], t[3];
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) {
+      double k2 = 0.0;
+      for (int i = 0; i < 3; ++i) k2 += K[r * 3 + i] * K[i * 3 + c];
+      E[r * 3 + c] = (r == c ? 1.0 : 0.0) + std::sin(theta) * K[r * 3 + c] + (1.0 - std::cos(theta)) * k2;
+    }
+  }
+  for (int r = 0; r < 3; ++r) {
+    t[r] = d[r];
+    for (int c = 0; c < 3; ++c) {
+      W[r * 3 + c] = 0.0;
+      for (int i = 0; i < 3; ++i) W[r * 3 + c] += E[r * 3 + i] * cam.rotation[i * 3 + c];
+      t[r] += E[r * 3 + c] * cam.translation[c];
+    }
+  }
+  float* m = ds.frames[f].c2w;
+  for (int r = 0; r < 3; ++r) {
+    m[r * 4 + 0] = float(W[0 * 3 + r]);
+    m[r * 4 + 1] = float(-W[1 * 3 + r]);
+    m[r * 4 + 2] = float(-W[2 * 3 + r]);
+    m[r * 4 + 3] = float(-(W[0 * 3 + r] * t[0] + W[1 * 3 + r] * t[1] + W[2 * 3 + r] * t[2]));
+  }
+}
+
+// Distance between camera centres and the angle between their rotations (degrees).
+void camera_error(const gs::Camera& a, const gs::Camera& b, double* shift, double* turn) {
+  double d2 = 0.0, trace = 0.0;
+  for (int k = 0; k < 3; ++k) d2 += double(a.position[k] - b.position[k]) * (a.position[k] - b.position[k]);
+  for (int r = 0; r < 3; ++r)
+    for (int k = 0; k < 3; ++k) trace += double(a.rotation[r * 3 + k]) * b.rotation[r * 3 + k];
+  *shift = std::sqrt(d2);
+  *turn = std::acos(std::clamp((trace - 1.0) / 2.0, -1.0, 1.0)) * 180.0 / M_PI;
+}
+
+// Moving the camera is moving every gaussian the other way: dL/dd = W sum
+// dL/dp and dL/dw = W sum((p - c) x dL/dp + tau), tau the orientation's share
+// from the quaternion gradient (q -> (1, phi / 2) q).  SH degree 0: colour
+// does not depend on the view direction, which the pose gradient leaves out.
+void test_pose_gradient(gs::Device& dev, float lambda) {
+  gs::Dataset ds = test_dataset(64, 48, 2, true, 3);
+  gs::SplatScene scene = test_scene(120, 0, 9);
+  const float bg[3] = {0.2f, 0.3f, 0.4f};
+  gs::TrainOptions o = frozen(0);
+  o.masked = true;
+  o.lambda_dssim = o.lambda_dssim_end = lambda;
+  o.refine_poses = true;
+  o.pose_from = 0;
+  o.lr_pose_shift = o.lr_pose_turn = 0.0f;  // measure only
+  gs::Trainer t(dev, ds, scene, o);
+  t.step_on(0, bg);
+  const double* g = t.last_pose_gradient();
+  const gs::Trainer::Gradients grads = t.first_step_gradients();
+  const gs::Camera cam = gs::dataset_camera(ds, ds.frames[0]);
+  double sum_d[3] = {0, 0, 0}, sum_w[3] = {0, 0, 0};
+  for (uint32_t i = 0; i < scene.count; ++i) {
+    const float* gp = &grads.params[size_t(i) * 16];
+    double rel[3];
+    for (int k = 0; k < 3; ++k) {
+      sum_d[k] += gp[k];
+      rel[k] = double(scene.positions[i * 3 + k]) - cam.position[k];
+    }
+    sum_w[0] += rel[1] * gp[2] - rel[2] * gp[1];
+    sum_w[1] += rel[2] * gp[0] - rel[0] * gp[2];
+    sum_w[2] += rel[0] * gp[1] - rel[1] * gp[0];
+    const float* q = &scene.rotations[i * 4];
+    const double len = std::sqrt(double(q[0]) * q[0] + double(q[1]) * q[1] + double(q[2]) * q[2] + double(q[3]) * q[3]);
+    const double w = q[0] / len, u[3] = {q[1] / len, q[2] / len, q[3] / len};
+    const double gw = gp[8] * len, gu[3] = {gp[9] * len, gp[10] * len, gp[11] * len};
+    const double ug[3] = {u[1] * gu[2] - u[2] * gu[1], u[2] * gu[0] - u[0] * gu[2], u[0] * gu[1] - u[1] * gu[0]};
+    for (int k = 0; k < 3; ++k) sum_w[k] += 0.5 * (-gw * u[k] + w * gu[k] + ug[k]);
+  }
+  double expected[6], worst = 0.0, largest = 0.0;
+  for (int r = 0; r < 3; ++r) {
+    expected[r] = expected[3 + r] = 0.0;
+    for (int c = 0; c < 3; ++c) {
+      expected[r] += cam.rotation[r * 3 + c] * sum_d[c];
+      expected[3 + r] += cam.rotation[r * 3 + c] * sum_w[c];
+    }
+  }
+  std::printf("  pose gradient (lambda %.1f):", lambda);
+  for (int k = 0; k < 6; ++k) {
+    std::printf(" %c%c %.4g/%.4g", k < 3 ? 'd' : 'w', "xyz"[k % 3], g[k], expected[k]);
+    worst = std::max(worst, std::fabs(g[k] - expected[k]));
+    largest = std::max(largest, std::fabs(expected[k]));
+  }
+  std::printf("\n");
+  EXPECT(largest > 0.0 && worst <= 1e-3 * largest, "pose gradient off by %.3g (largest %.3g)", worst, largest);
+}
+
+// One camera of a rendered pair moved 10 cm and about 1.7 degrees (3 m away:
+// about 2 px at 128 px); with the gaussians frozen, refining that view's pose
+// must bring it back.
+void test_pose_recovery(gs::Device& dev) {
+  gs::Dataset ds = test_dataset(128, 96, 2, false, 3);
+  const gs::SplatScene scene = test_scene(300, 0, 9);
+  render_targets(dev, ds, scene);
+  const gs::Camera truth = gs::dataset_camera(ds, ds.frames[1]);
+  gs::Dataset moved = ds;
+  const double d[3] = {0.07, -0.05, 0.05}, rot[3] = {0.012, 0.02, -0.01};
+  nudge_camera(moved, 1, d, rot);
+  gs::TrainOptions o = frozen(0);
+  o.iterations = 200;
+  o.refine_poses = true;
+  o.pose_from = 0;
+  o.lr_pose_shift = 2e-3f;
+  o.lr_pose_turn = 2e-3f;
+  gs::Trainer t(dev, moved, scene, o);
+  const float black[3] = {0, 0, 0};
+  for (int s = 0; s < 150; ++s) t.step_on(1, black);
+  double shift0, turn0, shift1, turn1;
+  camera_error(gs::dataset_camera(moved, moved.frames[1]), truth, &shift0, &turn0);
+  camera_error(t.camera_for(1), truth, &shift1, &turn1);
+  std::printf("  pose recovery: %.1f cm, %.2f deg -> %.1f cm, %.2f deg (%u updates)\n", 100 * shift0, turn0,
+              100 * shift1, turn1, t.stats().pose_updates);
+  EXPECT(shift1 < 0.25 * shift0 && turn1 < 0.25 * turn0, "pose not recovered: %.3f m %.2f deg left", shift1, turn1);
+  EXPECT(t.stats().pose_updates == 150, "pose updates %u", t.stats().pose_updates);
+}
+
+// Held-out views are aligned before evaluation: a held-out camera moved as
+// above scores better after alignment than at its given pose.
+void test_heldout_alignment(gs::Device& dev) {
+  gs::Dataset ds = test_dataset(128, 96, 9, false, 5);  // frames 0 and 8 held out
+  const gs::SplatScene scene = test_scene(300, 0, 9);
+  render_targets(dev, ds, scene);
+  const double d[3] = {0.03, -0.02, 0.02}, rot[3] = {0.004, 0.006, -0.003};
+  nudge_camera(ds, 0, d, rot);
+  gs::TrainOptions o = frozen(0);
+  o.iterations = 100;
+  o.refine_poses = true;
+  o.eval_pose_steps = 12;
+  gs::Trainer t(dev, ds, scene, o);
+  const gs::EvalResult r = t.evaluate();
+  std::printf("  held-out alignment: %u views, %.2f dB at the given poses -> %.2f dB aligned\n", r.views,
+              r.psnr_unaligned, r.psnr);
+  EXPECT(r.aligned && r.views == 2 && r.psnr > r.psnr_unaligned + 1.0, "alignment: %.2f -> %.2f dB",
+         r.psnr_unaligned, r.psnr);
+}
+
+// Views rendered from a scene, one darkened and tinted as a camera's exposure
+// and white balance would; with the gaussians frozen the trained views'
+// gains come out in that ratio, their geometric mean held at 1.
+void test_exposure_recovery(gs::Device& dev) {
+  constexpr uint32_t w = 128, h = 96;
+  gs::Dataset ds = test_dataset(w, h, 3, false, 3);  // frame 0 held out
+  const gs::SplatScene scene = test_scene(300, 0, 9);
+  render_targets(dev, ds, scene);
+  const float dim[3] = {0.8f, 0.85f, 0.9f};
+  for (size_t p = 0; p < size_t(w) * h; ++p) {
+    for (int c = 0; c < 3; ++c) {
+      uint8_t& v = ds.frames[1].rgba[p * 4 + c];
+      v = uint8_t(std::lround(v * dim[c]));
+    }
+  }
+  gs::TrainOptions o = frozen(0);
+  o.iterations = 100;
+  o.refine_exposure = true;
+  o.exposure_from = 0;
+  gs::Trainer t(dev, ds, scene, o);
+  const float black[3] = {0, 0, 0};
+  for (int s = 0; s < 30; ++s) t.step_on(1 + s % 2, black);
+  float g1[3], g2[3], g0[3];
+  t.view_gain(1, g1);
+  t.view_gain(2, g2);
+  t.view_gain(0, g0);  // held out, before every trained view: view 1's
+  double worst = 0.0;
+  for (int c = 0; c < 3; ++c) worst = std::max(worst, std::fabs(double(g1[c]) / g2[c] / dim[c] - 1.0));
+  std::printf("  exposure: gains %.3f %.3f %.3f / %.3f %.3f %.3f, ratio off by %.2f%%, held-out %.3f, %u updates\n",
+              g1[0], g1[1], g1[2], g2[0], g2[1], g2[2], 100.0 * worst, g0[1], t.stats().exposure_updates);
+  EXPECT(worst < 0.02, "exposure gains off by %.2f%%", 100.0 * worst);
+  EXPECT(std::fabs(std::log(double(g1[1])) + std::log(double(g2[1]))) < 1e-4, "gains not centred: %.4f %.4f", g1[1],
+         g2[1]);
+  EXPECT(g0[0] == g1[0] && g0[1] == g1[1] && g0[2] == g1[2], "held-out view takes its neighbour's gain");
+}
+
 }  // namespace
 
 int main() {
@@ -217,6 +412,11 @@ int main() {
     std::fflush(stdout);
     test_pair_order(dev);
     test_frozen_step(dev);
+    test_pose_gradient(dev, 0.0f);
+    test_pose_gradient(dev, 0.2f);
+    test_pose_recovery(dev);
+    test_heldout_alignment(dev);
+    test_exposure_recovery(dev);
     test_densify_at_capacity(dev);
     test_training_converges(dev, 1);
     test_training_converges(dev, 4);

@@ -25,9 +25,10 @@ def test_binding_covers_the_header():
 
 
 def test_frame_struct_layout():
-    # the C struct: camera (9+3+3+4 floats + 2 u32 = 84 B), 6 u32, 3 floats bg, 2+1+6+5+6 floats
+    # the C struct: camera (9+3+3+4 floats + 2 u32 = 84 B), 6 u32, 3 floats bg, 2+1+6+5+6 floats, 3 exposure
     assert ctypes.sizeof(gs.Camera) == 84
-    assert ctypes.sizeof(gs.Frame) == 84 + 6 * 4 + 3 * 4 + 3 * 4 + 6 * 4 + 5 * 4 + 6 * 4
+    assert ctypes.sizeof(gs.Frame) == 84 + 6 * 4 + 3 * 4 + 3 * 4 + 6 * 4 + 5 * 4 + 6 * 4 + 3 * 4
+    assert gs.Frame.exposure.offset == ctypes.sizeof(gs.Frame) - 12
 
 
 def test_errors_surface_as_exceptions_without_a_gpu():

@@ -372,3 +372,131 @@ def test_backward32_tiles_matches_backward64():
         scale = np.abs(g64[:, c]).max()
         assert scale > 0
         assert np.abs(g32[:, c] - g64[:, c]).max() <= 1e-3 * scale, c
+
+
+def _rodrigues(w):
+    w = np.asarray(w, np.float64)
+    th = np.linalg.norm(w)
+    if th == 0.0:
+        return np.eye(3)
+    k = w / th
+    K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+    return np.eye(3) + math.sin(th) * K + (1 - math.cos(th)) * K @ K
+
+
+def moved_camera(cam, d=(0.0, 0.0, 0.0), w=(0.0, 0.0, 0.0)):
+    """The trainer's pose correction applied to a camera: world -> camera
+    [R | t] becomes [E R | E t + d], E the rotation by w (trainer.cpp:1294-1311)."""
+    R = np.array(cam.rotation, np.float64).reshape(3, 3)
+    t = np.array(cam.translation, np.float64)
+    E = _rodrigues(w)
+    R2, t2 = E @ R, E @ t + np.asarray(d, np.float64)
+    out = gs.Camera()
+    ctypes_copy = [("fx", cam.fx), ("fy", cam.fy), ("cx", cam.cx), ("cy", cam.cy), ("width", cam.width),
+                   ("height", cam.height)]
+    for name, v in ctypes_copy:
+        setattr(out, name, v)
+    out.rotation[:] = [float(v) for v in R2.reshape(-1)]
+    out.translation[:] = [float(v) for v in t2]
+    out.position[:] = [float(v) for v in -R2.T @ t2]
+    return out
+
+
+def test_pose_adjoint64_by_central_differences():
+    """The summed pose gradient against central differences of the fp64 loss
+    as the camera shifts (d) and turns (w) about its centre, SH degree 0 (the
+    gradient leaves out colour's dependence on the view direction).  A camera
+    move shifts every splat's alpha cutoff at once, a jump the analytic
+    gradient leaves out (test_trainer.cpp:409-412), so the scene is a few
+    wide, faint splats whose 1/255 contour lies outside the image and whose
+    product of (1 - alpha) stays far above the 1e-4 stop: the loss is then
+    smooth in the pose; the loss is D-SSIM alone (lambda 1: no kinks of
+    |e| where a pixel's error changes sign).  The camera is stored in fp32,
+    so the step is 1e-3: the moved camera's rounding (~1e-7) stays far below
+    the difference, and the O(h^2) error measured ~1e-5 relative."""
+    w, h = 64, 48
+    rng = np.random.default_rng(48)
+    n = 6
+    params = np.zeros((n, 16), np.float64)
+    params[:, 0:3] = 0.3 * rng.uniform(-1, 1, (n, 3))
+    params[:, 4:7] = np.log(2.0 + 0.8 * rng.uniform(-1, 1, (n, 3)))
+    q = rng.uniform(-1, 1, (n, 4))
+    q[:, 0] += 1.5
+    params[:, 8:12] = q
+    params[:, 3] = np.log(0.4 / 0.6) + 0.3 * rng.uniform(-1, 1, n)
+    params[:, 7] = 1.0
+    params[:, 12:15] = 0.8 * rng.uniform(-1, 1, (n, 3))
+    sh = np.zeros((n, 0, 3))
+    cam = make_camera(w, h, 0.7)
+    target = smooth_target(w, h, 9)
+    bg = (0.2, 0.1, 0.0)
+    # every splat above the cutoff over the whole image (the premise)
+    rec = orc.project64(params, sh, cam, 0)
+    yy, xx = np.mgrid[0:h, 0:w] + 0.5
+    for r in rec:  # visible, mx, my, depth, opacity, conic abc, ...
+        dx, dy = r[1] - xx, r[2] - yy
+        power = -0.5 * (r[5] * dx * dx + r[7] * dy * dy) - r[6] * dx * dy
+        assert r[0] == 1 and (r[4] * np.exp(power)).min() > 2.0 / 255.0
+    _, g2, _, _ = orc.backward64(params, sh, cam, 0, target, 1.0, bg=bg)
+    grad = orc.pose_adjoint64(params, sh, cam, 0, g2).sum(0)
+    eps = 1e-3
+    fd = np.zeros(6)
+    for k in range(6):
+        e = np.zeros(6)
+        e[k] = eps
+        lp = orc.loss64(params, sh, moved_camera(cam, e[:3], e[3:]), 0, target, 1.0, bg=bg)
+        lm = orc.loss64(params, sh, moved_camera(cam, -e[:3], -e[3:]), 0, target, 1.0, bg=bg)
+        fd[k] = (lp - lm) / (2 * eps)
+    assert np.abs(fd).min() > 0.02 * np.abs(fd).max()  # every component is exercised
+    np.testing.assert_allclose(grad, fd, rtol=5e-4)
+
+
+def test_pose_adjoint64_identity_with_gaussian_gradients():
+    """The reference's check of the pose gradient (test_trainer.cpp:404-464):
+    moving the camera is moving every gaussian the other way, so dL/dd =
+    W sum dL/dp and dL/dw = W sum((p - c) x dL/dp + tau), tau the
+    orientation's share read off the quaternion gradient.  SH degree 0."""
+    w, h = 96, 72
+    params, sh = make_scene(150, 0, 47)
+    cam = make_camera(w, h, 1.9)
+    target = smooth_target(w, h, 13)
+    p64, s64 = params.astype(np.float64), sh.astype(np.float64)
+    _, g2, gp, _ = orc.backward64(p64, s64, cam, 0, target, 0.2, bg=(0.2, 0.3, 0.4))
+    got = orc.pose_adjoint64(p64, s64, cam, 0, g2).sum(0)
+    R = np.array(cam.rotation, np.float64).reshape(3, 3)
+    c = np.array(cam.position, np.float64)
+    sum_d = gp[:, 0:3].sum(0)
+    rel = p64[:, 0:3] - c
+    q = p64[:, 8:12]
+    ln = np.linalg.norm(q, axis=1, keepdims=True)
+    qw, u = q[:, :1] / ln, q[:, 1:] / ln
+    gw, gu = gp[:, 8:9] * ln, gp[:, 9:12] * ln
+    tau = 0.5 * (-gw * u + qw * gu + np.cross(u, gu))
+    sum_w = (np.cross(rel, gp[:, 0:3]) + tau).sum(0)
+    want = np.concatenate([R @ sum_d, R @ sum_w])
+    assert np.abs(want).max() > 0
+    # (the camera centre is stored in fp32)
+    assert np.abs(got - want).max() <= 1e-6 * np.abs(want).max(), (got, want)
+
+
+def test_exposure_sums64_known_answers():
+    """A render whose colour is the photo divided by k where both are opaque:
+    the least-squares gain sum(C P) / sum(C^2) is k per channel; pixels with
+    photo alpha <= 0.95 or transmittance >= 0.05 are left out, pixels with
+    alpha > 0 are counted (train_forward.comp:142-151)."""
+    rng = np.random.default_rng(3)
+    n = 500
+    target = rng.integers(1, 256, size=(n, 4), dtype=np.uint8)
+    target[:, 3] = rng.choice([0, 100, 242, 243, 255], size=n)
+    k = np.array([1.25, 0.8, 1.1])
+    image = np.zeros((n, 4))
+    image[:, :3] = target[:, :3] / 255.0 / k
+    image[:, 3] = rng.choice([0.0, 0.049, 0.05, 0.3], size=n)
+    s = orc.exposure_sums64(image, target)
+    fit = (target[:, 3] >= 243) & (image[:, 3] < 0.05)
+    assert s[3] == fit.sum() > 50
+    assert s[7] == (target[:, 3] > 0).sum()
+    np.testing.assert_allclose(s[4:7] / s[0:3], k, rtol=1e-12)
+    y = orc.target_colour64(target, bg=(0.5, 0.25, 0.0), masked=True, exposure=1.0 / k)
+    a = target[:, 3:4] / 255.0
+    np.testing.assert_allclose(y, (0.5, 0.25, 0.0) * (1 - a) + image[:, :3] * a, rtol=1e-12)
