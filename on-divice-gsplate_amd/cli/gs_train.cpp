@@ -5,6 +5,8 @@
 //            [--init-gaussians 150000] [--opacity-reset N (0 = off)] [--grad-threshold 2e-4]
 //            [--dssim 0.2] [--dssim-end L] [--no-masks] [--sparse-adam] [--preview <dir>] [--eval-every 1000]
 //            [--views-per-step V] [--raster-groups G] [--depth-bits B] [--tile-bits B] [--graph 0|1] [--autotune]
+//            [--refine-poses] [--pose-lr shift,turn] [--pose-lr-final F] [--pose-from N] [--eval-pose-steps N]
+//            [--pose-out poses.json] [--refine-exposure] (a brightness gain per view and colour channel)
 //
 // The dataset is a nerfstudio transforms.json folder (gs::load_dataset); the
 // initial gaussians come from its depth maps, sparse points or cameras.  The
@@ -21,6 +23,7 @@
 #include <cstring>
 #include <exception>
 #include <random>
+#include <stdexcept>
 #include <string>
 
 #include "gs/dataset.h"
@@ -60,7 +63,8 @@ gs::SplatScene random_scene(uint32_t n, int degree, uint32_t seed) {
 
 struct Args {
   bool synthetic = false;
-  std::string dataset, out, gt_path, preview;
+  std::string dataset, out, gt_path, preview, pose_out;
+  int pose_from = -1;
   uint32_t width = 360, iters = 3000, init_gaussians = 150000, eval_every = 1000;
   int reset_override = -1;
   float grad_threshold = -1.0f;
@@ -76,6 +80,8 @@ void usage(const char* argv0) {
                "         [--init-gaussians N] [--opacity-reset N] [--grad-threshold G] [--dssim L] [--dssim-end L]\n"
                "         [--no-masks] [--sparse-adam] [--preview <dir>] [--eval-every N] [--views-per-step V]\n"
                "         [--raster-groups G] [--depth-bits B] [--tile-bits B] [--graph 0|1] [--autotune]\n"
+               "         [--refine-poses] [--pose-lr shift,turn] [--pose-lr-final F] [--pose-from N]\n"
+               "         [--eval-pose-steps N] [--pose-out poses.json] [--refine-exposure]\n"
                "       %s --synthetic <out.ply> [--gt scene | --gt-random N] [--init-random N] [--views V]\n"
                "         [--size WxH] [--iters I] [--sh D] [--seed S] [--dssim L] [--no-densify] [...]\n",
                argv0, argv0);
@@ -124,6 +130,16 @@ Args parse(int argc, char** argv) {
     else if (s == "--tile-bits") a.o.tile_digit_bits = std::atoi(next());
     else if (s == "--graph") a.o.graph = std::atoi(next());
     else if (s == "--autotune") a.o.autotune = true;
+    else if (s == "--refine-poses") a.o.refine_poses = true;
+    else if (s == "--pose-lr") {
+      if (std::sscanf(next(), "%f,%f", &a.o.lr_pose_shift, &a.o.lr_pose_turn) != 2) {
+        throw std::invalid_argument("--pose-lr needs shift,turn");
+      }
+    } else if (s == "--pose-lr-final") a.o.lr_pose_final = float(std::atof(next()));
+    else if (s == "--pose-from") a.pose_from = std::atoi(next());
+    else if (s == "--eval-pose-steps") a.o.eval_pose_steps = uint32_t(std::atoi(next()));
+    else if (s == "--pose-out") a.pose_out = next();
+    else if (s == "--refine-exposure") a.o.refine_exposure = true;
     else if (a.synthetic && s == "--gt") a.gt_path = next();
     else if (a.synthetic && s == "--gt-random") a.gt_random = uint32_t(std::atoi(next()));
     else if (a.synthetic && s == "--init-random") a.init_random = uint32_t(std::atoi(next()));
@@ -138,13 +154,37 @@ Args parse(int argc, char** argv) {
 }
 
 // Keeps the schedule proportional for shorter or longer runs (the reference's scaled defaults).
-void scale_schedule(gs::TrainOptions& o, uint32_t iters) {
+void scale_schedule(gs::TrainOptions& o, uint32_t iters, int pose_from) {
   const gs::TrainOptions d = gs::scaled_options(iters);
   o.iterations = d.iterations;
   o.densify_from = d.densify_from;
   o.densify_until = d.densify_until;
   o.opacity_reset_interval = d.opacity_reset_interval;
   o.sh_degree_interval = d.sh_degree_interval;
+  o.pose_from = pose_from >= 0 ? uint32_t(pose_from) : d.pose_from;
+  o.exposure_from = d.exposure_from;
+}
+
+// Every frame's camera-to-world (OpenGL axes, row-major) after refinement, as
+// the reference's --pose-out writes it (cli/gs_train.cpp:284-302).
+void write_poses(const gs::Trainer& trainer, const std::string& path) {
+  FILE* f = std::fopen(path.c_str(), "w");
+  if (f == nullptr) throw std::runtime_error("cannot write " + path);
+  std::fprintf(f, "{\"frames\": [");
+  const gs::Dataset& d = trainer.dataset();
+  for (size_t i = 0; i < d.frames.size(); ++i) {
+    const gs::Camera c = trainer.camera_for(i);
+    std::fprintf(f, "%s\n{\"file_path\": \"%s\", \"holdout\": %s, \"transform_matrix\": [", i ? "," : "",
+                 d.frames[i].name.c_str(), d.frames[i].holdout ? "true" : "false");
+    for (int r = 0; r < 3; ++r) {
+      // columns W^T e0, -W^T e1, -W^T e2 and the centre
+      std::fprintf(f, "%s[%.9g,%.9g,%.9g,%.9g]", r ? "," : "", c.rotation[r], -c.rotation[3 + r], -c.rotation[6 + r],
+                   c.position[r]);
+    }
+    std::fprintf(f, ",[0,0,0,1]]}");
+  }
+  std::fprintf(f, "\n]}\n");
+  std::fclose(f);
 }
 
 }  // namespace
@@ -179,7 +219,7 @@ int main(int argc, char** argv) {
       init = gs::initialize_gaussians(ds, io, &st);
       std::printf("init from %s: %zu points -> %u gaussians (voxel %.4f)\n", st.source.c_str(), st.raw_points,
                   init.count, st.voxel_size);
-      scale_schedule(a.o, a.iters);
+      scale_schedule(a.o, a.iters, a.pose_from);
     } else {
       gs::SplatScene gt;
       if (!a.gt_path.empty()) {
@@ -201,7 +241,7 @@ int main(int argc, char** argv) {
         }
       }
       init = random_scene(a.init_random ? a.init_random : gt.count, a.o.sh_degree, 2002 + a.seed);
-      scale_schedule(a.o, a.iters);
+      scale_schedule(a.o, a.iters, a.pose_from);
       a.o.sh_degree_interval = 0;
       a.o.lambda_dssim_end = -1.0f;
       a.o.random_background = false;
@@ -230,7 +270,18 @@ int main(int argc, char** argv) {
       }
       if (a.eval_every > 0 && (st.step / a.eval_every != last_print / a.eval_every || trainer.finished())) {
         const gs::EvalResult e = trainer.evaluate();
-        std::printf("  eval: %u views psnr %.2f ssim %.4f\n", e.views, e.psnr, e.ssim);
+        std::printf("  eval: %u views psnr %.2f ssim %.4f", e.views, e.psnr, e.ssim);
+        if (e.aligned) std::printf(" (poses aligned; %.2f dB at the given poses)", e.psnr_unaligned);
+        std::printf("\n");
+        if (a.o.refine_poses) {
+          std::printf("  poses: %u updates, shift mean %.2f max %.2f cm, turn mean %.3f max %.3f deg\n",
+                      st.pose_updates, 100.0 * st.pose_shift_mean, 100.0 * st.pose_shift_max, st.pose_turn_mean,
+                      st.pose_turn_max);
+        }
+        if (a.o.refine_exposure) {
+          std::printf("  exposure: %u updates, brightness gains %.3f..%.3f, tint spread %.3f\n", st.exposure_updates,
+                      st.exposure_low, st.exposure_high, st.exposure_tint);
+        }
         if (!a.preview.empty()) {
           const gs::Dataset& d = trainer.dataset();
           const float black[3] = {0, 0, 0};
@@ -248,6 +299,7 @@ int main(int argc, char** argv) {
     std::printf("trained %u views in %u optimizer steps, %.1f s (%.1f views/s); %u densify events, %u overflow reruns\n",
                 st.step, st.optimizer_steps, secs, st.step / std::max(secs, 1e-9), st.densify_events,
                 st.overflow_steps);
+    if (!a.pose_out.empty()) write_poses(trainer, a.pose_out);
     gs::SplatScene result = trainer.export_scene();
     gs::convert_convention(result, gs::AxisConvention::RDF);
     gs::write_file(a.out, gs::save_ply(result));

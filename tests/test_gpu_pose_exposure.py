@@ -157,3 +157,45 @@ def test_exposure_loss_and_fit_sums(masked):
     np.testing.assert_array_equal(sums[[3, 7]], want[[3, 7]])
     np.testing.assert_allclose(sums, want, rtol=1e-5)
     np.testing.assert_allclose(fit.astype(np.float64).sum(0), sums, rtol=1e-6)
+
+
+def test_pose_gradient_in_a_batch_of_scenes():
+    """Pose gradients of views in a batch of scenes (gs_hip_batch_train) equal
+    each scene's own step's, read from the scene's workspace."""
+    case("pose gradient in a batch of 2 scenes (SH 2 120x90 x2, SH 0 64x48 x1)")
+    specs = [(900, 2, 120, 90, 2, 91), (400, 0, 64, 48, 1, 92)]
+    inputs = []
+    for n, deg, w, h, views, seed in specs:
+        params, sh = make_scene(n, deg, seed)
+        cams = [make_camera(w, h, 0.5 + 1.1 * v + 0.1 * seed) for v in range(views)]
+        targets = [smooth_target(w, h, seed + v) for v in range(views)]
+        inputs.append((params, sh, cams, targets))
+    results = []
+    for batched in (False, True):
+        spaces, bufs = [], []
+        try:
+            for (n, deg, w, h, views, _), (params, sh, cams, targets) in zip(specs, inputs):
+                ws = gs.Workspace(n, deg, w, h)
+                ws.set_views(views)
+                ws.load(params, sh)
+                spaces.append(ws)
+                bufs.append([gs.DeviceBuffer(t) for t in targets])
+            fps = [[gs.make_frame(c, s[0], s[1], flags=gs.FLAG_POSE_GRAD, tp=frozen()) for c in inp[2]]
+                   for s, inp in zip(specs, inputs)]
+            if batched:
+                with gs.Batch(spaces) as batch:
+                    batch.train(fps, [[b.ptr for b in bs] for bs in bufs])
+            else:
+                for ws, fs, bs in zip(spaces, fps, bufs):
+                    ws.train_views(fs, [b.ptr for b in bs])
+            results.append([[ws.pose_gradient(v) for v in range(s[4])] for ws, s in zip(spaces, specs)])
+        finally:
+            for ws in spaces:
+                ws.close()
+            for bs in bufs:
+                for b in bs:
+                    b.free()
+    for own, bat in zip(*results):
+        for a, b in zip(own, bat):
+            print(f"[info] own {np.array2string(a, precision=5)} batch {np.array2string(b, precision=5)}", flush=True)
+            assert np.abs(a).max() > 0 and np.abs(a - b).max() <= 1e-4 * np.abs(a).max()

@@ -34,8 +34,11 @@ def png(img):
             chunk(b"IDAT", zlib.compress(rows)) + chunk(b"IEND", b""))
 
 
-def write_dataset(root, n, deg, w, h, views, seed):
+def write_dataset(root, n, deg, w, h, views, seed, dims=None, opacity=0.0):
+    """dims: {frame: brightness factor} applied to those photos (a camera's
+    exposure); opacity: added to every opacity logit."""
     params, sh = make_scene(n, deg, seed, spread=0.6)
+    params[:, 3] += opacity
     frames = []
     os.makedirs(os.path.join(root, "images"), exist_ok=True)
     os.makedirs(os.path.join(root, "masks"), exist_ok=True)
@@ -46,7 +49,8 @@ def write_dataset(root, n, deg, w, h, views, seed):
         rgb, T = orc.render64(params, sh, cam, deg)
         cover = np.clip(1.0 - T.reshape(h, w), 1e-6, 1.0)
         # the photo: colour un-premultiplied where covered; the mask: coverage
-        img = np.clip(np.rint(rgb.reshape(h, w, 3) / cover[..., None] * 255.0), 0, 255).astype(np.uint8)
+        dim = (dims or {}).get(k, 1.0)
+        img = np.clip(np.rint(rgb.reshape(h, w, 3) / cover[..., None] * 255.0 * dim), 0, 255).astype(np.uint8)
         mask = np.clip(np.rint((1.0 - T.reshape(h, w)) * 255.0), 0, 255).astype(np.uint8)
         name, mname = f"images/frame_{k:03d}.png", f"masks/frame_{k:03d}.png"
         with open(os.path.join(root, name), "wb") as f:
@@ -81,3 +85,37 @@ def test_gs_train_on_a_transforms_json_folder(tmp_path):
     head = out.read_bytes()[:2048].decode("latin-1")
     count = int(re.search(r"element vertex (\d+)", head).group(1))
     assert count > 100
+
+
+def test_gs_train_refines_poses_and_exposure(tmp_path):
+    """--refine-poses / --refine-exposure / --pose-out (cli/gs_train.cpp:107-124,
+    284-302): frames 3 and 11 darkened as a camera's exposure would; the run
+    reports pose and gain updates, aligns the held-out views before
+    evaluating, and writes every frame's refined camera-to-world."""
+    if not os.path.exists(EXE):
+        pytest.fail(f"{EXE} is not built")
+    print("[case] gs_train --refine-poses --refine-exposure --pose-out", flush=True)
+    data = tmp_path / "scene"
+    # opaque enough that each view has > 500 pixels for the gain fit (trainer.cpp:1702)
+    write_dataset(str(data), 600, 1, 96, 72, 16, 62, dims={3: 0.8, 11: 0.85}, opacity=1.5)
+    out, poses = tmp_path / "out.ply", tmp_path / "poses.json"
+    cmd = [EXE, str(data), str(out), "--width", "96", "--iters", "800", "--sh", "1", "--init-gaussians", "2000",
+           "--max-gaussians", "6000", "--views-per-step", "4", "--eval-every", "800", "--refine-poses",
+           "--pose-from", "200", "--pose-lr", "5e-4,5e-4", "--refine-exposure", "--pose-out", str(poses)]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=240)
+    print(r.stdout[-3000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    m = re.search(r"poses: (\d+) updates", r.stdout)
+    assert m and int(m.group(1)) == (800 - 200) // 4 * 4  # every trained view from step 200
+    m = re.search(r"exposure: (\d+) updates, brightness gains ([0-9.]+)\.\.([0-9.]+)", r.stdout)
+    assert m and int(m.group(1)) > 0 and float(m.group(2)) < 0.97 < 1.0 <= float(m.group(3))
+    assert re.search(r"eval: 2 views psnr [0-9.]+ ssim [0-9.]+ \(poses aligned; [0-9.]+ dB at the given poses\)",
+                     r.stdout)
+    doc = json.loads(poses.read_text())
+    assert len(doc["frames"]) == 16
+    assert sum(f["holdout"] for f in doc["frames"]) == 2
+    given = json.loads((data / "transforms.json").read_text())["frames"]
+    for f, g in zip(doc["frames"], given):
+        M, G = np.array(f["transform_matrix"]), np.array(g["transform_matrix"])
+        assert np.abs(M[:3, :3] @ M[:3, :3].T - np.eye(3)).max() < 1e-5
+        assert np.abs(M[:3, 3] - G[:3, 3]).max() < 0.1  # corrections stay small on exact poses
