@@ -122,7 +122,6 @@ int create_view(gs_hip_workspace* ws, uint32_t index, View** out) {
   A(alloc(&v->depth_order, cap));
   A(alloc(&v->offsets, cap));
   A(alloc(&v->grad2d, cap * GS_HIP_GRAD2D_FLOATS));
-  A(alloc(&v->sh_input, cap * 8));
   A(alloc(&v->flags, cap));
   A(alloc(&v->scan_tmp, cap + 1));
   A(alloc(&v->ranges, (tiles + 1) * 2));
@@ -168,7 +167,6 @@ void destroy_view(View* v) {
   release(v->depth_order);
   release(v->offsets);
   release(v->grad2d);
-  release(v->sh_input);
   release(v->flags);
   release(v->scan_tmp);
   release(v->pair_tiles);
@@ -243,7 +241,6 @@ ViewDev make_view_dev(const View& v, const gs_hip_frame& f, uint32_t* totals, fl
   d.depth_order = v.depth_order;
   d.offsets = v.offsets;
   d.grad2d = v.grad2d;
-  d.sh_input = reinterpret_cast<float4*>(v.sh_input);
   d.pair_tiles = v.pair_tiles;
   d.pair_gaussians = v.pair_gaussians;
   d.ranges = reinterpret_cast<uint2*>(v.ranges);
@@ -330,6 +327,7 @@ int upload_entries(gs_hip_workspace* ws, const std::vector<TableEntry>& entries)
       d.m_sh = m->m_sh;
       d.v_sh = m->v_sh;
       d.stats = reinterpret_cast<float4*>(m->stats);
+      d.dir_grad = reinterpret_cast<float4*>(m->dir_grad);
       d.capacity = m->capacity;
       d.sh_coeffs = m->sh_coeffs;
       d.first_view = i;
@@ -586,6 +584,7 @@ int gs_hip_workspace_create(gs_hip_workspace** out, uint32_t capacity, int sh_de
   A(alloc(&ws->m_sh, shn));
   A(alloc(&ws->v_sh, shn));
   A(alloc(&ws->stats, cap * 4));
+  if (ws->sh_coeffs > 0) A(alloc(&ws->dir_grad, cap * 3 * 4));
   if (tiles > gsh::kMaxTilesPerView) A(gsh::set_error(GS_HIP_E_ARGUMENT, "workspace_create: more than 65535 tiles"));
   A(gsh::alloc_tables(ws, GS_HIP_MAX_VIEWS, 1, uint32_t(tiles) * GS_HIP_MAX_VIEWS));
   if (rc == 0) {
@@ -637,6 +636,7 @@ int gs_hip_workspace_destroy(gs_hip_workspace* ws) {
   release(ws->m_sh);
   release(ws->v_sh);
   release(ws->stats);
+  release(ws->dir_grad);
   gsh::release_tables(ws);
   if (ws->stream != nullptr) hipStreamDestroy(ws->stream);
   delete ws;

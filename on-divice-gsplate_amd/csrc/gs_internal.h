@@ -84,6 +84,7 @@ struct SceneDev {
   float* m_sh;
   float* v_sh;
   float4* stats;
+  float4* dir_grad;  // [3][capacity]: per group of 5 SH rest coefficients, its view-direction part of dL/dposition
   uint32_t capacity;
   uint32_t sh_coeffs;
   uint32_t first_view;
@@ -101,8 +102,7 @@ struct ViewDev {
   uint32_t* depth_keys;
   uint32_t* depth_order;
   uint32_t* offsets;
-  float* grad2d;     // [9][capacity]
-  float4* sh_input;  // [capacity][2]: (dL/dcolour, update flag in view 0), (direction, -)
+  float* grad2d;  // [9][capacity]
   // per pair
   uint32_t* pair_tiles;
   uint32_t* pair_gaussians;
@@ -118,7 +118,7 @@ struct ViewDev {
   float4* pose_grad;  // [capacity / 256][2]: pose gradient partials (GS_HIP_FLAG_POSE_GRAD)
   uint32_t* totals;  // [0] pairs, [1] emit mismatches, [5..7] densify counters, [8] visible splats (viewer)
   uint64_t pair_capacity;
-  uint32_t capacity;    // the scene's slot capacity: plane stride of grad2d and sh_input
+  uint32_t capacity;    // the scene's slot capacity: plane stride of grad2d
   uint32_t tile_count;  // tiles_x * tiles_y
 };
 
@@ -161,7 +161,6 @@ struct View {
   uint32_t* depth_order = nullptr;
   uint32_t* offsets = nullptr;
   float* grad2d = nullptr;
-  float* sh_input = nullptr;
   uint32_t* flags = nullptr;     // densify: per-slot actions
   uint32_t* scan_tmp = nullptr;  // per-slot scan output
   uint64_t pair_capacity = 0;
@@ -210,6 +209,7 @@ struct gs_hip_workspace {
   float* m_sh = nullptr;
   float* v_sh = nullptr;
   float* stats = nullptr;
+  float* dir_grad = nullptr;  // [3][capacity][4] (SH rest stored)
   std::vector<gsh::View*> views;  // views[0] always exists (a batch's launch context borrows its scenes' views)
   bool borrowed_views = false;
   // device tables: ViewDev[table_views], SceneDev[table_scenes] and the batched sort / scan segments

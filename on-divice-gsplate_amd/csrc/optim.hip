@@ -1,10 +1,16 @@
 // Per-slot update: the projection adjoint (2D splat gradient -> position,
 // log scale, quaternion, opacity logit, SH DC and rest), the densification
-// statistics, and Adam on all of them, fused in one pass
-// (engine/shaders/train_step.glsl:62-240, train_step_sh.comp:21-44).  One
-// thread per slot; the SH planes are coefficient-major, so the SH loop reads
-// and writes each of sh, m, v as 768 B contiguous per wave and coefficient.
-// HBM-bound: about 6 x (64 B + 12 B per SH coefficient) + 36 B per slot.
+// statistics, and Adam on all of them (engine/shaders/train_step.glsl:62-240,
+// train_step_sh.comp:21-44), in two HBM-bound passes over the slots:
+//   k_adam_sh  one thread per (slot, group of 5 SH rest coefficients): the
+//              group's colour gradient over the step's views, its share of
+//              the view-direction part of dL/dposition (written to the
+//              scene's dir_grad plane), then Adam on the group's coefficients
+//              (coefficient-major planes: 768 B contiguous per wave and
+//              coefficient);
+//   k_adam     one thread per slot: the rest of the adjoint per view, plus the
+//              dir_grad planes, then Adam on the 16 parameters.
+// Splitting the SH rest out keeps its 45 floats out of k_adam's registers.
 //
 // Densification (train_densify_mark.comp, train_densify_spawn.comp) by stream
 // compaction: the mark kernel flags prune / clone / split per slot, two
@@ -13,6 +19,7 @@
 // past the slots in use.  The slot each new gaussian lands in is therefore a
 // function of the flags alone, not of atomic order.
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "gs_common.h"
@@ -31,6 +38,7 @@ struct AdamArgs {
   float* __restrict__ m_sh;
   float* __restrict__ v_sh;
   float4* __restrict__ stats;
+  float4* __restrict__ dir_grad;           // [3][capacity]
   const gsh::ViewDev* __restrict__ views;  // the scene's views; views[0].f holds the optimizer's rates
   uint32_t capacity;
   uint32_t sh_coeffs;
@@ -48,6 +56,7 @@ __device__ __forceinline__ AdamArgs scene_args(const gsh::SceneDev& sc, const gs
   a.m_sh = sc.m_sh;
   a.v_sh = sc.v_sh;
   a.stats = sc.stats;
+  a.dir_grad = sc.dir_grad;
   a.views = views + sc.first_view;
   a.capacity = sc.capacity;
   a.sh_coeffs = sc.sh_coeffs;
@@ -90,51 +99,45 @@ __device__ __forceinline__ void add4(float4& a, float x, float y, float z, float
   a.x += x, a.y += y, a.z += z, a.w += w;
 }
 
-// d(colour)/d(direction) summed against dL/dcolour: sum_k (sh_k . dcol)
-// grad basis_k(dir), over the slot's SH rest coefficients (in registers).
-__device__ __forceinline__ void sh_direction_gradient(float x, float y, float z, uint32_t degree,
-                                                      const float (&rest)[GS_HIP_MAX_SH_REST][3], const float dcol[3],
-                                                      float ddir[3]) {
-  auto w = [&](int k) { return rest[k][0] * dcol[0] + rest[k][1] * dcol[1] + rest[k][2] * dcol[2]; };
-  auto add = [&](float wk, float gx, float gy, float gz) {
-    ddir[0] += wk * gx;
-    ddir[1] += wk * gy;
-    ddir[2] += wk * gz;
-  };
-  const float C1 = 0.4886025119029199f;
-  add(w(0), 0.0f, -C1, 0.0f);
-  add(w(1), 0.0f, 0.0f, C1);
-  add(w(2), -C1, 0.0f, 0.0f);
-  if (degree < 2u) return;
+// SH rest coefficients per k_adam_sh thread, and the groups of 15.
+constexpr int kShGroup = 5;
+constexpr int kShGroups = (GS_HIP_MAX_SH_REST + kShGroup - 1) / kShGroup;
+
+// d basis_{K+1} / d(direction) of SH rest coefficient K (0..14).
+template <int K>
+__device__ __forceinline__ void sh_rest_dbasis(float x, float y, float z, float& gx, float& gy, float& gz) {
+  constexpr float C1 = 0.4886025119029199f;
+  constexpr float A = 1.0925484305920792f, B = 0.31539156525252005f, C = 0.5462742152960396f;
+  constexpr float D = -0.5900435899266435f, E = 2.890611442640554f, F = -0.4570457994644658f;
+  constexpr float G = 0.3731763325901154f, H = 1.445305721320277f;
   const float xx = x * x, yy = y * y, zz = z * z;
-  const float A = 1.0925484305920792f, B = 0.31539156525252005f, C = 0.5462742152960396f;
-  add(w(3), A * y, A * x, 0.0f);
-  add(w(4), 0.0f, -A * z, -A * y);
-  add(w(5), -2.0f * B * x, -2.0f * B * y, 4.0f * B * z);
-  add(w(6), -A * z, 0.0f, -A * x);
-  add(w(7), 2.0f * C * x, -2.0f * C * y, 0.0f);
-  if (degree < 3u) return;
-  const float D = -0.5900435899266435f, E = 2.890611442640554f, F = -0.4570457994644658f;
-  const float G = 0.3731763325901154f, H = 1.445305721320277f;
-  add(w(8), D * 6.0f * x * y, D * (3.0f * xx - 3.0f * yy), 0.0f);
-  add(w(9), E * y * z, E * x * z, E * x * y);
-  add(w(10), F * (-2.0f * x * y), F * (4.0f * zz - xx - 3.0f * yy), F * 8.0f * y * z);
-  add(w(11), G * (-6.0f * x * z), G * (-6.0f * y * z), G * (6.0f * zz - 3.0f * xx - 3.0f * yy));
-  add(w(12), F * (4.0f * zz - 3.0f * xx - yy), F * (-2.0f * x * y), F * 8.0f * x * z);
-  add(w(13), H * 2.0f * x * z, H * (-2.0f * y * z), H * (xx - yy));
-  add(w(14), D * (3.0f * xx - 3.0f * yy), D * (-6.0f * x * y), 0.0f);
+  if constexpr (K == 0) gx = 0.0f, gy = -C1, gz = 0.0f;
+  else if constexpr (K == 1) gx = 0.0f, gy = 0.0f, gz = C1;
+  else if constexpr (K == 2) gx = -C1, gy = 0.0f, gz = 0.0f;
+  else if constexpr (K == 3) gx = A * y, gy = A * x, gz = 0.0f;
+  else if constexpr (K == 4) gx = 0.0f, gy = -A * z, gz = -A * y;
+  else if constexpr (K == 5) gx = -2.0f * B * x, gy = -2.0f * B * y, gz = 4.0f * B * z;
+  else if constexpr (K == 6) gx = -A * z, gy = 0.0f, gz = -A * x;
+  else if constexpr (K == 7) gx = 2.0f * C * x, gy = -2.0f * C * y, gz = 0.0f;
+  else if constexpr (K == 8) gx = D * 6.0f * x * y, gy = D * (3.0f * xx - 3.0f * yy), gz = 0.0f;
+  else if constexpr (K == 9) gx = E * y * z, gy = E * x * z, gz = E * x * y;
+  else if constexpr (K == 10) gx = F * (-2.0f * x * y), gy = F * (4.0f * zz - xx - 3.0f * yy), gz = F * 8.0f * y * z;
+  else if constexpr (K == 11)
+    gx = G * (-6.0f * x * z), gy = G * (-6.0f * y * z), gz = G * (6.0f * zz - 3.0f * xx - 3.0f * yy);
+  else if constexpr (K == 12) gx = F * (4.0f * zz - 3.0f * xx - yy), gy = F * (-2.0f * x * y), gz = F * 8.0f * x * z;
+  else if constexpr (K == 13) gx = H * 2.0f * x * z, gy = H * (-2.0f * y * z), gz = H * (xx - yy);
+  else gx = D * (3.0f * xx - 3.0f * yy), gy = D * (-6.0f * x * y), gz = 0.0f;
 }
 
 // The projection adjoint of one view (train_step.glsl:102-219): 2D splat
 // gradient g[9] -> dL/d(position, logit, log scale, quaternion, DC) added to
-// the accumulators, and the view's SH colour gradient and direction.  kPose:
-// also the slot's share of the camera pose gradient, pose[6] = dL/dd, dL/dw
-// (train_step.glsl:196-201).
+// the accumulators; the view direction's part of dL/dposition (colour through
+// the SH rest) comes from k_adam_sh.  kPose: also the slot's share of the
+// camera pose gradient, pose[6] = dL/dd, dL/dw (train_step.glsl:196-201).
 template <bool kPose = false>
 __device__ __forceinline__ void adjoint_view(const gs_hip_camera& cam, const float g[9], uint32_t clamped, float4 p0,
-                                             float4 p1, float4 q, const float (&rest)[GS_HIP_MAX_SH_REST][3],
-                                             int coeffs, uint32_t degree, float4& dp0, float4& dp1, float4& dq,
-                                             float4& dp3, float dcol[3], float dir[3], float* pose = nullptr) {
+                                             float4 p1, float4 q, float4& dp0, float4& dp1, float4& dq, float4& dp3,
+                                             float* pose = nullptr) {
   const float W = float(cam.width), H = float(cam.height);
   const float* V = cam.rotation;
   const float tx = V[0] * p0.x + V[1] * p0.y + V[2] * p0.z + cam.translation[0];
@@ -172,20 +175,10 @@ __device__ __forceinline__ void adjoint_view(const gs_hip_camera& cam, const flo
   const float C = U[1][0] * T[1][0] + U[1][1] * T[1][1] + U[1][2] * T[1][2] + kDilation;
   const float D = A * C - B * B;
 
-  // Colour: clamped channels get no gradient; DC and the view direction.
-  const float drx = p0.x - cam.position[0], dry = p0.y - cam.position[1], drz = p0.z - cam.position[2];
-  const float dlen = fmaxf(sqrtf(drx * drx + dry * dry + drz * drz), 1e-12f);
-  dir[0] = drx / dlen, dir[1] = dry / dlen, dir[2] = drz / dlen;
-  dcol[0] = (clamped & 1u) ? 0.0f : g[6];
-  dcol[1] = (clamped & 2u) ? 0.0f : g[7];
-  dcol[2] = (clamped & 4u) ? 0.0f : g[8];
+  // Colour: clamped channels get no gradient; DC.
   const float b0 = 0.28209479177387814f;
-  add4(dp3, b0 * dcol[0], b0 * dcol[1], b0 * dcol[2], 0.0f);
-  float ddir[3] = {0.0f, 0.0f, 0.0f};
-  if (coeffs > 0) sh_direction_gradient(dir[0], dir[1], dir[2], degree, rest, dcol, ddir);
-  const float dd = dir[0] * ddir[0] + dir[1] * ddir[1] + dir[2] * ddir[2];
-  const float dpos_dir[3] = {(ddir[0] - dir[0] * dd) / dlen, (ddir[1] - dir[1] * dd) / dlen,
-                             (ddir[2] - dir[2] * dd) / dlen};
+  add4(dp3, (clamped & 1u) ? 0.0f : b0 * g[6], (clamped & 2u) ? 0.0f : b0 * g[7], (clamped & 4u) ? 0.0f : b0 * g[8],
+       0.0f);
 
   // Opacity logit.
   const float op = 1.0f / (1.0f + __expf(-p0.w));
@@ -247,8 +240,8 @@ __device__ __forceinline__ void adjoint_view(const gs_hip_camera& cam, const flo
     pose[4] = tz * dtx - tx * dtz + (GW(0, 2) - GW(2, 0));
     pose[5] = tx * dty - ty * dtx + (GW(1, 0) - GW(0, 1));
   }
-  add4(dp0, V[0] * dtx + V[3] * dty + V[6] * dtz + dpos_dir[0], V[1] * dtx + V[4] * dty + V[7] * dtz + dpos_dir[1],
-       V[2] * dtx + V[5] * dty + V[8] * dtz + dpos_dir[2], dlogit);
+  add4(dp0, V[0] * dtx + V[3] * dty + V[6] * dtz, V[1] * dtx + V[4] * dty + V[7] * dtz,
+       V[2] * dtx + V[5] * dty + V[8] * dtz, dlogit);
 
   float dM[3][3];
   for (int r = 0; r < 3; ++r)
@@ -274,46 +267,27 @@ __device__ __forceinline__ void adjoint_view(const gs_hip_camera& cam, const flo
 
 // Adjoint of every view in `a.views`, summed and averaged, then one Adam step
 // on the 16 parameters; densify statistics count every view in which the
-// gaussian was visible.  Each view's colour gradient and direction go to its
-// sh_input for k_adam_sh (one thread per SH coefficient, fully coalesced on
-// the coefficient planes, and few registers here).
+// gaussian was visible.  The view-direction part of dL/dposition was summed
+// over the views by k_adam_sh (one plane per group of SH rest coefficients).
 __global__ void __launch_bounds__(256) k_adam(const gsh::SceneDev* __restrict__ scenes,
                                               const gsh::ViewDev* __restrict__ views, uint32_t count) {
   const AdamArgs a = scene_args(scenes[blockIdx.y], views, count);
   const gs_hip_frame& f = a.views[0].f;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= f.slots || skip_update(a)) return;
-  const uint32_t degree = f.sh_degree;
-  const int coeffs = min(sh_rest_coeffs(degree), int(a.sh_coeffs));
+  const int coeffs = min(sh_rest_coeffs(f.sh_degree), int(a.sh_coeffs));
   float4 p0 = a.params[i * 4u], p1 = a.params[i * 4u + 1u], q = a.params[i * 4u + 2u], p3 = a.params[i * 4u + 3u];
-  if (p1.w < 0.5f) {  // free slot
-    if (coeffs > 0) a.views[0].sh_input[size_t(i) * 2u] = make_float4(0, 0, 0, 0);
-    return;
-  }
-  // The slot's SH rest coefficients, once for every view's direction gradient.
-  float rest[GS_HIP_MAX_SH_REST][3];
-#pragma unroll
-  for (int k = 0; k < GS_HIP_MAX_SH_REST; ++k) {
-    rest[k][0] = rest[k][1] = rest[k][2] = 0.0f;
-    if (k < coeffs) {
-      const float* sk = a.sh + (size_t(k) * a.capacity + i) * 3u;
-      rest[k][0] = sk[0];
-      rest[k][1] = sk[1];
-      rest[k][2] = sk[2];
-    }
-  }
+  if (p1.w < 0.5f) return;  // free slot
   float4 dp0 = make_float4(0, 0, 0, 0), dp1 = dp0, dq = dp0, dp3 = dp0;
+  for (int k0 = 0; k0 < coeffs; k0 += kShGroup) {
+    const float4 d = a.dir_grad[size_t(k0 / kShGroup) * a.capacity + i];
+    add4(dp0, d.x, d.y, d.z, 0.0f);
+  }
   bool drew = false, seen = false;
   float4 st = a.stats[i];
   for (uint32_t v = 0; v < a.count; ++v) {
     const gsh::ViewDev& view = a.views[v];
-    if (view.tiles[i] == 0u) {
-      if (coeffs > 0) {
-        view.sh_input[size_t(i) * 2u] = make_float4(0, 0, 0, 0);
-        view.sh_input[size_t(i) * 2u + 1u] = make_float4(0, 0, 0, 0);
-      }
-      continue;
-    }
+    if (view.tiles[i] == 0u) continue;
     seen = true;
     float g[9];
     for (int c = 0; c < 9; ++c) {
@@ -326,17 +300,10 @@ __global__ void __launch_bounds__(256) k_adam(const gsh::SceneDev* __restrict__ 
     st.x += sqrtf(gx * gx + gy * gy);
     st.y += 1.0f;
     st.z = fmaxf(st.z, float(misc >> 3));
-    float dcol[3], dir[3];
-    adjoint_view(view.f.cam, g, misc & 7u, p0, p1, q, rest, coeffs, degree, dp0, dp1, dq, dp3, dcol, dir);
-    if (coeffs > 0) {
-      view.sh_input[size_t(i) * 2u] = make_float4(dcol[0], dcol[1], dcol[2], 0.0f);
-      view.sh_input[size_t(i) * 2u + 1u] = make_float4(dir[0], dir[1], dir[2], 0.0f);
-    }
+    adjoint_view(view.f.cam, g, misc & 7u, p0, p1, q, dp0, dp1, dq, dp3);
   }
   if (seen) a.stats[i] = st;
-  const bool update = drew || (f.flags & GS_HIP_FLAG_SPARSE_ADAM) == 0u;
-  if (coeffs > 0) a.views[0].sh_input[size_t(i) * 2u].w = update ? 1.0f : 0.0f;
-  if (!update) return;
+  if (!drew && (f.flags & GS_HIP_FLAG_SPARSE_ADAM) != 0u) return;
   const float inv = 1.0f / float(a.count);  // the mean over the views
   const float4 lr0 = make_float4(f.lr_position, f.lr_position, f.lr_position, f.lr_opacity);
   const float4 lr1 = make_float4(f.lr_scale, f.lr_scale, f.lr_scale, 0.0f);
@@ -375,7 +342,6 @@ __global__ void __launch_bounds__(256) k_pose_grad(const gsh::SceneDev* __restri
     live = p1.w >= 0.5f;  // not a free slot
   }
   __shared__ float s_part[6][4];
-  const float rest[GS_HIP_MAX_SH_REST][3] = {};
   const int wave = threadIdx.x >> 6;
   for (uint32_t v = 0; v < a.count; ++v) {
     const gsh::ViewDev& view = a.views[v];
@@ -385,8 +351,7 @@ __global__ void __launch_bounds__(256) k_pose_grad(const gsh::SceneDev* __restri
       float g[9];
       for (int c = 0; c < 9; ++c) g[c] = view.grad2d[size_t(c) * a.capacity + i];
       float4 dp0 = make_float4(0, 0, 0, 0), dp1 = dp0, dq = dp0, dp3 = dp0;
-      float dcol[3], dir[3];
-      adjoint_view<true>(view.f.cam, g, 0u, p0, p1, q, rest, 0, 0u, dp0, dp1, dq, dp3, dcol, dir, pose);
+      adjoint_view<true>(view.f.cam, g, 0u, p0, p1, q, dp0, dp1, dq, dp3, pose);
     }
     for (int k = 0; k < 6; ++k) {
       const float w = wave_sum(pose[k]);
@@ -405,74 +370,121 @@ __global__ void __launch_bounds__(256) k_pose_grad(const gsh::SceneDev* __restri
   }
 }
 
-// Adam on coefficients [k0, k0 + kShGroup) of one slot: all loads of the
-// group are issued before any store (the planes are distinct arrays, which
-// the restrict parameters tell the compiler), so a wave has 3 x kShGroup
-// coalesced 768 B loads in flight per array.
-constexpr int kShGroup = 5;
-
-__device__ __forceinline__ void adam_sh_group(float* __restrict__ sh, float* __restrict__ m_sh,
-                                              float* __restrict__ v_sh, const float (&g)[GS_HIP_MAX_SH_REST][3],
-                                              int k0, int coeffs, uint32_t capacity, uint32_t i, float inv,
-                                              const gs_hip_frame& f) {
-  float p[kShGroup][3], m[kShGroup][3], v[kShGroup][3];
+// One group of kShGroup SH rest coefficients of one slot (train_step.glsl:
+// 129-155, train_step_sh.comp:21-44): for every view of the step that sees
+// the slot, the colour gradient dcol (clamped channels 0) against the basis
+// at the view direction, and the group's share of d(colour)/d(direction)
+// against dcol, carried to dL/dposition through the direction's
+// normalisation; the latter summed over the views into dir_grad[group] for
+// k_adam, then Adam on the group's coefficients at the views' mean gradient.
+template <int kGroup>
+__device__ __forceinline__ void sh_group(const AdamArgs& a, const gs_hip_frame& f, uint32_t i, int coeffs) {
+  constexpr int k0 = kGroup * kShGroup;
+  const float4 p0 = a.params[i * 4u];
+  float rest[kShGroup][3], g[kShGroup][3];
+#pragma unroll
+  for (int d = 0; d < kShGroup; ++d) {
+    g[d][0] = g[d][1] = g[d][2] = 0.0f;
+    rest[d][0] = rest[d][1] = rest[d][2] = 0.0f;
+    if (k0 + d < coeffs) {
+      const float* sk = a.sh + (size_t(k0 + d) * a.capacity + i) * 3u;
+      rest[d][0] = sk[0];
+      rest[d][1] = sk[1];
+      rest[d][2] = sk[2];
+    }
+  }
+  const bool sparse = (f.flags & GS_HIP_FLAG_SPARSE_ADAM) != 0u;
+  bool drew = false;
+  float dpx = 0.0f, dpy = 0.0f, dpz = 0.0f;
+  // Every view's colour gradient and clamp bits are loaded before any is
+  // used (GRAD2D is zero where the view does not see the slot: the
+  // projection clears it, so no tile-count test is needed).
+  float4 gv[GS_HIP_MAX_VIEWS];
+#pragma unroll
+  for (uint32_t v = 0; v < GS_HIP_MAX_VIEWS; ++v) {
+    if (v >= a.count) break;
+    const gsh::ViewDev& view = a.views[v];
+    gv[v] = make_float4(view.grad2d[size_t(6) * a.capacity + i], view.grad2d[size_t(7) * a.capacity + i],
+                        view.grad2d[size_t(8) * a.capacity + i], __uint_as_float(view.misc[i]));
+  }
+#pragma unroll
+  for (uint32_t v = 0; v < GS_HIP_MAX_VIEWS; ++v) {
+    if (v >= a.count) break;
+    const gsh::ViewDev& view = a.views[v];
+    const float4 gc = gv[v];
+    drew = drew || gc.x != 0.0f || gc.y != 0.0f || gc.z != 0.0f;
+    if (sparse && !drew) {
+      for (int c = 0; c < 6; ++c) drew = drew || view.grad2d[size_t(c) * a.capacity + i] != 0.0f;
+    }
+    const uint32_t clamped = __float_as_uint(gc.w) & 7u;
+    const float dcol[3] = {clamped & 1u ? 0.0f : gc.x, clamped & 2u ? 0.0f : gc.y, clamped & 4u ? 0.0f : gc.z};
+    if (dcol[0] == 0.0f && dcol[1] == 0.0f && dcol[2] == 0.0f) continue;
+    const float drx = p0.x - view.f.cam.position[0], dry = p0.y - view.f.cam.position[1];
+    const float drz = p0.z - view.f.cam.position[2];
+    const float dlen = fmaxf(sqrtf(drx * drx + dry * dry + drz * drz), 1e-12f);
+    const float x = drx / dlen, y = dry / dlen, z = drz / dlen;
+    float basis[16];
+    sh_basis(x, y, z, f.sh_degree, basis);
+    float ddx = 0.0f, ddy = 0.0f, ddz = 0.0f;
+    auto term = [&](auto kc) {
+      constexpr int d = decltype(kc)::value;
+      if (k0 + d >= coeffs) return;
+      for (int c = 0; c < 3; ++c) g[d][c] += basis[k0 + d + 1] * dcol[c];
+      const float w = rest[d][0] * dcol[0] + rest[d][1] * dcol[1] + rest[d][2] * dcol[2];
+      float gx, gy, gz;
+      sh_rest_dbasis<k0 + d>(x, y, z, gx, gy, gz);
+      ddx += w * gx;
+      ddy += w * gy;
+      ddz += w * gz;
+    };
+    term(std::integral_constant<int, 0>{});
+    term(std::integral_constant<int, 1>{});
+    term(std::integral_constant<int, 2>{});
+    term(std::integral_constant<int, 3>{});
+    term(std::integral_constant<int, 4>{});
+    const float dd = x * ddx + y * ddy + z * ddz;
+    dpx += (ddx - x * dd) / dlen;
+    dpy += (ddy - y * dd) / dlen;
+    dpz += (ddz - z * dd) / dlen;
+  }
+  a.dir_grad[size_t(kGroup) * a.capacity + i] = make_float4(dpx, dpy, dpz, 0.0f);
+  if (sparse && !drew) return;  // kept by sparse Adam
+  const float inv = 1.0f / float(a.count);
+  float m[kShGroup][3], vv[kShGroup][3];
   for (int d = 0; d < kShGroup; ++d) {
     if (k0 + d >= coeffs) break;
-    const size_t o = (size_t(k0 + d) * capacity + i) * 3u;
-    for (int c = 0; c < 3; ++c) p[d][c] = sh[o + c], m[d][c] = m_sh[o + c], v[d][c] = v_sh[o + c];
+    const size_t o = (size_t(k0 + d) * a.capacity + i) * 3u;
+    for (int c = 0; c < 3; ++c) m[d][c] = a.m_sh[o + c], vv[d][c] = a.v_sh[o + c];
   }
   for (int d = 0; d < kShGroup; ++d) {
     if (k0 + d >= coeffs) break;
-    const size_t o = (size_t(k0 + d) * capacity + i) * 3u;
+    const size_t o = (size_t(k0 + d) * a.capacity + i) * 3u;
     for (int c = 0; c < 3; ++c) {
-      sh[o + c] = adam1(p[d][c], g[k0 + d][c] * inv, m[d][c], v[d][c], f.lr_sh_rest, f);
-      m_sh[o + c] = m[d][c];
-      v_sh[o + c] = v[d][c];
+      a.sh[o + c] = adam1(rest[d][c], g[d][c] * inv, m[d][c], vv[d][c], f.lr_sh_rest, f);
+      a.m_sh[o + c] = m[d][c];
+      a.v_sh[o + c] = vv[d][c];
     }
   }
 }
 
-// Adam on the SH coefficients beyond DC (train_step_sh.comp:21-44), one
-// thread per slot: g_k = mean over the views of basis_k(dir) dL/dcolour,
-// with each view's (dL/dcolour, dir) read once and the basis evaluated once
-// per view; then Adam on each coefficient plane (coalesced across the wave).
-__global__ void __launch_bounds__(256) k_adam_sh(const gsh::SceneDev* __restrict__ scenes,
-                                                 const gsh::ViewDev* __restrict__ views, uint32_t count) {
+// A block is the three groups of 64 slots, one wave each: every wave reads
+// its coefficient planes coalesced, and the views' per-slot gradients, clamp
+// bits and tile counts that all three read come from the CU's cache after the
+// first.
+constexpr int kShBlock = 64 * kShGroups;
+
+__global__ void __launch_bounds__(kShBlock) k_adam_sh(const gsh::SceneDev* __restrict__ scenes,
+                                                      const gsh::ViewDev* __restrict__ views, uint32_t count) {
   const AdamArgs a = scene_args(scenes[blockIdx.y], views, count);
   const gs_hip_frame& f = a.views[0].f;
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a.sh_coeffs == 0u || i >= f.slots || skip_update(a)) return;
-  if (a.views[0].sh_input[size_t(i) * 2u].w == 0.0f) return;  // free slot, or kept by sparse Adam
+  const int group = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+  const uint32_t i = blockIdx.x * 64u + (threadIdx.x & 63u);
   const int coeffs = min(sh_rest_coeffs(f.sh_degree), int(a.sh_coeffs));
-  float g[GS_HIP_MAX_SH_REST][3];
-  for (int k = 0; k < GS_HIP_MAX_SH_REST; ++k) g[k][0] = g[k][1] = g[k][2] = 0.0f;
-  // Every view's (dL/dcolour, direction) is loaded before any is used.
-  float4 cv[GS_HIP_MAX_VIEWS], dv[GS_HIP_MAX_VIEWS];
-#pragma unroll
-  for (uint32_t v = 0; v < GS_HIP_MAX_VIEWS; ++v) {
-    if (v >= a.count) break;
-    cv[v] = a.views[v].sh_input[size_t(i) * 2u];
-    dv[v] = a.views[v].sh_input[size_t(i) * 2u + 1u];
-  }
-#pragma unroll
-  for (uint32_t v = 0; v < GS_HIP_MAX_VIEWS; ++v) {
-    if (v >= a.count) break;
-    const float4 c = cv[v], d = dv[v];
-    if (c.x == 0.0f && c.y == 0.0f && c.z == 0.0f) continue;
-    float basis[16];
-    sh_basis(d.x, d.y, d.z, f.sh_degree, basis);
-    for (int k = 0; k < GS_HIP_MAX_SH_REST; ++k) {
-      g[k][0] += basis[k + 1] * c.x;
-      g[k][1] += basis[k + 1] * c.y;
-      g[k][2] += basis[k + 1] * c.z;
-    }
-  }
-  const float inv = 1.0f / float(a.count);
-#pragma unroll
-  for (int k0 = 0; k0 < GS_HIP_MAX_SH_REST; k0 += kShGroup) {
-    if (k0 >= coeffs) break;
-    adam_sh_group(a.sh, a.m_sh, a.v_sh, g, k0, coeffs, a.capacity, i, inv, f);
-  }
+  if (group * kShGroup >= coeffs || i >= f.slots || skip_update(a)) return;
+  if (a.params[i * 4u + 1u].w < 0.5f) return;  // free slot
+  if (group == 0) sh_group<0>(a, f, i, coeffs);
+  else if (group == 1) sh_group<1>(a, f, i, coeffs);
+  else sh_group<2>(a, f, i, coeffs);
 }
 
 // ---- densification ----------------------------------------------------------
@@ -642,12 +654,13 @@ int launch_adam(gs_hip_workspace* ws, uint32_t count) {
   if (ws->shape.max_capacity == 0) return 0;
   const dim3 grid(div_up(ws->shape.max_capacity, 256), ws->shape.scenes);
   timer_begin(ws, "adam");
-  k_adam<<<grid, 256, 0, ws->stream>>>(ws->dev_scenes, ws->dev_views, count);
-  GS_CHECK_LAUNCH("k_adam");
-  if (ws->shape.any_sh) {
-    k_adam_sh<<<grid, 256, 0, ws->stream>>>(ws->dev_scenes, ws->dev_views, count);
+  if (ws->shape.any_sh) {  // first: it reads the positions k_adam updates
+    k_adam_sh<<<dim3(div_up(ws->shape.max_capacity, 64), grid.y), kShBlock, 0, ws->stream>>>(ws->dev_scenes,
+                                                                                          ws->dev_views, count);
     GS_CHECK_LAUNCH("k_adam_sh");
   }
+  k_adam<<<grid, 256, 0, ws->stream>>>(ws->dev_scenes, ws->dev_views, count);
+  GS_CHECK_LAUNCH("k_adam");
   timer_end(ws);
   return 0;
 }
