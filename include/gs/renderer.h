@@ -20,6 +20,10 @@ namespace gs {
 class GpuScene {
  public:
   GpuScene(Device& device, const SplatScene& scene, uint32_t width, uint32_t height);
+  // An SPZ file decoded on the device (gs_hip_workspace_load_spz): the same
+  // scene as GpuScene(device, load_spz(data, size), ...), without the host
+  // unpacking.
+  GpuScene(Device& device, const uint8_t* spz, size_t size, uint32_t width, uint32_t height);
   ~GpuScene();
   GpuScene(const GpuScene&) = delete;
   GpuScene& operator=(const GpuScene&) = delete;
@@ -38,6 +42,9 @@ class GpuScene {
 };
 
 std::unique_ptr<GpuScene> upload_scene(Device& device, const SplatScene& scene, uint32_t width, uint32_t height);
+// A .ply or .spz file's scene on the device: SPZ decoded there, PLY through load_ply.
+std::unique_ptr<GpuScene> upload_splat_file(Device& device, const uint8_t* data, size_t size, uint32_t width,
+                                            uint32_t height);
 
 class SplatRenderer {
  public:
@@ -49,7 +56,7 @@ class SplatRenderer {
   uint32_t visible_count() const { return visible_; }
 
  private:
-  Device& device_;
+  [[maybe_unused]] Device& device_;
   GpuScene* scene_ = nullptr;
   uint32_t visible_ = 0;
 };

@@ -50,6 +50,8 @@ SplatScene load_spz(const uint8_t* data, size_t size);
 // Detects PLY or SPZ.
 SplatScene load_splat_file(const uint8_t* data, size_t size);
 std::vector<uint8_t> save_ply(const SplatScene& scene);
+// The header save_ply writes for `count` vertices of SH degree `sh_degree`.
+std::string ply_header(uint32_t count, int sh_degree);
 // RDF <-> RUB (a half turn about x): positions, rotations and SH signs.
 void convert_convention(SplatScene& scene, AxisConvention target);
 // A torus knot of anisotropic splats with degree-1 SH, in RUB convention.

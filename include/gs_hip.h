@@ -314,6 +314,27 @@ int gs_hip_batch_set_tuning(gs_hip_batch* batch, const gs_hip_tuning* tuning);
 /* Kernel timing of the last batch steps (as gs_hip_workspace_timings). */
 int gs_hip_batch_timings(gs_hip_batch* batch, int enable, const char** names, double* ms, int capacity, int* count);
 
+/* ---- 6. scene files on the device ------------------------------------------- */
+/* SPZ v2/v3 (the reference's load_spz, engine/src/splat_scene.cpp:367-448): the
+ * gzip stream is inflated on the host, the fixed-point attributes are unpacked
+ * on the device into slots [0, n) of the workspace (params, SH planes; moments
+ * and statistics zeroed), every value equal to gs::load_spz's.  The file's
+ * convention (RUB) is kept.  Requires capacity >= n; SH rest beyond the
+ * workspace's degree is dropped, missing coefficients are zero.  gs_hip_spz_info
+ * reads only the header (count, SH degree <= 3, antialiased flag), to size the
+ * workspace first.  SPZ v4 (zstd) is rejected. */
+int gs_hip_spz_info(const uint8_t* spz, size_t bytes, uint32_t* count, int* sh_degree, int* antialiased);
+int gs_hip_workspace_load_spz(gs_hip_workspace* ws, const uint8_t* spz, size_t bytes, uint32_t* count,
+                              int* sh_degree, int* antialiased);
+/* The alive slots among [0, slots) as 3DGS PLY vertex rows, in slot order (the
+ * reference's export_scene -> convert_convention -> save_ply,
+ * trainer.cpp:1773-1800, splat_scene.cpp:336-365): per row 17 + 3 K floats, x y z,
+ * nx ny nz = 0, f_dc_0..2, f_rest channel-major (K = the workspace's SH rest
+ * coefficients), opacity, scale_0..2, rot_0..3 (normalised); rdf != 0 applies the
+ * half turn about x (RUB -> RDF).  *count = alive slots; rows == NULL asks for the
+ * count only (rows needs count * (17 + 3 K) * 4 bytes). */
+int gs_hip_export_ply(gs_hip_workspace* ws, uint32_t slots, int rdf, uint8_t* rows, size_t bytes, uint32_t* count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -91,12 +91,21 @@ int main(int argc, char** argv) {
   }
   try {
     const std::string mode = argv[2];
-    gs::SplatScene scene = load_scene(argv[1]);
-    gs::Device device(0);
+    const std::string input = argv[1];
     if (mode == "--views") {
       if (argc < 5) throw std::runtime_error("--views needs <cameras.txt> <out_dir>");
       const std::vector<gs::Camera> cameras = read_cameras(argv[3]);
       const std::string out_dir = argv[4];
+      // A file goes to the device as it is (SPZ is unpacked there); synthetic scenes are built on the host.
+      const bool synthetic = input.rfind("synthetic", 0) == 0;
+      gs::SplatScene scene;
+      std::vector<uint8_t> bytes;
+      if (synthetic) {
+        scene = load_scene(input);
+      } else {
+        bytes = gs::read_file(input);
+      }
+      gs::Device device(0);
       // One device copy of the scene per output size.
       std::map<std::pair<uint32_t, uint32_t>, std::unique_ptr<gs::GpuScene>> scenes;
       gs::SplatRenderer renderer(device);
@@ -104,7 +113,14 @@ int main(int argc, char** argv) {
       for (size_t i = 0; i < cameras.size(); ++i) {
         const gs::Camera& c = cameras[i];
         auto& gpu = scenes[{c.width, c.height}];
-        if (!gpu) gpu = gs::upload_scene(device, scene, c.width, c.height);
+        if (!gpu) {
+          const auto tu = std::chrono::steady_clock::now();
+          gpu = synthetic ? gs::upload_scene(device, scene, c.width, c.height)
+                          : gs::upload_splat_file(device, bytes.data(), bytes.size(), c.width, c.height);
+          std::printf("scene on the device for %ux%u: %u splats, sh degree %d (%.0f ms)\n", c.width, c.height,
+                      gpu->count(), gpu->sh_degree(),
+                      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tu).count());
+        }
         renderer.attach(gpu.get());
         const std::vector<uint8_t> pixels = renderer.render(c, kClear);
         char name[32];
@@ -117,6 +133,8 @@ int main(int argc, char** argv) {
                   std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
       return 0;
     }
+    gs::SplatScene scene = load_scene(input);
+    gs::Device device(0);
     auto positional = [&](int i) {
       return argc > i && (argv[i][0] != '-' || std::isdigit(static_cast<unsigned char>(argv[i][1])));
     };

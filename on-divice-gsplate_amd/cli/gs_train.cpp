@@ -300,10 +300,11 @@ int main(int argc, char** argv) {
                 st.step, st.optimizer_steps, secs, st.step / std::max(secs, 1e-9), st.densify_events,
                 st.overflow_steps);
     if (!a.pose_out.empty()) write_poses(trainer, a.pose_out);
-    gs::SplatScene result = trainer.export_scene();
-    gs::convert_convention(result, gs::AxisConvention::RDF);
-    gs::write_file(a.out, gs::save_ply(result));
-    std::printf("wrote %s: %u gaussians\n", a.out.c_str(), result.count);
+    const auto t_export = clock::now();
+    const std::vector<uint8_t> ply = trainer.export_ply();  // rows written on the device
+    gs::write_file(a.out, ply);
+    std::printf("wrote %s: %u gaussians (%.0f ms)\n", a.out.c_str(), trainer.stats().gaussians,
+                std::chrono::duration<double, std::milli>(clock::now() - t_export).count());
     return 0;
   } catch (const std::exception& e) {
     std::fprintf(stderr, "gs_train: %s\n", e.what());

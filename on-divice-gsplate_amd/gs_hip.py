@@ -44,6 +44,7 @@ EXPORTED = [
     "gs_hip_workspace_overflows", "gs_hip_workspace_timings", "gs_hip_view_render",
     "gs_hip_batch_create", "gs_hip_batch_destroy", "gs_hip_batch_train", "gs_hip_batch_get_tuning",
     "gs_hip_batch_set_tuning", "gs_hip_batch_timings", "gs_hip_pose_gradient", "gs_hip_exposure_sums",
+    "gs_hip_spz_info", "gs_hip_workspace_load_spz", "gs_hip_export_ply",
 ]
 
 
@@ -372,6 +373,24 @@ class Workspace:
         check(self.lib.gs_hip_exposure_sums(self.handle, view, out))
         return np.array(out[:], np.float64)
 
+    def load_spz(self, data: bytes):
+        """Decodes an SPZ file into slots [0, n) on the device; returns (n, SH degree, antialiased)."""
+        n, deg, aa = ctypes.c_uint32(), ctypes.c_int(), ctypes.c_int()
+        buf = (ctypes.c_uint8 * len(data)).from_buffer_copy(data)
+        check(self.lib.gs_hip_workspace_load_spz(self.handle, buf, len(data), ctypes.byref(n), ctypes.byref(deg),
+                                                 ctypes.byref(aa)))
+        return n.value, deg.value, bool(aa.value)
+
+    def export_ply_rows(self, slots: int, rdf: bool = True) -> np.ndarray:
+        """The alive slots' 3DGS PLY vertex rows, float32 [alive, 17 + 3 K]."""
+        count = ctypes.c_uint32()
+        check(self.lib.gs_hip_export_ply(self.handle, slots, int(rdf), None, 0, ctypes.byref(count)))
+        per = 17 + 3 * self.sh_coeffs
+        out = np.empty((count.value, per), np.float32)
+        check(self.lib.gs_hip_export_ply(self.handle, slots, int(rdf), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                         out.nbytes, ctypes.byref(count)))
+        return out
+
     def render(self, frame: Frame) -> np.ndarray:
         out = np.empty((self.height, self.width, 4), np.uint8)
         check(self.lib.gs_hip_render(self.handle, ctypes.byref(frame),
@@ -498,6 +517,14 @@ class Batch:
         count = ctypes.c_int()
         check(self.lib.gs_hip_batch_timings(self.handle, int(enable), names, ms, cap, ctypes.byref(count)))
         return [(names[i].decode(), ms[i]) for i in range(count.value)]
+
+
+def spz_info(data: bytes):
+    """(count, SH degree, antialiased) from an SPZ file's header."""
+    n, deg, aa = ctypes.c_uint32(), ctypes.c_int(), ctypes.c_int()
+    buf = (ctypes.c_uint8 * len(data)).from_buffer_copy(data)
+    check(library().gs_hip_spz_info(buf, len(data), ctypes.byref(n), ctypes.byref(deg), ctypes.byref(aa)))
+    return n.value, deg.value, bool(aa.value)
 
 
 def synchronize() -> None:

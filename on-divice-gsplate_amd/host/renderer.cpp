@@ -58,12 +58,35 @@ GpuScene::GpuScene(Device& device, const SplatScene& scene, uint32_t width, uint
   }
 }
 
+GpuScene::GpuScene(Device& device, const uint8_t* spz, size_t size, uint32_t width, uint32_t height)
+    : width_(width), height_(height) {
+  (void)device;
+  int degree = 0, aa = 0;
+  check_status(gs_hip_spz_info(spz, size, &count_, &degree, &aa));
+  check_status(gs_hip_workspace_create(&ws_, std::max<uint32_t>(count_, 1), degree, width, height));
+  // SPZ's SH rest values are k / 128 with |k| <= 128: already fp16 values, as the viewer stores them.
+  const int rc = gs_hip_workspace_load_spz(ws_, spz, size, &count_, &sh_degree_, &aa);
+  if (rc != 0) {
+    char msg[512];
+    gs_hip_last_error(msg, sizeof(msg));
+    gs_hip_workspace_destroy(ws_);
+    throw std::runtime_error(msg);
+  }
+  antialiased_ = aa != 0;
+}
+
 GpuScene::~GpuScene() {
   if (ws_ != nullptr) gs_hip_workspace_destroy(ws_);
 }
 
 std::unique_ptr<GpuScene> upload_scene(Device& device, const SplatScene& scene, uint32_t width, uint32_t height) {
   return std::make_unique<GpuScene>(device, scene, width, height);
+}
+
+std::unique_ptr<GpuScene> upload_splat_file(Device& device, const uint8_t* data, size_t size, uint32_t width,
+                                            uint32_t height) {
+  if (size >= 2 && data[0] == 0x1f && data[1] == 0x8b) return std::make_unique<GpuScene>(device, data, size, width, height);
+  return std::make_unique<GpuScene>(device, load_splat_file(data, size), width, height);
 }
 
 std::vector<uint8_t> SplatRenderer::render(const Camera& camera, const float clear[3]) {

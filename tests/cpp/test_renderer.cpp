@@ -2,19 +2,25 @@
 // (engine/tests/test_splats.cpp:188-296): a single splat's falloff against
 // the closed form opacity exp(-d^2 / 2 (sigma^2 + 0.3)), depth order, the
 // Y-flip equivalence with SH parity signs, and a 20000-splat synthetic
-// scene's visible count and coverage.
+// scene's visible count and coverage.  Also SPZ files decoded on the device
+// (v2 and v3, SH degrees 0-4) against the host decoder, value for value and
+// image for image.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <random>
 #include <stdexcept>
 #include <vector>
+
+#include <zlib.h>
 
 #include "gs/camera.h"
 #include "gs/device.h"
 #include "gs/renderer.h"
 #include "gs/splat_scene.h"
+#include "gs_hip.h"
 #include "test_util.h"
 
 namespace {
@@ -180,6 +186,88 @@ void test_size_check(gs::Device& dev) {
   EXPECT(threw, "a camera of another size should be rejected");
 }
 
+std::vector<uint8_t> gzip(const std::vector<uint8_t>& data) {
+  z_stream zs{};
+  deflateInit2(&zs, Z_DEFAULT_COMPRESSION, Z_DEFLATED, 16 + MAX_WBITS, 8, Z_DEFAULT_STRATEGY);
+  std::vector<uint8_t> out(deflateBound(&zs, data.size()) + 64);
+  zs.next_in = const_cast<Bytef*>(data.data());
+  zs.avail_in = uInt(data.size());
+  zs.next_out = out.data();
+  zs.avail_out = uInt(out.size());
+  deflate(&zs, Z_FINISH);
+  out.resize(zs.total_out);
+  deflateEnd(&zs);
+  return out;
+}
+
+// An SPZ file of n splats with random attribute bytes (positions within a
+// few units at 12 fractional bits, every byte value of alpha, colour, scale,
+// rotation and SH reachable).
+std::vector<uint8_t> random_spz(uint32_t n, int degree, uint32_t version, uint32_t seed) {
+  const int coeffs = degree <= 0 ? 0 : (degree + 1) * (degree + 1) - 1;
+  const size_t rot = version >= 3 ? 4 : 3;
+  std::vector<uint8_t> raw(16 + size_t(n) * (9 + 1 + 3 + 3 + rot + size_t(coeffs) * 3));
+  const uint32_t magic = 0x5053474eu;
+  std::memcpy(&raw[0], &magic, 4);
+  std::memcpy(&raw[4], &version, 4);
+  std::memcpy(&raw[8], &n, 4);
+  raw[12] = uint8_t(degree);
+  raw[13] = 12;
+  raw[14] = uint8_t(seed & 1u);  // antialiased flag
+  std::mt19937 rng(seed);
+  std::uniform_int_distribution<int> byte(0, 255), pos(-3 * 4096, 3 * 4096);
+  size_t o = 16;
+  for (uint32_t i = 0; i < n * 3; ++i, o += 3) {
+    const int32_t v = pos(rng);
+    raw[o] = uint8_t(v), raw[o + 1] = uint8_t(v >> 8), raw[o + 2] = uint8_t(v >> 16);
+  }
+  for (; o < raw.size(); ++o) raw[o] = uint8_t(byte(rng));
+  return gzip(raw);
+}
+
+void test_spz_on_device(gs::Device& dev) {
+  for (uint32_t version : {2u, 3u}) {
+    for (int degree : {0, 1, 3, 4}) {
+      const uint32_t n = 3000;
+      const std::vector<uint8_t> file = random_spz(n, degree, version, 100 + version * 10 + uint32_t(degree));
+      const gs::SplatScene host = gs::load_spz(file.data(), file.size());
+      gs::GpuScene gpu(dev, file.data(), file.size(), 128, 96);
+      EXPECT(gpu.count() == n && gpu.sh_degree() == std::min(degree, 3) && gpu.antialiased() == host.antialiased,
+             "spz header on the device");
+      const int coeffs = gs::sh_rest_coeffs(gpu.sh_degree());
+      std::vector<float> p(size_t(n) * 16), rest(size_t(n) * std::max(coeffs, 1) * 3);
+      gs::check_status(gs_hip_workspace_store(gpu.workspace(), p.data(), coeffs > 0 ? rest.data() : nullptr, n));
+      size_t bad = 0;
+      for (uint32_t i = 0; i < n; ++i) {
+        const float* d = &p[size_t(i) * 16];
+        for (int k = 0; k < 3; ++k) {
+          bad += std::memcmp(&d[k], &host.positions[i * 3 + k], 4) != 0;
+          bad += std::memcmp(&d[4 + k], &host.log_scales[i * 3 + k], 4) != 0;
+          bad += std::memcmp(&d[12 + k], &host.sh_dc[i * 3 + k], 4) != 0;
+        }
+        bad += std::memcmp(&d[3], &host.opacity_logits[i], 4) != 0;
+        bad += d[7] != 1.0f;
+        for (int k = 0; k < 4; ++k) bad += std::memcmp(&d[8 + k], &host.rotations[i * 4 + k], 4) != 0;
+      }
+      for (size_t k = 0; k < host.sh_rest.size(); ++k) bad += std::memcmp(&rest[k], &host.sh_rest[k], 4) != 0;
+      EXPECT(bad == 0, "spz v%u degree %d: %zu values differ from the host decoder", version, degree, bad);
+      // and the same image as the host-decoded scene uploaded
+      gs::OrbitCamera orbit;
+      orbit.reset(gs::compute_bounds(host), false);
+      auto uploaded = gs::upload_scene(dev, host, 128, 96);
+      gs::SplatRenderer a(dev), b(dev);
+      a.attach(uploaded.get());
+      b.attach(&gpu);
+      const float clear[3] = {0.1f, 0.1f, 0.1f};
+      const auto ia = a.render(orbit.camera(128, 96), clear), ib = b.render(orbit.camera(128, 96), clear);
+      EXPECT(ia == ib && a.visible_count() == b.visible_count() && a.visible_count() > 100,
+             "spz v%u degree %d: device-decoded image differs (%u / %u visible)", version, degree,
+             a.visible_count(), b.visible_count());
+    }
+  }
+  std::printf("spz on the device: v2/v3, degrees 0-4, equal to the host decoder\n");
+}
+
 }  // namespace
 
 int main() {
@@ -192,6 +280,7 @@ int main() {
     test_flip_equivalence(dev);
     test_synthetic_render(dev);
     test_size_check(dev);
+    test_spz_on_device(dev);
   } catch (const std::exception& e) {
     std::fprintf(stderr, "exception: %s\n", e.what());
     return 2;

@@ -162,6 +162,35 @@ void test_densify_at_capacity(gs::Device& dev) {
   EXPECT(t.stats().densify_events >= 1, "densified");
 }
 
+// The PLY rows written on the device (Trainer::export_ply) equal the host
+// path, save_ply(export_scene() in RDF), byte for byte, with pruned (free)
+// slots among the alive ones (about half are pruned, none spawn) and SH
+// degree 2.
+void test_export_ply(gs::Device& dev) {
+  gs::TrainOptions o = frozen(2);
+  o.iterations = 4;
+  o.max_gaussians = 400;
+  o.densify_from = 0;
+  o.densify_until = 100;
+  o.densify_interval = 1;
+  o.densify_grad_threshold = 1e9f;
+  o.prune_opacity = 0.45f;
+  o.lr_position = 1e-3f;
+  o.lr_sh_rest = 1e-3f;
+  gs::Trainer t(dev, test_dataset(64, 48, 1, false, 23), test_scene(150, 2, 29), o);
+  const float bg[3] = {0, 0, 0};
+  t.step_on(0, bg);
+  t.step_on(0, bg);
+  gs::SplatScene host = t.export_scene();
+  gs::convert_convention(host, gs::AxisConvention::RDF);
+  const std::vector<uint8_t> want = gs::save_ply(host), got = t.export_ply();
+  EXPECT(t.stats().pruned > 0 && host.count < t.stats().slots, "free slots: %u alive of %u slots", host.count,
+         t.stats().slots);
+  EXPECT(got == want, "export_ply: %zu bytes vs save_ply %zu bytes", got.size(), want.size());
+  std::printf("  export_ply: %u gaussians of %u slots, %zu bytes, equal to save_ply\n", host.count, t.stats().slots,
+              got.size());
+}
+
 // views_per_step > 1: the batched step (mean gradient of the views, one launch
 // per stage), run for as many optimizer steps as the plain run, with the
 // launch shape autotuned on the first views.
@@ -418,6 +447,7 @@ int main() {
     test_heldout_alignment(dev);
     test_exposure_recovery(dev);
     test_densify_at_capacity(dev);
+    test_export_ply(dev);
     test_training_converges(dev, 1);
     test_training_converges(dev, 4);
   } catch (const std::exception& e) {
