@@ -35,22 +35,13 @@ struct LossArgs {
   float4* __restrict__ tile_loss;
 };
 
-// Render and target colour of pixel (x, y), channel c; zero outside the image.
-__device__ __forceinline__ void load_xy(const gs_hip_frame& f, const LossArgs& a, int x, int y, float& rx, float& ry,
-                                        float rgb_x[3], float rgb_y[3]) {
-  (void)rx;
-  (void)ry;
-  if (x < 0 || y < 0 || x >= int(f.cam.width) || y >= int(f.cam.height)) {
-    rgb_x[0] = rgb_x[1] = rgb_x[2] = 0.0f;
-    rgb_y[0] = rgb_y[1] = rgb_y[2] = 0.0f;
-    return;
-  }
-  const uint32_t id = uint32_t(y) * f.cam.width + uint32_t(x);
-  const float4 im = a.image[id];
+// Render x = C + T bg and target y of a pixel from its IMAGE texel and RGBA8
+// photo: the photo times exposure[] with GS_HIP_FLAG_EXPOSURE, over the
+// background by its alpha with GS_HIP_FLAG_MASKED_TARGET.
+__device__ __forceinline__ void xy_of(const gs_hip_frame& f, float4 im, uint32_t t, float rgb_x[3], float rgb_y[3]) {
   rgb_x[0] = im.x + im.w * f.background[0];
   rgb_x[1] = im.y + im.w * f.background[1];
   rgb_x[2] = im.z + im.w * f.background[2];
-  const uint32_t t = a.target[id];
   float tr = float(t & 0xffu) * (1.0f / 255.0f), tg = float((t >> 8) & 0xffu) * (1.0f / 255.0f);
   float tb = float((t >> 16) & 0xffu) * (1.0f / 255.0f);
   const float ta = float(t >> 24) * (1.0f / 255.0f);
@@ -66,6 +57,17 @@ __device__ __forceinline__ void load_xy(const gs_hip_frame& f, const LossArgs& a
   } else {
     rgb_y[0] = tr, rgb_y[1] = tg, rgb_y[2] = tb;
   }
+}
+
+// Halo elements per thread: the 26 x 26 halo over 256 threads.
+constexpr int kHaloIters = (kHalo * kHalo + 255) / 256;
+
+// Halo element e of the tile whose halo starts at (ox, oy): its pixel index,
+// or -1 outside the image (and past the halo).
+__device__ __forceinline__ int halo_pixel(const gs_hip_frame& f, int e, int ox, int oy) {
+  const int gx = ox + e % kHalo, gy = oy + e / kHalo;
+  const bool in = e < kHalo * kHalo && gx >= 0 && gy >= 0 && gx < int(f.cam.width) && gy < int(f.cam.height);
+  return in ? gy * int(f.cam.width) + gx : -1;
 }
 
 // Tile of block b among n: consecutive blocks go to the 8 XCDs in turn, so XCD x
@@ -143,13 +145,28 @@ __global__ void __launch_bounds__(256) k_ssim(const gsh::ViewDev* __restrict__ v
   const int px = tx * kTile + lx, py = ty * kTile + ly;
   const bool inside = px < int(f.cam.width) && py < int(f.cam.height);
   const uint32_t pixels = f.cam.width * f.cam.height;
-  for (int e = threadIdx.x; e < kHalo * kHalo; e += 256) {
-    float rx, ry, vx[3], vy[3];
-    load_xy(f, a, ox + e % kHalo, oy + e / kHalo, rx, ry, vx, vy);
-    const int at = (e / kHalo) * kStride + e % kHalo;
-    for (int c = 0; c < 3; ++c) {
-      s_x[c][at] = vx[c];
-      s_y[c][at] = vy[c];
+  {
+    // Every halo texel this thread stages is requested before any is used.
+    float4 im[kHaloIters];
+    uint32_t tg[kHaloIters];
+    int id[kHaloIters];
+#pragma unroll
+    for (int it = 0; it < kHaloIters; ++it) {
+      id[it] = halo_pixel(f, int(threadIdx.x) + it * 256, ox, oy);
+      im[it] = id[it] >= 0 ? a.image[id[it]] : make_float4(0, 0, 0, 0);
+      tg[it] = id[it] >= 0 ? a.target[id[it]] : 0u;
+    }
+#pragma unroll
+    for (int it = 0; it < kHaloIters; ++it) {
+      const int e = int(threadIdx.x) + it * 256;
+      if (e >= kHalo * kHalo) break;
+      float vx[3] = {0, 0, 0}, vy[3] = {0, 0, 0};
+      if (id[it] >= 0) xy_of(f, im[it], tg[it], vx, vy);
+      const int at = (e / kHalo) * kStride + e % kHalo;
+      for (int c = 0; c < 3; ++c) {
+        s_x[c][at] = vx[c];
+        s_y[c][at] = vy[c];
+      }
     }
   }
   __syncthreads();
@@ -239,8 +256,8 @@ __global__ void __launch_bounds__(256) k_loss_grad(const gsh::ViewDev* __restric
   const bool inside = px < int(f.cam.width) && py < int(f.cam.height);
   const uint32_t pixels = f.cam.width * f.cam.height;
   const uint32_t pix = uint32_t(py) * f.cam.width + uint32_t(px);
-  float rx, ry, x[3] = {0, 0, 0}, y[3] = {0, 0, 0};
-  if (inside) load_xy(f, a, px, py, rx, ry, x, y);
+  float x[3] = {0, 0, 0}, y[3] = {0, 0, 0};
+  if (inside) xy_of(f, a.image[pix], a.target[pix], x, y);
   const float scale = 1.0f / (3.0f * float(pixels));
   const float lambda = f.lambda_dssim;
   float grad[3];
@@ -250,13 +267,22 @@ __global__ void __launch_bounds__(256) k_loss_grad(const gsh::ViewDev* __restric
     grad[c] = sgn * ((1.0f - lambda) * scale);
   }
   if (lambda != 0.0f) {
+    int id[kHaloIters];
+#pragma unroll
+    for (int it = 0; it < kHaloIters; ++it) id[it] = halo_pixel(f, int(threadIdx.x) + it * 256, ox, oy);
     for (int c = 0; c < 3; ++c) {
-      for (int e = threadIdx.x; e < kHalo * kHalo; e += 256) {
-        const int gx = ox + e % kHalo, gy = oy + e / kHalo;
-        const bool in_image = gx >= 0 && gy >= 0 && gx < int(f.cam.width) && gy < int(f.cam.height);
-        const uint32_t id = uint32_t(gy) * f.cam.width + uint32_t(gx);
+      // The channel's nine halo values per thread, all requested before any is stored.
+      float mv[kHaloIters][3];
+#pragma unroll
+      for (int it = 0; it < kHaloIters; ++it) {
+        for (int q = 0; q < 3; ++q) mv[it][q] = id[it] >= 0 ? a.maps[size_t(3 * q + c) * pixels + id[it]] : 0.0f;
+      }
+#pragma unroll
+      for (int it = 0; it < kHaloIters; ++it) {
+        const int e = int(threadIdx.x) + it * 256;
+        if (e >= kHalo * kHalo) break;
         const int at = (e / kHalo) * kStride + e % kHalo;
-        for (int q = 0; q < 3; ++q) s_m[q][at] = in_image ? a.maps[size_t(3 * q + c) * pixels + id] : 0.0f;
+        for (int q = 0; q < 3; ++q) s_m[q][at] = mv[it][q];
       }
       __syncthreads();
       for (int e = threadIdx.x; e < kHalo * kTile; e += 256) {
