@@ -18,6 +18,39 @@
 
 namespace gsd {
 
+// A pointer read from a device table (ViewDev, SceneDev, a sort segment)
+// re-typed as global memory.  Accesses through the generic pointer compile
+// to flat instructions, which count against both the vector-memory and the
+// LDS/scalar wait counters, so the wait for the next table entry's scalar
+// load, or for an LDS read, also waits for every flat load, store or atomic
+// in flight; global instructions count only against the vector counter.
+#define GS_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ GS_GLOBAL T* gmem(T* p) {
+  return (GS_GLOBAL T*)p;
+}
+// Float add without a return value (the hardware atomic), through a global pointer.
+__device__ __forceinline__ void global_add(GS_GLOBAL float* p, float v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Vector loads and stores through the global view (HIP's vector types only
+// bind generic references, so these go through the native vector types).
+typedef float gs_f4v __attribute__((ext_vector_type(4)));
+typedef unsigned int gs_u2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float4 gload(const float4* p) {
+  const gs_f4v v = *(const GS_GLOBAL gs_f4v*)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint2 gload(const uint2* p) {
+  const gs_u2v v = *(const GS_GLOBAL gs_u2v*)p;
+  return make_uint2(v.x, v.y);
+}
+__device__ __forceinline__ void gstore(float4* p, float4 v) {
+  gs_f4v w;
+  w.x = v.x, w.y = v.y, w.z = v.z, w.w = v.w;
+  *(GS_GLOBAL gs_f4v*)p = w;
+}
+
 constexpr int kTile = GS_HIP_TILE;
 constexpr float kNear = 0.02f;        // train_common.glsl:65
 constexpr float kFar = 1000.0f;       // train_common.glsl:66

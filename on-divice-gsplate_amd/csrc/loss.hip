@@ -97,8 +97,8 @@ __device__ void exposure_fit(const gsh::ViewDev& view, uint32_t tile, int pix) {
   __shared__ float s_fit[8][4];
   float v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (pix >= 0) {
-    const float4 im = view.image[pix];
-    const uint32_t t = view.target[pix];
+    const float4 im = gload(view.image + pix);
+    const uint32_t t = gmem(view.target)[pix];
     const float ta = float(t >> 24) * (1.0f / 255.0f);
     if (ta > 0.95f && im.w < 0.05f) {
       const float photo[3] = {float(t & 0xffu) * (1.0f / 255.0f), float((t >> 8) & 0xffu) * (1.0f / 255.0f),
@@ -124,7 +124,7 @@ __device__ void exposure_fit(const gsh::ViewDev& view, uint32_t tile, int pix) {
       const int q = int(threadIdx.x) * 4 + k;
       r[k] = s_fit[q][0] + s_fit[q][1] + s_fit[q][2] + s_fit[q][3];
     }
-    view.tile_fit[size_t(tile) * 2u + threadIdx.x] = make_float4(r[0], r[1], r[2], r[3]);
+    gstore(view.tile_fit + size_t(tile) * 2u + threadIdx.x, make_float4(r[0], r[1], r[2], r[3]));
   }
 }
 
@@ -153,8 +153,8 @@ __global__ void __launch_bounds__(256) k_ssim(const gsh::ViewDev* __restrict__ v
 #pragma unroll
     for (int it = 0; it < kHaloIters; ++it) {
       id[it] = halo_pixel(f, int(threadIdx.x) + it * 256, ox, oy);
-      im[it] = id[it] >= 0 ? a.image[id[it]] : make_float4(0, 0, 0, 0);
-      tg[it] = id[it] >= 0 ? a.target[id[it]] : 0u;
+      im[it] = id[it] >= 0 ? gload(a.image + id[it]) : make_float4(0, 0, 0, 0);
+      tg[it] = id[it] >= 0 ? gmem(a.target)[id[it]] : 0u;
     }
 #pragma unroll
     for (int it = 0; it < kHaloIters; ++it) {
@@ -212,9 +212,9 @@ __global__ void __launch_bounds__(256) k_ssim(const gsh::ViewDev* __restrict__ v
     const float d_vx = -ssim / v;        // dSSIM/dE_xx
     const float A = 2.0f * my * cs * inv - ssim * 2.0f * mx / m - 2.0f * mx * d_vx - my * d_cxy;
     if (inside) {
-      a.maps[size_t(c) * pixels + pix] = A;
-      a.maps[size_t(3 + c) * pixels + pix] = d_vx;
-      a.maps[size_t(6 + c) * pixels + pix] = d_cxy;
+      gmem(a.maps)[size_t(c) * pixels + pix] = A;
+      gmem(a.maps)[size_t(3 + c) * pixels + pix] = d_vx;
+      gmem(a.maps)[size_t(6 + c) * pixels + pix] = d_cxy;
       const int centre = (ly + kSsimRadius) * kStride + lx + kSsimRadius;
       const float err = s_x[c][centre] - s_y[c][centre];
       abs_e += fabsf(err);
@@ -236,7 +236,7 @@ __global__ void __launch_bounds__(256) k_ssim(const gsh::ViewDev* __restrict__ v
   if (threadIdx.x == 0) {
     float r[3];
     for (int q = 0; q < 3; ++q) r[q] = s_sum[q][0] + s_sum[q][1] + s_sum[q][2] + s_sum[q][3];
-    a.tile_loss[tile] = make_float4(r[0], r[1], r[2], 0.0f);
+    gstore(a.tile_loss + tile, make_float4(r[0], r[1], r[2], 0.0f));
   }
   if ((f.flags & GS_HIP_FLAG_EXPOSURE) != 0u) exposure_fit(view, tile, inside ? int(pix) : -1);
 }
@@ -257,7 +257,7 @@ __global__ void __launch_bounds__(256) k_loss_grad(const gsh::ViewDev* __restric
   const uint32_t pixels = f.cam.width * f.cam.height;
   const uint32_t pix = uint32_t(py) * f.cam.width + uint32_t(px);
   float x[3] = {0, 0, 0}, y[3] = {0, 0, 0};
-  if (inside) xy_of(f, a.image[pix], a.target[pix], x, y);
+  if (inside) xy_of(f, gload(a.image + pix), gmem(a.target)[pix], x, y);
   const float scale = 1.0f / (3.0f * float(pixels));
   const float lambda = f.lambda_dssim;
   float grad[3];
@@ -275,7 +275,9 @@ __global__ void __launch_bounds__(256) k_loss_grad(const gsh::ViewDev* __restric
       float mv[kHaloIters][3];
 #pragma unroll
       for (int it = 0; it < kHaloIters; ++it) {
-        for (int q = 0; q < 3; ++q) mv[it][q] = id[it] >= 0 ? a.maps[size_t(3 * q + c) * pixels + id[it]] : 0.0f;
+        for (int q = 0; q < 3; ++q) {
+          mv[it][q] = id[it] >= 0 ? gmem(a.maps)[size_t(3 * q + c) * pixels + id[it]] : 0.0f;
+        }
       }
 #pragma unroll
       for (int it = 0; it < kHaloIters; ++it) {
@@ -302,7 +304,7 @@ __global__ void __launch_bounds__(256) k_loss_grad(const gsh::ViewDev* __restric
       __syncthreads();
     }
   }
-  if (inside) a.dldc[pix] = make_float4(grad[0], grad[1], grad[2], 0.0f);
+  if (inside) gstore(a.dldc + pix, make_float4(grad[0], grad[1], grad[2], 0.0f));
 }
 
 }  // namespace

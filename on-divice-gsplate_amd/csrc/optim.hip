@@ -85,13 +85,13 @@ __device__ __forceinline__ float adam1(float p, float g, float& m, float& v, flo
 
 __device__ __forceinline__ float4 adam4(float4 p, float4 g, float4* m_ptr, float4* v_ptr, float4 lr,
                                         const gs_hip_frame& f) {
-  float4 m = *m_ptr, v = *v_ptr;
+  float4 m = gload(m_ptr), v = gload(v_ptr);
   p.x = adam1(p.x, g.x, m.x, v.x, lr.x, f);
   p.y = adam1(p.y, g.y, m.y, v.y, lr.y, f);
   p.z = adam1(p.z, g.z, m.z, v.z, lr.z, f);
   p.w = adam1(p.w, g.w, m.w, v.w, lr.w, f);
-  *m_ptr = m;
-  *v_ptr = v;
+  gstore(m_ptr, m);
+  gstore(v_ptr, v);
   return p;
 }
 
@@ -276,33 +276,35 @@ __global__ void __launch_bounds__(256) k_adam(const gsh::SceneDev* __restrict__ 
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= f.slots || skip_update(a)) return;
   const int coeffs = min(sh_rest_coeffs(f.sh_degree), int(a.sh_coeffs));
-  float4 p0 = a.params[i * 4u], p1 = a.params[i * 4u + 1u], q = a.params[i * 4u + 2u], p3 = a.params[i * 4u + 3u];
+  float4 p0 = gload(a.params + i * 4u), p1 = gload(a.params + i * 4u + 1u), q = gload(a.params + i * 4u + 2u),
+         p3 = gload(a.params + i * 4u + 3u);
   if (p1.w < 0.5f) return;  // free slot
   float4 dp0 = make_float4(0, 0, 0, 0), dp1 = dp0, dq = dp0, dp3 = dp0;
   for (int k0 = 0; k0 < coeffs; k0 += kShGroup) {
-    const float4 d = a.dir_grad[size_t(k0 / kShGroup) * a.capacity + i];
+    const float4 d = gload(a.dir_grad + size_t(k0 / kShGroup) * a.capacity + i);
     add4(dp0, d.x, d.y, d.z, 0.0f);
   }
   bool drew = false, seen = false;
-  float4 st = a.stats[i];
+  float4 st = gload(a.stats + i);
   for (uint32_t v = 0; v < a.count; ++v) {
     const gsh::ViewDev& view = a.views[v];
-    if (view.tiles[i] == 0u) continue;
+    if (gmem(view.tiles)[i] == 0u) continue;
     seen = true;
     float g[9];
+    const GS_GLOBAL float* g2 = gmem(view.grad2d);
     for (int c = 0; c < 9; ++c) {
-      g[c] = view.grad2d[size_t(c) * a.capacity + i];
+      g[c] = g2[size_t(c) * a.capacity + i];
       drew = drew || g[c] != 0.0f;
     }
     // Densification statistics: view-space positional gradient in NDC units.
     const float gx = g[0] * (0.5f * float(view.f.cam.width)), gy = g[1] * (0.5f * float(view.f.cam.height));
-    const uint32_t misc = view.misc[i];
+    const uint32_t misc = gmem(view.misc)[i];
     st.x += sqrtf(gx * gx + gy * gy);
     st.y += 1.0f;
     st.z = fmaxf(st.z, float(misc >> 3));
     adjoint_view(view.f.cam, g, misc & 7u, p0, p1, q, dp0, dp1, dq, dp3);
   }
-  if (seen) a.stats[i] = st;
+  if (seen) gstore(a.stats + i, st);
   if (!drew && (f.flags & GS_HIP_FLAG_SPARSE_ADAM) != 0u) return;
   const float inv = 1.0f / float(a.count);  // the mean over the views
   const float4 lr0 = make_float4(f.lr_position, f.lr_position, f.lr_position, f.lr_opacity);
@@ -310,10 +312,10 @@ __global__ void __launch_bounds__(256) k_adam(const gsh::SceneDev* __restrict__ 
   const float4 lrq = make_float4(f.lr_rotation, f.lr_rotation, f.lr_rotation, f.lr_rotation);
   const float4 lr3 = make_float4(f.lr_sh_dc, f.lr_sh_dc, f.lr_sh_dc, 0.0f);
   auto scale4 = [inv](float4 v) { return make_float4(v.x * inv, v.y * inv, v.z * inv, v.w * inv); };
-  a.params[i * 4u] = adam4(p0, scale4(dp0), a.m_params + i * 4u, a.v_params + i * 4u, lr0, f);
-  a.params[i * 4u + 1u] = adam4(p1, scale4(dp1), a.m_params + i * 4u + 1u, a.v_params + i * 4u + 1u, lr1, f);
-  a.params[i * 4u + 2u] = adam4(q, scale4(dq), a.m_params + i * 4u + 2u, a.v_params + i * 4u + 2u, lrq, f);
-  a.params[i * 4u + 3u] = adam4(p3, scale4(dp3), a.m_params + i * 4u + 3u, a.v_params + i * 4u + 3u, lr3, f);
+  gstore(a.params + i * 4u, adam4(p0, scale4(dp0), a.m_params + i * 4u, a.v_params + i * 4u, lr0, f));
+  gstore(a.params + i * 4u + 1u, adam4(p1, scale4(dp1), a.m_params + i * 4u + 1u, a.v_params + i * 4u + 1u, lr1, f));
+  gstore(a.params + i * 4u + 2u, adam4(q, scale4(dq), a.m_params + i * 4u + 2u, a.v_params + i * 4u + 2u, lrq, f));
+  gstore(a.params + i * 4u + 3u, adam4(p3, scale4(dp3), a.m_params + i * 4u + 3u, a.v_params + i * 4u + 3u, lr3, f));
 }
 
 // Camera pose gradient of each view that asks for it (GS_HIP_FLAG_POSE_GRAD,
@@ -380,14 +382,14 @@ __global__ void __launch_bounds__(256) k_pose_grad(const gsh::SceneDev* __restri
 template <int kGroup>
 __device__ __forceinline__ void sh_group(const AdamArgs& a, const gs_hip_frame& f, uint32_t i, int coeffs) {
   constexpr int k0 = kGroup * kShGroup;
-  const float4 p0 = a.params[i * 4u];
+  const float4 p0 = gload(a.params + i * 4u);
   float rest[kShGroup][3], g[kShGroup][3];
 #pragma unroll
   for (int d = 0; d < kShGroup; ++d) {
     g[d][0] = g[d][1] = g[d][2] = 0.0f;
     rest[d][0] = rest[d][1] = rest[d][2] = 0.0f;
     if (k0 + d < coeffs) {
-      const float* sk = a.sh + (size_t(k0 + d) * a.capacity + i) * 3u;
+      const GS_GLOBAL float* sk = gmem(a.sh) + (size_t(k0 + d) * a.capacity + i) * 3u;
       rest[d][0] = sk[0];
       rest[d][1] = sk[1];
       rest[d][2] = sk[2];
@@ -404,8 +406,9 @@ __device__ __forceinline__ void sh_group(const AdamArgs& a, const gs_hip_frame& 
   for (uint32_t v = 0; v < GS_HIP_MAX_VIEWS; ++v) {
     if (v >= a.count) break;
     const gsh::ViewDev& view = a.views[v];
-    gv[v] = make_float4(view.grad2d[size_t(6) * a.capacity + i], view.grad2d[size_t(7) * a.capacity + i],
-                        view.grad2d[size_t(8) * a.capacity + i], __uint_as_float(view.misc[i]));
+    const GS_GLOBAL float* g2 = gmem(view.grad2d);
+    gv[v] = make_float4(g2[size_t(6) * a.capacity + i], g2[size_t(7) * a.capacity + i], g2[size_t(8) * a.capacity + i],
+                        __uint_as_float(gmem(view.misc)[i]));
   }
 #pragma unroll
   for (uint32_t v = 0; v < GS_HIP_MAX_VIEWS; ++v) {
@@ -414,7 +417,7 @@ __device__ __forceinline__ void sh_group(const AdamArgs& a, const gs_hip_frame& 
     const float4 gc = gv[v];
     drew = drew || gc.x != 0.0f || gc.y != 0.0f || gc.z != 0.0f;
     if (sparse && !drew) {
-      for (int c = 0; c < 6; ++c) drew = drew || view.grad2d[size_t(c) * a.capacity + i] != 0.0f;
+      for (int c = 0; c < 6; ++c) drew = drew || gmem(view.grad2d)[size_t(c) * a.capacity + i] != 0.0f;
     }
     const uint32_t clamped = __float_as_uint(gc.w) & 7u;
     const float dcol[3] = {clamped & 1u ? 0.0f : gc.x, clamped & 2u ? 0.0f : gc.y, clamped & 4u ? 0.0f : gc.z};
@@ -447,22 +450,22 @@ __device__ __forceinline__ void sh_group(const AdamArgs& a, const gs_hip_frame& 
     dpy += (ddy - y * dd) / dlen;
     dpz += (ddz - z * dd) / dlen;
   }
-  a.dir_grad[size_t(kGroup) * a.capacity + i] = make_float4(dpx, dpy, dpz, 0.0f);
+  gstore(a.dir_grad + size_t(kGroup) * a.capacity + i, make_float4(dpx, dpy, dpz, 0.0f));
   if (sparse && !drew) return;  // kept by sparse Adam
   const float inv = 1.0f / float(a.count);
   float m[kShGroup][3], vv[kShGroup][3];
   for (int d = 0; d < kShGroup; ++d) {
     if (k0 + d >= coeffs) break;
     const size_t o = (size_t(k0 + d) * a.capacity + i) * 3u;
-    for (int c = 0; c < 3; ++c) m[d][c] = a.m_sh[o + c], vv[d][c] = a.v_sh[o + c];
+    for (int c = 0; c < 3; ++c) m[d][c] = gmem(a.m_sh)[o + c], vv[d][c] = gmem(a.v_sh)[o + c];
   }
   for (int d = 0; d < kShGroup; ++d) {
     if (k0 + d >= coeffs) break;
     const size_t o = (size_t(k0 + d) * a.capacity + i) * 3u;
     for (int c = 0; c < 3; ++c) {
-      a.sh[o + c] = adam1(rest[d][c], g[d][c] * inv, m[d][c], vv[d][c], f.lr_sh_rest, f);
-      a.m_sh[o + c] = m[d][c];
-      a.v_sh[o + c] = vv[d][c];
+      gmem(a.sh)[o + c] = adam1(rest[d][c], g[d][c] * inv, m[d][c], vv[d][c], f.lr_sh_rest, f);
+      gmem(a.m_sh)[o + c] = m[d][c];
+      gmem(a.v_sh)[o + c] = vv[d][c];
     }
   }
 }
@@ -481,7 +484,7 @@ __global__ void __launch_bounds__(kShBlock) k_adam_sh(const gsh::SceneDev* __res
   const uint32_t i = blockIdx.x * 64u + (threadIdx.x & 63u);
   const int coeffs = min(sh_rest_coeffs(f.sh_degree), int(a.sh_coeffs));
   if (group * kShGroup >= coeffs || i >= f.slots || skip_update(a)) return;
-  if (a.params[i * 4u + 1u].w < 0.5f) return;  // free slot
+  if (gload(a.params + i * 4u + 1u).w < 0.5f) return;  // free slot
   if (group == 0) sh_group<0>(a, f, i, coeffs);
   else if (group == 1) sh_group<1>(a, f, i, coeffs);
   else sh_group<2>(a, f, i, coeffs);

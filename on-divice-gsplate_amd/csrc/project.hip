@@ -34,7 +34,7 @@ __device__ __forceinline__ void project_view(const gsh::ViewDev& vd, const Proje
                                              const float (&rest)[GS_HIP_MAX_SH_REST][3]) {
 #pragma clang fp contract(off)
   const gs_hip_frame& f = vd.f;
-  for (int c = 0; c < GS_HIP_GRAD2D_FLOATS; ++c) vd.grad2d[size_t(c) * a.capacity + i] = 0.0f;
+  for (int c = 0; c < GS_HIP_GRAD2D_FLOATS; ++c) gmem(vd.grad2d)[size_t(c) * a.capacity + i] = 0.0f;
   const float* R = f.cam.rotation;
   // Camera-space position, left to right sums.
   const float tx = R[0] * p0.x + R[1] * p0.y + R[2] * p0.z + f.cam.translation[0];
@@ -132,13 +132,13 @@ __device__ __forceinline__ void project_view(const gsh::ViewDev& vd, const Proje
     }
   }
   float4* r = vd.records + size_t(i) * 3u;
-  r[0] = rec.a;
-  r[1] = rec.b;
-  r[2] = rec.c;
-  vd.footprint[size_t(i) * 2u] = foot;
-  vd.footprint[size_t(i) * 2u + 1u] = make_float4(rec.a.x, rec.a.y, __uint_as_float(tiles), 0.0f);
-  vd.tiles[i] = tiles;
-  vd.misc[i] = __float_as_uint(rec.c.w);
+  gstore(r, rec.a);
+  gstore(r + 1, rec.b);
+  gstore(r + 2, rec.c);
+  gstore(vd.footprint + size_t(i) * 2u, foot);
+  gstore(vd.footprint + size_t(i) * 2u + 1u, make_float4(rec.a.x, rec.a.y, __uint_as_float(tiles), 0.0f));
+  gmem(vd.tiles)[i] = tiles;
+  gmem(vd.misc)[i] = __float_as_uint(rec.c.w);
   const bool depth16 = (f.flags & GS_HIP_FLAG_DEPTH16) != 0u;
   uint32_t key = depth16 ? 0xffffu : 0xffffffffu;
   if (tiles > 0u) {
@@ -151,8 +151,8 @@ __device__ __forceinline__ void project_view(const gsh::ViewDev& vd, const Proje
       key = __float_as_uint(tz);  // z > 0: the bits order like the values
     }
   }
-  vd.depth_keys[i] = key;
-  vd.depth_order[i] = i;
+  gmem(vd.depth_keys)[i] = key;
+  gmem(vd.depth_order)[i] = i;
 }
 
 // One thread per slot of scene blockIdx.y (its model and views from the tables).
@@ -168,8 +168,8 @@ __global__ void __launch_bounds__(256) k_project(const gsh::SceneDev* __restrict
   a.count = sc.views;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.views[0].f.slots) return;
-  const float4 p0 = a.params[i * 4u], p1 = a.params[i * 4u + 1u];
-  const float4 q = a.params[i * 4u + 2u], p3 = a.params[i * 4u + 3u];
+  const float4 p0 = gload(a.params + i * 4u), p1 = gload(a.params + i * 4u + 1u);
+  const float4 q = gload(a.params + i * 4u + 2u), p3 = gload(a.params + i * 4u + 3u);
   // SH rest of a live slot, once for all views.
   float rest[GS_HIP_MAX_SH_REST][3];
 #pragma unroll
@@ -178,7 +178,7 @@ __global__ void __launch_bounds__(256) k_project(const gsh::SceneDev* __restrict
 #pragma unroll
     for (int k = 0; k < GS_HIP_MAX_SH_REST; ++k) {
       if (k < int(a.sh_coeffs)) {
-        const float* s = a.sh + (size_t(k) * a.capacity + i) * 3u;
+        const GS_GLOBAL float* s = gmem(a.sh) + (size_t(k) * a.capacity + i) * 3u;
         rest[k][0] = s[0];
         rest[k][1] = s[1];
         rest[k][2] = s[2];

@@ -102,7 +102,7 @@ __device__ __forceinline__ float box_min_quadratic(float a, float b, float c, fl
 __device__ __forceinline__ uint32_t stage(Staging& s, const float4* __restrict__ records, uint32_t g, int t, int tx0,
                                           int ty0) {
   const float4* r = records + size_t(g) * 3u;
-  const float4 ra = r[0], rb = r[1], rc = r[2];
+  const float4 ra = gload(r), rb = gload(r + 1), rc = gload(r + 2);
   s.a[t] = make_float4(ra.x, ra.y, -0.5f * ra.z, -ra.w);
   s.b[t] = make_float4(-0.5f * rb.x, rb.y, rb.z, rb.w);
   s.blue[t] = rc.x;
@@ -158,7 +158,7 @@ __device__ __forceinline__ SplatEval eval_splat(float4 sa, float4 sb, float cx, 
 __device__ __forceinline__ void forward_tile(const gsh::ViewDev& a, uint32_t tile, Staging& s) {
   const gs_hip_frame& f = a.f;
   const PixelCoord pc = pixel_coord(f, tile);
-  const uint2 range = a.ranges[pc.tile];
+  const uint2 range = gload(a.ranges + pc.tile);
   const float cx = float(pc.px) + 0.5f, cy = float(pc.py) + 0.5f;
   bool done = !pc.inside;
   float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
@@ -169,7 +169,7 @@ __device__ __forceinline__ void forward_tile(const gsh::ViewDev& a, uint32_t til
     const uint32_t n = min(uint32_t(kBatch), range.y - start);
     uint32_t hits = 0u;
     if (threadIdx.x < n) {
-      hits = stage(s, a.records, a.pair_gaussians[start + threadIdx.x], threadIdx.x, pc.tx0, pc.ty0);
+      hits = stage(s, a.records, gmem(a.pair_gaussians)[start + threadIdx.x], threadIdx.x, pc.tx0, pc.ty0);
     }
     publish_masks(s, hits, pc.wave, pc.lane);
     __syncthreads();
@@ -208,8 +208,8 @@ __device__ __forceinline__ void forward_tile(const gsh::ViewDev& a, uint32_t til
   }
   if (pc.inside) {
     const uint32_t pix = uint32_t(pc.py) * f.cam.width + uint32_t(pc.px);
-    a.image[pix] = make_float4(cr, cg, cb, T);
-    a.last[pix] = last;
+    gstore(a.image + pix, make_float4(cr, cg, cb, T));
+    gmem(a.last)[pix] = last;
   }
 }
 
@@ -229,16 +229,16 @@ __device__ __forceinline__ void backward_tile(const gsh::ViewDev& a, uint32_t ti
   float* s_grad = sh.grad;
   uint32_t* s_wave_last = sh.wave_last;
   const PixelCoord pc = pixel_coord(f, tile);
-  const uint2 range = a.ranges[pc.tile];
+  const uint2 range = gload(a.ranges + pc.tile);
   const float cx = float(pc.px) + 0.5f, cy = float(pc.py) + 0.5f;
 
   float T_final = 0.0f, gr = 0.0f, gg = 0.0f, gb = 0.0f;
   uint32_t last = 0u;
   if (pc.inside) {
     const uint32_t pix = uint32_t(pc.py) * f.cam.width + uint32_t(pc.px);
-    T_final = a.image[pix].w;
-    last = a.last[pix];
-    const float4 d = a.dldc[pix];
+    T_final = gload(a.image + pix).w;
+    last = gmem(a.last)[pix];
+    const float4 d = gload(a.dldc + pix);
     gr = d.x, gg = d.y, gb = d.z;
   }
   // T_final/(1 - alpha) times bg . dL/dC: the background seen through the splat.
@@ -259,7 +259,7 @@ __device__ __forceinline__ void backward_tile(const gsh::ViewDev& a, uint32_t ti
     uint32_t hits = 0u;
     if (threadIdx.x < n) {
       const uint32_t k = bstart + threadIdx.x;
-      const uint32_t g = a.pair_gaussians[k];
+      const uint32_t g = gmem(a.pair_gaussians)[k];
       s_g[threadIdx.x] = g;
       for (int c = 0; c < kGrad; ++c) s_grad[threadIdx.x * kGrad + c] = 0.0f;
       hits = stage(s, a.records, g, threadIdx.x, pc.tx0, pc.ty0);
@@ -320,7 +320,7 @@ __device__ __forceinline__ void backward_tile(const gsh::ViewDev& a, uint32_t ti
       const uint32_t g = s_g[threadIdx.x];
       for (int c = 0; c < kGrad; ++c) {
         const float v = s_grad[threadIdx.x * kGrad + c];
-        if (v != 0.0f) unsafeAtomicAdd(&a.grad2d[size_t(c) * capacity + g], v);
+        if (v != 0.0f) global_add(gmem(a.grad2d) + size_t(c) * capacity + g, v);
       }
     }
     __syncthreads();

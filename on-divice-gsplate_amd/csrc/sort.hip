@@ -72,19 +72,26 @@ __global__ void __launch_bounds__(kSortThreads) k_radix_upsweep(const SortSeg* _
   const uint32_t n = seg_count(g.count, g.count_max);
   const uint32_t radix = 1u << nbits, mask = radix - 1u;
   if (blockIdx.x * kSortItems >= n) {  // past the segment's items: zero counts
-    for (uint32_t d = threadIdx.x; d < radix; d += kSortThreads) g.hist[size_t(d) * g.blocks + blockIdx.x] = 0u;
+    for (uint32_t d = threadIdx.x; d < radix; d += kSortThreads) gmem(g.hist)[size_t(d) * g.blocks + blockIdx.x] = 0u;
     return;
   }
-  const uint32_t* __restrict__ keys = from_alt ? g.keys_alt : g.keys;
+  const GS_GLOBAL uint32_t* __restrict__ keys = gmem(from_alt ? g.keys_alt : g.keys);
   const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
   for (int d = threadIdx.x; d < kWaves * 256; d += kSortThreads) (&h[0][0])[d] = 0u;
   __syncthreads();
   const uint32_t wbase = blockIdx.x * kSortItems + wave * (kSortItems / kWaves);
   const uint64_t below_me = lanemask_lt();
+  uint32_t kv[kSortRounds];  // every round's key requested before any is ranked
+#pragma unroll
+  for (int r = 0; r < kSortRounds; ++r) {
+    const uint32_t idx = wbase + r * 64 + lane;
+    kv[r] = idx < n ? keys[idx] : 0u;
+  }
+#pragma unroll
   for (int r = 0; r < kSortRounds; ++r) {
     const uint32_t idx = wbase + r * 64 + lane;
     const bool valid = idx < n;
-    const uint32_t digit = valid ? (keys[idx] >> shift) & mask : 0u;
+    const uint32_t digit = valid ? (kv[r] >> shift) & mask : 0u;
     const uint64_t peers = match_digit(digit, nbits, __ballot(valid));
     if (valid && (peers & below_me) == 0ull) h[wave][digit] += __popcll(peers);
   }
@@ -92,7 +99,7 @@ __global__ void __launch_bounds__(kSortThreads) k_radix_upsweep(const SortSeg* _
   for (uint32_t d = threadIdx.x; d < radix; d += kSortThreads) {
     uint32_t s = 0;
     for (int w = 0; w < kWaves; ++w) s += h[w][d];
-    g.hist[size_t(d) * g.blocks + blockIdx.x] = s;
+    gmem(g.hist)[size_t(d) * g.blocks + blockIdx.x] = s;
   }
 }
 
@@ -161,16 +168,16 @@ __global__ void __launch_bounds__(kSortThreads) k_radix_scatter(const SortSeg* _
   const uint32_t n = seg_count(g.count, g.count_max);
   const uint32_t base = blockIdx.x * kSortItems;
   if (base >= n || blockIdx.x >= g.blocks) return;
-  const uint32_t* __restrict__ keys_in = from_alt ? g.keys_alt : g.keys;
-  const uint32_t* __restrict__ vals_in = from_alt ? g.vals_alt : g.vals;
-  uint32_t* __restrict__ keys_out = from_alt ? g.keys : g.keys_alt;
-  uint32_t* __restrict__ vals_out = from_alt ? g.vals : g.vals_alt;
+  const GS_GLOBAL uint32_t* __restrict__ keys_in = gmem(from_alt ? g.keys_alt : g.keys);
+  const GS_GLOBAL uint32_t* __restrict__ vals_in = gmem(from_alt ? g.vals_alt : g.vals);
+  GS_GLOBAL uint32_t* __restrict__ keys_out = gmem(from_alt ? g.keys : g.keys_alt);
+  GS_GLOBAL uint32_t* __restrict__ vals_out = gmem(from_alt ? g.vals : g.vals_alt);
   const uint32_t radix = 1u << nbits, mask = radix - 1u;
   const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
   // Output start of each digit: the digits below it over the whole segment
   // (a block scan of the totals) plus this digit's items in earlier blocks.
-  block_scan_256(threadIdx.x < radix ? g.digit_totals[threadIdx.x] : 0u, s_tmp, s_global);
-  if (threadIdx.x < radix) s_global[threadIdx.x] += g.hist[size_t(threadIdx.x) * g.blocks + blockIdx.x];
+  block_scan_256(threadIdx.x < radix ? gmem(g.digit_totals)[threadIdx.x] : 0u, s_tmp, s_global);
+  if (threadIdx.x < radix) s_global[threadIdx.x] += gmem(g.hist)[size_t(threadIdx.x) * g.blocks + blockIdx.x];
   for (int d = threadIdx.x; d < kWaves * 256; d += kSortThreads) (&s_wave[0][0])[d] = 0u;
   __syncthreads();
   const uint32_t wbase = base + wave * (kSortItems / kWaves);
@@ -237,7 +244,31 @@ constexpr int kScanRounds = 16;
 constexpr int kScanItems = kScanThreads * kScanRounds;
 
 __device__ __forceinline__ uint32_t load_value(const ScanSeg& s, uint32_t idx) {
-  return s.gather != nullptr ? s.values[s.gather[idx]] : s.values[idx] >> s.shift;
+  return s.gather != nullptr ? gmem(s.values)[gmem(s.gather)[idx]] : gmem(s.values)[idx] >> s.shift;
+}
+
+// A thread's kScanRounds values of a scan block, all requested before any is
+// used (zero past the count).
+__device__ __forceinline__ void load_rounds(const ScanSeg& g, uint32_t base, uint32_t count,
+                                            uint32_t (&v)[kScanRounds]) {
+  const GS_GLOBAL uint32_t* values = gmem(g.values);
+  if (g.gather != nullptr) {  // the indices first, then the values they name
+    const GS_GLOBAL uint32_t* gather = gmem(g.gather);
+    uint32_t at[kScanRounds];
+#pragma unroll
+    for (int r = 0; r < kScanRounds; ++r) {
+      const uint32_t idx = base + r * kScanThreads + threadIdx.x;
+      at[r] = idx < count ? gather[idx] : 0xffffffffu;
+    }
+#pragma unroll
+    for (int r = 0; r < kScanRounds; ++r) v[r] = at[r] != 0xffffffffu ? values[at[r]] : 0u;
+  } else {
+#pragma unroll
+    for (int r = 0; r < kScanRounds; ++r) {
+      const uint32_t idx = base + r * kScanThreads + threadIdx.x;
+      v[r] = idx < count ? values[idx] >> g.shift : 0u;
+    }
+  }
 }
 
 __global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const ScanSeg* __restrict__ segs) {
@@ -245,12 +276,12 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const ScanSeg* __r
   const ScanSeg& g = segs[blockIdx.y];
   if (blockIdx.x >= g.blocks) return;  // a smaller segment of the launch
   const uint32_t count = seg_count(g.count, g.count_max);
-  uint32_t sum = 0;
   const uint32_t base = blockIdx.x * kScanItems;
-  for (int r = 0; r < kScanRounds; ++r) {
-    const uint32_t idx = base + r * kScanThreads + threadIdx.x;
-    if (idx < count) sum += load_value(g, idx);
-  }
+  uint32_t vals[kScanRounds];
+  load_rounds(g, base, count, vals);
+  uint32_t sum = 0;
+#pragma unroll
+  for (int r = 0; r < kScanRounds; ++r) sum += vals[r];
   for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
   if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = sum;
   __syncthreads();
@@ -300,9 +331,12 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_down(const ScanSeg* __res
   if (threadIdx.x == 0) s_carry = g.block_sums[blockIdx.x];
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t vals[kScanRounds];
+  load_rounds(g, base, count, vals);
+#pragma unroll
   for (int r = 0; r < kScanRounds; ++r) {
     const uint32_t idx = base + r * kScanThreads + threadIdx.x;
-    const uint32_t v = idx < count ? load_value(g, idx) : 0u;
+    const uint32_t v = vals[r];
     uint32_t incl = v;
     for (int o = 1; o < 64; o <<= 1) {
       const uint32_t t = __shfl_up(incl, o, 64);
@@ -312,7 +346,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_down(const ScanSeg* __res
     __syncthreads();
     uint32_t before = s_carry;
     for (int w = 0; w < wave; ++w) before += s_wave[w];
-    if (idx < count) g.out[idx] = before + incl - v;
+    if (idx < count) gmem(g.out)[idx] = before + incl - v;
     __syncthreads();
     if (threadIdx.x == 0) s_carry += s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
     __syncthreads();
@@ -339,13 +373,13 @@ __global__ void __launch_bounds__(256) k_emit(const ViewDev* __restrict__ views)
   uint32_t i = 0u, count = 0u;
   float4 fp = make_float4(0, 0, 0, 0), r0 = fp;
   if (k < f.slots) {
-    i = v.depth_order[k];
+    i = gmem(v.depth_order)[k];
     // One 32 B line per slot: the depth order makes these reads random.
-    fp = v.footprint[size_t(i) * 2u];
-    r0 = v.footprint[size_t(i) * 2u + 1u];  // mean x, mean y, tiles
+    fp = gload(v.footprint + size_t(i) * 2u);
+    r0 = gload(v.footprint + size_t(i) * 2u + 1u);  // mean x, mean y, tiles
     count = __float_as_uint(r0.z);
   }
-  const uint64_t p0 = count > 0u ? uint64_t(v.offsets[k]) : 0ull;
+  const uint64_t p0 = count > 0u ? uint64_t(gmem(v.offsets)[k]) : 0ull;
   const uint64_t cap = v.pair_capacity;
   const uint32_t pad_tile = f.tiles_x * f.tiles_y;
   if (count > 0u && count <= kWaveEmit) {
@@ -360,16 +394,16 @@ __global__ void __launch_bounds__(256) k_emit(const ViewDev* __restrict__ views)
         footprint_row_tiles(r0.x, r0.y, fp.x, fp.y, fp.z, fp.w, row_first, row_last, ty, f.cam.width, x0, x1);
         wanted += x1 - x0;
         for (uint32_t x = x0; x < x1 && p < end; ++x, ++p) {
-          v.pair_tiles[p] = ty * f.tiles_x + x;
-          v.pair_gaussians[p] = i;
+          gmem(v.pair_tiles)[p] = ty * f.tiles_x + x;
+          gmem(v.pair_gaussians)[p] = i;
         }
       }
     }
     if (wanted != count) {
       atomicAdd(&v.totals[1], 1u);
       for (; p < end; ++p) {
-        v.pair_tiles[p] = pad_tile;
-        v.pair_gaussians[p] = i;
+        gmem(v.pair_tiles)[p] = pad_tile;
+        gmem(v.pair_gaussians)[p] = i;
       }
     }
   }
@@ -396,8 +430,8 @@ __global__ void __launch_bounds__(256) k_emit(const ViewDev* __restrict__ views)
         for (uint32_t x = x0 + uint32_t(lane); x < x1; x += 64u) {
           const uint64_t q = p + (x - x0);
           if (q < end) {
-            v.pair_tiles[q] = ty * f.tiles_x + x;
-            v.pair_gaussians[q] = bi;
+            gmem(v.pair_tiles)[q] = ty * f.tiles_x + x;
+            gmem(v.pair_gaussians)[q] = bi;
           }
         }
         p += x1 - x0;
@@ -406,8 +440,8 @@ __global__ void __launch_bounds__(256) k_emit(const ViewDev* __restrict__ views)
     if (wanted != bcount) {
       if (lane == 0) atomicAdd(&v.totals[1], 1u);
       for (uint64_t q = min(p, end) + uint64_t(lane); q < end; q += 64u) {
-        v.pair_tiles[q] = pad_tile;
-        v.pair_gaussians[q] = bi;
+        gmem(v.pair_tiles)[q] = pad_tile;
+        gmem(v.pair_gaussians)[q] = bi;
       }
     }
   }
@@ -426,17 +460,19 @@ __global__ void __launch_bounds__(256) k_ranges(const ViewDev* __restrict__ view
   const ViewDev& v = views[blockIdx.y];
   const uint32_t count = uint32_t(std::min<uint64_t>(v.totals[0], v.pair_capacity));
   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < count; s += gridDim.x * blockDim.x) {
-    const uint32_t tile = v.pair_tiles[s];
+    const GS_GLOBAL uint32_t* pt = gmem(v.pair_tiles);
+    GS_GLOBAL uint32_t* rg = gmem(reinterpret_cast<uint32_t*>(v.ranges));  // [tile][2]: first, end
+    const uint32_t tile = pt[s];
     if (s == 0u) {
-      v.ranges[tile].x = 0u;
+      rg[size_t(tile) * 2u] = 0u;
     } else {
-      const uint32_t prev = v.pair_tiles[s - 1u];
+      const uint32_t prev = pt[s - 1u];
       if (prev != tile) {
-        v.ranges[prev].y = s;
-        v.ranges[tile].x = s;
+        rg[size_t(prev) * 2u + 1u] = s;
+        rg[size_t(tile) * 2u] = s;
       }
     }
-    if (s == count - 1u) v.ranges[tile].y = count;
+    if (s == count - 1u) rg[size_t(tile) * 2u + 1u] = count;
   }
 }
 
