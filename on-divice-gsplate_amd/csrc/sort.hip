@@ -26,6 +26,7 @@
 
 #include "gs_common.h"
 #include "gs_internal.h"
+#include "wave.h"
 
 namespace {
 
@@ -364,12 +365,40 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_down(const ScanSeg* __res
 // differs (it cannot: same code, same inputs) is padded with the unused tile
 // `tiles` and counted in totals[1].
 constexpr uint32_t kWaveEmit = 32;
+// Pairs of a wave's small splats staged per pass in LDS, then stored
+// contiguously (lane l writes positions l, l + 64, ...): the wave's slots are
+// consecutive in depth order, so their pairs form one run [lo, hi).
+constexpr uint32_t kEmitWindow = 512;
+constexpr uint32_t kNoTile = 0xffffffffu;
+
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t w = (uint64_t(__shfl_xor(uint32_t(v >> 32), o, 64)) << 32) | __shfl_xor(uint32_t(v), o, 64);
+    v = min(v, w);
+  }
+  return uniform_u64(v);
+}
+
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t w = (uint64_t(__shfl_xor(uint32_t(v >> 32), o, 64)) << 32) | __shfl_xor(uint32_t(v), o, 64);
+    v = max(v, w);
+  }
+  return uniform_u64(v);
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
 
 __global__ void __launch_bounds__(256) k_emit(const ViewDev* __restrict__ views) {
+  __shared__ uint32_t s_tile[4][kEmitWindow];
+  __shared__ uint32_t s_gauss[4][kEmitWindow];
   const ViewDev& v = views[blockIdx.y];
   const gs_hip_frame& f = v.f;
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = int(threadIdx.x & 63u);
+  const int lane = int(threadIdx.x & 63u), wave = int(threadIdx.x >> 6);
   uint32_t i = 0u, count = 0u;
   float4 fp = make_float4(0, 0, 0, 0), r0 = fp;
   if (k < f.slots) {
@@ -382,30 +411,58 @@ __global__ void __launch_bounds__(256) k_emit(const ViewDev* __restrict__ views)
   const uint64_t p0 = count > 0u ? uint64_t(gmem(v.offsets)[k]) : 0ull;
   const uint64_t cap = v.pair_capacity;
   const uint32_t pad_tile = f.tiles_x * f.tiles_y;
-  if (count > 0u && count <= kWaveEmit) {
-    uint64_t p = p0;
-    const uint64_t end = min(p + count, cap);
-    uint32_t wanted = 0;
-    int row_first, row_last;
-    footprint_rows(r0.y, fp.z, fp.w, f.cam.height, row_first, row_last);
-    if (row_first <= row_last) {
-      for (uint32_t ty = uint32_t(row_first) / kTile; ty <= uint32_t(row_last) / kTile; ++ty) {
-        uint32_t x0, x1;
-        footprint_row_tiles(r0.x, r0.y, fp.x, fp.y, fp.z, fp.w, row_first, row_last, ty, f.cam.width, x0, x1);
-        wanted += x1 - x0;
-        for (uint32_t x = x0; x < x1 && p < end; ++x, ++p) {
-          gmem(v.pair_tiles)[p] = ty * f.tiles_x + x;
-          gmem(v.pair_gaussians)[p] = i;
+  GS_GLOBAL uint32_t* const out_tiles = gmem(v.pair_tiles);
+  GS_GLOBAL uint32_t* const out_gauss = gmem(v.pair_gaussians);
+  const bool small = count > 0u && count <= kWaveEmit;
+  const uint64_t end = min(p0 + count, cap);
+  int row_first = 0, row_last = -1;
+  if (small) footprint_rows(r0.y, fp.z, fp.w, f.cam.height, row_first, row_last);
+  // The run of the wave's small-splat pairs (wave-uniform); windows skip the
+  // stretches that only large splats fill.
+  const uint64_t hi = wave_max_u64(small ? end : 0ull);
+  const uint64_t lo = wave_min_u64(small && p0 < end ? p0 : ~0ull);
+  for (uint64_t win = lo; win < hi;) {
+    for (uint32_t e = uint32_t(lane); e < kEmitWindow; e += 64u) s_tile[wave][e] = kNoTile;
+    wave_sync();
+    if (small) {
+      // The slot's tiles row by row, as the projection counted them (train_emit.comp:26-64);
+      // the ones in this window go to LDS.
+      uint64_t p = p0;
+      uint32_t wanted = 0;
+      if (row_first <= row_last) {
+        for (uint32_t ty = uint32_t(row_first) / kTile; ty <= uint32_t(row_last) / kTile; ++ty) {
+          uint32_t x0, x1;
+          footprint_row_tiles(r0.x, r0.y, fp.x, fp.y, fp.z, fp.w, row_first, row_last, ty, f.cam.width, x0, x1);
+          wanted += x1 - x0;
+          for (uint32_t x = x0; x < x1 && p < end; ++x, ++p) {
+            if (p >= win && p < win + kEmitWindow) {
+              s_tile[wave][p - win] = ty * f.tiles_x + x;
+              s_gauss[wave][p - win] = i;
+            }
+          }
+        }
+      }
+      if (wanted != count) {
+        if (win == lo) atomicAdd(&v.totals[1], 1u);
+        for (; p < end; ++p) {
+          if (p >= win && p < win + kEmitWindow) {
+            s_tile[wave][p - win] = pad_tile;
+            s_gauss[wave][p - win] = i;
+          }
         }
       }
     }
-    if (wanted != count) {
-      atomicAdd(&v.totals[1], 1u);
-      for (; p < end; ++p) {
-        gmem(v.pair_tiles)[p] = pad_tile;
-        gmem(v.pair_gaussians)[p] = i;
+    wave_sync();
+    for (uint32_t e = uint32_t(lane); e < kEmitWindow && win + e < hi; e += 64u) {
+      const uint32_t t = s_tile[wave][e];
+      if (t != kNoTile) {  // positions of the wave's large splats are written below
+        out_tiles[win + e] = t;
+        out_gauss[win + e] = s_gauss[wave][e];
       }
     }
+    wave_sync();
+    const uint64_t next = win + kEmitWindow;
+    win = wave_min_u64(small && end > next ? max(p0, next) : ~0ull);
   }
   // Large splats, one at a time per wave: every lane takes part.
   uint64_t big = __ballot(count > kWaveEmit);
@@ -430,8 +487,8 @@ __global__ void __launch_bounds__(256) k_emit(const ViewDev* __restrict__ views)
         for (uint32_t x = x0 + uint32_t(lane); x < x1; x += 64u) {
           const uint64_t q = p + (x - x0);
           if (q < end) {
-            gmem(v.pair_tiles)[q] = ty * f.tiles_x + x;
-            gmem(v.pair_gaussians)[q] = bi;
+            out_tiles[q] = ty * f.tiles_x + x;
+            out_gauss[q] = bi;
           }
         }
         p += x1 - x0;
@@ -440,8 +497,8 @@ __global__ void __launch_bounds__(256) k_emit(const ViewDev* __restrict__ views)
     if (wanted != bcount) {
       if (lane == 0) atomicAdd(&v.totals[1], 1u);
       for (uint64_t q = min(p, end) + uint64_t(lane); q < end; q += 64u) {
-        gmem(v.pair_tiles)[q] = pad_tile;
-        gmem(v.pair_gaussians)[q] = bi;
+        out_tiles[q] = pad_tile;
+        out_gauss[q] = bi;
       }
     }
   }
