@@ -243,9 +243,11 @@ __device__ __forceinline__ void backward_tile(const gsh::ViewDev& a, uint32_t ti
   }
   // T_final/(1 - alpha) times bg . dL/dC: the background seen through the splat.
   const float bg_term = T_final * (f.background[0] * gr + f.background[1] * gg + f.background[2] * gb);
-  // Back-to-front state: T in front of the current splat, colour behind it.
-  float T = T_final, acc_r = 0.0f, acc_g = 0.0f, acc_b = 0.0f;
-  float last_alpha = 0.0f, last_r = 0.0f, last_g = 0.0f, last_b = 0.0f;
+  // Back-to-front state: T in front of the current splat, and the colour
+  // behind it carried as its dot product with dL/dC (dL/dC is constant along
+  // the walk, so acc' = a c + (1 - a) acc gives acc'.g = a (c.g) + (1 - a) acc.g:
+  // one running value instead of three channels).
+  float T = T_final, acc_dot = 0.0f, last_alpha = 0.0f, last_dot = 0.0f;
 
   const uint32_t wl = wave_max_u32(last);
   if (pc.lane == 0) s_wave_last[pc.wave] = wl;
@@ -287,11 +289,9 @@ __device__ __forceinline__ void backward_tile(const gsh::ViewDev& a, uint32_t ti
         // state and add zeros (selects, so no 0 * inf can leak in).
         const float inv = __builtin_amdgcn_rcpf(1.0f - alpha);
         const float T_in = T * inv;  // transmittance in front of this splat
-        const float ar = fmaf(last_alpha, last_r - acc_r, acc_r);
-        const float ag = fmaf(last_alpha, last_g - acc_g, acc_g);
-        const float ab = fmaf(last_alpha, last_b - acc_b, acc_b);
-        const float dot = (sb.z - ar) * gr + (sb.w - ag) * gg + (blue - ab) * gb;
-        const float dL_dalpha = dot * T_in - bg_term * inv;
+        const float col_dot = fmaf(sb.z, gr, fmaf(sb.w, gg, blue * gb));  // colour . dL/dC
+        const float acc = fmaf(last_alpha, last_dot - acc_dot, acc_dot);    // (colour behind) . dL/dC
+        const float dL_dalpha = (col_dot - acc) * T_in - bg_term * inv;
         const bool unclamped = contributes && e.raw < kMaxAlpha;
         const float wT = contributes ? alpha * T_in : 0.0f;
         const float dpow = unclamped ? e.raw * dL_dalpha : 0.0f;
@@ -304,12 +304,8 @@ __device__ __forceinline__ void backward_tile(const gsh::ViewDev& a, uint32_t ti
         const float g4 = -0.5f * dpow * e.dy * e.dy;
         const float g6 = wT * gr, g7 = wT * gg, g8 = wT * gb;
         T = contributes ? T_in : T;
-        acc_r = contributes ? ar : acc_r;
-        acc_g = contributes ? ag : acc_g;
-        acc_b = contributes ? ab : acc_b;
-        last_r = contributes ? sb.z : last_r;
-        last_g = contributes ? sb.w : last_g;
-        last_b = contributes ? blue : last_b;
+        acc_dot = contributes ? acc : acc_dot;
+        last_dot = contributes ? col_dot : last_dot;
         last_alpha = contributes ? alpha : last_alpha;
         const float z = red.reduce(g0, g1, g2, g3, g4, g5, g6, g7, g8);
         if (slot >= 0) atomicAdd(&s_grad[j * kGrad + slot], z);
