@@ -103,6 +103,29 @@ __device__ __forceinline__ void add4(float4& a, float x, float y, float z, float
 constexpr int kShGroup = 5;
 constexpr int kShGroups = (GS_HIP_MAX_SH_REST + kShGroup - 1) / kShGroup;
 
+// basis_{K+1}(direction) of SH rest coefficient K (0..14): the terms of
+// sh_basis (gs_common.h), one at a time.
+template <int K>
+__device__ __forceinline__ float sh_rest_basis(float x, float y, float z) {
+#pragma clang fp contract(off)
+  const float xx = x * x, yy = y * y, zz = z * z;
+  if constexpr (K == 0) return -0.4886025119029199f * y;
+  else if constexpr (K == 1) return 0.4886025119029199f * z;
+  else if constexpr (K == 2) return -0.4886025119029199f * x;
+  else if constexpr (K == 3) return 1.0925484305920792f * x * y;
+  else if constexpr (K == 4) return -1.0925484305920792f * y * z;
+  else if constexpr (K == 5) return 0.31539156525252005f * (2.0f * zz - xx - yy);
+  else if constexpr (K == 6) return -1.0925484305920792f * x * z;
+  else if constexpr (K == 7) return 0.5462742152960396f * (xx - yy);
+  else if constexpr (K == 8) return -0.5900435899266435f * y * (3.0f * xx - yy);
+  else if constexpr (K == 9) return 2.890611442640554f * x * y * z;
+  else if constexpr (K == 10) return -0.4570457994644658f * y * (4.0f * zz - xx - yy);
+  else if constexpr (K == 11) return 0.3731763325901154f * z * (2.0f * zz - 3.0f * xx - 3.0f * yy);
+  else if constexpr (K == 12) return -0.4570457994644658f * x * (4.0f * zz - xx - yy);
+  else if constexpr (K == 13) return 1.445305721320277f * z * (xx - yy);
+  else return -0.5900435899266435f * x * (xx - 3.0f * yy);
+}
+
 // d basis_{K+1} / d(direction) of SH rest coefficient K (0..14).
 template <int K>
 __device__ __forceinline__ void sh_rest_dbasis(float x, float y, float z, float& gx, float& gy, float& gz) {
@@ -398,23 +421,24 @@ __device__ __forceinline__ void sh_group(const AdamArgs& a, const gs_hip_frame& 
   const bool sparse = (f.flags & GS_HIP_FLAG_SPARSE_ADAM) != 0u;
   bool drew = false;
   float dpx = 0.0f, dpy = 0.0f, dpz = 0.0f;
-  // Every view's colour gradient and clamp bits are loaded before any is
-  // used (GRAD2D is zero where the view does not see the slot: the
+  // Each view's colour gradient and clamp bits, requested two views ahead of
+  // their use (GRAD2D is zero where the view does not see the slot: the
   // projection clears it, so no tile-count test is needed).
-  float4 gv[GS_HIP_MAX_VIEWS];
-#pragma unroll
-  for (uint32_t v = 0; v < GS_HIP_MAX_VIEWS; ++v) {
-    if (v >= a.count) break;
+  auto fetch = [&](uint32_t v) {
+    if (v >= a.count) return make_float4(0, 0, 0, 0);
     const gsh::ViewDev& view = a.views[v];
     const GS_GLOBAL float* g2 = gmem(view.grad2d);
-    gv[v] = make_float4(g2[size_t(6) * a.capacity + i], g2[size_t(7) * a.capacity + i], g2[size_t(8) * a.capacity + i],
-                        __uint_as_float(gmem(view.misc)[i]));
-  }
+    return make_float4(g2[size_t(6) * a.capacity + i], g2[size_t(7) * a.capacity + i], g2[size_t(8) * a.capacity + i],
+                       __uint_as_float(gmem(view.misc)[i]));
+  };
+  float4 ahead0 = fetch(0), ahead1 = fetch(1);
 #pragma unroll
   for (uint32_t v = 0; v < GS_HIP_MAX_VIEWS; ++v) {
     if (v >= a.count) break;
     const gsh::ViewDev& view = a.views[v];
-    const float4 gc = gv[v];
+    const float4 gc = ahead0;
+    ahead0 = ahead1;
+    ahead1 = fetch(v + 2);
     drew = drew || gc.x != 0.0f || gc.y != 0.0f || gc.z != 0.0f;
     if (sparse && !drew) {
       for (int c = 0; c < 6; ++c) drew = drew || gmem(view.grad2d)[size_t(c) * a.capacity + i] != 0.0f;
@@ -426,13 +450,12 @@ __device__ __forceinline__ void sh_group(const AdamArgs& a, const gs_hip_frame& 
     const float drz = p0.z - view.f.cam.position[2];
     const float dlen = fmaxf(sqrtf(drx * drx + dry * dry + drz * drz), 1e-12f);
     const float x = drx / dlen, y = dry / dlen, z = drz / dlen;
-    float basis[16];
-    sh_basis(x, y, z, f.sh_degree, basis);
     float ddx = 0.0f, ddy = 0.0f, ddz = 0.0f;
     auto term = [&](auto kc) {
       constexpr int d = decltype(kc)::value;
       if (k0 + d >= coeffs) return;
-      for (int c = 0; c < 3; ++c) g[d][c] += basis[k0 + d + 1] * dcol[c];
+      const float b = sh_rest_basis<k0 + d>(x, y, z);
+      for (int c = 0; c < 3; ++c) g[d][c] += b * dcol[c];
       const float w = rest[d][0] * dcol[0] + rest[d][1] * dcol[1] + rest[d][2] * dcol[2];
       float gx, gy, gz;
       sh_rest_dbasis<k0 + d>(x, y, z, gx, gy, gz);
