@@ -40,7 +40,7 @@ extern "C" {
 #define GS_HIP_MAX_SH_REST 15     /* degree 3: 15 coefficients beyond DC */
 #define GS_HIP_PARAM_FLOATS 16    /* per slot: [pos, logit | logscale, alive | quat wxyz | dc, -] */
 #define GS_HIP_RECORD_FLOATS 12   /* projected raster record per slot */
-#define GS_HIP_GRAD2D_FLOATS 9    /* dL/d(mean xy, conic abc, opacity, colour rgb), planar */
+#define GS_HIP_GRAD2D_FLOATS 9    /* dL/d(mean xy, conic abc, opacity, colour rgb), per slot */
 #define GS_HIP_MAX_VIEWS 8        /* views of one batched optimizer step */
 
 /* frame flags */
@@ -119,7 +119,7 @@ typedef struct gs_hip_workspace gs_hip_workspace;
  *  TILES, DEPTH_KEYS, DEPTH_ORDER, OFFSETS  uint32[capacity]  (OFFSETS in depth order; with
  *                               GS_HIP_FLAG_DEPTH16 a key is the 16-bit depth | tiles << 16, sorted by
  *                               its lower half, 0xffff for a slot that touches no tile)
- *  GRAD2D                       float[9][capacity]  (planar)
+ *  GRAD2D                       float[capacity][9]  (the nine sums of a slot together)
  *  PAIR_TILES, PAIR_GAUSSIANS   uint32[pair_capacity] (sorted)
  *  RANGES                       uint32[tiles + 1][2]
  *  IMAGE                        float[H*W][4]  (rendered rgb, final transmittance)

@@ -102,7 +102,7 @@ struct ViewDev {
   uint32_t* depth_keys;
   uint32_t* depth_order;
   uint32_t* offsets;
-  float* grad2d;  // [9][capacity]
+  float* grad2d;  // [capacity][9]
   // per pair
   uint32_t* pair_tiles;
   uint32_t* pair_gaussians;
@@ -118,7 +118,7 @@ struct ViewDev {
   float4* pose_grad;  // [capacity / 256][2]: pose gradient partials (GS_HIP_FLAG_POSE_GRAD)
   uint32_t* totals;  // [0] pairs, [1] emit mismatches, [5..7] densify counters, [8] visible splats (viewer)
   uint64_t pair_capacity;
-  uint32_t capacity;    // the scene's slot capacity: plane stride of grad2d
+  uint32_t capacity;    // the scene's slot capacity
   uint32_t tile_count;  // tiles_x * tiles_y
 };
 

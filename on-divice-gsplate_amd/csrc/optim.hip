@@ -316,7 +316,7 @@ __global__ void __launch_bounds__(256) k_adam(const gsh::SceneDev* __restrict__ 
     float g[9];
     const GS_GLOBAL float* g2 = gmem(view.grad2d);
     for (int c = 0; c < 9; ++c) {
-      g[c] = g2[size_t(c) * a.capacity + i];
+      g[c] = g2[size_t(i) * 9u + c];
       drew = drew || g[c] != 0.0f;
     }
     // Densification statistics: view-space positional gradient in NDC units.
@@ -374,7 +374,7 @@ __global__ void __launch_bounds__(256) k_pose_grad(const gsh::SceneDev* __restri
     float pose[6] = {0, 0, 0, 0, 0, 0};
     if (live && view.tiles[i] != 0u) {
       float g[9];
-      for (int c = 0; c < 9; ++c) g[c] = view.grad2d[size_t(c) * a.capacity + i];
+      for (int c = 0; c < 9; ++c) g[c] = gmem(view.grad2d)[size_t(i) * 9u + c];
       float4 dp0 = make_float4(0, 0, 0, 0), dp1 = dp0, dq = dp0, dp3 = dp0;
       adjoint_view<true>(view.f.cam, g, 0u, p0, p1, q, dp0, dp1, dq, dp3, pose);
     }
@@ -428,7 +428,7 @@ __device__ __forceinline__ void sh_group(const AdamArgs& a, const gs_hip_frame& 
     if (v >= a.count) return make_float4(0, 0, 0, 0);
     const gsh::ViewDev& view = a.views[v];
     const GS_GLOBAL float* g2 = gmem(view.grad2d);
-    return make_float4(g2[size_t(6) * a.capacity + i], g2[size_t(7) * a.capacity + i], g2[size_t(8) * a.capacity + i],
+    return make_float4(g2[size_t(i) * 9u + 6u], g2[size_t(i) * 9u + 7u], g2[size_t(i) * 9u + 8u],
                        __uint_as_float(gmem(view.misc)[i]));
   };
   float4 ahead0 = fetch(0), ahead1 = fetch(1);
@@ -441,7 +441,7 @@ __device__ __forceinline__ void sh_group(const AdamArgs& a, const gs_hip_frame& 
     ahead1 = fetch(v + 2);
     drew = drew || gc.x != 0.0f || gc.y != 0.0f || gc.z != 0.0f;
     if (sparse && !drew) {
-      for (int c = 0; c < 6; ++c) drew = drew || gmem(view.grad2d)[size_t(c) * a.capacity + i] != 0.0f;
+      for (int c = 0; c < 6; ++c) drew = drew || gmem(view.grad2d)[size_t(i) * 9u + c] != 0.0f;
     }
     const uint32_t clamped = __float_as_uint(gc.w) & 7u;
     const float dcol[3] = {clamped & 1u ? 0.0f : gc.x, clamped & 2u ? 0.0f : gc.y, clamped & 4u ? 0.0f : gc.z};

@@ -8,7 +8,7 @@
 // wave reads 768 B contiguous per coefficient) are read once per step, not
 // once per view.  Per view it writes a 48 B raster record, a 32 B footprint
 // (all the pair emitter reads of a slot), tiles, misc, key and order, and
-// clears the nine planar GRAD2D accumulators.
+// clears the slot's nine GRAD2D accumulators.
 // HBM-bound.
 #include "gs_common.h"
 #include "gs_internal.h"
@@ -34,7 +34,7 @@ __device__ __forceinline__ void project_view(const gsh::ViewDev& vd, const Proje
                                              const float (&rest)[GS_HIP_MAX_SH_REST][3]) {
 #pragma clang fp contract(off)
   const gs_hip_frame& f = vd.f;
-  for (int c = 0; c < GS_HIP_GRAD2D_FLOATS; ++c) gmem(vd.grad2d)[size_t(c) * a.capacity + i] = 0.0f;
+  for (int c = 0; c < GS_HIP_GRAD2D_FLOATS; ++c) gmem(vd.grad2d)[size_t(i) * GS_HIP_GRAD2D_FLOATS + c] = 0.0f;
   const float* R = f.cam.rotation;
   // Camera-space position, left to right sums.
   const float tx = R[0] * p0.x + R[1] * p0.y + R[2] * p0.z + f.cam.translation[0];

@@ -24,7 +24,10 @@
 // per pixel is stored by the forward besides T, LAST and dL/dC.  The nine
 // per-splat gradients are summed over each wave by a transposed reduction
 // (one quad ends with each value), over the four waves in LDS (ds_add_f32),
-// and added to the slot's planar GRAD2D with one global float atomic each.
+// and added to the slot's GRAD2D with one global float atomic each.  GRAD2D
+// keeps a slot's nine sums together and the flush gives consecutive lanes
+// consecutive (splat, value) entries, so one atomic instruction covers about
+// seven slots' cache lines instead of 64.
 //
 // Work: the (view, tile) items of all views of the step in longest-list-first
 // order.  Either one workgroup per item (the hardware dispatches them in
@@ -222,7 +225,7 @@ struct BackwardShared {
 
 // Recompute-transmittance adjoint of one tile of one view.
 __device__ __forceinline__ void backward_tile(const gsh::ViewDev& a, uint32_t tile, BackwardShared& sh,
-                                              uint32_t capacity, const Reduce9& red, int slot) {
+                                              const Reduce9& red, int slot) {
   const gs_hip_frame& f = a.f;
   Staging& s = sh.s;
   uint32_t* s_g = sh.g;
@@ -312,12 +315,10 @@ __device__ __forceinline__ void backward_tile(const gsh::ViewDev& a, uint32_t ti
       }
     }
     __syncthreads();
-    if (threadIdx.x < n) {
-      const uint32_t g = s_g[threadIdx.x];
-      for (int c = 0; c < kGrad; ++c) {
-        const float v = s_grad[threadIdx.x * kGrad + c];
-        if (v != 0.0f) global_add(gmem(a.grad2d) + size_t(c) * capacity + g, v);
-      }
+    for (uint32_t e = threadIdx.x; e < n * uint32_t(kGrad); e += kThreads) {
+      const float v = s_grad[e];
+      const uint32_t j = e / uint32_t(kGrad);
+      if (v != 0.0f) global_add(gmem(a.grad2d) + size_t(s_g[j]) * kGrad + (e - j * uint32_t(kGrad)), v);
     }
     __syncthreads();
     bend = bstart;
@@ -357,7 +358,7 @@ __global__ void __launch_bounds__(kThreads) k_backward(RasterArgs a) {
   while (item < a.items) {
     const uint32_t w = a.work[item];
     const gsh::ViewDev& view = a.views[w >> gsh::kTileBits];
-    backward_tile(view, w & gsh::kMaxTilesPerView, sh, view.capacity, red, slot);
+    backward_tile(view, w & gsh::kMaxTilesPerView, sh, red, slot);
     if (a.persistent == 0u) break;
     __syncthreads();
     item = next_item(a, &s_item);

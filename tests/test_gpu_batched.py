@@ -44,7 +44,7 @@ def step_state(ws, n, views):
             "ranges": ws.download(gs.BUF_RANGES, np.uint32, (T + 1) * 2, view=v),
             "image": ws.download(gs.BUF_IMAGE, np.float32, view=v),
             "last": ws.download(gs.BUF_LAST, np.uint32, view=v),
-            "grad2d": ws.download(gs.BUF_GRAD2D, np.float32, 9 * n, view=v).reshape(9, n),
+            "grad2d": ws.download(gs.BUF_GRAD2D, np.float32, 9 * n, view=v).reshape(n, 9).T,
         })
     return out
 
@@ -158,7 +158,7 @@ def test_pair_overflow_retry_matches_a_step_with_room():
                 frame = gs.make_frame(cam, n, deg, tp=train_params(), step=1)
                 pairs, losses = ws.train_views([frame], [target.ptr])
                 results.append((pairs[0], losses[0], ws.store(n)[0], ws.overflows(),
-                                ws.download(gs.BUF_GRAD2D, np.float32, 9 * n).reshape(9, n)))
+                                ws.download(gs.BUF_GRAD2D, np.float32, 9 * n).reshape(n, 9).T))
     finally:
         target.free()
     (p0, l0, a0, o0, g0), (p1, l1, a1, o1, g1) = results

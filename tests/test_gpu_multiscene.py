@@ -60,7 +60,7 @@ def view_state(ws, n, views):
             "ranges": ws.download(gs.BUF_RANGES, np.uint32, (T + 1) * 2, view=v),
             "last": ws.download(gs.BUF_LAST, np.uint32, view=v),
             "image": ws.download(gs.BUF_IMAGE, np.float32, view=v).view(np.uint32),
-            "grad2d": ws.download(gs.BUF_GRAD2D, np.float32, 9 * n, view=v).reshape(9, n),
+            "grad2d": ws.download(gs.BUF_GRAD2D, np.float32, 9 * n, view=v).reshape(n, 9).T,
         })
     return out
 

@@ -165,7 +165,7 @@ def test_forward_loss_backward_match_oracle(name, n, deg, w, h, seed):
         last = ws.download(gs.BUF_LAST, np.uint32, w * h)
         dldc = ws.download(gs.BUF_DLDC, np.float32).reshape(-1, 4)
         ws.backward(frame)
-        g2 = ws.download(gs.BUF_GRAD2D, np.float32, 9 * n).reshape(9, n).T
+        g2 = ws.download(gs.BUF_GRAD2D, np.float32, 9 * n).reshape(n, 9)
     # forward: the same records and pairs through the oracle's compositor
     # Pixels that meet a splat at a skip/clamp/stop threshold (within 1e-5
     # relative) may take the other branch under another fp32 evaluation
@@ -231,7 +231,7 @@ def test_adjoint_matches_oracle():
         ws.set_target(target)
         frame = gs.make_frame(cam, n, deg, tp=frozen())
         ws.train_step(frame)
-        g2 = ws.download(gs.BUF_GRAD2D, np.float32, 9 * n).reshape(9, n).T.astype(np.float64)
+        g2 = ws.download(gs.BUF_GRAD2D, np.float32, 9 * n).reshape(n, 9).astype(np.float64)
         gp, gsh = _first_step_gradients(ws, n)
         after = ws.store(n)[0]
     np.testing.assert_array_equal(after, params)  # lr 0: parameters stay put
@@ -407,7 +407,7 @@ def test_train_views_matches_single_view_steps():
             ws.load(params, sh)
             ws.set_target(targets[v])
             _, losses = ws.train_step(gs.make_frame(cams[v], n, deg, tp=frozen()))
-            s_g2.append(ws.download(gs.BUF_GRAD2D, np.float32, 9 * n).reshape(9, n).T)
+            s_g2.append(ws.download(gs.BUF_GRAD2D, np.float32, 9 * n).reshape(n, 9))
             gp, gsh = _first_step_gradients(ws, n)
             s_gp.append(gp)
             s_gsh.append(gsh)
@@ -421,7 +421,7 @@ def test_train_views_matches_single_view_steps():
             frames = [gs.make_frame(c, n, deg, tp=frozen()) for c in cams]
             pairs, losses = ws.train_views(frames, [b.ptr for b in bufs])
             for v in range(V):
-                g2 = ws.download(gs.BUF_GRAD2D, np.float32, 9 * n, view=v).reshape(9, n).T
+                g2 = ws.download(gs.BUF_GRAD2D, np.float32, 9 * n, view=v).reshape(n, 9)
                 scale = np.abs(s_g2[v]).max(axis=0) + 1e-30
                 assert (np.abs(g2 - s_g2[v]).max(axis=0) / scale).max() < 1e-5, v
                 assert losses[v] == pytest.approx(s_loss[v], rel=1e-6)

@@ -57,7 +57,7 @@ def test_pose_gradient_matches_oracle(deg, lam):
         frame = gs.make_frame(cam, n, deg, background=(0.2, 0.3, 0.4), flags=gs.FLAG_POSE_GRAD, tp=frozen(lam))
         ws.train_step(frame)
         got = ws.pose_gradient()
-        g2 = ws.download(gs.BUF_GRAD2D, np.float32, 9 * n).reshape(9, n).T.astype(np.float64)
+        g2 = ws.download(gs.BUF_GRAD2D, np.float32, 9 * n).reshape(n, 9).astype(np.float64)
         m = ws.download(gs.BUF_M_PARAMS, np.float32, n * 16).reshape(n, 16) / (1 - np.float32(0.9))
         # a pose-only step: the same gradient, the model and moments left alone
         p_before = ws.store(n)[0]
