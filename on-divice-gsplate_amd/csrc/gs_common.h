@@ -41,6 +41,11 @@ __device__ __forceinline__ float4 gload(const float4* p) {
   const gs_f4v v = *(const GS_GLOBAL gs_f4v*)p;
   return make_float4(v.x, v.y, v.z, v.w);
 }
+typedef unsigned int gs_u4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint4 gload(const uint4* p) {
+  const gs_u4v v = *(const GS_GLOBAL gs_u4v*)p;
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
 __device__ __forceinline__ uint2 gload(const uint2* p) {
   const gs_u2v v = *(const GS_GLOBAL gs_u2v*)p;
   return make_uint2(v.x, v.y);

@@ -62,9 +62,9 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t digit, int nbits, uint6
 }
 
 // Histogram of block (blockIdx.x) of segment blockIdx.y; blocks past the
-// segment's length write zeros.  Each wave counts a contiguous 1024 items;
-// the lanes holding the same digit add their count once (no LDS atomics
-// piling on one counter when neighbouring keys share a digit).
+// segment's length write zeros.  Each wave counts a contiguous 1024 items
+// into its own LDS histogram with one LDS atomic per item (a count does not
+// depend on the order of the additions).
 __global__ void __launch_bounds__(kSortThreads) k_radix_upsweep(const SortSeg* __restrict__ segs, int shift,
                                                                 int nbits, int from_alt) {
   __shared__ uint32_t h[kWaves][256];
@@ -81,8 +81,7 @@ __global__ void __launch_bounds__(kSortThreads) k_radix_upsweep(const SortSeg* _
   for (int d = threadIdx.x; d < kWaves * 256; d += kSortThreads) (&h[0][0])[d] = 0u;
   __syncthreads();
   const uint32_t wbase = blockIdx.x * kSortItems + wave * (kSortItems / kWaves);
-  const uint64_t below_me = lanemask_lt();
-  uint32_t kv[kSortRounds];  // every round's key requested before any is ranked
+  uint32_t kv[kSortRounds];  // every round's key requested before any is counted
 #pragma unroll
   for (int r = 0; r < kSortRounds; ++r) {
     const uint32_t idx = wbase + r * 64 + lane;
@@ -90,11 +89,7 @@ __global__ void __launch_bounds__(kSortThreads) k_radix_upsweep(const SortSeg* _
   }
 #pragma unroll
   for (int r = 0; r < kSortRounds; ++r) {
-    const uint32_t idx = wbase + r * 64 + lane;
-    const bool valid = idx < n;
-    const uint32_t digit = valid ? (kv[r] >> shift) & mask : 0u;
-    const uint64_t peers = match_digit(digit, nbits, __ballot(valid));
-    if (valid && (peers & below_me) == 0ull) h[wave][digit] += __popcll(peers);
+    if (wbase + r * 64 + lane < n) atomicAdd(&h[wave][(kv[r] >> shift) & mask], 1u);
   }
   __syncthreads();
   for (uint32_t d = threadIdx.x; d < radix; d += kSortThreads) {
@@ -516,20 +511,35 @@ __global__ void __launch_bounds__(256) k_clear_ranges(const ViewDev* __restrict_
 __global__ void __launch_bounds__(256) k_ranges(const ViewDev* __restrict__ views) {
   const ViewDev& v = views[blockIdx.y];
   const uint32_t count = uint32_t(std::min<uint64_t>(v.totals[0], v.pair_capacity));
-  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < count; s += gridDim.x * blockDim.x) {
-    const GS_GLOBAL uint32_t* pt = gmem(v.pair_tiles);
-    GS_GLOBAL uint32_t* rg = gmem(reinterpret_cast<uint32_t*>(v.ranges));  // [tile][2]: first, end
-    const uint32_t tile = pt[s];
-    if (s == 0u) {
-      rg[size_t(tile) * 2u] = 0u;
+  const uint32_t* pt_flat = v.pair_tiles;
+  const GS_GLOBAL uint32_t* pt = gmem(pt_flat);
+  GS_GLOBAL uint32_t* rg = gmem(reinterpret_cast<uint32_t*>(v.ranges));  // [tile][2]: first, end
+  // Four consecutive pairs per thread and iteration (one 16 B load; the
+  // buffer is 256 B aligned), plus the pair before them.
+  const uint32_t quads = (count + 3u) / 4u;
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < quads; q += gridDim.x * blockDim.x) {
+    const uint32_t s0 = q * 4u;
+    uint32_t t[4];
+    if (s0 + 4u <= count) {
+      const uint4 w = gload(reinterpret_cast<const uint4*>(pt_flat + s0));
+      t[0] = w.x, t[1] = w.y, t[2] = w.z, t[3] = w.w;
     } else {
-      const uint32_t prev = pt[s - 1u];
-      if (prev != tile) {
+      for (int i = 0; i < 4; ++i) t[i] = s0 + i < count ? pt[s0 + i] : 0u;
+    }
+    uint32_t prev = s0 > 0u ? pt[s0 - 1u] : 0u;
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t s = s0 + uint32_t(i);
+      if (s >= count) break;
+      const uint32_t tile = t[i];
+      if (s == 0u) {
+        rg[size_t(tile) * 2u] = 0u;
+      } else if (prev != tile) {
         rg[size_t(prev) * 2u + 1u] = s;
         rg[size_t(tile) * 2u] = s;
       }
+      if (s == count - 1u) rg[size_t(tile) * 2u + 1u] = count;
+      prev = tile;
     }
-    if (s == count - 1u) rg[size_t(tile) * 2u + 1u] = count;
   }
 }
 
