@@ -76,12 +76,14 @@ __device__ __forceinline__ PixelCoord pixel_coord(const gs_hip_frame& f, uint32_
   return c;
 }
 
-// One batch of splats in LDS, in evaluation form.
+// One batch of splats in LDS, in evaluation form: three float4 per splat,
+// so one address register and three immediate offsets reach a splat.
 struct Staging {
-  float4 a[kBatch];  // mean x, mean y, A = -a/2, B = -b
-  float4 b[kBatch];  // C = -c/2, opacity, r, g
-  float blue[kBatch];
+  float4 sp[kBatch * 3];  // [3j]: mean x, mean y, A = -a/2, B = -b; [3j + 1]: C = -c/2, opacity, r, g; [3j + 2].x: b
   uint64_t mask[4][kBatch / 64];  // [quadrant][loading wave]
+  __device__ __forceinline__ float4 a(int j) const { return sp[3 * j]; }
+  __device__ __forceinline__ float4 b(int j) const { return sp[3 * j + 1]; }
+  __device__ __forceinline__ float blue(int j) const { return sp[3 * j + 2].x; }
 };
 
 // Smallest q(u, v) = a u^2 / 2 + b u v + c v^2 / 2 (a, c > 0, ac > b^2) over
@@ -106,9 +108,9 @@ __device__ __forceinline__ uint32_t stage(Staging& s, const float4* __restrict__
                                           int ty0) {
   const float4* r = records + size_t(g) * 3u;
   const float4 ra = gload(r), rb = gload(r + 1), rc = gload(r + 2);
-  s.a[t] = make_float4(ra.x, ra.y, -0.5f * ra.z, -ra.w);
-  s.b[t] = make_float4(-0.5f * rb.x, rb.y, rb.z, rb.w);
-  s.blue[t] = rc.x;
+  s.sp[3 * t] = make_float4(ra.x, ra.y, -0.5f * ra.z, -ra.w);
+  s.sp[3 * t + 1] = make_float4(-0.5f * rb.x, rb.y, rb.z, rb.w);
+  s.sp[3 * t + 2].x = rc.x;
   const uint32_t lo = __float_as_uint(rc.y), hi = __float_as_uint(rc.z);
   uint32_t hits = 0u;
   for (int q = 0; q < 4; ++q) {
@@ -163,12 +165,14 @@ __device__ __forceinline__ void forward_tile(const gsh::ViewDev& a, uint32_t til
   const PixelCoord pc = pixel_coord(f, tile);
   const uint2 range = gload(a.ranges + pc.tile);
   const float cx = float(pc.px) + 0.5f, cy = float(pc.py) + 0.5f;
-  bool done = !pc.inside;
-  float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
+  // Ts is T while the pixel composites and -T once it has stopped (or lies
+  // outside the image): "not done" is Ts > 0, one compare whose lane mask is
+  // the wave's ballot.
+  float Ts = pc.inside ? 1.0f : -1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
   uint32_t last = 0u;
 
   for (uint32_t start = range.x; start < range.y; start += kBatch) {
-    if (__syncthreads_and(done)) break;
+    if (__syncthreads_and(Ts < 0.0f)) break;
     const uint32_t n = min(uint32_t(kBatch), range.y - start);
     uint32_t hits = 0u;
     if (threadIdx.x < n) {
@@ -176,39 +180,35 @@ __device__ __forceinline__ void forward_tile(const gsh::ViewDev& a, uint32_t til
     }
     publish_masks(s, hits, pc.wave, pc.lane);
     __syncthreads();
-    if (__ballot(!done) != 0ull) {
-      const uint32_t base = start - range.x + 1u;  // LAST of splat j is base + j
-      for (int chunk = 0; chunk < kBatch / 64; ++chunk) {
-        uint64_t m = uniform_u64(s.mask[pc.wave][chunk]);
-        bool live = true;
-        while (m != 0ull) {
-          const int j = chunk * 64 + __builtin_ctzll(m);
-          m &= m - 1ull;
-          // Branch-free: lanes that are done or skip this splat add zero.
-          const float4 sb = s.b[j];
-          const float blue = s.blue[j];
-          const SplatEval e = eval_splat(s.a[j], sb, cx, cy);
-          const float next_T = T * (1.0f - e.alpha);
-          const bool ok = !done && e.power <= 0.0f && e.alpha >= kMinAlpha;
-          const bool stop = ok && next_T < kMinTransmittance;
-          const bool take = ok && !stop;
-          const float w = take ? e.alpha * T : 0.0f;
-          cr = fmaf(sb.z, w, cr);
-          cg = fmaf(sb.w, w, cg);
-          cb = fmaf(blue, w, cb);
-          T = take ? next_T : T;
-          last = take ? base + uint32_t(j) : last;
-          done = done || stop;
-          if (__ballot(!done) == 0ull) {
-            live = false;
-            break;
-          }
-        }
-        if (!live) break;
+    // The walk ends with the masks or when every pixel of the wave is done;
+    // both loop conditions are wave-uniform (scalar registers).
+    uint64_t alive = __ballot(Ts > 0.0f);
+    const uint32_t base = start - range.x + 1u;  // LAST of splat j is base + j
+    for (int chunk = 0; chunk < kBatch / 64 && alive != 0ull; ++chunk) {
+      uint64_t m = uniform_u64(s.mask[pc.wave][chunk]);
+      while (m != 0ull && alive != 0ull) {
+        const int j = chunk * 64 + __builtin_ctzll(m);
+        m &= m - 1ull;
+        // Branch-free: lanes that are done or skip this splat add zero.
+        const float4 sb = s.b(j);
+        const float blue = s.blue(j);
+        const SplatEval e = eval_splat(s.a(j), sb, cx, cy);
+        const float next_T = Ts * (1.0f - e.alpha);
+        const bool ok = Ts > 0.0f && e.power <= 0.0f && e.alpha >= kMinAlpha;
+        const bool stop = ok && next_T < kMinTransmittance;
+        const bool take = ok && !stop;
+        const float w = take ? e.alpha * Ts : 0.0f;
+        cr = fmaf(sb.z, w, cr);
+        cg = fmaf(sb.w, w, cg);
+        cb = fmaf(blue, w, cb);
+        last = take ? base + uint32_t(j) : last;
+        Ts = take ? next_T : (stop ? -Ts : Ts);
+        alive = __ballot(Ts > 0.0f);
       }
     }
     __syncthreads();
   }
+  const float T = fabsf(Ts);
   if (pc.inside) {
     const uint32_t pix = uint32_t(pc.py) * f.cam.width + uint32_t(pc.px);
     gstore(a.image + pix, make_float4(cr, cg, cb, T));
@@ -281,9 +281,9 @@ __device__ __forceinline__ void backward_tile(const gsh::ViewDev& a, uint32_t ti
         const int bit = 63 - __builtin_clzll(m);
         m ^= 1ull << bit;
         const int j = chunk * 64 + bit;
-        const float4 sa = s.a[j];
-        const float4 sb = s.b[j];
-        const float blue = s.blue[j];
+        const float4 sa = s.a(j);
+        const float4 sb = s.b(j);
+        const float blue = s.blue(j);
         const SplatEval e = eval_splat(sa, sb, cx, cy);
         const float alpha = e.alpha;
         const bool contributes = base + uint32_t(j) < last && e.power <= 0.0f && alpha >= kMinAlpha;
