@@ -77,10 +77,18 @@ __device__ __forceinline__ bool skip_update(const AdamArgs& a) {
   return false;
 }
 
+// The gradient side of the optimizer (adjoint, SH gradients, Adam's step)
+// is compared with the fp64 oracle within a tolerance, so it uses the
+// hardware reciprocal and square root (1 ulp) instead of the correctly
+// rounded sequences (about ten instructions each); the projection, whose
+// records are bit-exact with the oracle, keeps the exact ones.
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 __device__ __forceinline__ float adam1(float p, float g, float& m, float& v, float lr, const gs_hip_frame& f) {
   m = g * (1.0f - f.beta1) + m * f.beta1;
   v = g * g * (1.0f - f.beta2) + v * f.beta2;
-  return p - lr * (m * f.bias1) / (sqrtf(v * f.bias2) + f.epsilon);
+  return p - lr * (m * f.bias1) * fast_rcp(fast_sqrt(v * f.bias2) + f.epsilon);
 }
 
 __device__ __forceinline__ float4 adam4(float4 p, float4 g, float4* m_ptr, float4* v_ptr, float4 lr,
@@ -178,8 +186,9 @@ __device__ __forceinline__ void adjoint_view(const gs_hip_camera& cam, const flo
     T[0][c] = j00 * V[c] + j02 * V[6 + c];
     T[1][c] = j11 * V[3 + c] + j12 * V[6 + c];
   }
-  const float qlen = fmaxf(sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w), 1e-12f);
-  const float qw = q.x / qlen, qx = q.y / qlen, qy = q.z / qlen, qz = q.w / qlen;
+  const float qlen = fmaxf(fast_sqrt(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w), 1e-12f);
+  const float iq = fast_rcp(qlen);
+  const float qw = q.x * iq, qx = q.y * iq, qy = q.z * iq, qz = q.w * iq;
   float R[3][3];
   R[0][0] = 1.0f - 2.0f * (qy * qy + qz * qz), R[0][1] = 2.0f * (qx * qy - qw * qz), R[0][2] = 2.0f * (qx * qz + qw * qy);
   R[1][0] = 2.0f * (qx * qy + qw * qz), R[1][1] = 1.0f - 2.0f * (qx * qx + qz * qz), R[1][2] = 2.0f * (qy * qz - qw * qx);
@@ -204,12 +213,12 @@ __device__ __forceinline__ void adjoint_view(const gs_hip_camera& cam, const flo
        0.0f);
 
   // Opacity logit.
-  const float op = 1.0f / (1.0f + __expf(-p0.w));
+  const float op = fast_rcp(1.0f + __expf(-p0.w));
   const float dlogit = g[5] * op * (1.0f - op);
 
   // Conic (C/D, -B/D, A/D) -> dilated covariance.
   const float ga = g[2], gb = g[3], gc = g[4];
-  const float inv_d2 = 1.0f / (D * D);
+  const float inv_d2 = fast_rcp(D * D);
   const float gA = (-C * C * ga + B * C * gb - B * B * gc) * inv_d2;
   const float gB = (2.0f * B * C * ga - (D + 2.0f * B * B) * gb + 2.0f * A * B * gc) * inv_d2;
   const float gC = (-B * B * ga + A * B * gb - A * A * gc) * inv_d2;
@@ -285,7 +294,7 @@ __device__ __forceinline__ void adjoint_view(const gs_hip_camera& cam, const flo
   const float dz = 2.0f * (-2.0f * qz * gR[0][0] - qw * gR[0][1] + qx * gR[0][2] + qw * gR[1][0] -
                            2.0f * qz * gR[1][1] + qy * gR[1][2] + qx * gR[2][0] + qy * gR[2][1]);
   const float qd = qw * dw + qx * dx + qy * dy + qz * dz;
-  add4(dq, (dw - qw * qd) / qlen, (dx - qx * qd) / qlen, (dy - qy * qd) / qlen, (dz - qz * qd) / qlen);
+  add4(dq, (dw - qw * qd) * iq, (dx - qx * qd) * iq, (dy - qy * qd) * iq, (dz - qz * qd) * iq);
 }
 
 // Adjoint of every view in `a.views`, summed and averaged, then one Adam step
@@ -448,8 +457,9 @@ __device__ __forceinline__ void sh_group(const AdamArgs& a, const gs_hip_frame& 
     if (dcol[0] == 0.0f && dcol[1] == 0.0f && dcol[2] == 0.0f) continue;
     const float drx = p0.x - view.f.cam.position[0], dry = p0.y - view.f.cam.position[1];
     const float drz = p0.z - view.f.cam.position[2];
-    const float dlen = fmaxf(sqrtf(drx * drx + dry * dry + drz * drz), 1e-12f);
-    const float x = drx / dlen, y = dry / dlen, z = drz / dlen;
+    const float dlen = fmaxf(fast_sqrt(drx * drx + dry * dry + drz * drz), 1e-12f);
+    const float il = fast_rcp(dlen);
+    const float x = drx * il, y = dry * il, z = drz * il;
     float ddx = 0.0f, ddy = 0.0f, ddz = 0.0f;
     auto term = [&](auto kc) {
       constexpr int d = decltype(kc)::value;
@@ -469,9 +479,9 @@ __device__ __forceinline__ void sh_group(const AdamArgs& a, const gs_hip_frame& 
     term(std::integral_constant<int, 3>{});
     term(std::integral_constant<int, 4>{});
     const float dd = x * ddx + y * ddy + z * ddz;
-    dpx += (ddx - x * dd) / dlen;
-    dpy += (ddy - y * dd) / dlen;
-    dpz += (ddz - z * dd) / dlen;
+    dpx += (ddx - x * dd) * il;
+    dpy += (ddy - y * dd) * il;
+    dpz += (ddz - z * dd) * il;
   }
   gstore(a.dir_grad + size_t(kGroup) * a.capacity + i, make_float4(dpx, dpy, dpz, 0.0f));
   if (sparse && !drew) return;  // kept by sparse Adam
