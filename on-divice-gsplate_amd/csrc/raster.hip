@@ -246,11 +246,11 @@ __device__ __forceinline__ void backward_tile(const gsh::ViewDev& a, uint32_t ti
   }
   // T_final/(1 - alpha) times bg . dL/dC: the background seen through the splat.
   const float bg_term = T_final * (f.background[0] * gr + f.background[1] * gg + f.background[2] * gb);
-  // Back-to-front state: T in front of the current splat, and the colour
-  // behind it carried as its dot product with dL/dC (dL/dC is constant along
-  // the walk, so acc' = a c + (1 - a) acc gives acc'.g = a (c.g) + (1 - a) acc.g:
+  // Back-to-front state: T behind the current splat, and the colour behind
+  // it carried as its dot product with dL/dC (dL/dC is constant along the
+  // walk, so acc' = a c + (1 - a) acc gives acc'.g = a (c.g) + (1 - a) acc.g:
   // one running value instead of three channels).
-  float T = T_final, acc_dot = 0.0f, last_alpha = 0.0f, last_dot = 0.0f;
+  float T = T_final, acc_dot = 0.0f;
 
   const uint32_t wl = wave_max_u32(last);
   if (pc.lane == 0) s_wave_last[pc.wave] = wl;
@@ -289,36 +289,40 @@ __device__ __forceinline__ void backward_tile(const gsh::ViewDev& a, uint32_t ti
         const bool contributes = base + uint32_t(j) < last && e.power <= 0.0f && alpha >= kMinAlpha;
         if (__ballot(contributes) == 0ull) continue;
         // Branch-free from here: lanes that do not contribute keep their
-        // state and add zeros (selects, so no 0 * inf can leak in).
+        // state and add zeros: their alpha counts as 0 where it scales, and
+        // selects keep exp's possible inf out of the rest.
+        const float a_eff = contributes ? alpha : 0.0f;
         const float inv = __builtin_amdgcn_rcpf(1.0f - alpha);
         const float T_in = T * inv;  // transmittance in front of this splat
         const float col_dot = fmaf(sb.z, gr, fmaf(sb.w, gg, blue * gb));  // colour . dL/dC
-        const float acc = fmaf(last_alpha, last_dot - acc_dot, acc_dot);    // (colour behind) . dL/dC
-        const float dL_dalpha = (col_dot - acc) * T_in - bg_term * inv;
+        const float dL_dalpha = (col_dot - acc_dot) * T_in - bg_term * inv;
         const bool unclamped = contributes && e.raw < kMaxAlpha;
-        const float wT = contributes ? alpha * T_in : 0.0f;
+        const float wT = a_eff * T_in;
         const float dpow = unclamped ? e.raw * dL_dalpha : 0.0f;
         const float g5 = unclamped ? e.G * dL_dalpha : 0.0f;
         const float dxp = dpow * e.dx;
         const float g0 = dpow * (fmaf(sa.w, e.dy, e.adx) + e.adx);  // dpow (2A dx + B dy)
         const float g1 = dpow * fmaf(sa.w, e.dx, 2.0f * e.cdy);     // dpow (B dx + 2C dy)
-        const float g2 = -0.5f * dxp * e.dx;
-        const float g3 = -dxp * e.dy;
-        const float g4 = -0.5f * dpow * e.dy * e.dy;
+        // dL/d(A, B, C) of power = A dx^2 + B dx dy + C dy^2; the flush scales
+        // them to the conic's a, b, c (A = -a/2, B = -b, C = -c/2), exactly.
+        const float g2 = dxp * e.dx;
+        const float g3 = dxp * e.dy;
+        const float g4 = (dpow * e.dy) * e.dy;
         const float g6 = wT * gr, g7 = wT * gg, g8 = wT * gb;
         T = contributes ? T_in : T;
-        acc_dot = contributes ? acc : acc_dot;
-        last_dot = contributes ? col_dot : last_dot;
-        last_alpha = contributes ? alpha : last_alpha;
+        acc_dot = fmaf(a_eff, col_dot - acc_dot, acc_dot);  // this splat joins the colour behind the next
         const float z = red.reduce(g0, g1, g2, g3, g4, g5, g6, g7, g8);
-        if (slot >= 0) atomicAdd(&s_grad[j * kGrad + slot], z);
+        if (slot >= 0) {
+          __hip_atomic_fetch_add((__attribute__((address_space(3))) float*)s_grad + (uint32_t(j) * kGrad + uint32_t(slot)),
+                                 z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
       }
     }
     __syncthreads();
     for (uint32_t e = threadIdx.x; e < n * uint32_t(kGrad); e += kThreads) {
-      const float v = s_grad[e];
-      const uint32_t j = e / uint32_t(kGrad);
-      if (v != 0.0f) global_add(gmem(a.grad2d) + size_t(s_g[j]) * kGrad + (e - j * uint32_t(kGrad)), v);
+      const uint32_t j = e / uint32_t(kGrad), c = e - j * uint32_t(kGrad);
+      const float v = s_grad[e] * (c == 3u ? -1.0f : (c == 2u || c == 4u ? -0.5f : 1.0f));
+      if (v != 0.0f) global_add(gmem(a.grad2d) + size_t(s_g[j]) * kGrad + c, v);
     }
     __syncthreads();
     bend = bstart;
