@@ -224,7 +224,8 @@ int gs_hip_train_step(gs_hip_workspace* ws, const gs_hip_frame* frame, uint64_t*
  * overflow the buffers leaves the model alone and runs again with larger
  * buffers (trainer.cpp:819-836).  frames[v] differ only in camera (the
  * optimizer reads frames[0]'s rates and step).  targets[v]: device RGBA8 of
- * view v, or NULL (targets or an entry) to use what view v's TARGET holds.
+ * view v, read in place while the call runs (TARGET is left as it was), or
+ * NULL (targets or an entry) to use what view v's TARGET holds.
  * pairs[count], losses[3 * count] may be NULL. */
 int gs_hip_train_views(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t count, const void* const* targets,
                        uint64_t* pairs, double* losses);

@@ -36,8 +36,15 @@ void timer_begin(gs_hip_workspace* ws, const char* name) {
   Timer& t = ws->timer;
   if (!t.enabled) return;
   if (t.used == t.start.size()) {
+    // Timing-only events: without the system-scope fence, recording one
+    // neither writes back and invalidates the caches nor opens a gap of
+    // about 10 us before the next kernel (elapsed times are read after a
+    // device synchronize).
     hipEvent_t a, b;
-    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+    if (hipEventCreateWithFlags(&a, hipEventDisableSystemFence) != hipSuccess ||
+        hipEventCreateWithFlags(&b, hipEventDisableSystemFence) != hipSuccess) {
+      return;
+    }
     t.start.push_back(a);
     t.stop.push_back(b);
     t.names.emplace_back();
@@ -378,10 +385,12 @@ int upload_entries(gs_hip_workspace* ws, const std::vector<TableEntry>& entries)
   return 0;
 }
 
-int upload_table(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t count) {
+int upload_table(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t count, const void* const* targets) {
   if (count == 0 || count > ws->views.size()) return set_error(GS_HIP_E_ARGUMENT, "view count out of range");
   std::vector<TableEntry> entries(count);
-  for (uint32_t v = 0; v < count; ++v) entries[v] = TableEntry{ws->views[v], &frames[v], ws, nullptr};
+  for (uint32_t v = 0; v < count; ++v) {
+    entries[v] = TableEntry{ws->views[v], &frames[v], ws, targets != nullptr ? targets[v] : nullptr};
+  }
   return upload_entries(ws, entries);
 }
 

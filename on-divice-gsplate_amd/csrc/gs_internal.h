@@ -272,7 +272,10 @@ int reserve_sort(View& v, uint64_t items);
 // Writes frames[0, count) into the device table for views [0, count)
 // (stream-ordered copy from a pinned staging slot) with every view's buffer
 // pointers, and the sort / scan segments when the buffers moved.
-int upload_table(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t count);
+// targets (may be NULL, as may its entries): device RGBA8 photos the loss
+// kernels read in place instead of the views' own TARGET buffers.
+int upload_table(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t count,
+                 const void* const* targets = nullptr);
 // The same for any entries (a batch's views of several scenes, grouped by
 // scene): sets ws->shape and ws->tile_base.
 int upload_entries(gs_hip_workspace* ws, const std::vector<TableEntry>& entries);

@@ -793,13 +793,7 @@ int run_step(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t count, c
   int rc = check_frames(ws, frames, count);
   if (rc != 0) return rc;
   const bool depth16 = (frames[0].flags & GS_HIP_FLAG_DEPTH16) != 0u;
-  const size_t image_bytes = size_t(ws->width) * ws->height * 4;
-  if (targets != nullptr) {
-    for (uint32_t v = 0; v < count; ++v) {
-      if (targets[v] == nullptr) continue;
-      GS_HIP_TRY(hipMemcpyAsync(ws->views[v]->target, targets[v], image_bytes, hipMemcpyDeviceToDevice, ws->stream));
-    }
-  }
+  // The caller's targets are read in place (through the view table), not copied.
   // Headroom over the largest count seen (trainer.cpp:588-592).
   const uint64_t wanted = ws->max_pairs_seen + ws->max_pairs_seen / 8;
   if (wanted > view0(ws).pair_capacity) {
@@ -807,7 +801,7 @@ int run_step(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t count, c
     if (rc != 0) return rc;
   }
   for (int attempt = 0;; ++attempt) {
-    rc = upload_table(ws, frames, count);
+    rc = upload_table(ws, frames, count, targets);
     if (rc != 0) return rc;
     rc = launch_step(ws, count, depth16, train, losses);
     if (rc != 0) return rc;
