@@ -64,7 +64,9 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t digit, int nbits, uint6
 // Histogram of block (blockIdx.x) of segment blockIdx.y; blocks past the
 // segment's length write zeros.  Each wave counts a contiguous 1024 items
 // into its own LDS histogram with one LDS atomic per item (a count does not
-// depend on the order of the additions).
+// depend on the order of the additions), or one per round when the round's
+// items share their digit (the top digit of a depth key mostly does: 64
+// lanes on one counter would take 64 passes).
 __global__ void __launch_bounds__(kSortThreads) k_radix_upsweep(const SortSeg* __restrict__ segs, int shift,
                                                                 int nbits, int from_alt) {
   __shared__ uint32_t h[kWaves][256];
@@ -89,7 +91,16 @@ __global__ void __launch_bounds__(kSortThreads) k_radix_upsweep(const SortSeg* _
   }
 #pragma unroll
   for (int r = 0; r < kSortRounds; ++r) {
-    if (wbase + r * 64 + lane < n) atomicAdd(&h[wave][(kv[r] >> shift) & mask], 1u);
+    const bool valid = wbase + r * 64 + lane < n;
+    const uint32_t digit = (kv[r] >> shift) & mask;
+    const uint64_t active = __ballot(valid);
+    if (active == 0ull) continue;
+    const uint32_t first = __builtin_amdgcn_readfirstlane(valid ? digit : 0xffffffffu);
+    if (__ballot(valid && digit == first) == active) {
+      if (lane == __ffsll(static_cast<long long>(active)) - 1) atomicAdd(&h[wave][first], uint32_t(__popcll(active)));
+    } else if (valid) {
+      atomicAdd(&h[wave][digit], 1u);
+    }
   }
   __syncthreads();
   for (uint32_t d = threadIdx.x; d < radix; d += kSortThreads) {
