@@ -270,15 +270,21 @@ __global__ void __launch_bounds__(256) k_loss_grad(const gsh::ViewDev* __restric
     int id[kHaloIters];
 #pragma unroll
     for (int it = 0; it < kHaloIters; ++it) id[it] = halo_pixel(f, int(threadIdx.x) + it * 256, ox, oy);
-    for (int c = 0; c < 3; ++c) {
-      // The channel's nine halo values per thread, all requested before any is stored.
-      float mv[kHaloIters][3];
+    // A channel's nine halo values per thread, all requested at once; the
+    // next channel's are requested before this channel's blurs, so their
+    // latency passes under that work.
+    float mv[kHaloIters][3];
+    auto request = [&](int c) {
 #pragma unroll
       for (int it = 0; it < kHaloIters; ++it) {
         for (int q = 0; q < 3; ++q) {
           mv[it][q] = id[it] >= 0 ? gmem(a.maps)[size_t(3 * q + c) * pixels + id[it]] : 0.0f;
         }
       }
+    };
+    request(0);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
 #pragma unroll
       for (int it = 0; it < kHaloIters; ++it) {
         const int e = int(threadIdx.x) + it * 256;
@@ -287,6 +293,7 @@ __global__ void __launch_bounds__(256) k_loss_grad(const gsh::ViewDev* __restric
         for (int q = 0; q < 3; ++q) s_m[q][at] = mv[it][q];
       }
       __syncthreads();
+      if (c + 1 < 3) request(c + 1);
       for (int e = threadIdx.x; e < kHalo * kTile; e += 256) {
         const int row = e / kTile, col = e % kTile;
         for (int q = 0; q < 3; ++q) {
