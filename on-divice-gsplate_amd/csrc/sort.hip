@@ -565,11 +565,17 @@ __global__ void __launch_bounds__(1024) k_work_order(const ViewDev* __restrict__
   if (threadIdx.x < 33) s_count[threadIdx.x] = 0u;
   if (threadIdx.x < 2) counters[threadIdx.x] = 0u;
   __syncthreads();
+  // Most tiles fall into a few buckets, so the lanes of a wave that share
+  // one add their count once (an LDS atomic from 64 lanes on one counter
+  // would take 64 passes).
+  const uint64_t below_me = lanemask_lt();
   for (uint32_t v = 0; v < count; ++v) {
     const ViewDev& view = views[v];
     for (uint32_t t = threadIdx.x; t < view.tile_count; t += blockDim.x) {
       const uint2 r = view.ranges[t];
-      atomicAdd(&s_count[__clz(r.y - r.x)], 1u);
+      const uint32_t b = __clz(r.y - r.x);
+      const uint64_t peers = match_digit(b, 6, __ballot(true));
+      if ((peers & below_me) == 0ull) atomicAdd(&s_count[b], uint32_t(__popcll(peers)));
     }
   }
   __syncthreads();
@@ -586,7 +592,13 @@ __global__ void __launch_bounds__(1024) k_work_order(const ViewDev* __restrict__
     const ViewDev& view = views[v];
     for (uint32_t t = threadIdx.x; t < view.tile_count; t += blockDim.x) {
       const uint2 r = view.ranges[t];
-      work[atomicAdd(&s_count[__clz(r.y - r.x)], 1u)] = (v << gsh::kTileBits) | t;
+      const uint32_t b = __clz(r.y - r.x);
+      const uint64_t peers = match_digit(b, 6, __ballot(true));
+      const int leader = __ffsll(static_cast<long long>(peers)) - 1;
+      uint32_t first = 0u;
+      if ((peers & below_me) == 0ull) first = atomicAdd(&s_count[b], uint32_t(__popcll(peers)));
+      first = __shfl(first, leader, 64);
+      work[first + uint32_t(__popcll(peers & below_me))] = (v << gsh::kTileBits) | t;
     }
   }
 }
