@@ -128,8 +128,13 @@ def run(gs, ws, cams, target_bufs, order, per_step, views_total, warmup_views, d
             t0 = time.perf_counter()
         base = state["views"]
         vs = [int(order[(base + k) % len(order)]) for k in range(per_step)]
-        frames = [gs.make_frame(cams[v], state["slots"], deg, step=state["step"], tp=tp, flags=state["flags"])
-                  for v in vs]
+        # The step's frames differ only in their camera: one is built, the others copied.
+        f0 = gs.make_frame(cams[vs[0]], state["slots"], deg, step=state["step"], tp=tp, flags=state["flags"])
+        frames = [f0]
+        for v in vs[1:]:
+            f = gs.Frame.from_buffer_copy(f0)
+            f.cam = cams[v]
+            frames.append(f)
         if per_step == 1:
             ws.set_target_device(target_bufs[vs[0]].ptr)
             pairs, _ = ws.train_step(frames[0], want_losses=False)
