@@ -207,3 +207,34 @@ def test_autotune_keeps_a_valid_shape_and_the_model():
     assert best.raster_groups_per_cu in (0, 1, 2, 4, 8)
     assert 4 <= best.depth_digit_bits <= 8 and 4 <= best.tile_digit_bits <= 8
     np.testing.assert_array_equal(before.view(np.uint32), after.view(np.uint32))
+
+
+def test_step_targets_are_read_in_place():
+    """A step's device targets are read where they lie: TARGET keeps what
+    set_target put there, a step given the same photo through its pointer
+    equals a step on TARGET, and a later step without targets reads TARGET."""
+    case("targets read in place: TARGET untouched, pointer step == TARGET step")
+    n, deg, w, h = 900, 1, 96, 72
+    params, sh = make_scene(n, deg, 76)
+    cam = make_camera(w, h, 1.2)
+    photo, other = smooth_target(w, h, 76), smooth_target(w, h, 77)
+    buf = gs.DeviceBuffer(photo)
+    try:
+        frame = gs.make_frame(cam, n, deg, tp=train_params(), step=1, flags=gs.FLAG_NO_UPDATE)
+        with gs.Workspace(n, deg, w, h) as ws:
+            ws.load(params, sh)
+            ws.set_target(other)
+            _, by_pointer = ws.train_views([frame], [buf.ptr])
+            g_pointer = ws.download(gs.BUF_GRAD2D, np.float32, 9 * n)
+            kept = ws.download(gs.BUF_TARGET, np.uint8)
+            _, on_other = ws.train_views([frame], None)
+            ws.set_target(photo)
+            _, on_target = ws.train_views([frame], None)
+            g_target = ws.download(gs.BUF_GRAD2D, np.float32, 9 * n)
+    finally:
+        buf.free()
+    np.testing.assert_array_equal(kept, other.reshape(-1))
+    np.testing.assert_allclose(np.asarray(by_pointer), np.asarray(on_target), rtol=1e-6)
+    assert abs(on_other[0][0] - on_target[0][0]) > 1e-3 * abs(on_target[0][0])
+    scale = np.abs(g_target).max()
+    assert scale > 0 and np.abs(g_pointer - g_target).max() <= 1e-5 * scale
