@@ -238,3 +238,34 @@ def test_step_targets_are_read_in_place():
     assert abs(on_other[0][0] - on_target[0][0]) > 1e-3 * abs(on_target[0][0])
     scale = np.abs(g_target).max()
     assert scale > 0 and np.abs(g_pointer - g_target).max() <= 1e-5 * scale
+
+
+def test_work_order_is_every_tile_once_longest_first():
+    """The rasterizers' work list (TILE_ORDER) after a 3-view step holds every
+    (view, tile) exactly once, as view << 16 | tile, bucketed by
+    floor(log2(list length)) longest first, empty tiles last."""
+    case("work order: a permutation of the 3 views' tiles, longest lists first")
+    n, deg, w, h = 2500, 1, 200, 150
+    params, sh = make_scene(n, deg, 78)
+    cams = [make_camera(w, h, 0.4 + 1.1 * v) for v in range(3)]
+    bufs = [gs.DeviceBuffer(smooth_target(w, h, 78 + v)) for v in range(3)]
+    try:
+        with gs.Workspace(n, deg, w, h) as ws:
+            ws.set_views(3)
+            ws.load(params, sh)
+            ws.train_views([gs.make_frame(c, n, deg, tp=train_params(), step=1) for c in cams], [b.ptr for b in bufs])
+            T = ws.tiles_x * ws.tiles_y
+            order = ws.download(gs.BUF_TILE_ORDER, np.uint32, 3 * T)
+            lengths = []
+            for v in range(3):
+                r = ws.download(gs.BUF_RANGES, np.uint32, (T + 1) * 2, view=v).reshape(-1, 2)[:T]
+                lengths.append((r[:, 1] - r[:, 0]).astype(np.int64))
+    finally:
+        for b in bufs:
+            b.free()
+    want = sorted((v << 16) | t for v in range(3) for t in range(T))
+    assert sorted(order.tolist()) == want
+    lens = np.array([lengths[x >> 16][x & 0xffff] for x in order.tolist()])
+    bucket = np.where(lens > 0, 31 - np.floor(np.log2(np.maximum(lens, 1))).astype(np.int64), 32)
+    assert np.all(np.diff(bucket) >= 0), "buckets out of order"
+    assert lens.max() > 0
