@@ -121,7 +121,9 @@ __device__ __forceinline__ uint32_t stage(Staging& s, const float4* __restrict__
     const float a = ra.z, b = ra.w, c = rb.x;
     const float limit = __logf(255.0f * rb.y);  // drawn where q <= ln(255 o)
     const float slack = 0.02f + 0.01f * fabsf(limit);
-    const float ia = 1.0f / a, ic = 1.0f / c;
+    // The clamped vertices need only be near the edges' minima (the slack is
+    // far above an ulp of them): the hardware reciprocal will do.
+    const float ia = __builtin_amdgcn_rcpf(a), ic = __builtin_amdgcn_rcpf(c);
     for (int q = 0; q < 4; ++q) {
       if (((hits >> q) & 1u) == 0u) continue;
       const float x0 = float(tx0 + (q & 1) * 8) + 0.5f, y0 = float(ty0 + (q >> 1) * 8) + 0.5f;
