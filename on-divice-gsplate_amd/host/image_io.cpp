@@ -230,6 +230,10 @@ struct Huffman {
       valptr[len] = k;
       mincode[len] = code;
       for (int i = 0; i < counts[len - 1]; ++i, ++k, ++code) {
+        // A canonical code of `len` bits is below 2^len; a table whose counts
+        // assign more codes than that is not a prefix code (and would index
+        // past the lookup tables).
+        if (code >= (1 << len)) fail("JPEG: bad Huffman table (more codes than their lengths allow)");
         if (len <= kLookBits) {
           const int shift = kLookBits - len;
           for (int f = 0; f < (1 << shift); ++f) {

@@ -29,6 +29,11 @@ def test_image_io():
     _run("test_image_io", 120)
 
 
+def test_image_io_under_asan():
+    """The image/JSON tests again, built with AddressSanitizer and UBSan."""
+    _run("asan_image_io", 300)
+
+
 def test_dataset_loader():
     _run("test_dataset", 120)
 

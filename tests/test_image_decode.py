@@ -29,18 +29,33 @@ import pytest
 from helpers import ROOT
 
 EXE = os.path.join(ROOT, "on-divice-gsplate_amd", "_build", "test_image_io")
+# The same decoders built with AddressSanitizer (host code only).
+ASAN_EXE = os.path.join(ROOT, "on-divice-gsplate_amd", "_build", "asan_image_io")
 
 
-def decode(path, channels, tmp_path):
-    if not os.path.exists(EXE):
-        pytest.fail(f"{EXE} is not built: run make -C on-divice-gsplate_amd tests")
-    out = str(tmp_path / "decoded.bin")
-    r = subprocess.run([EXE, "decode", str(path), str(channels), out], capture_output=True, text=True, timeout=60)
+def _decode_with(exe, path, channels, tmp_path):
+    if not os.path.exists(exe):
+        pytest.fail(f"{exe} is not built: run make -C on-divice-gsplate_amd tests")
+    out = str(tmp_path / ("decoded_%s.bin" % os.path.basename(exe)))
+    r = subprocess.run([exe, "decode", str(path), str(channels), out], capture_output=True, text=True, timeout=60)
+    assert "AddressSanitizer" not in r.stderr and "runtime error" not in r.stderr, r.stderr
     if r.returncode != 0:
         return None, r.stderr
     raw = open(out, "rb").read()
     w, h, c = struct.unpack("<3I", raw[:12])
     return np.frombuffer(raw[12:], np.uint8).reshape(h, w, c), ""
+
+
+def decode(path, channels, tmp_path, exe=None):
+    """Decodes with the test binary; with exe=None also with the
+    AddressSanitizer build, which must give the same result."""
+    if exe is not None:
+        return _decode_with(exe, path, channels, tmp_path)
+    got, err = _decode_with(EXE, path, channels, tmp_path)
+    got_asan, err_asan = _decode_with(ASAN_EXE, path, channels, tmp_path)
+    assert (got is None) == (got_asan is None), err + err_asan
+    assert got is None or np.array_equal(got, got_asan)
+    return got, err
 
 
 # ---- PNG -------------------------------------------------------------------------
@@ -490,6 +505,28 @@ def test_jpeg_rejects_progressive(tmp_path):
     f.write_bytes(b"\xff\xd8" + sof2 + b"\xff\xd9")
     got, err = decode(f, 3, tmp_path)
     assert got is None and "progressive" in err
+
+
+@pytest.mark.parametrize("exe", [EXE, ASAN_EXE], ids=["plain", "asan"])
+@pytest.mark.parametrize("counts", [[255] + [0] * 15, [0, 5] + [0] * 14, [0] * 8 + [0, 0, 0, 0, 0, 0, 0, 255]],
+                         ids=["len1x255", "len2x5", "len16x255"])
+def test_jpeg_rejects_oversubscribed_huffman_table(counts, exe, tmp_path):
+    """A DHT whose counts assign more codes of a length than that length has
+    (255 one-bit codes; five two-bit codes; 255 sixteen-bit codes after none
+    shorter fit, which is legal, so only the first two are bad) is rejected
+    with an error, and decoding it touches no memory outside the tables."""
+    total = sum(counts)
+    seg = bytes([0x00]) + bytes(counts) + bytes(range(total))
+    dht = b"\xff\xc4" + struct.pack(">H", 2 + len(seg)) + seg
+    f = tmp_path / "h.jpg"
+    f.write_bytes(b"\xff\xd8" + dht + b"\xff\xd9")
+    got, err = decode(f, 3, tmp_path, exe)
+    assert got is None
+    assert "AddressSanitizer" not in err and "runtime error" not in err, err
+    if counts[0] > 2 or counts[1] > 4:
+        assert "bad Huffman table" in err, err
+    else:
+        assert "no image data" in err, err
 
 
 def test_zigzag_table_in_the_decoder_matches_the_walk():
