@@ -35,6 +35,7 @@
 // from a device counter until none are left.
 #include <algorithm>
 
+#include "cull.h"
 #include "gs_common.h"
 #include "gs_internal.h"
 #include "wave.h"
@@ -86,24 +87,12 @@ struct Staging {
   __device__ __forceinline__ float blue(int j) const { return sp[3 * j + 2].x; }
 };
 
-// Smallest q(u, v) = a u^2 / 2 + b u v + c v^2 / 2 (a, c > 0, ac > b^2) over
-// the box [u0, u1] x [v0, v1]: 0 when it holds the origin, else the least of
-// the four edges' minima (each a clamped 1D vertex).
-__device__ __forceinline__ float box_min_quadratic(float a, float b, float c, float ia, float ic, float u0, float u1,
-                                                   float v0, float v1) {
-  if (u0 <= 0.0f && 0.0f <= u1 && v0 <= 0.0f && 0.0f <= v1) return 0.0f;
-  auto q = [&](float u, float v) { return 0.5f * a * u * u + b * u * v + 0.5f * c * v * v; };
-  const float va = fminf(fmaxf(-b * u0 * ic, v0), v1), vb = fminf(fmaxf(-b * u1 * ic, v0), v1);
-  const float ua = fminf(fmaxf(-b * v0 * ia, u0), u1), ub = fminf(fmaxf(-b * v1 * ia, u0), u1);
-  return fminf(fminf(q(u0, va), q(u1, vb)), fminf(q(ua, v0), q(ub, v1)));
-}
-
 // Loads pair k's record into slot t and returns which quadrants it can draw
 // in (bit q): its footprint bbox meets the quadrant and, at some point of the
 // quadrant's pixel-centre rectangle, o exp(-q) reaches 1/255 (q the conic's
-// quadratic form), with a margin far above the rounding of the per-pixel
-// evaluation, so a quadrant is dropped only where every pixel skips the
-// splat anyway.
+// quadratic form), with a margin above the rounding of the per-pixel
+// evaluation (gsh::quadrant_may_draw), so a quadrant is dropped only where
+// every pixel skips the splat anyway.
 __device__ __forceinline__ uint32_t stage(Staging& s, const float4* __restrict__ records, uint32_t g, int t, int tx0,
                                           int ty0) {
   const float4* r = records + size_t(g) * 3u;
@@ -120,15 +109,15 @@ __device__ __forceinline__ uint32_t stage(Staging& s, const float4* __restrict__
   if (hits != 0u) {
     const float a = ra.z, b = ra.w, c = rb.x;
     const float limit = __logf(255.0f * rb.y);  // drawn where q <= ln(255 o)
-    const float slack = 0.02f + 0.01f * fabsf(limit);
-    // The clamped vertices need only be near the edges' minima (the slack is
-    // far above an ulp of them): the hardware reciprocal will do.
+    // The clamped vertices need only be near the edges' minima: the hardware
+    // reciprocal will do.
     const float ia = __builtin_amdgcn_rcpf(a), ic = __builtin_amdgcn_rcpf(c);
     for (int q = 0; q < 4; ++q) {
       if (((hits >> q) & 1u) == 0u) continue;
       const float x0 = float(tx0 + (q & 1) * 8) + 0.5f, y0 = float(ty0 + (q >> 1) * 8) + 0.5f;
-      const float m = box_min_quadratic(a, b, c, ia, ic, ra.x - (x0 + 7.0f), ra.x - x0, ra.y - (y0 + 7.0f), ra.y - y0);
-      if (m > limit + slack) hits &= ~(1u << q);
+      if (!gsh::quadrant_may_draw(a, b, c, ia, ic, limit, ra.x - (x0 + 7.0f), ra.x - x0, ra.y - (y0 + 7.0f),
+                                  ra.y - y0))
+        hits &= ~(1u << q);
     }
   }
   return hits;

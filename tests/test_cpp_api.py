@@ -34,6 +34,12 @@ def test_image_io_under_asan():
     _run("asan_image_io", 300)
 
 
+def test_quadrant_cull():
+    """The rasterizers' quadrant cull keeps every 8x8 box that holds a pixel a
+    needle splat draws (csrc/cull.h, run on the host)."""
+    _run("test_cull", 120)
+
+
 def test_dataset_loader():
     _run("test_dataset", 120)
 
