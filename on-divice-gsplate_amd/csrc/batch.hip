@@ -85,6 +85,7 @@ int gs_hip_batch_destroy(gs_hip_batch* b) {
   if (c.stream != nullptr) hipStreamSynchronize(c.stream);
   gsh::drop_graph(&c);
   gsh::release_tables(&c);
+  gsh::release_timer(c.timer);
   if (c.stream != nullptr) hipStreamDestroy(c.stream);
   c.stream = nullptr;
   delete b;

@@ -60,6 +60,16 @@ void timer_end(gs_hip_workspace* ws) {
   ++t.used;
 }
 
+void release_timer(Timer& t) {
+  for (hipEvent_t e : t.start) hipEventDestroy(e);
+  for (hipEvent_t e : t.stop) hipEventDestroy(e);
+  t.start.clear();
+  t.stop.clear();
+  t.names.clear();
+  t.used = 0;
+  t.enabled = false;
+}
+
 void drop_graph(gs_hip_workspace* ws) {
   if (ws->graph_exec != nullptr) hipGraphExecDestroy(ws->graph_exec);
   if (ws->graph != nullptr) hipGraphDestroy(ws->graph);
@@ -647,6 +657,7 @@ int gs_hip_workspace_destroy(gs_hip_workspace* ws) {
   release(ws->stats);
   release(ws->dir_grad);
   gsh::release_tables(ws);
+  gsh::release_timer(ws->timer);
   if (ws->stream != nullptr) hipStreamDestroy(ws->stream);
   delete ws;
   return 0;

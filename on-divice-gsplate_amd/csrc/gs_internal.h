@@ -262,6 +262,8 @@ struct TableEntry {
 // Kernel timing on the workspace stream (no-ops unless enabled).
 void timer_begin(gs_hip_workspace* ws, const char* name);
 void timer_end(gs_hip_workspace* ws);
+// Destroys the timing events (at the workspace's or batch's end).
+void release_timer(Timer& t);
 // Allocates / frees one view's buffers.
 int create_view(gs_hip_workspace* ws, uint32_t index, View** out);
 void destroy_view(View* v);

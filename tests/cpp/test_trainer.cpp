@@ -16,66 +16,66 @@
 #include "gs/device.h"
 #include "gs/splat_scene.h"
 #include "gs/trainer.h"
+#include "gs_hip.h"
 #include "test_util.h"
 
 namespace {
 
-// Orbit at radius 3 with a wobbling height, fx = fy = 0.9 W, random-noise
-// targets (make_test_dataset, train_selftest.cpp:285-312).
+// Seeded test data in the shape of the reference's test generators
+// (train_selftest.cpp:285-332): frames on an orbit of radius 3 about the
+// origin whose height wobbles three times per turn, fx = fy = 0.9 W, photos of
+// uniform random bytes (alpha 255 unless masked), every 8th frame held out
+// when there are several; gaussians in a cube of half-width 0.7 with scales
+// near 0.08, quaternions leaning to w, mid opacities and small SH.
 gs::Dataset test_dataset(uint32_t w, uint32_t h, size_t frames, bool masks, uint32_t seed) {
   gs::Dataset ds;
   ds.name = "test";
-  ds.width = w;
-  ds.height = h;
-  ds.fx = ds.fy = 0.9f * w;
-  ds.cx = w * 0.5f;
-  ds.cy = h * 0.5f;
+  ds.width = w, ds.height = h;
+  ds.fx = ds.fy = 0.9f * float(w);
+  ds.cx = 0.5f * float(w), ds.cy = 0.5f * float(h);
   ds.has_masks = masks;
-  std::mt19937 rng(seed);
-  std::uniform_int_distribution<int> byte(0, 255);
-  for (size_t i = 0; i < frames; ++i) {
-    const float a = 6.2831853f * float(i) / float(std::max<size_t>(frames, 1));
-    const float eye[3] = {3.0f * std::sin(a), 0.6f + 0.3f * std::sin(3.0f * a), 3.0f * std::cos(a)};
-    const float origin[3] = {0, 0, 0}, up[3] = {0, 1, 0};
-    const gs::Camera c = gs::look_at(eye, origin, up, w, h, 60.0f);
-    gs::DatasetFrame f;
-    for (int r = 0; r < 3; ++r) {
-      f.c2w[r * 4 + 0] = c.rotation[r];
-      f.c2w[r * 4 + 1] = -c.rotation[3 + r];
-      f.c2w[r * 4 + 2] = -c.rotation[6 + r];
-      f.c2w[r * 4 + 3] = eye[r];
-    }
-    f.c2w[12] = f.c2w[13] = f.c2w[14] = 0.0f;
-    f.c2w[15] = 1.0f;
-    f.rgba.resize(size_t(w) * h * 4);
-    for (uint8_t& v : f.rgba) v = uint8_t(byte(rng));
-    if (!masks)
-      for (size_t p = 0; p < size_t(w) * h; ++p) f.rgba[p * 4 + 3] = 255;
-    f.holdout = frames > 1 && i % 8 == 0;
-    ds.frames.push_back(std::move(f));
-  }
   ds.camera_extent = 3.3f;
+  std::mt19937 gen(seed);
+  std::uniform_int_distribution<int> noise(0, 255);
+  const float origin[3] = {0, 0, 0}, up[3] = {0, 1, 0};
+  ds.frames.resize(frames);
+  for (size_t i = 0; i < frames; ++i) {
+    gs::DatasetFrame& f = ds.frames[i];
+    const float turn = 6.2831853f * float(i) / float(frames);
+    const float eye[3] = {3.0f * std::sin(turn), 0.6f + 0.3f * std::sin(3.0f * turn), 3.0f * std::cos(turn)};
+    const gs::Camera c = gs::look_at(eye, origin, up, w, h, 60.0f);
+    // Camera-to-world in OpenGL axes: the camera's x, -y, -z axes as columns, then its centre.
+    for (int r = 0; r < 3; ++r) {
+      const float col[4] = {c.rotation[r], -c.rotation[3 + r], -c.rotation[6 + r], eye[r]};
+      std::copy_n(col, 4, &f.c2w[r * 4]);
+    }
+    const float bottom[4] = {0, 0, 0, 1};
+    std::copy_n(bottom, 4, &f.c2w[12]);
+    f.rgba.resize(size_t(w) * h * 4);
+    std::generate(f.rgba.begin(), f.rgba.end(), [&] { return uint8_t(noise(gen)); });
+    for (size_t p = 3; !masks && p < f.rgba.size(); p += 4) f.rgba[p] = 255;
+    f.holdout = frames > 1 && i % 8 == 0;
+  }
   return ds;
 }
 
-// make_test_scene (train_selftest.cpp:314-332).
 gs::SplatScene test_scene(uint32_t n, int degree, uint32_t seed) {
   gs::SplatScene s;
   s.convention = gs::AxisConvention::RUB;
   s.resize(n, degree);
-  std::mt19937 rng(seed);
-  std::uniform_real_distribution<float> u(-1.0f, 1.0f);
-  for (uint32_t i = 0; i < n; ++i) {
-    for (int k = 0; k < 3; ++k) {
-      s.positions[i * 3 + k] = 0.7f * u(rng);
-      s.log_scales[i * 3 + k] = std::log(0.08f + 0.06f * u(rng));
-      s.sh_dc[i * 3 + k] = 0.8f * u(rng);
-    }
-    for (int k = 0; k < 4; ++k) s.rotations[i * 4 + k] = u(rng);
-    s.rotations[i * 4] += 1.5f;
-    s.opacity_logits[i] = 0.6f * u(rng) - 0.2f;
-  }
-  for (float& v : s.sh_rest) v = 0.25f * u(rng);
+  std::mt19937 gen(seed);
+  std::uniform_real_distribution<float> sym(-1.0f, 1.0f);
+  const auto fill = [&](std::vector<float>& values, float scale, float offset) {
+    for (float& v : values) v = offset + scale * sym(gen);
+  };
+  fill(s.positions, 0.7f, 0.0f);
+  fill(s.log_scales, 0.06f, 0.08f);
+  for (float& v : s.log_scales) v = std::log(v);
+  fill(s.sh_dc, 0.8f, 0.0f);
+  fill(s.opacity_logits, 0.6f, -0.2f);
+  fill(s.rotations, 1.0f, 0.0f);
+  for (uint32_t i = 0; i < n; ++i) s.rotations[size_t(i) * 4] += 1.5f;
+  fill(s.sh_rest, 0.25f, 0.0f);
   return s;
 }
 
@@ -162,10 +162,12 @@ void test_densify_at_capacity(gs::Device& dev) {
   EXPECT(t.stats().densify_events >= 1, "densified");
 }
 
-// The PLY rows written on the device (Trainer::export_ply) equal the host
-// path, save_ply(export_scene() in RDF), byte for byte, with pruned (free)
-// slots among the alive ones (about half are pruned, none spawn) and SH
-// degree 2.
+// export_scene and export_ply (rows written on the device, compacted by a
+// scan) against the raw slots read back with gs_hip_workspace_store: the
+// alive slots in slot order, quaternions divided by their length, and
+// export_ply byte for byte equal to save_ply of that scene in RDF; with
+// pruned (free) slots among the alive ones (about half are pruned, none
+// spawn) and SH degree 2.
 void test_export_ply(gs::Device& dev) {
   gs::TrainOptions o = frozen(2);
   o.iterations = 4;
@@ -181,14 +183,60 @@ void test_export_ply(gs::Device& dev) {
   const float bg[3] = {0, 0, 0};
   t.step_on(0, bg);
   t.step_on(0, bg);
-  gs::SplatScene host = t.export_scene();
-  gs::convert_convention(host, gs::AxisConvention::RDF);
-  const std::vector<uint8_t> want = gs::save_ply(host), got = t.export_ply();
-  EXPECT(t.stats().pruned > 0 && host.count < t.stats().slots, "free slots: %u alive of %u slots", host.count,
-         t.stats().slots);
-  EXPECT(got == want, "export_ply: %zu bytes vs save_ply %zu bytes", got.size(), want.size());
-  std::printf("  export_ply: %u gaussians of %u slots, %zu bytes, equal to save_ply\n", host.count, t.stats().slots,
-              got.size());
+  const uint32_t slots = t.stats().slots, K = t.sh_stride() / 3;
+  std::vector<float> raw(size_t(slots) * 16), raw_sh(size_t(slots) * t.sh_stride());
+  gs::check_status(gs_hip_workspace_store(t.workspace(), raw.data(), raw_sh.data(), slots));
+  gs::SplatScene want;
+  want.convention = gs::AxisConvention::RUB;
+  std::vector<uint32_t> alive;
+  for (uint32_t i = 0; i < slots; ++i)
+    if (raw[size_t(i) * 16 + 7] > 0.5f) alive.push_back(i);
+  want.resize(uint32_t(alive.size()), 2);
+  for (size_t o2 = 0; o2 < alive.size(); ++o2) {
+    const float* r = &raw[size_t(alive[o2]) * 16];
+    const float norm = std::sqrt(r[8] * r[8] + r[9] * r[9] + r[10] * r[10] + r[11] * r[11]);
+    for (int k = 0; k < 3; ++k) {
+      want.positions[o2 * 3 + k] = r[k];
+      want.log_scales[o2 * 3 + k] = r[4 + k];
+      want.sh_dc[o2 * 3 + k] = r[12 + k];
+    }
+    for (int k = 0; k < 4; ++k) want.rotations[o2 * 4 + k] = r[8 + k] / norm;
+    want.opacity_logits[o2] = r[3];
+    for (uint32_t k = 0; k < 3 * K; ++k) want.sh_rest[o2 * 3 * K + k] = raw_sh[size_t(alive[o2]) * 3 * K + k];
+  }
+  const gs::SplatScene got_scene = t.export_scene();
+  EXPECT(got_scene.count == want.count && got_scene.positions == want.positions &&
+             got_scene.log_scales == want.log_scales && got_scene.rotations == want.rotations &&
+             got_scene.opacity_logits == want.opacity_logits && got_scene.sh_dc == want.sh_dc &&
+             got_scene.sh_rest == want.sh_rest,
+         "export_scene differs from the stored slots (%u vs %u gaussians)", got_scene.count, want.count);
+  gs::convert_convention(want, gs::AxisConvention::RDF);
+  const std::vector<uint8_t> ply = t.export_ply(), ply_want = gs::save_ply(want);
+  EXPECT(t.stats().pruned > 0 && want.count < slots, "free slots: %u alive of %u slots", want.count, slots);
+  EXPECT(ply == ply_want, "export_ply: %zu bytes vs save_ply %zu bytes", ply.size(), ply_want.size());
+  std::printf("  export: %u gaussians of %u slots, PLY %zu bytes, equal to the stored slots\n", want.count, slots,
+              ply.size());
+}
+
+// A step of more views than there are training frames takes each training
+// frame once: three training frames (frame 0 held out), eight views per step.
+// The pose gradient is measured (rates 0) for each view of the step, so the
+// pose updates count the views stepped.
+void test_views_per_step_draws_distinct_frames(gs::Device& dev) {
+  gs::TrainOptions o = frozen(0);
+  o.iterations = 12;
+  o.views_per_step = 8;
+  o.refine_poses = true;
+  o.pose_from = 0;
+  o.lr_pose_shift = o.lr_pose_turn = 0.0f;
+  gs::Trainer t(dev, test_dataset(64, 48, 4, false, 41), test_scene(100, 0, 43), o);
+  t.step();
+  EXPECT(t.stats().step == 3 && t.stats().optimizer_steps == 1 && t.stats().pose_updates == 3,
+         "one step over 3 training frames: %u views, %u optimizer steps, %u pose updates", t.stats().step,
+         t.stats().optimizer_steps, t.stats().pose_updates);
+  t.steps(3);
+  EXPECT(t.stats().step == 12 && t.stats().pose_updates == 12, "four steps: %u views, %u pose updates",
+         t.stats().step, t.stats().pose_updates);
 }
 
 // views_per_step > 1: the batched step (mean gradient of the views, one launch
@@ -249,56 +297,80 @@ void render_targets(gs::Device& dev, gs::Dataset& ds, const gs::SplatScene& scen
   }
 }
 
-// Moves frame f's camera as the pose correction does: world -> camera [W | t]
-// becomes [E W | E t + d], E the rotation by w; stored back as camera-to-world
-// in OpenGL axes.
+// Small 3x3 algebra for the pose tests (row-major, double).
+struct M3 {
+  double a[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  double operator()(int r, int c) const { return a[r * 3 + c]; }
+};
+
+M3 mul(const M3& x, const M3& y) {
+  M3 z;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) z.a[r * 3 + c] = x(r, 0) * y(0, c) + x(r, 1) * y(1, c) + x(r, 2) * y(2, c);
+  return z;
+}
+
+M3 from_floats(const float* m) {
+  M3 z;
+  for (int k = 0; k < 9; ++k) z.a[k] = m[k];
+  return z;
+}
+
+// The rotation by angle |w| about w, through its unit quaternion.
+M3 turn_matrix(const double w[3]) {
+  const double angle = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+  const double s = angle > 0.0 ? std::sin(0.5 * angle) / angle : 0.5;
+  const double q0 = std::cos(0.5 * angle), q1 = s * w[0], q2 = s * w[1], q3 = s * w[2];
+  M3 z;
+  const double m[9] = {1 - 2 * (q2 * q2 + q3 * q3), 2 * (q1 * q2 - q0 * q3),     2 * (q1 * q3 + q0 * q2),
+                       2 * (q1 * q2 + q0 * q3),     1 - 2 * (q1 * q1 + q3 * q3), 2 * (q2 * q3 - q0 * q1),
+                       2 * (q1 * q3 - q0 * q2),     2 * (q2 * q3 + q0 * q1),     1 - 2 * (q1 * q1 + q2 * q2)};
+  std::copy_n(m, 9, z.a);
+  return z;
+}
+
+// Frame f's camera moved as a pose correction moves it: world -> camera
+// [R | t] becomes [E R | E t + d] with E = turn_matrix(w).  Written back as
+// camera-to-world in OpenGL axes.
 void nudge_camera(gs::Dataset& ds, size_t f, const double d[3], const double w[3]) {
   const gs::Camera cam = gs::dataset_camera(ds, ds.frames[f]);
-  const double theta = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
-  const double k[3] = {w[0] / theta, w[1] / theta, w[2] / theta};
-  const double K[9] = {0, -k[2], k[1], k[2], 0, -k[0], -k[1], k[0], 0};
-  double E[9], W[9], t[3];
+  const M3 E = turn_matrix(w), R = mul(E, from_floats(cam.rotation));
+  double t[3];
   for (int r = 0; r < 3; ++r) {
-    for (int c = 0; c < 3; ++c) {
-      double k2 = 0.0;
-      for (int i = 0; i < 3; ++i) k2 += K[r * 3 + i] * K[i * 3 + c];
-      E[r * 3 + c] = (r == c ? 1.0 : 0.0) + std::sin(theta) * K[r * 3 + c] + (1.0 - std::cos(theta)) * k2;
-    }
+    t[r] = d[r] + E(r, 0) * cam.translation[0] + E(r, 1) * cam.translation[1] + E(r, 2) * cam.translation[2];
   }
-  for (int r = 0; r < 3; ++r) {
-    t[r] = d[r];
-    for (int c = 0; c < 3; ++c) {
-      W[r * 3 + c] = 0.0;
-      for (int i = 0; i < 3; ++i) W[r * 3 + c] += E[r * 3 + i] * cam.rotation[i * 3 + c];
-      t[r] += E[r * 3 + c] * cam.translation[c];
-    }
-  }
+  // camera -> world: rotation R^T (columns = camera axes; OpenGL flips y and z), centre -R^T t.
+  const double flip[3] = {1.0, -1.0, -1.0};
   float* m = ds.frames[f].c2w;
   for (int r = 0; r < 3; ++r) {
-    m[r * 4 + 0] = float(W[0 * 3 + r]);
-    m[r * 4 + 1] = float(-W[1 * 3 + r]);
-    m[r * 4 + 2] = float(-W[2 * 3 + r]);
-    m[r * 4 + 3] = float(-(W[0 * 3 + r] * t[0] + W[1 * 3 + r] * t[1] + W[2 * 3 + r] * t[2]));
+    for (int axis = 0; axis < 3; ++axis) m[r * 4 + axis] = float(flip[axis] * R(axis, r));
+    m[r * 4 + 3] = float(-(R(0, r) * t[0] + R(1, r) * t[1] + R(2, r) * t[2]));
   }
 }
 
-// Distance between camera centres and the angle between their rotations (degrees).
+// Distance between camera centres, and the angle of the turn from one
+// camera's orientation to the other's in degrees (|Ra - Rb|_F = 2 sqrt 2 sin(angle / 2)).
 void camera_error(const gs::Camera& a, const gs::Camera& b, double* shift, double* turn) {
-  double d2 = 0.0, trace = 0.0;
-  for (int k = 0; k < 3; ++k) d2 += double(a.position[k] - b.position[k]) * (a.position[k] - b.position[k]);
-  for (int r = 0; r < 3; ++r)
-    for (int k = 0; k < 3; ++k) trace += double(a.rotation[r * 3 + k]) * b.rotation[r * 3 + k];
-  *shift = std::sqrt(d2);
-  *turn = std::acos(std::clamp((trace - 1.0) / 2.0, -1.0, 1.0)) * 180.0 / M_PI;
+  double centre = 0.0, frob = 0.0;
+  for (int k = 0; k < 3; ++k) centre += std::pow(double(a.position[k]) - b.position[k], 2);
+  for (int k = 0; k < 9; ++k) frob += std::pow(double(a.rotation[k]) - b.rotation[k], 2);
+  *shift = std::sqrt(centre);
+  *turn = 2.0 * std::asin(std::min(1.0, std::sqrt(frob) / (2.0 * std::sqrt(2.0)))) * 180.0 / M_PI;
 }
 
-// Moving the camera is moving every gaussian the other way: dL/dd = W sum
-// dL/dp and dL/dw = W sum((p - c) x dL/dp + tau), tau the orientation's share
-// from the quaternion gradient (q -> (1, phi / 2) q).  SH degree 0: colour
-// does not depend on the view direction, which the pose gradient leaves out.
+// The pose gradient (the reference's criterion, test_trainer.cpp:404-464)
+// against the per-gaussian gradients: each of the six camera motions acts on
+// the image as a rigid motion of every gaussian, so dL/d(motion) is the sum
+// over gaussians of their gradients dotted with that motion's velocity.  For
+// camera axis k (R the world -> camera rotation, c the camera centre, a =
+// R^T e_k the axis in world coordinates): a shift d = e_k (x_cam = R p + t + d)
+// is every position moving by a; a turn about k (x_cam = E (R p + t)) is every
+// position moving by a x (p - c) and every quaternion by (0, a / 2) q.
+// SH degree 0: colour does not depend on the view direction, which the pose
+// gradient leaves out.
 void test_pose_gradient(gs::Device& dev, float lambda) {
   gs::Dataset ds = test_dataset(64, 48, 2, true, 3);
-  gs::SplatScene scene = test_scene(120, 0, 9);
+  const gs::SplatScene scene = test_scene(120, 0, 9);
   const float bg[3] = {0.2f, 0.3f, 0.4f};
   gs::TrainOptions o = frozen(0);
   o.masked = true;
@@ -308,40 +380,38 @@ void test_pose_gradient(gs::Device& dev, float lambda) {
   o.lr_pose_shift = o.lr_pose_turn = 0.0f;  // measure only
   gs::Trainer t(dev, ds, scene, o);
   t.step_on(0, bg);
-  const double* g = t.last_pose_gradient();
+  const double* got = t.last_pose_gradient();
   const gs::Trainer::Gradients grads = t.first_step_gradients();
   const gs::Camera cam = gs::dataset_camera(ds, ds.frames[0]);
-  double sum_d[3] = {0, 0, 0}, sum_w[3] = {0, 0, 0};
-  for (uint32_t i = 0; i < scene.count; ++i) {
-    const float* gp = &grads.params[size_t(i) * 16];
-    double rel[3];
-    for (int k = 0; k < 3; ++k) {
-      sum_d[k] += gp[k];
-      rel[k] = double(scene.positions[i * 3 + k]) - cam.position[k];
-    }
-    sum_w[0] += rel[1] * gp[2] - rel[2] * gp[1];
-    sum_w[1] += rel[2] * gp[0] - rel[0] * gp[2];
-    sum_w[2] += rel[0] * gp[1] - rel[1] * gp[0];
-    const float* q = &scene.rotations[i * 4];
-    const double len = std::sqrt(double(q[0]) * q[0] + double(q[1]) * q[1] + double(q[2]) * q[2] + double(q[3]) * q[3]);
-    const double w = q[0] / len, u[3] = {q[1] / len, q[2] / len, q[3] / len};
-    const double gw = gp[8] * len, gu[3] = {gp[9] * len, gp[10] * len, gp[11] * len};
-    const double ug[3] = {u[1] * gu[2] - u[2] * gu[1], u[2] * gu[0] - u[0] * gu[2], u[0] * gu[1] - u[1] * gu[0]};
-    for (int k = 0; k < 3; ++k) sum_w[k] += 0.5 * (-gw * u[k] + w * gu[k] + ug[k]);
-  }
-  double expected[6], worst = 0.0, largest = 0.0;
-  for (int r = 0; r < 3; ++r) {
-    expected[r] = expected[3 + r] = 0.0;
-    for (int c = 0; c < 3; ++c) {
-      expected[r] += cam.rotation[r * 3 + c] * sum_d[c];
-      expected[3 + r] += cam.rotation[r * 3 + c] * sum_w[c];
+  double want[6] = {0, 0, 0, 0, 0, 0};
+  for (int k = 0; k < 3; ++k) {
+    const double a[3] = {cam.rotation[k * 3 + 0], cam.rotation[k * 3 + 1], cam.rotation[k * 3 + 2]};
+    for (uint32_t i = 0; i < scene.count; ++i) {
+      const float* g = &grads.params[size_t(i) * 16];
+      const float* p = &scene.positions[size_t(i) * 3];
+      const float* q = &scene.rotations[size_t(i) * 4];
+      const double rel[3] = {p[0] - double(cam.position[0]), p[1] - double(cam.position[1]),
+                             p[2] - double(cam.position[2])};
+      // velocity of the position under the turn: a x rel
+      const double v[3] = {a[1] * rel[2] - a[2] * rel[1], a[2] * rel[0] - a[0] * rel[2], a[0] * rel[1] - a[1] * rel[0]};
+      // velocity of the quaternion (w, x, y, z) under the turn: (0, a / 2) * q
+      const double dq[4] = {-0.5 * (a[0] * q[1] + a[1] * q[2] + a[2] * q[3]),
+                            0.5 * (a[0] * q[0] + a[1] * q[3] - a[2] * q[2]),
+                            0.5 * (a[1] * q[0] + a[2] * q[1] - a[0] * q[3]),
+                            0.5 * (a[2] * q[0] + a[0] * q[2] - a[1] * q[1])};
+      for (int c = 0; c < 3; ++c) {
+        want[k] += g[c] * a[c];
+        want[3 + k] += g[c] * v[c];
+      }
+      for (int c = 0; c < 4; ++c) want[3 + k] += g[8 + c] * dq[c];
     }
   }
+  double worst = 0.0, largest = 0.0;
   std::printf("  pose gradient (lambda %.1f):", lambda);
   for (int k = 0; k < 6; ++k) {
-    std::printf(" %c%c %.4g/%.4g", k < 3 ? 'd' : 'w', "xyz"[k % 3], g[k], expected[k]);
-    worst = std::max(worst, std::fabs(g[k] - expected[k]));
-    largest = std::max(largest, std::fabs(expected[k]));
+    std::printf(" %s%c %.4g/%.4g", k < 3 ? "shift " : "turn ", "xyz"[k % 3], got[k], want[k]);
+    worst = std::max(worst, std::fabs(got[k] - want[k]));
+    largest = std::max(largest, std::fabs(want[k]));
   }
   std::printf("\n");
   EXPECT(largest > 0.0 && worst <= 1e-3 * largest, "pose gradient off by %.3g (largest %.3g)", worst, largest);
@@ -448,6 +518,7 @@ int main() {
     test_exposure_recovery(dev);
     test_densify_at_capacity(dev);
     test_export_ply(dev);
+    test_views_per_step_draws_distinct_frames(dev);
     test_training_converges(dev, 1);
     test_training_converges(dev, 4);
   } catch (const std::exception& e) {
