@@ -26,21 +26,30 @@
  *
  * Parity status per function (what pins it, checked in tests/test_oracle.py):
  *   orc_expf, orc_logf        libm exp/log to <= 2 ulp over sweeps
- *   orc_project64             closed forms (isotropic splat, camera axes), and
- *                             the reference's single-splat falloff KAT
+ *   orc_project64             closed forms: an isotropic splat on the axis (the
+ *                             reference's single-splat falloff KAT), and rotated,
+ *                             anisotropic splats head-on and off-axis with the
+ *                             Jacobian's 1.3x clamp active against a first-
+ *                             principles numpy construction (axis-angle rotation,
+ *                             J V R S^2 R^T V^T J^T + 0.3 I) that a transposed
+ *                             rotation or a missing clamp demonstrably misses
  *   orc_project32             orc_project64 to 1e-5 relative
  *   orc_sh_basis64            scipy's complex spherical harmonics (real form)
  *   orc_footprint_tiles32     brute-force per-pixel alpha >= 1/255 rule
  *                             (test_trainer.cpp:235-289)
  *   orc_bin32                 Python sorted() on (tile, key, index) tuples
- *   orc_render64              closed forms (one splat, two splats in depth order)
+ *   orc_render64              closed forms (one splat, two splats in depth order);
+ *                             its tile lists equal orc_render64_brute bit for bit
  *   orc_forward32             orc_render64 within fp32 tolerance
  *   orc_ssim64                an independent numpy/scipy.ndimage SSIM;
  *                             SSIM(x, x) = 1
  *   orc_loss_image64          central FD of itself (its gradient output)
  *   orc_backward32            orc_backward64 on small scenes (same image)
  *   orc_adjoint64             central FD of orc_loss64 through orc_backward64
- *   orc_backward64            central FD of orc_loss64 (h = 1e-4 .. 1e-6)
+ *   orc_backward64            central FD of orc_loss64 (h = 1e-4 .. 1e-6), and
+ *                             orc_fd_frozen64 on every parameter of a 120-splat scene
+ *   orc_fd_frozen64           orc_backward64 (tight), and plain central FD of
+ *                             orc_loss64 wherever no 1/255 contour crosses a pixel
  *   orc_adam                  closed form; lr 0 leaves m/(1-b1) = g
  *   orc_densify_mark          the predicate of train_densify_mark.comp:28-53
  *   orc_half                  exhaustive: every fp16 value and the midpoints between
@@ -50,11 +59,19 @@
  *   orc_pose_adjoint64        central FD of orc_loss64 under camera shifts and turns
  *                             (SH degree 0), and the reference's identity with the
  *                             per-gaussian gradients (test_trainer.cpp:404-464)
+ *
+ * Multithreaded with OpenMP (the GPU box's host cores); every sum whose order
+ * matters is taken in a fixed order, so results do not depend on the thread
+ * count except for orc_backward32's per-gaussian sums, which add per-pair
+ * partial sums in pair order.
  */
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define TILE 16
 #define NEAR 0.02
@@ -252,6 +269,7 @@ void orc_project32(const float* params, const float* sh, uint32_t n, uint32_t co
                    uint32_t sh_degree, int depth16, float* records, float* footprint, uint32_t* tiles,
                    uint32_t* keys) {
   const float* R = cam->rotation;
+#pragma omp parallel for schedule(static)
   for (uint32_t i = 0; i < n; ++i) {
     const float* p = params + (size_t)i * 16;
     float* rec = records + (size_t)i * 12;
@@ -370,10 +388,10 @@ typedef struct {
   uint32_t tile, key, gaussian;
 } pair_t;
 
+/* Within one tile: by depth key, then gaussian index. */
 static int pair_cmp(const void* a, const void* b) {
   const pair_t* x = (const pair_t*)a;
   const pair_t* y = (const pair_t*)b;
-  if (x->tile != y->tile) return x->tile < y->tile ? -1 : 1;
   if (x->key != y->key) return x->key < y->key ? -1 : 1;
   if (x->gaussian != y->gaussian) return x->gaussian < y->gaussian ? -1 : 1;
   return 0;
@@ -381,43 +399,67 @@ static int pair_cmp(const void* a, const void* b) {
 
 /* Emits every (tile, gaussian) pair from records/footprints, sorts them by
  * (tile, key, index) and fills ranges [(tiles+1)][2].  Returns the pair
- * count; writes up to max_pairs pairs. */
+ * count; writes up to max_pairs pairs.  (A slot whose footprint walk finds
+ * fewer tiles than its count pads with the overflow tile `tiles`.) */
 uint64_t orc_bin32(const float* records, const float* footprint, const uint32_t* tiles, const uint32_t* keys,
                    uint32_t n, uint32_t width, uint32_t height, uint32_t* pair_tiles, uint32_t* pair_gaussians,
                    uint64_t max_pairs, uint32_t* ranges) {
   const uint32_t tiles_x = (width + TILE - 1) / TILE, tiles_y = (height + TILE - 1) / TILE;
-  uint64_t total = 0;
-  for (uint32_t i = 0; i < n; ++i) total += tiles[i];
-  pair_t* pairs = (pair_t*)malloc(sizeof(pair_t) * (total > 0 ? total : 1));
-  uint32_t* buf = (uint32_t*)malloc(sizeof(uint32_t) * (tiles_x * tiles_y + 1));
-  uint64_t p = 0;
-  for (uint32_t i = 0; i < n; ++i) {
-    if (tiles[i] == 0) continue;
-    const float* r = records + (size_t)i * 12;
-    const float* f = footprint + (size_t)i * 4;
-    const uint32_t c = orc_footprint_tiles32(r[0], r[1], f[0], f[1], f[2], f[3], width, height, buf,
-                                             tiles_x * tiles_y + 1);
-    for (uint32_t k = 0; k < tiles[i]; ++k) {
-      pairs[p].tile = k < c ? buf[k] : tiles_x * tiles_y;
-      pairs[p].key = keys[i];
-      pairs[p].gaussian = i;
-      ++p;
-    }
-  }
-  qsort(pairs, (size_t)total, sizeof(pair_t), pair_cmp);
   const uint32_t T = tiles_x * tiles_y;
-  memset(ranges, 0, sizeof(uint32_t) * 2 * (T + 1));
-  for (uint64_t s = 0; s < total; ++s) {
-    if (s < max_pairs) {
-      pair_tiles[s] = pairs[s].tile;
-      pair_gaussians[s] = pairs[s].gaussian;
+  uint64_t* first = (uint64_t*)malloc(sizeof(uint64_t) * ((size_t)n + 1));
+  first[0] = 0;
+  for (uint32_t i = 0; i < n; ++i) first[i + 1] = first[i] + tiles[i];
+  const uint64_t total = first[n];
+  pair_t* emitted = (pair_t*)malloc(sizeof(pair_t) * (total > 0 ? total : 1));
+  pair_t* pairs = (pair_t*)malloc(sizeof(pair_t) * (total > 0 ? total : 1));
+#pragma omp parallel
+  {
+    uint32_t* buf = (uint32_t*)malloc(sizeof(uint32_t) * (T + 1));
+#pragma omp for schedule(dynamic, 1024)
+    for (uint32_t i = 0; i < n; ++i) {
+      if (tiles[i] == 0) continue;
+      const float* r = records + (size_t)i * 12;
+      const float* f = footprint + (size_t)i * 4;
+      const uint32_t c = orc_footprint_tiles32(r[0], r[1], f[0], f[1], f[2], f[3], width, height, buf, T + 1);
+      for (uint32_t k = 0; k < tiles[i]; ++k) {
+        pair_t* q = &emitted[first[i] + k];
+        q->tile = k < c ? buf[k] : T;
+        q->key = keys[i];
+        q->gaussian = i;
+      }
     }
-    const uint32_t t = pairs[s].tile;
-    if (s == 0 || pairs[s - 1].tile != t) ranges[2 * t] = (uint32_t)s;
-    if (s + 1 == total || pairs[s + 1].tile != t) ranges[2 * t + 1] = (uint32_t)(s + 1);
+    free(buf);
   }
+  /* Stable counting sort by tile (pairs stay in gaussian order within a
+   * tile), then each tile by (key, gaussian). */
+  uint64_t* start = (uint64_t*)calloc((size_t)T + 2, sizeof(uint64_t));
+  for (uint64_t s = 0; s < total; ++s) ++start[emitted[s].tile + 1];
+  for (uint32_t t = 0; t <= T; ++t) start[t + 1] += start[t];
+  uint64_t* fill = (uint64_t*)malloc(sizeof(uint64_t) * ((size_t)T + 1));
+  memcpy(fill, start, sizeof(uint64_t) * ((size_t)T + 1));
+  for (uint64_t s = 0; s < total; ++s) pairs[fill[emitted[s].tile]++] = emitted[s];
+#pragma omp parallel for schedule(dynamic, 16)
+  for (uint32_t t = 0; t <= T; ++t) {
+    if (start[t + 1] - start[t] > 1) qsort(pairs + start[t], (size_t)(start[t + 1] - start[t]), sizeof(pair_t), pair_cmp);
+  }
+  memset(ranges, 0, sizeof(uint32_t) * 2 * (T + 1));
+  for (uint32_t t = 0; t <= T; ++t) {
+    if (start[t + 1] > start[t]) {
+      ranges[2 * t] = (uint32_t)start[t];
+      ranges[2 * t + 1] = (uint32_t)start[t + 1];
+    }
+  }
+  const uint64_t kept = total < max_pairs ? total : max_pairs;
+#pragma omp parallel for schedule(static)
+  for (uint64_t s = 0; s < kept; ++s) {
+    pair_tiles[s] = pairs[s].tile;
+    pair_gaussians[s] = pairs[s].gaussian;
+  }
+  free(first);
+  free(emitted);
   free(pairs);
-  free(buf);
+  free(start);
+  free(fill);
   return total;
 }
 
@@ -439,6 +481,7 @@ static int near_threshold(float power, float raw) {
 void orc_forward32(const float* records, const uint32_t* pair_gaussians, const uint32_t* ranges, uint32_t width,
                    uint32_t height, float* image, uint32_t* last, uint8_t* ambiguous) {
   const uint32_t tiles_x = (width + TILE - 1) / TILE;
+#pragma omp parallel for schedule(dynamic, 4)
   for (uint32_t py = 0; py < height; ++py) {
     for (uint32_t px = 0; px < width; ++px) {
       const uint32_t tile = (py / TILE) * tiles_x + px / TILE;
@@ -477,72 +520,103 @@ void orc_forward32(const float* records, const uint32_t* pair_gaussians, const u
 /* ===================== fp32 tile backward (train_backward_serial.glsl) ====== */
 /* Per pixel, back to front over its tile's list from its last contributor,
  * with the forward's final T (image[..][3]) and dL/dC (dldc [H*W][4]); the
- * per-pair terms of splat_pixel_grad (train_common.glsl:228-253) summed per
- * gaussian in double into grad2d [n][9].  tainted [n] (may be null): 1 for
- * every splat met at a decision threshold (near_threshold) and for every
- * gaussian that contributes at a pixel whose walk meets one, where another
- * evaluation may skip or clamp differently and so shift that pixel's terms. */
+ * per-pair terms of splat_pixel_grad (train_common.glsl:228-253), summed per
+ * pair over the tile's pixels in double and then per gaussian (in pair
+ * order) into grad2d [n][9].
+ * A pixel is ambiguous when its walk meets a splat at a decision threshold
+ * (near_threshold): another fp32 evaluation may skip or clamp that splat the
+ * other way and so shift every term of that pixel.  tainted [n] (may be null):
+ * 1 for every gaussian with a term at an ambiguous pixel.  share [n][9] (may be
+ * null): the sum of |term| over ambiguous pixels, per gaussian and value — how
+ * far another evaluation's decisions can move the gaussian's sums. */
 void orc_backward32(const float* records, const uint32_t* pair_gaussians, const uint32_t* ranges, const float* image,
                     const uint32_t* last, const float* dldc, const float bg[3], uint32_t n, uint32_t width,
-                    uint32_t height, double* grad2d, uint8_t* tainted) {
-  const uint32_t tiles_x = (width + TILE - 1) / TILE;
-  memset(grad2d, 0, sizeof(double) * n * 9);
-  for (uint32_t py = 0; py < height; ++py) {
-    for (uint32_t px = 0; px < width; ++px) {
-      const size_t pix = (size_t)py * width + px;
-      const uint32_t tile = (py / TILE) * tiles_x + px / TILE;
-      const uint32_t start = ranges[2 * tile];
-      const float cx = (float)px + 0.5f, cy = (float)py + 0.5f;
-      const float T_final = image[pix * 4 + 3];
-      const float* dC = dldc + pix * 4;
-      const float bg_dot = bg[0] * dC[0] + bg[1] * dC[1] + bg[2] * dC[2];
-      float T = T_final, acc[3] = {0, 0, 0}, last_alpha = 0.0f, last_color[3] = {0, 0, 0};
-      int amb = 0;
-      if (tainted) {
-        for (uint32_t k = start + last[pix]; k-- > start;) {
+                    uint32_t height, double* grad2d, uint8_t* tainted, double* share) {
+  const uint32_t tiles_x = (width + TILE - 1) / TILE, tiles_y = (height + TILE - 1) / TILE;
+  const uint32_t T = tiles_x * tiles_y;
+  const uint64_t pairs = ranges[2 * T + 1] > 0 ? ranges[2 * T + 1] : 0;
+  uint64_t total = 0;
+  for (uint32_t t = 0; t <= T; ++t)
+    if (ranges[2 * t + 1] > total) total = ranges[2 * t + 1];
+  (void)pairs;
+  double* terms = (double*)calloc((size_t)(total > 0 ? total : 1) * 9, sizeof(double));
+  double* amb_terms = share ? (double*)calloc((size_t)(total > 0 ? total : 1) * 9, sizeof(double)) : NULL;
+  uint8_t* amb_pair = tainted ? (uint8_t*)calloc((size_t)(total > 0 ? total : 1), 1) : NULL;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (uint32_t tile = 0; tile < T; ++tile) {
+    const uint32_t start = ranges[2 * tile];
+    const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
+    for (uint32_t py = ty * TILE; py < (ty + 1) * TILE && py < height; ++py) {
+      for (uint32_t px = tx * TILE; px < (tx + 1) * TILE && px < width; ++px) {
+        const size_t pix = (size_t)py * width + px;
+        const float cx = (float)px + 0.5f, cy = (float)py + 0.5f;
+        const float T_final = image[pix * 4 + 3];
+        const float* dC = dldc + pix * 4;
+        const float bg_dot = bg[0] * dC[0] + bg[1] * dC[1] + bg[2] * dC[2];
+        float Tc = T_final, acc[3] = {0, 0, 0}, last_alpha = 0.0f, last_color[3] = {0, 0, 0};
+        int amb = 0;
+        for (uint32_t k = start + last[pix]; k-- > start && !amb;) {
           const float* r = records + (size_t)pair_gaussians[k] * 12;
           const float dx = r[0] - cx, dy = r[1] - cy;
           const float power = -0.5f * (r[2] * dx * dx + r[4] * dy * dy) - r[3] * dx * dy;
-          if (near_threshold(power, r[5] * expf(power))) {
-            amb = 1;
-            tainted[pair_gaussians[k]] = 1; /* in or out of the sum depending on the evaluation */
+          amb = near_threshold(power, r[5] * expf(power));
+        }
+        for (uint32_t k = start + last[pix]; k-- > start;) {
+          const uint32_t i = pair_gaussians[k];
+          const float* r = records + (size_t)i * 12;
+          const float dx = r[0] - cx, dy = r[1] - cy;
+          const float power = -0.5f * (r[2] * dx * dx + r[4] * dy * dy) - r[3] * dx * dy;
+          const float G = expf(power);
+          const float raw = r[5] * G;
+          const float alpha = fminf(0.99f, raw);
+          if (amb && amb_pair) amb_pair[k] = 1; /* in or out of the sum, or weighted otherwise, elsewhere */
+          if (power > 0.0f || alpha < 1.0f / 255.0f) continue;
+          Tc = Tc / (1.0f - alpha);
+          for (int c = 0; c < 3; ++c) acc[c] = last_alpha * last_color[c] + (1.0f - last_alpha) * acc[c];
+          float dL_dalpha = 0.0f;
+          for (int c = 0; c < 3; ++c) {
+            last_color[c] = r[6 + c];
+            dL_dalpha += (r[6 + c] - acc[c]) * dC[c];
           }
-        }
-      }
-      for (uint32_t k = start + last[pix]; k-- > start;) {
-        const uint32_t i = pair_gaussians[k];
-        const float* r = records + (size_t)i * 12;
-        const float dx = r[0] - cx, dy = r[1] - cy;
-        const float power = -0.5f * (r[2] * dx * dx + r[4] * dy * dy) - r[3] * dx * dy;
-        const float G = expf(power);
-        const float raw = r[5] * G;
-        const float alpha = fminf(0.99f, raw);
-        if (power > 0.0f || alpha < 1.0f / 255.0f) continue;
-        T = T / (1.0f - alpha);
-        for (int c = 0; c < 3; ++c) acc[c] = last_alpha * last_color[c] + (1.0f - last_alpha) * acc[c];
-        float dL_dalpha = 0.0f;
-        for (int c = 0; c < 3; ++c) {
-          last_color[c] = r[6 + c];
-          dL_dalpha += (r[6 + c] - acc[c]) * dC[c];
-        }
-        dL_dalpha *= T;
-        last_alpha = alpha;
-        dL_dalpha += (-T_final / (1.0f - alpha)) * bg_dot;
-        double* o = grad2d + (size_t)i * 9;
-        if (amb) tainted[i] = 1;
-        for (int c = 0; c < 3; ++c) o[6 + c] += (double)(alpha * T * dC[c]);
-        if (raw < 0.99f) {
-          const float dpow = raw * dL_dalpha;
-          o[5] += (double)(G * dL_dalpha);
-          o[0] += (double)(dpow * -(r[2] * dx + r[3] * dy));
-          o[1] += (double)(dpow * -(r[4] * dy + r[3] * dx));
-          o[2] += (double)(dpow * (-0.5f * dx * dx));
-          o[3] += (double)(dpow * (-dx * dy));
-          o[4] += (double)(dpow * (-0.5f * dy * dy));
+          dL_dalpha *= Tc;
+          last_alpha = alpha;
+          dL_dalpha += (-T_final / (1.0f - alpha)) * bg_dot;
+          double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+          for (int c = 0; c < 3; ++c) v[6 + c] = (double)(alpha * Tc * dC[c]);
+          if (raw < 0.99f) {
+            const float dpow = raw * dL_dalpha;
+            v[5] = (double)(G * dL_dalpha);
+            v[0] = (double)(dpow * -(r[2] * dx + r[3] * dy));
+            v[1] = (double)(dpow * -(r[4] * dy + r[3] * dx));
+            v[2] = (double)(dpow * (-0.5f * dx * dx));
+            v[3] = (double)(dpow * (-dx * dy));
+            v[4] = (double)(dpow * (-0.5f * dy * dy));
+          }
+          double* o = terms + (size_t)k * 9;
+          for (int c = 0; c < 9; ++c) o[c] += v[c];
+          if (amb && amb_terms) {
+            double* a = amb_terms + (size_t)k * 9;
+            for (int c = 0; c < 9; ++c) a[c] += fabs(v[c]);
+          }
         }
       }
     }
   }
+  memset(grad2d, 0, sizeof(double) * n * 9);
+  if (tainted) memset(tainted, 0, n);
+  if (share) memset(share, 0, sizeof(double) * n * 9);
+  for (uint32_t tile = 0; tile < T; ++tile) {
+    for (uint32_t k = ranges[2 * tile]; k < ranges[2 * tile + 1]; ++k) {
+      const uint32_t i = pair_gaussians[k];
+      for (int c = 0; c < 9; ++c) grad2d[(size_t)i * 9 + c] += terms[(size_t)k * 9 + c];
+      if (tainted && amb_pair[k]) tainted[i] = 1;
+      if (share)
+        for (int c = 0; c < 9; ++c) share[(size_t)i * 9 + c] += amb_terms[(size_t)k * 9 + c];
+    }
+  }
+  free(terms);
+  free(amb_terms);
+  free(amb_pair);
 }
 
 /* ===================== viewer renderer (splat_preprocess.comp, splat.vert/.frag) === */
@@ -854,10 +928,12 @@ static scene64_t build_scene64(const double* params, const double* sh, uint32_t 
   s.refs = (ref_t*)calloc(n > 0 ? n : 1, sizeof(ref_t));
   s.order = (uint32_t*)malloc(sizeof(uint32_t) * (n > 0 ? n : 1));
   s.count = 0;
+#pragma omp parallel for schedule(static)
   for (uint32_t i = 0; i < n; ++i) {
     s.refs[i] = project64(cam, params + (size_t)i * 16, sh + (size_t)i * coeffs_in * 3, degree);
-    if (s.refs[i].visible) s.order[s.count++] = i;
   }
+  for (uint32_t i = 0; i < n; ++i)
+    if (s.refs[i].visible) s.order[s.count++] = i;
   g_sort_refs = s.refs;
   qsort(s.order, s.count, sizeof(uint32_t), order_cmp);
   return s;
@@ -868,14 +944,121 @@ static void free_scene64(scene64_t* s) {
   free(s->order);
 }
 
-/* Brute-force compositing of every visible gaussian, front to back
- * (train_selftest.cpp:243-256).  rgb [H*W][3] = C + T bg; T [H*W]. */
+/* The pixels where a splat can reach alpha >= 1/255: its conic form q =
+ * a dx^2 + 2 b dx dy + c dy^2 is at most 2 ln(255 o) there, an ellipse whose
+ * half extents are sqrt(qmax A) and sqrt(qmax C) (A, C the dilated
+ * covariance's diagonal); one pixel of margin.  0 when it reaches none. */
+static int bbox64(const ref_t* g, uint32_t W, uint32_t H, int box[4]) {
+  if (!g->visible || !(g->opacity >= MIN_ALPHA)) return 0;
+  const double qmax = 2.0 * log(255.0 * g->opacity);
+  if (!(qmax >= 0.0)) return 0;
+  const double ex = sqrt(qmax * g->cov[0]) + 1.0, ey = sqrt(qmax * g->cov[2]) + 1.0;
+  const double x0 = fmax(floor(g->mx - ex - 0.5), 0.0), x1 = fmin(ceil(g->mx + ex - 0.5), (double)W - 1.0);
+  const double y0 = fmax(floor(g->my - ey - 0.5), 0.0), y1 = fmin(ceil(g->my + ey - 0.5), (double)H - 1.0);
+  if (!(x0 <= x1 && y0 <= y1)) return 0;
+  box[0] = (int)x0, box[1] = (int)x1, box[2] = (int)y0, box[3] = (int)y1;
+  return 1;
+}
+
+/* Per 16x16 tile, the visible splats whose bbox64 meets it, in depth order:
+ * list[start[t] .. start[t + 1]). */
+typedef struct {
+  uint32_t tiles_x, tiles_y;
+  uint64_t* start;
+  uint32_t* list;
+} tiles64_t;
+
+static tiles64_t bin_scene64(const scene64_t* s, uint32_t W, uint32_t H) {
+  tiles64_t t;
+  t.tiles_x = (W + TILE - 1) / TILE;
+  t.tiles_y = (H + TILE - 1) / TILE;
+  const uint32_t T = t.tiles_x * t.tiles_y;
+  t.start = (uint64_t*)calloc((size_t)T + 1, sizeof(uint64_t));
+  int* boxes = (int*)malloc(sizeof(int) * 4 * (s->count > 0 ? s->count : 1));
+  uint8_t* hit = (uint8_t*)malloc(s->count > 0 ? s->count : 1);
+  for (uint32_t k = 0; k < s->count; ++k) {
+    int* b = boxes + 4 * k;
+    hit[k] = (uint8_t)bbox64(&s->refs[s->order[k]], W, H, b);
+    if (!hit[k]) continue;
+    for (int ty = b[2] / TILE; ty <= b[3] / TILE; ++ty)
+      for (int tx = b[0] / TILE; tx <= b[1] / TILE; ++tx) ++t.start[(size_t)ty * t.tiles_x + tx + 1];
+  }
+  for (uint32_t k = 0; k < T; ++k) t.start[k + 1] += t.start[k];
+  uint64_t* fill = (uint64_t*)malloc(sizeof(uint64_t) * ((size_t)T + 1));
+  memcpy(fill, t.start, sizeof(uint64_t) * ((size_t)T + 1));
+  t.list = (uint32_t*)malloc(sizeof(uint32_t) * (t.start[T] > 0 ? t.start[T] : 1));
+  for (uint32_t k = 0; k < s->count; ++k) {
+    if (!hit[k]) continue;
+    const int* b = boxes + 4 * k;
+    for (int ty = b[2] / TILE; ty <= b[3] / TILE; ++ty)
+      for (int tx = b[0] / TILE; tx <= b[1] / TILE; ++tx) t.list[fill[(size_t)ty * t.tiles_x + tx]++] = s->order[k];
+  }
+  free(boxes);
+  free(hit);
+  free(fill);
+  return t;
+}
+
+static void free_tiles64(tiles64_t* t) {
+  free(t->start);
+  free(t->list);
+}
+
+/* Front-to-back compositing at one pixel over its tile's list
+ * (train_selftest.cpp:243-256); returns the final T, writes C (before the
+ * background) and, if asked, the list position after the last contributor. */
+static double composite64(const scene64_t* s, const tiles64_t* t, uint32_t x, uint32_t y, double C[3],
+                          uint64_t* end) {
+  const uint32_t tile = (y / TILE) * t->tiles_x + x / TILE;
+  double T = 1;
+  C[0] = C[1] = C[2] = 0;
+  uint64_t stop = t->start[tile];
+  for (uint64_t k = t->start[tile]; k < t->start[tile + 1]; ++k) {
+    const ref_t* g = &s->refs[t->list[k]];
+    const double dx = g->mx - (x + 0.5), dy = g->my - (y + 0.5);
+    const double power = -0.5 * (g->con[0] * dx * dx + g->con[2] * dy * dy) - g->con[1] * dx * dy;
+    if (power > 0) continue;
+    double alpha = g->opacity * exp(power);
+    if (alpha > MAX_ALPHA) alpha = MAX_ALPHA;
+    if (alpha < MIN_ALPHA) continue;
+    if (T * (1 - alpha) < MIN_T) break;
+    for (int c = 0; c < 3; ++c) C[c] += g->color[c] * alpha * T;
+    T *= 1 - alpha;
+    stop = k + 1;
+  }
+  if (end != NULL) *end = stop;
+  return T;
+}
+
+/* Compositing of every visible gaussian at every pixel, front to back.  A
+ * splat outside its bbox64 has alpha < 1/255 there and is skipped, so the
+ * tile lists give the brute-force result (same operations in the same
+ * order).  rgb [H*W][3] = C + T bg; T [H*W] (may be null). */
 static void render_scene64(const scene64_t* s, const orc_camera* cam, const double bg[3], double* rgb, double* Tout) {
+  tiles64_t t = bin_scene64(s, cam->width, cam->height);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (uint32_t y = 0; y < cam->height; ++y) {
+    for (uint32_t x = 0; x < cam->width; ++x) {
+      double C[3];
+      const double T = composite64(s, &t, x, y, C, NULL);
+      const size_t pix = (size_t)y * cam->width + x;
+      for (int c = 0; c < 3; ++c) rgb[pix * 3 + c] = C[c] + T * bg[c];
+      if (Tout != NULL) Tout[pix] = T;
+    }
+  }
+  free_tiles64(&t);
+}
+
+/* The brute-force loop over every visible gaussian, kept as the check of
+ * the tiled render. */
+void orc_render64_brute(const double* params, const double* sh, uint32_t n, uint32_t coeffs_in,
+                        const orc_camera* cam, int degree, const double bg[3], double* rgb) {
+  scene64_t s = build_scene64(params, sh, n, coeffs_in, cam, degree);
   for (uint32_t y = 0; y < cam->height; ++y) {
     for (uint32_t x = 0; x < cam->width; ++x) {
       double T = 1, C[3] = {0, 0, 0};
-      for (uint32_t k = 0; k < s->count; ++k) {
-        const ref_t* g = &s->refs[s->order[k]];
+      for (uint32_t k = 0; k < s.count; ++k) {
+        const ref_t* g = &s.refs[s.order[k]];
         const double dx = g->mx - (x + 0.5), dy = g->my - (y + 0.5);
         const double power = -0.5 * (g->con[0] * dx * dx + g->con[2] * dy * dy) - g->con[1] * dx * dy;
         if (power > 0) continue;
@@ -888,9 +1071,9 @@ static void render_scene64(const scene64_t* s, const orc_camera* cam, const doub
       }
       const size_t pix = (size_t)y * cam->width + x;
       for (int c = 0; c < 3; ++c) rgb[pix * 3 + c] = C[c] + T * bg[c];
-      if (Tout != NULL) Tout[pix] = T;
     }
   }
+  free_scene64(&s);
 }
 
 void orc_render64(const double* params, const double* sh, uint32_t n, uint32_t coeffs_in, const orc_camera* cam,
@@ -911,6 +1094,7 @@ static void window64(double w[11]) {
 static void blur64(const double* in, double* out, double* tmp, uint32_t W, uint32_t H) {
   double w[11];
   window64(w);
+#pragma omp parallel for schedule(static)
   for (uint32_t r = 0; r < H; ++r)
     for (uint32_t c = 0; c < W; ++c) {
       double s = 0;
@@ -920,6 +1104,7 @@ static void blur64(const double* in, double* out, double* tmp, uint32_t W, uint3
       }
       tmp[(size_t)r * W + c] = s;
     }
+#pragma omp parallel for schedule(static)
   for (uint32_t r = 0; r < H; ++r)
     for (uint32_t c = 0; c < W; ++c) {
       double s = 0;
@@ -951,12 +1136,15 @@ double orc_ssim64(const double* x, const double* y, uint32_t W, uint32_t H, doub
       in[4 * n + p] = a * b;
     }
     for (int q = 0; q < 5; ++q) blur64(in + q * n, mu + q * n, tmp, W, H);
+    /* Each pixel's SSIM kept, then summed in pixel order: the same total for any thread count. */
+    double* smap = (double*)malloc(sizeof(double) * n);
+#pragma omp parallel for schedule(static)
     for (size_t p = 0; p < n; ++p) {
       const double mx = mu[p], my = mu[n + p];
       const double vx = mu[2 * n + p] - mx * mx, vy = mu[3 * n + p] - my * my, cxy = mu[4 * n + p] - mx * my;
       const double l = 2 * mx * my + c1, cs = 2 * cxy + c2, m = mx * mx + my * my + c1, v = vx + vy + c2;
       const double s = l * cs / (m * v);
-      sum += s;
+      smap[p] = s;
       if (grad != NULL) {
         /* d s / d mu_x (holding E_xx, E_xy), d s / d E_xx, d s / d E_xy */
         const double d_exy = 2 * l / (m * v);
@@ -967,6 +1155,8 @@ double orc_ssim64(const double* x, const double* y, uint32_t W, uint32_t H, doub
         maps[2 * n + p] = d_exy;
       }
     }
+    for (size_t p = 0; p < n; ++p) sum += smap[p];
+    free(smap);
     if (grad != NULL) {
       /* The window is symmetric: the adjoint of the blur is the blur. */
       for (int q = 0; q < 3; ++q) blur64(maps + q * n, blurred + q * n, tmp, W, H);
@@ -1045,6 +1235,7 @@ static void adjoint_scene64(const scene64_t* sc, const double* params, const dou
   for (int r = 0; r < 3; ++r)
     for (int c = 0; c < 3; ++c) V[r][c] = cam->rotation[r * 3 + c];
   (void)Rc;
+#pragma omp parallel for schedule(static)
   for (uint32_t i = 0; i < n; ++i) {
     const ref_t* g = &sc->refs[i];
     if (!g->visible) continue;
@@ -1248,14 +1439,16 @@ double orc_backward64(const double* params, const double* sh, uint32_t n, uint32
   const double loss = orc_loss_image64(rgb, tgt, W, H, lambda, dldc);
   memset(grad2d, 0, sizeof(double) * n * 9);
   uint32_t* contrib = (uint32_t*)malloc(sizeof(uint32_t) * (s.count > 0 ? s.count : 1));
+  tiles64_t tl = bin_scene64(&s, W, H);
   for (uint32_t y = 0; y < H; ++y) {
     for (uint32_t x = 0; x < W; ++x) {
       const size_t pix = (size_t)y * W + x;
       /* Forward again: the contributors of this pixel, front to back. */
       uint32_t m = 0;
       double T = 1;
-      for (uint32_t k = 0; k < s.count; ++k) {
-        const ref_t* g = &s.refs[s.order[k]];
+      const uint32_t tile = (y / TILE) * tl.tiles_x + x / TILE;
+      for (uint64_t k = tl.start[tile]; k < tl.start[tile + 1]; ++k) {
+        const ref_t* g = &s.refs[tl.list[k]];
         const double dx = g->mx - (x + 0.5), dy = g->my - (y + 0.5);
         const double power = -0.5 * (g->con[0] * dx * dx + g->con[2] * dy * dy) - g->con[1] * dx * dy;
         if (power > 0) continue;
@@ -1264,7 +1457,7 @@ double orc_backward64(const double* params, const double* sh, uint32_t n, uint32
         if (alpha < MIN_ALPHA) continue;
         if (T * (1 - alpha) < MIN_T) break;
         T *= 1 - alpha;
-        contrib[m++] = s.order[k];
+        contrib[m++] = tl.list[k];
       }
       const double* dC = dldc + pix * 3;
       const double T_final = Tf[pix];
@@ -1304,6 +1497,7 @@ double orc_backward64(const double* params, const double* sh, uint32_t n, uint32
     }
   }
   free(contrib);
+  free_tiles64(&tl);
   adjoint_scene64(&s, params, sh, n, coeffs_in, cam, degree, grad2d, gparams, gsh, NULL);
   free(rgb);
   free(Tf);
@@ -1313,9 +1507,171 @@ double orc_backward64(const double* params, const double* sh, uint32_t n, uint32
   return loss;
 }
 
+/* ---- central differences with the compositor's branches frozen ---------- */
+/* Sum over pixels of [x0, x1] x [y0, y1] (and channels) of SSIM of x
+ * against y, both full [H*W][3] images, with the zero padding of the full
+ * image (window sums as orc_ssim64). */
+static double ssim_region64(const double* x, const double* y, uint32_t W, uint32_t H, int x0, int x1, int y0, int y1,
+                            double c1, double c2) {
+  double w[11];
+  window64(w);
+  double sum = 0;
+  for (int py = y0; py <= y1; ++py) {
+    for (int px = x0; px <= x1; ++px) {
+      for (int c = 0; c < 3; ++c) {
+        double m[5] = {0, 0, 0, 0, 0};
+        for (int dy = -5; dy <= 5; ++dy) {
+          const int yy = py + dy;
+          if (yy < 0 || yy >= (int)H) continue;
+          double r[5] = {0, 0, 0, 0, 0};
+          for (int dx = -5; dx <= 5; ++dx) {
+            const int xx = px + dx;
+            if (xx < 0 || xx >= (int)W) continue;
+            const double a = x[((size_t)yy * W + xx) * 3 + c], b = y[((size_t)yy * W + xx) * 3 + c];
+            const double wx = w[dx + 5];
+            r[0] += wx * a, r[1] += wx * b, r[2] += wx * a * a, r[3] += wx * b * b, r[4] += wx * a * b;
+          }
+          for (int q = 0; q < 5; ++q) m[q] += w[dy + 5] * r[q];
+        }
+        const double vx = m[2] - m[0] * m[0], vy = m[3] - m[1] * m[1], cxy = m[4] - m[0] * m[1];
+        sum += (2 * m[0] * m[1] + c1) * (2 * cxy + c2) / ((m[0] * m[0] + m[1] * m[1] + c1) * (vx + vy + c2));
+      }
+    }
+  }
+  return sum;
+}
+
+/* Central differences (step h) of the fp64 loss (orc_loss64 with an
+ * unmasked target: (1 - lambda) mean |e| + lambda (1 - mean SSIM)) by single
+ * parameters, with every branch of the compositor frozen as it went in the
+ * unperturbed render: which splats a pixel composites (alpha >= 1/255 and
+ * power <= 0), where alpha is clamped at 0.99, where the walk stops
+ * (T < 1e-4), which colour channels are clamped at 0, and the depth order.
+ * That is the function whose derivative the analytic backward computes: it
+ * leaves out the jumps where a splat's 1/255 contour crosses a pixel centre,
+ * which at 1280x720 a step of 1e-4 in a position makes likely for most
+ * splats and which would swamp the difference quotient.  Only the pixels the
+ * perturbed gaussian composites change, so the L1 part is summed over its
+ * bbox64 and the SSIM part over that box grown by the 5-pixel window radius.
+ * Samples: gaussian[k], column[k] (0..15: the parameter slot; 16 + 3 j + c:
+ * SH rest coefficient j, channel c).  fd[k] is NaN where the perturbation
+ * makes the gaussian invisible.  Returns the unperturbed loss. */
+double orc_fd_frozen64(const double* params, const double* sh, uint32_t n, uint32_t coeffs_in, const orc_camera* cam,
+                       int degree, const double bg[3], const uint8_t* target, double lambda, uint32_t count,
+                       const uint32_t* gaussian, const uint32_t* column, double h, double* fd) {
+  const uint32_t W = cam->width, H = cam->height;
+  const size_t pixels = (size_t)W * H;
+  const double c1 = 0.01 * 0.01, c2 = 0.03 * 0.03, norm = 1.0 / (3.0 * (double)pixels);
+  scene64_t s = build_scene64(params, sh, n, coeffs_in, cam, degree);
+  tiles64_t tl = bin_scene64(&s, W, H);
+  double* x = (double*)malloc(sizeof(double) * pixels * 3);
+  double* y = (double*)malloc(sizeof(double) * pixels * 3);
+  uint64_t* stop = (uint64_t*)malloc(sizeof(uint64_t) * pixels);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (uint32_t py = 0; py < H; ++py)
+    for (uint32_t px = 0; px < W; ++px) {
+      double C[3];
+      const size_t pix = (size_t)py * W + px;
+      const double T = composite64(&s, &tl, px, py, C, &stop[pix]);
+      for (int c = 0; c < 3; ++c) x[pix * 3 + c] = C[c] + T * bg[c];
+    }
+  target64(target, W, H, bg, 0, y);
+  double* ximg = (double*)malloc(sizeof(double) * pixels * 3);
+  memcpy(ximg, x, sizeof(double) * pixels * 3);
+  const double base = orc_loss_image64(ximg, y, W, H, lambda, NULL);
+  free(ximg);
+#pragma omp parallel
+  {
+    double* xp = (double*)malloc(sizeof(double) * pixels * 3);
+    memcpy(xp, x, sizeof(double) * pixels * 3);
+    double* row = (double*)malloc(sizeof(double) * 16 + sizeof(double) * (coeffs_in > 0 ? coeffs_in : 1) * 3);
+    double* rest = row + 16;
+#pragma omp for schedule(dynamic, 1)
+    for (uint32_t k = 0; k < count; ++k) {
+      const uint32_t i = gaussian[k], col = column[k];
+      const ref_t* g0 = &s.refs[i];
+      int box[4];
+      fd[k] = NAN;
+      if (i >= n || !bbox64(g0, W, H, box)) {
+        fd[k] = 0.0; /* composites nowhere: the frozen loss does not depend on it */
+        continue;
+      }
+      const int sx0 = box[0] - 5 < 0 ? 0 : box[0] - 5, sx1 = box[1] + 5 >= (int)W ? (int)W - 1 : box[1] + 5;
+      const int sy0 = box[2] - 5 < 0 ? 0 : box[2] - 5, sy1 = box[3] + 5 >= (int)H ? (int)H - 1 : box[3] + 5;
+      double delta[2];
+      int ok = 1;
+      for (int side = 0; side < 2 && ok; ++side) {
+        memcpy(row, params + (size_t)i * 16, sizeof(double) * 16);
+        if (coeffs_in > 0) memcpy(rest, sh + (size_t)i * coeffs_in * 3, sizeof(double) * coeffs_in * 3);
+        const double step = side == 0 ? h : -h;
+        if (col < 16) row[col] += step;
+        else rest[col - 16] += step;
+        const ref_t g1 = project64(cam, row, rest, degree);
+        if (!g1.visible) {
+          ok = 0;
+          break;
+        }
+        double l1 = 0, ssim_new, ssim_old;
+        for (int py = box[2]; py <= box[3]; ++py) {
+          for (int px = box[0]; px <= box[1]; ++px) {
+            const uint32_t tile = ((uint32_t)py / TILE) * tl.tiles_x + (uint32_t)px / TILE;
+            double T = 1, C[3] = {0, 0, 0};
+            const size_t pix = (size_t)py * W + px;
+            for (uint64_t q = tl.start[tile]; q < stop[pix]; ++q) {
+              const uint32_t j = tl.list[q];
+              const ref_t* g = &s.refs[j];
+              const double dx = g->mx - (px + 0.5), dy = g->my - (py + 0.5);
+              const double power = -0.5 * (g->con[0] * dx * dx + g->con[2] * dy * dy) - g->con[1] * dx * dy;
+              if (power > 0) continue;
+              const double raw = g->opacity * exp(power);
+              if (raw < MIN_ALPHA) continue; /* (a clamped raw is >= 0.99: never below 1/255) */
+              double alpha = raw > MAX_ALPHA ? MAX_ALPHA : raw;
+              const double* col3 = g->color;
+              if (j == i) { /* the perturbed splat, on the branches it took unperturbed */
+                if (raw < MAX_ALPHA) {
+                  const double ex = g1.mx - (px + 0.5), ey = g1.my - (py + 0.5);
+                  const double pw = -0.5 * (g1.con[0] * ex * ex + g1.con[2] * ey * ey) - g1.con[1] * ex * ey;
+                  alpha = g1.opacity * exp(pw);
+                }
+                col3 = g1.color; /* clamped channels: see below */
+              }
+              for (int c = 0; c < 3; ++c) {
+                const double cv = (j == i && g->clamped[c]) ? 0.0 : col3[c];
+                C[c] += cv * alpha * T;
+              }
+              T *= 1 - alpha;
+            }
+            for (int c = 0; c < 3; ++c) {
+              const double v = C[c] + T * bg[c];
+              l1 += fabs(v - y[pix * 3 + c]) - fabs(x[pix * 3 + c] - y[pix * 3 + c]);
+              xp[pix * 3 + c] = v;
+            }
+          }
+        }
+        ssim_new = lambda != 0.0 ? ssim_region64(xp, y, W, H, sx0, sx1, sy0, sy1, c1, c2) : 0.0;
+        for (int py = box[2]; py <= box[3]; ++py)
+          memcpy(xp + ((size_t)py * W + box[0]) * 3, x + ((size_t)py * W + box[0]) * 3,
+                 sizeof(double) * 3 * (size_t)(box[1] - box[0] + 1));
+        ssim_old = lambda != 0.0 ? ssim_region64(xp, y, W, H, sx0, sx1, sy0, sy1, c1, c2) : 0.0;
+        delta[side] = (1 - lambda) * l1 * norm - lambda * (ssim_new - ssim_old) * norm;
+      }
+      if (ok) fd[k] = (delta[0] - delta[1]) / (2 * h);
+    }
+    free(xp);
+    free(row);
+  }
+  free(x);
+  free(y);
+  free(stop);
+  free_tiles64(&tl);
+  free_scene64(&s);
+  return base;
+}
+
 /* ---- Adam (train_step.glsl:62-68) ---------------------------------------- */
 void orc_adam(float* p, const float* g, float* m, float* v, const float* lr, uint64_t count, float beta1, float beta2,
               float eps, float bias1, float bias2) {
+#pragma omp parallel for schedule(static)
   for (uint64_t i = 0; i < count; ++i) {
     m[i] = g[i] * (1.0f - beta1) + m[i] * beta1;
     v[i] = g[i] * g[i] * (1.0f - beta2) + v[i] * beta2;

@@ -55,7 +55,10 @@ def library() -> ctypes.CDLL:
             "orc_backward64": ([P(f64), P(f64), u32, u32, cam_p, i32, P(f64), P(ctypes.c_uint8), i32, f64, P(f64),
                                 P(f64), P(f64)], f64),
             "orc_backward32": ([P(f32), P(u32), P(u32), P(f32), P(u32), P(f32), P(f32), u32, u32, u32, P(f64),
-                                P(ctypes.c_uint8)], None),
+                                P(ctypes.c_uint8), P(f64)], None),
+            "orc_render64_brute": ([P(f64), P(f64), u32, u32, cam_p, i32, P(f64), P(f64)], None),
+            "orc_fd_frozen64": ([P(f64), P(f64), u32, u32, cam_p, i32, P(f64), P(ctypes.c_uint8), f64, u32, P(u32),
+                                 P(u32), f64, P(f64)], f64),
             "orc_adjoint64": ([P(f64), P(f64), u32, u32, cam_p, i32, P(f64), P(f64), P(f64)], None),
             "orc_pose_adjoint64": ([P(f64), P(f64), u32, u32, cam_p, i32, P(f64), P(f64)], None),
             "orc_adam": ([P(f32), P(f32), P(f32), P(f32), P(f32), u64, f32, f32, f32, f32, f32], None),
@@ -186,6 +189,36 @@ def render64(params, sh, cam, degree: int, bg=(0.0, 0.0, 0.0)):
     return rgb, T
 
 
+def render64_brute(params, sh, cam, degree: int, bg=(0.0, 0.0, 0.0)) -> np.ndarray:
+    """rgb [HW, 3] of the loop over every visible gaussian at every pixel (the
+    check of render64's tile lists)."""
+    params, sh, shp, n, coeffs = _scene64(params, sh)
+    rgb = np.zeros((cam.height * cam.width, 3), np.float64)
+    bgv = np.asarray(bg, np.float64)
+    library().orc_render64_brute(_p(params, ctypes.c_double), _p(shp, ctypes.c_double), n, coeffs,
+                                 ctypes.byref(camera(cam)), degree, _p(bgv, ctypes.c_double), _p(rgb, ctypes.c_double))
+    return rgb
+
+
+def fd_frozen64(params, sh, cam, degree: int, target, lam: float, gaussians, columns, h: float = 1e-4,
+                bg=(0.0, 0.0, 0.0)):
+    """(loss, fd [k]): central differences of the fp64 loss (unmasked target)
+    by parameter columns[k] of gaussian gaussians[k] (0..15 the slot, 16 + 3 j + c
+    SH rest coefficient j channel c), with the compositor's branches frozen as
+    in the unperturbed render (oracle.c orc_fd_frozen64)."""
+    params, sh, shp, n, coeffs = _scene64(params, sh)
+    t = np.ascontiguousarray(target, np.uint8).reshape(-1)
+    gi = np.ascontiguousarray(gaussians, np.uint32)
+    ci = np.ascontiguousarray(columns, np.uint32)
+    fd = np.zeros(gi.size, np.float64)
+    bgv = np.asarray(bg, np.float64)
+    L = library().orc_fd_frozen64(_p(params, ctypes.c_double), _p(shp, ctypes.c_double), n, coeffs,
+                                  ctypes.byref(camera(cam)), degree, _p(bgv, ctypes.c_double), _p(t, ctypes.c_uint8),
+                                  lam, gi.size, _p(gi, ctypes.c_uint32), _p(ci, ctypes.c_uint32), h,
+                                  _p(fd, ctypes.c_double))
+    return L, fd
+
+
 def ssim64(x, y, width: int, height: int, grad: bool = False):
     x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
     y = np.ascontiguousarray(y, np.float64).reshape(-1, 3)
@@ -251,10 +284,12 @@ def densify_mark(params, stats, grad_threshold, clone_max_scale, prune_opacity, 
 
 
 def backward32(records, pair_gaussians, ranges, image, last, dldc, bg, width: int, height: int,
-               flags: bool = False):
+               flags: bool = False, share: bool = False):
     """grad2d [n, 9] (double sums) of the tile backward on the given forward state;
-    with flags=True also tainted [n] (bool): gaussians that contribute at a pixel
-    whose walk meets a splat at a skip/clamp threshold."""
+    with flags=True also tainted [n] (bool): gaussians that have a term at a pixel
+    whose walk meets a splat at a skip/clamp threshold; with share=True also
+    share [n, 9]: the sum of those terms' magnitudes (how far another
+    evaluation's decisions can move each sum)."""
     records = np.ascontiguousarray(records, np.float32).reshape(-1, 12)
     n = records.shape[0]
     pg = np.ascontiguousarray(pair_gaussians, np.uint32)
@@ -262,6 +297,7 @@ def backward32(records, pair_gaussians, ranges, image, last, dldc, bg, width: in
         pg = np.zeros(1, np.uint32)
     out = np.zeros((n, 9), np.float64)
     tainted = np.zeros(max(n, 1), np.uint8)
+    sh9 = np.zeros((max(n, 1), 9), np.float64)
     bgv = np.asarray(bg, np.float32)
     library().orc_backward32(_p(records, ctypes.c_float), _p(pg, ctypes.c_uint32),
                              _p(np.ascontiguousarray(ranges, np.uint32), ctypes.c_uint32),
@@ -269,10 +305,14 @@ def backward32(records, pair_gaussians, ranges, image, last, dldc, bg, width: in
                              _p(np.ascontiguousarray(last, np.uint32), ctypes.c_uint32),
                              _p(np.ascontiguousarray(dldc, np.float32), ctypes.c_float), _p(bgv, ctypes.c_float),
                              n, width, height, _p(out, ctypes.c_double),
-                             _p(tainted, ctypes.c_uint8) if flags else None)
+                             _p(tainted, ctypes.c_uint8) if (flags or share) else None,
+                             _p(sh9, ctypes.c_double) if share else None)
+    res = (out,)
     if flags:
-        return out, tainted[:n].astype(bool)
-    return out
+        res += (tainted[:n].astype(bool),)
+    if share:
+        res += (sh9[:n],)
+    return res if len(res) > 1 else out
 
 
 def adjoint64(params, sh, cam, degree: int, grad2d):

@@ -500,3 +500,154 @@ def test_exposure_sums64_known_answers():
     y = orc.target_colour64(target, bg=(0.5, 0.25, 0.0), masked=True, exposure=1.0 / k)
     a = target[:, 3:4] / 255.0
     np.testing.assert_allclose(y, (0.5, 0.25, 0.0) * (1 - a) + image[:, :3] * a, rtol=1e-12)
+
+
+def test_render64_tile_lists_equal_brute_force():
+    """render64 walks per-tile lists of the splats whose 1/255 ellipse box
+    meets the tile; the loop over every visible gaussian gives the same
+    image bit for bit (same operations in the same order), including wide
+    splats past the image edge and faint ones near the 1/255 cut."""
+    for n, deg, w, h, seed, widen in ((300, 2, 96, 72, 5, 0.0), (150, 0, 70, 45, 6, 1.5), (60, 3, 33, 17, 7, 3.0)):
+        params, sh = make_scene(n, deg, seed)
+        params[: n // 3, 4:7] += widen
+        params[n // 3: n // 2, 3] = -5.4  # opacity ~ 1/220: the ellipse is tiny or empty
+        cam = make_camera(w, h, 0.7)
+        rgb, _ = orc.render64(params, sh, cam, deg, bg=(0.1, 0.2, 0.3))
+        brute = orc.render64_brute(params, sh, cam, deg, bg=(0.1, 0.2, 0.3))
+        np.testing.assert_array_equal(rgb, brute)
+
+
+def test_fd_frozen64_matches_the_analytic_backward():
+    """Central differences with the compositor's branches frozen (the
+    function the analytic backward differentiates) agree with the fp64
+    analytic gradient to 1e-5 of each group's largest value plus 1e-4 of the
+    difference itself, on every parameter of every gaussian of a 120-splat
+    scene (SH degree 2, background, lambda 0.2); and plain central
+    differences of the loss (jumps included) agree with them wherever no
+    1/255 contour crosses a pixel centre (most samples at this size)."""
+    w, h = 80, 60
+    params, sh = make_scene(120, 2, 45, spread=0.5)
+    cam = make_camera(w, h, 0.9)
+    target = smooth_target(w, h, 8)
+    bg = (0.2, 0.1, 0.0)
+    p64, s64 = params.astype(np.float64), sh.astype(np.float64)
+    L, _, gp, gsh = orc.backward64(p64, s64, cam, 2, target, 0.2, bg=bg)
+    cols = [0, 1, 2, 3, 4, 5, 6, 8, 9, 10, 11, 12, 13, 14]
+    gi = np.repeat(np.arange(120), len(cols) + 6)
+    ci = np.tile(np.array(cols + [16, 17, 18, 16 + 3 * 7, 16 + 3 * 7 + 1, 16 + 3 * 7 + 2]), 120)
+    L2, fd = orc.fd_frozen64(p64, s64, cam, 2, target, 0.2, gi, ci, h=1e-4, bg=bg)
+    assert L2 == pytest.approx(L, rel=1e-12)
+    assert not np.isnan(fd).any()
+    g = np.where(ci < 16, gp[gi, np.minimum(ci, 15)], gsh.reshape(120, -1)[gi, np.maximum(ci - 16, 0)])
+    groups = {"position": [0, 1, 2], "opacity": [3], "scale": [4, 5, 6], "rotation": [8, 9, 10, 11],
+              "dc": [12, 13, 14], "sh": list(range(16, 16 + 24))}
+    for name, cs in groups.items():
+        sel = np.isin(ci, cs)
+        scale = np.abs(g[sel]).max()
+        assert scale > 0
+        err = np.abs(g[sel] - fd[sel])
+        assert np.all(err <= 1e-5 * scale + 1e-4 * np.abs(fd[sel])), f"{name}: worst {err.max():.3g} of {scale:.3g}"
+    # plain differences of the loss with its jumps: mostly the same
+    rng = np.random.default_rng(3)
+    agree = 0
+    picks = rng.choice(len(gi), 40, replace=False)
+    for k in picks:
+        i, c = int(gi[k]), int(ci[k])
+        pp, pm = p64.copy(), p64.copy()
+        sp, sm = s64.copy(), s64.copy()
+        if c < 16:
+            pp[i, c] += 1e-4
+            pm[i, c] -= 1e-4
+        else:
+            sp[i].reshape(-1)[c - 16] += 1e-4
+            sm[i].reshape(-1)[c - 16] -= 1e-4
+        plain = (orc.loss64(pp, sp, cam, 2, target, 0.2, bg=bg) - orc.loss64(pm, sm, cam, 2, target, 0.2, bg=bg)) / 2e-4
+        agree += abs(plain - fd[k]) <= 1e-3 * abs(fd[k]) + 1e-9
+    assert agree >= 30, f"plain and frozen differences agree on only {agree} of 40 samples"
+
+
+def _quat_rotation_by_axis_angle(q):
+    """The rotation a quaternion (w, x, y, z) stands for, from first
+    principles: angle 2 atan2(|v|, w) about v / |v|, through Rodrigues'
+    formula R = I + sin(t) K + (1 - cos(t)) K^2 (K the cross-product matrix)."""
+    q = np.asarray(q, np.float64) / np.linalg.norm(q)
+    w, v = q[0], q[1:]
+    s = np.linalg.norm(v)
+    if s == 0.0:
+        return np.eye(3)
+    k = v / s
+    t = 2.0 * math.atan2(s, w)
+    K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+    return np.eye(3) + math.sin(t) * K + (1 - math.cos(t)) * K @ K
+
+
+def _splat_2d(p, cam, transpose=False, clamp=True):
+    """Screen mean and dilated 2D covariance of one gaussian, from first
+    principles: Sigma = R S^2 R^T in the world, the camera's rotation V and
+    the perspective Jacobian J at the camera-space mean (x/z and y/z clamped
+    to 1.3x the half field of view, the rasterizers' guard band), Sigma_2D =
+    J V Sigma V^T J^T + 0.3 I.  transpose / clamp=False: the two mistakes the
+    test must catch."""
+    V = np.asarray(cam.rotation, np.float64).reshape(3, 3)
+    t = V @ p[0:3] + np.asarray(cam.translation, np.float64)
+    R = _quat_rotation_by_axis_angle(p[8:12])
+    if transpose:
+        R = R.T
+    S = np.diag(np.exp(p[4:7]))
+    sigma = R @ S @ S @ R.T
+    fx, fy, cx, cy = cam.fx, cam.fy, cam.cx, cam.cy
+    u, v = t[0] / t[2], t[1] / t[2]
+    if clamp:
+        lx = 1.3 * max(cx, cam.width - cx) / fx
+        ly = 1.3 * max(cy, cam.height - cy) / fy
+        u, v = min(max(u, -lx), lx), min(max(v, -ly), ly)
+    J = np.array([[fx / t[2], 0.0, -fx * u / t[2]], [0.0, fy / t[2], -fy * v / t[2]]])
+    cov = J @ V @ sigma @ V.T @ J.T + 0.3 * np.eye(2)
+    mean = np.array([fx * t[0] / t[2] + cx, fy * t[1] / t[2] + cy])
+    return mean, cov
+
+
+@pytest.mark.parametrize("where", ["head-on", "off-axis-clamped"])
+def test_project64_rotated_anisotropic_closed_form(where):
+    """orc_project64's mean, dilated covariance and conic for rotated,
+    strongly anisotropic splats (scales 0.3 / 0.05 / 0.01, quaternions far from
+    the identity), seen head-on near the image centre and far outside the
+    field of view where the Jacobian's clamp is active, against the first-
+    principles construction above (1e-10 relative).  The same construction
+    with the rotation transposed, or without the clamp, misses the oracle by
+    far more: the test tells those mistakes apart."""
+    cam = gs.look_at_camera((0.4, 0.3, 3.0), (0, 0, 0), 160, 120)
+    rng = np.random.default_rng(12)
+    checked = 0
+    for _ in range(20):
+        p = np.zeros(16, np.float64)
+        p[3], p[7] = 2.0, 1.0
+        p[4:7] = np.log([0.3, 0.05, 0.01])
+        p[8:12] = rng.normal(size=4) + np.array([0.2, 0.0, 0.0, 0.0])
+        if where == "head-on":
+            p[0:3] = rng.uniform(-0.2, 0.2, 3)
+        else:
+            p[0:3] = np.array([7.0, -2.5, -0.5]) + rng.uniform(-0.5, 0.5, 3)
+        V = np.asarray(cam.rotation, np.float64).reshape(3, 3)
+        t = V @ p[0:3] + np.asarray(cam.translation, np.float64)
+        lx = 1.3 * max(cam.cx, cam.width - cam.cx) / cam.fx
+        if where != "head-on":
+            assert abs(t[0] / t[2]) > lx  # the clamp is active
+        out = orc.project64(p[None, :], None, cam, 0)[0]
+        assert out[0] == 1.0
+        mean, cov = _splat_2d(p, cam)
+        con = np.linalg.inv(cov)
+        np.testing.assert_allclose(out[1:3], mean, rtol=1e-10)
+        np.testing.assert_allclose(out[11:14], [cov[0, 0], cov[0, 1], cov[1, 1]], rtol=1e-10,
+                                   atol=1e-10 * np.abs(cov).max())
+        np.testing.assert_allclose(out[5:8], [con[0, 0], con[0, 1], con[1, 1]], rtol=1e-10,
+                                   atol=1e-10 * np.abs(con).max())
+        got = np.array([out[11], out[12], out[13]])
+        for wrong in (dict(transpose=True), dict(clamp=False)):
+            if where == "head-on" and "clamp" in wrong:
+                continue  # inside the field of view the clamp changes nothing
+            _, bad = _splat_2d(p, cam, **wrong)
+            miss = np.abs(np.array([bad[0, 0], bad[0, 1], bad[1, 1]]) - got).max() / np.abs(got).max()
+            assert miss > 1e-3, f"{wrong}: only {miss:.2g} off"
+        checked += 1
+    assert checked == 20
