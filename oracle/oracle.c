@@ -464,14 +464,23 @@ uint64_t orc_bin32(const float* records, const float* footprint, const uint32_t*
 }
 
 /* ===================== fp32 tile compositor (train_forward.comp:77-117) ===== */
-/* A splat evaluation whose skip / clamp / stop decision sits within 1e-5
- * relative of its threshold (power > 0, alpha < 1/255, raw alpha >= 0.99):
- * another fp32 evaluation order or exp (the GPU's, the Vulkan driver's) may
- * decide it the other way, so a comparison at the bit level is meaningless
- * there.  Pixels that evaluate such a splat are reported as ambiguous. */
-static int near_threshold(float power, float raw) {
-  const float a = 1.0f / 255.0f;
-  return power > -1e-6f || fabsf(raw - a) <= 1e-5f * a || fabsf(raw - 0.99f) <= 1e-5f * 0.99f;
+/* A splat evaluation whose skip / clamp / stop decision sits near its
+ * threshold (power > 0, alpha < 1/255, raw alpha >= 0.99): another fp32
+ * evaluation order or exp (the GPU's, the Vulkan driver's) may decide it the
+ * other way, so a comparison at the bit level is meaningless there.  "Near":
+ * within 1e-5 relative in alpha, plus the rounding of the power, which is a
+ * sum of terms that cancel (a dx^2 / 2, b dx dy, c dy^2 / 2: for a needle
+ * splat far along its axis each is ~100 while the power is ~-5), so another
+ * order of operations moves it by some ulp of the largest term: 2^-19 (16
+ * ulp) of `mag`, their magnitudes' sum.  Pixels that evaluate such a splat
+ * are reported as ambiguous. */
+static int near_threshold(float power, float raw, float mag) {
+  const float a = 1.0f / 255.0f, band = 1e-5f + 0x1p-19f * mag;
+  return power > -1e-6f - 0x1p-19f * mag || fabsf(raw - a) <= band * a || fabsf(raw - 0.99f) <= band * 0.99f;
+}
+
+static float power_magnitude(const float* r, float dx, float dy) {
+  return 0.5f * (fabsf(r[2]) * dx * dx + fabsf(r[4]) * dy * dy) + fabsf(r[3] * dx * dy);
 }
 
 /* image [H*W][4] = (C rgb, final T); last [H*W] = 1-based position of the
@@ -495,7 +504,7 @@ void orc_forward32(const float* records, const uint32_t* pair_gaussians, const u
         const float dx = r[0] - cx, dy = r[1] - cy;
         const float power = -0.5f * (r[2] * dx * dx + r[4] * dy * dy) - r[3] * dx * dy;
         const float raw = r[5] * expf(power);
-        amb |= near_threshold(power, raw);
+        amb |= near_threshold(power, raw, power_magnitude(r, dx, dy));
         if (power > 0.0f) continue;
         const float alpha = fminf(0.99f, raw);
         if (alpha < 1.0f / 255.0f) continue;
@@ -527,8 +536,17 @@ void orc_forward32(const float* records, const uint32_t* pair_gaussians, const u
  * (near_threshold): another fp32 evaluation may skip or clamp that splat the
  * other way and so shift every term of that pixel.  tainted [n] (may be null):
  * 1 for every gaussian with a term at an ambiguous pixel.  share [n][9] (may be
- * null): the sum of |term| over ambiguous pixels, per gaussian and value — how
- * far another evaluation's decisions can move the gaussian's sums. */
+ * null), per gaussian and value, how far another fp32 evaluation can move the
+ * sums: the |terms| at ambiguous pixels, plus every |term| times the pixel's
+ * conditioning and the accumulation's rounding.  The walk recovers T by
+ * dividing by 1 - alpha, so an alpha that another exp / evaluation order
+ * makes differ by d (relative) moves the recovered T, and every term behind
+ * it, by d alpha / (1 - alpha) relative; with d = 2^-20 (8 ulp) the pixel's
+ * conditioning is 2^-20 times the sum of alpha / (1 - alpha) over its
+ * contributors.  A sum of fp32 terms of either sign (the GPU's wave, LDS and
+ * global float additions) rounds to a few ulp of the sum of their
+ * magnitudes, which can far exceed the sum itself (a wide splat's conic
+ * terms cancel): 2^-18 (32 ulp) of every |term|. */
 void orc_backward32(const float* records, const uint32_t* pair_gaussians, const uint32_t* ranges, const float* image,
                     const uint32_t* last, const float* dldc, const float bg[3], uint32_t n, uint32_t width,
                     uint32_t height, double* grad2d, uint8_t* tainted, double* share) {
@@ -555,12 +573,17 @@ void orc_backward32(const float* records, const uint32_t* pair_gaussians, const 
         const float bg_dot = bg[0] * dC[0] + bg[1] * dC[1] + bg[2] * dC[2];
         float Tc = T_final, acc[3] = {0, 0, 0}, last_alpha = 0.0f, last_color[3] = {0, 0, 0};
         int amb = 0;
-        for (uint32_t k = start + last[pix]; k-- > start && !amb;) {
+        double cond = 0.0;
+        for (uint32_t k = start + last[pix]; k-- > start;) {
           const float* r = records + (size_t)pair_gaussians[k] * 12;
           const float dx = r[0] - cx, dy = r[1] - cy;
           const float power = -0.5f * (r[2] * dx * dx + r[4] * dy * dy) - r[3] * dx * dy;
-          amb = near_threshold(power, r[5] * expf(power));
+          const float raw = r[5] * expf(power);
+          amb |= near_threshold(power, raw, power_magnitude(r, dx, dy));
+          const double a = raw > 0.99f ? 0.99 : (double)raw;
+          if (power <= 0.0f && a >= 1.0 / 255.0) cond += a / (1.0 - a);
         }
+        cond = cond * 0x1p-20 + 0x1p-18;
         for (uint32_t k = start + last[pix]; k-- > start;) {
           const uint32_t i = pair_gaussians[k];
           const float* r = records + (size_t)i * 12;
@@ -570,19 +593,21 @@ void orc_backward32(const float* records, const uint32_t* pair_gaussians, const 
           const float raw = r[5] * G;
           const float alpha = fminf(0.99f, raw);
           if (amb && amb_pair) amb_pair[k] = 1; /* in or out of the sum, or weighted otherwise, elsewhere */
-          if (power > 0.0f || alpha < 1.0f / 255.0f) continue;
-          Tc = Tc / (1.0f - alpha);
-          for (int c = 0; c < 3; ++c) acc[c] = last_alpha * last_color[c] + (1.0f - last_alpha) * acc[c];
+          const int skipped = power > 0.0f || alpha < 1.0f / 255.0f;
+          /* A splat skipped here at a threshold may be composited by another
+           * evaluation: its term as if composited goes into the share. */
+          const int hypothetical = skipped && amb_terms && near_threshold(power, raw, power_magnitude(r, dx, dy));
+          if (skipped && !hypothetical) continue;
+          const float a_k = hypothetical ? fmaxf(alpha, 1.0f / 255.0f) : alpha;
+          const float T_k = Tc / (1.0f - a_k);
+          float acc_k[3];
+          for (int c = 0; c < 3; ++c) acc_k[c] = last_alpha * last_color[c] + (1.0f - last_alpha) * acc[c];
           float dL_dalpha = 0.0f;
-          for (int c = 0; c < 3; ++c) {
-            last_color[c] = r[6 + c];
-            dL_dalpha += (r[6 + c] - acc[c]) * dC[c];
-          }
-          dL_dalpha *= Tc;
-          last_alpha = alpha;
-          dL_dalpha += (-T_final / (1.0f - alpha)) * bg_dot;
+          for (int c = 0; c < 3; ++c) dL_dalpha += (r[6 + c] - acc_k[c]) * dC[c];
+          dL_dalpha *= T_k;
+          dL_dalpha += (-T_final / (1.0f - a_k)) * bg_dot;
           double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-          for (int c = 0; c < 3; ++c) v[6 + c] = (double)(alpha * Tc * dC[c]);
+          for (int c = 0; c < 3; ++c) v[6 + c] = (double)(a_k * T_k * dC[c]);
           if (raw < 0.99f) {
             const float dpow = raw * dL_dalpha;
             v[5] = (double)(G * dL_dalpha);
@@ -592,12 +617,19 @@ void orc_backward32(const float* records, const uint32_t* pair_gaussians, const 
             v[3] = (double)(dpow * (-dx * dy));
             v[4] = (double)(dpow * (-0.5f * dy * dy));
           }
+          if (amb_terms) {
+            double* a = amb_terms + (size_t)k * 9;
+            for (int c = 0; c < 9; ++c) a[c] += fabs(v[c]) * (amb ? 1.0 : cond);
+          }
+          if (hypothetical) continue;
+          Tc = T_k;
+          for (int c = 0; c < 3; ++c) {
+            acc[c] = acc_k[c];
+            last_color[c] = r[6 + c];
+          }
+          last_alpha = alpha;
           double* o = terms + (size_t)k * 9;
           for (int c = 0; c < 9; ++c) o[c] += v[c];
-          if (amb && amb_terms) {
-            double* a = amb_terms + (size_t)k * 9;
-            for (int c = 0; c < 9; ++c) a[c] += fabs(v[c]);
-          }
         }
       }
     }

@@ -288,8 +288,9 @@ def backward32(records, pair_gaussians, ranges, image, last, dldc, bg, width: in
     """grad2d [n, 9] (double sums) of the tile backward on the given forward state;
     with flags=True also tainted [n] (bool): gaussians that have a term at a pixel
     whose walk meets a splat at a skip/clamp threshold; with share=True also
-    share [n, 9]: the sum of those terms' magnitudes (how far another
-    evaluation's decisions can move each sum)."""
+    share [n, 9]: how far another fp32 evaluation can move each sum (those
+    terms' magnitudes, plus every term's magnitude times its pixel's
+    conditioning; oracle.c orc_backward32)."""
     records = np.ascontiguousarray(records, np.float32).reshape(-1, 12)
     n = records.shape[0]
     pg = np.ascontiguousarray(pair_gaussians, np.uint32)

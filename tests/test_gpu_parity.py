@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 from helpers import gs, make_camera, make_scene, noise_target, orc, smooth_target
+from parity_check import check_backward
 
 pytestmark = pytest.mark.gpu
 
@@ -195,20 +196,9 @@ def test_forward_loss_backward_match_oracle(name, n, deg, w, h, seed):
     sign_ok = np.abs(e) > 1e-6
     dd = np.abs(dldc[:, :3] - d_ref)
     assert dd[sign_ok].max() <= 1e-4 * np.abs(d_ref).max()
-    # backward: float-atomic sums against the oracle's tile backward on the same state
-    # Gaussians contributing at a pixel whose walk meets a threshold-ambiguous
-    # splat are held to a looser bound; every other one to 1e-4 of the largest.
-    o_g2, tainted = orc.backward32(rec, pg, ranges, img, last, dldc, bg, w, h, flags=True)
-    # (tainted = the gaussians drawn at the few ambiguous pixels, so a sanity
-    # bound only: the strict check must still cover most of them)
-    visible = tiles > 0
-    assert tainted.sum() <= 0.25 * max(1, visible.sum()), f"{tainted.sum()} tainted of {visible.sum()}"
-    for c in range(9):
-        scale = np.abs(o_g2[:, c]).max()
-        err = np.abs(g2[:, c] - o_g2[:, c])
-        assert err[~tainted].max() <= 1e-4 * scale, f"grad2d[{c}]: err {err[~tainted].max():.3g} vs max {scale:.3g}"
-        assert err.max() <= 5e-2 * scale, f"grad2d[{c}] (tainted): err {err.max():.3g} vs max {scale:.3g}"
-    print(f"[info] {name}: {tainted.sum()} of {visible.sum()} visible gaussians tainted", flush=True)
+    # backward: float-atomic sums against the oracle's tile backward on the same state, each
+    # gaussian within 1e-4 of the value's largest plus its share at ambiguous pixels (parity_check.py)
+    check_backward(rec, pg, ranges, img, last, dldc, bg, w, h, g2, tiles, name)
     print(f"[ok] {name}: pairs {pairs}, loss {L_gpu:.6f}", flush=True)
 
 
@@ -438,3 +428,52 @@ def test_train_views_matches_single_view_steps():
         assert np.abs(gp[:, cols] - mean_gp[:, cols]).max() <= 1e-5 * scale, cols
     assert np.abs(gsh - mean_gsh).max() <= 1e-5 * np.abs(mean_gsh).max()
     np.testing.assert_array_equal(stats[:, 1], np.sum(s_vis, axis=0).astype(np.float32))
+
+
+def test_gradients_at_config2_size_against_frozen_branch_differences():
+    """FP64 central differences (h = 1e-4) on 1000 sampled parameters of the
+    200k-gaussian SH3 1280x720 view, against the GPU's first-step gradient
+    (train_selftest.cpp:356-486's check_gradients at this size).  The
+    differences are of the fp64 loss with the compositor's branches frozen as
+    in the unperturbed render (oracle.c orc_fd_frozen64): at 1280x720 a 1e-4
+    move of a splat makes some pixel centre cross its 1/255 contour for most
+    splats, a jump the analytic gradient (the reference's and this one) leaves
+    out.  Samples: gaussians that drew (nonzero 2D gradient), every parameter
+    column and the 45 SH rest values.  Rule per sample: |g - fd| <= 0.05 |fd|
+    + 1e-4 max|fd| of its group + 2e-6 x (64 x 48) / (W x H) (the reference's
+    absolute floor, scaled from its 64x48 test image to this mean over 3WH
+    values); at most 5% of a group's samples off."""
+    case("gradients vs frozen-branch central differences at 200k / 1280x720")
+    n, deg, w, h, seed = 200_000, 3, 1280, 720, 42
+    params, sh = gs.synthetic_scene(n, deg, seed)
+    cam = gs.look_at_camera((0.0, 0.5, 3.0), (0, 0, 0), w, h)
+    target = smooth_target(w, h, seed)
+    bg, lam = (0.1, 0.2, 0.3), 0.2
+    with gs.Workspace(n, deg, w, h) as ws:
+        ws.load(params, sh)
+        ws.set_target(target)
+        ws.train_step(gs.make_frame(cam, n, deg, background=bg, tp=frozen(lam)), want_losses=False)
+        g2 = ws.download(gs.BUF_GRAD2D, np.float32, 9 * n).reshape(n, 9)
+        gp, gsh = _first_step_gradients(ws, n)
+    drew = np.nonzero(np.any(g2 != 0, axis=1))[0]
+    rng = np.random.default_rng(4004)
+    cols = np.array([0, 1, 2, 3, 4, 5, 6, 8, 9, 10, 11, 12, 13, 14] + [16 + k for k in range(45)])
+    gi = rng.choice(drew, 1000, replace=False).astype(np.uint32)
+    ci = rng.choice(cols, 1000).astype(np.uint32)
+    _, fd = orc.fd_frozen64(params.astype(np.float64), sh.astype(np.float64), cam, deg, target, lam, gi, ci, h=1e-4,
+                            bg=bg)
+    g = np.where(ci < 16, gp[gi, np.minimum(ci, 15)], gsh.reshape(n, -1)[gi, np.maximum(ci.astype(int) - 16, 0)])
+    assert not np.isnan(fd).any()
+    floor = 2e-6 * (64 * 48) / (w * h)
+    groups = {"position": [0, 1, 2], "opacity": [3], "scale": [4, 5, 6], "rotation": [8, 9, 10, 11],
+              "dc": [12, 13, 14], "sh rest": list(range(16, 61))}
+    for gname, cs in groups.items():
+        sel = np.isin(ci, cs)
+        big = np.abs(fd[sel]).max()
+        err = np.abs(g[sel] - fd[sel])
+        off = err > 0.05 * np.abs(fd[sel]) + 1e-4 * big + floor
+        rel = np.median(err / (np.abs(fd[sel]) + 1e-4 * big))
+        print(f"[info] {gname}: {sel.sum()} samples, {off.sum()} off, median error {rel:.2g} of |fd|, largest |fd| "
+              f"{big:.3g}", flush=True)
+        assert sel.sum() >= 10 and big > 0
+        assert off.sum() <= sel.sum() // 20, f"{gname}: {off.sum()} of {sel.sum()} off"

@@ -2,8 +2,8 @@
 engine/shaders/train_step.glsl:223-230): a gaussian that drew nothing in the
 step's views (off screen, or behind an opaque wall past where every pixel it
 covers had stopped) keeps its parameters, SH coefficients and every Adam
-moment bit for bit, while every gaussian that drew is stepped exactly as
-without the flag.  The moments start random, so a dense step would move the
+moment bit for bit, while every gaussian that drew is stepped as without
+the flag (to the float atomics' reordering between two runs).  The moments start random, so a dense step would move the
 gaussians that drew nothing (their momentum), which the test also checks.
 """
 import numpy as np
@@ -110,15 +110,16 @@ def test_sparse_adam_keeps_gaussians_that_drew_nothing(views):
     assert not drew[away].any()
     assert (~drew[hidden]).sum() >= 30, f"only {(~drew[hidden]).sum()} of 40 gaussians behind the wall drew nothing"
     assert (tiles[hidden] > 0).sum() >= 30  # binned into tiles (seen), yet past every pixel's stop
-    assert drew[:300].sum() >= 100 and drew[300:304].all()
+    assert drew[:300].sum() >= 100 and drew[300]  # the wall's front layer (the ones behind may be past the stop)
     idle = ~drew
     print(f"[info] {idle.sum()} of {n} gaussians drew nothing ({(idle & (tiles > 0)).sum()} of them in some tile)",
           flush=True)
     for name in ("params", "sh", "m", "v", "msh", "vsh"):
         np.testing.assert_array_equal(rows(name, sparse[name], idle), rows(name, before[name], idle),
                                       err_msg=f"{name} of gaussians that drew nothing moved")
-        np.testing.assert_array_equal(rows(name, sparse[name], drew), rows(name, dense[name], drew),
-                                      err_msg=f"{name} of gaussians that drew differ from a dense step")
+        # (two runs: the backward's float atomics may add in another order, an ulp or so)
+        np.testing.assert_allclose(rows(name, sparse[name], drew), rows(name, dense[name], drew), rtol=1e-5,
+                                   atol=1e-9, err_msg=f"{name} of gaussians that drew differ from a dense step")
     # Without the flag their momentum moves them.
     assert np.any(rows("params", dense["params"], idle) != rows("params", before["params"], idle))
     assert np.any(rows("sh", dense["sh"], idle) != rows("sh", before["sh"], idle))
