@@ -483,47 +483,81 @@ static float power_magnitude(const float* r, float dx, float dy) {
   return 0.5f * (fabsf(r[2]) * dx * dx + fabsf(r[4]) * dy * dy) + fabsf(r[3] * dx * dy);
 }
 
+/* Per pair, the power below which its splat's alpha is under 1/255:
+ * ln(1 / (255 o)).  A walk skips a splat whose power is below that by more
+ * than twice near_threshold's band without evaluating exp (it would be
+ * skipped anyway, and no other evaluation could draw it). */
+static float* faint_powers(const float* records, const uint32_t* pair_gaussians, uint64_t pairs) {
+  float* cut = (float*)malloc(sizeof(float) * (pairs > 0 ? pairs : 1));
+#pragma omp parallel for schedule(static)
+  for (uint64_t k = 0; k < pairs; ++k) cut[k] = logf(1.0f / (255.0f * records[(size_t)pair_gaussians[k] * 12 + 5]));
+  return cut;
+}
+
+static int clearly_faint(float power, float cut, float mag) { return power < cut - (2e-5f + 0x1p-18f * mag); }
+
 /* image [H*W][4] = (C rgb, final T); last [H*W] = 1-based position of the
  * last composited splat in its tile's range; ambiguous [H*W] (may be null):
  * 1 where the pixel met a splat at a decision threshold (near_threshold) or
  * stopped within 1e-5 relative of T = 1e-4. */
 void orc_forward32(const float* records, const uint32_t* pair_gaussians, const uint32_t* ranges, uint32_t width,
                    uint32_t height, float* image, uint32_t* last, uint8_t* ambiguous) {
-  const uint32_t tiles_x = (width + TILE - 1) / TILE;
-#pragma omp parallel for schedule(dynamic, 4)
-  for (uint32_t py = 0; py < height; ++py) {
-    for (uint32_t px = 0; px < width; ++px) {
-      const uint32_t tile = (py / TILE) * tiles_x + px / TILE;
-      const uint32_t start = ranges[2 * tile], end = ranges[2 * tile + 1];
-      const float cx = (float)px + 0.5f, cy = (float)py + 0.5f;
-      float T = 1.0f, C[3] = {0.0f, 0.0f, 0.0f};
-      uint32_t lst = 0;
-      int amb = 0;
-      for (uint32_t k = start; k < end; ++k) {
-        const float* r = records + (size_t)pair_gaussians[k] * 12;
-        const float dx = r[0] - cx, dy = r[1] - cy;
-        const float power = -0.5f * (r[2] * dx * dx + r[4] * dy * dy) - r[3] * dx * dy;
-        const float raw = r[5] * expf(power);
-        amb |= near_threshold(power, raw, power_magnitude(r, dx, dy));
-        if (power > 0.0f) continue;
-        const float alpha = fminf(0.99f, raw);
-        if (alpha < 1.0f / 255.0f) continue;
-        const float next_T = T * (1.0f - alpha);
-        amb |= fabsf(next_T - 1e-4f) <= 1e-5f * 1e-4f;
-        if (next_T < 1e-4f) break;
-        for (int c = 0; c < 3; ++c) C[c] += r[6 + c] * alpha * T;
-        T = next_T;
-        lst = k - start + 1;
-      }
-      const size_t pix = (size_t)py * width + px;
-      image[pix * 4 + 0] = C[0];
-      image[pix * 4 + 1] = C[1];
-      image[pix * 4 + 2] = C[2];
-      image[pix * 4 + 3] = T;
-      last[pix] = lst;
-      if (ambiguous) ambiguous[pix] = (uint8_t)amb;
-    }
+  const uint32_t tiles_x = (width + TILE - 1) / TILE, tiles_y = (height + TILE - 1) / TILE;
+  const uint32_t T = tiles_x * tiles_y;
+  uint64_t total = 0, longest = 0;
+  for (uint32_t t = 0; t <= T; ++t) {
+    if (ranges[2 * t + 1] > total) total = ranges[2 * t + 1];
+    if (t < T && ranges[2 * t + 1] - ranges[2 * t] > longest) longest = ranges[2 * t + 1] - ranges[2 * t];
   }
+  float* cut = faint_powers(records, pair_gaussians, total);
+#pragma omp parallel
+  {
+    /* A tile's records copied together (the same values, so the same
+     * results; the pixels of a tile then read them from one block). */
+    float* staged = (float*)malloc(sizeof(float) * 12 * (longest > 0 ? longest : 1));
+#pragma omp for schedule(dynamic, 1)
+    for (uint32_t tile = 0; tile < T; ++tile) {
+      const uint32_t start = ranges[2 * tile], end = ranges[2 * tile + 1];
+      for (uint32_t k = start; k < end; ++k)
+        memcpy(staged + (size_t)(k - start) * 12, records + (size_t)pair_gaussians[k] * 12, 48);
+      const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
+      for (uint32_t py = ty * TILE; py < (ty + 1) * TILE && py < height; ++py) {
+        for (uint32_t px = tx * TILE; px < (tx + 1) * TILE && px < width; ++px) {
+          const float cx = (float)px + 0.5f, cy = (float)py + 0.5f;
+          float Tr = 1.0f, C[3] = {0.0f, 0.0f, 0.0f};
+          uint32_t lst = 0;
+          int amb = 0;
+          for (uint32_t k = start; k < end; ++k) {
+            const float* r = staged + (size_t)(k - start) * 12;
+            const float dx = r[0] - cx, dy = r[1] - cy;
+            const float power = -0.5f * (r[2] * dx * dx + r[4] * dy * dy) - r[3] * dx * dy;
+            const float mag = power_magnitude(r, dx, dy);
+            if (clearly_faint(power, cut[k], mag)) continue;
+            const float raw = r[5] * expf(power);
+            amb |= near_threshold(power, raw, mag);
+            if (power > 0.0f) continue;
+            const float alpha = fminf(0.99f, raw);
+            if (alpha < 1.0f / 255.0f) continue;
+            const float next_T = Tr * (1.0f - alpha);
+            amb |= fabsf(next_T - 1e-4f) <= 1e-5f * 1e-4f;
+            if (next_T < 1e-4f) break;
+            for (int c = 0; c < 3; ++c) C[c] += r[6 + c] * alpha * Tr;
+            Tr = next_T;
+            lst = k - start + 1;
+          }
+          const size_t pix = (size_t)py * width + px;
+          image[pix * 4 + 0] = C[0];
+          image[pix * 4 + 1] = C[1];
+          image[pix * 4 + 2] = C[2];
+          image[pix * 4 + 3] = Tr;
+          last[pix] = lst;
+          if (ambiguous) ambiguous[pix] = (uint8_t)amb;
+        }
+      }
+    }
+    free(staged);
+  }
+  free(cut);
 }
 
 /* ===================== fp32 tile backward (train_backward_serial.glsl) ====== */
@@ -560,9 +594,18 @@ void orc_backward32(const float* records, const uint32_t* pair_gaussians, const 
   double* terms = (double*)calloc((size_t)(total > 0 ? total : 1) * 9, sizeof(double));
   double* amb_terms = share ? (double*)calloc((size_t)(total > 0 ? total : 1) * 9, sizeof(double)) : NULL;
   uint8_t* amb_pair = tainted ? (uint8_t*)calloc((size_t)(total > 0 ? total : 1), 1) : NULL;
-#pragma omp parallel for schedule(dynamic, 1)
+  float* cut = faint_powers(records, pair_gaussians, total);
+  uint64_t longest = 0;
+  for (uint32_t t = 0; t < T; ++t)
+    if (ranges[2 * t + 1] - ranges[2 * t] > longest) longest = ranges[2 * t + 1] - ranges[2 * t];
+#pragma omp parallel
+  {
+  float* staged = (float*)malloc(sizeof(float) * 12 * (longest > 0 ? longest : 1));
+#pragma omp for schedule(dynamic, 1)
   for (uint32_t tile = 0; tile < T; ++tile) {
     const uint32_t start = ranges[2 * tile];
+    for (uint32_t k = start; k < ranges[2 * tile + 1]; ++k)
+      memcpy(staged + (size_t)(k - start) * 12, records + (size_t)pair_gaussians[k] * 12, 48);
     const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
     for (uint32_t py = ty * TILE; py < (ty + 1) * TILE && py < height; ++py) {
       for (uint32_t px = tx * TILE; px < (tx + 1) * TILE && px < width; ++px) {
@@ -574,21 +617,23 @@ void orc_backward32(const float* records, const uint32_t* pair_gaussians, const 
         float Tc = T_final, acc[3] = {0, 0, 0}, last_alpha = 0.0f, last_color[3] = {0, 0, 0};
         int amb = 0;
         double cond = 0.0;
-        for (uint32_t k = start + last[pix]; k-- > start;) {
-          const float* r = records + (size_t)pair_gaussians[k] * 12;
+        for (uint32_t k = start + last[pix]; (tainted || share) && k-- > start;) {
+          const float* r = staged + (size_t)(k - start) * 12;
           const float dx = r[0] - cx, dy = r[1] - cy;
           const float power = -0.5f * (r[2] * dx * dx + r[4] * dy * dy) - r[3] * dx * dy;
+          const float mag = power_magnitude(r, dx, dy);
+          if (clearly_faint(power, cut[k], mag)) continue;
           const float raw = r[5] * expf(power);
-          amb |= near_threshold(power, raw, power_magnitude(r, dx, dy));
+          amb |= near_threshold(power, raw, mag);
           const double a = raw > 0.99f ? 0.99 : (double)raw;
           if (power <= 0.0f && a >= 1.0 / 255.0) cond += a / (1.0 - a);
         }
         cond = cond * 0x1p-20 + 0x1p-18;
         for (uint32_t k = start + last[pix]; k-- > start;) {
-          const uint32_t i = pair_gaussians[k];
-          const float* r = records + (size_t)i * 12;
+          const float* r = staged + (size_t)(k - start) * 12;
           const float dx = r[0] - cx, dy = r[1] - cy;
           const float power = -0.5f * (r[2] * dx * dx + r[4] * dy * dy) - r[3] * dx * dy;
+          if (clearly_faint(power, cut[k], power_magnitude(r, dx, dy))) continue;
           const float G = expf(power);
           const float raw = r[5] * G;
           const float alpha = fminf(0.99f, raw);
@@ -634,6 +679,8 @@ void orc_backward32(const float* records, const uint32_t* pair_gaussians, const 
       }
     }
   }
+  free(staged);
+  }
   memset(grad2d, 0, sizeof(double) * n * 9);
   if (tainted) memset(tainted, 0, n);
   if (share) memset(share, 0, sizeof(double) * n * 9);
@@ -649,6 +696,7 @@ void orc_backward32(const float* records, const uint32_t* pair_gaussians, const 
   free(terms);
   free(amb_terms);
   free(amb_pair);
+  free(cut);
 }
 
 /* ===================== viewer renderer (splat_preprocess.comp, splat.vert/.frag) === */
@@ -829,6 +877,7 @@ typedef struct {
   double cov[3]; /* dilated a, b, c */
   double color[3];
   int clamped[3];
+  double faint; /* ln(1 / (255 opacity)): below this power alpha < 1/255 */
 } ref_t;
 
 static void quat_matrix64(const double q[4], double R[3][3]) {
@@ -886,6 +935,7 @@ static ref_t project64(const orc_camera* cam, const double* p, const double* sh,
   const double det = A * C - B * B;
   if (det <= 0) return r;
   r.visible = 1;
+  r.faint = -log(255.0 * opacity);
   r.mx = fx * t[0] / z + cx;
   r.my = fy * t[1] / z + cy;
   r.depth = z;
@@ -1049,7 +1099,7 @@ static double composite64(const scene64_t* s, const tiles64_t* t, uint32_t x, ui
     const ref_t* g = &s->refs[t->list[k]];
     const double dx = g->mx - (x + 0.5), dy = g->my - (y + 0.5);
     const double power = -0.5 * (g->con[0] * dx * dx + g->con[2] * dy * dy) - g->con[1] * dx * dy;
-    if (power > 0) continue;
+    if (power > 0 || power < g->faint - 1e-9) continue; /* (the second: alpha < 1/255, exp not needed) */
     double alpha = g->opacity * exp(power);
     if (alpha > MAX_ALPHA) alpha = MAX_ALPHA;
     if (alpha < MIN_ALPHA) continue;

@@ -387,3 +387,76 @@ def view_render32(params, sh, cam, degree: int, antialiased: bool = False, clear
                                 ctypes.byref(camera(cam)), degree, int(antialiased), _p(cl, ctypes.c_float),
                                 _p(out, ctypes.c_uint8), ctypes.byref(vis))
     return out, int(vis.value)
+
+
+def adam_rates(n: int, lr_position: float, lr_opacity: float, lr_scale: float, lr_rotation: float,
+               lr_sh_dc: float) -> np.ndarray:
+    """[n, 16] per-parameter learning rates in the slot layout
+    (train_step.glsl:232-240): position, opacity | scale, - | rotation | DC, -."""
+    lr = np.zeros(16, np.float32)
+    lr[0:3] = lr_position
+    lr[3] = lr_opacity
+    lr[4:7] = lr_scale
+    lr[8:12] = lr_rotation
+    lr[12:15] = lr_sh_dc
+    return np.tile(lr, (n, 1))
+
+
+def train(params, sh, deg: int, cams, targets, steps: int, lr, lr_sh_rest: float, lam: float = 0.2, order=None,
+          bg=(0.0, 0.0, 0.0), step0: int = 0, state=None):
+    """`steps` training steps of the CPU restatement, one view each (view
+    order[k], or k mod views): fp32 projection, binning, compositing and tile
+    backward; fp64 loss, SSIM and projection adjoint; fp32 Adam with
+    beta 0.9 / 0.999, eps 1e-15 and the bias corrections of step step0 + k + 1.
+    lr: [n, 16] rates (adam_rates).  state: the Adam moments (m, v, m_sh,
+    v_sh) to continue from, kept in this dict across calls when given (a new
+    dict starts at zero).  Returns (params, sh) after the steps."""
+    p, s = np.array(params, np.float32, copy=True), np.array(sh, np.float32, copy=True)
+    if state is None:
+        state = {}
+    if not state:
+        state.update(m=np.zeros_like(p), v=np.zeros_like(p), msh=np.zeros_like(s), vsh=np.zeros_like(s))
+    m, v, msh, vsh = state["m"], state["v"], state["msh"], state["vsh"]
+    lr_rest = np.full(s.shape, lr_sh_rest, np.float32)
+    for k in range(steps):
+        view = int(order[k]) if order is not None else k % len(cams)
+        cam, target = cams[view], targets[view]
+        w, h = cam.width, cam.height
+        rec, foot, tiles, keys = project32(p, s, cam, deg)
+        pt, pg, ranges = bin32(rec, foot, tiles, keys, w, h)
+        img, last = forward32(rec, pg, ranges, w, h)
+        x = img[:, :3].astype(np.float64) + img[:, 3:4].astype(np.float64) * np.asarray(bg, np.float64)
+        y = np.asarray(target).reshape(-1, 4)[:, :3] / 255.0
+        _, dldc = loss_image64(x, y, w, h, lam, grad=True)
+        d4 = np.zeros((w * h, 4), np.float32)
+        d4[:, :3] = dldc
+        g2 = backward32(rec, pg, ranges, img, last, d4, bg, w, h)
+        gp, gsh = adjoint64(p, s, cam, deg, g2)
+        b1, b2 = 1.0 / (1.0 - 0.9 ** (step0 + k + 1)), 1.0 / (1.0 - 0.999 ** (step0 + k + 1))
+        p, m, v = adam(p, gp.astype(np.float32), m, v, lr, bias1=b1, bias2=b2)
+        if s.size:
+            s, msh, vsh = adam(s, gsh.astype(np.float32), msh, vsh, lr_rest, bias1=b1, bias2=b2)
+    state.update(m=m, v=v, msh=msh, vsh=vsh)
+    return p, s
+
+
+def mean_psnr(params, sh, deg: int, cams, targets, bg=(0.0, 0.0, 0.0)) -> float:
+    """Mean over the views of the PSNR of the fp64 render against the RGBA8 targets' colour."""
+    out = []
+    for cam, t in zip(cams, targets):
+        rgb, _ = render64(params, sh, cam, deg, bg)
+        y = np.asarray(t).reshape(-1, 4)[:, :3] / 255.0
+        out.append(-10.0 * np.log10(max(float(np.mean((rgb - y) ** 2)), 1e-12)))
+    return float(np.mean(out))
+
+
+def render_targets64(params, sh, deg: int, cams):
+    """RGBA8 targets (alpha 255) from the fp64 render, rounded to nearest."""
+    out = []
+    for cam in cams:
+        rgb, _ = render64(params, sh, cam, deg)
+        t = np.zeros((cam.height, cam.width, 4), np.uint8)
+        t[..., :3] = np.clip(np.rint(rgb.reshape(cam.height, cam.width, 3) * 255.0), 0, 255).astype(np.uint8)
+        t[..., 3] = 255
+        out.append(t)
+    return out

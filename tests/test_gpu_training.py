@@ -122,3 +122,47 @@ def test_training_psnr_matches_oracle_after_n_steps(name, n, deg, w, h, views, s
     assert o_psnr > start + 0.5, "the oracle's training does not improve the fit"
     assert abs(g_psnr - o_psnr) <= 0.01
     assert dp <= 1e-3 * max(1.0, np.abs(o_p).max())
+
+
+def test_small_scene_300_steps_matches_the_oracle_trajectory():
+    """The small-scene configuration in full (tests/golden/make_c1_trajectory.py:
+    20k gaussians SH0, 24 orbit views at 360x640, 0.8 L1 + 0.2 D-SSIM, Adam
+    1.6e-4 / 5e-3, 300 steps, no densification): the HIP trainer's model after
+    300 steps has the mean PSNR (fp64 render over the 24 views) of the oracle's
+    CPU trajectory, committed as a fixture, within 0.01 dB."""
+    import hashlib
+    import json
+    import os
+    print("[case] 20k SH0 360x640, 24 views, 300 steps vs the oracle's trajectory", flush=True)
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    with open(os.path.join(here, "c1_trajectory.json")) as f:
+        want = json.load(f)
+    n, w, h, views, steps = 20_000, 360, 640, 24, want["steps"]
+    cams = gs.orbit_cameras(views, w, h)
+    gt_p, gt_sh = gs.synthetic_scene(n, 0, 1001)
+    targets = orc.render_targets64(gt_p, gt_sh, 0, cams)
+    digest = hashlib.sha256()
+    for t in targets:
+        digest.update(t.tobytes())
+    assert digest.hexdigest() == want["targets_sha256"], "the regenerated targets differ from the fixture's"
+    params, sh = gs.synthetic_scene(n, 0, 2002)
+    tp = gs.TrainParams(lambda_dssim=0.2, lr_position=1.6e-4, lr_scale=5e-3, lr_rotation=5e-3, lr_opacity=5e-3,
+                        lr_sh_dc=5e-3, lr_sh_rest=5e-3)
+    bufs = [gs.DeviceBuffer(t) for t in targets]
+    trace = {}
+    try:
+        with gs.Workspace(n, 0, w, h) as ws:
+            ws.load(params, sh)
+            for k in range(steps):
+                ws.set_target_device(bufs[k % views].ptr)
+                ws.train_step(gs.make_frame(cams[k % views], n, 0, step=k, tp=tp), want_losses=False)
+                if (k + 1) in (100, 200, steps):
+                    p, s = ws.store(n)
+                    trace[k + 1] = orc.mean_psnr(p, s, 0, cams, targets)
+    finally:
+        for b in bufs:
+            b.free()
+    for k, v in trace.items():
+        print(f"[info] step {k}: GPU {v:.4f} dB, oracle {want['psnr_every_25_steps'][str(k)]:.4f} dB", flush=True)
+    assert want["psnr_final"] > want["psnr_start"] + 8.0
+    assert abs(trace[steps] - want["psnr_final"]) <= 0.01
