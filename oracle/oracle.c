@@ -497,11 +497,15 @@ static float* faint_powers(const float* records, const uint32_t* pair_gaussians,
 static int clearly_faint(float power, float cut, float mag) { return power < cut - (2e-5f + 0x1p-18f * mag); }
 
 /* image [H*W][4] = (C rgb, final T); last [H*W] = 1-based position of the
- * last composited splat in its tile's range; ambiguous [H*W] (may be null):
+ * last composited splat in its tile's range; spread [H*W] (may be null): how
+ * far another fp32 evaluation of the same splats can move the pixel's values
+ * (each contributor's alpha uncertain by 1e-5 plus 16 ulp of its power's
+ * terms, through its weight in C and T) -- ill-conditioned where needle
+ * splats far along their axis cancel terms of ~1e5; ambiguous [H*W] (may be null):
  * 1 where the pixel met a splat at a decision threshold (near_threshold) or
  * stopped within 1e-5 relative of T = 1e-4. */
 void orc_forward32(const float* records, const uint32_t* pair_gaussians, const uint32_t* ranges, uint32_t width,
-                   uint32_t height, float* image, uint32_t* last, uint8_t* ambiguous) {
+                   uint32_t height, float* image, uint32_t* last, uint8_t* ambiguous, float* spread) {
   const uint32_t tiles_x = (width + TILE - 1) / TILE, tiles_y = (height + TILE - 1) / TILE;
   const uint32_t T = tiles_x * tiles_y;
   uint64_t total = 0, longest = 0;
@@ -527,6 +531,7 @@ void orc_forward32(const float* records, const uint32_t* pair_gaussians, const u
           float Tr = 1.0f, C[3] = {0.0f, 0.0f, 0.0f};
           uint32_t lst = 0;
           int amb = 0;
+          double sp = 0.0;
           for (uint32_t k = start; k < end; ++k) {
             const float* r = staged + (size_t)(k - start) * 12;
             const float dx = r[0] - cx, dy = r[1] - cy;
@@ -542,6 +547,10 @@ void orc_forward32(const float* records, const uint32_t* pair_gaussians, const u
             amb |= fabsf(next_T - 1e-4f) <= 1e-5f * 1e-4f;
             if (next_T < 1e-4f) break;
             for (int c = 0; c < 3; ++c) C[c] += r[6 + c] * alpha * Tr;
+            /* alpha's relative uncertainty under another evaluation (16 ulp of
+             * the power's terms, 1e-5 for exp), times how far it moves C and T */
+            const double cmax = fmax(fmax(r[6], r[7]), fmax(r[8], 1.0f));
+            sp += (1e-5 + 0x1p-19 * (double)mag) * alpha * (Tr * 2.0 * cmax + 1.0 / (1.0 - alpha));
             Tr = next_T;
             lst = k - start + 1;
           }
@@ -552,6 +561,7 @@ void orc_forward32(const float* records, const uint32_t* pair_gaussians, const u
           image[pix * 4 + 3] = Tr;
           last[pix] = lst;
           if (ambiguous) ambiguous[pix] = (uint8_t)amb;
+          if (spread) spread[pix] = (float)sp;
         }
       }
     }

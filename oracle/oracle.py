@@ -46,7 +46,7 @@ def library() -> ctypes.CDLL:
             "orc_footprint_tiles32": ([f32, f32, f32, f32, f32, f32, u32, u32, P(u32), u32], u32),
             "orc_project32": ([P(f32), P(f32), u32, u32, cam_p, u32, i32, P(f32), P(f32), P(u32), P(u32)], None),
             "orc_bin32": ([P(f32), P(f32), P(u32), P(u32), u32, u32, u32, P(u32), P(u32), u64, P(u32)], u64),
-            "orc_forward32": ([P(f32), P(u32), P(u32), u32, u32, P(f32), P(u32), P(ctypes.c_uint8)], None),
+            "orc_forward32": ([P(f32), P(u32), P(u32), u32, u32, P(f32), P(u32), P(ctypes.c_uint8), P(f32)], None),
             "orc_project64": ([P(f64), P(f64), u32, u32, cam_p, i32, P(f64)], None),
             "orc_render64": ([P(f64), P(f64), u32, u32, cam_p, i32, P(f64), P(f64), P(f64)], None),
             "orc_ssim64": ([P(f64), P(f64), u32, u32, f64, f64, P(f64)], f64),
@@ -145,21 +145,28 @@ def bin32(records, foot, tiles, keys, width: int, height: int):
     return pt[:total], pg[:total], ranges
 
 
-def forward32(records, pair_gaussians, ranges, width: int, height: int, flags: bool = False):
+def forward32(records, pair_gaussians, ranges, width: int, height: int, flags: bool = False, spread: bool = False):
     """(image [HW,4], last [HW]) of the fp32 tile compositor; with flags=True also
-    ambiguous [HW] (bool): pixels that met a splat at a skip/clamp/stop threshold."""
+    ambiguous [HW] (bool): pixels that met a splat at a skip/clamp/stop threshold;
+    with spread=True also spread [HW]: how far another fp32 evaluation of the same
+    splats can move each pixel's values (oracle.c orc_forward32)."""
     image = np.zeros((height * width, 4), np.float32)
     last = np.zeros(height * width, np.uint32)
     amb = np.zeros(height * width, np.uint8)
+    spr = np.zeros(height * width, np.float32)
     pg = np.ascontiguousarray(pair_gaussians, np.uint32)
     if pg.size == 0:
         pg = np.zeros(1, np.uint32)
     library().orc_forward32(_p(np.ascontiguousarray(records, np.float32), ctypes.c_float), _p(pg, ctypes.c_uint32),
                             _p(np.ascontiguousarray(ranges, np.uint32), ctypes.c_uint32), width, height,
-                            _p(image, ctypes.c_float), _p(last, ctypes.c_uint32), _p(amb, ctypes.c_uint8))
+                            _p(image, ctypes.c_float), _p(last, ctypes.c_uint32), _p(amb, ctypes.c_uint8),
+                            _p(spr, ctypes.c_float))
+    out = (image, last)
     if flags:
-        return image, last, amb.astype(bool)
-    return image, last
+        out += (amb.astype(bool),)
+    if spread:
+        out += (spr,)
+    return out
 
 
 def _scene64(params, sh):
