@@ -215,6 +215,21 @@ int gs_hip_densify(gs_hip_workspace* ws, const gs_hip_frame* frame, uint32_t* sl
 /* Caps every alive opacity logit at max_logit and clears its Adam moments
  * (train_reset_opacity.comp:17-23). */
 int gs_hip_reset_opacity(gs_hip_workspace* ws, const gs_hip_frame* frame, float max_logit);
+/* The loss stage of gs_hip_loss on caller-owned device buffers, allocating
+ * nothing.  No counterpart in the reference: for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  image: float4 [H*W] (C rgb, T), target: RGBA8 [H*W],
+ * dldc: float4 [H*W] written (dL/dC rgb, 0); the frame supplies the size,
+ * tile grid, background, lambda, SSIM constants and flags (MASKED_TARGET,
+ * EXPOSURE).  The SSIM maps and per-tile sums live in a file-scope device
+ * array of the library (about 73 MiB, held by each GPU it runs on until the
+ * process ends), so at most 2^21 pixels and 2^14 tiles (GS_HIP_E_CAPACITY
+ * beyond).  stream: a hipStream_t, or NULL for the null stream; every launch
+ * goes there.  losses (may be NULL: then nothing waits, and the call can be
+ * captured): sum |e|, sum e^2, sum SSIM, copied back after a wait on the
+ * stream.  Calls that overlap on one GPU share the array and race. */
+int gs_hip_loss_noalloc(const gs_hip_frame* frame, const void* image, const void* target, void* dldc, void* stream,
+                        double losses[3]);
 /* One full training view-step: project, bin, forward, loss, backward, adam. */
 int gs_hip_train_step(gs_hip_workspace* ws, const gs_hip_frame* frame, uint64_t* pairs, double losses[3]);
 /* One batched optimizer step over `count` views of the scene (count <= the

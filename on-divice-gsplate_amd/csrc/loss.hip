@@ -11,6 +11,8 @@
 // k_loss_grad: blurs A, B, C back with the same window and forms
 //   dL/dx_q = (1 - lambda) sign(e_q) / (3HW)
 //             - lambda / (3HW) ((G*A)(q) + 2 x_q (G*B)(q) + y_q (G*C)(q)).
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "gs_common.h"
@@ -385,3 +387,115 @@ int gs_hip_exposure_sums(gs_hip_workspace* ws, uint32_t view, double sums[8]) {
 }
 
 }  // extern "C"
+
+// ---- the loss without allocation -------------------------------------------------
+// gs_hip_loss_noalloc runs k_ssim and k_loss_grad on caller-owned image,
+// target and dL/dC buffers; everything else they use lives in `scratch`, a
+// file-scope device array each GPU holds from the library's load to the
+// process's end: [0, 1 KiB) the one-view table, then the three loss sums
+// (double), the per-tile loss and exposure-fit partial sums, and the nine
+// SSIM gradient maps.  Calls that overlap on one GPU share it and race.
+namespace {
+
+constexpr uint32_t kNoAllocMaxPixels = 1u << 21;  // e.g. 1920 x 1080 = 2,073,600
+constexpr uint32_t kNoAllocMaxTiles = 1u << 14;   // 1920 x 1080: 120 x 68 = 8160
+constexpr size_t kNoAllocSums = 1024;             // double[3] after the table
+constexpr size_t kNoAllocTileLoss = 4096;
+constexpr size_t kNoAllocTileFit = kNoAllocTileLoss + size_t(kNoAllocMaxTiles) * 16;
+constexpr size_t kNoAllocMaps = kNoAllocTileFit + size_t(kNoAllocMaxTiles) * 32;
+constexpr size_t kNoAllocBytes = kNoAllocMaps + size_t(9) * kNoAllocMaxPixels * 4;  // ~72.8 MiB
+static_assert(sizeof(gsh::ViewDev) <= kNoAllocSums, "the view table must fit before the sums");
+
+__device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// The device address of this GPU's `scratch`, looked up on its first call
+// with that device current (the host-side name is not a device address).
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[64] = {};
+
+int scratch_address(void** out) {
+  int dev = 0;
+  GS_HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return gsh::set_error(GS_HIP_E_STATE, "loss_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (rc != hipSuccess || p == nullptr) {
+      return gsh::set_error(GS_HIP_E_HIP, std::string("loss_noalloc: hipGetSymbolAddress(scratch): ") +
+                                              hipGetErrorString(rc));
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+// Writes the one-view table into scratch (the loss kernels read their view
+// from device memory; a kernel writes it so the call stays capturable).
+__global__ void k_put_view(gsh::ViewDev v, gsh::ViewDev* __restrict__ dst) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = v;
+}
+
+// The three loss sums over the tiles, in tile order, in double (the order
+// gs_hip_loss sums them on the host).
+__global__ void k_sum_tile_losses(const float4* __restrict__ tile_loss, uint32_t tiles, double* __restrict__ sums) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (uint32_t t = 0; t < tiles; ++t) {
+    const float4 v = tile_loss[t];
+    s0 += v.x;
+    s1 += v.y;
+    s2 += v.z;
+  }
+  sums[0] = s0, sums[1] = s1, sums[2] = s2;
+}
+
+}  // namespace
+
+extern "C" int gs_hip_loss_noalloc(const gs_hip_frame* f, const void* image, const void* target, void* dldc,
+                                   void* stream, double losses[3]) {
+  gsh::clear_error();
+  if (f == nullptr || image == nullptr || target == nullptr || dldc == nullptr) {
+    return gsh::set_error(GS_HIP_E_ARGUMENT, "loss_noalloc: null argument");
+  }
+  const uint32_t w = f->cam.width, h = f->cam.height;
+  const uint64_t pixels = uint64_t(w) * h, tiles = uint64_t(f->tiles_x) * f->tiles_y;
+  if (w == 0 || h == 0 || f->tiles_x != (w + kTile - 1) / kTile || f->tiles_y != (h + kTile - 1) / kTile) {
+    return gsh::set_error(GS_HIP_E_ARGUMENT, "loss_noalloc: image size and tile grid disagree");
+  }
+  if (pixels > kNoAllocMaxPixels || tiles > kNoAllocMaxTiles) {
+    return gsh::set_error(GS_HIP_E_CAPACITY, "loss_noalloc: " + std::to_string(w) + "x" + std::to_string(h) +
+                                                 " exceeds the scratch limit of " + std::to_string(kNoAllocMaxPixels) +
+                                                 " pixels and " + std::to_string(kNoAllocMaxTiles) + " tiles");
+  }
+  void* base = nullptr;
+  if (int rc = scratch_address(&base)) return rc;
+  unsigned char* s = static_cast<unsigned char*>(base);
+  gsh::ViewDev v{};
+  v.f = *f;
+  v.image = static_cast<float4*>(const_cast<void*>(image));
+  v.target = static_cast<const uint32_t*>(target);
+  v.dldc = static_cast<float4*>(dldc);
+  v.maps = reinterpret_cast<float*>(s + kNoAllocMaps);
+  v.tile_loss = reinterpret_cast<float4*>(s + kNoAllocTileLoss);
+  v.tile_fit = reinterpret_cast<float4*>(s + kNoAllocTileFit);
+  v.tile_count = uint32_t(tiles);
+  auto* table = reinterpret_cast<gsh::ViewDev*>(s);
+  double* sums = reinterpret_cast<double*>(s + kNoAllocSums);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  k_put_view<<<1, 64, 0, st>>>(v, table);
+  GS_CHECK_LAUNCH("k_put_view");
+  const dim3 grid(uint32_t(tiles), 1);
+  k_ssim<<<grid, 256, 0, st>>>(table);
+  GS_CHECK_LAUNCH("k_ssim");
+  k_loss_grad<<<grid, 256, 0, st>>>(table);
+  GS_CHECK_LAUNCH("k_loss_grad");
+  if (losses != nullptr) {
+    k_sum_tile_losses<<<1, 64, 0, st>>>(v.tile_loss, uint32_t(tiles), sums);
+    GS_CHECK_LAUNCH("k_sum_tile_losses");
+    GS_HIP_TRY(hipMemcpyAsync(losses, sums, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+    GS_HIP_TRY(hipStreamSynchronize(st));
+  }
+  return 0;
+}

@@ -44,7 +44,7 @@ EXPORTED = [
     "gs_hip_workspace_overflows", "gs_hip_workspace_timings", "gs_hip_view_render",
     "gs_hip_batch_create", "gs_hip_batch_destroy", "gs_hip_batch_train", "gs_hip_batch_get_tuning",
     "gs_hip_batch_set_tuning", "gs_hip_batch_timings", "gs_hip_pose_gradient", "gs_hip_exposure_sums",
-    "gs_hip_spz_info", "gs_hip_workspace_load_spz", "gs_hip_export_ply",
+    "gs_hip_spz_info", "gs_hip_workspace_load_spz", "gs_hip_export_ply", "gs_hip_loss_noalloc",
 ]
 
 
@@ -108,6 +108,7 @@ def library() -> ctypes.CDLL:
             "gs_hip_project": [vp, frame_p], "gs_hip_bin": [vp, frame_p, u64p], "gs_hip_forward": [vp, frame_p],
             "gs_hip_set_target": [vp, ctypes.POINTER(ctypes.c_uint8)], "gs_hip_set_target_device": [vp, vp],
             "gs_hip_loss": [vp, frame_p, dp],
+            "gs_hip_loss_noalloc": [frame_p, vp, vp, vp, vp, dp],
             "gs_hip_backward": [vp, frame_p], "gs_hip_adam": [vp, frame_p],
             "gs_hip_densify": [vp, frame_p, u32p, u32p],
             "gs_hip_reset_opacity": [vp, frame_p, ctypes.c_float], "gs_hip_train_step": [vp, frame_p, u64p, dp],
@@ -529,6 +530,22 @@ def spz_info(data: bytes):
 
 def synchronize() -> None:
     check(library().gs_hip_synchronize())
+
+
+def loss_noalloc(frame: Frame, image_ptr, target_ptr, dldc_ptr, want_losses: bool = True, stream=None):
+    """gs_hip_loss_noalloc: the loss stage on caller-owned device buffers
+    (float4 image, RGBA8 target, float4 dL/dC out), allocating nothing.
+    Returns the three loss sums (or None)."""
+    losses = (ctypes.c_double * 3)()
+    check(library().gs_hip_loss_noalloc(ctypes.byref(frame), _vptr(image_ptr), _vptr(target_ptr), _vptr(dldc_ptr),
+                                        _vptr(stream), losses if want_losses else None))
+    return tuple(losses) if want_losses else None
+
+
+def _vptr(p):
+    if p is None or isinstance(p, int):
+        return p
+    return p.value if isinstance(p, ctypes.c_void_p) else int(p)
 
 
 def device_alloc(nbytes: int) -> int:
