@@ -50,31 +50,41 @@ SH_COEFFS = {0: 0, 1: 3, 2: 8, 3: 15}
 
 def kernel_bytes(name, n_slots, n_visible, pairs, pixels, coeffs, views):
     """Algorithmic HBM bytes of one launch of each kernel family (fp32) over
-    `views` views, as counted in the design document's roofline table;
-    `pairs` and `n_visible` are per view."""
-    sh = 12 * coeffs
-    if name == "project":  # parameters + SH once, per-view record/footprint/tiles/misc/key/order/grad2d
-        return n_slots * 64 + n_visible * sh + views * n_slots * 116
-    if name == "forward":
-        return views * (pairs * 52 + pixels * 20)
-    if name == "backward":
-        return views * (pairs * 52 + pixels * 20 + n_visible * 36)
-    if name == "ssim":
-        return views * pixels * (16 + 4 + 36)
-    if name == "loss_grad":
-        return views * pixels * (36 + 16 + 4 + 12)
-    if name == "adam":  # state r/w once; per view: grad2d, tiles, misc, SH input w + r
-        return n_slots * (3 * 64 * 2 + 3 * sh * 2 + sh + 32) + views * n_slots * (36 + 4 + 4 + 64)
+    `views` views, from the per-unit figures of the step's byte model
+    (B_G per gaussian, B_P = 132 per pair, B_px = 76 per pixel): projection
+    D + 44 per gaussian (parameters in; record and count out); emission 48
+    per visible gaussian (re-read + offset) + 12 per pair (key + value out);
+    one tile-sort pass 24 per pair (read + write); ranges 8 per pair; forward
+    44 per pair (index + 40 B record) + 28 per pixel; SSIM 16 per pixel
+    (render + target in) and the loss gradient 12 (dL/dC out); backward 44
+    per pair + 20 per pixel + 36 per visible gaussian (the 2D gradient sums);
+    adjoint + Adam 6 D + 16 (state, statistics) per slot once per step and
+    36 per slot per view (the views' 2D gradients).  `pairs` and `n_visible` are per view; D = 4 (11 + 3 K) bytes
+    of parameters per gaussian with K = coeffs + 1 SH coefficients per
+    channel."""
+    D = 4 * (11 + 3 * (coeffs + 1))
+    if name == "project":
+        return views * n_slots * (D + 44)
     if name == "emit":
-        return views * (n_visible * (48 + 16 + 8) + pairs * 8)
-    if name in ("tile_sort",):
-        return views * pairs * 8 * 2
-    if name == "depth_sort":
-        return views * n_slots * 8 * 2
-    if name == "scan":
-        return views * n_slots * 12
+        return views * (n_visible * 48 + pairs * 12)
+    if name == "tile_sort":
+        return views * pairs * 24
     if name == "ranges":
         return views * pairs * 8
+    if name == "forward":
+        return views * (pairs * 44 + pixels * 28)
+    if name == "ssim":
+        return views * pixels * 16
+    if name == "loss_grad":
+        return views * pixels * 12
+    if name == "backward":
+        return views * (pairs * 44 + pixels * 20 + n_visible * 36)
+    if name == "adam":  # state read + written once; each view's 2D gradient read
+        return n_slots * (6 * D + 16) + views * n_slots * 36
+    if name == "depth_sort":  # one pass over (key, value): not in the step model, which counts the pairs' sort
+        return views * n_slots * 16
+    if name == "scan":
+        return views * n_slots * 8
     return None
 
 
@@ -86,30 +96,52 @@ def step_bytes(n, pairs, pixels, degree):
     return BG * n + 132 * pairs + 76 * pixels
 
 
-def cpu_baseline(gs, orc, params, sh, cam, target, degree, budget_s):
-    """The oracle's CPU restatement of one training step (project, bin,
-    composite, loss + SSIM gradient, tile backward, projection adjoint, Adam),
-    single-threaded, on the same 1M-gaussian 1280x720 view."""
-    w, h = cam.width, cam.height
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "host"
+
+
+def cpu_threads():
+    """The OpenMP threads the oracle runs on: OMP_NUM_THREADS if set, else every host core."""
+    try:
+        return int(os.environ.get("OMP_NUM_THREADS", "")) or os.cpu_count()
+    except ValueError:
+        return os.cpu_count()
+
+
+def cpu_baseline(gs, orc, params, sh, cams, targets, order, degree, steps):
+    """The oracle's CPU restatement of the training step (fp32 projection,
+    binning, compositing and tile backward; fp64 loss, SSIM and projection
+    adjoint; fp32 Adam), multithreaded over the host cores, for `steps` plain
+    steps of the benchmark's own scene and views.  Returns (seconds per step, mean pairs)."""
+    n = params.shape[0]
+    lr = orc.adam_rates(n, 1.6e-4, 5e-3, 5e-3, 5e-3, 5e-3)
+    cams_o = [cams[int(v)] for v in order[:steps]]
+    tgts_o = [targets[int(v)] for v in order[:steps]]
     t0 = time.perf_counter()
-    rec, foot, tiles, keys = orc.project32(params, sh, cam, degree)
-    pt, pg, ranges = orc.bin32(rec, foot, tiles, keys, w, h)
-    img, last = orc.forward32(rec, pg, ranges, w, h)
-    x = img[:, :3].astype(np.float64) + img[:, 3:4].astype(np.float64) * 0.0
-    y = target.reshape(-1, 4)[:, :3] / 255.0
-    _, dldc = orc.loss_image64(x, y, w, h, 0.2, grad=True)
-    d4 = np.zeros((w * h, 4), np.float32)
-    d4[:, :3] = dldc
-    g2 = orc.backward32(rec, pg, ranges, img, last, d4, (0.0, 0.0, 0.0), w, h)
-    gp, gsh = orc.adjoint64(params, sh, cam, degree, g2)
-    lr = np.full(params.shape, 5e-3, np.float32)
-    orc.adam(params.reshape(-1), gp.astype(np.float32).reshape(-1), np.zeros(params.size, np.float32),
-             np.zeros(params.size, np.float32), lr.reshape(-1))
-    if gsh.size:
-        orc.adam(sh.reshape(-1), gsh.astype(np.float32).reshape(-1), np.zeros(sh.size, np.float32),
-                 np.zeros(sh.size, np.float32), np.full(sh.size, 5e-3, np.float32))
+    orc.train(params, sh, degree, cams_o, tgts_o, steps, lr, 5e-3, 0.2)
+    return (time.perf_counter() - t0) / steps
+
+
+def cpu_small_scene(gs, orc):
+    """The small-scene configuration run in full on the CPU restatement
+    (tests/golden/make_c1_trajectory.py): 20k gaussians SH0, 24 orbit views at
+    360x640, 300 steps.  Returns (seconds, PSNR before, PSNR after)."""
+    W, H, N = 360, 640, 20_000
+    cams = gs.orbit_cameras(24, W, H)
+    gt_p, gt_sh = gs.synthetic_scene(N, 0, 1001)
+    targets = orc.render_targets64(gt_p, gt_sh, 0, cams)
+    p0, s0 = gs.synthetic_scene(N, 0, 2002)
+    lr = orc.adam_rates(N, 1.6e-4, 5e-3, 5e-3, 5e-3, 5e-3)
+    t0 = time.perf_counter()
+    p, s = orc.train(p0, s0, 0, cams, targets, 300, lr, 5e-3, 0.2)
     dt = time.perf_counter() - t0
-    return dt, int(len(pt))
+    return dt, orc.mean_psnr(p0, s0, 0, cams, targets), orc.mean_psnr(p, s, 0, cams, targets)
 
 
 def run(gs, ws, cams, target_bufs, order, per_step, views_total, warmup_views, deg, tp, densify_every, state,
@@ -155,6 +187,19 @@ def run(gs, ws, cams, target_bufs, order, per_step, views_total, warmup_views, d
 
 
 TRAFFIC_PROFILE = os.path.join("profiles", "hbm_traffic_8views_1M_1280x720_pmc.md")
+SQ_PROFILE = os.path.join("profiles", "sq_counters_8views_1M_1280x720_pmc.md")
+VALU_PEAK = 1024 * 2.4e9 / 4  # wave64 VALU instructions per second: 1024 SIMDs, 4 cycles each
+
+
+def profiled_valu(kernel):
+    """VALU instructions of one 8-view launch of `kernel` from the SQ counter
+    summary of this workload, or None."""
+    try:
+        text = open(os.path.join(ROOT, SQ_PROFILE)).read()
+    except OSError:
+        return None
+    m = re.search(r"^\| k_%s \| \d+ \| \d+ \| ([0-9.e+]+) \|" % re.escape(kernel), text, re.M)
+    return float(m.group(1)) if m else None
 
 
 def profiled_traffic(kernel):
@@ -324,6 +369,39 @@ def multi_scene(gs, args, device):
             t.free()
 
 
+def heldout_psnr(gs, ws, cams, targets, slots, deg):
+    """Mean PSNR (dB) over held-out views of the workspace's model (float
+    render over black against the RGBA8 target's colour)."""
+    out = []
+    for cam, t in zip(cams, targets):
+        ws.render(gs.make_frame(cam, slots, deg))
+        img = ws.download(gs.BUF_IMAGE, np.float32).reshape(-1, 4)[:, :3].astype(np.float64)
+        y = t.reshape(-1, 4)[:, :3] / 255.0
+        out.append(-10.0 * math.log10(max(float(np.mean((img - y) ** 2)), 1e-12)))
+    return float(np.mean(out))
+
+
+def depth_key_difference(gs, params, sh, cam, target_buf, deg):
+    """One view's step (model untouched) under the 16-bit log-depth pair
+    order and under the float-bits order: the image and 2D-gradient
+    differences the order makes (ties broken differently between splats of
+    equal 16-bit key)."""
+    n = params.shape[0]
+    out = {}
+    with gs.Workspace(n, deg, cam.width, cam.height) as ws:
+        ws.load(params, sh)
+        for name, flags in (("depth16", gs.FLAG_DEPTH16), ("float", 0)):
+            ws.train_views([gs.make_frame(cam, n, deg, flags=flags | gs.FLAG_NO_UPDATE)], [target_buf.ptr],
+                           want_losses=False)
+            out[name] = (ws.download(gs.BUF_IMAGE, np.float32).reshape(-1, 4),
+                         ws.download(gs.BUF_GRAD2D, np.float32, 9 * n).reshape(n, 9))
+    (i16, g16), (i32, g32) = out["depth16"], out["float"]
+    scale = np.abs(g32).max(axis=0) + 1e-30
+    return {"image_max_abs": float(np.abs(i16 - i32).max()), "image_mean_abs": float(np.abs(i16 - i32).mean()),
+            "pixels_differing": int((np.abs(i16 - i32).max(axis=1) > 0).sum()),
+            "grad2d_max_rel": float((np.abs(g16 - g32).max(axis=0) / scale).max())}
+
+
 def kernel_table(events, steps, slots_mean, n_visible, mean_pairs_per_view, per_step, pixels, coeffs):
     per_kernel = {}
     for name, ms in events:
@@ -352,6 +430,9 @@ def main():
     ap.add_argument("--views", type=int, default=64)
     ap.add_argument("--densify-every", type=int, default=100, help="view-iterations between densifications")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=8, help="oracle training steps timed for the CPU baseline")
+    ap.add_argument("--no-cpu-small-scene", action="store_true",
+                    help="skip the CPU baseline's full small-scene run (300 steps at 20k gaussians)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="default launch shapes instead of the autotuner's")
     ap.add_argument("--shape", type=str, default=None, metavar="G,D,T,GRAPH",
@@ -391,6 +472,12 @@ def main():
             img = gt.render(gs.make_frame(cam, N, deg))
             img[..., 3] = 255
             targets.append(img)
+        # Held-out views (not trained on): the models' quality after equal view-iterations.
+        held_cams = gs.jitter_cameras(8, W, H, 4004)
+        held_targets = []
+        for cam in held_cams:
+            img = gt.render(gs.make_frame(cam, N, deg))
+            held_targets.append(img)
     target_bufs = [gs.DeviceBuffer(t) for t in targets]
     del gt_params, gt_sh
 
@@ -401,8 +488,11 @@ def main():
     views_total = args.steps * args.batch
     warmup_views = args.warmup * args.batch
     results = {}
-    for mode, per_step in (("plain", 1), ("batched", args.batch)):
-        if mode == "plain" and args.no_plain:
+    modes = [("plain", 1, key_flags), ("batched", args.batch, key_flags)]
+    if not args.no_plain and args.depth_key == 16:
+        modes.append(("plain_float_key", 1, 0))  # the configuration's own (tile << 32) | float-bits(z) order
+    for mode, per_step, mode_flags in modes:
+        if mode.startswith("plain") and args.no_plain:
             continue
         log(f"{mode}: {views_total} view-iterations, {per_step} view(s) per optimizer step")
         ws = gs.Workspace(2 * N, deg, W, H)
@@ -415,12 +505,12 @@ def main():
             log(f"{mode}: launch shape {tuning} (given)")
         elif not args.no_autotune:
             vs = [int(order[k % len(order)]) for k in range(per_step)]
-            frames = [gs.make_frame(cams[v], N, deg, step=0, tp=tp, flags=key_flags) for v in vs]
+            frames = [gs.make_frame(cams[v], N, deg, step=0, tp=tp, flags=mode_flags) for v in vs]
             best, tune_ms = ws.autotune(frames, [target_bufs[v].ptr for v in vs], repeats=3)
             tuning = best.as_dict()
             log(f"{mode}: autotuned launch shape {tuning} ({tune_ms:.3f} ms per step while tuning)")
         graph = bool(tuning and tuning["graph"])
-        state = {"slots": N, "densified": 0, "pairs": 0, "views": 0, "step": 0, "flags": key_flags}
+        state = {"slots": N, "densified": 0, "pairs": 0, "views": 0, "step": 0, "flags": mode_flags}
         # Per-kernel HIP events need eager launches: with graph replay chosen,
         # the timed run replays the graph and a short eager pass after it
         # times the kernels.
@@ -448,6 +538,8 @@ def main():
             "kernel_timing": "eager pass after the timed run" if graph else "the timed run",
         }
         log(f"{mode}: {views_per_s:.1f} view-iterations/s ({1000 * elapsed / steps:.3f} ms per optimizer step)")
+        results[mode]["heldout_psnr"] = heldout_psnr(gs, ws, held_cams, held_targets, state["slots"], deg)
+        log(f"{mode}: held-out PSNR {results[mode]['heldout_psnr']:.3f} dB after {state['views']} view-iterations")
         ws.close()
 
     main_mode = "batched"
@@ -456,16 +548,27 @@ def main():
     per_view = {k: v["ms_per_step"] / args.batch for k, v in kernels.items()}
     dominant = max(per_view, key=per_view.get) if per_view else None
     roofline = None
+    default_workload = (args.batch == 8 and N == 1_000_000 and (W, H) == (1280, 720) and deg == 3 and
+                        args.depth_key == 16)
     if dominant is not None and "GBps" in kernels[dominant]:
         ach = kernels[dominant]["GBps"]
-        default_workload = (args.batch == 8 and N == 1_000_000 and (W, H) == (1280, 720) and deg == 3 and
-                            args.depth_key == 16)
         traffic = profiled_traffic(dominant) if default_workload else None
         roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK / 1e9,
                     "unit": "GB/s", "frac": round(ach * 1e9 / HBM_PEAK, 5), "traffic": traffic,
                     "traffic_source": TRAFFIC_PROFILE if traffic is not None else None,
                     "algorithmic_bytes": kernels[dominant].get("algorithmic_bytes"),
+                    "bytes_model": "44 B per pair + 20 B per pixel + 36 B per visible gaussian, per view",
                     "ms_avg": round(kernels[dominant]["ms_avg"], 4)}
+    # The rasterizers' VALU issue rate: wave64 VALU instructions per launch (the
+    # committed SQ counter profile of this workload) over the launch time,
+    # against 1024 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction.
+    valu = {}
+    for k in ("backward", "forward"):
+        insts = profiled_valu(k) if default_workload else None
+        if insts is not None and k in kernels:
+            rate = insts / (kernels[k]["ms_avg"] * 1e-3)
+            valu[k] = {"insts_per_launch": insts, "achieved": round(rate / 1e12, 4), "peak": VALU_PEAK / 1e12,
+                       "unit": "T wave-instructions/s", "frac": round(rate / VALU_PEAK, 4)}
     # Whole-step bytes per view-iteration; the batched step reads and writes the
     # optimizer state once per `batch` views.
     D = 4 * (11 + 3 * (deg + 1) ** 2)
@@ -475,14 +578,29 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        log("cpu baseline: the oracle's single-threaded step on one view")
         orc = _load("gs_oracle", "oracle/oracle.py")
-        dt, cpu_pairs = cpu_baseline(gs, orc, params.copy(), sh.copy(), cams[int(order[0])], targets[int(order[0])],
-                                     deg, 30.0)
-        cpu = {"value": round(1.0 / dt, 5), "unit": "iter/s", "cores": 1, "kind": "port",
-               "sample": f"one full training step (1 view, {N} gaussians, SH{deg}, {W}x{H}, {cpu_pairs} pairs) "
-                         f"through the oracle's C restatement in {dt:.1f} s on {platform.processor() or 'host'} "
-                         f"({os.cpu_count()} cores present, 1 used)"}
+        threads = cpu_threads()
+        log(f"cpu baseline: the oracle's training step, {args.cpu_steps} steps of this scene on {threads} threads")
+        dt = cpu_baseline(gs, orc, params.copy(), sh.copy(), cams, targets, order, deg, args.cpu_steps)
+        cpu = {"value": round(1.0 / dt, 5), "unit": "iter/s", "cores": threads, "kind": "port",
+               "sample": f"{args.cpu_steps} plain training steps (1 view each: project, bin, composite, loss + SSIM "
+                         f"gradient, tile backward, projection adjoint, Adam) of this {N}-gaussian SH{deg} {W}x{H} "
+                         f"scene through the oracle's C restatement (OpenMP) on {cpu_model()}, {threads} threads "
+                         f"({os.cpu_count()} present): {dt:.2f} s per step; own CPU path, not the reference's "
+                         "Vulkan-on-lavapipe"}
+        if not args.no_cpu_small_scene:
+            log("cpu baseline: the small-scene configuration in full (20k SH0 360x640, 300 steps)")
+            ds, p0, p1 = cpu_small_scene(gs, orc)
+            cpu["small_scene_full"] = {"seconds": round(ds, 2), "iters_per_s": round(300 / ds, 3), "steps": 300,
+                                       "psnr_start": round(p0, 4), "psnr_end": round(p1, 4),
+                                       "workload": "20k gaussians SH0, 24 orbit views at 360x640, 0.8 L1 + 0.2 "
+                                                   "D-SSIM, Adam 1.6e-4 / 5e-3, no densification"}
+        log(f"cpu baseline: {cpu['value']} iter/s at the benchmark scene")
+    if args.depth_key == 16:
+        log("depth-key difference on one view")
+        key_diff = depth_key_difference(gs, params, sh, cams[int(order[0])], target_bufs[int(order[0])], deg)
+    else:
+        key_diff = None
 
     plain = results.get("plain")
     out = {
@@ -522,6 +640,18 @@ def main():
                          step_bytes(plain["slots_mean"], plain["mean_pairs"], W * H, deg) * plain["views_per_s"] /
                          HBM_PEAK, 5)},
         "roofline": roofline,
+        "valu": valu or None,
+        "plain_float_key": None if "plain_float_key" not in results else {
+            "iters_per_s": round(results["plain_float_key"]["views_per_s"], 3),
+            "ms_per_step": round(results["plain_float_key"]["ms_per_step"], 4),
+            "heldout_psnr": round(results["plain_float_key"]["heldout_psnr"], 4),
+            "order": "(tile << 32) | float bits of z, ties by gaussian index"},
+        "depth_key_difference": key_diff,
+        "heldout_psnr": {"batched": round(r["heldout_psnr"], 4),
+                         "plain": None if plain is None else round(plain["heldout_psnr"], 4),
+                         "views": 8, "view_iterations": args.steps * args.batch + args.warmup * args.batch,
+                         "note": "equal view-iterations: the plain run takes one Adam step per view, the batched "
+                                 f"run one per {args.batch} views"},
         "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in sorted(kernels.items(), key=lambda kv: -kv[1]["ms_per_step"])},
         "cpu_baseline": cpu,

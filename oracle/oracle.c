@@ -1014,8 +1014,10 @@ static int order_cmp(const void* a, const void* b) {
   return i < j ? -1 : (i > j ? 1 : 0);
 }
 
-static scene64_t build_scene64(const double* params, const double* sh, uint32_t n, uint32_t coeffs_in,
-                               const orc_camera* cam, int degree) {
+/* The fp64 projection of every slot, and the visible ones in depth order
+ * (sorted only when `sorted`: the adjoint needs no order). */
+static scene64_t project_scene64(const double* params, const double* sh, uint32_t n, uint32_t coeffs_in,
+                                 const orc_camera* cam, int degree, int sorted) {
   scene64_t s;
   s.refs = (ref_t*)calloc(n > 0 ? n : 1, sizeof(ref_t));
   s.order = (uint32_t*)malloc(sizeof(uint32_t) * (n > 0 ? n : 1));
@@ -1026,9 +1028,16 @@ static scene64_t build_scene64(const double* params, const double* sh, uint32_t 
   }
   for (uint32_t i = 0; i < n; ++i)
     if (s.refs[i].visible) s.order[s.count++] = i;
-  g_sort_refs = s.refs;
-  qsort(s.order, s.count, sizeof(uint32_t), order_cmp);
+  if (sorted) {
+    g_sort_refs = s.refs;
+    qsort(s.order, s.count, sizeof(uint32_t), order_cmp);
+  }
   return s;
+}
+
+static scene64_t build_scene64(const double* params, const double* sh, uint32_t n, uint32_t coeffs_in,
+                               const orc_camera* cam, int degree) {
+  return project_scene64(params, sh, n, coeffs_in, cam, degree, 1);
 }
 
 static void free_scene64(scene64_t* s) {
@@ -1492,7 +1501,7 @@ static void adjoint_scene64(const scene64_t* sc, const double* params, const dou
 
 void orc_adjoint64(const double* params, const double* sh, uint32_t n, uint32_t coeffs_in, const orc_camera* cam,
                    int degree, const double* grad2d, double* gparams, double* gsh) {
-  scene64_t s = build_scene64(params, sh, n, coeffs_in, cam, degree);
+  scene64_t s = project_scene64(params, sh, n, coeffs_in, cam, degree, 0);
   adjoint_scene64(&s, params, sh, n, coeffs_in, cam, degree, grad2d, gparams, gsh, NULL);
   free_scene64(&s);
 }

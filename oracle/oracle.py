@@ -440,11 +440,17 @@ def train(params, sh, deg: int, cams, targets, steps: int, lr, lr_sh_rest: float
         g2 = backward32(rec, pg, ranges, img, last, d4, bg, w, h)
         gp, gsh = adjoint64(p, s, cam, deg, g2)
         b1, b2 = 1.0 / (1.0 - 0.9 ** (step0 + k + 1)), 1.0 / (1.0 - 0.999 ** (step0 + k + 1))
-        p, m, v = adam(p, gp.astype(np.float32), m, v, lr, bias1=b1, bias2=b2)
+        _adam_in_place(p, gp.astype(np.float32), m, v, lr, b1, b2)
         if s.size:
-            s, msh, vsh = adam(s, gsh.astype(np.float32), msh, vsh, lr_rest, bias1=b1, bias2=b2)
+            _adam_in_place(s, gsh.astype(np.float32), msh, vsh, lr_rest, b1, b2)
     state.update(m=m, v=v, msh=msh, vsh=vsh)
     return p, s
+
+
+def _adam_in_place(p, g, m, v, lr, bias1, bias2, beta1=0.9, beta2=0.999, eps=1e-15):
+    """orc_adam on the arrays themselves (contiguous float32)."""
+    library().orc_adam(_p(p, ctypes.c_float), _p(g, ctypes.c_float), _p(m, ctypes.c_float), _p(v, ctypes.c_float),
+                       _p(lr, ctypes.c_float), p.size, beta1, beta2, eps, bias1, bias2)
 
 
 def mean_psnr(params, sh, deg: int, cams, targets, bg=(0.0, 0.0, 0.0)) -> float:
