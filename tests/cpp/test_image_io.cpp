@@ -99,6 +99,20 @@ void test_json() {
     }
   }
   EXPECT(thrown == 8, "malformed JSON must throw (%d of 8 did)", thrown);
+  // Nesting is held on the heap: 400 levels parse, past 512 is refused with an error.
+  const std::string deep = std::string(400, '[') + "7" + std::string(400, ']');
+  const gs::JsonValue* d = nullptr;
+  gs::JsonValue dv = gs::parse_json(deep);
+  d = &dv;
+  for (int k = 0; k < 400 && d != nullptr; ++k) d = d->array.empty() ? nullptr : &d->array[0];
+  EXPECT(d != nullptr && d->type == gs::JsonValue::Type::Number && d->number == 7.0, "400 nested arrays");
+  bool too_deep = false;
+  try {
+    gs::parse_json(std::string(600, '[') + std::string(600, ']'));
+  } catch (const std::runtime_error& e) {
+    too_deep = std::string(e.what()).find("nesting") != std::string::npos;
+  }
+  EXPECT(too_deep, "600 nested arrays must be refused");
 }
 
 int decode_to_file(const char* in, int channels, const char* out) {
