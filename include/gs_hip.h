@@ -11,7 +11,10 @@
  *      slot capacity and an image size (the role of the reference's
  *      VkContext-owned Trainer buffers, engine/src/trainer.cpp:303-386)
  *   3. the stages of one training step (engine/src/trainer.cpp:665-776)
- *   4. the trainer handle: the gs::Trainer C++ class behind a C surface, the
+ *   4. launch shapes (the reference's reduction / alternative choices)
+ *   5. batches of scenes trained together (no counterpart in the reference)
+ *   6. scene files on the device (SPZ load, PLY rows)
+ *   7. the trainer handle: the gs::Trainer C++ class behind a C surface, the
  *      shape a JNI/ctypes binding uses (app/.../NativeTrainer.kt:8-47)
  *
  * Threading: a workspace or trainer is used by one thread at a time; calls
@@ -350,6 +353,90 @@ int gs_hip_workspace_load_spz(gs_hip_workspace* ws, const uint8_t* spz, size_t b
  * half turn about x (RUB -> RDF).  *count = alive slots; rows == NULL asks for the
  * count only (rows needs count * (17 + 3 K) * 4 bytes). */
 int gs_hip_export_ply(gs_hip_workspace* ws, uint32_t slots, int rdf, uint8_t* rows, size_t bytes, uint32_t* count);
+
+/* ---- 7. the trainer handle ------------------------------------------------- */
+/* A whole training session behind one handle: the reference's
+ * NativeTrainer (app/src/main/java/com/ondevice/gsplat/NativeTrainer.kt:8-47,
+ * app/src/main/cpp/jni_bridge.cpp:815-1108) with a status code per call in
+ * place of its JSON strings.  One handle per session; a handle is used by
+ * one thread at a time. */
+typedef struct gs_hip_trainer gs_hip_trainer;
+
+/* NativeTrainer.prepare's arguments (jni_bridge.cpp:816-819). */
+typedef struct gs_hip_trainer_options {
+  int device;               /* HIP device index */
+  uint32_t width;           /* images are resampled so their short side is this (never enlarged) */
+  uint32_t iterations;      /* views to train on; the schedule scales to it (gs::scaled_options) */
+  uint32_t max_gaussians;   /* slot capacity; the initial gaussians take at most half, capped at 150000 */
+  int sh_degree;            /* 0..3 */
+  int masked;               /* train on the masked subject when the capture has masks */
+  int refine_poses;         /* camera pose refinement */
+  int refine_exposure;      /* per-view exposure and white balance gains */
+  uint32_t views_per_step;  /* views per optimizer step, 1..GS_HIP_MAX_VIEWS (0 means 1); the reference trains one */
+  uint32_t seed;
+} gs_hip_trainer_options;
+
+/* What prepare reports about the folder and the initial gaussians (jni_bridge.cpp:886-903). */
+typedef struct gs_hip_trainer_info {
+  uint32_t frames, holdout, skipped; /* frames loaded, held out (every 8th), left out as unreadable */
+  uint32_t width, height;            /* training resolution */
+  int masks, depth;                  /* the capture has masks / depth maps */
+  uint32_t points;                   /* sparse points in the capture */
+  uint64_t init_points;              /* points the initial gaussians were built from */
+  uint32_t init_gaussians;
+  char init_source[16];              /* "depth", "points" or "random" */
+} gs_hip_trainer_info;
+
+/* The training statistics step() reports (jni_bridge.cpp:926-960, write_stats). */
+typedef struct gs_hip_trainer_stats {
+  uint32_t step, iterations;  /* views trained on so far, of the run's */
+  uint32_t gaussians;         /* alive */
+  uint32_t optimizer_steps;
+  int sh_degree;              /* active */
+  uint32_t densify_events, overflow_steps;
+  uint64_t pairs;             /* (tile, gaussian) pairs of the last step */
+  double l1, psnr, ssim, loss; /* of the last step's view(s) */
+  double step_ms;             /* moving average of the wall-clock step time */
+  int finished;
+} gs_hip_trainer_stats;
+
+/* GPU milliseconds of each pass of a profiled step (hipEvents). */
+typedef struct gs_hip_pass_times {
+  double project, sort, rasterize, ssim, backward, optimize;
+  int valid;
+} gs_hip_pass_times;
+
+/* NativeTrainer.evaluate: the held-out views over black (jni_bridge.cpp:1005-1058). */
+typedef struct gs_hip_trainer_eval {
+  double psnr, ssim, l1;
+  double psnr_unaligned;  /* at the given poses, when the views were aligned first (refine_poses) */
+  uint32_t views;
+  int aligned;
+} gs_hip_trainer_eval;
+
+/* NativeTrainer.prepare: loads `dataset_dir`/transforms.json, builds the
+ * initial gaussians and the trainer.  GS_HIP_E_IO for an unreadable folder. */
+int gs_hip_trainer_create(const char* dataset_dir, const gs_hip_trainer_options* options, gs_hip_trainer** out,
+                          gs_hip_trainer_info* info);
+/* NativeTrainer.release. */
+int gs_hip_trainer_destroy(gs_hip_trainer* trainer);
+/* NativeTrainer.step: up to `count` steps (fewer at the end of the run);
+ * profile != 0 times the first one pass by pass into *times.  done, stats
+ * and times may be NULL. */
+int gs_hip_trainer_steps(gs_hip_trainer* trainer, uint32_t count, int profile, uint32_t* done,
+                         gs_hip_trainer_stats* stats, gs_hip_pass_times* times);
+int gs_hip_trainer_get_stats(gs_hip_trainer* trainer, gs_hip_trainer_stats* stats);
+/* NativeTrainer.evaluate. */
+int gs_hip_trainer_evaluate(gs_hip_trainer* trainer, gs_hip_trainer_eval* out);
+/* NativeTrainer.preview: held-out view `index` (modulo their count; frame 0
+ * when none is held out) as RGBA8, the render followed by the photo, both over
+ * the background (0.08, 0.09, 0.11): 2 W H 4 bytes.  rgba == NULL asks only
+ * for width and height. */
+int gs_hip_trainer_preview(gs_hip_trainer* trainer, uint32_t index, uint8_t* rgba, size_t bytes, uint32_t* width,
+                           uint32_t* height);
+/* NativeTrainer.exportPly: the alive gaussians as a 3DGS PLY file (RDF),
+ * rows written on the device.  count and bytes may be NULL. */
+int gs_hip_trainer_export_ply(gs_hip_trainer* trainer, const char* path, uint32_t* count, uint64_t* bytes);
 
 #ifdef __cplusplus
 }

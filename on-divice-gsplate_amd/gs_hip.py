@@ -45,6 +45,8 @@ EXPORTED = [
     "gs_hip_batch_create", "gs_hip_batch_destroy", "gs_hip_batch_train", "gs_hip_batch_get_tuning",
     "gs_hip_batch_set_tuning", "gs_hip_batch_timings", "gs_hip_pose_gradient", "gs_hip_exposure_sums",
     "gs_hip_spz_info", "gs_hip_workspace_load_spz", "gs_hip_export_ply", "gs_hip_loss_noalloc",
+    "gs_hip_trainer_create", "gs_hip_trainer_destroy", "gs_hip_trainer_steps", "gs_hip_trainer_get_stats",
+    "gs_hip_trainer_evaluate", "gs_hip_trainer_preview", "gs_hip_trainer_export_ply",
 ]
 
 
@@ -126,6 +128,14 @@ def library() -> ctypes.CDLL:
             "gs_hip_batch_get_tuning": [vp, ctypes.POINTER(Tuning)],
             "gs_hip_batch_set_tuning": [vp, ctypes.POINTER(Tuning)],
             "gs_hip_batch_timings": [vp, i32, ctypes.POINTER(ctypes.c_char_p), dp, i32, ctypes.POINTER(ctypes.c_int)],
+            "gs_hip_trainer_create": [ctypes.c_char_p, ctypes.POINTER(TrainerOptions), ctypes.POINTER(vp),
+                                      ctypes.POINTER(TrainerInfo)],
+            "gs_hip_trainer_destroy": [vp],
+            "gs_hip_trainer_steps": [vp, u32, i32, u32p, ctypes.POINTER(TrainerStats), ctypes.POINTER(PassTimes)],
+            "gs_hip_trainer_get_stats": [vp, ctypes.POINTER(TrainerStats)],
+            "gs_hip_trainer_evaluate": [vp, ctypes.POINTER(TrainerEval)],
+            "gs_hip_trainer_preview": [vp, u32, ctypes.POINTER(ctypes.c_uint8), sz, u32p, u32p],
+            "gs_hip_trainer_export_ply": [vp, ctypes.c_char_p, u32p, u64p],
         }
         for name, args in sig.items():
             fn = getattr(lib, name)
@@ -540,6 +550,100 @@ def loss_noalloc(frame: Frame, image_ptr, target_ptr, dldc_ptr, want_losses: boo
     check(library().gs_hip_loss_noalloc(ctypes.byref(frame), _vptr(image_ptr), _vptr(target_ptr), _vptr(dldc_ptr),
                                         _vptr(stream), losses if want_losses else None))
     return tuple(losses) if want_losses else None
+
+
+# ---- the trainer handle (NativeTrainer.kt:8-47) ---------------------------------
+class TrainerOptions(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("width", ctypes.c_uint32), ("iterations", ctypes.c_uint32),
+                ("max_gaussians", ctypes.c_uint32), ("sh_degree", ctypes.c_int), ("masked", ctypes.c_int),
+                ("refine_poses", ctypes.c_int), ("refine_exposure", ctypes.c_int),
+                ("views_per_step", ctypes.c_uint32), ("seed", ctypes.c_uint32)]
+
+
+class TrainerInfo(ctypes.Structure):
+    _fields_ = [("frames", ctypes.c_uint32), ("holdout", ctypes.c_uint32), ("skipped", ctypes.c_uint32),
+                ("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("masks", ctypes.c_int),
+                ("depth", ctypes.c_int), ("points", ctypes.c_uint32), ("init_points", ctypes.c_uint64),
+                ("init_gaussians", ctypes.c_uint32), ("init_source", ctypes.c_char * 16)]
+
+
+class TrainerStats(ctypes.Structure):
+    _fields_ = [("step", ctypes.c_uint32), ("iterations", ctypes.c_uint32), ("gaussians", ctypes.c_uint32),
+                ("optimizer_steps", ctypes.c_uint32), ("sh_degree", ctypes.c_int),
+                ("densify_events", ctypes.c_uint32), ("overflow_steps", ctypes.c_uint32), ("pairs", ctypes.c_uint64),
+                ("l1", ctypes.c_double), ("psnr", ctypes.c_double), ("ssim", ctypes.c_double),
+                ("loss", ctypes.c_double), ("step_ms", ctypes.c_double), ("finished", ctypes.c_int)]
+
+
+class PassTimes(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("project", "sort", "rasterize", "ssim", "backward", "optimize")] + \
+        [("valid", ctypes.c_int)]
+
+
+class TrainerEval(ctypes.Structure):
+    _fields_ = [("psnr", ctypes.c_double), ("ssim", ctypes.c_double), ("l1", ctypes.c_double),
+                ("psnr_unaligned", ctypes.c_double), ("views", ctypes.c_uint32), ("aligned", ctypes.c_int)]
+
+
+class NativeTrainer:
+    """A training session on a nerfstudio folder: the reference's
+    NativeTrainer (prepare / step / preview / evaluate / exportPly / release)
+    over the gs_hip_trainer_* calls."""
+
+    def __init__(self, dataset_dir: str, width: int = 512, iterations: int = 3000, max_gaussians: int = 300000,
+                 sh_degree: int = 3, masked: bool = True, refine_poses: bool = False, refine_exposure: bool = False,
+                 views_per_step: int = 1, seed: int = 1, device: int = 0):
+        self.lib = library()
+        o = TrainerOptions(device, width, iterations, max_gaussians, sh_degree, int(masked), int(refine_poses),
+                           int(refine_exposure), views_per_step, seed)
+        self.info = TrainerInfo()
+        h = ctypes.c_void_p()
+        check(self.lib.gs_hip_trainer_create(os.fsencode(dataset_dir), ctypes.byref(o), ctypes.byref(h),
+                                             ctypes.byref(self.info)))
+        self.handle = h
+
+    def step(self, count: int, profile: bool = False):
+        """(steps run, TrainerStats, PassTimes or None)."""
+        done, st, times = ctypes.c_uint32(), TrainerStats(), PassTimes()
+        check(self.lib.gs_hip_trainer_steps(self.handle, count, int(profile), ctypes.byref(done), ctypes.byref(st),
+                                            ctypes.byref(times)))
+        return done.value, st, (times if times.valid else None)
+
+    def stats(self) -> TrainerStats:
+        st = TrainerStats()
+        check(self.lib.gs_hip_trainer_get_stats(self.handle, ctypes.byref(st)))
+        return st
+
+    def evaluate(self) -> TrainerEval:
+        e = TrainerEval()
+        check(self.lib.gs_hip_trainer_evaluate(self.handle, ctypes.byref(e)))
+        return e
+
+    def preview(self, index: int) -> tuple[np.ndarray, np.ndarray]:
+        """(render, photo), each uint8[H][W][4], over the preview background."""
+        w, h = ctypes.c_uint32(), ctypes.c_uint32()
+        check(self.lib.gs_hip_trainer_preview(self.handle, index, None, 0, ctypes.byref(w), ctypes.byref(h)))
+        out = np.zeros((2, h.value, w.value, 4), np.uint8)
+        check(self.lib.gs_hip_trainer_preview(self.handle, index, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                              out.nbytes, ctypes.byref(w), ctypes.byref(h)))
+        return out[0], out[1]
+
+    def export_ply(self, path: str) -> tuple[int, int]:
+        """(gaussians, file bytes)."""
+        n, b = ctypes.c_uint32(), ctypes.c_uint64()
+        check(self.lib.gs_hip_trainer_export_ply(self.handle, os.fsencode(path), ctypes.byref(n), ctypes.byref(b)))
+        return n.value, b.value
+
+    def release(self) -> None:
+        if getattr(self, "handle", None):
+            check(self.lib.gs_hip_trainer_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
 
 
 def _vptr(p):

@@ -3,6 +3,9 @@ include/gs_hip.h declares; the Python binding names the same set."""
 import ctypes
 import os
 import re
+import subprocess
+
+import pytest
 
 from helpers import ROOT, gs
 
@@ -37,3 +40,48 @@ def test_errors_surface_as_exceptions_without_a_gpu():
     rc = lib.gs_hip_workspace_create(None, 16, 0, 64, 48)
     assert rc == 2  # GS_HIP_E_ARGUMENT, before any device call
     assert lib.gs_hip_last_error(buf, 256) == 2 and b"null" in buf.value
+
+
+def test_trainer_structs_match_the_header(tmp_path):
+    """Every field of the trainer handle's structs sits at the offset the C
+    compiler gives it (gcc on include/gs_hip.h)."""
+    structs = [gs.TrainerOptions, gs.TrainerInfo, gs.TrainerStats, gs.PassTimes, gs.TrainerEval]
+    c_names = {gs.TrainerOptions: "gs_hip_trainer_options", gs.TrainerInfo: "gs_hip_trainer_info",
+               gs.TrainerStats: "gs_hip_trainer_stats", gs.PassTimes: "gs_hip_pass_times",
+               gs.TrainerEval: "gs_hip_trainer_eval"}
+    lines = []
+    for s in structs:
+        c = c_names[s]
+        lines.append(f'printf("{c} %zu\\n", sizeof({c}));')
+        for name, _ in s._fields_:
+            lines.append(f'printf("{c}.{name} %zu\\n", offsetof({c}, {name}));')
+    src = tmp_path / "layout.c"
+    src.write_text("#include <stddef.h>\n#include <stdio.h>\n#include \"gs_hip.h\"\nint main(void) {\n" +
+                   "\n".join(lines) + "\nreturn 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                        check=True).stdout.splitlines())
+    for s in structs:
+        c = c_names[s]
+        assert int(got[c]) == ctypes.sizeof(s), c
+        for name, _ in s._fields_:
+            assert int(got[f"{c}.{name}"]) == getattr(s, name).offset, (c, name)
+
+
+def test_trainer_create_reports_a_missing_folder_without_a_gpu(tmp_path):
+    """The folder is read before the device is touched: a missing one is an
+    I/O error with the path in the message, and no handle comes back."""
+    lib = gs.library()
+    o = gs.TrainerOptions(0, 64, 100, 1000, 1, 1, 0, 0, 1, 1)
+    h = ctypes.c_void_p(1)
+    missing = str(tmp_path / "no_such_scene")
+    rc = lib.gs_hip_trainer_create(missing.encode(), ctypes.byref(o), ctypes.byref(h), None)
+    buf = ctypes.create_string_buffer(512)
+    assert rc == 4 and h.value is None  # GS_HIP_E_IO
+    assert lib.gs_hip_last_error(buf, 512) == 4 and b"no_such_scene" in buf.value
+    o.sh_degree = 4
+    assert lib.gs_hip_trainer_create(missing.encode(), ctypes.byref(o), ctypes.byref(h), None) == 2
+    with pytest.raises(gs.GsHipError, match="I/O|no_such_scene|transforms"):
+        gs.NativeTrainer(missing, width=64, iterations=100, max_gaussians=1000, sh_degree=1)
+    assert lib.gs_hip_trainer_destroy(None) == 0
