@@ -402,6 +402,46 @@ def depth_key_difference(gs, params, sh, cam, target_buf, deg):
             "grad2d_max_rel": float((np.abs(g16 - g32).max(axis=0) / scale).max())}
 
 
+def kernel_check(gs, repeats=20):
+    """The single-view forward + backward kernel check: 200k gaussians of the
+    benchmark distributions at SH degree 3, one camera on the radius-3 orbit
+    at 1280x720, the target a render of an independently seeded 200k scene.
+    Full training steps with FLAG_NO_UPDATE (the model stays put, so every
+    repeat sees the same pairs) timed kernel by kernel with HIP events; the
+    rasterizers' Mpix/s and algorithmic GB/s from the step byte model."""
+    n, deg, w, h = 200_000, 3, 1280, 720
+    cam = gs.orbit_cameras(1, w, h)[0]
+    gt_params, gt_sh = gs.synthetic_scene(n, deg, 7001)
+    with gs.Workspace(n, deg, w, h) as gt:
+        gt.load(gt_params, gt_sh)
+        target = gt.render(gs.make_frame(cam, n, deg))
+    target[..., 3] = 255
+    params, sh = gs.synthetic_scene(n, deg, 7002)
+    out = {"workload": "single view, 200k gaussians SH3 fp32, 1280x720, radius-3 orbit camera, 16x16 tiles, "
+                       "float-bits depth key; NO_UPDATE training steps", "repeats": repeats}
+    with gs.Workspace(n, deg, w, h) as ws:
+        ws.load(params, sh)
+        ws.set_target(target)
+        frame = gs.make_frame(cam, n, deg, flags=gs.FLAG_NO_UPDATE)
+        for _ in range(3):
+            pairs, _ = ws.train_step(frame, want_losses=False)
+        gs.synchronize()
+        ws.timings(enable=True)
+        for _ in range(repeats):
+            ws.train_step(frame, want_losses=False)
+        gs.synchronize()
+        events = ws.timings(enable=False)
+        visible = int((ws.download(gs.BUF_TILES, np.uint32, n) > 0).sum())
+    table = kernel_table(events, repeats, n, visible, pairs, 1, w * h, SH_COEFFS[deg])
+    out.update({"pairs": int(pairs), "visible_gaussians": visible})
+    for k in ("forward", "backward"):
+        v = table[k]
+        out[k] = {"ms": round(v["ms_avg"], 4), "mpix_per_s": round(w * h / (v["ms_avg"] * 1e-3) / 1e6, 1),
+                  "algorithmic_bytes": v["algorithmic_bytes"], "GBps": round(v["GBps"], 1),
+                  "frac": round(v["GBps"] * 1e9 / HBM_PEAK, 4)}
+    return out
+
+
 def kernel_table(events, steps, slots_mean, n_visible, mean_pairs_per_view, per_step, pixels, coeffs):
     per_kernel = {}
     for name, ms in events:
@@ -434,6 +474,8 @@ def main():
     ap.add_argument("--no-cpu-small-scene", action="store_true",
                     help="skip the CPU baseline's full small-scene run (300 steps at 20k gaussians)")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-kernel-check", action="store_true",
+                    help="skip the single-view 200k SH3 forward/backward kernel check")
     ap.add_argument("--no-autotune", action="store_true", help="default launch shapes instead of the autotuner's")
     ap.add_argument("--shape", type=str, default=None, metavar="G,D,T,GRAPH",
                     help="fixed launch shape instead of autotuning: raster groups per CU (0 = one per tile), depth "
@@ -602,6 +644,13 @@ def main():
     else:
         key_diff = None
 
+    check = None
+    if not args.no_kernel_check:
+        log("kernel check: single view, 200k gaussians SH3, 1280x720")
+        check = kernel_check(gs)
+        log(f"kernel check: forward {check['forward']['mpix_per_s']} Mpix/s, backward "
+            f"{check['backward']['mpix_per_s']} Mpix/s")
+
     plain = results.get("plain")
     out = {
         "metric": "train iter/s (fwd+bwd+SSIM+Adam) @1M Gaussians 1280x720; batched vs plain, %HBM",
@@ -647,6 +696,7 @@ def main():
             "heldout_psnr": round(results["plain_float_key"]["heldout_psnr"], 4),
             "order": "(tile << 32) | float bits of z, ties by gaussian index"},
         "depth_key_difference": key_diff,
+        "kernel_check": check,
         "heldout_psnr": {"batched": round(r["heldout_psnr"], 4),
                          "plain": None if plain is None else round(plain["heldout_psnr"], 4),
                          "views": 8, "view_iterations": args.steps * args.batch + args.warmup * args.batch,

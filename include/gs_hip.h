@@ -56,6 +56,12 @@ extern "C" {
                                          with gs_hip_pose_gradient; with NO_UPDATE the reference's kFlagPoseOnly */
 #define GS_HIP_FLAG_EXPOSURE 32u      /* photo rgb times exposure[] (train_forward.comp:123) and the view's gain
                                          fit sums (142-151), read with gs_hip_exposure_sums */
+#define GS_HIP_FLAG_DETERMINISTIC 64u /* the backward sums each slot's GRAD2D in an order independent of scheduling:
+                                         a tile's four quadrant waves in a fixed order, then the tiles as 64-bit fixed
+                                         point (2^-44 resolution, |value| < 2^18), so a view's GRAD2D is bitwise the
+                                         same in every run and in any batch (the reference's pair-sum pass,
+                                         train_pair_sum.comp:17-32, is atomic-free); set on any view of a step, it
+                                         applies to all; 72 B per slot per view, allocated on first use */
 
 /* Pinhole camera, scene -> camera, OpenCV axes (x right, y down, z forward).
  * Same fields as gs::Camera (engine/include/gs/camera.h:10-16). */
@@ -286,7 +292,9 @@ typedef struct gs_hip_tuning {
   uint32_t raster_groups_per_cu; /* persistent rasterizers: workgroups per CU pulling (view, tile) work;
                                     0 = one workgroup per work item */
   uint32_t depth_digit_bits;     /* radix digit width of the depth sort, 4..8 */
-  uint32_t tile_digit_bits;      /* radix digit width of the tile sort, 4..8 */
+  uint32_t tile_digit_bits;      /* tile sort: 4..8 = LSD radix passes of that digit width (default 8);
+                                    0 = one stable counting pass over tile grids up to 16383 tiles (larger
+                                    ones fall back to 8-bit digits) */
   uint32_t graph;                /* 1: capture the batched step once and replay it as a HIP graph */
 } gs_hip_tuning;
 int gs_hip_workspace_get_tuning(gs_hip_workspace* ws, gs_hip_tuning* out);

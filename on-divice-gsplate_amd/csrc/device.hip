@@ -184,6 +184,7 @@ void destroy_view(View* v) {
   release(v->depth_order);
   release(v->offsets);
   release(v->grad2d);
+  release(v->grad_fx);
   release(v->flags);
   release(v->scan_tmp);
   release(v->pair_tiles);
@@ -201,6 +202,7 @@ void destroy_view(View* v) {
   release(v->sort.vals_alt);
   release(v->sort.hist);
   release(v->sort.digit_totals);
+  release(v->sort.tile_hist);
   delete v;
 }
 
@@ -211,13 +213,17 @@ int reserve_sort(View& v, uint64_t items) {
   release(s.vals_alt);
   release(s.hist);
   release(s.digit_totals);
+  release(s.tile_hist);
   s = SortScratch{};
   const uint64_t blocks = (items + 4095) / 4096;
+  const uint64_t cols = std::max<uint64_t>(div_up(items, kTileChunk), 1);
   int rc = alloc(&s.keys_alt, items);
   if (rc == 0) rc = alloc(&s.vals_alt, items);
   if (rc == 0) rc = alloc(&s.hist, std::max<uint64_t>(blocks, 1) * 256);
   if (rc == 0) rc = alloc(&s.digit_totals, 256);
+  if (rc == 0 && v.tile_grid + 1 <= kCountingMaxBins) rc = alloc(&s.tile_hist, cols * (v.tile_grid + 1));
   if (rc != 0) return rc;
+  s.tile_cols = s.tile_hist != nullptr ? uint32_t(cols) : 0u;
   s.items = items;
   return 0;
 }
@@ -246,7 +252,8 @@ int reserve_pairs(gs_hip_workspace* ws, uint64_t pairs) {
 
 namespace {
 
-ViewDev make_view_dev(const View& v, const gs_hip_frame& f, uint32_t* totals, float* tile_loss, const void* target) {
+ViewDev make_view_dev(const View& v, const gs_hip_frame& f, uint32_t* totals, float* tile_loss, const void* target,
+                      bool counting_tile_sort, bool deterministic) {
   ViewDev d;
   std::memset(&d, 0, sizeof(d));
   d.f = f;
@@ -258,8 +265,13 @@ ViewDev make_view_dev(const View& v, const gs_hip_frame& f, uint32_t* totals, fl
   d.depth_order = v.depth_order;
   d.offsets = v.offsets;
   d.grad2d = v.grad2d;
+  d.grad_fx = deterministic ? v.grad_fx : nullptr;
   d.pair_tiles = v.pair_tiles;
   d.pair_gaussians = v.pair_gaussians;
+  d.emit_tiles = counting_tile_sort ? v.sort.keys_alt : v.pair_tiles;
+  d.emit_gaussians = counting_tile_sort ? v.sort.vals_alt : v.pair_gaussians;
+  d.tile_hist = v.sort.tile_hist;
+  d.tile_cols = v.sort.tile_cols;
   d.ranges = reinterpret_cast<uint2*>(v.ranges);
   d.image = reinterpret_cast<float4*>(v.image);
   d.last = v.last;
@@ -361,14 +373,31 @@ int upload_entries(gs_hip_workspace* ws, const std::vector<TableEntry>& entries)
     shape.max_pair_capacity = std::max(shape.max_pair_capacity, v.pair_capacity);
     sig = mix64(mix64(mix64(sig, reinterpret_cast<uintptr_t>(&v)), v.pair_capacity), v.sort.items);
     // The flags that change the launch sequence (a captured graph holds one).
-    sig = mix64(sig, e.frame->flags & (GS_HIP_FLAG_DEPTH16 | GS_HIP_FLAG_POSE_GRAD));
+    sig = mix64(sig, e.frame->flags & (GS_HIP_FLAG_DEPTH16 | GS_HIP_FLAG_POSE_GRAD | GS_HIP_FLAG_DETERMINISTIC));
     shape.any_pose = shape.any_pose || (e.frame->flags & GS_HIP_FLAG_POSE_GRAD) != 0u;
+    shape.deterministic = shape.deterministic || (e.frame->flags & GS_HIP_FLAG_DETERMINISTIC) != 0u;
   }
   shape.scenes = uint32_t(scenes.size());
   shape.tile_key_bits = uint32_t(bits_for(shape.max_tiles + 1));
+  shape.counting_tile_sort = ws->tuning.tile_digit_bits == 0u && shape.max_tiles + 1 <= kCountingMaxBins;
+  for (const TableEntry& e : entries) {
+    shape.counting_tile_sort = shape.counting_tile_sort && e.view->sort.tile_hist != nullptr &&
+                               uint64_t(e.view->sort.tile_cols) * kTileChunk >= e.view->pair_capacity;
+  }
   if (shape.scenes > ws->table_scenes) return set_error(GS_HIP_E_ARGUMENT, "too many scenes in one step");
   if (shape.total_tiles > ws->work_entries || shape.total_tiles > ws->tile_loss_entries) {
     return set_error(GS_HIP_E_ARGUMENT, "the step's tiles exceed the launch context's tables");
+  }
+  if (shape.deterministic) {  // the fixed-point sums, made on first use
+    for (const TableEntry& e : entries) {
+      if (e.view->grad_fx == nullptr) {
+        GS_HIP_TRY(hipStreamSynchronize(ws->stream));
+        const uint64_t n = uint64_t(std::max<uint32_t>(e.view->capacity, 1)) * GS_HIP_GRAD2D_FLOATS;
+        int rc = alloc(&e.view->grad_fx, n);
+        if (rc == 0) rc = fill_u32(ws, reinterpret_cast<uint32_t*>(e.view->grad_fx), n * 2, 0u);
+        if (rc != 0) return rc;
+      }
+    }
   }
   if (sig != ws->table_signature || ws->table_signature == 0) {
     const int rc = upload_segments(ws, entries);
@@ -385,7 +414,8 @@ int upload_entries(gs_hip_workspace* ws, const std::vector<TableEntry>& entries)
   for (uint32_t i = 0; i < count; ++i) {
     const TableEntry& e = entries[i];
     host[i] = make_view_dev(*e.view, *e.frame, ws->totals_all + size_t(i) * 16,
-                            ws->tile_loss_all + size_t(ws->tile_base[i]) * 4, e.target);
+                            ws->tile_loss_all + size_t(ws->tile_base[i]) * 4, e.target, shape.counting_tile_sort,
+                            shape.deterministic);
   }
   SceneDev* hs = ws->host_scenes + size_t(s) * ws->table_scenes;
   std::copy(scenes.begin(), scenes.end(), hs);
@@ -796,8 +826,9 @@ int gs_hip_workspace_get_tuning(gs_hip_workspace* ws, gs_hip_tuning* out) {
 
 int gs_hip_workspace_set_tuning(gs_hip_workspace* ws, const gs_hip_tuning* t) {
   if (ws == nullptr || t == nullptr) return gsh::set_error(GS_HIP_E_ARGUMENT, "set_tuning: null");
-  if (t->depth_digit_bits < 4 || t->depth_digit_bits > 8 || t->tile_digit_bits < 4 || t->tile_digit_bits > 8) {
-    return gsh::set_error(GS_HIP_E_ARGUMENT, "set_tuning: digit widths are 4..8 bits");
+  if (t->depth_digit_bits < 4 || t->depth_digit_bits > 8 ||
+      (t->tile_digit_bits != 0 && (t->tile_digit_bits < 4 || t->tile_digit_bits > 8))) {
+    return gsh::set_error(GS_HIP_E_ARGUMENT, "set_tuning: digit widths are 4..8 bits (tiles: 0 for one counting pass)");
   }
   if (t->raster_groups_per_cu > 64) return gsh::set_error(GS_HIP_E_ARGUMENT, "set_tuning: at most 64 groups per CU");
   const bool changed = std::memcmp(&ws->tuning, t, sizeof(*t)) != 0;

@@ -103,9 +103,17 @@ struct ViewDev {
   uint32_t* depth_order;
   uint32_t* offsets;
   float* grad2d;  // [capacity][9]
+  unsigned long long* grad_fx;  // [capacity][9] fixed-point sums of the deterministic backward, or null
   // per pair
   uint32_t* pair_tiles;
   uint32_t* pair_gaussians;
+  // where emission writes the pairs: the pair buffers (the radix tile sort
+  // sorts them in place) or the sort's alternates (the counting tile sort
+  // scatters them from there into the pair buffers)
+  uint32_t* emit_tiles;
+  uint32_t* emit_gaussians;
+  uint32_t* tile_hist;  // counting tile sort: [tile_cols][tile_count + 1] per-chunk tile counts
+  uint32_t tile_cols;
   // per tile / pixel
   uint2* ranges;
   float4* image;
@@ -138,8 +146,10 @@ struct StepShape {
   uint32_t max_tiles = 0;          // the largest tile grid (per-tile grids)
   uint32_t total_tiles = 0;        // every view's tiles (rasterizer work items)
   uint32_t tile_key_bits = 0;      // tile sort key width
+  bool counting_tile_sort = false; // one counting pass (tuning.tile_digit_bits == 0 and the tile grids fit)
   bool any_sh = false;             // some scene stores SH rest coefficients
   bool any_pose = false;           // some view asks for its pose gradient
+  bool deterministic = false;      // some view asks for GS_HIP_FLAG_DETERMINISTIC (then every view's backward is)
 };
 
 // Radix sort scratch of one view, for up to `items` (key, value) pairs.
@@ -149,7 +159,15 @@ struct SortScratch {
   uint32_t* hist = nullptr;          // [256][blocks]
   uint32_t* digit_totals = nullptr;  // [256]
   uint64_t items = 0;
+  uint32_t* tile_hist = nullptr;     // counting tile sort: [tile_cols][tiles + 1]
+  uint32_t tile_cols = 0;            // chunks of kTileChunk pairs the table holds
 };
+
+// Counting tile sort: pairs per chunk (one wave ranks a chunk), and the
+// largest tile grid (+1 for the bin of a slot whose tiles were miscounted)
+// its per-chunk LDS counters hold.
+constexpr uint32_t kTileChunk = 16384;
+constexpr uint32_t kCountingMaxBins = 16384;
 
 // One view's transient buffers (host handles of what ViewDev points at).
 struct View {
@@ -161,6 +179,7 @@ struct View {
   uint32_t* depth_order = nullptr;
   uint32_t* offsets = nullptr;
   float* grad2d = nullptr;
+  unsigned long long* grad_fx = nullptr;  // deterministic backward: [capacity][9], made on first use
   uint32_t* flags = nullptr;     // densify: per-slot actions
   uint32_t* scan_tmp = nullptr;  // per-slot scan output
   uint64_t pair_capacity = 0;

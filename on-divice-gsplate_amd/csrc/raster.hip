@@ -207,15 +207,30 @@ __device__ __forceinline__ void forward_tile(const gsh::ViewDev& a, uint32_t til
   }
 }
 
+// kDet: the deterministic variant keeps each wave's sums apart ([wave][splat][9])
+// and adds them in wave order, then adds the tile's sums to the slot as 64-bit
+// fixed point (integer additions commute, so the result does not depend on
+// which tile or wave came first).
+template <bool kDet>
 struct BackwardShared {
   Staging s;
   uint32_t g[kBatch];
-  float grad[kBatch * kGrad];  // [splat][9]: the nine lanes of one reduction hit nine banks
+  float grad[(kDet ? 4 : 1) * kBatch * kGrad];  // [wave][splat][9]: the nine lanes of one reduction hit nine banks
   uint32_t wave_last[4];
 };
 
+// Fixed point of the deterministic sums: 2^-44 resolution, |value| < 2^18.
+constexpr float kFixedScale = 17592186044416.0f;  // 2^44
+constexpr double kFixedUnit = 1.0 / 17592186044416.0;
+constexpr float kFixedLimit = 262144.0f;  // 2^18
+
+__device__ __forceinline__ long long to_fixed(float v) {
+  return __float2ll_rn(fminf(fmaxf(v, -kFixedLimit), kFixedLimit) * kFixedScale);
+}
+
 // Recompute-transmittance adjoint of one tile of one view.
-__device__ __forceinline__ void backward_tile(const gsh::ViewDev& a, uint32_t tile, BackwardShared& sh,
+template <bool kDet>
+__device__ __forceinline__ void backward_tile(const gsh::ViewDev& a, uint32_t tile, BackwardShared<kDet>& sh,
                                               const Reduce9& red, int slot) {
   const gs_hip_frame& f = a.f;
   Staging& s = sh.s;
@@ -257,7 +272,8 @@ __device__ __forceinline__ void backward_tile(const gsh::ViewDev& a, uint32_t ti
       const uint32_t k = bstart + threadIdx.x;
       const uint32_t g = gmem(a.pair_gaussians)[k];
       s_g[threadIdx.x] = g;
-      for (int c = 0; c < kGrad; ++c) s_grad[threadIdx.x * kGrad + c] = 0.0f;
+      for (int w = 0; w < (kDet ? 4 : 1); ++w)
+        for (int c = 0; c < kGrad; ++c) s_grad[(w * kBatch + threadIdx.x) * kGrad + c] = 0.0f;
       hits = stage(s, a.records, g, threadIdx.x, pc.tx0, pc.ty0);
       const uint32_t pos = k - range.x;
       for (int q = 0; q < 4; ++q)
@@ -304,16 +320,30 @@ __device__ __forceinline__ void backward_tile(const gsh::ViewDev& a, uint32_t ti
         acc_dot = fmaf(a_eff, col_dot - acc_dot, acc_dot);  // this splat joins the colour behind the next
         const float z = red.reduce(g0, g1, g2, g3, g4, g5, g6, g7, g8);
         if (slot >= 0) {
-          __hip_atomic_fetch_add((__attribute__((address_space(3))) float*)s_grad + (uint32_t(j) * kGrad + uint32_t(slot)),
-                                 z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          if (kDet) {
+            s_grad[(pc.wave * kBatch + j) * kGrad + slot] = z;  // this wave's only write of the value
+          } else {
+            __hip_atomic_fetch_add((__attribute__((address_space(3))) float*)s_grad + (uint32_t(j) * kGrad + uint32_t(slot)),
+                                   z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          }
         }
       }
     }
     __syncthreads();
     for (uint32_t e = threadIdx.x; e < n * uint32_t(kGrad); e += kThreads) {
       const uint32_t j = e / uint32_t(kGrad), c = e - j * uint32_t(kGrad);
-      const float v = s_grad[e] * (c == 3u ? -1.0f : (c == 2u || c == 4u ? -0.5f : 1.0f));
-      if (v != 0.0f) global_add(gmem(a.grad2d) + size_t(s_g[j]) * kGrad + c, v);
+      const float scale = c == 3u ? -1.0f : (c == 2u || c == 4u ? -0.5f : 1.0f);
+      if (kDet) {
+        const uint32_t w = kBatch * kGrad;
+        const float v = ((s_grad[e] + s_grad[w + e]) + (s_grad[2 * w + e] + s_grad[3 * w + e])) * scale;
+        if (v != 0.0f) {
+          __hip_atomic_fetch_add(gmem(a.grad_fx) + size_t(s_g[j]) * kGrad + c, static_cast<unsigned long long>(to_fixed(v)),
+                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      } else {
+        const float v = s_grad[e] * scale;
+        if (v != 0.0f) global_add(gmem(a.grad2d) + size_t(s_g[j]) * kGrad + c, v);
+      }
     }
     __syncthreads();
     bend = bstart;
@@ -344,8 +374,9 @@ __global__ void __launch_bounds__(kThreads) k_forward(RasterArgs a) {
   }
 }
 
+template <bool kDet>
 __global__ void __launch_bounds__(kThreads) k_backward(RasterArgs a) {
-  __shared__ BackwardShared sh;
+  __shared__ BackwardShared<kDet> sh;
   __shared__ uint32_t s_item;
   const Reduce9 red;
   const int slot = red.slot();
@@ -353,10 +384,22 @@ __global__ void __launch_bounds__(kThreads) k_backward(RasterArgs a) {
   while (item < a.items) {
     const uint32_t w = a.work[item];
     const gsh::ViewDev& view = a.views[w >> gsh::kTileBits];
-    backward_tile(view, w & gsh::kMaxTilesPerView, sh, red, slot);
+    backward_tile<kDet>(view, w & gsh::kMaxTilesPerView, sh, red, slot);
     if (a.persistent == 0u) break;
     __syncthreads();
     item = next_item(a, &s_item);
+  }
+}
+
+// Deterministic backward: the fixed-point sums become GRAD2D and are cleared
+// for the next step (they are zero between steps: cleared when made, and
+// after every use).
+__global__ void __launch_bounds__(256) k_grad_from_fixed(const gsh::ViewDev* __restrict__ views) {
+  const gsh::ViewDev& v = views[blockIdx.y];
+  const uint64_t n = uint64_t(v.f.slots) * kGrad;
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
+    gmem(v.grad2d)[i] = float(double(static_cast<long long>(gmem(v.grad_fx)[i])) * kFixedUnit);
+    gmem(v.grad_fx)[i] = 0ull;
   }
 }
 
@@ -400,8 +443,16 @@ int launch_backward(gs_hip_workspace* ws, uint32_t count) {
     if (rc != 0) return rc;
   }
   timer_begin(ws, "backward");
-  k_backward<<<grid, kThreads, 0, ws->stream>>>(a);
-  GS_CHECK_LAUNCH("k_backward");
+  if (ws->shape.deterministic) {
+    k_backward<true><<<grid, kThreads, 0, ws->stream>>>(a);
+    GS_CHECK_LAUNCH("k_backward");
+    const uint32_t blocks = std::min<uint32_t>(div_up(uint64_t(ws->shape.max_capacity) * kGrad, 256), 4096);
+    k_grad_from_fixed<<<dim3(std::max<uint32_t>(blocks, 1), count), 256, 0, ws->stream>>>(ws->dev_views);
+    GS_CHECK_LAUNCH("k_grad_from_fixed");
+  } else {
+    k_backward<false><<<grid, kThreads, 0, ws->stream>>>(a);
+    GS_CHECK_LAUNCH("k_backward");
+  }
   timer_end(ws);
   return 0;
 }

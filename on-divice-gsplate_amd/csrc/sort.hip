@@ -35,6 +35,8 @@ using gsh::ScanSeg;
 using gsh::SortSeg;
 using gsh::ViewDev;
 using gsh::div_up;
+using gsh::kCountingMaxBins;
+using gsh::kTileChunk;
 
 constexpr int kSortThreads = 256;
 constexpr int kSortRounds = 16;
@@ -417,8 +419,8 @@ __global__ void __launch_bounds__(256) k_emit(const ViewDev* __restrict__ views)
   const uint64_t p0 = count > 0u ? uint64_t(gmem(v.offsets)[k]) : 0ull;
   const uint64_t cap = v.pair_capacity;
   const uint32_t pad_tile = f.tiles_x * f.tiles_y;
-  GS_GLOBAL uint32_t* const out_tiles = gmem(v.pair_tiles);
-  GS_GLOBAL uint32_t* const out_gauss = gmem(v.pair_gaussians);
+  GS_GLOBAL uint32_t* const out_tiles = gmem(v.emit_tiles);
+  GS_GLOBAL uint32_t* const out_gauss = gmem(v.emit_gaussians);
   const bool small = count > 0u && count <= kWaveEmit;
   const uint64_t end = min(p0 + count, cap);
   int row_first = 0, row_last = -1;
@@ -603,6 +605,180 @@ __global__ void __launch_bounds__(1024) k_work_order(const ViewDev* __restrict__
   }
 }
 
+// ---- counting tile sort ------------------------------------------------------
+// One stable counting pass over the pairs in emission order instead of LSD
+// radix passes: the tile grid (<= kCountingMaxBins bins, the last one for
+// slots whose tiles were miscounted) fits one chunk's counters in LDS.
+//   k_tile_hist:    per chunk of kTileChunk pairs, its count of each tile
+//                   ([col][bin] table, LDS atomics: a count ignores order);
+//   k_tile_colscan: per tile, the exclusive prefix over chunks (in place)
+//                   and the tile's total;
+//   k_tile_starts:  per view, the exclusive prefix over tiles: the ranges;
+//   k_tile_scatter: one wave per chunk walks its pairs in order, ranking
+//                   each among equal tiles of its round with ballots and
+//                   placing it at the tile's start + earlier chunks' count +
+//                   the chunk's running count (LDS), so each tile keeps the
+//                   emission (depth) order.
+// Chunks are dealt to the XCDs in contiguous runs, so one XCD writes one
+// contiguous stretch of each tile's range and its L2 merges the lines.
+
+__device__ __forceinline__ uint32_t pair_count(const ViewDev& v) {
+  return uint32_t(min(uint64_t(v.totals[0]), v.pair_capacity));
+}
+
+__global__ void __launch_bounds__(256) k_tile_hist(const ViewDev* __restrict__ views) {
+  extern __shared__ uint32_t s_bins[];
+  const ViewDev& v = views[blockIdx.y];
+  const uint32_t n = pair_count(v), bins = v.tile_count + 1u;
+  const uint32_t col = blockIdx.x;
+  if (uint64_t(col) * kTileChunk >= n) return;
+  for (uint32_t t = threadIdx.x; t < bins; t += blockDim.x) s_bins[t] = 0u;
+  __syncthreads();
+  const GS_GLOBAL uint32_t* keys = gmem(v.emit_tiles);
+  const uint32_t first = col * kTileChunk, last = uint32_t(min(uint64_t(first) + kTileChunk, uint64_t(n)));
+  constexpr int kUnroll = 8;
+  for (uint32_t base = first; base < last; base += 256u * kUnroll) {
+    uint32_t key[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const uint32_t idx = base + uint32_t(u) * 256u + threadIdx.x;
+      key[u] = idx < last ? keys[idx] : 0xffffffffu;
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u)
+      if (key[u] < bins) atomicAdd(&s_bins[key[u]], 1u);
+  }
+  __syncthreads();
+  GS_GLOBAL uint32_t* row = gmem(v.tile_hist) + size_t(col) * bins;
+  for (uint32_t t = threadIdx.x; t < bins; t += blockDim.x) row[t] = s_bins[t];
+}
+
+// One thread per (view, bin): the chunks' counts of the bin become their
+// exclusive prefix; the bin's total goes to ranges[bin].y until the starts
+// are known.  The counts are requested in groups of 16 before they are summed.
+__global__ void __launch_bounds__(256) k_tile_colscan(const ViewDev* __restrict__ views) {
+  const ViewDev& v = views[blockIdx.y];
+  const uint32_t bins = v.tile_count + 1u;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= bins) return;
+  const uint32_t cols = uint32_t((uint64_t(pair_count(v)) + kTileChunk - 1u) / kTileChunk);
+  GS_GLOBAL uint32_t* table = gmem(v.tile_hist);
+  uint32_t run = 0u;
+  constexpr uint32_t kGroup = 16;
+  for (uint32_t c0 = 0; c0 < cols; c0 += kGroup) {
+    uint32_t x[kGroup];
+#pragma unroll
+    for (uint32_t k = 0; k < kGroup; ++k) x[k] = c0 + k < cols ? table[size_t(c0 + k) * bins + t] : 0u;
+#pragma unroll
+    for (uint32_t k = 0; k < kGroup; ++k) {
+      if (c0 + k < cols) table[size_t(c0 + k) * bins + t] = run;
+      run += x[k];
+    }
+  }
+  gmem(reinterpret_cast<uint32_t*>(v.ranges))[size_t(t) * 2u + 1u] = run;
+}
+
+// One workgroup per view: the exclusive prefix of the bins' totals; each
+// bin's range (first, end), (0, 0) for an empty one.
+__global__ void __launch_bounds__(1024) k_tile_starts(const ViewDev* __restrict__ views) {
+  __shared__ uint32_t s_wave[16];
+  const ViewDev& v = views[blockIdx.y];
+  const uint32_t bins = v.tile_count + 1u;
+  const uint32_t per = (bins + 1023u) / 1024u;  // <= kCountingMaxBins / 1024
+  GS_GLOBAL uint32_t* rg = gmem(reinterpret_cast<uint32_t*>(v.ranges));
+  const uint32_t t0 = threadIdx.x * per;
+  uint32_t tot[kCountingMaxBins / 1024];
+  uint32_t sum = 0u;
+#pragma unroll
+  for (uint32_t k = 0; k < kCountingMaxBins / 1024; ++k) {
+    tot[k] = k < per && t0 + k < bins ? rg[size_t(t0 + k) * 2u + 1u] : 0u;
+    sum += tot[k];
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t incl = sum;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  if (lane == 63) s_wave[wave] = incl;
+  __syncthreads();
+  uint32_t start = incl - sum;
+  for (int w = 0; w < wave; ++w) start += s_wave[w];
+#pragma unroll
+  for (uint32_t k = 0; k < kCountingMaxBins / 1024; ++k) {
+    if (k < per && t0 + k < bins) {
+      const uint2 r = tot[k] > 0u ? make_uint2(start, start + tot[k]) : make_uint2(0u, 0u);
+      rg[size_t(t0 + k) * 2u] = r.x;
+      rg[size_t(t0 + k) * 2u + 1u] = r.y;
+      start += tot[k];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(64) k_tile_scatter(const ViewDev* __restrict__ views, int key_bits) {
+  extern __shared__ uint32_t s_run[];
+  // Contiguous runs of (view, chunk) per XCD: workgroups go to the XCDs in
+  // turn, and the grid's size is a multiple of 8.
+  const uint32_t lin = blockIdx.y * gridDim.x + blockIdx.x;
+  const uint32_t idx = (lin % 8u) * (gridDim.x * gridDim.y / 8u) + lin / 8u;
+  const uint32_t view = idx / gridDim.x, col = idx % gridDim.x;
+  const ViewDev& v = views[view];
+  const uint32_t n = pair_count(v), bins = v.tile_count + 1u;
+  if (uint64_t(col) * kTileChunk >= n) return;
+  const uint32_t lane = threadIdx.x;
+  const GS_GLOBAL uint32_t* prefix = gmem(v.tile_hist) + size_t(col) * bins;
+  const GS_GLOBAL uint32_t* rg = gmem(reinterpret_cast<const uint32_t*>(v.ranges));
+  for (uint32_t t = lane; t < bins; t += 64u) s_run[t] = rg[size_t(t) * 2u] + prefix[t];
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  const GS_GLOBAL uint32_t* in_t = gmem(v.emit_tiles);
+  const GS_GLOBAL uint32_t* in_g = gmem(v.emit_gaussians);
+  GS_GLOBAL uint32_t* out_t = gmem(v.pair_tiles);
+  GS_GLOBAL uint32_t* out_g = gmem(v.pair_gaussians);
+  const uint64_t below_me = lanemask_lt();
+  const uint32_t first = col * kTileChunk, last = uint32_t(min(uint64_t(first) + kTileChunk, uint64_t(n)));
+  constexpr int kUnroll = 8;
+  for (uint32_t base = first; base < last; base += 64u * kUnroll) {
+    uint32_t key[kUnroll], val[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const uint32_t i = base + uint32_t(u) * 64u + lane;
+      key[u] = i < last ? in_t[i] : 0u;
+      val[u] = i < last ? in_g[i] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const bool valid = base + uint32_t(u) * 64u + lane < last && key[u] < bins;
+      const uint64_t active = __ballot(valid);
+      if (active == 0ull) continue;
+      const uint64_t peers = match_digit(key[u], key_bits, active);
+      const uint32_t before = valid ? s_run[key[u]] : 0u;  // every peer reads, then the first one writes
+      if (valid && (peers & below_me) == 0ull) s_run[key[u]] = before + uint32_t(__popcll(peers));
+      if (valid) {
+        const uint32_t pos = before + uint32_t(__popcll(peers & below_me));
+        out_t[pos] = key[u];
+        out_g[pos] = val[u];
+      }
+    }
+  }
+}
+
+int counting_tile_sort(gs_hip_workspace* ws, uint32_t count) {
+  const uint32_t bins = ws->shape.max_tiles + 1u;
+  const uint32_t cols = std::max<uint32_t>(div_up(ws->shape.max_pair_capacity, kTileChunk), 1u);
+  const size_t lds = size_t(bins) * 4u;
+  k_tile_hist<<<dim3(cols, count), 256, lds, ws->stream>>>(ws->dev_views);
+  GS_CHECK_LAUNCH("k_tile_hist");
+  k_tile_colscan<<<dim3(div_up(bins, 256), count), 256, 0, ws->stream>>>(ws->dev_views);
+  GS_CHECK_LAUNCH("k_tile_colscan");
+  k_tile_starts<<<dim3(1, count), 1024, 0, ws->stream>>>(ws->dev_views);
+  GS_CHECK_LAUNCH("k_tile_starts");
+  const uint32_t grid_x = (count % 8u == 0u) ? cols : (cols + 7u) / 8u * 8u;  // grid_x * count divisible by 8
+  k_tile_scatter<<<dim3(grid_x, count), 64, lds, ws->stream>>>(ws->dev_views, int(ws->shape.tile_key_bits));
+  GS_CHECK_LAUNCH("k_tile_scatter");
+  return 0;
+}
+
 // Batched LSD sort of `count` segments by key bits [0, bits), `digit` bits per pass.
 int radix_sort(gs_hip_workspace* ws, const SortSeg* segs, uint32_t count, uint64_t items_max, int bits, int digit) {
   if (bits <= 0 || count == 0 || items_max == 0) return 0;
@@ -674,12 +850,20 @@ int launch_clear_ranges(gs_hip_workspace* ws, uint32_t count) {
 int launch_tile_sort_and_ranges(gs_hip_workspace* ws, uint32_t count) {
   const uint64_t cap = ws->shape.max_pair_capacity;
   timer_begin(ws, "tile_sort");
-  int rc = radix_sort(ws, ws->dev_tile_sort, count, cap, int(ws->shape.tile_key_bits), int(ws->tuning.tile_digit_bits));
+  int rc = 0;
+  if (ws->shape.counting_tile_sort) {
+    rc = counting_tile_sort(ws, count);  // also writes the ranges
+  } else {
+    const int digit = ws->tuning.tile_digit_bits != 0u ? int(ws->tuning.tile_digit_bits) : 8;
+    rc = radix_sort(ws, ws->dev_tile_sort, count, cap, int(ws->shape.tile_key_bits), digit);
+  }
   timer_end(ws);
   if (rc != 0) return rc;
   timer_begin(ws, "ranges");
-  k_ranges<<<dim3(std::min<uint32_t>(div_up(cap, 256), 2048), count), 256, 0, ws->stream>>>(ws->dev_views);
-  GS_CHECK_LAUNCH("k_ranges");
+  if (!ws->shape.counting_tile_sort) {
+    k_ranges<<<dim3(std::min<uint32_t>(div_up(cap, 256), 2048), count), 256, 0, ws->stream>>>(ws->dev_views);
+    GS_CHECK_LAUNCH("k_ranges");
+  }
   k_work_order<<<1, 1024, 0, ws->stream>>>(ws->dev_views, count, ws->work, ws->work_counters);
   GS_CHECK_LAUNCH("k_work_order");
   timer_end(ws);

@@ -65,7 +65,7 @@ int gs_hip_autotune(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t c
     c.depth_digit_bits = bits;
     consider(c);
   }
-  for (uint32_t bits : {8u, 6u, 4u}) {
+  for (uint32_t bits : {0u, 8u, 6u, 4u}) {
     gs_hip_tuning c = cur;
     c.tile_digit_bits = bits;
     consider(c);

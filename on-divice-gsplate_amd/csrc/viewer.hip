@@ -191,8 +191,8 @@ __global__ void __launch_bounds__(256) k_view_emit(const gsh::ViewDev* __restric
   const uint64_t end = std::min<uint64_t>(p + count, v.pair_capacity);
   for (uint32_t ty = lo >> 16; ty <= (hi >> 16); ++ty) {
     for (uint32_t tx = lo & 0xffffu; tx <= (hi & 0xffffu) && p < end; ++tx, ++p) {
-      v.pair_tiles[p] = ty * v.f.tiles_x + tx;
-      v.pair_gaussians[p] = i;
+      v.emit_tiles[p] = ty * v.f.tiles_x + tx;
+      v.emit_gaussians[p] = i;
     }
   }
 }

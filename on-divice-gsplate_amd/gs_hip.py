@@ -25,6 +25,7 @@ FLAG_DEPTH16 = 4
 FLAG_NO_UPDATE = 8
 FLAG_POSE_GRAD = 16
 FLAG_EXPOSURE = 32
+FLAG_DETERMINISTIC = 64
 VIEW_ANTIALIASED = 1
 
 (BUF_PARAMS, BUF_SH, BUF_M_PARAMS, BUF_V_PARAMS, BUF_M_SH, BUF_V_SH, BUF_STATS, BUF_RECORDS, BUF_FOOTPRINT,
@@ -128,6 +129,12 @@ def library() -> ctypes.CDLL:
             "gs_hip_batch_get_tuning": [vp, ctypes.POINTER(Tuning)],
             "gs_hip_batch_set_tuning": [vp, ctypes.POINTER(Tuning)],
             "gs_hip_batch_timings": [vp, i32, ctypes.POINTER(ctypes.c_char_p), dp, i32, ctypes.POINTER(ctypes.c_int)],
+            "gs_hip_pose_gradient": [vp, u32, dp], "gs_hip_exposure_sums": [vp, u32, dp],
+            "gs_hip_spz_info": [ctypes.POINTER(ctypes.c_uint8), sz, u32p, ctypes.POINTER(ctypes.c_int),
+                                ctypes.POINTER(ctypes.c_int)],
+            "gs_hip_workspace_load_spz": [vp, ctypes.POINTER(ctypes.c_uint8), sz, u32p, ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_int)],
+            "gs_hip_export_ply": [vp, u32, i32, ctypes.POINTER(ctypes.c_uint8), sz, u32p],
             "gs_hip_trainer_create": [ctypes.c_char_p, ctypes.POINTER(TrainerOptions), ctypes.POINTER(vp),
                                       ctypes.POINTER(TrainerInfo)],
             "gs_hip_trainer_destroy": [vp],
@@ -137,6 +144,9 @@ def library() -> ctypes.CDLL:
             "gs_hip_trainer_preview": [vp, u32, ctypes.POINTER(ctypes.c_uint8), sz, u32p, u32p],
             "gs_hip_trainer_export_ply": [vp, ctypes.c_char_p, u32p, u64p],
         }
+        missing = sorted(set(EXPORTED) - set(sig))
+        if missing:  # an unsigned call would pass Python ints as C int, truncating sizes and pointers
+            raise RuntimeError(f"gs_hip.py declares no argument types for {missing}")
         for name, args in sig.items():
             fn = getattr(lib, name)
             fn.argtypes = args

@@ -85,3 +85,11 @@ def test_trainer_create_reports_a_missing_folder_without_a_gpu(tmp_path):
     with pytest.raises(gs.GsHipError, match="I/O|no_such_scene|transforms"):
         gs.NativeTrainer(missing, width=64, iterations=100, max_gaussians=1000, sh_degree=1)
     assert lib.gs_hip_trainer_destroy(None) == 0
+
+
+def test_every_entry_point_has_argument_types():
+    """Without argtypes ctypes passes a Python int as a C int: a size_t or a
+    pointer above 2^31 would arrive truncated."""
+    lib = gs.library()
+    for name in gs.EXPORTED:
+        assert getattr(lib, name).argtypes is not None, name

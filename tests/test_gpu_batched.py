@@ -102,6 +102,7 @@ def compare(a, b):
 SHAPE_CASES = [
     ("persistent-1", dict(BASE, raster_groups_per_cu=1)),
     ("persistent-4", dict(BASE, raster_groups_per_cu=4)),
+    ("counting-tile-sort", dict(BASE, tile_digit_bits=0)),
     ("digits-6-6", dict(BASE, depth_digit_bits=6, tile_digit_bits=6)),
     ("digits-5-4", dict(BASE, depth_digit_bits=5, tile_digit_bits=4)),
     ("graph", dict(BASE, graph=1)),
@@ -119,22 +120,23 @@ def test_launch_shapes_compute_the_same_step(name, tuning):
 
 
 def test_depth16_order_with_small_digits():
-    case("depth16 keys (2 passes of 8 bits, or 4 of 4) give the same pairs")
+    case("depth16 keys (2 passes of 8 bits, or 4 of 4; radix or counting tile sort) give the same pairs")
     n, deg, w, h = 1200, 1, 128, 96
     params, sh = make_scene(n, deg, 72)
     cam = make_camera(w, h, 1.0)
     out = []
-    for bits in (8, 4):
+    for bits, tile_bits in ((8, 8), (4, 4), (8, 0)):
         with gs.Workspace(n, deg, w, h) as ws:
-            ws.set_tuning(depth_digit_bits=bits, tile_digit_bits=bits)
+            ws.set_tuning(depth_digit_bits=bits, tile_digit_bits=tile_bits)
             ws.load(params, sh)
             frame = gs.make_frame(cam, n, deg, flags=gs.FLAG_DEPTH16)
             ws.project(frame)
             pairs = ws.bin(frame)
             out.append((ws.download(gs.BUF_PAIR_GAUSSIANS, np.uint32, pairs),
                         ws.download(gs.BUF_DEPTH_ORDER, np.uint32, n)))
-    np.testing.assert_array_equal(out[0][0], out[1][0])
-    np.testing.assert_array_equal(out[0][1], out[1][1])
+    for other in out[1:]:
+        np.testing.assert_array_equal(out[0][0], other[0])
+        np.testing.assert_array_equal(out[0][1], other[1])
 
 
 def test_pair_overflow_retry_matches_a_step_with_room():
@@ -205,7 +207,7 @@ def test_autotune_keeps_a_valid_shape_and_the_model():
     print(f"[info] best {best.as_dict()} at {ms:.3f} ms", flush=True)
     assert ms > 0
     assert best.raster_groups_per_cu in (0, 1, 2, 4, 8)
-    assert 4 <= best.depth_digit_bits <= 8 and 4 <= best.tile_digit_bits <= 8
+    assert 4 <= best.depth_digit_bits <= 8 and (best.tile_digit_bits == 0 or 4 <= best.tile_digit_bits <= 8)
     np.testing.assert_array_equal(before.view(np.uint32), after.view(np.uint32))
 
 
