@@ -281,9 +281,10 @@ def multi_scene(gs, args, device):
     batch = gs.Batch(scenes)
     # Launch shape: the candidates timed on two untimed (NO_UPDATE) batch steps each.
     best, best_ms = None, None
-    for shape in ((0, 8, 8, 0), (0, 8, 6, 1), (0, 6, 6, 1), (2, 8, 6, 1)):
+    for shape in ((0, 8, 8, 0, 0), (0, 8, 6, 1, 0), (0, 6, 6, 1, 0), (2, 8, 6, 1, 0), (0, 6, 6, 1, 1),
+                  (0, 6, 6, 1, 2)):
         batch.set_tuning(raster_groups_per_cu=shape[0], depth_digit_bits=shape[1], tile_digit_bits=shape[2],
-                         graph=shape[3])
+                         graph=shape[3], bin_shift=shape[4])
         fps, tps = frames_of(0, flags=gs.FLAG_NO_UPDATE)
         batch.train(fps, tps, want_losses=False)
         gs.synchronize()
@@ -294,7 +295,8 @@ def multi_scene(gs, args, device):
         ms = 1000 * (time.perf_counter() - t0) / 2
         if best_ms is None or ms < best_ms:
             best, best_ms = shape, ms
-    batch.set_tuning(raster_groups_per_cu=best[0], depth_digit_bits=best[1], tile_digit_bits=best[2], graph=best[3])
+    batch.set_tuning(raster_groups_per_cu=best[0], depth_digit_bits=best[1], tile_digit_bits=best[2], graph=best[3],
+                     bin_shift=best[4])
     log(f"multi-scene: launch shape {best} ({best_ms:.2f} ms per step while choosing)")
     steps, warmup = args.steps, args.warmup
     pixels = sum(2 * w * h for (w, h) in res)
@@ -325,7 +327,8 @@ def multi_scene(gs, args, device):
     batch.close()
     # The same scenes, one scene's 2-view step after another (each scene's own launches).
     for ws in scenes:
-        ws.set_tuning(raster_groups_per_cu=best[0], depth_digit_bits=best[1], tile_digit_bits=best[2], graph=best[3])
+        ws.set_tuning(raster_groups_per_cu=best[0], depth_digit_bits=best[1], tile_digit_bits=best[2], graph=best[3],
+                      bin_shift=best[4])
     for s in range(-warmup, steps):
         if s == 0:
             gs.synchronize()
@@ -351,7 +354,7 @@ def multi_scene(gs, args, device):
                    "scenes": 16, "gaussians": counts, "gaussians_total": sum(counts),
                    "resolutions": [f"{w}x{h}" for (w, h) in res], "views_per_scene": 2, "sh_degree": deg,
                    "tuning": {"raster_groups_per_cu": best[0], "depth_digit_bits": best[1],
-                              "tile_digit_bits": best[2], "graph": best[3]}, "device": device},
+                              "tile_digit_bits": best[2], "graph": best[3], "bin_shift": best[4]}, "device": device},
         "sequential_scenes": {"iters_per_s": round(views / seq_s, 3), "ms_per_step": round(1000 * seq_s, 4)},
         "speedup_batch_vs_sequential": round(seq_s / batch_s, 3),
         "mpix_per_s": round(pixels / batch_s / 1e6, 2),
@@ -477,9 +480,10 @@ def main():
     ap.add_argument("--no-kernel-check", action="store_true",
                     help="skip the single-view 200k SH3 forward/backward kernel check")
     ap.add_argument("--no-autotune", action="store_true", help="default launch shapes instead of the autotuner's")
-    ap.add_argument("--shape", type=str, default=None, metavar="G,D,T,GRAPH",
+    ap.add_argument("--shape", type=str, default=None, metavar="G,D,T,GRAPH[,BIN]",
                     help="fixed launch shape instead of autotuning: raster groups per CU (0 = one per tile), depth "
-                         "and tile radix digit bits, graph replay 0/1")
+                         "and tile radix digit bits (tile 0: one counting pass), graph replay 0/1, pair bin shift "
+                         "0..2 (bins of 16 << shift pixels; default 0)")
     ap.add_argument("--depth-key", type=int, default=16, choices=(16, 32),
                     help="pair order within a tile: the reference's 16-bit log-depth key (train_project.comp:110) "
                          "or the float bits of the depth")
@@ -542,8 +546,9 @@ def main():
         ws.load(params, sh)
         tuning, tune_ms = None, None
         if args.shape:
-            g, d, t, gr = (int(x) for x in args.shape.split(","))
-            tuning = ws.set_tuning(raster_groups_per_cu=g, depth_digit_bits=d, tile_digit_bits=t, graph=gr).as_dict()
+            g, d, t, gr, *rest = (int(x) for x in args.shape.split(","))
+            tuning = ws.set_tuning(raster_groups_per_cu=g, depth_digit_bits=d, tile_digit_bits=t, graph=gr,
+                                   bin_shift=rest[0] if rest else 0).as_dict()
             log(f"{mode}: launch shape {tuning} (given)")
         elif not args.no_autotune:
             vs = [int(order[k % len(order)]) for k in range(per_step)]

@@ -125,12 +125,12 @@ typedef struct gs_hip_workspace gs_hip_workspace;
  *  RECORDS                      float[capacity][12] (mean xy, conic abc, opacity, rgb, bbox lo/hi, radius|clamp)
  *  FOOTPRINT                    float[capacity][8]  (dilated cov2D abc, threshold | mean xy, tiles as u32 bits, 0):
  *                               everything the pair emitter reads of a slot, in one 32 B line
- *  TILES, DEPTH_KEYS, DEPTH_ORDER, OFFSETS  uint32[capacity]  (OFFSETS in depth order; with
+ *  TILES, DEPTH_KEYS, DEPTH_ORDER, OFFSETS  uint32[capacity]  (TILES: bins touched; OFFSETS in depth order; with
  *                               GS_HIP_FLAG_DEPTH16 a key is the 16-bit depth | tiles << 16, sorted by
  *                               its lower half, 0xffff for a slot that touches no tile)
  *  GRAD2D                       float[capacity][9]  (the nine sums of a slot together)
  *  PAIR_TILES, PAIR_GAUSSIANS   uint32[pair_capacity] (sorted)
- *  RANGES                       uint32[tiles + 1][2]
+ *  RANGES                       uint32[bins + 1][2] (bins = the tiles with bin_shift 0)
  *  IMAGE                        float[H*W][4]  (rendered rgb, final transmittance)
  *  LAST                         uint32[H*W]    (1-based position of the last contributor)
  *  TARGET                       uint32[H*W]    (RGBA8)
@@ -296,6 +296,11 @@ typedef struct gs_hip_tuning {
                                     0 = one stable counting pass over tile grids up to 16383 tiles (larger
                                     ones fall back to 8-bit digits) */
   uint32_t graph;                /* 1: capture the batched step once and replay it as a HIP graph */
+  uint32_t bin_shift;            /* pairs are binned by squares of (16 << bin_shift) pixels, 0..2: 0 = the
+                                    16x16 raster tiles themselves (the reference's pairs and ranges); 1, 2 =
+                                    32x32 / 64x64 bins, each raster tile walking its bin's list (fewer pairs to
+                                    emit and sort; the same splats drawn at every pixel in the same order, so
+                                    the same image and gradients) */
 } gs_hip_tuning;
 int gs_hip_workspace_get_tuning(gs_hip_workspace* ws, gs_hip_tuning* out);
 int gs_hip_workspace_set_tuning(gs_hip_workspace* ws, const gs_hip_tuning* tuning);

@@ -252,8 +252,13 @@ int reserve_pairs(gs_hip_workspace* ws, uint64_t pairs) {
 
 namespace {
 
+uint32_t bins_along(uint32_t pixels, uint32_t shift) {
+  const uint32_t side = uint32_t(GS_HIP_TILE) << shift;
+  return (pixels + side - 1) / side;
+}
+
 ViewDev make_view_dev(const View& v, const gs_hip_frame& f, uint32_t* totals, float* tile_loss, const void* target,
-                      bool counting_tile_sort, bool deterministic) {
+                      bool counting_tile_sort, bool deterministic, uint32_t bin_shift) {
   ViewDev d;
   std::memset(&d, 0, sizeof(d));
   d.f = f;
@@ -285,6 +290,9 @@ ViewDev make_view_dev(const View& v, const gs_hip_frame& f, uint32_t* totals, fl
   d.pair_capacity = v.pair_capacity;
   d.capacity = v.capacity;
   d.tile_count = v.tile_grid;
+  d.bin_shift = bin_shift;
+  d.bins_x = bins_along(f.cam.width, bin_shift);
+  d.bin_count = d.bins_x * bins_along(f.cam.height, bin_shift);
   return d;
 }
 
@@ -334,8 +342,9 @@ int bits_for(uint32_t n) {
 
 }  // namespace
 
-int upload_entries(gs_hip_workspace* ws, const std::vector<TableEntry>& entries) {
+int upload_entries(gs_hip_workspace* ws, const std::vector<TableEntry>& entries, int bin_shift) {
   const uint32_t count = uint32_t(entries.size());
+  const uint32_t shift = bin_shift >= 0 ? uint32_t(bin_shift) : ws->tuning.bin_shift;
   if (count == 0 || count > ws->table_views) return set_error(GS_HIP_E_ARGUMENT, "view count out of range");
   // Shape, scenes (runs of entries of one scene) and each view's first tile in the per-tile arrays.
   StepShape shape;
@@ -370,16 +379,19 @@ int upload_entries(gs_hip_workspace* ws, const std::vector<TableEntry>& entries)
     ws->tile_base[i] = shape.total_tiles;
     shape.total_tiles += v.tile_grid;
     shape.max_tiles = std::max(shape.max_tiles, v.tile_grid);
+    shape.max_bins = std::max(shape.max_bins, bins_along(e.frame->cam.width, shift) *
+                                                  bins_along(e.frame->cam.height, shift));
     shape.max_pair_capacity = std::max(shape.max_pair_capacity, v.pair_capacity);
     sig = mix64(mix64(mix64(sig, reinterpret_cast<uintptr_t>(&v)), v.pair_capacity), v.sort.items);
     // The flags that change the launch sequence (a captured graph holds one).
     sig = mix64(sig, e.frame->flags & (GS_HIP_FLAG_DEPTH16 | GS_HIP_FLAG_POSE_GRAD | GS_HIP_FLAG_DETERMINISTIC));
+    sig = mix64(sig, shift);
     shape.any_pose = shape.any_pose || (e.frame->flags & GS_HIP_FLAG_POSE_GRAD) != 0u;
     shape.deterministic = shape.deterministic || (e.frame->flags & GS_HIP_FLAG_DETERMINISTIC) != 0u;
   }
   shape.scenes = uint32_t(scenes.size());
-  shape.tile_key_bits = uint32_t(bits_for(shape.max_tiles + 1));
-  shape.counting_tile_sort = ws->tuning.tile_digit_bits == 0u && shape.max_tiles + 1 <= kCountingMaxBins;
+  shape.tile_key_bits = uint32_t(bits_for(shape.max_bins + 1));
+  shape.counting_tile_sort = ws->tuning.tile_digit_bits == 0u && shape.max_bins + 1 <= kCountingMaxBins;
   for (const TableEntry& e : entries) {
     shape.counting_tile_sort = shape.counting_tile_sort && e.view->sort.tile_hist != nullptr &&
                                uint64_t(e.view->sort.tile_cols) * kTileChunk >= e.view->pair_capacity;
@@ -415,7 +427,7 @@ int upload_entries(gs_hip_workspace* ws, const std::vector<TableEntry>& entries)
     const TableEntry& e = entries[i];
     host[i] = make_view_dev(*e.view, *e.frame, ws->totals_all + size_t(i) * 16,
                             ws->tile_loss_all + size_t(ws->tile_base[i]) * 4, e.target, shape.counting_tile_sort,
-                            shape.deterministic);
+                            shape.deterministic, shift);
   }
   SceneDev* hs = ws->host_scenes + size_t(s) * ws->table_scenes;
   std::copy(scenes.begin(), scenes.end(), hs);
@@ -425,13 +437,14 @@ int upload_entries(gs_hip_workspace* ws, const std::vector<TableEntry>& entries)
   return 0;
 }
 
-int upload_table(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t count, const void* const* targets) {
+int upload_table(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t count, const void* const* targets,
+                 int bin_shift) {
   if (count == 0 || count > ws->views.size()) return set_error(GS_HIP_E_ARGUMENT, "view count out of range");
   std::vector<TableEntry> entries(count);
   for (uint32_t v = 0; v < count; ++v) {
     entries[v] = TableEntry{ws->views[v], &frames[v], ws, targets != nullptr ? targets[v] : nullptr};
   }
-  return upload_entries(ws, entries);
+  return upload_entries(ws, entries, bin_shift);
 }
 
 int alloc_tables(gs_hip_workspace* ws, uint32_t views, uint32_t scenes, uint32_t tile_entries) {
@@ -610,6 +623,7 @@ int gs_hip_workspace_create(gs_hip_workspace** out, uint32_t capacity, int sh_de
   ws->tuning.depth_digit_bits = 8;
   ws->tuning.tile_digit_bits = 8;
   ws->tuning.graph = 0;
+  ws->tuning.bin_shift = 0;
   const uint64_t cap = std::max<uint32_t>(capacity, 1);
   const uint64_t shn = cap * ws->sh_coeffs * 3;
   const uint64_t tiles = gsh::tile_count(ws);
@@ -831,6 +845,7 @@ int gs_hip_workspace_set_tuning(gs_hip_workspace* ws, const gs_hip_tuning* t) {
     return gsh::set_error(GS_HIP_E_ARGUMENT, "set_tuning: digit widths are 4..8 bits (tiles: 0 for one counting pass)");
   }
   if (t->raster_groups_per_cu > 64) return gsh::set_error(GS_HIP_E_ARGUMENT, "set_tuning: at most 64 groups per CU");
+  if (t->bin_shift > 2) return gsh::set_error(GS_HIP_E_ARGUMENT, "set_tuning: bin_shift is 0..2");
   const bool changed = std::memcmp(&ws->tuning, t, sizeof(*t)) != 0;
   ws->tuning = *t;
   if (changed) gsh::drop_graph(ws);

@@ -217,18 +217,20 @@ __device__ __forceinline__ void footprint_rows(float my, float c, float t, uint3
   footprint_pixels(v - ey, v + ey, height, first, last);
 }
 
-// Tiles [x0, x1) of tile row `ty` that the ellipse reaches within pixel rows
-// [row_first, row_last].  At height dy the ellipse spans
+// Bins [x0, x1) of bin row `ty` (bins of 2^side_log2 pixels: 16 for the
+// raster tiles) that the ellipse reaches within pixel rows [row_first,
+// row_last].  At height dy the ellipse spans
 // dx = (b/c) dy -+ sqrt((det/c)(t - dy^2/c)); over [y0, y1] the right end is
 // largest at the clamped height of the rightmost point, b sqrt(t/a), and the
 // left end smallest at that of the leftmost, -b sqrt(t/a).
 __device__ __forceinline__ void footprint_row_tiles(float mx, float my, float a, float b, float c, float t,
                                                     int row_first, int row_last, uint32_t ty, uint32_t width,
-                                                    uint32_t& x0, uint32_t& x1) {
+                                                    uint32_t& x0, uint32_t& x1, int side_log2 = 4) {
 #pragma clang fp contract(off)
   const float u = mx - 0.5f, v = my - 0.5f;
-  const float y0 = float(max(int(ty) * kTile, row_first)) - v;
-  const float y1 = float(min(int(ty) * kTile + kTile - 1, row_last)) - v;
+  const int top = int(ty) << side_log2;
+  const float y0 = float(max(top, row_first)) - v;
+  const float y1 = float(min(top + (1 << side_log2) - 1, row_last)) - v;
   const float k = b / c;
   const float w = (a * c - b * b) / c;
   const float side = b * sqrtf(t / a);
@@ -242,8 +244,8 @@ __device__ __forceinline__ void footprint_row_tiles(float mx, float my, float a,
     x0 = x1 = 0u;
     return;
   }
-  x0 = uint32_t(first) / kTile;
-  x1 = uint32_t(last) / kTile + 1u;
+  x0 = uint32_t(first) >> side_log2;
+  x1 = (uint32_t(last) >> side_log2) + 1u;
 }
 
 // ---- raster record --------------------------------------------------------

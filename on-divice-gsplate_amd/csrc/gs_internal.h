@@ -127,8 +127,20 @@ struct ViewDev {
   uint32_t* totals;  // [0] pairs, [1] emit mismatches, [5..7] densify counters, [8] visible splats (viewer)
   uint64_t pair_capacity;
   uint32_t capacity;    // the scene's slot capacity
-  uint32_t tile_count;  // tiles_x * tiles_y
+  uint32_t tile_count;  // tiles_x * tiles_y (16x16 raster tiles)
+  // Pair bins: squares of (16 << bin_shift) pixels; pair keys, ranges and
+  // the TILES counts are per bin, and raster tile (tx, ty) walks the list of
+  // bin (tx >> bin_shift, ty >> bin_shift).
+  uint32_t bin_shift;
+  uint32_t bins_x;
+  uint32_t bin_count;  // bins_x * bins_y
 };
+
+// The bin raster tile `tile` walks.
+__host__ __device__ inline uint32_t bin_of_tile(const ViewDev& v, uint32_t tile) {
+  const uint32_t tx = tile % v.f.tiles_x, ty = tile / v.f.tiles_x;
+  return (ty >> v.bin_shift) * v.bins_x + (tx >> v.bin_shift);
+}
 
 // Rasterizer work items name (view, tile): tiles per view stay below 2^16.
 constexpr uint32_t kTileBits = 16;
@@ -145,7 +157,8 @@ struct StepShape {
   uint64_t max_pair_capacity = 0;  // the largest pair buffer (per-pair grids)
   uint32_t max_tiles = 0;          // the largest tile grid (per-tile grids)
   uint32_t total_tiles = 0;        // every view's tiles (rasterizer work items)
-  uint32_t tile_key_bits = 0;      // tile sort key width
+  uint32_t max_bins = 0;           // the most pair bins of a view
+  uint32_t tile_key_bits = 0;      // tile sort key width (bins)
   bool counting_tile_sort = false; // one counting pass (tuning.tile_digit_bits == 0 and the tile grids fit)
   bool any_sh = false;             // some scene stores SH rest coefficients
   bool any_pose = false;           // some view asks for its pose gradient
@@ -296,10 +309,11 @@ int reserve_sort(View& v, uint64_t items);
 // targets (may be NULL, as may its entries): device RGBA8 photos the loss
 // kernels read in place instead of the views' own TARGET buffers.
 int upload_table(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t count,
-                 const void* const* targets = nullptr);
+                 const void* const* targets = nullptr, int bin_shift = -1);
 // The same for any entries (a batch's views of several scenes, grouped by
 // scene): sets ws->shape and ws->tile_base.
-int upload_entries(gs_hip_workspace* ws, const std::vector<TableEntry>& entries);
+// bin_shift < 0: the launch context's tuning.bin_shift.
+int upload_entries(gs_hip_workspace* ws, const std::vector<TableEntry>& entries, int bin_shift = -1);
 // Batched stages over views [0, count) of the uploaded table.
 int launch_project(gs_hip_workspace* ws, uint32_t count);
 int launch_depth_sort_and_scan(gs_hip_workspace* ws, uint32_t count, bool depth16);

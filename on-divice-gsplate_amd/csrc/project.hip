@@ -89,9 +89,10 @@ __device__ __forceinline__ void project_view(const gsh::ViewDev& vd, const Proje
       int row_first, row_last;
       footprint_rows(my, cc, thr, f.cam.height, row_first, row_last);
       if (row_first <= row_last) {
-        for (uint32_t tyi = uint32_t(row_first) / kTile; tyi <= uint32_t(row_last) / kTile; ++tyi) {
+        const int side_log2 = 4 + int(vd.bin_shift);
+        for (uint32_t tyi = uint32_t(row_first) >> side_log2; tyi <= (uint32_t(row_last) >> side_log2); ++tyi) {
           uint32_t x0, x1;
-          footprint_row_tiles(mx, my, ca, cb, cc, thr, row_first, row_last, tyi, f.cam.width, x0, x1);
+          footprint_row_tiles(mx, my, ca, cb, cc, thr, row_first, row_last, tyi, f.cam.width, x0, x1, side_log2);
           tiles += x1 - x0;
         }
       }

@@ -154,7 +154,7 @@ __device__ __forceinline__ SplatEval eval_splat(float4 sa, float4 sb, float cx, 
 __device__ __forceinline__ void forward_tile(const gsh::ViewDev& a, uint32_t tile, Staging& s) {
   const gs_hip_frame& f = a.f;
   const PixelCoord pc = pixel_coord(f, tile);
-  const uint2 range = gload(a.ranges + pc.tile);
+  const uint2 range = gload(a.ranges + gsh::bin_of_tile(a, pc.tile));
   const float cx = float(pc.px) + 0.5f, cy = float(pc.py) + 0.5f;
   // Ts is T while the pixel composites and -T once it has stopped (or lies
   // outside the image): "not done" is Ts > 0, one compare whose lane mask is
@@ -238,7 +238,7 @@ __device__ __forceinline__ void backward_tile(const gsh::ViewDev& a, uint32_t ti
   float* s_grad = sh.grad;
   uint32_t* s_wave_last = sh.wave_last;
   const PixelCoord pc = pixel_coord(f, tile);
-  const uint2 range = gload(a.ranges + pc.tile);
+  const uint2 range = gload(a.ranges + gsh::bin_of_tile(a, pc.tile));
   const float cx = float(pc.px) + 0.5f, cy = float(pc.py) + 0.5f;
 
   float T_final = 0.0f, gr = 0.0f, gg = 0.0f, gb = 0.0f;

@@ -418,7 +418,8 @@ __global__ void __launch_bounds__(256) k_emit(const ViewDev* __restrict__ views)
   }
   const uint64_t p0 = count > 0u ? uint64_t(gmem(v.offsets)[k]) : 0ull;
   const uint64_t cap = v.pair_capacity;
-  const uint32_t pad_tile = f.tiles_x * f.tiles_y;
+  const uint32_t pad_tile = v.bin_count;
+  const int side_log2 = 4 + int(v.bin_shift);
   GS_GLOBAL uint32_t* const out_tiles = gmem(v.emit_tiles);
   GS_GLOBAL uint32_t* const out_gauss = gmem(v.emit_gaussians);
   const bool small = count > 0u && count <= kWaveEmit;
@@ -438,13 +439,14 @@ __global__ void __launch_bounds__(256) k_emit(const ViewDev* __restrict__ views)
       uint64_t p = p0;
       uint32_t wanted = 0;
       if (row_first <= row_last) {
-        for (uint32_t ty = uint32_t(row_first) / kTile; ty <= uint32_t(row_last) / kTile; ++ty) {
+        for (uint32_t ty = uint32_t(row_first) >> side_log2; ty <= (uint32_t(row_last) >> side_log2); ++ty) {
           uint32_t x0, x1;
-          footprint_row_tiles(r0.x, r0.y, fp.x, fp.y, fp.z, fp.w, row_first, row_last, ty, f.cam.width, x0, x1);
+          footprint_row_tiles(r0.x, r0.y, fp.x, fp.y, fp.z, fp.w, row_first, row_last, ty, f.cam.width, x0, x1,
+                              side_log2);
           wanted += x1 - x0;
           for (uint32_t x = x0; x < x1 && p < end; ++x, ++p) {
             if (p >= win && p < win + kEmitWindow) {
-              s_tile[wave][p - win] = ty * f.tiles_x + x;
+              s_tile[wave][p - win] = ty * v.bins_x + x;
               s_gauss[wave][p - win] = i;
             }
           }
@@ -488,14 +490,15 @@ __global__ void __launch_bounds__(256) k_emit(const ViewDev* __restrict__ views)
     int row_first, row_last;
     footprint_rows(bmy, bfp.z, bfp.w, f.cam.height, row_first, row_last);
     if (row_first <= row_last) {
-      for (uint32_t ty = uint32_t(row_first) / kTile; ty <= uint32_t(row_last) / kTile; ++ty) {
+      for (uint32_t ty = uint32_t(row_first) >> side_log2; ty <= (uint32_t(row_last) >> side_log2); ++ty) {
         uint32_t x0, x1;
-        footprint_row_tiles(bmx, bmy, bfp.x, bfp.y, bfp.z, bfp.w, row_first, row_last, ty, f.cam.width, x0, x1);
+        footprint_row_tiles(bmx, bmy, bfp.x, bfp.y, bfp.z, bfp.w, row_first, row_last, ty, f.cam.width, x0, x1,
+                            side_log2);
         wanted += x1 - x0;
         for (uint32_t x = x0 + uint32_t(lane); x < x1; x += 64u) {
           const uint64_t q = p + (x - x0);
           if (q < end) {
-            out_tiles[q] = ty * f.tiles_x + x;
+            out_tiles[q] = ty * v.bins_x + x;
             out_gauss[q] = bi;
           }
         }
@@ -574,7 +577,7 @@ __global__ void __launch_bounds__(1024) k_work_order(const ViewDev* __restrict__
   for (uint32_t v = 0; v < count; ++v) {
     const ViewDev& view = views[v];
     for (uint32_t t = threadIdx.x; t < view.tile_count; t += blockDim.x) {
-      const uint2 r = view.ranges[t];
+      const uint2 r = view.ranges[gsh::bin_of_tile(view, t)];
       const uint32_t b = __clz(r.y - r.x);
       const uint64_t peers = match_digit(b, 6, __ballot(true));
       if ((peers & below_me) == 0ull) atomicAdd(&s_count[b], uint32_t(__popcll(peers)));
@@ -593,7 +596,7 @@ __global__ void __launch_bounds__(1024) k_work_order(const ViewDev* __restrict__
   for (uint32_t v = 0; v < count; ++v) {
     const ViewDev& view = views[v];
     for (uint32_t t = threadIdx.x; t < view.tile_count; t += blockDim.x) {
-      const uint2 r = view.ranges[t];
+      const uint2 r = view.ranges[gsh::bin_of_tile(view, t)];
       const uint32_t b = __clz(r.y - r.x);
       const uint64_t peers = match_digit(b, 6, __ballot(true));
       const int leader = __ffsll(static_cast<long long>(peers)) - 1;
@@ -629,7 +632,7 @@ __device__ __forceinline__ uint32_t pair_count(const ViewDev& v) {
 __global__ void __launch_bounds__(256) k_tile_hist(const ViewDev* __restrict__ views) {
   extern __shared__ uint32_t s_bins[];
   const ViewDev& v = views[blockIdx.y];
-  const uint32_t n = pair_count(v), bins = v.tile_count + 1u;
+  const uint32_t n = pair_count(v), bins = v.bin_count + 1u;
   const uint32_t col = blockIdx.x;
   if (uint64_t(col) * kTileChunk >= n) return;
   for (uint32_t t = threadIdx.x; t < bins; t += blockDim.x) s_bins[t] = 0u;
@@ -658,7 +661,7 @@ __global__ void __launch_bounds__(256) k_tile_hist(const ViewDev* __restrict__ v
 // are known.  The counts are requested in groups of 16 before they are summed.
 __global__ void __launch_bounds__(256) k_tile_colscan(const ViewDev* __restrict__ views) {
   const ViewDev& v = views[blockIdx.y];
-  const uint32_t bins = v.tile_count + 1u;
+  const uint32_t bins = v.bin_count + 1u;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= bins) return;
   const uint32_t cols = uint32_t((uint64_t(pair_count(v)) + kTileChunk - 1u) / kTileChunk);
@@ -683,7 +686,7 @@ __global__ void __launch_bounds__(256) k_tile_colscan(const ViewDev* __restrict_
 __global__ void __launch_bounds__(1024) k_tile_starts(const ViewDev* __restrict__ views) {
   __shared__ uint32_t s_wave[16];
   const ViewDev& v = views[blockIdx.y];
-  const uint32_t bins = v.tile_count + 1u;
+  const uint32_t bins = v.bin_count + 1u;
   const uint32_t per = (bins + 1023u) / 1024u;  // <= kCountingMaxBins / 1024
   GS_GLOBAL uint32_t* rg = gmem(reinterpret_cast<uint32_t*>(v.ranges));
   const uint32_t t0 = threadIdx.x * per;
@@ -723,7 +726,7 @@ __global__ void __launch_bounds__(64) k_tile_scatter(const ViewDev* __restrict__
   const uint32_t idx = (lin % 8u) * (gridDim.x * gridDim.y / 8u) + lin / 8u;
   const uint32_t view = idx / gridDim.x, col = idx % gridDim.x;
   const ViewDev& v = views[view];
-  const uint32_t n = pair_count(v), bins = v.tile_count + 1u;
+  const uint32_t n = pair_count(v), bins = v.bin_count + 1u;
   if (uint64_t(col) * kTileChunk >= n) return;
   const uint32_t lane = threadIdx.x;
   const GS_GLOBAL uint32_t* prefix = gmem(v.tile_hist) + size_t(col) * bins;
@@ -764,7 +767,7 @@ __global__ void __launch_bounds__(64) k_tile_scatter(const ViewDev* __restrict__
 }
 
 int counting_tile_sort(gs_hip_workspace* ws, uint32_t count) {
-  const uint32_t bins = ws->shape.max_tiles + 1u;
+  const uint32_t bins = ws->shape.max_bins + 1u;
   const uint32_t cols = std::max<uint32_t>(div_up(ws->shape.max_pair_capacity, kTileChunk), 1u);
   const size_t lds = size_t(bins) * 4u;
   k_tile_hist<<<dim3(cols, count), 256, lds, ws->stream>>>(ws->dev_views);

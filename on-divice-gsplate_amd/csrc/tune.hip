@@ -1,9 +1,11 @@
 // The launch-shape autotuner: times the batched step of the caller's views
 // under candidate shapes and keeps the fastest.  Shapes: persistent
 // rasterizer workgroups per CU (or one workgroup per tile), radix digit
-// widths of the depth and tile sorts, and graph replay of the whole step.
-// The tile stays 16x16, the reference's (trainer.h:351), so that the pair
-// order and tile ranges remain those of the reference.
+// widths of the depth and tile sorts (or one counting pass), the pair bin
+// size, and graph replay of the whole step.  Raster tiles stay 16x16, the
+// reference's (trainer.h:351); with bin_shift 0 the pairs and ranges are the
+// reference's, with 1 or 2 each tile walks its 32x32 or 64x64 bin's list and
+// draws the same splats in the same order.
 //
 // The search is coordinate-wise (one knob at a time, the others at their
 // best so far), which needs a dozen shapes instead of the full product.  Each
@@ -55,6 +57,13 @@ int gs_hip_autotune(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t c
       cur_ms = ms;
     }
   };
+  // The bin size first: it changes how many pairs the sorts see, and so which
+  // digit widths win.
+  for (uint32_t shift : {0u, 1u, 2u}) {
+    gs_hip_tuning c = cur;
+    c.bin_shift = shift;
+    consider(c);
+  }
   for (uint32_t groups : {0u, 1u, 2u, 4u, 8u}) {
     gs_hip_tuning c = cur;
     c.raster_groups_per_cu = groups;

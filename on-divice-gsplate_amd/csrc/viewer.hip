@@ -290,7 +290,7 @@ int gs_hip_view_render(gs_hip_workspace* ws, const gs_hip_frame* f, const float 
   a.out = reinterpret_cast<uint8_t*>(v.image);
   a.visible = v.totals + 8;
   for (int attempt = 0;; ++attempt) {
-    int rc = gsh::upload_table(ws, f, 1);
+    int rc = gsh::upload_table(ws, f, 1, nullptr, 0);  // the viewer bins by its 16x16 tiles
     if (rc == 0) rc = gsh::fill_u32(ws, a.visible, 1, 0u);
     if (rc != 0) return rc;
     if (ws->capacity > 0) {

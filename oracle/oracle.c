@@ -37,7 +37,10 @@
  *   orc_sh_basis64            scipy's complex spherical harmonics (real form)
  *   orc_footprint_tiles32     brute-force per-pixel alpha >= 1/255 rule
  *                             (test_trainer.cpp:235-289)
- *   orc_bin32                 Python sorted() on (tile, key, index) tuples
+ *   orc_footprint_bins32      the same rule at 32 and 64 px bins, and each bin's
+ *                             set holds the union of its 16 px tiles' sets
+ *   orc_bin32, orc_bin32_bins Python sorted() on (bin, key, index) tuples
+ *   orc_bin_counts32          orc_footprint_bins32 slot by slot
  *   orc_render64              closed forms (one splat, two splats in depth order);
  *                             its tile lists equal orc_render64_brute bit for bit
  *   orc_forward32             orc_render64 within fp32 tolerance
@@ -219,10 +222,12 @@ static void footprint_rows(float my, float c, float t, uint32_t height, int* fir
   footprint_pixels(v - ey, v + ey, height, first, last);
 }
 
+/* Bins of `bin` pixels (TILE for the reference's tiles) the ellipse reaches
+ * in bin row ty. */
 static void footprint_row_tiles(float mx, float my, float a, float b, float c, float t, int row_first, int row_last,
-                                uint32_t ty, uint32_t width, uint32_t* x0, uint32_t* x1) {
+                                uint32_t ty, uint32_t width, uint32_t bin, uint32_t* x0, uint32_t* x1) {
   const float u = mx - 0.5f, v = my - 0.5f;
-  const int top = (int)ty * TILE, bottom = (int)ty * TILE + TILE - 1;
+  const int top = (int)(ty * bin), bottom = (int)(ty * bin + bin) - 1;
   const float y0 = (float)(top > row_first ? top : row_first) - v;
   const float y1 = (float)(bottom < row_last ? bottom : row_last) - v;
   const float k = b / c;
@@ -238,28 +243,35 @@ static void footprint_row_tiles(float mx, float my, float a, float b, float c, f
     *x0 = *x1 = 0u;
     return;
   }
-  *x0 = (uint32_t)first / TILE;
-  *x1 = (uint32_t)last / TILE + 1u;
+  *x0 = (uint32_t)first / bin;
+  *x1 = (uint32_t)last / bin + 1u;
 }
 
-/* Tiles (ty * tiles_x + tx, row by row) the footprint reaches; returns the
- * count, writes at most max_out of them. */
-uint32_t orc_footprint_tiles32(float mx, float my, float a, float b, float c, float t, uint32_t width,
-                               uint32_t height, uint32_t* out, uint32_t max_out) {
-  const uint32_t tiles_x = (width + TILE - 1) / TILE;
+/* Bins of `side` pixels (by * bins_x + bx, row by row) the footprint reaches;
+ * returns the count, writes at most max_out of them.  side = TILE gives the
+ * reference's tiles; the kernels' coarser pair bins (32, 64) use the same
+ * rule per bin row. */
+uint32_t orc_footprint_bins32(float mx, float my, float a, float b, float c, float t, uint32_t width,
+                              uint32_t height, uint32_t side, uint32_t* out, uint32_t max_out) {
+  const uint32_t bins_x = (width + side - 1) / side;
   int rf, rl;
   footprint_rows(my, c, t, height, &rf, &rl);
   uint32_t n = 0;
   if (rf > rl) return 0;
-  for (uint32_t ty = (uint32_t)rf / TILE; ty <= (uint32_t)rl / TILE; ++ty) {
+  for (uint32_t ty = (uint32_t)rf / side; ty <= (uint32_t)rl / side; ++ty) {
     uint32_t x0, x1;
-    footprint_row_tiles(mx, my, a, b, c, t, rf, rl, ty, width, &x0, &x1);
+    footprint_row_tiles(mx, my, a, b, c, t, rf, rl, ty, width, side, &x0, &x1);
     for (uint32_t x = x0; x < x1; ++x) {
-      if (out != NULL && n < max_out) out[n] = ty * tiles_x + x;
+      if (out != NULL && n < max_out) out[n] = ty * bins_x + x;
       ++n;
     }
   }
   return n;
+}
+
+uint32_t orc_footprint_tiles32(float mx, float my, float a, float b, float c, float t, uint32_t width,
+                               uint32_t height, uint32_t* out, uint32_t max_out) {
+  return orc_footprint_bins32(mx, my, a, b, c, t, width, height, TILE, out, max_out);
 }
 
 /* ===================== fp32 projection (train_project.comp:26-112) ========= */
@@ -324,7 +336,7 @@ void orc_project32(const float* params, const float* sh, uint32_t n, uint32_t co
         if (rf <= rl) {
           for (uint32_t tyi = (uint32_t)rf / TILE; tyi <= (uint32_t)rl / TILE; ++tyi) {
             uint32_t x0, x1;
-            footprint_row_tiles(mx, my, ca, cb, cc, thr, rf, rl, tyi, cam->width, &x0, &x1);
+            footprint_row_tiles(mx, my, ca, cb, cc, thr, rf, rl, tyi, cam->width, TILE, &x0, &x1);
             count += x1 - x0;
           }
         }
@@ -401,10 +413,10 @@ static int pair_cmp(const void* a, const void* b) {
  * (tile, key, index) and fills ranges [(tiles+1)][2].  Returns the pair
  * count; writes up to max_pairs pairs.  (A slot whose footprint walk finds
  * fewer tiles than its count pads with the overflow tile `tiles`.) */
-uint64_t orc_bin32(const float* records, const float* footprint, const uint32_t* tiles, const uint32_t* keys,
-                   uint32_t n, uint32_t width, uint32_t height, uint32_t* pair_tiles, uint32_t* pair_gaussians,
-                   uint64_t max_pairs, uint32_t* ranges) {
-  const uint32_t tiles_x = (width + TILE - 1) / TILE, tiles_y = (height + TILE - 1) / TILE;
+uint64_t orc_bin32_bins(const float* records, const float* footprint, const uint32_t* tiles, const uint32_t* keys,
+                        uint32_t n, uint32_t width, uint32_t height, uint32_t side, uint32_t* pair_tiles,
+                        uint32_t* pair_gaussians, uint64_t max_pairs, uint32_t* ranges) {
+  const uint32_t tiles_x = (width + side - 1) / side, tiles_y = (height + side - 1) / side;
   const uint32_t T = tiles_x * tiles_y;
   uint64_t* first = (uint64_t*)malloc(sizeof(uint64_t) * ((size_t)n + 1));
   first[0] = 0;
@@ -420,7 +432,7 @@ uint64_t orc_bin32(const float* records, const float* footprint, const uint32_t*
       if (tiles[i] == 0) continue;
       const float* r = records + (size_t)i * 12;
       const float* f = footprint + (size_t)i * 4;
-      const uint32_t c = orc_footprint_tiles32(r[0], r[1], f[0], f[1], f[2], f[3], width, height, buf, T + 1);
+      const uint32_t c = orc_footprint_bins32(r[0], r[1], f[0], f[1], f[2], f[3], width, height, side, buf, T + 1);
       for (uint32_t k = 0; k < tiles[i]; ++k) {
         pair_t* q = &emitted[first[i] + k];
         q->tile = k < c ? buf[k] : T;
@@ -461,6 +473,26 @@ uint64_t orc_bin32(const float* records, const float* footprint, const uint32_t*
   free(start);
   free(fill);
   return total;
+}
+
+uint64_t orc_bin32(const float* records, const float* footprint, const uint32_t* tiles, const uint32_t* keys,
+                   uint32_t n, uint32_t width, uint32_t height, uint32_t* pair_tiles, uint32_t* pair_gaussians,
+                   uint64_t max_pairs, uint32_t* ranges) {
+  return orc_bin32_bins(records, footprint, tiles, keys, n, width, height, TILE, pair_tiles, pair_gaussians,
+                        max_pairs, ranges);
+}
+
+/* Bins of `side` pixels each slot's footprint reaches (0 for a slot the
+ * projection culled: tiles[i] == 0). */
+void orc_bin_counts32(const float* records, const float* footprint, const uint32_t* tiles, uint32_t n,
+                      uint32_t width, uint32_t height, uint32_t side, uint32_t* counts) {
+#pragma omp parallel for schedule(static)
+  for (uint32_t i = 0; i < n; ++i) {
+    const float* r = records + (size_t)i * 12;
+    const float* f = footprint + (size_t)i * 4;
+    counts[i] = tiles[i] == 0 ? 0u : orc_footprint_bins32(r[0], r[1], f[0], f[1], f[2], f[3], width, height, side,
+                                                          NULL, 0);
+  }
 }
 
 /* ===================== fp32 tile compositor (train_forward.comp:77-117) ===== */

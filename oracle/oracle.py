@@ -46,6 +46,9 @@ def library() -> ctypes.CDLL:
             "orc_footprint_tiles32": ([f32, f32, f32, f32, f32, f32, u32, u32, P(u32), u32], u32),
             "orc_project32": ([P(f32), P(f32), u32, u32, cam_p, u32, i32, P(f32), P(f32), P(u32), P(u32)], None),
             "orc_bin32": ([P(f32), P(f32), P(u32), P(u32), u32, u32, u32, P(u32), P(u32), u64, P(u32)], u64),
+            "orc_bin32_bins": ([P(f32), P(f32), P(u32), P(u32), u32, u32, u32, u32, P(u32), P(u32), u64, P(u32)], u64),
+            "orc_footprint_bins32": ([f32, f32, f32, f32, f32, f32, u32, u32, u32, P(u32), u32], u32),
+            "orc_bin_counts32": ([P(f32), P(f32), P(u32), u32, u32, u32, u32, P(u32)], None),
             "orc_forward32": ([P(f32), P(u32), P(u32), u32, u32, P(f32), P(u32), P(ctypes.c_uint8), P(f32)], None),
             "orc_project64": ([P(f64), P(f64), u32, u32, cam_p, i32, P(f64)], None),
             "orc_render64": ([P(f64), P(f64), u32, u32, cam_p, i32, P(f64), P(f64), P(f64)], None),
@@ -105,12 +108,24 @@ def sh_basis64(d, degree: int) -> np.ndarray:
     return out
 
 
-def footprint_tiles32(mx, my, a, b, c, t, width, height) -> np.ndarray:
+def footprint_tiles32(mx, my, a, b, c, t, width, height, side: int = 16) -> np.ndarray:
+    """The bins of `side` pixels (16: the reference's tiles) a footprint reaches."""
     lib = library()
-    n = lib.orc_footprint_tiles32(mx, my, a, b, c, t, width, height, None, 0)
+    n = lib.orc_footprint_bins32(mx, my, a, b, c, t, width, height, side, None, 0)
     out = np.zeros(max(n, 1), np.uint32)
-    lib.orc_footprint_tiles32(mx, my, a, b, c, t, width, height, _p(out, ctypes.c_uint32), n)
+    lib.orc_footprint_bins32(mx, my, a, b, c, t, width, height, side, _p(out, ctypes.c_uint32), n)
     return out[:n]
+
+
+def bin_counts32(records, foot, tiles, width: int, height: int, side: int) -> np.ndarray:
+    """Bins of `side` pixels each slot's footprint reaches (0 where tiles is 0)."""
+    n = records.shape[0]
+    out = np.zeros(n, np.uint32)
+    library().orc_bin_counts32(_p(np.ascontiguousarray(records, np.float32), ctypes.c_float),
+                               _p(np.ascontiguousarray(foot, np.float32), ctypes.c_float),
+                               _p(np.ascontiguousarray(tiles, np.uint32), ctypes.c_uint32), n, width, height, side,
+                               _p(out, ctypes.c_uint32))
+    return out
 
 
 def project32(params, sh, cam, sh_degree: int, depth16: bool = False):
@@ -129,18 +144,21 @@ def project32(params, sh, cam, sh_degree: int, depth16: bool = False):
     return records, foot, tiles, keys
 
 
-def bin32(records, foot, tiles, keys, width: int, height: int):
+def bin32(records, foot, tiles, keys, width: int, height: int, side: int = 16):
+    """Pairs sorted by (bin, key, index) and the bins' ranges; `tiles` holds
+    each slot's count of bins of `side` pixels (16: the reference's tiles)."""
     n = records.shape[0]
     total = int(np.sum(tiles, dtype=np.uint64))
     pt = np.zeros(max(total, 1), np.uint32)
     pg = np.zeros(max(total, 1), np.uint32)
-    T = ((width + 15) // 16) * ((height + 15) // 16)
+    T = ((width + side - 1) // side) * ((height + side - 1) // side)
     ranges = np.zeros((T + 1, 2), np.uint32)
-    got = library().orc_bin32(_p(np.ascontiguousarray(records, np.float32), ctypes.c_float),
-                              _p(np.ascontiguousarray(foot, np.float32), ctypes.c_float),
-                              _p(np.ascontiguousarray(tiles, np.uint32), ctypes.c_uint32),
-                              _p(np.ascontiguousarray(keys, np.uint32), ctypes.c_uint32), n, width, height,
-                              _p(pt, ctypes.c_uint32), _p(pg, ctypes.c_uint32), total, _p(ranges, ctypes.c_uint32))
+    got = library().orc_bin32_bins(_p(np.ascontiguousarray(records, np.float32), ctypes.c_float),
+                                   _p(np.ascontiguousarray(foot, np.float32), ctypes.c_float),
+                                   _p(np.ascontiguousarray(tiles, np.uint32), ctypes.c_uint32),
+                                   _p(np.ascontiguousarray(keys, np.uint32), ctypes.c_uint32), n, width, height, side,
+                                   _p(pt, ctypes.c_uint32), _p(pg, ctypes.c_uint32), total,
+                                   _p(ranges, ctypes.c_uint32))
     assert got == total
     return pt[:total], pg[:total], ranges
 

@@ -42,13 +42,15 @@ def test_errors_surface_as_exceptions_without_a_gpu():
     assert lib.gs_hip_last_error(buf, 256) == 2 and b"null" in buf.value
 
 
-def test_trainer_structs_match_the_header(tmp_path):
-    """Every field of the trainer handle's structs sits at the offset the C
-    compiler gives it (gcc on include/gs_hip.h)."""
-    structs = [gs.TrainerOptions, gs.TrainerInfo, gs.TrainerStats, gs.PassTimes, gs.TrainerEval]
+def test_binding_structs_match_the_header(tmp_path):
+    """Every field of the binding's structs sits at the offset the C compiler
+    gives it (gcc on include/gs_hip.h)."""
+    structs = [gs.TrainerOptions, gs.TrainerInfo, gs.TrainerStats, gs.PassTimes, gs.TrainerEval, gs.Tuning,
+               gs.Frame, gs.Camera]
     c_names = {gs.TrainerOptions: "gs_hip_trainer_options", gs.TrainerInfo: "gs_hip_trainer_info",
                gs.TrainerStats: "gs_hip_trainer_stats", gs.PassTimes: "gs_hip_pass_times",
-               gs.TrainerEval: "gs_hip_trainer_eval"}
+               gs.TrainerEval: "gs_hip_trainer_eval", gs.Tuning: "gs_hip_tuning", gs.Frame: "gs_hip_frame",
+               gs.Camera: "gs_hip_camera"}
     lines = []
     for s in structs:
         c = c_names[s]

@@ -144,15 +144,20 @@ def test_depth16_key_matches_reference_formula():
     assert np.all(keys[~vis] == 0xFFFFFFFF)
 
 
-def test_footprint_against_brute_force():
+@pytest.mark.parametrize("side", [16, 32, 64])
+def test_footprint_against_brute_force(side):
     """engine/tests/test_trainer.cpp:235-289: every tile holding a pixel with
-    d^T conic d <= 2 ln(255 o) is emitted, once, and few others."""
-    w, h = 64, 48
-    params, sh = make_scene(200, 1, 13)
+    d^T conic d <= 2 ln(255 o) is emitted, once, and few others -- for the
+    reference's 16 px tiles and the kernels' coarser 32 / 64 px pair bins."""
+    w, h = 160, 112
+    params, sh = make_scene(300, 1, 13)
     params[::2, 4] += 1.5  # half of them stretched
     cam = make_camera(w, h, 0.0)
     rec, foot, tiles, _ = orc.project32(params, sh, cam, 1)
-    tx = (w + 15) // 16
+    counts = orc.bin_counts32(rec, foot, tiles, w, h, side)
+    if side == 16:
+        np.testing.assert_array_equal(counts, tiles)
+    tx = (w + side - 1) // side
     needed = emitted = 0
     ys, xs = np.mgrid[0:h, 0:w]
     for i in np.nonzero(tiles)[0]:
@@ -160,10 +165,10 @@ def test_footprint_against_brute_force():
         t = 2.0 * math.log(255.0 * q[5])
         dx, dy = q[0] - (xs + 0.5), q[1] - (ys + 0.5)
         inside = q[2] * dx * dx + 2 * q[3] * dx * dy + q[4] * dy * dy <= t
-        want = set(((ys[inside] // 16) * tx + xs[inside] // 16).tolist())
+        want = set(((ys[inside] // side) * tx + xs[inside] // side).tolist())
         got = orc.footprint_tiles32(*[float(v) for v in (q[0], q[1], foot[i, 0], foot[i, 1], foot[i, 2],
-                                                          foot[i, 3])], w, h)
-        assert len(got) == tiles[i]
+                                                          foot[i, 3])], w, h, side)
+        assert len(got) == counts[i]
         assert len(set(got.tolist())) == len(got), "duplicate tile"
         assert want <= set(got.tolist()), f"gaussian {i}: missing tiles"
         needed += len(want)
@@ -172,21 +177,55 @@ def test_footprint_against_brute_force():
     assert emitted <= needed + needed // 20
 
 
-def test_bin_order_is_tile_depth_index():
+@pytest.mark.parametrize("shift", [1, 2])
+def test_coarse_bins_hold_their_tiles(shift):
+    """A bin of 16 << shift pixels lists a splat wherever one of its 16 px
+    tiles does (so a raster tile walking its bin's list meets every splat of
+    its own list, in the same depth order), and few more."""
+    w, h, side = 200, 136, 16 << shift
+    params, sh = make_scene(400, 2, 17)
+    params[::3, 4:6] += 1.2
+    cam = make_camera(w, h, 0.4)
+    rec, foot, tiles, keys = orc.project32(params, sh, cam, 2, depth16=True)
+    tx, bx = (w + 15) // 16, (w + side - 1) // side
+    extra = total = 0
+    for i in np.nonzero(tiles)[0]:
+        args = [float(v) for v in (rec[i, 0], rec[i, 1], *foot[i])]
+        fine = orc.footprint_tiles32(*args, w, h, 16)
+        coarse = set(orc.footprint_tiles32(*args, w, h, side).tolist())
+        want = {int((t // tx) >> shift) * bx + int((t % tx) >> shift) for t in fine.tolist()}
+        assert want <= coarse, f"gaussian {i}"
+        extra += len(coarse - want)
+        total += len(coarse)
+    assert total > 300 and extra <= total // 50
+    counts = orc.bin_counts32(rec, foot, tiles, w, h, side)
+    pt, pg, ranges = orc.bin32(rec, foot, counts, keys, w, h, side)
+    fine_pt, fine_pg, fine_ranges = orc.bin32(rec, foot, tiles, keys, w, h)
+    for t in range(tx * ((h + 15) // 16)):
+        b = ((t // tx) >> shift) * bx + ((t % tx) >> shift)
+        own = fine_pg[fine_ranges[t, 0]:fine_ranges[t, 1]].tolist()
+        lst = pg[ranges[b, 0]:ranges[b, 1]].tolist()
+        pos = {g: k for k, g in enumerate(lst)}
+        assert all(g in pos for g in own) and [pos[g] for g in own] == sorted(pos[g] for g in own)
+
+
+@pytest.mark.parametrize("side", [16, 32])
+def test_bin_order_is_tile_depth_index(side):
     w, h = 64, 48
     params, sh = make_scene(400, 1, 37)
     cam = make_camera(w, h, 0.0)
     rec, foot, tiles, keys = orc.project32(params, sh, cam, 1, depth16=True)
-    pt, pg, ranges = orc.bin32(rec, foot, tiles, keys, w, h)
+    counts = orc.bin_counts32(rec, foot, tiles, w, h, side)
+    pt, pg, ranges = orc.bin32(rec, foot, counts, keys, w, h, side)
     pairs = []
     for i in np.nonzero(tiles)[0]:
-        for t in orc.footprint_tiles32(*[float(v) for v in (rec[i, 0], rec[i, 1], *foot[i])], w, h):
+        for t in orc.footprint_tiles32(*[float(v) for v in (rec[i, 0], rec[i, 1], *foot[i])], w, h, side):
             pairs.append((int(t), int(keys[i]), int(i)))
     want = sorted(pairs)
-    assert len(want) > 500 and len({p[1] for p in want}) > 100
+    assert len(want) > (500 if side == 16 else 200) and len({p[1] for p in want}) > 100
     assert [p[0] for p in want] == pt.tolist()
     assert [p[2] for p in want] == pg.tolist()
-    T = ((w + 15) // 16) * ((h + 15) // 16)
+    T = ((w + side - 1) // side) * ((h + side - 1) // side)
     for t in range(T):
         s, e = ranges[t]
         assert np.all(pt[s:e] == t)
