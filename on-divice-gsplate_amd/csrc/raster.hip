@@ -79,9 +79,10 @@ __device__ __forceinline__ PixelCoord pixel_coord(const gs_hip_frame& f, uint32_
 
 // One batch of splats in LDS, in evaluation form: three float4 per splat,
 // so one address register and three immediate offsets reach a splat.
+template <int B = kBatch>
 struct Staging {
-  float4 sp[kBatch * 3];  // [3j]: mean x, mean y, A = -a/2, B = -b; [3j + 1]: C = -c/2, opacity, r, g; [3j + 2].x: b
-  uint64_t mask[4][kBatch / 64];  // [quadrant][loading wave]
+  float4 sp[B * 3];  // [3j]: mean x, mean y, A = -a/2, B = -b; [3j + 1]: C = -c/2, opacity, r, g; [3j + 2].x: b
+  uint64_t mask[4][B / 64];  // [quadrant][loading wave]
   __device__ __forceinline__ float4 a(int j) const { return sp[3 * j]; }
   __device__ __forceinline__ float4 b(int j) const { return sp[3 * j + 1]; }
   __device__ __forceinline__ float blue(int j) const { return sp[3 * j + 2].x; }
@@ -93,7 +94,8 @@ struct Staging {
 // quadratic form), with a margin above the rounding of the per-pixel
 // evaluation (gsh::quadrant_may_draw), so a quadrant is dropped only where
 // every pixel skips the splat anyway.
-__device__ __forceinline__ uint32_t stage(Staging& s, const float4* __restrict__ records, uint32_t g, int t, int tx0,
+template <int B>
+__device__ __forceinline__ uint32_t stage(Staging<B>& s, const float4* __restrict__ records, uint32_t g, int t, int tx0,
                                           int ty0) {
   const float4* r = records + size_t(g) * 3u;
   const float4 ra = gload(r), rb = gload(r + 1), rc = gload(r + 2);
@@ -123,7 +125,8 @@ __device__ __forceinline__ uint32_t stage(Staging& s, const float4* __restrict__
   return hits;
 }
 
-__device__ __forceinline__ void publish_masks(Staging& s, uint32_t hits, int wave, int lane) {
+template <int B>
+__device__ __forceinline__ void publish_masks(Staging<B>& s, uint32_t hits, int wave, int lane) {
   for (int q = 0; q < 4; ++q) {
     const uint64_t m = __ballot((hits >> q) & 1u);
     if (lane == 0) s.mask[q][wave] = m;
@@ -150,8 +153,10 @@ __device__ __forceinline__ SplatEval eval_splat(float4 sa, float4 sb, float cx, 
   return e;
 }
 
-// Front-to-back compositing of one tile of one view.
-__device__ __forceinline__ void forward_tile(const gsh::ViewDev& a, uint32_t tile, Staging& s) {
+// Front-to-back compositing of one tile of one view, staging B splats per
+// batch (B / 256 per thread).
+template <int B>
+__device__ __forceinline__ void forward_tile(const gsh::ViewDev& a, uint32_t tile, Staging<B>& s) {
   const gs_hip_frame& f = a.f;
   const PixelCoord pc = pixel_coord(f, tile);
   const uint2 range = gload(a.ranges + gsh::bin_of_tile(a, pc.tile));
@@ -162,20 +167,21 @@ __device__ __forceinline__ void forward_tile(const gsh::ViewDev& a, uint32_t til
   float Ts = pc.inside ? 1.0f : -1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
   uint32_t last = 0u;
 
-  for (uint32_t start = range.x; start < range.y; start += kBatch) {
+  for (uint32_t start = range.x; start < range.y; start += B) {
     if (__syncthreads_and(Ts < 0.0f)) break;
-    const uint32_t n = min(uint32_t(kBatch), range.y - start);
-    uint32_t hits = 0u;
-    if (threadIdx.x < n) {
-      hits = stage(s, a.records, gmem(a.pair_gaussians)[start + threadIdx.x], threadIdx.x, pc.tx0, pc.ty0);
+    const uint32_t n = min(uint32_t(B), range.y - start);
+    for (int r = 0; r < B / kThreads; ++r) {
+      const uint32_t t = uint32_t(r * kThreads) + threadIdx.x;
+      uint32_t hits = 0u;
+      if (t < n) hits = stage(s, a.records, gmem(a.pair_gaussians)[start + t], int(t), pc.tx0, pc.ty0);
+      publish_masks(s, hits, r * (kThreads / 64) + pc.wave, pc.lane);
     }
-    publish_masks(s, hits, pc.wave, pc.lane);
     __syncthreads();
     // The walk ends with the masks or when every pixel of the wave is done;
     // both loop conditions are wave-uniform (scalar registers).
     uint64_t alive = __ballot(Ts > 0.0f);
     const uint32_t base = start - range.x + 1u;  // LAST of splat j is base + j
-    for (int chunk = 0; chunk < kBatch / 64 && alive != 0ull; ++chunk) {
+    for (int chunk = 0; chunk < B / 64 && alive != 0ull; ++chunk) {
       uint64_t m = uniform_u64(s.mask[pc.wave][chunk]);
       while (m != 0ull && alive != 0ull) {
         const int j = chunk * 64 + __builtin_ctzll(m);
@@ -213,7 +219,7 @@ __device__ __forceinline__ void forward_tile(const gsh::ViewDev& a, uint32_t til
 // which tile or wave came first).
 template <bool kDet>
 struct BackwardShared {
-  Staging s;
+  Staging<> s;
   uint32_t g[kBatch];
   float grad[(kDet ? 4 : 1) * kBatch * kGrad];  // [wave][splat][9]: the nine lanes of one reduction hit nine banks
   uint32_t wave_last[4];
@@ -233,7 +239,7 @@ template <bool kDet>
 __device__ __forceinline__ void backward_tile(const gsh::ViewDev& a, uint32_t tile, BackwardShared<kDet>& sh,
                                               const Reduce9& red, int slot) {
   const gs_hip_frame& f = a.f;
-  Staging& s = sh.s;
+  Staging<>& s = sh.s;
   uint32_t* s_g = sh.g;
   float* s_grad = sh.grad;
   uint32_t* s_wave_last = sh.wave_last;
@@ -361,13 +367,14 @@ __device__ __forceinline__ uint32_t next_item(const RasterArgs& a, uint32_t* s_i
   return item;
 }
 
+template <int B>
 __global__ void __launch_bounds__(kThreads) k_forward(RasterArgs a) {
-  __shared__ Staging s;
+  __shared__ Staging<B> s;
   __shared__ uint32_t s_item;
   uint32_t item = a.persistent != 0u ? next_item(a, &s_item) : blockIdx.x;
   while (item < a.items) {
     const uint32_t w = a.work[item];
-    forward_tile(a.views[w >> gsh::kTileBits], w & gsh::kMaxTilesPerView, s);
+    forward_tile<B>(a.views[w >> gsh::kTileBits], w & gsh::kMaxTilesPerView, s);
     if (a.persistent == 0u) break;
     __syncthreads();
     item = next_item(a, &s_item);
@@ -428,7 +435,7 @@ int launch_forward(gs_hip_workspace* ws, uint32_t count) {
     if (rc != 0) return rc;
   }
   timer_begin(ws, "forward");
-  k_forward<<<grid, kThreads, 0, ws->stream>>>(a);
+  k_forward<kBatch><<<grid, kThreads, 0, ws->stream>>>(a);
   GS_CHECK_LAUNCH("k_forward");
   timer_end(ws);
   return 0;
