@@ -186,31 +186,77 @@ def run(gs, ws, cams, target_bufs, order, per_step, views_total, warmup_views, d
     return time.perf_counter() - t0, steps
 
 
+def mean_tile_pairs(gs, ws, cams, target_bufs, order, per_step, state, deg, tp):
+    """(16x16 tile, gaussian) overlaps per view, the pair count of the byte
+    model, whatever bin size the run used: every training camera once,
+    NO_UPDATE steps with 16 px bins and no graph, then the run's launch shape
+    back.  Also returns the pairs per view at the run's own bin size over the
+    same cameras."""
+    shape = ws.tuning()
+    views = [int(v) for v in dict.fromkeys(int(v) for v in order)]
+    counts = {}
+    for shift in sorted({0, shape.bin_shift}):
+        ws.set_tuning(graph=0, bin_shift=shift)
+        total = 0
+        for k in range(0, len(views), per_step):
+            vs = views[k:k + per_step]
+            frames = [gs.make_frame(cams[v], state["slots"], deg, step=state["step"], tp=tp,
+                                    flags=state["flags"] | gs.FLAG_NO_UPDATE) for v in vs]
+            if len(vs) == 1:
+                ws.set_target_device(target_bufs[vs[0]].ptr)
+                pairs, _ = ws.train_step(frames[0], want_losses=False)
+                total += pairs
+            else:
+                pairs, _ = ws.train_views(frames, [target_bufs[v].ptr for v in vs], want_losses=False)
+                total += sum(pairs)
+        counts[shift] = total / len(views)
+    ws.set_tuning(graph=shape.graph, bin_shift=shape.bin_shift)
+    return counts[0], counts[shape.bin_shift]
+
+
 TRAFFIC_PROFILE = os.path.join("profiles", "hbm_traffic_8views_1M_1280x720_pmc.md")
 SQ_PROFILE = os.path.join("profiles", "sq_counters_8views_1M_1280x720_pmc.md")
 VALU_PEAK = 1024 * 2.4e9 / 4  # wave64 VALU instructions per second: 1024 SIMDs, 4 cycles each
 
 
-def profiled_valu(kernel):
+def profile_shape(text):
+    """The launch shape a profile summary was taken at, from the `--shape G,D,T,GRAPH[,BIN]` of its
+    command: (raster groups per CU, depth digit bits, tile digit bits, bin shift); graph replay
+    leaves the kernels as they are and is not part of it."""
+    m = re.search(r"--shape ([0-9]+),([0-9]+),([0-9]+),[0-9]+(?:,([0-9]+))?", text)
+    if m is None:
+        return None
+    return int(m.group(1)), int(m.group(2)), int(m.group(3)), int(m.group(4) or 0)
+
+
+def profile_row(path, kernel, columns, tuning):
+    """The first `columns` numbers of `kernel`'s row in the profile summary at
+    `path` when that summary was taken at the run's pair-bin size, else None.
+    The rasterizers' work depends on the bin size; the sort digit widths and
+    the grid mode (one workgroup per tile or persistent) leave it as it is."""
+    try:
+        text = open(os.path.join(ROOT, path)).read()
+    except OSError:
+        return None
+    shape = profile_shape(text)
+    if tuning is None or shape is None or shape[3] != tuning["bin_shift"]:
+        return None
+    m = re.search(r"^\| k_%s \|((?: [0-9.e+]+ \|){%d})" % (re.escape(kernel), columns), text, re.M)
+    return [float(x) for x in m.group(1).split("|")[:-1]] if m else None
+
+
+def profiled_valu(kernel, tuning):
     """VALU instructions of one 8-view launch of `kernel` from the SQ counter
-    summary of this workload, or None."""
-    try:
-        text = open(os.path.join(ROOT, SQ_PROFILE)).read()
-    except OSError:
-        return None
-    m = re.search(r"^\| k_%s \| \d+ \| \d+ \| ([0-9.e+]+) \|" % re.escape(kernel), text, re.M)
-    return float(m.group(1)) if m else None
+    summary of this workload at the run's launch shape, or None."""
+    row = profile_row(SQ_PROFILE, kernel, 3, tuning)
+    return row[2] if row else None
 
 
-def profiled_traffic(kernel):
+def profiled_traffic(kernel, tuning):
     """HBM bytes of one 8-view launch of `kernel` (calibrated FETCH_SIZE + WRITE_SIZE) from the
-    rocprofv3 summary of this workload, or None."""
-    try:
-        text = open(os.path.join(ROOT, TRAFFIC_PROFILE)).read()
-    except OSError:
-        return None
-    m = re.search(r"^\| k_%s \| \d+ \| ([0-9.]+) \| ([0-9.]+) \| ([0-9.]+) \|" % re.escape(kernel), text, re.M)
-    return int(float(m.group(3)) * 1e6) if m else None
+    rocprofv3 summary of this workload at the run's launch shape, or None."""
+    row = profile_row(TRAFFIC_PROFILE, kernel, 4, tuning)
+    return int(row[3] * 1e6) if row else None
 
 
 RESOLUTIONS = ((640, 360), (1280, 720), (1920, 1080))
@@ -556,7 +602,9 @@ def main():
             best, tune_ms = ws.autotune(frames, [target_bufs[v].ptr for v in vs], repeats=3)
             tuning = best.as_dict()
             log(f"{mode}: autotuned launch shape {tuning} ({tune_ms:.3f} ms per step while tuning)")
-        graph = bool(tuning and tuning["graph"])
+        if tuning is None:
+            tuning = ws.tuning().as_dict()
+        graph = bool(tuning["graph"])
         state = {"slots": N, "densified": 0, "pairs": 0, "views": 0, "step": 0, "flags": mode_flags}
         # Per-kernel HIP events need eager launches: with graph replay chosen,
         # the timed run replays the graph and a short eager pass after it
@@ -572,13 +620,18 @@ def main():
             kernel_steps = steps
         events = ws.timings(enable=False)
         views_per_s = steps * per_step / elapsed
-        mean_pairs = state["pairs"] / (steps * per_step)
+        mean_bin_pairs = state["pairs"] / (steps * per_step)
+        # The byte model counts 16x16 tile pairs; with coarser bins the run's
+        # own pair count is lower than that, by the bins' sharing of entries.
+        tile_pairs, probe_bin_pairs = mean_tile_pairs(gs, ws, cams, target_bufs, order, per_step, state, deg, tp)
+        mean_pairs = tile_pairs
         slots_mean = 0.5 * (state["slots_start"] + state["slots"])
         tiles_touched = ws.download(gs.BUF_TILES, np.uint32, state["slots"])
         n_visible = int((tiles_touched > 0).sum())
         results[mode] = {
             "views_per_s": views_per_s, "optimizer_steps_per_s": steps / elapsed, "ms_per_step": 1000 * elapsed / steps,
-            "mean_pairs": mean_pairs, "slots_start": state["slots_start"], "slots_end": state["slots"],
+            "mean_pairs": mean_pairs, "mean_bin_pairs": mean_bin_pairs, "probe_bin_pairs": probe_bin_pairs,
+            "slots_start": state["slots_start"], "slots_end": state["slots"],
             "densify_events": state["densified"], "slots_mean": slots_mean, "n_visible": n_visible,
             "kernels": kernel_table(events, kernel_steps, slots_mean, n_visible, mean_pairs, per_step, W * H, coeffs),
             "tuning": tuning, "tune_ms": tune_ms, "overflow_steps": ws.overflows(),
@@ -599,7 +652,7 @@ def main():
                         args.depth_key == 16)
     if dominant is not None and "GBps" in kernels[dominant]:
         ach = kernels[dominant]["GBps"]
-        traffic = profiled_traffic(dominant) if default_workload else None
+        traffic = profiled_traffic(dominant, r["tuning"]) if default_workload else None
         roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK / 1e9,
                     "unit": "GB/s", "frac": round(ach * 1e9 / HBM_PEAK, 5), "traffic": traffic,
                     "traffic_source": TRAFFIC_PROFILE if traffic is not None else None,
@@ -611,7 +664,7 @@ def main():
     # against 1024 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction.
     valu = {}
     for k in ("backward", "forward"):
-        insts = profiled_valu(k) if default_workload else None
+        insts = profiled_valu(k, r["tuning"]) if default_workload else None
         if insts is not None and k in kernels:
             rate = insts / (kernels[k]["ms_avg"] * 1e-3)
             valu[k] = {"insts_per_launch": insts, "achieved": round(rate / 1e12, 4), "peak": VALU_PEAK / 1e12,
@@ -677,6 +730,7 @@ def main():
                                "view-iterations",
                    "views_per_step": args.batch, "gaussians_start": N, "gaussians_end": r["slots_end"],
                    "width": W, "height": H, "sh_degree": deg, "mean_pairs_per_view": int(r["mean_pairs"]),
+                   "mean_bin_pairs_per_view": int(r["mean_bin_pairs"]),
                    "depth_key_bits": args.depth_key,
                    "densify_events": r["densify_events"], "device": device, "tuning": r["tuning"],
                    "kernel_timing": r["kernel_timing"], "overflow_steps": r["overflow_steps"]},

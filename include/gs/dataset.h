@@ -14,30 +14,42 @@
 
 namespace gs {
 
+// One photo of a capture.  c2w: camera-to-world, row-major, OpenGL axes
+// (x right, y up, looking down -z).  rgba: the photo at the training
+// resolution, alpha the subject mask (255 where there is none).  depth:
+// metres at its own (usually lower) resolution, 0 where unknown, with an
+// optional 0..255 confidence per sample.  holdout: kept for evaluation.
 struct DatasetFrame {
   std::string name;
-  float c2w[16];              // camera-to-world, OpenGL convention, row-major
-  std::vector<uint8_t> rgba;  // training resolution; alpha = subject mask (255 without masks)
+  float c2w[16];
+  std::vector<uint8_t> rgba;
   uint32_t depth_width = 0, depth_height = 0;
-  std::vector<float> depth;                // metres, 0 = missing (low resolution, as phones give it)
-  std::vector<uint8_t> depth_confidence;   // 0..255 per depth sample, or empty
-  bool holdout = false;                    // reserved for evaluation
+  std::vector<float> depth;
+  std::vector<uint8_t> depth_confidence;
+  bool holdout = false;
 };
 
+// A capture at one training resolution with shared pinhole intrinsics.
+// points / point_colors: a sparse cloud (xyz, rgb) when the capture has one.
+// camera_center / camera_extent: the mean camera position and 1.1 times the
+// farthest camera from it (the scene's length scale).  world_up: the
+// capture's up direction when it states one.  skipped: frames whose files
+// could not be read.
 struct Dataset {
   std::string name;
-  uint32_t width = 0, height = 0;  // training resolution
+  uint32_t width = 0, height = 0;
   float fx = 0, fy = 0, cx = 0, cy = 0;
   bool has_masks = false;
   bool has_depth = false;
   std::vector<DatasetFrame> frames;
-  std::vector<float> points;          // optional sparse point cloud, xyz
-  std::vector<uint8_t> point_colors;  // rgb per point
+  std::vector<float> points;
+  std::vector<uint8_t> point_colors;
   float camera_center[3] = {0, 0, 0};
-  float camera_extent = 1.0f;      // 1.1 x the largest camera distance from camera_center
-  float world_up[3] = {0, 1, 0};   // gsplat_lab.world_up when the capture gives it
-  std::vector<std::string> skipped;  // frames left out because their files could not be read
+  float camera_extent = 1.0f;
+  float world_up[3] = {0, 1, 0};
+  std::vector<std::string> skipped;
 
+  // Frames trained on / held out.
   size_t train_count() const;
   size_t holdout_count() const;
 };
@@ -60,18 +72,23 @@ void compute_camera_bounds(Dataset& dataset);
 Dataset make_orbit_dataset(uint32_t views, uint32_t width, uint32_t height, float radius, float height_y,
                            float fov_y_deg, uint32_t holdout_every = 0);
 
+// How the initial gaussians are built: at most max_gaussians of them; with
+// masks and subject_only, depth samples outside the subject are dropped, as
+// are samples whose confidence (scaled to 0..1) is below min_confidence.
 struct InitOptions {
   uint32_t max_gaussians = 150000;
-  bool subject_only = true;     // with masks: keep only depth points inside the subject
-  float min_confidence = 0.3f;  // depth confidence threshold, 0..1
+  bool subject_only = true;
+  float min_confidence = 0.3f;
   float initial_opacity = 0.2f;
   uint32_t seed = 1;
 };
 
+// What initialisation used: "depth", "points" or "random"; the points it
+// started from; the voxel size they were merged at (0: no merging).
 struct InitStats {
-  std::string source;  // "depth", "points" or "random"
+  std::string source;
   size_t raw_points = 0;
-  float voxel_size = 0;  // 0 when no merging was needed
+  float voxel_size = 0;
 };
 
 // Initial gaussians (SH degree 0, RUB) from the training frames' depth maps,

@@ -9,14 +9,17 @@
 
 namespace gs {
 
+// One value; `type` names the member that holds it (the names of the
+// reference's DOM, so code written against it reads these unchanged).
 struct JsonValue {
   enum class Type { Null, Bool, Number, String, Array, Object };
+
   Type type = Type::Null;
-  bool boolean = false;
-  double number = 0.0;
-  std::string string;
+  std::vector<std::pair<std::string, JsonValue>> object;  // members in file order
   std::vector<JsonValue> array;
-  std::vector<std::pair<std::string, JsonValue>> object;
+  std::string string;
+  double number = 0.0;
+  bool boolean = false;
 
   // The member `key` of an object (the last one if repeated), else nullptr.
   const JsonValue* find(const std::string& key) const;

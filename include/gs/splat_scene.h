@@ -10,11 +10,10 @@
 
 namespace gs {
 
-// Axis convention of a scene's coordinates.
-enum class AxisConvention {
-  RDF,  // right, down, forward: COLMAP and the standard 3DGS .ply layout
-  RUB,  // right, up, back: three.js, and the convention SPZ stores internally
-};
+// Which way a scene's x, y, z axes point.  RDF: x right, y down, z forward
+// (COLMAP; what 3DGS .ply files hold).  RUB: x right, y up, z back (three.js;
+// the axes SPZ files are stored in).
+enum class AxisConvention { RDF, RUB };
 
 // Splats in the 3DGS training parameterisation: log scales, opacity logits,
 // unnormalised (w, x, y, z) quaternions, raw SH coefficients.
@@ -23,22 +22,27 @@ struct SplatScene {
   int sh_degree = 0;         // 0..3
   bool antialiased = false;  // SPZ flag: render with the mip-splatting opacity compensation
   AxisConvention convention = AxisConvention::RDF;
-  std::vector<float> positions;       // 3 per splat
-  std::vector<float> log_scales;      // 3 per splat
-  std::vector<float> rotations;       // 4 per splat, (w, x, y, z)
-  std::vector<float> opacity_logits;  // 1 per splat
-  std::vector<float> sh_dc;           // 3 per splat
-  std::vector<float> sh_rest;         // sh_rest_coeffs(sh_degree) * 3 per splat, [coeff][rgb]
+  // Per splat, splat-major: xyz; log sx sy sz; quaternion w x y z; logit;
+  // DC rgb; then the rest coefficients as [coefficient][rgb].
+  std::vector<float> positions;
+  std::vector<float> log_scales;
+  std::vector<float> rotations;
+  std::vector<float> opacity_logits;
+  std::vector<float> sh_dc;
+  std::vector<float> sh_rest;
 
+  // `n` zeroed splats of SH degree `degree`.
   void resize(uint32_t n, int degree);
 };
 
 // SH coefficients beyond DC: 0, 3, 8, 15 for degrees 0..3.
 int sh_rest_coeffs(int degree);
 
+// Where a scene sits: the median position on each axis and the distance
+// from there that 90% of the splats lie within.
 struct SceneBounds {
-  float center[3];  // per-axis median of positions
-  float radius;     // 90th percentile distance from the center
+  float center[3];
+  float radius;
 };
 SceneBounds compute_bounds(const SplatScene& scene);
 

@@ -4,22 +4,24 @@
 //   gs_render <scene> <out.ppm> [width height yaw_deg pitch_deg] [--look x,y,z] [--distance d]
 //       one orbit view of the scene (800x600, yaw 0, pitch 0.25 rad by default)
 //   gs_render <scene> --views <cameras.txt> <out_dir>
-//       one image per line of cameras.txt, written as out_dir/view_00000.ppm, ...
-//       Each line: width height fx fy cx cy, then the 3x4 world-to-camera matrix
-//       (OpenCV convention: x right, y down, z forward) row by row.
+//       renders every camera of cameras.txt to out_dir/view_NNNNN.ppm (N from
+//       00000, in file order).  A camera is a line of 18 numbers: image width and
+//       height, fx fy cx cy in pixels, and the world -> camera transform [R | t]
+//       (camera axes x right, y down, z forward) as three rows of four; blank
+//       lines are skipped.
 //
 // <scene> is a .ply/.spz file or synthetic:N.  The app's places and
 // relighting (--place, --yaw, --relight) are not part of this build.
+#include <algorithm>
 #include <cctype>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <exception>
-#include <fstream>
 #include <map>
 #include <memory>
-#include <sstream>
+#include <stdexcept>
 #include <string>
 #include <utility>
 #include <vector>
@@ -33,6 +35,10 @@ namespace {
 
 constexpr float kClear[3] = {0.08f, 0.09f, 0.11f};
 
+double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
 gs::SplatScene load_scene(const std::string& input) {
   const auto t0 = std::chrono::steady_clock::now();
   gs::SplatScene scene;
@@ -45,37 +51,89 @@ gs::SplatScene load_scene(const std::string& input) {
   }
   std::printf("loaded %u splats, sh degree %d, %s, antialiased %d (%.0f ms)\n", scene.count, scene.sh_degree,
               scene.convention == gs::AxisConvention::RDF ? "RDF" : "RUB", scene.antialiased ? 1 : 0,
-              std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+              ms_since(t0));
   return scene;
 }
 
+// Up to `max` leading numbers of `line` into out[]; returns how many numbers
+// the line starts with, counting one past `max` when there are more.
+int scan_numbers(const std::string& line, double* out, int max) {
+  const char* p = line.c_str();
+  int n = 0;
+  for (char* end = nullptr; n <= max; p = end) {
+    const double x = std::strtod(p, &end);
+    if (end == p) break;
+    if (n < max) out[n] = x;
+    ++n;
+  }
+  return n;
+}
+
+// One cameras.txt row: width height fx fy cx cy, then [R | t] row by row.
+gs::Camera camera_from_row(const double* v) {
+  gs::Camera c{};
+  c.width = uint32_t(v[0]);
+  c.height = uint32_t(v[1]);
+  c.fx = float(v[2]), c.fy = float(v[3]), c.cx = float(v[4]), c.cy = float(v[5]);
+  for (int k = 0; k < 12; ++k) {
+    const int row = k / 4, col = k % 4;
+    (col == 3 ? c.translation[row] : c.rotation[row * 3 + col]) = float(v[6 + k]);
+  }
+  const float* R = c.rotation;
+  const float* t = c.translation;
+  for (int k = 0; k < 3; ++k) c.position[k] = -(R[k] * t[0] + R[3 + k] * t[1] + R[6 + k] * t[2]);  // -R^T t
+  return c;
+}
+
+// The cameras of a cameras.txt file (the format of engine/cli/gs_render.cpp:145-175).
 std::vector<gs::Camera> read_cameras(const std::string& path) {
-  std::ifstream file(path);
-  if (!file) throw std::runtime_error("cannot open " + path);
+  const std::vector<uint8_t> raw = gs::read_file(path);
+  const std::string text(raw.begin(), raw.end());
   std::vector<gs::Camera> cameras;
-  std::string line;
-  while (std::getline(file, line)) {
-    std::istringstream in(line);
-    std::vector<double> v;
-    double x;
-    while (v.size() < 19 && in >> x) v.push_back(x);
-    if (v.empty()) continue;
-    if (v.size() != 18) throw std::runtime_error("camera line needs 18 numbers: " + line);
-    gs::Camera c{};
-    c.width = uint32_t(v[0]);
-    c.height = uint32_t(v[1]);
-    c.fx = float(v[2]), c.fy = float(v[3]), c.cx = float(v[4]), c.cy = float(v[5]);
-    for (int r = 0; r < 3; ++r) {
-      for (int k = 0; k < 3; ++k) c.rotation[r * 3 + k] = float(v[6 + r * 4 + k]);
-      c.translation[r] = float(v[6 + r * 4 + 3]);
-    }
-    for (int k = 0; k < 3; ++k) {  // centre = -R^T t
-      c.position[k] = -(c.rotation[k] * c.translation[0] + c.rotation[3 + k] * c.translation[1] +
-                        c.rotation[6 + k] * c.translation[2]);
-    }
-    cameras.push_back(c);
+  size_t start = 0;
+  while (start < text.size()) {
+    const size_t newline = std::min(text.find('\n', start), text.size());
+    const std::string line = text.substr(start, newline - start);
+    start = newline + 1;
+    double row[18];
+    const int n = scan_numbers(line, row, 18);
+    if (n == 0) continue;
+    if (n != 18) throw std::runtime_error("camera line needs 18 numbers: " + line);
+    cameras.push_back(camera_from_row(row));
   }
   return cameras;
+}
+
+// Every camera to out_dir/view_NNNNN.ppm.  A file goes to the device as it is
+// (SPZ is unpacked there), once per output size; synthetic scenes are built
+// on the host.
+void render_views(const std::string& input, const std::vector<gs::Camera>& cameras, const std::string& out_dir) {
+  const bool synthetic = input.rfind("synthetic", 0) == 0;
+  const gs::SplatScene scene = synthetic ? load_scene(input) : gs::SplatScene{};
+  const std::vector<uint8_t> bytes = synthetic ? std::vector<uint8_t>{} : gs::read_file(input);
+  gs::Device device(0);
+  std::map<std::pair<uint32_t, uint32_t>, std::unique_ptr<gs::GpuScene>> by_size;
+  gs::SplatRenderer renderer(device);
+  const auto t0 = std::chrono::steady_clock::now();
+  size_t done = 0;
+  for (const gs::Camera& c : cameras) {
+    std::unique_ptr<gs::GpuScene>& on_device = by_size[{c.width, c.height}];
+    if (on_device == nullptr) {
+      const auto t_up = std::chrono::steady_clock::now();
+      on_device = synthetic ? gs::upload_scene(device, scene, c.width, c.height)
+                            : gs::upload_splat_file(device, bytes.data(), bytes.size(), c.width, c.height);
+      std::printf("scene on the device for %ux%u: %u splats, sh degree %d (%.0f ms)\n", c.width, c.height,
+                  on_device->count(), on_device->sh_degree(), ms_since(t_up));
+    }
+    renderer.attach(on_device.get());
+    const std::vector<uint8_t> pixels = renderer.render(c, kClear);
+    char name[32];
+    std::snprintf(name, sizeof(name), "/view_%05zu.ppm", done++);
+    gs::write_ppm(out_dir + name, pixels, c.width, c.height);
+    std::printf("\rview %zu/%zu (%u visible)", done, cameras.size(), renderer.visible_count());
+    std::fflush(stdout);
+  }
+  std::printf("\n%zu views in %.1f ms\n", cameras.size(), ms_since(t0));
 }
 
 }  // namespace
@@ -94,43 +152,7 @@ int main(int argc, char** argv) {
     const std::string input = argv[1];
     if (mode == "--views") {
       if (argc < 5) throw std::runtime_error("--views needs <cameras.txt> <out_dir>");
-      const std::vector<gs::Camera> cameras = read_cameras(argv[3]);
-      const std::string out_dir = argv[4];
-      // A file goes to the device as it is (SPZ is unpacked there); synthetic scenes are built on the host.
-      const bool synthetic = input.rfind("synthetic", 0) == 0;
-      gs::SplatScene scene;
-      std::vector<uint8_t> bytes;
-      if (synthetic) {
-        scene = load_scene(input);
-      } else {
-        bytes = gs::read_file(input);
-      }
-      gs::Device device(0);
-      // One device copy of the scene per output size.
-      std::map<std::pair<uint32_t, uint32_t>, std::unique_ptr<gs::GpuScene>> scenes;
-      gs::SplatRenderer renderer(device);
-      const auto t0 = std::chrono::steady_clock::now();
-      for (size_t i = 0; i < cameras.size(); ++i) {
-        const gs::Camera& c = cameras[i];
-        auto& gpu = scenes[{c.width, c.height}];
-        if (!gpu) {
-          const auto tu = std::chrono::steady_clock::now();
-          gpu = synthetic ? gs::upload_scene(device, scene, c.width, c.height)
-                          : gs::upload_splat_file(device, bytes.data(), bytes.size(), c.width, c.height);
-          std::printf("scene on the device for %ux%u: %u splats, sh degree %d (%.0f ms)\n", c.width, c.height,
-                      gpu->count(), gpu->sh_degree(),
-                      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tu).count());
-        }
-        renderer.attach(gpu.get());
-        const std::vector<uint8_t> pixels = renderer.render(c, kClear);
-        char name[32];
-        std::snprintf(name, sizeof(name), "/view_%05zu.ppm", i);
-        gs::write_ppm(out_dir + name, pixels, c.width, c.height);
-        std::printf("\rview %zu/%zu (%u visible)", i + 1, cameras.size(), renderer.visible_count());
-        std::fflush(stdout);
-      }
-      std::printf("\n%zu views in %.1f ms\n", cameras.size(),
-                  std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+      render_views(input, read_cameras(argv[3]), argv[4]);
       return 0;
     }
     gs::SplatScene scene = load_scene(input);
@@ -147,12 +169,12 @@ int main(int argc, char** argv) {
     for (int i = 3; i + 1 < argc; ++i) {  // close-ups
       const std::string a = argv[i];
       if (a == "--look") {
-        float q[3] = {0, 0, 0};
-        std::sscanf(argv[i + 1], "%f,%f,%f", &q[0], &q[1], &q[2]);
-        const float s = orbit.flip_y ? -1.0f : 1.0f;
-        orbit.target[0] = q[0];
-        orbit.target[1] = s * q[1];
-        orbit.target[2] = s * q[2];
+        // A point of the scene; the orbit turns Y-down data half a turn about X.
+        float p[3] = {0, 0, 0};
+        std::sscanf(argv[i + 1], "%f,%f,%f", &p[0], &p[1], &p[2]);
+        const float turn = orbit.flip_y ? -1.0f : 1.0f;
+        const float shown[3] = {p[0], turn * p[1], turn * p[2]};
+        std::copy_n(shown, 3, orbit.target);
       }
       if (a == "--distance") orbit.distance = float(std::atof(argv[i + 1]));
     }
@@ -161,8 +183,7 @@ int main(int argc, char** argv) {
     renderer.attach(gpu.get());
     const auto t0 = std::chrono::steady_clock::now();
     const std::vector<uint8_t> pixels = renderer.render(orbit.camera(width, height), kClear);
-    std::printf("rendered %ux%u: %u visible, %.1f ms on %s\n", width, height, renderer.visible_count(),
-                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(),
+    std::printf("rendered %ux%u: %u visible, %.1f ms on %s\n", width, height, renderer.visible_count(), ms_since(t0),
                 device.name().c_str());
     gs::write_ppm(argv[2], pixels, width, height);
     return 0;

@@ -178,21 +178,17 @@ __global__ void __launch_bounds__(256) k_ssim(const gsh::ViewDev* __restrict__ v
   for (int c = 0; c < 3; ++c) {
     for (int e = threadIdx.x; e < kHalo * kTile; e += 256) {
       const int row = e / kTile, col = e % kTile;
-      float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f, m3 = 0.0f, m4 = 0.0f;
+      // Horizontal pass of the five moments x, y, xx, yy, xy.
+      float sum_x = 0.0f, sum_y = 0.0f, sum_xx = 0.0f, sum_yy = 0.0f, sum_xy = 0.0f;
+      const float* row_x = &s_x[c][row * kStride + col];
+      const float* row_y = &s_y[c][row * kStride + col];
       for (int k = 0; k <= 2 * kSsimRadius; ++k) {
-        const float w = kSsimWindow[k];
-        const float x = s_x[c][row * kStride + col + k], y = s_y[c][row * kStride + col + k];
-        m0 += w * x;
-        m1 += w * y;
-        m2 += w * x * x;
-        m3 += w * y * y;
-        m4 += w * x * y;
+        const float w = kSsimWindow[k], x = row_x[k], y = row_y[k], wx = w * x, wy = w * y;
+        sum_x += w * x, sum_y += w * y;
+        sum_xx += wx * x, sum_yy += wy * y, sum_xy += wx * y;
       }
-      s_h[0][e] = m0;
-      s_h[1][e] = m1;
-      s_h[2][e] = m2;
-      s_h[3][e] = m3;
-      s_h[4][e] = m4;
+      const float sums[5] = {sum_x, sum_y, sum_xx, sum_yy, sum_xy};
+      for (int m = 0; m < 5; ++m) s_h[m][e] = sums[m];
     }
     __syncthreads();
     float mx = 0.0f, my = 0.0f, exx = 0.0f, eyy = 0.0f, exy = 0.0f;

@@ -10,10 +10,12 @@
 namespace gs {
 namespace {
 
+// a x b: component i is a[i+1] b[i+2] - a[i+2] b[i+1], indices mod 3.
 void cross(const double a[3], const double b[3], double out[3]) {
-  out[0] = a[1] * b[2] - a[2] * b[1];
-  out[1] = a[2] * b[0] - a[0] * b[2];
-  out[2] = a[0] * b[1] - a[1] * b[0];
+  for (int i = 0; i < 3; ++i) {
+    const int j = (i + 1) % 3, k = (i + 2) % 3;
+    out[i] = a[j] * b[k] - a[k] * b[j];
+  }
 }
 
 void normalise(double v[3]) {

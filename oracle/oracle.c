@@ -149,33 +149,26 @@ float orc_logf(float x) {
 }
 
 /* ===================== SH basis ============================================ */
-/* Real SH to degree 3 in the 3DGS convention (train_common.glsl:84-109). */
+/* Real SH to degree 3 in the 3DGS convention (train_common.glsl:84-109):
+ * basis k is its normalisation times a polynomial in the direction, band by
+ * band (m = -l..l within degree l). */
 void orc_sh_basis64(const double d[3], int degree, double b[16]) {
+  static const double norm0 = 0.28209479177387814, norm1 = 0.4886025119029199;
+  static const double norm2[5] = {1.0925484305920792, -1.0925484305920792, 0.31539156525252005,
+                                  -1.0925484305920792, 0.5462742152960396};
+  static const double norm3[7] = {-0.5900435899266435, 2.890611442640554, -0.4570457994644658, 0.3731763325901154,
+                                  -0.4570457994644658, 1.445305721320277, -0.5900435899266435};
   const double x = d[0], y = d[1], z = d[2];
-  for (int k = 0; k < 16; ++k) b[k] = 0.0;
-  b[0] = 0.28209479177387814;
-  if (degree >= 1) {
-    b[1] = -0.4886025119029199 * y;
-    b[2] = 0.4886025119029199 * z;
-    b[3] = -0.4886025119029199 * x;
-  }
-  if (degree >= 2) {
-    const double xx = x * x, yy = y * y, zz = z * z;
-    b[4] = 1.0925484305920792 * x * y;
-    b[5] = -1.0925484305920792 * y * z;
-    b[6] = 0.31539156525252005 * (2 * zz - xx - yy);
-    b[7] = -1.0925484305920792 * x * z;
-    b[8] = 0.5462742152960396 * (xx - yy);
-    if (degree >= 3) {
-      b[9] = -0.5900435899266435 * y * (3 * xx - yy);
-      b[10] = 2.890611442640554 * x * y * z;
-      b[11] = -0.4570457994644658 * y * (4 * zz - xx - yy);
-      b[12] = 0.3731763325901154 * z * (2 * zz - 3 * xx - 3 * yy);
-      b[13] = -0.4570457994644658 * x * (4 * zz - xx - yy);
-      b[14] = 1.445305721320277 * z * (xx - yy);
-      b[15] = -0.5900435899266435 * x * (xx - 3 * yy);
-    }
-  }
+  const double xx = x * x, yy = y * y, zz = z * z;
+  const double band1[3] = {-y, z, -x};
+  const double band2[5] = {x * y, y * z, 2 * zz - xx - yy, x * z, xx - yy};
+  const double band3[7] = {y * (3 * xx - yy),     x * y * z,         y * (4 * zz - xx - yy), z * (2 * zz - 3 * xx - 3 * yy),
+                           x * (4 * zz - xx - yy), z * (xx - yy), x * (xx - 3 * yy)};
+  memset(b, 0, sizeof(double) * 16);
+  b[0] = norm0;
+  for (int k = 0; degree >= 1 && k < 3; ++k) b[1 + k] = norm1 * band1[k];
+  for (int k = 0; degree >= 2 && k < 5; ++k) b[4 + k] = norm2[k] * band2[k];
+  for (int k = 0; degree >= 3 && k < 7; ++k) b[9 + k] = norm3[k] * band3[k];
 }
 
 static void sh_basis32(float x, float y, float z, uint32_t degree, float b[16]) {
@@ -922,18 +915,16 @@ typedef struct {
   double faint; /* ln(1 / (255 opacity)): below this power alpha < 1/255 */
 } ref_t;
 
+/* Rotation of the normalised quaternion (w, x, y, z), row-major. */
 static void quat_matrix64(const double q[4], double R[3][3]) {
-  const double n = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
-  const double w = q[0] / n, x = q[1] / n, y = q[2] / n, z = q[3] / n;
-  R[0][0] = 1 - 2 * (y * y + z * z);
-  R[0][1] = 2 * (x * y - w * z);
-  R[0][2] = 2 * (x * z + w * y);
-  R[1][0] = 2 * (x * y + w * z);
-  R[1][1] = 1 - 2 * (x * x + z * z);
-  R[1][2] = 2 * (y * z - w * x);
-  R[2][0] = 2 * (x * z - w * y);
-  R[2][1] = 2 * (y * z + w * x);
-  R[2][2] = 1 - 2 * (x * x + y * y);
+  const double inv = 1.0 / sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+  const double w = q[0] * inv, x = q[1] * inv, y = q[2] * inv, z = q[3] * inv;
+  const double xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z;
+  const double wx = w * x, wy = w * y, wz = w * z;
+  const double rows[3][3] = {{1 - 2 * (yy + zz), 2 * (xy - wz), 2 * (xz + wy)},
+                             {2 * (xy + wz), 1 - 2 * (xx + zz), 2 * (yz - wx)},
+                             {2 * (xz - wy), 2 * (yz + wx), 1 - 2 * (xx + yy)}};
+  memcpy(R, rows, sizeof(rows));
 }
 
 /* p: 16 params; sh: coeffs*3 ([coeff][rgb]).  train_selftest.cpp:79-150. */
@@ -976,18 +967,15 @@ static ref_t project64(const orc_camera* cam, const double* p, const double* sh,
   const double A = cov[0][0] + DILATION, B = cov[0][1], C = cov[1][1] + DILATION;
   const double det = A * C - B * B;
   if (det <= 0) return r;
+  const double inv_z = 1.0 / z, inv_det = 1.0 / det;
   r.visible = 1;
-  r.faint = -log(255.0 * opacity);
-  r.mx = fx * t[0] / z + cx;
-  r.my = fy * t[1] / z + cy;
   r.depth = z;
+  r.mx = cx + fx * t[0] * inv_z;
+  r.my = cy + fy * t[1] * inv_z;
+  r.cov[0] = A, r.cov[1] = B, r.cov[2] = C;
+  r.con[0] = C * inv_det, r.con[1] = -B * inv_det, r.con[2] = A * inv_det;
   r.opacity = opacity;
-  r.con[0] = C / det;
-  r.con[1] = -B / det;
-  r.con[2] = A / det;
-  r.cov[0] = A;
-  r.cov[1] = B;
-  r.cov[2] = C;
+  r.faint = -log(255.0 * opacity);
   double dir[3], len = 0;
   for (int k = 0; k < 3; ++k) {
     dir[k] = p[k] - cam->position[k];

@@ -6,10 +6,12 @@
 // (v2 and v3, SH degrees 0-4) against the host decoder, value for value and
 // image for image.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <random>
 #include <stdexcept>
 #include <vector>
@@ -27,36 +29,29 @@ namespace {
 
 constexpr float kShC0 = 0.28209479177387814f;
 
-struct TestSplat {
-  float pos[3];
-  float scale;
-  float rgb[3];
-  float opacity;
-};
-
-gs::SplatScene splats_scene(const std::vector<TestSplat>& splats) {
+// A scene of round splats (one scale on every axis, no rotation) in Y-up
+// axes; each is (x, y, z, size, r, g, b, opacity).
+gs::SplatScene round_splats(std::initializer_list<std::array<float, 8>> splats) {
   gs::SplatScene s;
   s.convention = gs::AxisConvention::RUB;
   s.resize(uint32_t(splats.size()), 0);
-  for (size_t i = 0; i < splats.size(); ++i) {
-    const TestSplat& t = splats[i];
-    for (int k = 0; k < 3; ++k) {
-      s.positions[i * 3 + k] = t.pos[k];
-      s.log_scales[i * 3 + k] = std::log(t.scale);
-      s.sh_dc[i * 3 + k] = (t.rgb[k] - 0.5f) / kShC0;
-    }
-    s.rotations[i * 4] = 1.0f;
-    s.opacity_logits[i] = std::log(t.opacity / (1.0f - t.opacity));
+  uint32_t i = 0;
+  for (const auto& sp : splats) {
+    std::copy(sp.begin(), sp.begin() + 3, s.positions.begin() + 3 * i);
+    std::fill(s.log_scales.begin() + 3 * i, s.log_scales.begin() + 3 * i + 3, std::log(sp[3]));
+    for (int c = 0; c < 3; ++c) s.sh_dc[3 * i + c] = (sp[4 + c] - 0.5f) / kShC0;  // colour = 0.5 + C0 dc
+    s.rotations[4 * i] = 1.0f;
+    s.opacity_logits[i] = -std::log(1.0f / sp[7] - 1.0f);
+    ++i;
   }
   return s;
 }
 
-gs::OrbitCamera front_camera() {
-  gs::OrbitCamera orbit;
-  orbit.yaw = 0.0f;
-  orbit.pitch = 0.0f;
-  orbit.distance = 3.0f;
-  return orbit;
+// Looking at the origin along -z from `distance` (the orbit's start angles).
+gs::OrbitCamera facing_origin(float distance) {
+  gs::OrbitCamera cam;
+  cam.distance = distance;
+  return cam;
 }
 
 std::vector<uint8_t> render(gs::Device& dev, const gs::SplatScene& scene, const gs::OrbitCamera& orbit, uint32_t w,
@@ -74,41 +69,42 @@ const uint8_t* pixel(const std::vector<uint8_t>& img, uint32_t w, uint32_t x, ui
   return &img[(size_t(y) * w + x) * 4];
 }
 
+// One round red splat of world size 0.1 at the origin, 3 units away: along a
+// diagonal from the centre, red follows opacity exp(-d^2 / 2 (sigma^2 + 0.3))
+// with sigma its projected size in pixels (within 6 of 255); green and blue
+// stay near zero; far corners keep the black clear colour.
 void test_single_splat(gs::Device& dev) {
-  const uint32_t w = 64, h = 64;
-  const float opacity = 0.95f;
+  const uint32_t n = 64;
+  const float o = 0.95f;
   uint32_t visible = 0;
-  const auto img = render(dev, splats_scene({{{0, 0, 0}, 0.1f, {1, 0, 0}, opacity}}), front_camera(), w, h, &visible);
-  EXPECT(visible == 1, "single splat: visible %u", visible);
-  const float focal = 0.5f * 64.0f / std::tan(25.0f * 3.14159265f / 180.0f);
-  const float sigma = focal * 0.1f / 3.0f;
-  const float var = sigma * sigma + 0.3f;
-  for (int dx : {0, 2, 4}) {
-    const float px = dx + 0.5f, py = 0.5f;
-    const float alpha = std::min(opacity * std::exp(-0.5f * (px * px + py * py) / var), 0.99f);
-    const uint8_t* p = pixel(img, w, 32 + dx, 32);
-    EXPECT(std::fabs(p[0] - 255.0f * alpha) <= 6.0f, "falloff at dx=%d: red %u, expected %.1f", dx, p[0],
-           255.0f * alpha);
-    EXPECT(p[1] <= 2 && p[2] <= 2, "single splat should be pure red (g %u b %u)", p[1], p[2]);
+  const auto img = render(dev, round_splats({{0, 0, 0, 0.1f, 1, 0, 0, o}}), facing_origin(3.0f), n, n, &visible);
+  EXPECT(visible == 1, "one splat drawn, not %u", visible);
+  // The orbit camera's 50 degree field of view spans the short side.
+  const float f = 32.0f / std::tan(0.5f * 50.0f * 3.14159265f / 180.0f);
+  const float var = (f * 0.1f / 3.0f) * (f * 0.1f / 3.0f) + 0.3f;
+  for (int k = 0; k <= 3; ++k) {
+    const float d2 = 2.0f * (k + 0.5f) * (k + 0.5f);  // pixel (32 + k, 32 + k), centre offsets k + 0.5
+    const float want = 255.0f * std::min(o * std::exp(-0.5f * d2 / var), 0.99f);
+    const uint8_t* p = pixel(img, n, 32 + k, 32 + k);
+    EXPECT(std::fabs(float(p[0]) - want) <= 6.0f, "diagonal step %d: red %u, closed form %.1f", k, p[0], want);
+    EXPECT(int(p[1]) + int(p[2]) <= 4, "diagonal step %d: green %u / blue %u in a red splat", k, p[1], p[2]);
   }
-  const uint8_t* corner = pixel(img, w, 2, 2);
-  EXPECT(corner[0] == 0 && corner[1] == 0 && corner[2] == 0, "background should stay black");
-  std::printf("single splat: centre red %u\n", pixel(img, w, 32, 32)[0]);
+  for (uint32_t c : {1u, 62u}) {
+    const uint8_t* p = pixel(img, n, c, c);
+    EXPECT(int(p[0]) + int(p[1]) + int(p[2]) == 0, "corner (%u, %u) is not the clear colour", c, c);
+  }
+  std::printf("single splat: centre red %u\n", pixel(img, n, 32, 32)[0]);
 }
 
+// Two opaque splats on the view axis, red and blue: the nearer one (larger z:
+// the camera looks down -z from z = 3) owns the centre pixel.
 void test_depth_order(gs::Device& dev) {
-  for (bool red_in_front : {true, false}) {
-    const float red_z = red_in_front ? 0.5f : -0.5f;  // the camera sits at +z: larger z is nearer
-    const auto img = render(dev,
-                            splats_scene({{{0, 0, red_z}, 0.1f, {1, 0, 0}, 0.9f},
-                                          {{0, 0, -red_z}, 0.1f, {0, 0, 1}, 0.9f}}),
-                            front_camera(), 64, 64);
+  for (float red_z : {0.5f, -0.5f}) {
+    const auto img = render(dev, round_splats({{0, 0, red_z, 0.1f, 1, 0, 0, 0.9f}, {0, 0, -red_z, 0.1f, 0, 0, 1, 0.9f}}),
+                            facing_origin(3.0f), 64, 64);
     const uint8_t* p = pixel(img, 64, 32, 32);
-    if (red_in_front) {
-      EXPECT(p[0] > 180 && p[2] < 60, "red in front: got r %u b %u", p[0], p[2]);
-    } else {
-      EXPECT(p[2] > 180 && p[0] < 60, "blue in front: got r %u b %u", p[0], p[2]);
-    }
+    const int near = red_z > 0.0f ? p[0] : p[2], far = red_z > 0.0f ? p[2] : p[0];
+    EXPECT(near > 180 && far < 60, "%s splat nearer: centre r %u b %u", red_z > 0.0f ? "red" : "blue", p[0], p[2]);
   }
 }
 
@@ -176,13 +172,13 @@ void test_synthetic_render(gs::Device& dev) {
 }
 
 void test_size_check(gs::Device& dev) {
-  auto gpu = gs::upload_scene(dev, splats_scene({{{0, 0, 0}, 0.1f, {1, 0, 0}, 0.5f}}), 64, 64);
+  auto gpu = gs::upload_scene(dev, round_splats({{0, 0, 0, 0.1f, 1, 0, 0, 0.5f}}), 64, 64);
   gs::SplatRenderer r(dev);
   r.attach(gpu.get());
   const float clear[3] = {0, 0, 0};
   bool threw = false;
   try {
-    r.render(front_camera().camera(32, 32), clear);
+    r.render(facing_origin(3.0f).camera(32, 32), clear);
   } catch (const std::invalid_argument&) {
     threw = true;
   }

@@ -8,6 +8,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <initializer_list>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -66,31 +67,37 @@ void test_ply_roundtrip() {
   EXPECT(threw, "ascii PLY must be rejected");
 }
 
+// An SPZ payload before compression: the 16-byte header (magic "NGSP",
+// version, count, SH degree, 12 fractional position bits, flags, reserved),
+// then whatever the test appends, little-endian.
+struct SpzPayload {
+  std::vector<uint8_t> bytes;
+  SpzPayload(uint32_t version, uint32_t count, int degree) {
+    put(0x5053474e, 4);
+    put(version, 4);
+    put(count, 4);
+    put(uint32_t(degree) | 12u << 8, 4);
+  }
+  void put(uint32_t v, int width) {
+    for (int i = 0; i < width; ++i) bytes.push_back(uint8_t(v >> (8 * i)));
+  }
+  void put_each(std::initializer_list<int> values) {
+    for (int v : values) bytes.push_back(uint8_t(v));
+  }
+};
+
 void test_spz_v2() {
-  std::vector<uint8_t> raw;
-  auto put32 = [&](uint32_t v) {
-    for (int i = 0; i < 4; ++i) raw.push_back(uint8_t(v >> (8 * i)));
-  };
-  put32(0x5053474e);
-  put32(2);
-  put32(2);
-  raw.push_back(1);   // SH degree
-  raw.push_back(12);  // fractional bits
-  raw.push_back(0);
-  raw.push_back(0);
+  SpzPayload spz(2, 2, 1);
   const float pos[2][3] = {{1.5f, -2.25f, 0.125f}, {-0.5f, 0.0f, 3.0f}};
-  for (const auto& p : pos)
-    for (float v : p) {
-      const int32_t fixed = int32_t(std::lround(v * 4096.0f));
-      for (int i = 0; i < 3; ++i) raw.push_back(uint8_t(fixed >> (8 * i)));
-    }
-  raw.push_back(255);
-  raw.push_back(128);
-  for (int i = 0; i < 6; ++i) raw.push_back(uint8_t(64 + i * 20));
-  for (int i = 0; i < 6; ++i) raw.push_back(uint8_t(100 + i));
-  for (uint8_t b : {128, 128, 128, 255, 128, 128}) raw.push_back(b);
-  for (int i = 0; i < 18; ++i) raw.push_back(uint8_t(128 + i));
-  const std::vector<uint8_t> file = gzip(raw);
+  for (const auto& p : pos) {
+    for (float v : p) spz.put(uint32_t(int32_t(std::lround(v * 4096.0f))), 3);  // 24-bit fixed point
+  }
+  spz.put_each({255, 128});                             // alphas
+  spz.put_each({64, 84, 104, 124, 144, 164});           // colours
+  spz.put_each({100, 101, 102, 103, 104, 105});         // log scales
+  spz.put_each({128, 128, 128, 255, 128, 128});         // quaternions (x y z): identity, then a half turn about x
+  for (int i = 0; i < 18; ++i) spz.put(128 + i, 1);     // SH rest, 9 per gaussian
+  const std::vector<uint8_t> file = gzip(spz.bytes);
   const gs::SplatScene s = gs::load_spz(file.data(), file.size());
   EXPECT(s.count == 2 && s.sh_degree == 1, "spz header %u %d", s.count, s.sh_degree);
   EXPECT(s.convention == gs::AxisConvention::RUB, "spz is RUB");
@@ -103,43 +110,29 @@ void test_spz_v2() {
   EXPECT(std::fabs(s.rotations[0] - 1.0f) < 0.01f, "identity rotation w %f", s.rotations[0]);
   EXPECT(std::fabs(s.rotations[5] - 1.0f) < 0.01f, "x rotation x %f", s.rotations[5]);
   EXPECT(std::fabs(s.sh_rest[5] - 5.0f / 128.0f) < 1e-6f, "spz SH layout");
-  bool threw = false;
+  // A version 4 header, uncompressed, is refused by name.
+  const SpzPayload v4(4, 0, 0);
+  std::string message;
   try {
-    const uint8_t v4[8] = {0x4e, 0x47, 0x53, 0x50, 4, 0, 0, 0};
-    gs::load_splat_file(v4, sizeof(v4));
+    gs::load_splat_file(v4.bytes.data(), 8);
   } catch (const std::runtime_error& e) {
-    threw = std::string(e.what()).find("v4") != std::string::npos;
+    message = e.what();
   }
-  EXPECT(threw, "spz v4 rejected by name");
+  EXPECT(message.find("v4") != std::string::npos, "spz v4 rejected by name (\"%s\")", message.c_str());
 }
 
+// Smallest-three packing of (w, x, y, z) = (0.8, 0.6, 0, 0): the largest
+// component, w, is index 3 of (x, y, z, w) in the top two bits; the other
+// three follow as 9-bit magnitudes over 1/sqrt(2) with a sign bit, 10 bits
+// each, the last (z) in the low bits.
 void test_spz_v3_quaternion() {
-  // Smallest-three packing of (w, x, y, z) = (0.8, 0.6, 0, 0): largest is w (index 3 in xyzw).
-  std::vector<uint8_t> raw;
-  auto put32 = [&](uint32_t v) {
-    for (int i = 0; i < 4; ++i) raw.push_back(uint8_t(v >> (8 * i)));
-  };
-  put32(0x5053474e);
-  put32(3);
-  put32(1);
-  raw.push_back(0);
-  raw.push_back(12);
-  raw.push_back(0);
-  raw.push_back(0);
-  for (int i = 0; i < 9; ++i) raw.push_back(0);
-  raw.push_back(200);
-  for (int i = 0; i < 3; ++i) raw.push_back(128);
-  for (int i = 0; i < 3; ++i) raw.push_back(100);
-  // components x (i=0), y (1), z (2) packed from the last down: z first in the low bits.
-  const uint32_t mask = 511;
-  auto mag = [&](float v) { return uint32_t(std::lround(std::fabs(v) / 0.70710678f * mask)); };
-  uint32_t bits = 0;
-  bits |= mag(0.0f);                 // z
-  bits |= mag(0.0f) << 10;           // y
-  bits |= mag(0.6f) << 20;           // x
-  bits |= 3u << 30;                  // largest: w
-  put32(bits);
-  const std::vector<uint8_t> file = gzip(raw);
+  SpzPayload spz(3, 1, 0);
+  for (int i = 0; i < 9; ++i) spz.put(0, 1);  // position at the origin
+  spz.put_each({200, 128, 128, 128, 100, 100, 100});  // alpha, colour, log scales
+  const auto magnitude = [](float v) { return uint32_t(std::lround(std::fabs(v) / 0.70710678f * 511.0f)); };
+  const uint32_t packed = 3u << 30 | magnitude(0.6f) << 20 | magnitude(0.0f) << 10 | magnitude(0.0f);
+  spz.put(packed, 4);
+  const std::vector<uint8_t> file = gzip(spz.bytes);
   const gs::SplatScene s = gs::load_spz(file.data(), file.size());
   EXPECT(std::fabs(s.rotations[0] - 0.8f) < 2e-3f && std::fabs(s.rotations[1] - 0.6f) < 2e-3f,
          "v3 quaternion (%f %f %f %f)", s.rotations[0], s.rotations[1], s.rotations[2], s.rotations[3]);
