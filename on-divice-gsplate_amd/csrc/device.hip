@@ -293,6 +293,9 @@ ViewDev make_view_dev(const View& v, const gs_hip_frame& f, uint32_t* totals, fl
   d.bin_shift = bin_shift;
   d.bins_x = bins_along(f.cam.width, bin_shift);
   d.bin_count = d.bins_x * bins_along(f.cam.height, bin_shift);
+  const float W = float(f.cam.width), H = float(f.cam.height);
+  d.lim_x = 1.3f * std::max(f.cam.cx, W - f.cam.cx) / f.cam.fx;
+  d.lim_y = 1.3f * std::max(f.cam.cy, H - f.cam.cy) / f.cam.fy;
   return d;
 }
 

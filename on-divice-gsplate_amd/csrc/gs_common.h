@@ -217,29 +217,44 @@ __device__ __forceinline__ void footprint_rows(float my, float c, float t, uint3
   footprint_pixels(v - ey, v + ey, height, first, last);
 }
 
+// The terms of an ellipse's row spans that do not depend on the row: centre
+// (u, v) in pixel-index coordinates, k = b/c, w = det/c, the height of the
+// rightmost point b sqrt(t/a), and c, t.
+struct RowSpan {
+  float u, v, k, w, side, c, t;
+};
+
+__device__ __forceinline__ RowSpan row_span(float mx, float my, float a, float b, float c, float t) {
+#pragma clang fp contract(off)
+  RowSpan s;
+  s.u = mx - 0.5f;
+  s.v = my - 0.5f;
+  s.k = b / c;
+  s.w = (a * c - b * b) / c;
+  s.side = b * sqrtf(t / a);
+  s.c = c;
+  s.t = t;
+  return s;
+}
+
 // Bins [x0, x1) of bin row `ty` (bins of 2^side_log2 pixels: 16 for the
 // raster tiles) that the ellipse reaches within pixel rows [row_first,
 // row_last].  At height dy the ellipse spans
 // dx = (b/c) dy -+ sqrt((det/c)(t - dy^2/c)); over [y0, y1] the right end is
 // largest at the clamped height of the rightmost point, b sqrt(t/a), and the
 // left end smallest at that of the leftmost, -b sqrt(t/a).
-__device__ __forceinline__ void footprint_row_tiles(float mx, float my, float a, float b, float c, float t,
-                                                    int row_first, int row_last, uint32_t ty, uint32_t width,
-                                                    uint32_t& x0, uint32_t& x1, int side_log2 = 4) {
+__device__ __forceinline__ void span_row_tiles(const RowSpan& s, int row_first, int row_last, uint32_t ty,
+                                               uint32_t width, uint32_t& x0, uint32_t& x1, int side_log2 = 4) {
 #pragma clang fp contract(off)
-  const float u = mx - 0.5f, v = my - 0.5f;
   const int top = int(ty) << side_log2;
-  const float y0 = float(max(top, row_first)) - v;
-  const float y1 = float(min(top + (1 << side_log2) - 1, row_last)) - v;
-  const float k = b / c;
-  const float w = (a * c - b * b) / c;
-  const float side = b * sqrtf(t / a);
-  const float dr = fminf(fmaxf(side, y0), y1);
-  const float dl = fminf(fmaxf(-side, y0), y1);
-  const float x_max = k * dr + sqrtf(fmaxf(w * (t - dr * dr / c), 0.0f));
-  const float x_min = k * dl - sqrtf(fmaxf(w * (t - dl * dl / c), 0.0f));
+  const float y0 = float(max(top, row_first)) - s.v;
+  const float y1 = float(min(top + (1 << side_log2) - 1, row_last)) - s.v;
+  const float dr = fminf(fmaxf(s.side, y0), y1);
+  const float dl = fminf(fmaxf(-s.side, y0), y1);
+  const float x_max = s.k * dr + sqrtf(fmaxf(s.w * (s.t - dr * dr / s.c), 0.0f));
+  const float x_min = s.k * dl - sqrtf(fmaxf(s.w * (s.t - dl * dl / s.c), 0.0f));
   int first, last;
-  footprint_pixels(u + x_min, u + x_max, width, first, last);
+  footprint_pixels(s.u + x_min, s.u + x_max, width, first, last);
   if (first > last) {
     x0 = x1 = 0u;
     return;

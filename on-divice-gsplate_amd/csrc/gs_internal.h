@@ -134,6 +134,9 @@ struct ViewDev {
   uint32_t bin_shift;
   uint32_t bins_x;
   uint32_t bin_count;  // bins_x * bins_y
+  // The projection's ray clamp, 1.3 max(cx, W - cx) / fx and likewise in y,
+  // in IEEE single precision like the device expression it replaces.
+  float lim_x, lim_y;
 };
 
 // The bin raster tile `tile` walks.

@@ -166,17 +166,17 @@ __device__ __forceinline__ void sh_rest_dbasis(float x, float y, float z, float&
 // the SH rest) comes from k_adam_sh.  kPose: also the slot's share of the
 // camera pose gradient, pose[6] = dL/dd, dL/dw (train_step.glsl:196-201).
 template <bool kPose = false>
-__device__ __forceinline__ void adjoint_view(const gs_hip_camera& cam, const float g[9], uint32_t clamped, float4 p0,
+__device__ __forceinline__ void adjoint_view(const gsh::ViewDev& vd, const float g[9], uint32_t clamped, float4 p0,
                                              float4 p1, float4 q, float4& dp0, float4& dp1, float4& dq, float4& dp3,
                                              float* pose = nullptr) {
-  const float W = float(cam.width), H = float(cam.height);
+  const gs_hip_camera& cam = vd.f.cam;
   const float* V = cam.rotation;
   const float tx = V[0] * p0.x + V[1] * p0.y + V[2] * p0.z + cam.translation[0];
   const float ty = V[3] * p0.x + V[4] * p0.y + V[5] * p0.z + cam.translation[1];
   const float tz = V[6] * p0.x + V[7] * p0.y + V[8] * p0.z + cam.translation[2];
-  const float fx = cam.fx, fy = cam.fy, cx = cam.cx, cy = cam.cy;
+  const float fx = cam.fx, fy = cam.fy;
   const float inv_z = 1.0f / tz;
-  const float lim_x = 1.3f * fmaxf(cx, W - cx) / fx, lim_y = 1.3f * fmaxf(cy, H - cy) / fy;
+  const float lim_x = vd.lim_x, lim_y = vd.lim_y;
   const float rx = tx * inv_z, ry = ty * inv_z;
   const bool clamp_x = rx < -lim_x || rx > lim_x, clamp_y = ry < -lim_y || ry > lim_y;
   const float ux = fminf(fmaxf(rx, -lim_x), lim_x), uy = fminf(fmaxf(ry, -lim_y), lim_y);
@@ -334,7 +334,7 @@ __global__ void __launch_bounds__(256) k_adam(const gsh::SceneDev* __restrict__ 
     st.x += sqrtf(gx * gx + gy * gy);
     st.y += 1.0f;
     st.z = fmaxf(st.z, float(misc >> 3));
-    adjoint_view(view.f.cam, g, misc & 7u, p0, p1, q, dp0, dp1, dq, dp3);
+    adjoint_view(view, g, misc & 7u, p0, p1, q, dp0, dp1, dq, dp3);
   }
   if (seen) gstore(a.stats + i, st);
   if (!drew && (f.flags & GS_HIP_FLAG_SPARSE_ADAM) != 0u) return;
@@ -385,7 +385,7 @@ __global__ void __launch_bounds__(256) k_pose_grad(const gsh::SceneDev* __restri
       float g[9];
       for (int c = 0; c < 9; ++c) g[c] = gmem(view.grad2d)[size_t(i) * 9u + c];
       float4 dp0 = make_float4(0, 0, 0, 0), dp1 = dp0, dq = dp0, dp3 = dp0;
-      adjoint_view<true>(view.f.cam, g, 0u, p0, p1, q, dp0, dp1, dq, dp3, pose);
+      adjoint_view<true>(view, g, 0u, p0, p1, q, dp0, dp1, dq, dp3, pose);
     }
     for (int k = 0; k < 6; ++k) {
       const float w = wave_sum(pose[k]);

@@ -26,12 +26,40 @@ struct ProjectArgs {
   uint32_t count;      // views
 };
 
+// What a slot's projection needs that no view changes: opacity, its
+// footprint threshold, and M = R S (the normalised quaternion's rotation,
+// columns scaled by exp(log scale)); computed once per step, not per view.
+struct SlotShape {
+  float opacity, thr;
+  float m00, m01, m02, m10, m11, m12, m20, m21, m22;
+};
+
+__device__ __forceinline__ SlotShape slot_shape(float4 p0, float4 p1, float4 q) {
+#pragma clang fp contract(off)
+  SlotShape s;
+  s.opacity = gs_sigmoid(p0.w);
+  s.thr = footprint_threshold(s.opacity);
+  const float qlen = fmaxf(sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w), 1e-12f);
+  const float w = q.x / qlen, x = q.y / qlen, y = q.z / qlen, z = q.w / qlen;
+  const float sx = gs_expf(p1.x), sy = gs_expf(p1.y), sz = gs_expf(p1.z);
+  s.m00 = (1.0f - 2.0f * (y * y + z * z)) * sx;
+  s.m01 = (2.0f * (x * y - w * z)) * sy;
+  s.m02 = (2.0f * (x * z + w * y)) * sz;
+  s.m10 = (2.0f * (x * y + w * z)) * sx;
+  s.m11 = (1.0f - 2.0f * (x * x + z * z)) * sy;
+  s.m12 = (2.0f * (y * z - w * x)) * sz;
+  s.m20 = (2.0f * (x * z - w * y)) * sx;
+  s.m21 = (2.0f * (y * z + w * x)) * sy;
+  s.m22 = (1.0f - 2.0f * (x * x + y * y)) * sz;
+  return s;
+}
+
 // One view's projection of slot i (the arithmetic of the single-view
 // kernel, term for term, with contraction off: records, tile counts and keys
 // are bit-exact with the oracle's fp32 restatement).
 __device__ __forceinline__ void project_view(const gsh::ViewDev& vd, const ProjectArgs& a, uint32_t i, float4 p0,
-                                             float4 p1, float4 q, float4 p3,
-                                             const float (&rest)[GS_HIP_MAX_SH_REST][3]) {
+                                             float4 p1, const SlotShape& sh, float4 p3,
+                                             const float (&rest)[GS_HIP_MAX_SH_REST][3], float log_depth_range) {
 #pragma clang fp contract(off)
   const gs_hip_frame& f = vd.f;
   for (int c = 0; c < GS_HIP_GRAD2D_FLOATS; ++c) gmem(vd.grad2d)[size_t(i) * GS_HIP_GRAD2D_FLOATS + c] = 0.0f;
@@ -40,7 +68,7 @@ __device__ __forceinline__ void project_view(const gsh::ViewDev& vd, const Proje
   const float tx = R[0] * p0.x + R[1] * p0.y + R[2] * p0.z + f.cam.translation[0];
   const float ty = R[3] * p0.x + R[4] * p0.y + R[5] * p0.z + f.cam.translation[1];
   const float tz = R[6] * p0.x + R[7] * p0.y + R[8] * p0.z + f.cam.translation[2];
-  const float opacity = gs_sigmoid(p0.w);
+  const float opacity = sh.opacity;
 
   uint32_t tiles = 0u;
   Record rec;
@@ -48,13 +76,11 @@ __device__ __forceinline__ void project_view(const gsh::ViewDev& vd, const Proje
   float4 foot = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if (p1.w > 0.5f && tz > kNear && opacity >= kMinAlpha) {
     const float fx = f.cam.fx, fy = f.cam.fy, cx = f.cam.cx, cy = f.cam.cy;
-    const float W = float(f.cam.width), H = float(f.cam.height);
     const float inv_z = 1.0f / tz;
     const float mx = fx * tx * inv_z + cx;
     const float my = fy * ty * inv_z + cy;
     // Jacobian of the perspective map, with the ray clamped to 1.3x the frustum.
-    const float lim_x = 1.3f * fmaxf(cx, W - cx) / fx;
-    const float lim_y = 1.3f * fmaxf(cy, H - cy) / fy;
+    const float lim_x = vd.lim_x, lim_y = vd.lim_y;
     const float rx = fminf(fmaxf(tx * inv_z, -lim_x), lim_x);
     const float ry = fminf(fmaxf(ty * inv_z, -lim_y), lim_y);
     const float j00 = fx * inv_z, j02 = -fx * rx * inv_z;
@@ -62,16 +88,8 @@ __device__ __forceinline__ void project_view(const gsh::ViewDev& vd, const Proje
     // T = J W (2x3), W the world -> camera rotation.
     const float t00 = j00 * R[0] + j02 * R[6], t01 = j00 * R[1] + j02 * R[7], t02 = j00 * R[2] + j02 * R[8];
     const float t10 = j11 * R[3] + j12 * R[6], t11 = j11 * R[4] + j12 * R[7], t12 = j11 * R[5] + j12 * R[8];
-    // Rotation of the normalised quaternion (w, x, y, z), columns scaled by exp(log scale): M = R S.
-    const float qlen = fmaxf(sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w), 1e-12f);
-    const float w = q.x / qlen, x = q.y / qlen, y = q.z / qlen, z = q.w / qlen;
-    const float sx = gs_expf(p1.x), sy = gs_expf(p1.y), sz = gs_expf(p1.z);
-    const float m00 = (1.0f - 2.0f * (y * y + z * z)) * sx, m01 = (2.0f * (x * y - w * z)) * sy,
-                m02 = (2.0f * (x * z + w * y)) * sz;
-    const float m10 = (2.0f * (x * y + w * z)) * sx, m11 = (1.0f - 2.0f * (x * x + z * z)) * sy,
-                m12 = (2.0f * (y * z - w * x)) * sz;
-    const float m20 = (2.0f * (x * z - w * y)) * sx, m21 = (2.0f * (y * z + w * x)) * sy,
-                m22 = (1.0f - 2.0f * (x * x + y * y)) * sz;
+    const float m00 = sh.m00, m01 = sh.m01, m02 = sh.m02, m10 = sh.m10, m11 = sh.m11, m12 = sh.m12;
+    const float m20 = sh.m20, m21 = sh.m21, m22 = sh.m22;
     // V = T M; cov2D = V V^T (= T M M^T T^T), dilated by 0.3 px^2.
     const float v00 = t00 * m00 + t01 * m10 + t02 * m20, v01 = t00 * m01 + t01 * m11 + t02 * m21,
                 v02 = t00 * m02 + t01 * m12 + t02 * m22;
@@ -85,14 +103,15 @@ __device__ __forceinline__ void project_view(const gsh::ViewDev& vd, const Proje
       const float mid = 0.5f * (ca + cc);
       const float lambda = mid + sqrtf(fmaxf(0.1f, mid * mid - det));
       const float radius = ceilf(3.0f * sqrtf(lambda));
-      const float thr = footprint_threshold(opacity);
+      const float thr = sh.thr;
       int row_first, row_last;
       footprint_rows(my, cc, thr, f.cam.height, row_first, row_last);
       if (row_first <= row_last) {
         const int side_log2 = 4 + int(vd.bin_shift);
+        const RowSpan span = row_span(mx, my, ca, cb, cc, thr);
         for (uint32_t tyi = uint32_t(row_first) >> side_log2; tyi <= (uint32_t(row_last) >> side_log2); ++tyi) {
           uint32_t x0, x1;
-          footprint_row_tiles(mx, my, ca, cb, cc, thr, row_first, row_last, tyi, f.cam.width, x0, x1, side_log2);
+          span_row_tiles(span, row_first, row_last, tyi, f.cam.width, x0, x1, side_log2);
           tiles += x1 - x0;
         }
       }
@@ -146,7 +165,7 @@ __device__ __forceinline__ void project_view(const gsh::ViewDev& vd, const Proje
     if (depth16) {
       // The reference's 16-bit log depth (train_project.comp:110) in the lower half; the sort
       // ranks those 16 bits only, and the tile count (< 2^16) rides in the upper half.
-      const float u = gs_logf(tz / kNear) / gs_logf(kFar / kNear);
+      const float u = gs_logf(tz / kNear) / log_depth_range;
       key = uint32_t(fminf(fmaxf(u, 0.0f), 1.0f) * 65535.0f) | (tiles << 16);
     } else {
       key = __float_as_uint(tz);  // z > 0: the bits order like the values
@@ -186,7 +205,9 @@ __global__ void __launch_bounds__(256) k_project(const gsh::SceneDev* __restrict
       }
     }
   }
-  for (uint32_t v = 0; v < a.count; ++v) project_view(a.views[v], a, i, p0, p1, q, p3, rest);
+  const SlotShape shape = slot_shape(p0, p1, q);
+  const float log_depth_range = gs_logf(kFar / kNear);
+  for (uint32_t v = 0; v < a.count; ++v) project_view(a.views[v], a, i, p0, p1, shape, p3, rest, log_depth_range);
 }
 
 }  // namespace

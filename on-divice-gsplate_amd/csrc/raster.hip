@@ -4,13 +4,16 @@
 //
 // Both kernels stage the tile's splats through LDS in batches of 256 (one
 // coalesced 48 B record load per thread).  The staging thread rewrites the
-// record into evaluation form, (mean, -a/2, -b), (-c/2, opacity, r, g), b, so
-// the per-pixel exponent is power = dx (A dx + B dy) + C dy^2 (the same
-// quadratic form as the reference's -(a dx^2 + c dy^2)/2 - b dx dy), and
-// tests the footprint bbox against the four quadrants: a ballot per quadrant
-// gives each wave a 64-bit mask of the batch's splats that can reach its
-// pixels.  The mask is moved to scalar registers, so the walk over its bits
-// is scalar control flow and the vector unit only evaluates splats.
+// record into evaluation form, (mean, A, B), (C, opacity, r, g), b with
+// (A, B, C) = -(a/2, b, c/2), so the per-pixel exponent is
+// power = dx (A dx + B dy) + C dy^2 (the same quadratic form as the
+// reference's -(a dx^2 + c dy^2)/2 - b dx dy), and tests the footprint bbox
+// against the four quadrants: a ballot per quadrant gives each wave a 64-bit
+// mask of the batch's splats that can reach its pixels.  The mask is moved to
+// scalar registers, so the walk over its bits is scalar control flow and the
+// vector unit only evaluates splats.  (Scaling (A, B, C) by log2(e) to save
+// exp's multiply moves the power by a few ulps of its terms, which for long
+// thin splats is more than the oracle's rounding model of the power allows.)
 //
 // Forward (engine/shaders/train_forward.comp:59-141): alpha = min(0.99,
 // o exp(power)), skipped below 1/255 or for power > 0; a pixel stops before
@@ -21,8 +24,12 @@
 // of train_backward_serial.glsl): each pixel starts at its final T and walks
 // its tile's list back to front from its last contributor, recovering
 // T_i = T_{i+1} / (1 - alpha_i) and the colour behind each splat; nothing
-// per pixel is stored by the forward besides T, LAST and dL/dC.  The nine
-// per-splat gradients are summed over each wave by a transposed reduction
+// per pixel is stored by the forward besides T, LAST and dL/dC.  A pixel's
+// share of the splat's mean and conic gradients is dL/dpower times a
+// polynomial in (dx, dy) whose coefficients are the splat's, and
+// dL/dpower = o dL/do, so each pixel adds only g dx, g dy, g dx^2, g dx dy,
+// g dy^2 with g = dL/do; the splat's constants are applied once to the
+// tile's sums.  The nine per-splat sums are summed over each wave by a transposed reduction
 // (one quad ends with each value), over the four waves in LDS (ds_add_f32),
 // and added to the slot's GRAD2D with one global float atomic each.  GRAD2D
 // keeps a slot's nine sums together and the flush gives consecutive lanes
@@ -81,7 +88,9 @@ __device__ __forceinline__ PixelCoord pixel_coord(const gs_hip_frame& f, uint32_
 // so one address register and three immediate offsets reach a splat.
 template <int B = kBatch>
 struct Staging {
-  float4 sp[B * 3];  // [3j]: mean x, mean y, A = -a/2, B = -b; [3j + 1]: C = -c/2, opacity, r, g; [3j + 2].x: b
+  // [3j]: mean x, mean y, A, B; [3j + 1]: C, opacity, r, g; [3j + 2].x: b;
+  // (A, B, C) = -(a/2, b, c/2)
+  float4 sp[B * 3];
   uint64_t mask[4][B / 64];  // [quadrant][loading wave]
   __device__ __forceinline__ float4 a(int j) const { return sp[3 * j]; }
   __device__ __forceinline__ float4 b(int j) const { return sp[3 * j + 1]; }
@@ -136,17 +145,15 @@ __device__ __forceinline__ void publish_masks(Staging<B>& s, uint32_t hits, int 
 // Power and alpha of a splat at a pixel centre, shared by both kernels so
 // they make the same skip decisions on the same bits.
 struct SplatEval {
-  float dx, dy, adx, cdy, power, G, raw, alpha;
+  float dx, dy, power, G, raw, alpha;  // G = exp(power)
 };
 
 __device__ __forceinline__ SplatEval eval_splat(float4 sa, float4 sb, float cx, float cy) {
   SplatEval e;
   e.dx = sa.x - cx;
   e.dy = sa.y - cy;
-  e.adx = sa.z * e.dx;
-  e.cdy = sb.x * e.dy;
-  const float u = fmaf(sa.w, e.dy, e.adx);  // A dx + B dy
-  e.power = fmaf(e.dx, u, e.cdy * e.dy);
+  const float u = fmaf(sa.w, e.dy, sa.z * e.dx);  // A dx + B dy
+  e.power = fmaf(e.dx, u, (sb.x * e.dy) * e.dy);
   e.G = __expf(e.power);
   e.raw = sb.y * e.G;
   e.alpha = fminf(kMaxAlpha, e.raw);
@@ -234,6 +241,34 @@ __device__ __forceinline__ long long to_fixed(float v) {
   return __float2ll_rn(fminf(fmaxf(v, -kFixedLimit), kFixedLimit) * kFixedScale);
 }
 
+// The tile's sum of the per-pixel value c of staged splat j (the waves'
+// sums added in wave order for the deterministic variant).
+template <bool kDet>
+__device__ __forceinline__ float tile_sum(const float* s_grad, uint32_t j, uint32_t c) {
+  const uint32_t e = j * uint32_t(kGrad) + c;
+  if (!kDet) return s_grad[e];
+  constexpr uint32_t w = kBatch * kGrad;
+  return (s_grad[e] + s_grad[w + e]) + (s_grad[2 * w + e] + s_grad[3 * w + e]);
+}
+
+// GRAD2D value c of staged splat j from the tile's sums of g dx, g dy,
+// g dx^2, g dx dy, g dy^2, g (g = dL/do) and the colour gradient: with
+// power = A dx^2 + B dx dy + C dy^2, (A, B, C) = -(a/2, b, c/2), and
+// dL/dpower = o g,
+//   dL/dmean = o (sum g d power/d mean) = o (2A sx + B sy, B sx + 2C sy),
+//   dL/d(a, b, c) = -o (sxx / 2, sxy, syy / 2).
+template <bool kDet>
+__device__ __forceinline__ float tile_gradient(const Staging<>& s, const float* s_grad, uint32_t j, uint32_t c) {
+  const float sum = tile_sum<kDet>(s_grad, j, c);
+  if (c >= 5u) return sum;
+  const float4 sa = s.a(int(j));
+  const float4 sb = s.b(int(j));
+  const float o = sb.y;
+  if (c == 0u) return o * fmaf(2.0f * sa.z, sum, sa.w * tile_sum<kDet>(s_grad, j, 1u));
+  if (c == 1u) return o * fmaf(2.0f * sb.x, sum, sa.w * tile_sum<kDet>(s_grad, j, 0u));
+  return (c == 3u ? -o : -0.5f * o) * sum;
+}
+
 // Recompute-transmittance adjoint of one tile of one view.
 template <bool kDet>
 __device__ __forceinline__ void backward_tile(const gsh::ViewDev& a, uint32_t tile, BackwardShared<kDet>& sh,
@@ -308,23 +343,17 @@ __device__ __forceinline__ void backward_tile(const gsh::ViewDev& a, uint32_t ti
         const float inv = __builtin_amdgcn_rcpf(1.0f - alpha);
         const float T_in = T * inv;  // transmittance in front of this splat
         const float col_dot = fmaf(sb.z, gr, fmaf(sb.w, gg, blue * gb));  // colour . dL/dC
-        const float dL_dalpha = (col_dot - acc_dot) * T_in - bg_term * inv;
+        const float behind = col_dot - acc_dot;
+        const float dL_dalpha = behind * T_in - bg_term * inv;
         const bool unclamped = contributes && e.raw < kMaxAlpha;
         const float wT = a_eff * T_in;
-        const float dpow = unclamped ? e.raw * dL_dalpha : 0.0f;
+        // dL/do = exp(power) dL/dalpha (alpha unclamped), and dL/dpower = o dL/do.
         const float g5 = unclamped ? e.G * dL_dalpha : 0.0f;
-        const float dxp = dpow * e.dx;
-        const float g0 = dpow * (fmaf(sa.w, e.dy, e.adx) + e.adx);  // dpow (2A dx + B dy)
-        const float g1 = dpow * fmaf(sa.w, e.dx, 2.0f * e.cdy);     // dpow (B dx + 2C dy)
-        // dL/d(A, B, C) of power = A dx^2 + B dx dy + C dy^2; the flush scales
-        // them to the conic's a, b, c (A = -a/2, B = -b, C = -c/2), exactly.
-        const float g2 = dxp * e.dx;
-        const float g3 = dxp * e.dy;
-        const float g4 = (dpow * e.dy) * e.dy;
+        const float gx = g5 * e.dx, gy = g5 * e.dy;
         const float g6 = wT * gr, g7 = wT * gg, g8 = wT * gb;
         T = contributes ? T_in : T;
-        acc_dot = fmaf(a_eff, col_dot - acc_dot, acc_dot);  // this splat joins the colour behind the next
-        const float z = red.reduce(g0, g1, g2, g3, g4, g5, g6, g7, g8);
+        acc_dot = fmaf(a_eff, behind, acc_dot);  // this splat joins the colour behind the next
+        const float z = red.reduce(gx, gy, gx * e.dx, gx * e.dy, gy * e.dy, g5, g6, g7, g8);
         if (slot >= 0) {
           if (kDet) {
             s_grad[(pc.wave * kBatch + j) * kGrad + slot] = z;  // this wave's only write of the value
@@ -338,17 +367,13 @@ __device__ __forceinline__ void backward_tile(const gsh::ViewDev& a, uint32_t ti
     __syncthreads();
     for (uint32_t e = threadIdx.x; e < n * uint32_t(kGrad); e += kThreads) {
       const uint32_t j = e / uint32_t(kGrad), c = e - j * uint32_t(kGrad);
-      const float scale = c == 3u ? -1.0f : (c == 2u || c == 4u ? -0.5f : 1.0f);
+      const float v = tile_gradient<kDet>(s, s_grad, j, c);
+      if (v == 0.0f) continue;
       if (kDet) {
-        const uint32_t w = kBatch * kGrad;
-        const float v = ((s_grad[e] + s_grad[w + e]) + (s_grad[2 * w + e] + s_grad[3 * w + e])) * scale;
-        if (v != 0.0f) {
-          __hip_atomic_fetch_add(gmem(a.grad_fx) + size_t(s_g[j]) * kGrad + c, static_cast<unsigned long long>(to_fixed(v)),
-                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
+        __hip_atomic_fetch_add(gmem(a.grad_fx) + size_t(s_g[j]) * kGrad + c, static_cast<unsigned long long>(to_fixed(v)),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       } else {
-        const float v = s_grad[e] * scale;
-        if (v != 0.0f) global_add(gmem(a.grad2d) + size_t(s_g[j]) * kGrad + c, v);
+        global_add(gmem(a.grad2d) + size_t(s_g[j]) * kGrad + c, v);
       }
     }
     __syncthreads();

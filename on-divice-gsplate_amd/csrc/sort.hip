@@ -426,6 +426,7 @@ __global__ void __launch_bounds__(256) k_emit(const ViewDev* __restrict__ views)
   const uint64_t end = min(p0 + count, cap);
   int row_first = 0, row_last = -1;
   if (small) footprint_rows(r0.y, fp.z, fp.w, f.cam.height, row_first, row_last);
+  const RowSpan span = row_span(r0.x, r0.y, fp.x, fp.y, fp.z, fp.w);
   // The run of the wave's small-splat pairs (wave-uniform); windows skip the
   // stretches that only large splats fill.
   const uint64_t hi = wave_max_u64(small ? end : 0ull);
@@ -441,8 +442,7 @@ __global__ void __launch_bounds__(256) k_emit(const ViewDev* __restrict__ views)
       if (row_first <= row_last) {
         for (uint32_t ty = uint32_t(row_first) >> side_log2; ty <= (uint32_t(row_last) >> side_log2); ++ty) {
           uint32_t x0, x1;
-          footprint_row_tiles(r0.x, r0.y, fp.x, fp.y, fp.z, fp.w, row_first, row_last, ty, f.cam.width, x0, x1,
-                              side_log2);
+          span_row_tiles(span, row_first, row_last, ty, f.cam.width, x0, x1, side_log2);
           wanted += x1 - x0;
           for (uint32_t x = x0; x < x1 && p < end; ++x, ++p) {
             if (p >= win && p < win + kEmitWindow) {
@@ -490,10 +490,10 @@ __global__ void __launch_bounds__(256) k_emit(const ViewDev* __restrict__ views)
     int row_first, row_last;
     footprint_rows(bmy, bfp.z, bfp.w, f.cam.height, row_first, row_last);
     if (row_first <= row_last) {
+      const RowSpan bspan = row_span(bmx, bmy, bfp.x, bfp.y, bfp.z, bfp.w);
       for (uint32_t ty = uint32_t(row_first) >> side_log2; ty <= (uint32_t(row_last) >> side_log2); ++ty) {
         uint32_t x0, x1;
-        footprint_row_tiles(bmx, bmy, bfp.x, bfp.y, bfp.z, bfp.w, row_first, row_last, ty, f.cam.width, x0, x1,
-                            side_log2);
+        span_row_tiles(bspan, row_first, row_last, ty, f.cam.width, x0, x1, side_log2);
         wanted += x1 - x0;
         for (uint32_t x = x0 + uint32_t(lane); x < x1; x += 64u) {
           const uint64_t q = p + (x - x0);
