@@ -58,10 +58,11 @@ def kernel_bytes(name, n_slots, n_visible, pairs, pixels, coeffs, views):
     44 per pair (index + 40 B record) + 28 per pixel; SSIM 16 per pixel
     (render + target in) and the loss gradient 12 (dL/dC out); backward 44
     per pair + 20 per pixel + 36 per visible gaussian (the 2D gradient sums);
-    adjoint + Adam 6 D + 16 (state, statistics) per slot once per step and
-    36 per slot per view (the views' 2D gradients).  `pairs` and `n_visible` are per view; D = 4 (11 + 3 K) bytes
-    of parameters per gaussian with K = coeffs + 1 SH coefficients per
-    channel."""
+    adjoint + Adam on the 16-float slots 6 x 64 + 32 + 48 (state,
+    statistics, the three SH groups' direction terms) per slot and 44 per
+    slot per view (tiles, 2D gradient, misc); SH rest Adam 6 x 12 coeffs + 64
+    per slot (state, position, direction terms) and 16 per slot per view.  `pairs` and `n_visible` are per view; D = 4 (11 + 3 K) bytes of
+    parameters per gaussian with K = coeffs + 1 SH coefficients per channel."""
     D = 4 * (11 + 3 * (coeffs + 1))
     if name == "project":
         return views * n_slots * (D + 44)
@@ -79,8 +80,10 @@ def kernel_bytes(name, n_slots, n_visible, pairs, pixels, coeffs, views):
         return views * pixels * 12
     if name == "backward":
         return views * (pairs * 44 + pixels * 20 + n_visible * 36)
-    if name == "adam":  # state read + written once; each view's 2D gradient read
-        return n_slots * (6 * D + 16) + views * n_slots * 36
+    if name == "adam":  # 16-float state read + written, statistics, SH direction terms; per view tiles, GRAD2D, misc
+        return n_slots * (6 * 64 + 32 + (48 if coeffs else 0)) + views * n_slots * 44
+    if name == "adam_sh":  # SH rest state read + written, position, direction terms out; per view colour gradient, misc
+        return n_slots * (6 * 12 * coeffs + 16 + 48) + views * n_slots * 16
     if name == "depth_sort":  # one pass over (key, value): not in the step model, which counts the pairs' sort
         return views * n_slots * 16
     if name == "scan":

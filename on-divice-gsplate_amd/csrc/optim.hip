@@ -689,12 +689,14 @@ int launch_adam(gs_hip_workspace* ws, uint32_t count) {
   if (count == 0 || count != ws->shape.views) return set_error(GS_HIP_E_ARGUMENT, "adam: view count out of range");
   if (ws->shape.max_capacity == 0) return 0;
   const dim3 grid(div_up(ws->shape.max_capacity, 256), ws->shape.scenes);
-  timer_begin(ws, "adam");
   if (ws->shape.any_sh) {  // first: it reads the positions k_adam updates
+    timer_begin(ws, "adam_sh");
     k_adam_sh<<<dim3(div_up(ws->shape.max_capacity, 64), grid.y), kShBlock, 0, ws->stream>>>(ws->dev_scenes,
                                                                                           ws->dev_views, count);
     GS_CHECK_LAUNCH("k_adam_sh");
+    timer_end(ws);
   }
+  timer_begin(ws, "adam");
   k_adam<<<grid, 256, 0, ws->stream>>>(ws->dev_scenes, ws->dev_views, count);
   GS_CHECK_LAUNCH("k_adam");
   timer_end(ws);
