@@ -137,7 +137,9 @@ typedef struct gs_hip_workspace gs_hip_workspace;
  *  DLDC                         float[H*W][4]  (dL/dC rgb, -)
  *  TILE_LOSS                    float[tiles][4] (sum |e|, sum e^2, sum SSIM, -)
  *  TOTALS                       uint32[16]     ([0] pairs of the last bin, [1] emit mismatches)
- *  TILE_ORDER                   uint32[tiles]  (the rasterizers' launch order: longest lists first)
+ *  TILE_ORDER                   uint32[views][4 quads] (the rasterizers' launch order: view << 16 | tile
+ *                               in 2x2 quads of tiles, 0xffffffff for a member outside the grid, the
+ *                               quads with the longest lists first)
  *  TILE_FIT                     float[tiles][8] (with GS_HIP_FLAG_EXPOSURE: sum C^2 rgb, fitted pixels,
  *                               sum C photo rgb, subject pixels; C the model's colour before the background)
  *  POSE_GRAD                    float[ceil(capacity / 256)][8] (with GS_HIP_FLAG_POSE_GRAD, per 256 slots:

@@ -69,7 +69,7 @@ int gs_hip_batch_create(gs_hip_batch** out, gs_hip_workspace* const* scenes, uin
   int rc = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) == hipSuccess
                ? 0
                : gsh::set_error(GS_HIP_E_HIP, "hipStreamCreate failed");
-  if (rc == 0 && tiles > 0xffffffffull) rc = gsh::set_error(GS_HIP_E_CAPACITY, "batch_create: too many tiles");
+  if (rc == 0 && tiles * 4u > 0xffffffffull) rc = gsh::set_error(GS_HIP_E_CAPACITY, "batch_create: too many tiles");
   if (rc == 0) rc = gsh::alloc_tables(&c, gsh::kMaxBatchViews, gsh::kMaxBatchScenes, uint32_t(tiles));
   if (rc == 0 && hipStreamSynchronize(c.stream) != hipSuccess) rc = gsh::set_error(GS_HIP_E_HIP, "batch init sync");
   if (rc != 0) {

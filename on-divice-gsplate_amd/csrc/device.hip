@@ -381,6 +381,7 @@ int upload_entries(gs_hip_workspace* ws, const std::vector<TableEntry>& entries,
     if (scenes.back().views > GS_HIP_MAX_VIEWS) return set_error(GS_HIP_E_ARGUMENT, "more views of one scene than a step holds");
     ws->tile_base[i] = shape.total_tiles;
     shape.total_tiles += v.tile_grid;
+    shape.work_items += quad_items(e.frame->tiles_x, e.frame->tiles_y);
     shape.max_tiles = std::max(shape.max_tiles, v.tile_grid);
     shape.max_bins = std::max(shape.max_bins, bins_along(e.frame->cam.width, shift) *
                                                   bins_along(e.frame->cam.height, shift));
@@ -400,7 +401,7 @@ int upload_entries(gs_hip_workspace* ws, const std::vector<TableEntry>& entries,
                                uint64_t(e.view->sort.tile_cols) * kTileChunk >= e.view->pair_capacity;
   }
   if (shape.scenes > ws->table_scenes) return set_error(GS_HIP_E_ARGUMENT, "too many scenes in one step");
-  if (shape.total_tiles > ws->work_entries || shape.total_tiles > ws->tile_loss_entries) {
+  if (shape.work_items > ws->work_entries || shape.total_tiles > ws->tile_loss_entries) {
     return set_error(GS_HIP_E_ARGUMENT, "the step's tiles exceed the launch context's tables");
   }
   if (shape.deterministic) {  // the fixed-point sums, made on first use
@@ -453,7 +454,8 @@ int upload_table(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t coun
 int alloc_tables(gs_hip_workspace* ws, uint32_t views, uint32_t scenes, uint32_t tile_entries) {
   ws->table_views = views;
   ws->table_scenes = scenes;
-  ws->work_entries = tile_entries;
+  // Quads of 2x2 tiles: at most 4x the tiles (a grid of one tile).
+  ws->work_entries = tile_entries * 4u;
   ws->tile_loss_entries = tile_entries;
   int rc = 0;
   auto A = [&](int r) {
@@ -465,7 +467,7 @@ int alloc_tables(gs_hip_workspace* ws, uint32_t views, uint32_t scenes, uint32_t
   A(alloc(&ws->dev_tile_sort, views));
   A(alloc(&ws->dev_tile_scan, views));
   A(alloc(&ws->dev_scan_tmp, 1));
-  A(alloc(&ws->work, std::max<uint32_t>(tile_entries, 1)));
+  A(alloc(&ws->work, std::max<uint32_t>(ws->work_entries, 1)));
   A(alloc(&ws->work_counters, 16));
   A(alloc(&ws->totals_all, size_t(16) * views));
   A(alloc(&ws->tile_loss_all, size_t(std::max<uint32_t>(tile_entries, 1)) * 4));
@@ -742,7 +744,9 @@ int gs_hip_workspace_view_buffer(gs_hip_workspace* ws, uint32_t view, int which,
     case GS_HIP_BUF_DLDC: p = v.dldc, b = pixels * 16; break;
     case GS_HIP_BUF_TILE_LOSS: p = v.tile_loss, b = tiles * 16; break;
     case GS_HIP_BUF_TOTALS: p = v.totals, b = 64; break;
-    case GS_HIP_BUF_TILE_ORDER: p = ws->work, b = tiles * 4 * ws->views.size(); break;
+    case GS_HIP_BUF_TILE_ORDER:
+      p = ws->work, b = size_t(gsh::quad_items(ws->tiles_x, ws->tiles_y)) * 4 * ws->views.size();
+      break;
     case GS_HIP_BUF_TILE_FIT: p = v.tile_fit, b = tiles * 32; break;
     case GS_HIP_BUF_POSE_GRAD: p = v.pose_grad, b = gsh::div_up(uint32_t(std::max<size_t>(cap, 1)), 256) * 32; break;
     default: return gsh::set_error(GS_HIP_E_ARGUMENT, "workspace_buffer: unknown buffer");

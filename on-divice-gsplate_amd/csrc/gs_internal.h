@@ -148,6 +148,15 @@ __host__ __device__ inline uint32_t bin_of_tile(const ViewDev& v, uint32_t tile)
 // Rasterizer work items name (view, tile): tiles per view stay below 2^16.
 constexpr uint32_t kTileBits = 16;
 constexpr uint32_t kMaxTilesPerView = (1u << kTileBits) - 1u;
+// A rasterizer work item for a quad member outside the tile grid.
+constexpr uint32_t kNoWork = 0xffffffffu;
+
+// Rasterizer work items of a view: its tiles in 2x2 quads (the quads of the
+// last column and row may stick out of the grid).
+__host__ __device__ inline uint32_t quads_x(uint32_t tiles_x) { return (tiles_x + 1u) / 2u; }
+__host__ __device__ inline uint32_t quad_items(uint32_t tiles_x, uint32_t tiles_y) {
+  return 4u * quads_x(tiles_x) * ((tiles_y + 1u) / 2u);
+}
 // A launch context's table sizes: a workspace's own views, or a batch's.
 constexpr uint32_t kMaxBatchViews = 64;
 constexpr uint32_t kMaxBatchScenes = 32;
@@ -159,7 +168,8 @@ struct StepShape {
   uint32_t max_capacity = 0;       // the largest scene's slot capacity (per-slot grids)
   uint64_t max_pair_capacity = 0;  // the largest pair buffer (per-pair grids)
   uint32_t max_tiles = 0;          // the largest tile grid (per-tile grids)
-  uint32_t total_tiles = 0;        // every view's tiles (rasterizer work items)
+  uint32_t total_tiles = 0;        // every view's tiles (per-tile loss entries)
+  uint32_t work_items = 0;         // rasterizer work items: 4 per 2x2 quad of tiles of every view
   uint32_t max_bins = 0;           // the most pair bins of a view
   uint32_t tile_key_bits = 0;      // tile sort key width (bins)
   bool counting_tile_sort = false; // one counting pass (tuning.tile_digit_bits == 0 and the tile grids fit)

@@ -37,9 +37,14 @@
 // seven slots' cache lines instead of 64.
 //
 // Work: the (view, tile) items of all views of the step in longest-list-first
-// order.  Either one workgroup per item (the hardware dispatches them in
-// order), or a persistent grid of a few workgroups per CU that take items
-// from a device counter until none are left.
+// order, in 2x2 quads of neighbouring tiles.  Either one workgroup per item,
+// or a persistent grid of a few workgroups per CU that take items from a
+// device counter until none are left.  With one workgroup per item the
+// hardware deals workgroups to the 8 XCDs in turn (workgroup b to XCD b mod
+// 8); workgroup b takes item 32 (b / 32) + 4 (b mod 8) + (b / 8) mod 4, so
+// the four tiles of a quad run on one XCD at about the same time and share
+// their records through its L2, while the list is still taken in order
+// 32 items at a time.
 #include <algorithm>
 
 #include "cull.h"
@@ -392,14 +397,22 @@ __device__ __forceinline__ uint32_t next_item(const RasterArgs& a, uint32_t* s_i
   return item;
 }
 
+// Workgroup b's item when each takes one (see the work order above).
+__device__ __forceinline__ uint32_t xcd_item(uint32_t b, uint32_t items) {
+  constexpr uint32_t kXcds = 8;
+  constexpr uint32_t kRound = kXcds * 4u;
+  if (b >= items - items % kRound) return b;
+  return kRound * (b / kRound) + 4u * (b % kXcds) + (b / kXcds) % 4u;
+}
+
 template <int B>
 __global__ void __launch_bounds__(kThreads) k_forward(RasterArgs a) {
   __shared__ Staging<B> s;
   __shared__ uint32_t s_item;
-  uint32_t item = a.persistent != 0u ? next_item(a, &s_item) : blockIdx.x;
+  uint32_t item = a.persistent != 0u ? next_item(a, &s_item) : xcd_item(blockIdx.x, a.items);
   while (item < a.items) {
     const uint32_t w = a.work[item];
-    forward_tile<B>(a.views[w >> gsh::kTileBits], w & gsh::kMaxTilesPerView, s);
+    if (w != gsh::kNoWork) forward_tile<B>(a.views[w >> gsh::kTileBits], w & gsh::kMaxTilesPerView, s);
     if (a.persistent == 0u) break;
     __syncthreads();
     item = next_item(a, &s_item);
@@ -412,11 +425,10 @@ __global__ void __launch_bounds__(kThreads) k_backward(RasterArgs a) {
   __shared__ uint32_t s_item;
   const Reduce9 red;
   const int slot = red.slot();
-  uint32_t item = a.persistent != 0u ? next_item(a, &s_item) : blockIdx.x;
+  uint32_t item = a.persistent != 0u ? next_item(a, &s_item) : xcd_item(blockIdx.x, a.items);
   while (item < a.items) {
     const uint32_t w = a.work[item];
-    const gsh::ViewDev& view = a.views[w >> gsh::kTileBits];
-    backward_tile<kDet>(view, w & gsh::kMaxTilesPerView, sh, red, slot);
+    if (w != gsh::kNoWork) backward_tile<kDet>(a.views[w >> gsh::kTileBits], w & gsh::kMaxTilesPerView, sh, red, slot);
     if (a.persistent == 0u) break;
     __syncthreads();
     item = next_item(a, &s_item);
@@ -440,7 +452,7 @@ RasterArgs make_args(gs_hip_workspace* ws, uint32_t count, int which, uint32_t& 
   a.views = ws->dev_views;
   a.work = ws->work;
   a.counter = ws->work_counters + which;
-  a.items = ws->shape.total_tiles;
+  a.items = ws->shape.work_items;
   (void)count;
   a.persistent = ws->tuning.raster_groups_per_cu > 0 ? 1u : 0u;
   grid = a.persistent ? std::min<uint32_t>(a.items, ws->cus * ws->tuning.raster_groups_per_cu) : a.items;

@@ -560,25 +560,43 @@ __global__ void __launch_bounds__(256) k_ranges(const ViewDev* __restrict__ view
 }
 
 // The rasterizers' work: every (view, tile) of the step, as view << 16 | tile,
-// bucketed by floor(log2(list length)), longest first, empty tiles last
-// (longest-processing-time-first scheduling within a factor of two, across
-// all views).  One workgroup; also resets the persistent rasterizers' work
-// counters.
+// in 2x2 quads of neighbouring tiles (four consecutive items, kNoWork for a
+// member outside the grid), the quads bucketed by floor(log2(their longest
+// list)), longest first, empty quads last (longest-processing-time-first
+// scheduling within a factor of two, across all views).  The rasterizers
+// hand a quad's four items to workgroups on one XCD, so the records the
+// neighbours share (a whole bin's list with pair bins wider than a tile)
+// are fetched from HBM once into that XCD's L2.  One workgroup; also resets
+// the persistent rasterizers' work counters.
+__device__ __forceinline__ uint32_t quad_bucket(const ViewDev& view, uint32_t q) {
+  const uint32_t tx = view.f.tiles_x, ty = view.f.tiles_y, qx = gsh::quads_x(tx);
+  const uint32_t x0 = (q % qx) * 2u, y0 = (q / qx) * 2u;
+  uint32_t longest = 0u;
+  for (uint32_t m = 0; m < 4u; ++m) {
+    const uint32_t x = x0 + (m & 1u), y = y0 + (m >> 1);
+    if (x < tx && y < ty) {
+      const uint2 r = view.ranges[gsh::bin_of_tile(view, y * tx + x)];
+      longest = max(longest, r.y - r.x);
+    }
+  }
+  return __clz(longest);
+}
+
 __global__ void __launch_bounds__(1024) k_work_order(const ViewDev* __restrict__ views, uint32_t count,
                                                      uint32_t* __restrict__ work, uint32_t* __restrict__ counters) {
   __shared__ uint32_t s_count[33];
   if (threadIdx.x < 33) s_count[threadIdx.x] = 0u;
   if (threadIdx.x < 2) counters[threadIdx.x] = 0u;
   __syncthreads();
-  // Most tiles fall into a few buckets, so the lanes of a wave that share
+  // Most quads fall into a few buckets, so the lanes of a wave that share
   // one add their count once (an LDS atomic from 64 lanes on one counter
   // would take 64 passes).
   const uint64_t below_me = lanemask_lt();
   for (uint32_t v = 0; v < count; ++v) {
     const ViewDev& view = views[v];
-    for (uint32_t t = threadIdx.x; t < view.tile_count; t += blockDim.x) {
-      const uint2 r = view.ranges[gsh::bin_of_tile(view, t)];
-      const uint32_t b = __clz(r.y - r.x);
+    const uint32_t quads = gsh::quad_items(view.f.tiles_x, view.f.tiles_y) / 4u;
+    for (uint32_t q = threadIdx.x; q < quads; q += blockDim.x) {
+      const uint32_t b = quad_bucket(view, q);
       const uint64_t peers = match_digit(b, 6, __ballot(true));
       if ((peers & below_me) == 0ull) atomicAdd(&s_count[b], uint32_t(__popcll(peers)));
     }
@@ -595,15 +613,21 @@ __global__ void __launch_bounds__(1024) k_work_order(const ViewDev* __restrict__
   __syncthreads();
   for (uint32_t v = 0; v < count; ++v) {
     const ViewDev& view = views[v];
-    for (uint32_t t = threadIdx.x; t < view.tile_count; t += blockDim.x) {
-      const uint2 r = view.ranges[gsh::bin_of_tile(view, t)];
-      const uint32_t b = __clz(r.y - r.x);
+    const uint32_t tx = view.f.tiles_x, ty = view.f.tiles_y, qx = gsh::quads_x(tx);
+    const uint32_t quads = gsh::quad_items(tx, ty) / 4u;
+    for (uint32_t q = threadIdx.x; q < quads; q += blockDim.x) {
+      const uint32_t b = quad_bucket(view, q);
       const uint64_t peers = match_digit(b, 6, __ballot(true));
       const int leader = __ffsll(static_cast<long long>(peers)) - 1;
       uint32_t first = 0u;
       if ((peers & below_me) == 0ull) first = atomicAdd(&s_count[b], uint32_t(__popcll(peers)));
       first = __shfl(first, leader, 64);
-      work[first + uint32_t(__popcll(peers & below_me))] = (v << gsh::kTileBits) | t;
+      const uint32_t at = 4u * (first + uint32_t(__popcll(peers & below_me)));
+      const uint32_t x0 = (q % qx) * 2u, y0 = (q / qx) * 2u;
+      for (uint32_t m = 0; m < 4u; ++m) {
+        const uint32_t x = x0 + (m & 1u), y = y0 + (m >> 1);
+        work[at + m] = x < tx && y < ty ? (v << gsh::kTileBits) | (y * tx + x) : gsh::kNoWork;
+      }
     }
   }
 }

@@ -243,10 +243,12 @@ def test_step_targets_are_read_in_place():
 
 
 def test_work_order_is_every_tile_once_longest_first():
-    """The rasterizers' work list (TILE_ORDER) after a 3-view step holds every
-    (view, tile) exactly once, as view << 16 | tile, bucketed by
-    floor(log2(list length)) longest first, empty tiles last."""
-    case("work order: a permutation of the 3 views' tiles, longest lists first")
+    """The rasterizers' work list (TILE_ORDER) after a 3-view step: 2x2 quads
+    of neighbouring tiles of one view as four consecutive entries
+    (view << 16 | tile, 0xffffffff for a member outside the tile grid),
+    holding every (view, tile) exactly once, the quads bucketed by
+    floor(log2(their longest list)) longest first, empty quads last."""
+    case("work order: the 3 views' tiles in 2x2 quads, longest lists first")
     n, deg, w, h = 2500, 1, 200, 150
     params, sh = make_scene(n, deg, 78)
     cams = [make_camera(w, h, 0.4 + 1.1 * v) for v in range(3)]
@@ -256,8 +258,10 @@ def test_work_order_is_every_tile_once_longest_first():
             ws.set_views(3)
             ws.load(params, sh)
             ws.train_views([gs.make_frame(c, n, deg, tp=train_params(), step=1) for c in cams], [b.ptr for b in bufs])
-            T = ws.tiles_x * ws.tiles_y
-            order = ws.download(gs.BUF_TILE_ORDER, np.uint32, 3 * T)
+            tx, ty = ws.tiles_x, ws.tiles_y
+            T = tx * ty
+            quads = ((tx + 1) // 2) * ((ty + 1) // 2)
+            order = ws.download(gs.BUF_TILE_ORDER, np.uint32, 3 * 4 * quads)
             lengths = []
             for v in range(3):
                 r = ws.download(gs.BUF_RANGES, np.uint32, (T + 1) * 2, view=v).reshape(-1, 2)[:T]
@@ -265,9 +269,22 @@ def test_work_order_is_every_tile_once_longest_first():
     finally:
         for b in bufs:
             b.free()
+    assert tx % 2 == 1 or ty % 2 == 1, "the case covers quads that stick out of the grid"
+    entries = [x for x in order.tolist() if x != 0xFFFFFFFF]
     want = sorted((v << 16) | t for v in range(3) for t in range(T))
-    assert sorted(order.tolist()) == want
-    lens = np.array([lengths[x >> 16][x & 0xffff] for x in order.tolist()])
+    assert sorted(entries) == want
+    assert order.size - len(entries) == 3 * (4 * quads - T)
+    longest = []
+    for g in order.reshape(-1, 4).tolist():
+        members = [x for x in g if x != 0xFFFFFFFF]
+        assert members, "a quad with no tile"
+        views = {x >> 16 for x in members}
+        assert len(views) == 1, "a quad over two views"
+        xs = {(x & 0xFFFF) % tx for x in members}
+        ys = {(x & 0xFFFF) // tx for x in members}
+        assert max(xs) - min(xs) <= 1 and max(ys) - min(ys) <= 1 and min(xs) % 2 == 0 and min(ys) % 2 == 0
+        longest.append(max(lengths[x >> 16][x & 0xFFFF] for x in members))
+    lens = np.array(longest)
     bucket = np.where(lens > 0, 31 - np.floor(np.log2(np.maximum(lens, 1))).astype(np.int64), 32)
     assert np.all(np.diff(bucket) >= 0), "buckets out of order"
     assert lens.max() > 0
