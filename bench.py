@@ -12,8 +12,10 @@ the library's autotuner on that mode's first views.  Synthetic data, inputs
 resident in HBM before the timed region.
 
 Prints one JSON line: the metric, per-kernel HIP-event times over the timed
-region, the roofline figure of the dominant kernel and of the whole step, and
-a CPU baseline (the oracle's restatement of the same step on the host).
+region, the roofline figure of the dominant kernel and of the whole step, the
+held-out PSNR of plain and batched training at equal view-iterations (a
+curve from a common start), and a CPU baseline (the oracle's restatement of
+the same step on the host).
 """
 import argparse
 import importlib.util
@@ -164,7 +166,8 @@ def run(gs, ws, cams, target_bufs, order, per_step, views_total, warmup_views, d
         base = state["views"]
         vs = [int(order[(base + k) % len(order)]) for k in range(per_step)]
         # The step's frames differ only in their camera: one is built, the others copied.
-        f0 = gs.make_frame(cams[vs[0]], state["slots"], deg, step=state["step"], tp=tp, flags=state["flags"])
+        f0 = gs.make_frame(cams[vs[0]], state["slots"], deg, step=state["step"], tp=tp, flags=state["flags"],
+                           batch_scaling=state["batch_scaling"])
         frames = [f0]
         for v in vs[1:]:
             f = gs.Frame.from_buffer_copy(f0)
@@ -454,6 +457,40 @@ def depth_key_difference(gs, params, sh, cam, target_buf, deg):
             "grad2d_max_rel": float((np.abs(g16 - g32).max(axis=0) / scale).max())}
 
 
+def quality_curve(gs, cams, target_bufs, order, params, sh, held_cams, held_targets, deg, tp, flags, batch,
+                  densify_every, checkpoints):
+    """Held-out PSNR of plain (one view per optimizer step) and batched
+    (`batch` views per step; Adam's hyperparameters as given, and scaled for
+    the mean of `batch` views) training from the same start, over the same
+    view sequence and densification schedule, at each of `checkpoints`
+    view-iterations, with the training time (held-out renders excluded) to
+    reach it.  Default launch shapes, eager steps, the deterministic backward
+    (so that a rerun gives the same curve)."""
+    n = params.shape[0]
+    w, h = cams[0].width, cams[0].height
+    curve = {"view_iterations": list(checkpoints)}
+    for mode, per_step, scaling in (("plain", 1, 1), ("batched", batch, 1), ("batched_lr_scaled", batch, batch)):
+        ws = gs.Workspace(2 * n, deg, w, h)
+        ws.set_views(per_step)
+        ws.load(params, sh)
+        state = {"slots": n, "densified": 0, "pairs": 0, "views": 0, "step": 0,
+                 "flags": flags | gs.FLAG_DETERMINISTIC, "batch_scaling": scaling}
+        psnr, seconds, done, total = [], [], 0, 0.0
+        for c in checkpoints:
+            elapsed, _ = run(gs, ws, cams, target_bufs, order, per_step, c - done, 0, deg, tp, densify_every, state,
+                             timed=False)
+            total += elapsed
+            done = c
+            psnr.append(round(heldout_psnr(gs, ws, held_cams, held_targets, state["slots"], deg), 4))
+            seconds.append(round(total, 4))
+        curve[mode + "_db"] = psnr
+        curve[mode + "_s"] = seconds
+        curve[mode + "_gaussians_end"] = state["slots"]
+        ws.close()
+        log(f"quality: {mode} {psnr} dB at {list(checkpoints)} view-iterations ({seconds[-1]:.2f} s)")
+    return curve
+
+
 def kernel_check(gs, repeats=20):
     """The single-view forward + backward kernel check: 200k gaussians of the
     benchmark distributions at SH degree 3, one camera on the radius-3 orbit
@@ -533,6 +570,13 @@ def main():
                     help="fixed launch shape instead of autotuning: raster groups per CU (0 = one per tile), depth "
                          "and tile radix digit bits (tile 0: one counting pass), graph replay 0/1, pair bin shift "
                          "0..2 (bins of 16 << shift pixels; default 0)")
+    ap.add_argument("--no-quality", action="store_true",
+                    help="skip the held-out PSNR curve of plain vs batched training at equal view-iterations")
+    ap.add_argument("--quality-views", type=str, default="200,400,800,1600",
+                    help="view-iteration checkpoints of the quality curve")
+    ap.add_argument("--batch-scaling", choices=("none", "sqrt"), default="none",
+                    help="batched steps: Adam's hyperparameters as given (none) or scaled for the mean of the "
+                         "step's views (sqrt: rates x sqrt(views), betas to the power views)")
     ap.add_argument("--depth-key", type=int, default=16, choices=(16, 32),
                     help="pair order within a tile: the reference's 16-bit log-depth key (train_project.comp:110) "
                          "or the float bits of the depth")
@@ -608,12 +652,16 @@ def main():
         if tuning is None:
             tuning = ws.tuning().as_dict()
         graph = bool(tuning["graph"])
-        state = {"slots": N, "densified": 0, "pairs": 0, "views": 0, "step": 0, "flags": mode_flags}
+        state = {"slots": N, "densified": 0, "pairs": 0, "views": 0, "step": 0, "flags": mode_flags,
+                 "batch_scaling": per_step if args.batch_scaling == "sqrt" else 1}
         # Per-kernel HIP events need eager launches: with graph replay chosen,
         # the timed run replays the graph and a short eager pass after it
         # times the kernels.
         elapsed, steps = run(gs, ws, cams, target_bufs, order, per_step, views_total, warmup_views, deg, tp,
                              args.densify_every, state, timed=not graph)
+        # The model's quality now, before the kernel-timing pass below trains it further.
+        heldout = heldout_psnr(gs, ws, held_cams, held_targets, state["slots"], deg)
+        log(f"{mode}: held-out PSNR {heldout:.3f} dB after {state['views']} view-iterations")
         if graph:
             probe = dict(state)
             probe_views = max(per_step, (views_total // 5 // per_step) * per_step)
@@ -641,8 +689,7 @@ def main():
             "kernel_timing": "eager pass after the timed run" if graph else "the timed run",
         }
         log(f"{mode}: {views_per_s:.1f} view-iterations/s ({1000 * elapsed / steps:.3f} ms per optimizer step)")
-        results[mode]["heldout_psnr"] = heldout_psnr(gs, ws, held_cams, held_targets, state["slots"], deg)
-        log(f"{mode}: held-out PSNR {results[mode]['heldout_psnr']:.3f} dB after {state['views']} view-iterations")
+        results[mode]["heldout_psnr"] = heldout
         ws.close()
 
     main_mode = "batched"
@@ -706,6 +753,16 @@ def main():
         key_diff = None
 
     check = None
+    quality = None
+    if not args.no_quality:
+        checkpoints = [int(x) for x in args.quality_views.split(",") if x]
+        quality = quality_curve(gs, cams, target_bufs, order, params, sh, held_cams, held_targets, deg, tp, key_flags,
+                                args.batch, args.densify_every, checkpoints)
+        quality["note"] = ("held-out PSNR (8 views not trained on) after equal view-iterations, same start, view "
+                           f"order and densification every {args.densify_every} view-iterations; plain takes one Adam "
+                           f"step per view, batched one per {args.batch} views on their mean gradient (lr_scaled: "
+                           f"rates x sqrt({args.batch}), betas ^{args.batch}); seconds are training time to each "
+                           "checkpoint (eager, default launch shape, deterministic backward)")
     if not args.no_kernel_check:
         log("kernel check: single view, 200k gaussians SH3, 1280x720")
         check = kernel_check(gs)
@@ -759,6 +816,7 @@ def main():
             "order": "(tile << 32) | float bits of z, ties by gaussian index"},
         "depth_key_difference": key_diff,
         "kernel_check": check,
+        "quality_curve": quality,
         "heldout_psnr": {"batched": round(r["heldout_psnr"], 4),
                          "plain": None if plain is None else round(plain["heldout_psnr"], 4),
                          "views": 8, "view_iterations": args.steps * args.batch + args.warmup * args.batch,

@@ -205,7 +205,13 @@ __device__ __forceinline__ void adjoint_view(const gsh::ViewDev& vd, const float
   const float A = U[0][0] * T[0][0] + U[0][1] * T[0][1] + U[0][2] * T[0][2] + kDilation;
   const float B = U[0][0] * T[1][0] + U[0][1] * T[1][1] + U[0][2] * T[1][2];
   const float C = U[1][0] * T[1][0] + U[1][1] * T[1][1] + U[1][2] * T[1][2] + kDilation;
-  const float D = A * C - B * B;
+  // The determinant and the gradient through the inverse in fp64: the conic
+  // of a needle splat is ill-conditioned (A C / D reaches ~100 on a 1M scene),
+  // and in fp32 that alone moved its rotation gradient by ~1e-4 of the
+  // largest rotation gradient of the scene.  The slot loop waits on memory,
+  // so the fp64 arithmetic costs no time.
+  const double Ad = A, Bd = B, Cd = C;
+  const double D = Ad * Cd - Bd * Bd;
 
   // Colour: clamped channels get no gradient; DC.
   const float b0 = 0.28209479177387814f;
@@ -217,11 +223,11 @@ __device__ __forceinline__ void adjoint_view(const gsh::ViewDev& vd, const float
   const float dlogit = g[5] * op * (1.0f - op);
 
   // Conic (C/D, -B/D, A/D) -> dilated covariance.
-  const float ga = g[2], gb = g[3], gc = g[4];
-  const float inv_d2 = fast_rcp(D * D);
-  const float gA = (-C * C * ga + B * C * gb - B * B * gc) * inv_d2;
-  const float gB = (2.0f * B * C * ga - (D + 2.0f * B * B) * gb + 2.0f * A * B * gc) * inv_d2;
-  const float gC = (-B * B * ga + A * B * gb - A * A * gc) * inv_d2;
+  const double ga = g[2], gb = g[3], gc = g[4];
+  const double inv_d2 = 1.0 / (D * D);
+  const float gA = float((-Cd * Cd * ga + Bd * Cd * gb - Bd * Bd * gc) * inv_d2);
+  const float gB = float((2.0 * Bd * Cd * ga - (D + 2.0 * Bd * Bd) * gb + 2.0 * Ad * Bd * gc) * inv_d2);
+  const float gC = float((-Bd * Bd * ga + Ad * Bd * gb - Ad * Ad * gc) * inv_d2);
   const float G2[2][2] = {{gA, 0.5f * gB}, {0.5f * gB, gC}};
   float G2T[2][3];
   for (int r = 0; r < 2; ++r)
@@ -301,8 +307,10 @@ __device__ __forceinline__ void adjoint_view(const gsh::ViewDev& vd, const float
 // on the 16 parameters; densify statistics count every view in which the
 // gaussian was visible.  The view-direction part of dL/dposition was summed
 // over the views by k_adam_sh (one plane per group of SH rest coefficients).
-__global__ void __launch_bounds__(256) k_adam(const gsh::SceneDev* __restrict__ scenes,
-                                              const gsh::ViewDev* __restrict__ views, uint32_t count) {
+// Four waves per SIMD: its fp64 determinant would otherwise take it to 132
+// VGPRs and three.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+k_adam(const gsh::SceneDev* __restrict__ scenes, const gsh::ViewDev* __restrict__ views, uint32_t count) {
   const AdamArgs a = scene_args(scenes[blockIdx.y], views, count);
   const gs_hip_frame& f = a.views[0].f;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -222,8 +222,15 @@ class TrainParams:
 
 
 def make_frame(cam: Camera, slots: int, sh_degree: int, step: int = 0, background=(0.0, 0.0, 0.0), flags: int = 0,
-               tp: TrainParams | None = None) -> Frame:
+               tp: TrainParams | None = None, batch_scaling: int = 1) -> Frame:
+    """One view's frame at optimizer step `step` (0-based).  batch_scaling = B
+    > 1: Adam's hyperparameters for a step on the mean gradient of B views,
+    scaled to stand in for B single-view steps (rates x sqrt(B), beta1^B and
+    beta2^B, bias correction with the scaled betas; the rule of Zhao et al.,
+    "On Scaling Up 3D Gaussian Splatting Training", 2024)."""
     tp = tp or TrainParams()
+    k = max(1, int(batch_scaling))
+    rate = math.sqrt(k)
     f = Frame()
     f.cam = cam
     f.tiles_x = (cam.width + TILE - 1) // TILE
@@ -235,12 +242,13 @@ def make_frame(cam: Camera, slots: int, sh_degree: int, step: int = 0, backgroun
     f.background[:] = [float(b) for b in background]
     f.lambda_dssim = tp.lambda_dssim
     f.ssim_c1, f.ssim_c2 = 0.01 ** 2, 0.03 ** 2
-    f.lr_position, f.lr_scale, f.lr_rotation = tp.lr_position, tp.lr_scale, tp.lr_rotation
-    f.lr_opacity, f.lr_sh_dc, f.lr_sh_rest = tp.lr_opacity, tp.lr_sh_dc, tp.lr_sh_rest
-    f.beta1, f.beta2, f.epsilon = 0.9, 0.999, 1e-15
+    f.lr_position, f.lr_scale, f.lr_rotation = tp.lr_position * rate, tp.lr_scale * rate, tp.lr_rotation * rate
+    f.lr_opacity, f.lr_sh_dc, f.lr_sh_rest = tp.lr_opacity * rate, tp.lr_sh_dc * rate, tp.lr_sh_rest * rate
+    beta1, beta2 = 0.9 ** k, 0.999 ** k
+    f.beta1, f.beta2, f.epsilon = beta1, beta2, 1e-15
     n = step + 1.0
-    f.bias1 = 1.0 / (1.0 - 0.9 ** n)
-    f.bias2 = 1.0 / (1.0 - 0.999 ** n)
+    f.bias1 = 1.0 / (1.0 - beta1 ** n)
+    f.bias2 = 1.0 / (1.0 - beta2 ** n)
     f.grad_threshold, f.clone_max_scale, f.prune_opacity = tp.grad_threshold, tp.clone_max_scale, tp.prune_opacity
     f.prune_max_scale = 0.0
     f.prune_max_radius = 0.0
