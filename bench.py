@@ -226,13 +226,13 @@ VALU_PEAK = 1024 * 2.4e9 / 4  # wave64 VALU instructions per second: 1024 SIMDs,
 
 
 def profile_shape(text):
-    """The launch shape a profile summary was taken at, from the `--shape G,D,T,GRAPH[,BIN]` of its
-    command: (raster groups per CU, depth digit bits, tile digit bits, bin shift); graph replay
-    leaves the kernels as they are and is not part of it."""
-    m = re.search(r"--shape ([0-9]+),([0-9]+),([0-9]+),[0-9]+(?:,([0-9]+))?", text)
+    """The launch shape a profile summary was taken at, from the `--shape G,D,T,GRAPH[,BIN[,STORED]]`
+    of its command: (raster groups per CU, depth digit bits, tile digit bits, bin shift, stored
+    state); graph replay leaves the kernels as they are and is not part of it."""
+    m = re.search(r"--shape ([0-9]+),([0-9]+),([0-9]+),[0-9]+(?:,([0-9]+))?(?:,([0-9]+))?", text)
     if m is None:
         return None
-    return int(m.group(1)), int(m.group(2)), int(m.group(3)), int(m.group(4) or 0)
+    return int(m.group(1)), int(m.group(2)), int(m.group(3)), int(m.group(4) or 0), int(m.group(5) or 0)
 
 
 def profile_row(path, kernel, columns, tuning):
@@ -245,8 +245,11 @@ def profile_row(path, kernel, columns, tuning):
     except OSError:
         return None
     shape = profile_shape(text)
-    if tuning is None or shape is None or shape[3] != tuning["bin_shift"]:
+    if (tuning is None or shape is None or shape[3] != tuning["bin_shift"] or
+            shape[4] != tuning.get("stored_state", 0)):
         return None
+    if kernel == "backward" and tuning.get("stored_state", 0):
+        kernel = "backward_stored"
     m = re.search(r"^\| k_%s \|((?: [0-9.e+]+ \|){%d})" % (re.escape(kernel), columns), text, re.M)
     return [float(x) for x in m.group(1).split("|")[:-1]] if m else None
 
@@ -331,56 +334,73 @@ def multi_scene(gs, args, device):
         return fps, tps
 
     batch = gs.Batch(scenes)
-    # Launch shape: the candidates timed on two untimed (NO_UPDATE) batch steps each.
-    best, best_ms = None, None
-    for shape in ((0, 8, 8, 0, 0), (0, 8, 6, 1, 0), (0, 6, 6, 1, 0), (2, 8, 6, 1, 0), (0, 6, 6, 1, 1),
-                  (0, 6, 6, 1, 2)):
-        batch.set_tuning(raster_groups_per_cu=shape[0], depth_digit_bits=shape[1], tile_digit_bits=shape[2],
-                         graph=shape[3], bin_shift=shape[4])
-        fps, tps = frames_of(0, flags=gs.FLAG_NO_UPDATE)
-        batch.train(fps, tps, want_losses=False)
-        gs.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(2):
-            batch.train(fps, tps, want_losses=False)
-        gs.synchronize()
-        ms = 1000 * (time.perf_counter() - t0) / 2
-        if best_ms is None or ms < best_ms:
-            best, best_ms = shape, ms
-    batch.set_tuning(raster_groups_per_cu=best[0], depth_digit_bits=best[1], tile_digit_bits=best[2], graph=best[3],
-                     bin_shift=best[4])
-    log(f"multi-scene: launch shape {best} ({best_ms:.2f} ms per step while choosing)")
     steps, warmup = args.steps, args.warmup
     pixels = sum(2 * w * h for (w, h) in res)
-    pairs_total = 0
-    for s in range(-warmup, steps):
-        if s == 0:
+    step_no = [0]
+
+    def choose_shape(stored):
+        """The fastest of a few launch shapes, each timed on two untimed (NO_UPDATE) batch steps."""
+        best, best_ms = None, None
+        for shape in ((0, 8, 8, 0, 0), (0, 8, 6, 1, 0), (0, 6, 6, 1, 0), (2, 8, 6, 1, 0), (0, 6, 6, 1, 1),
+                      (0, 6, 6, 1, 2)):
+            batch.set_tuning(raster_groups_per_cu=shape[0], depth_digit_bits=shape[1], tile_digit_bits=shape[2],
+                             graph=shape[3], bin_shift=shape[4], stored_state=stored)
+            fps, tps = frames_of(0, flags=gs.FLAG_NO_UPDATE)
+            batch.train(fps, tps, want_losses=False)
             gs.synchronize()
-            batch.timings(enable=True)
             t0 = time.perf_counter()
-        fps, tps = frames_of(s + warmup)
-        pairs, _ = batch.train(fps, tps, want_losses=False)
-        if s >= 0:
-            pairs_total += sum(pairs)
-        if s >= 0 and (s + 1) % max(1, steps // 4) == 0:
-            log(f"  batch steps: {s + 1}/{steps}")
-    gs.synchronize()
-    batch_s = (time.perf_counter() - t0) / steps
-    events = batch.timings(enable=False)
+            for _ in range(2):
+                batch.train(fps, tps, want_losses=False)
+            gs.synchronize()
+            ms = 1000 * (time.perf_counter() - t0) / 2
+            if best_ms is None or ms < best_ms:
+                best, best_ms = shape, ms
+        batch.set_tuning(raster_groups_per_cu=best[0], depth_digit_bits=best[1], tile_digit_bits=best[2],
+                         graph=best[3], bin_shift=best[4], stored_state=stored)
+        log(f"multi-scene ({'stored-state' if stored else 'recompute'} backward): launch shape {best} "
+            f"({best_ms:.2f} ms per step while choosing)")
+        return best
+
+    def timed_batch(label):
+        """`warmup` + `steps` training steps of the batch; seconds per step, pairs, kernel events."""
+        pairs_total = 0
+        for s in range(-warmup, steps):
+            if s == 0:
+                gs.synchronize()
+                batch.timings(enable=True)
+                t0 = time.perf_counter()
+            fps, tps = frames_of(step_no[0])
+            step_no[0] += 1
+            pairs, _ = batch.train(fps, tps, want_losses=False)
+            if s >= 0:
+                pairs_total += sum(pairs)
+            if s >= 0 and (s + 1) % max(1, steps // 4) == 0:
+                log(f"  {label} batch steps: {s + 1}/{steps}")
+        gs.synchronize()
+        return (time.perf_counter() - t0) / steps, pairs_total, batch.timings(enable=False)
+
+    # The recompute backward (the configuration's), then the stored-state
+    # backward on the same scenes: its per-view bucket states are allocated on
+    # its first step, so the memory in use after each run is that variant's.
+    best = choose_shape(0)
+    batch_s, pairs_total, events = timed_batch("recompute")
     peak = device_used_bytes()
-    # The stored-list alternative keeps every pixel's contributors (gaussian index + alpha, 8 B each) from
-    # the forward for the backward; the recompute backward keeps LAST (4 B) and T (in the image) per pixel.
+    # A stored-list backward would keep every pixel's contributors (gaussian index + alpha, 8 B each) from
+    # the forward; the recompute backward keeps LAST (4 B) and T (in the image) per pixel.
     stored_list, recompute_state = 0, 0
     for ws, n in zip(scenes, counts):
         for v in range(2):
             last = ws.download(gs.BUF_LAST, np.uint32, view=v)
             stored_list += 8 * int(last.astype(np.int64).sum())
             recompute_state += 8 * last.size
+    best_stored = choose_shape(1)
+    stored_s, _, stored_events = timed_batch("stored-state")
+    stored_peak = device_used_bytes()
     batch.close()
     # The same scenes, one scene's 2-view step after another (each scene's own launches).
     for ws in scenes:
         ws.set_tuning(raster_groups_per_cu=best[0], depth_digit_bits=best[1], tile_digit_bits=best[2], graph=best[3],
-                      bin_shift=best[4])
+                      bin_shift=best[4], stored_state=0)
     for s in range(-warmup, steps):
         if s == 0:
             gs.synchronize()
@@ -390,9 +410,11 @@ def multi_scene(gs, args, device):
             ws.train_views(f, t, want_losses=False)
     gs.synchronize()
     seq_s = (time.perf_counter() - t0) / steps
-    kern = {}
+    kern, stored_kern = {}, {}
     for name, ms in events:
         kern[name] = kern.get(name, 0.0) + ms / steps
+    for name, ms in stored_events:
+        stored_kern[name] = stored_kern.get(name, 0.0) + ms / steps
     views = 32
     out = {
         "metric": "multi-scene train iter/s (fwd+bwd+SSIM+Adam): 16 scenes, ~6M gaussians, 2 views per scene per step",
@@ -413,7 +435,17 @@ def multi_scene(gs, args, device):
         "mean_pairs_per_view": int(pairs_total / (steps * views)),
         "memory": {"device_bytes_in_use": peak, "device_bytes_before": used0,
                    "recompute_per_pixel_state_bytes": recompute_state,
-                   "stored_list_per_pixel_bytes": stored_list},
+                   "stored_list_per_pixel_bytes_counted": stored_list},
+        "stored_state": {
+            "note": "the same batch with the stored-state backward (the forward stores each pixel's (C, T) every "
+                    "64 splats of its tile's walk; one wave per 64-splat bucket sweeps the pixels front to back)",
+            "iters_per_s": round(views / stored_s, 3), "ms_per_step": round(1000 * stored_s, 4),
+            "recompute_speedup": round(stored_s / batch_s, 3),
+            "device_bytes_in_use": stored_peak,
+            "bucket_state_bytes": None if stored_peak is None or peak is None else stored_peak - peak,
+            "tuning": {"raster_groups_per_cu": best_stored[0], "depth_digit_bits": best_stored[1],
+                       "tile_digit_bits": best_stored[2], "graph": best_stored[3], "bin_shift": best_stored[4]},
+            "kernels_ms_per_step": {k: round(v, 4) for k, v in sorted(stored_kern.items(), key=lambda kv: -kv[1])}},
         "kernels_ms_per_step": {k: round(v, 4) for k, v in sorted(kern.items(), key=lambda kv: -kv[1])},
     }
     print(json.dumps(out), flush=True)
@@ -566,10 +598,10 @@ def main():
     ap.add_argument("--no-kernel-check", action="store_true",
                     help="skip the single-view 200k SH3 forward/backward kernel check")
     ap.add_argument("--no-autotune", action="store_true", help="default launch shapes instead of the autotuner's")
-    ap.add_argument("--shape", type=str, default=None, metavar="G,D,T,GRAPH[,BIN]",
+    ap.add_argument("--shape", type=str, default=None, metavar="G,D,T,GRAPH[,BIN[,STORED]]",
                     help="fixed launch shape instead of autotuning: raster groups per CU (0 = one per tile), depth "
                          "and tile radix digit bits (tile 0: one counting pass), graph replay 0/1, pair bin shift "
-                         "0..2 (bins of 16 << shift pixels; default 0)")
+                         "0..2 (bins of 16 << shift pixels; default 0), backward 0/1 (1: stored state; default 0)")
     ap.add_argument("--no-quality", action="store_true",
                     help="skip the held-out PSNR curve of plain vs batched training at equal view-iterations")
     ap.add_argument("--quality-views", type=str, default="200,400,800,1600",
@@ -641,7 +673,8 @@ def main():
         if args.shape:
             g, d, t, gr, *rest = (int(x) for x in args.shape.split(","))
             tuning = ws.set_tuning(raster_groups_per_cu=g, depth_digit_bits=d, tile_digit_bits=t, graph=gr,
-                                   bin_shift=rest[0] if rest else 0).as_dict()
+                                   bin_shift=rest[0] if rest else 0,
+                                   stored_state=rest[1] if len(rest) > 1 else 0).as_dict()
             log(f"{mode}: launch shape {tuning} (given)")
         elif not args.no_autotune:
             vs = [int(order[k % len(order)]) for k in range(per_step)]

@@ -303,6 +303,13 @@ typedef struct gs_hip_tuning {
                                     32x32 / 64x64 bins, each raster tile walking its bin's list (fewer pairs to
                                     emit and sort; the same splats drawn at every pixel in the same order, so
                                     the same image and gradients) */
+  uint32_t stored_state;         /* backward: 0 = each pixel re-walks its tile's list back to front from its
+                                    final T (nothing stored per pixel but T, LAST and dL/dC); 1 = the forward
+                                    stores every pixel's (C, T) at each 64th list position and a wave per
+                                    64-splat bucket sweeps the pixels front to back, one splat per lane
+                                    (train_backward_bucket.glsl's scheme; needs per-view state memory of
+                                    4 KB per 64 pairs).  Steps with GS_HIP_FLAG_DETERMINISTIC or coarse bins
+                                    (bin_shift > 0) re-walk. */
 } gs_hip_tuning;
 int gs_hip_workspace_get_tuning(gs_hip_workspace* ws, gs_hip_tuning* out);
 int gs_hip_workspace_set_tuning(gs_hip_workspace* ws, const gs_hip_tuning* tuning);

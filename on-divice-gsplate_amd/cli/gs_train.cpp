@@ -5,7 +5,7 @@
 //            [--init-gaussians 150000] [--opacity-reset N (0 = off)] [--grad-threshold 2e-4]
 //            [--dssim 0.2] [--dssim-end L] [--no-masks] [--sparse-adam] [--preview <dir>] [--eval-every 1000]
 //            [--views-per-step V] [--raster-groups G] [--depth-bits B] [--tile-bits B] [--graph 0|1]
-//            [--bin-shift S] [--autotune]
+//            [--bin-shift S] [--stored-state 0|1] [--autotune]
 //            [--refine-poses] [--pose-lr shift,turn] [--pose-lr-final F] [--pose-from N] [--eval-pose-steps N]
 //            [--pose-out poses.json] [--refine-exposure] (a brightness gain per view and colour channel)
 //
@@ -81,7 +81,7 @@ void usage(const char* argv0) {
                "         [--init-gaussians N] [--opacity-reset N] [--grad-threshold G] [--dssim L] [--dssim-end L]\n"
                "         [--no-masks] [--sparse-adam] [--preview <dir>] [--eval-every N] [--views-per-step V]\n"
                "         [--raster-groups G] [--depth-bits B] [--tile-bits B] [--graph 0|1] [--bin-shift S]\n"
-               "         [--autotune]\n"
+               "         [--stored-state 0|1] [--autotune]\n"
                "         [--refine-poses] [--pose-lr shift,turn] [--pose-lr-final F] [--pose-from N]\n"
                "         [--eval-pose-steps N] [--pose-out poses.json] [--refine-exposure]\n"
                "       %s --synthetic <out.ply> [--gt scene | --gt-random N] [--init-random N] [--views V]\n"
@@ -132,6 +132,7 @@ Args parse(int argc, char** argv) {
     else if (s == "--tile-bits") a.o.tile_digit_bits = std::atoi(next());
     else if (s == "--graph") a.o.graph = std::atoi(next());
     else if (s == "--bin-shift") a.o.bin_shift = std::atoi(next());
+    else if (s == "--stored-state") a.o.stored_state = std::atoi(next());
     else if (s == "--autotune") a.o.autotune = true;
     else if (s == "--refine-poses") a.o.refine_poses = true;
     else if (s == "--pose-lr") {

@@ -64,6 +64,7 @@ int gs_hip_batch_create(gs_hip_batch** out, gs_hip_workspace* const* scenes, uin
   c.tuning.tile_digit_bits = 8;
   c.tuning.graph = 0;
   c.tuning.bin_shift = 0;
+  c.tuning.stored_state = 0;
   uint64_t tiles = 0;
   for (gs_hip_workspace* s : b->scenes) tiles += uint64_t(GS_HIP_MAX_VIEWS) * gsh::tile_count(s);
   int rc = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) == hipSuccess

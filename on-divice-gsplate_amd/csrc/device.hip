@@ -185,6 +185,7 @@ void destroy_view(View* v) {
   release(v->offsets);
   release(v->grad2d);
   release(v->grad_fx);
+  release(v->bucket_state);
   release(v->flags);
   release(v->scan_tmp);
   release(v->pair_tiles);
@@ -258,7 +259,7 @@ uint32_t bins_along(uint32_t pixels, uint32_t shift) {
 }
 
 ViewDev make_view_dev(const View& v, const gs_hip_frame& f, uint32_t* totals, float* tile_loss, const void* target,
-                      bool counting_tile_sort, bool deterministic, uint32_t bin_shift) {
+                      bool counting_tile_sort, bool deterministic, uint32_t bin_shift, bool stored_state) {
   ViewDev d;
   std::memset(&d, 0, sizeof(d));
   d.f = f;
@@ -296,6 +297,8 @@ ViewDev make_view_dev(const View& v, const gs_hip_frame& f, uint32_t* totals, fl
   const float W = float(f.cam.width), H = float(f.cam.height);
   d.lim_x = 1.3f * std::max(f.cam.cx, W - f.cam.cx) / f.cam.fx;
   d.lim_y = 1.3f * std::max(f.cam.cy, H - f.cam.cy) / f.cam.fy;
+  d.bucket_state = stored_state ? reinterpret_cast<float4*>(v.bucket_state) : nullptr;
+  d.bucket_slots = stored_state ? v.bucket_slots : 0;
   return d;
 }
 
@@ -415,6 +418,39 @@ int upload_entries(gs_hip_workspace* ws, const std::vector<TableEntry>& entries,
       }
     }
   }
+  // Stored-state backward: each view's bucket states for its pair buffers
+  // (4 KB per 64 pairs, per tile of a bin), and the launch context's bucket
+  // list (a tile's first bucket has no stored state), made on first use and
+  // grown with the pair buffers.
+  // With coarse bins a tile's walk holds its bin's other tiles' splats, and
+  // a bin's slots would be taken 4^bin_shift times over: those steps re-walk.
+  shape.stored_state = ws->tuning.stored_state != 0u && !shape.deterministic && shift == 0u;
+  if (shape.stored_state) {
+    uint64_t listed = 0;
+    for (const TableEntry& e : entries) {
+      const uint32_t bins = bins_along(e.frame->cam.width, shift) * bins_along(e.frame->cam.height, shift);
+      const uint64_t slots = bucket_slots_for(e.view->pair_capacity, bins, shift);
+      if (e.view->bucket_slots < slots) {
+        GS_HIP_TRY(hipStreamSynchronize(ws->stream));
+        release(e.view->bucket_state);
+        e.view->bucket_slots = 0;
+        const int rc = alloc(&e.view->bucket_state, slots * kBucketPixels * 4);
+        if (rc != 0) return rc;
+        e.view->bucket_slots = slots;
+      }
+      listed += slots + e.view->tile_grid;
+      sig = mix64(mix64(sig, reinterpret_cast<uintptr_t>(e.view->bucket_state)), e.view->bucket_slots);
+    }
+    if (ws->bucket_list_entries < listed) {
+      GS_HIP_TRY(hipStreamSynchronize(ws->stream));
+      release(ws->bucket_list);
+      ws->bucket_list_entries = 0;
+      const int rc = alloc(&ws->bucket_list, listed);
+      if (rc != 0) return rc;
+      ws->bucket_list_entries = listed;
+    }
+    sig = mix64(mix64(sig, reinterpret_cast<uintptr_t>(ws->bucket_list)), ws->bucket_list_entries);
+  }
   if (sig != ws->table_signature || ws->table_signature == 0) {
     const int rc = upload_segments(ws, entries);
     if (rc != 0) return rc;
@@ -431,7 +467,7 @@ int upload_entries(gs_hip_workspace* ws, const std::vector<TableEntry>& entries,
     const TableEntry& e = entries[i];
     host[i] = make_view_dev(*e.view, *e.frame, ws->totals_all + size_t(i) * 16,
                             ws->tile_loss_all + size_t(ws->tile_base[i]) * 4, e.target, shape.counting_tile_sort,
-                            shape.deterministic, shift);
+                            shape.deterministic, shift, shape.stored_state);
   }
   SceneDev* hs = ws->host_scenes + size_t(s) * ws->table_scenes;
   std::copy(scenes.begin(), scenes.end(), hs);
@@ -500,6 +536,8 @@ void release_tables(gs_hip_workspace* ws) {
   release(ws->dev_scan_tmp);
   release(ws->work);
   release(ws->work_counters);
+  release(ws->bucket_list);
+  ws->bucket_list_entries = 0;
   release(ws->totals_all);
   release(ws->tile_loss_all);
   if (ws->host_views != nullptr) hipHostFree(ws->host_views);
@@ -629,6 +667,7 @@ int gs_hip_workspace_create(gs_hip_workspace** out, uint32_t capacity, int sh_de
   ws->tuning.tile_digit_bits = 8;
   ws->tuning.graph = 0;
   ws->tuning.bin_shift = 0;
+  ws->tuning.stored_state = 0;
   const uint64_t cap = std::max<uint32_t>(capacity, 1);
   const uint64_t shn = cap * ws->sh_coeffs * 3;
   const uint64_t tiles = gsh::tile_count(ws);
@@ -853,6 +892,7 @@ int gs_hip_workspace_set_tuning(gs_hip_workspace* ws, const gs_hip_tuning* t) {
   }
   if (t->raster_groups_per_cu > 64) return gsh::set_error(GS_HIP_E_ARGUMENT, "set_tuning: at most 64 groups per CU");
   if (t->bin_shift > 2) return gsh::set_error(GS_HIP_E_ARGUMENT, "set_tuning: bin_shift is 0..2");
+  if (t->stored_state > 1) return gsh::set_error(GS_HIP_E_ARGUMENT, "set_tuning: stored_state is 0 or 1");
   const bool changed = std::memcmp(&ws->tuning, t, sizeof(*t)) != 0;
   ws->tuning = *t;
   if (changed) gsh::drop_graph(ws);

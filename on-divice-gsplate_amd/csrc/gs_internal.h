@@ -137,6 +137,39 @@ struct ViewDev {
   // The projection's ray clamp, 1.3 max(cx, W - cx) / fx and likewise in y,
   // in IEEE single precision like the device expression it replaces.
   float lim_x, lim_y;
+  // Stored-state backward: per-pixel (C rgb, T) at the start of every 64-pair
+  // bucket after a tile's first, kBucketPixels float4 per bucket slot (null
+  // when the step re-walks); see bucket_slot.
+  float4* bucket_state;
+  uint64_t bucket_slots;
+};
+
+// Stored-state backward: list positions per bucket (one wave's splats) and
+// the pixels of a bucket slot (a 16x16 tile, quadrant-major: pixel 64 q + l
+// is lane l of the wave that owns quadrant q).
+constexpr uint32_t kBucketSplats = 64;
+constexpr uint32_t kBucketPixels = 256;
+
+// The slot of bucket k >= 1 of raster tile `tile` walking the list of bin
+// `bin` that starts at pair `range_x`.  Bins' lists partition the pairs, so
+// with L = the bin's length, (range_x / 64 + bin) grows by at least
+// floor(L / 64) + 1 from one bin to the next, more than the L / 64 buckets
+// after the first that its list can need: a bin's slots never meet the
+// next's, and every tile of the bin gets its own column of the 4^bin_shift.
+__host__ __device__ inline uint64_t bucket_slot(uint32_t range_x, uint32_t bin, uint32_t k, uint32_t tile,
+                                                uint32_t tiles_x, uint32_t bin_shift) {
+  const uint32_t mask = (1u << bin_shift) - 1u;
+  const uint32_t sub = ((tile / tiles_x) & mask) << bin_shift | ((tile % tiles_x) & mask);
+  return ((uint64_t(range_x / kBucketSplats) + bin + k - 1u) << (2u * bin_shift)) + sub;
+}
+// Slots a view needs for up to `pairs` pairs in `bins` bins.
+__host__ __device__ inline uint64_t bucket_slots_for(uint64_t pairs, uint32_t bins, uint32_t bin_shift) {
+  return (pairs / kBucketSplats + bins + 1u) << (2u * bin_shift);
+}
+// A bucket of the stored-state backward: which (view, tile) and which 64 pairs.
+struct BucketEntry {
+  uint32_t item;  // view << kTileBits | tile
+  uint32_t k;     // list positions [64 k, 64 k + 64) of the tile's walk
 };
 
 // The bin raster tile `tile` walks.
@@ -176,6 +209,7 @@ struct StepShape {
   bool any_sh = false;             // some scene stores SH rest coefficients
   bool any_pose = false;           // some view asks for its pose gradient
   bool deterministic = false;      // some view asks for GS_HIP_FLAG_DETERMINISTIC (then every view's backward is)
+  bool stored_state = false;       // the stored-state backward runs (tuning.stored_state, not deterministic)
 };
 
 // Radix sort scratch of one view, for up to `items` (key, value) pairs.
@@ -206,6 +240,8 @@ struct View {
   uint32_t* offsets = nullptr;
   float* grad2d = nullptr;
   unsigned long long* grad_fx = nullptr;  // deterministic backward: [capacity][9], made on first use
+  float* bucket_state = nullptr;  // stored-state backward: [bucket_slots][256][4], made on first use
+  uint64_t bucket_slots = 0;
   uint32_t* flags = nullptr;     // densify: per-slot actions
   uint32_t* scan_tmp = nullptr;  // per-slot scan output
   uint64_t pair_capacity = 0;
@@ -267,7 +303,9 @@ struct gs_hip_workspace {
   gsh::ScanSeg* dev_scan_tmp = nullptr;    // [1]: host-count scans (densify)
   uint32_t* work = nullptr;           // rasterizer work items (view << 16 | tile), longest lists first
   uint32_t work_entries = 0;
-  uint32_t* work_counters = nullptr;  // [0] forward, [1] backward (persistent rasterizers)
+  uint32_t* work_counters = nullptr;  // [0] forward, [1] backward (persistent rasterizers), [2] buckets listed, [3] taken
+  gsh::BucketEntry* bucket_list = nullptr;  // stored-state backward: the step's buckets, listed by the forward
+  uint64_t bucket_list_entries = 0;
   uint32_t* totals_all = nullptr;     // [table_views][16]
   float* tile_loss_all = nullptr;     // [tile_loss_entries][4], view v's tiles from tile_base[v]
   uint32_t tile_loss_entries = 0;

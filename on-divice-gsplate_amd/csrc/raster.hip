@@ -59,6 +59,7 @@ using namespace gsd;
 constexpr int kThreads = 256;
 constexpr int kBatch = 256;
 constexpr int kGrad = GS_HIP_GRAD2D_FLOATS;
+constexpr uint32_t kBucketWavesPerCu = 20;  // stored-state backward: one-wave workgroups per CU (LDS allows 21)
 
 struct RasterArgs {
   const gsh::ViewDev* __restrict__ views;
@@ -66,6 +67,10 @@ struct RasterArgs {
   uint32_t* __restrict__ counter;     // persistent mode: next work item
   uint32_t items;                     // every view's tiles
   uint32_t persistent;
+  // Stored-state backward: the forward lists every walked bucket here.
+  gsh::BucketEntry* __restrict__ bucket_list;
+  uint32_t* __restrict__ bucket_counts;  // [0] listed by the forward, [1] taken by the backward
+  uint64_t bucket_entries;
 };
 
 struct PixelCoord {
@@ -166,12 +171,18 @@ __device__ __forceinline__ SplatEval eval_splat(float4 sa, float4 sb, float cx, 
 }
 
 // Front-to-back compositing of one tile of one view, staging B splats per
-// batch (B / 256 per thread).
-template <int B>
-__device__ __forceinline__ void forward_tile(const gsh::ViewDev& a, uint32_t tile, Staging<B>& s) {
+// batch (B / 256 per thread).  kStore (the stored-state backward): at list
+// position 64 k, k >= 1, each wave that still composites stores its pixels'
+// (C, T) in the tile's bucket slot k, and the tile's buckets (as far as its
+// pixels' last contributors reach) are listed for the backward.
+template <int B, bool kStore>
+__device__ __forceinline__ void forward_tile(const gsh::ViewDev& a, uint32_t item, Staging<B>& s,
+                                             const RasterArgs& ra, uint32_t* s_last) {
   const gs_hip_frame& f = a.f;
+  const uint32_t tile = item & gsh::kMaxTilesPerView;
   const PixelCoord pc = pixel_coord(f, tile);
-  const uint2 range = gload(a.ranges + gsh::bin_of_tile(a, pc.tile));
+  const uint32_t bin = gsh::bin_of_tile(a, pc.tile);
+  const uint2 range = gload(a.ranges + bin);
   const float cx = float(pc.px) + 0.5f, cy = float(pc.py) + 0.5f;
   // Ts is T while the pixel composites and -T once it has stopped (or lies
   // outside the image): "not done" is Ts > 0, one compare whose lane mask is
@@ -194,6 +205,14 @@ __device__ __forceinline__ void forward_tile(const gsh::ViewDev& a, uint32_t til
     uint64_t alive = __ballot(Ts > 0.0f);
     const uint32_t base = start - range.x + 1u;  // LAST of splat j is base + j
     for (int chunk = 0; chunk < B / 64 && alive != 0ull; ++chunk) {
+      if constexpr (kStore) {
+        const uint32_t k = (start - range.x) / gsh::kBucketSplats + uint32_t(chunk);
+        if (k != 0u && uint32_t(chunk) * 64u < n) {
+          const uint64_t slot = gsh::bucket_slot(range.x, bin, k, tile, f.tiles_x, a.bin_shift);
+          gstore(a.bucket_state + slot * gsh::kBucketPixels + uint32_t(pc.wave * 64 + pc.lane),
+                 make_float4(cr, cg, cb, Ts));
+        }
+      }
       uint64_t m = uniform_u64(s.mask[pc.wave][chunk]);
       while (m != 0ull && alive != 0ull) {
         const int j = chunk * 64 + __builtin_ctzll(m);
@@ -222,6 +241,21 @@ __device__ __forceinline__ void forward_tile(const gsh::ViewDev& a, uint32_t til
     const uint32_t pix = uint32_t(pc.py) * f.cam.width + uint32_t(pc.px);
     gstore(a.image + pix, make_float4(cr, cg, cb, T));
     gmem(a.last)[pix] = last;
+  }
+  if constexpr (kStore) {
+    const uint32_t wl = wave_max_u32(last);
+    if (pc.lane == 0) s_last[pc.wave] = wl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const uint32_t tile_last = max(max(s_last[0], s_last[1]), max(s_last[2], s_last[3]));
+      const uint32_t buckets = (tile_last + gsh::kBucketSplats - 1u) / gsh::kBucketSplats;
+      if (buckets != 0u) {
+        const uint32_t first = atomicAdd(ra.bucket_counts, buckets);
+        for (uint32_t k = 0; k < buckets; ++k) {
+          if (uint64_t(first) + k < ra.bucket_entries) ra.bucket_list[first + k] = gsh::BucketEntry{item, k};
+        }
+      }
+    }
   }
 }
 
@@ -405,14 +439,15 @@ __device__ __forceinline__ uint32_t xcd_item(uint32_t b, uint32_t items) {
   return kRound * (b / kRound) + 4u * (b % kXcds) + (b / kXcds) % 4u;
 }
 
-template <int B>
+template <int B, bool kStore>
 __global__ void __launch_bounds__(kThreads) k_forward(RasterArgs a) {
   __shared__ Staging<B> s;
   __shared__ uint32_t s_item;
+  __shared__ uint32_t s_last[4];
   uint32_t item = a.persistent != 0u ? next_item(a, &s_item) : xcd_item(blockIdx.x, a.items);
   while (item < a.items) {
     const uint32_t w = a.work[item];
-    if (w != gsh::kNoWork) forward_tile<B>(a.views[w >> gsh::kTileBits], w & gsh::kMaxTilesPerView, s);
+    if (w != gsh::kNoWork) forward_tile<B, kStore>(a.views[w >> gsh::kTileBits], w, s, a, s_last);
     if (a.persistent == 0u) break;
     __syncthreads();
     item = next_item(a, &s_item);
@@ -435,6 +470,197 @@ __global__ void __launch_bounds__(kThreads) k_backward(RasterArgs a) {
   }
 }
 
+// Stored-state backward (train_backward_bucket.glsl's scheme, "Taming 3DGS",
+// Mallick et al. 2024, laid out for wave64): one wave per bucket, i.e. 64
+// consecutive splats of a tile's walk, lane l owning splat 64 k + l.  The
+// tile's pixels that still composite at the bucket's start (LAST > 64 k) in
+// quadrants some splat of the bucket can draw in pass through the lanes as a
+// pipeline, front to back: at step s lane l works on pixel s - l and hands
+// its state to lane l + 1 (DPP wave_shr), so each lane sums its own splat's
+// gradient in registers and nothing is reduced across lanes.  A pixel enters
+// with the state the forward stored at the bucket's start, T (exact: the
+// forward's own recurrence) and R = dL/dC . (colour of everything from here
+// on, background included) = dL/dC . (C_out - C_front); a splat it draws
+// gives
+//   dL/dalpha = T (dL/dC . c) - R' / (1 - alpha),  R' = R - (dL/dC . c) alpha T,
+// and hands on T (1 - alpha) and R'.  The waves share nothing, so a
+// workgroup is one wave; a persistent grid of them takes buckets from the
+// list the forward wrote until it runs out.
+struct BucketShared {
+  union {
+    float4 glast[gsh::kBucketPixels];        // pixel e of the feed: dL/dC, LAST (bits)
+    float grad[gsh::kBucketSplats * kGrad];  // after the sweep: each splat's nine sums
+  };
+  float2 state[gsh::kBucketPixels];   // T, R at the bucket's start
+  float2 centre[gsh::kBucketPixels];  // pixel centre
+  uint32_t g[gsh::kBucketSplats];
+};
+
+__device__ __forceinline__ void backward_bucket(const gsh::ViewDev& a, uint32_t tile, uint32_t k, BucketShared& sh) {
+  const gs_hip_frame& f = a.f;
+  const int lane = int(threadIdx.x);
+  const int tx0 = int(tile % f.tiles_x) * kTile, ty0 = int(tile / f.tiles_x) * kTile;
+  const uint32_t bin = gsh::bin_of_tile(a, tile);
+  const uint2 range = gload(a.ranges + bin);
+  const uint32_t first = k * gsh::kBucketSplats;
+
+  // The tile's pixels, quadrant q in register q (lane = its offset in the quadrant).
+  uint32_t last[4], pix[4];
+  bool inside[4];
+  uint32_t most = 0u;
+  for (int q = 0; q < 4; ++q) {
+    const int px = tx0 + (q & 1) * 8 + (lane & 7), py = ty0 + (q >> 1) * 8 + (lane >> 3);
+    inside[q] = px < int(f.cam.width) && py < int(f.cam.height);
+    pix[q] = inside[q] ? uint32_t(py) * f.cam.width + uint32_t(px) : 0u;
+    last[q] = inside[q] ? gmem(a.last)[pix[q]] : 0u;
+    most = max(most, last[q]);
+  }
+  const uint32_t tile_last = __builtin_amdgcn_readfirstlane(wave_max_u32(most));
+
+  // This lane's splat, in evaluation form (as staged by the tile kernels).
+  const uint32_t pos = first + uint32_t(lane);
+  const bool valid = pos < tile_last;
+  float4 sa = make_float4(0.0f, 0.0f, 0.0f, 0.0f), sb = sa;
+  float blue = 0.0f;
+  uint32_t g = 0u, hits = 0u;
+  if (valid) {
+    g = gmem(a.pair_gaussians)[range.x + pos];
+    const float4* r = a.records + size_t(g) * 3u;
+    const float4 ra = gload(r), rb = gload(r + 1), rc = gload(r + 2);
+    sa = make_float4(ra.x, ra.y, -0.5f * ra.z, -ra.w);
+    sb = make_float4(-0.5f * rb.x, rb.y, rb.z, rb.w);
+    blue = rc.x;
+    const uint32_t lo = __float_as_uint(rc.y), hi = __float_as_uint(rc.z);
+    for (int q = 0; q < 4; ++q) {
+      const int qx = tx0 + (q & 1) * 8, qy = ty0 + (q >> 1) * 8;
+      hits |= bbox_hits(lo, hi, qx, qy, qx + 7, qy + 7) ? (1u << q) : 0u;
+    }
+    if (hits != 0u) {
+      const float limit = __logf(255.0f * rb.y);
+      const float ia = __builtin_amdgcn_rcpf(ra.z), ic = __builtin_amdgcn_rcpf(rb.x);
+      for (int q = 0; q < 4; ++q) {
+        if (((hits >> q) & 1u) == 0u) continue;
+        const float x0 = float(tx0 + (q & 1) * 8) + 0.5f, y0 = float(ty0 + (q >> 1) * 8) + 0.5f;
+        if (!gsh::quadrant_may_draw(ra.z, ra.w, rb.x, ia, ic, limit, ra.x - (x0 + 7.0f), ra.x - x0,
+                                    ra.y - (y0 + 7.0f), ra.y - y0))
+          hits &= ~(1u << q);
+      }
+    }
+  }
+  sh.g[lane] = g;
+
+  // The feed: pixels that still composite at the bucket's start, in quadrants
+  // some splat of the bucket may draw in.
+  const float bg0 = f.background[0], bg1 = f.background[1], bg2 = f.background[2];
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  uint32_t n = 0u;
+  for (int q = 0; q < 4; ++q) {
+    const bool drawn = __ballot((hits >> q) & 1u) != 0ull;
+    const bool active = drawn && last[q] > first;
+    const uint64_t m = __ballot(active);
+    if (active) {
+      const uint32_t e = n + uint32_t(__popcll(m & below));
+      const float4 img = gload(a.image + pix[q]);
+      const float4 d = gload(a.dldc + pix[q]);
+      float4 front = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+      if (k != 0u) {
+        const uint64_t slot = gsh::bucket_slot(range.x, bin, k, tile, f.tiles_x, a.bin_shift);
+        front = gload(a.bucket_state + slot * gsh::kBucketPixels + uint32_t(q * 64 + lane));
+      }
+      const float out_dot = d.x * fmaf(img.w, bg0, img.x) + d.y * fmaf(img.w, bg1, img.y) + d.z * fmaf(img.w, bg2, img.z);
+      const float front_dot = d.x * front.x + d.y * front.y + d.z * front.z;
+      sh.glast[e] = make_float4(d.x, d.y, d.z, __uint_as_float(last[q]));
+      sh.state[e] = make_float2(front.w, out_dot - front_dot);
+      sh.centre[e] = make_float2(float(tx0 + (q & 1) * 8 + (lane & 7)) + 0.5f,
+                                 float(ty0 + (q >> 1) * 8 + (lane >> 3)) + 0.5f);
+    }
+    n += uint32_t(__popcll(m));
+  }
+  __syncthreads();
+
+  // The pipeline: at step s this lane holds pixel s - lane's T and R (shifted
+  // in from the previous lane) and reads the pixel's constants from the feed.
+  float T = 0.0f, R = 0.0f;
+  float S0 = 0.0f, S1 = 0.0f, S2 = 0.0f, S3 = 0.0f, S4 = 0.0f, S5 = 0.0f, S6 = 0.0f, S7 = 0.0f, S8 = 0.0f;
+  const uint32_t steps = n == 0u ? 0u : n + gsh::kBucketSplats - 1u;
+  for (uint32_t s = 0; s < steps; ++s) {
+    const float2 in = s < n ? sh.state[s] : make_float2(0.0f, 0.0f);
+    T = wave_shift_in(in.x, T);
+    R = wave_shift_in(in.y, R);
+    const uint32_t e = s - uint32_t(lane);  // wraps for lanes not reached yet
+    const bool here = e < n;
+    const uint32_t ec = here ? e : 0u;
+    const float4 gl = sh.glast[ec];
+    const float2 c = sh.centre[ec];
+    const SplatEval ev = eval_splat(sa, sb, c.x, c.y);
+    // Non-short-circuit: the feed entry is read whether or not it is this lane's.
+    const bool contributes = valid & here & (pos < __float_as_uint(gl.w)) & (ev.power <= 0.0f) &
+                             (ev.alpha >= kMinAlpha);
+    // Lanes that do not contribute add exact zeros (selects, not products: a
+    // lane between pixels may hold anything).
+    const float gc = fmaf(sb.z, gl.x, fmaf(sb.w, gl.y, blue * gl.z));  // colour . dL/dC
+    const float w = contributes ? ev.alpha * T : 0.0f;
+    const float rest = fmaf(-gc, w, R);
+    const float inv = __builtin_amdgcn_rcpf(1.0f - ev.alpha);
+    const float dL_dalpha = fmaf(T, gc, -rest * inv);
+    const float g5 = contributes && ev.raw < kMaxAlpha ? ev.G * dL_dalpha : 0.0f;
+    const float gx = g5 * ev.dx, gy = g5 * ev.dy;
+    S0 += gx;
+    S1 += gy;
+    S2 = fmaf(gx, ev.dx, S2);
+    S3 = fmaf(gx, ev.dy, S3);
+    S4 = fmaf(gy, ev.dy, S4);
+    S5 += g5;
+    S6 = fmaf(w, gl.x, S6);
+    S7 = fmaf(w, gl.y, S7);
+    S8 = fmaf(w, gl.z, S8);
+    T = contributes ? T * (1.0f - ev.alpha) : T;
+    R = contributes ? rest : R;
+  }
+
+  // The splat's constants applied once (as tile_gradient), then the flush:
+  // consecutive lanes add consecutive (splat, value) entries.
+  __syncthreads();  // the feed is read; its memory takes the sums
+  const float o = sb.y;
+  float* out = sh.grad + lane * kGrad;
+  out[0] = o * fmaf(2.0f * sa.z, S0, sa.w * S1);
+  out[1] = o * fmaf(2.0f * sb.x, S1, sa.w * S0);
+  out[2] = -0.5f * o * S2;
+  out[3] = -o * S3;
+  out[4] = -0.5f * o * S4;
+  out[5] = S5;
+  out[6] = S6;
+  out[7] = S7;
+  out[8] = S8;
+  __syncthreads();
+  const uint32_t splats = min(gsh::kBucketSplats, tile_last > first ? tile_last - first : 0u);
+  for (uint32_t e = uint32_t(lane); e < splats * uint32_t(kGrad); e += 64u) {
+    const uint32_t j = e / uint32_t(kGrad), c = e - j * uint32_t(kGrad);
+    const float v = sh.grad[e];
+    if (v != 0.0f) global_add(gmem(a.grad2d) + size_t(sh.g[j]) * kGrad + c, v);
+  }
+  __syncthreads();
+}
+
+// The next list entry for this wave (one atomic by lane 0, in a scalar register).
+__device__ __forceinline__ uint32_t take_bucket(uint32_t* counter) {
+  uint32_t v = 0u;
+  if (threadIdx.x == 0) v = atomicAdd(counter, 1u);
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+__global__ void __launch_bounds__(64) k_backward_stored(RasterArgs a) {
+  __shared__ BucketShared sh;
+  const uint32_t listed = uint32_t(min(uint64_t(gmem(a.bucket_counts)[0]), a.bucket_entries));
+  for (;;) {
+    const uint32_t taken = take_bucket(a.bucket_counts + 1);
+    if (taken >= listed) break;
+    const gsh::BucketEntry b = a.bucket_list[taken];
+    const uint32_t item = __builtin_amdgcn_readfirstlane(b.item), k = __builtin_amdgcn_readfirstlane(b.k);
+    backward_bucket(a.views[item >> gsh::kTileBits], item & gsh::kMaxTilesPerView, k, sh);
+  }
+}
+
 // Deterministic backward: the fixed-point sums become GRAD2D and are cleared
 // for the next step (they are zero between steps: cleared when made, and
 // after every use).
@@ -453,6 +679,9 @@ RasterArgs make_args(gs_hip_workspace* ws, uint32_t count, int which, uint32_t& 
   a.work = ws->work;
   a.counter = ws->work_counters + which;
   a.items = ws->shape.work_items;
+  a.bucket_list = ws->bucket_list;
+  a.bucket_counts = ws->work_counters + 2;
+  a.bucket_entries = ws->shape.stored_state ? ws->bucket_list_entries : 0;
   (void)count;
   a.persistent = ws->tuning.raster_groups_per_cu > 0 ? 1u : 0u;
   grid = a.persistent ? std::min<uint32_t>(a.items, ws->cus * ws->tuning.raster_groups_per_cu) : a.items;
@@ -472,7 +701,11 @@ int launch_forward(gs_hip_workspace* ws, uint32_t count) {
     if (rc != 0) return rc;
   }
   timer_begin(ws, "forward");
-  k_forward<kBatch><<<grid, kThreads, 0, ws->stream>>>(a);
+  if (ws->shape.stored_state) {
+    k_forward<kBatch, true><<<grid, kThreads, 0, ws->stream>>>(a);
+  } else {
+    k_forward<kBatch, false><<<grid, kThreads, 0, ws->stream>>>(a);
+  }
   GS_CHECK_LAUNCH("k_forward");
   timer_end(ws);
   return 0;
@@ -482,6 +715,17 @@ int launch_backward(gs_hip_workspace* ws, uint32_t count) {
   uint32_t grid = 0;
   const RasterArgs a = make_args(ws, count, 1, grid);
   if (grid == 0) return 0;
+  if (ws->shape.stored_state) {
+    // Waves take buckets from the list the forward wrote (its count was
+    // cleared with the work order) until it runs out.
+    const int rc = fill_u32(ws, a.bucket_counts + 1, 1, 0u);
+    if (rc != 0) return rc;
+    timer_begin(ws, "backward");
+    k_backward_stored<<<ws->cus * kBucketWavesPerCu, 64, 0, ws->stream>>>(a);
+    GS_CHECK_LAUNCH("k_backward_stored");
+    timer_end(ws);
+    return 0;
+  }
   if (a.persistent != 0u) {
     const int rc = fill_u32(ws, a.counter, 1, 0u);
     if (rc != 0) return rc;

@@ -586,7 +586,7 @@ __global__ void __launch_bounds__(1024) k_work_order(const ViewDev* __restrict__
                                                      uint32_t* __restrict__ work, uint32_t* __restrict__ counters) {
   __shared__ uint32_t s_count[33];
   if (threadIdx.x < 33) s_count[threadIdx.x] = 0u;
-  if (threadIdx.x < 2) counters[threadIdx.x] = 0u;
+  if (threadIdx.x < 4) counters[threadIdx.x] = 0u;
   __syncthreads();
   // Most quads fall into a few buckets, so the lanes of a wave that share
   // one add their count once (an LDS atomic from 64 lanes on one counter

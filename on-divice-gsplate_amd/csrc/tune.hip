@@ -2,7 +2,8 @@
 // under candidate shapes and keeps the fastest.  Shapes: persistent
 // rasterizer workgroups per CU (or one workgroup per tile), radix digit
 // widths of the depth and tile sorts (or one counting pass), the pair bin
-// size, and graph replay of the whole step.  Raster tiles stay 16x16, the
+// size, the backward (re-walk or stored state) and graph replay of the whole
+// step.  Raster tiles stay 16x16, the
 // reference's (trainer.h:351); with bin_shift 0 the pairs and ranges are the
 // reference's, with 1 or 2 each tile walks its 32x32 or 64x64 bin's list and
 // draws the same splats in the same order.
@@ -62,6 +63,11 @@ int gs_hip_autotune(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t c
   for (uint32_t shift : {0u, 1u, 2u}) {
     gs_hip_tuning c = cur;
     c.bin_shift = shift;
+    consider(c);
+  }
+  for (uint32_t stored : {0u, 1u}) {
+    gs_hip_tuning c = cur;
+    c.stored_state = stored;
     consider(c);
   }
   for (uint32_t groups : {0u, 1u, 2u, 4u, 8u}) {

@@ -91,6 +91,13 @@ struct Reduce9 {
   }
 };
 
+// Every lane takes the previous lane's v (DPP wave_shr:1); lane 0, which has
+// no previous lane, takes `first`.
+__device__ __forceinline__ float wave_shift_in(float first, float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, first),
+                                                               __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, false));
+}
+
 // A wave-uniform 64-bit value (read from LDS or computed the same in every
 // lane) moved to scalar registers, so loops over it run on the scalar unit.
 __device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {

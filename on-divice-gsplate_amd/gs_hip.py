@@ -60,7 +60,8 @@ class Camera(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("raster_groups_per_cu", ctypes.c_uint32), ("depth_digit_bits", ctypes.c_uint32),
-                ("tile_digit_bits", ctypes.c_uint32), ("graph", ctypes.c_uint32), ("bin_shift", ctypes.c_uint32)]
+                ("tile_digit_bits", ctypes.c_uint32), ("graph", ctypes.c_uint32), ("bin_shift", ctypes.c_uint32),
+                ("stored_state", ctypes.c_uint32)]
 
     def as_dict(self) -> dict:
         return {name: int(getattr(self, name)) for name, _ in self._fields_}
@@ -442,10 +443,11 @@ class Workspace:
         return t
 
     def set_tuning(self, raster_groups_per_cu=None, depth_digit_bits=None, tile_digit_bits=None, graph=None,
-                   bin_shift=None) -> Tuning:
+                   bin_shift=None, stored_state=None) -> Tuning:
         t = self.tuning()
         for name, value in (("raster_groups_per_cu", raster_groups_per_cu), ("depth_digit_bits", depth_digit_bits),
-                            ("tile_digit_bits", tile_digit_bits), ("graph", graph), ("bin_shift", bin_shift)):
+                            ("tile_digit_bits", tile_digit_bits), ("graph", graph), ("bin_shift", bin_shift),
+                            ("stored_state", stored_state)):
             if value is not None:
                 setattr(t, name, int(value))
         check(self.lib.gs_hip_workspace_set_tuning(self.handle, ctypes.byref(t)))

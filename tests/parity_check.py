@@ -158,6 +158,7 @@ def describe(rec, pg, ranges, img, last, g2, o_g2, share, i, w, h):
 
 
 def check_backward(rec, pg, ranges, img, last, dldc, bg, w, h, g2, tiles, label, cap=0.02):
+    assert np.all(np.isfinite(g2)), f"{label}: {int((~np.isfinite(g2)).sum())} non-finite 2D gradient values"
     o_g2, tainted, share = orc.backward32(rec, pg, ranges, img, last, dldc, bg, w, h, flags=True, share=True)
     visible = tiles > 0
     loosened = np.zeros(len(tiles), bool)
@@ -227,3 +228,20 @@ def check_adjoint(gp, gsh, params, sh, cams, deg, g2_views, label, tol=1e-4):
     if r_gsh.size:
         assert np.abs(gsh - r_gsh).max() <= tol * view_max_sh
     print(f"[ok] {label}: adjoint within {tol:g} of each group's largest", flush=True)
+
+
+def check_view(ws, view, pairs, params, sh, cam, target, losses, depth16, label, bg=(0.0, 0.0, 0.0), lam=0.2,
+               deg=3):
+    n = params.shape[0]
+    w, h = cam.width, cam.height
+    state = view_state(ws, n, pairs, view)
+    check_project_bin(state, params, sh, cam, deg, depth16, label)
+    rec, foot, tiles, keys, order, pt, pg, ranges = state
+    img = ws.download(gs.BUF_IMAGE, np.float32, view=view).reshape(-1, 4)
+    last = ws.download(gs.BUF_LAST, np.uint32, w * h, view=view)
+    dldc = ws.download(gs.BUF_DLDC, np.float32, view=view).reshape(-1, 4)
+    g2 = ws.download(gs.BUF_GRAD2D, np.float32, 9 * n, view=view).reshape(n, 9)
+    check_forward(rec, pg, ranges, img, last, w, h, label)
+    check_loss(img, dldc, losses, target, bg, lam, w, h, label)
+    check_backward(rec, pg, ranges, img, last, dldc, bg, w, h, g2, tiles, label)
+    return g2

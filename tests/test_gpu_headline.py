@@ -13,6 +13,8 @@
 * The 16-scene batch (about 6M gaussians, 640x360 / 1280x720 / 1920x1080):
   one view of the smallest scene at each resolution checked as above inside
   a batch step of all 16 scenes.
+The batched cases run with both backwards: the recompute walk and the
+stored-state (bucket) sweep.
 """
 import math
 
@@ -20,8 +22,7 @@ import numpy as np
 import pytest
 
 from helpers import gs, orc
-from parity_check import (check_adjoint, check_backward, check_forward, check_loss, check_project_bin,
-                          first_step_gradients, frozen, view_state)
+from parity_check import check_adjoint, check_view, first_step_gradients, frozen
 
 pytestmark = pytest.mark.gpu
 
@@ -49,23 +50,6 @@ def gt_targets(cams, n=N, deg=DEG, seed=1001):
             img[..., 3] = 255
             out.append(img)
     return out
-
-
-def check_view(ws, view, pairs, params, sh, cam, target, losses, depth16, label, bg=(0.0, 0.0, 0.0), lam=0.2,
-               deg=DEG):
-    n = params.shape[0]
-    w, h = cam.width, cam.height
-    state = view_state(ws, n, pairs, view)
-    check_project_bin(state, params, sh, cam, deg, depth16, label)
-    rec, foot, tiles, keys, order, pt, pg, ranges = state
-    img = ws.download(gs.BUF_IMAGE, np.float32, view=view).reshape(-1, 4)
-    last = ws.download(gs.BUF_LAST, np.uint32, w * h, view=view)
-    dldc = ws.download(gs.BUF_DLDC, np.float32, view=view).reshape(-1, 4)
-    g2 = ws.download(gs.BUF_GRAD2D, np.float32, 9 * n, view=view).reshape(n, 9)
-    check_forward(rec, pg, ranges, img, last, w, h, label)
-    check_loss(img, dldc, losses, target, bg, lam, w, h, label)
-    check_backward(rec, pg, ranges, img, last, dldc, bg, w, h, g2, tiles, label)
-    return g2
 
 
 def test_full_step_1m_plain_float_key_then_densify():
@@ -116,8 +100,13 @@ def test_full_step_1m_plain_float_key_then_densify():
     np.testing.assert_allclose(p_after[splits, 4:7], params[splits, 4:7] - np.log(np.float32(1.6)), atol=1e-6)
 
 
-def test_batched_8_views_1m_depth16():
-    case("1M SH3 1280x720, 8 jittered views in one batched step, 16-bit depth key")
+BACKWARDS = [0, 1]
+BACKWARD_IDS = ["recompute", "stored-state"]
+
+
+@pytest.mark.parametrize("stored", BACKWARDS, ids=BACKWARD_IDS)
+def test_batched_8_views_1m_depth16(stored):
+    case(f"1M SH3 1280x720, 8 jittered views in one batched step, 16-bit depth key, {BACKWARD_IDS[stored]} backward")
     cams = gs.jitter_cameras(8, W, H, 3003)
     targets = gt_targets(cams)
     bufs = [gs.DeviceBuffer(t) for t in targets]
@@ -125,6 +114,7 @@ def test_batched_8_views_1m_depth16():
     try:
         with gs.Workspace(N, DEG, W, H) as ws:
             ws.set_views(8)
+            ws.set_tuning(stored_state=stored)
             ws.load(params, sh)
             frames = [gs.make_frame(c, N, DEG, tp=frozen(), flags=gs.FLAG_DEPTH16) for c in cams]
             pairs, losses = ws.train_views(frames, [b.ptr for b in bufs])
@@ -140,8 +130,10 @@ def test_batched_8_views_1m_depth16():
     check_adjoint(gp, gsh, params, sh, cams, DEG, g2s, "1M batched mean of 8 views")
 
 
-def test_multi_scene_batch_one_view_per_resolution():
-    case("16-scene batch (~6M gaussians): one view of the smallest scene at each resolution")
+@pytest.mark.parametrize("stored", BACKWARDS, ids=BACKWARD_IDS)
+def test_multi_scene_batch_one_view_per_resolution(stored):
+    case(f"16-scene batch (~6M gaussians): one view of the smallest scene at each resolution, "
+         f"{BACKWARD_IDS[stored]} backward")
     rng_seed = 5000
     # The benchmark's workload draw (bench.py multi_scene_workload): counts and resolutions.
     for seed in range(rng_seed, rng_seed + 10000):
@@ -176,6 +168,7 @@ def test_multi_scene_batch_one_view_per_resolution():
             tgts.append((ts, [gs.DeviceBuffer(t) for t in ts]))
             states.append((p, sh) if s in picks.values() else None)
         with gs.Batch(scenes) as batch:
+            batch.set_tuning(stored_state=stored)
             fps = [[gs.make_frame(c, int(n), DEG, tp=frozen(), flags=gs.FLAG_DEPTH16) for c in cs]
                    for n, cs in zip(counts, cams)]
             pairs, losses = batch.train(fps, [[b.ptr for b in t[1]] for t in tgts])
