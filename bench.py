@@ -343,6 +343,8 @@ def multi_scene(gs, args, device):
         best, best_ms = None, None
         for shape in ((0, 8, 8, 0, 0), (0, 8, 6, 1, 0), (0, 6, 6, 1, 0), (2, 8, 6, 1, 0), (0, 6, 6, 1, 1),
                       (0, 6, 6, 1, 2)):
+            if stored and shape[4] != 0:
+                continue  # the stored-state backward runs with the 16 px tiles' own lists only
             batch.set_tuning(raster_groups_per_cu=shape[0], depth_digit_bits=shape[1], tile_digit_bits=shape[2],
                              graph=shape[3], bin_shift=shape[4], stored_state=stored)
             fps, tps = frames_of(0, flags=gs.FLAG_NO_UPDATE)
@@ -385,17 +387,19 @@ def multi_scene(gs, args, device):
     best = choose_shape(0)
     batch_s, pairs_total, events = timed_batch("recompute")
     peak = device_used_bytes()
+    best_stored = choose_shape(1)
+    stored_s, _, stored_events = timed_batch("stored-state")
+    stored_peak = device_used_bytes()
     # A stored-list backward would keep every pixel's contributors (gaussian index + alpha, 8 B each) from
-    # the forward; the recompute backward keeps LAST (4 B) and T (in the image) per pixel.
+    # the forward; the recompute backward keeps LAST (4 B) and T (in the image) per pixel.  Counted from
+    # LAST after the stored-state run, whose 16 px tile lists make LAST a position among the tile's own
+    # splats (in a coarse bin's list it would count the bin's other tiles' splats too).
     stored_list, recompute_state = 0, 0
     for ws, n in zip(scenes, counts):
         for v in range(2):
             last = ws.download(gs.BUF_LAST, np.uint32, view=v)
             stored_list += 8 * int(last.astype(np.int64).sum())
             recompute_state += 8 * last.size
-    best_stored = choose_shape(1)
-    stored_s, _, stored_events = timed_batch("stored-state")
-    stored_peak = device_used_bytes()
     batch.close()
     # The same scenes, one scene's 2-view step after another (each scene's own launches).
     for ws in scenes:
