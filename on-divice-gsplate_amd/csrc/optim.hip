@@ -444,9 +444,8 @@ __device__ __forceinline__ void sh_group(const AdamArgs& a, const gs_hip_frame& 
   auto fetch = [&](uint32_t v) {
     if (v >= a.count) return make_float4(0, 0, 0, 0);
     const gsh::ViewDev& view = a.views[v];
-    const GS_GLOBAL float* g2 = gmem(view.grad2d);
-    return make_float4(g2[size_t(i) * 9u + 6u], g2[size_t(i) * 9u + 7u], g2[size_t(i) * 9u + 8u],
-                       __uint_as_float(gmem(view.misc)[i]));
+    const GS_GLOBAL float* col = gmem(view.grad2d) + size_t(i) * 9u + 6u;
+    return make_float4(col[0], col[1], col[2], __uint_as_float(gmem(view.misc)[i]));
   };
   float4 ahead0 = fetch(0), ahead1 = fetch(1);
 #pragma unroll
@@ -494,20 +493,24 @@ __device__ __forceinline__ void sh_group(const AdamArgs& a, const gs_hip_frame& 
   gstore(a.dir_grad + size_t(kGroup) * a.capacity + i, make_float4(dpx, dpy, dpz, 0.0f));
   if (sparse && !drew) return;  // kept by sparse Adam
   const float inv = 1.0f / float(a.count);
+  // A slot's three channels of a coefficient are stored together per plane,
+  // so each plane's three stores merge into one 12-byte store instruction.
+  auto plane = [&](float* base, int d) { return gmem(base) + (size_t(k0 + d) * a.capacity + i) * 3u; };
   float m[kShGroup][3], vv[kShGroup][3];
   for (int d = 0; d < kShGroup; ++d) {
     if (k0 + d >= coeffs) break;
-    const size_t o = (size_t(k0 + d) * a.capacity + i) * 3u;
-    for (int c = 0; c < 3; ++c) m[d][c] = gmem(a.m_sh)[o + c], vv[d][c] = gmem(a.v_sh)[o + c];
+    for (int c = 0; c < 3; ++c) m[d][c] = plane(a.m_sh, d)[c], vv[d][c] = plane(a.v_sh, d)[c];
   }
   for (int d = 0; d < kShGroup; ++d) {
     if (k0 + d >= coeffs) break;
-    const size_t o = (size_t(k0 + d) * a.capacity + i) * 3u;
-    for (int c = 0; c < 3; ++c) {
-      gmem(a.sh)[o + c] = adam1(rest[d][c], g[d][c] * inv, m[d][c], vv[d][c], f.lr_sh_rest, f);
-      gmem(a.m_sh)[o + c] = m[d][c];
-      gmem(a.v_sh)[o + c] = vv[d][c];
-    }
+    float p[3];
+    for (int c = 0; c < 3; ++c) p[c] = adam1(rest[d][c], g[d][c] * inv, m[d][c], vv[d][c], f.lr_sh_rest, f);
+    GS_GLOBAL float* ps = plane(a.sh, d);
+    GS_GLOBAL float* pm = plane(a.m_sh, d);
+    GS_GLOBAL float* pv = plane(a.v_sh, d);
+    for (int c = 0; c < 3; ++c) ps[c] = p[c];
+    for (int c = 0; c < 3; ++c) pm[c] = m[d][c];
+    for (int c = 0; c < 3; ++c) pv[c] = vv[d][c];
   }
 }
 
