@@ -40,6 +40,12 @@ def test_quadrant_cull():
     _run("test_cull", 120)
 
 
+def test_bucket_slots():
+    """The stored-state backward's bucket slots: one per (bin, raster tile,
+    bucket) within the view's reserve (csrc/gs_internal.h, on the host)."""
+    _run("test_buckets", 120)
+
+
 def test_dataset_loader():
     _run("test_dataset", 120)
 
