@@ -245,10 +245,11 @@ def profile_row(path, kernel, columns, tuning):
     except OSError:
         return None
     shape = profile_shape(text)
-    if (tuning is None or shape is None or shape[3] != tuning["bin_shift"] or
-            shape[4] != tuning.get("stored_state", 0)):
+    # The stored-state backward runs with 16 px bins only (coarser bins re-walk).
+    stored = int(bool(tuning and tuning.get("stored_state", 0)) and tuning["bin_shift"] == 0)
+    if tuning is None or shape is None or shape[3] != tuning["bin_shift"] or int(shape[4] and shape[3] == 0) != stored:
         return None
-    if kernel == "backward" and tuning.get("stored_state", 0):
+    if kernel == "backward" and stored:
         kernel = "backward_stored"
     m = re.search(r"^\| k_%s \|((?: [0-9.e+]+ \|){%d})" % (re.escape(kernel), columns), text, re.M)
     return [float(x) for x in m.group(1).split("|")[:-1]] if m else None
@@ -460,6 +461,10 @@ def multi_scene(gs, args, device):
             t.free()
 
 
+def rnd(v, digits=4):
+    return None if v is None else round(v, digits)
+
+
 def heldout_psnr(gs, ws, cams, targets, slots, deg):
     """Mean PSNR (dB) over held-out views of the workspace's model (float
     render over black against the RGBA8 target's colour)."""
@@ -606,6 +611,9 @@ def main():
                     help="fixed launch shape instead of autotuning: raster groups per CU (0 = one per tile), depth "
                          "and tile radix digit bits (tile 0: one counting pass), graph replay 0/1, pair bin shift "
                          "0..2 (bins of 16 << shift pixels; default 0), backward 0/1 (1: stored state; default 0)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="only the timed steps after the targets are made (no held-out renders, 16 px pair probe "
+                         "or depth-key comparison): for profiler runs")
     ap.add_argument("--no-quality", action="store_true",
                     help="skip the held-out PSNR curve of plain vs batched training at equal view-iterations")
     ap.add_argument("--quality-views", type=str, default="200,400,800,1600",
@@ -691,27 +699,30 @@ def main():
         graph = bool(tuning["graph"])
         state = {"slots": N, "densified": 0, "pairs": 0, "views": 0, "step": 0, "flags": mode_flags,
                  "batch_scaling": per_step if args.batch_scaling == "sqrt" else 1}
-        # Per-kernel HIP events need eager launches: with graph replay chosen,
-        # the timed run replays the graph and a short eager pass after it
-        # times the kernels.
+        # The timed run records no per-kernel events (creating and recording
+        # them is host work inside every step, and graph replay needs the
+        # eager launches they bracket); a short eager pass after it times the
+        # kernels.
         elapsed, steps = run(gs, ws, cams, target_bufs, order, per_step, views_total, warmup_views, deg, tp,
-                             args.densify_every, state, timed=not graph)
+                             args.densify_every, state, timed=False)
         # The model's quality now, before the kernel-timing pass below trains it further.
-        heldout = heldout_psnr(gs, ws, held_cams, held_targets, state["slots"], deg)
-        log(f"{mode}: held-out PSNR {heldout:.3f} dB after {state['views']} view-iterations")
-        if graph:
-            probe = dict(state)
-            probe_views = max(per_step, (views_total // 5 // per_step) * per_step)
-            run(gs, ws, cams, target_bufs, order, per_step, probe_views, 0, deg, tp, 0, probe, timed=True)
-            kernel_steps = probe_views // per_step
-        else:
-            kernel_steps = steps
+        heldout = None if args.no_extras else heldout_psnr(gs, ws, held_cams, held_targets, state["slots"], deg)
+        if heldout is not None:
+            log(f"{mode}: held-out PSNR {heldout:.3f} dB after {state['views']} view-iterations")
+        probe = dict(state)
+        probe_views = max(per_step, (views_total // 5 // per_step) * per_step)
+        run(gs, ws, cams, target_bufs, order, per_step, probe_views, 0, deg, tp, 0, probe, timed=True)
+        kernel_steps = probe_views // per_step
         events = ws.timings(enable=False)
         views_per_s = steps * per_step / elapsed
         mean_bin_pairs = state["pairs"] / (steps * per_step)
         # The byte model counts 16x16 tile pairs; with coarser bins the run's
         # own pair count is lower than that, by the bins' sharing of entries.
-        tile_pairs, probe_bin_pairs = mean_tile_pairs(gs, ws, cams, target_bufs, order, per_step, state, deg, tp)
+        if args.no_extras:  # the run's own pairs (16 px tile pairs only at bin shift 0)
+            tile_pairs = probe_bin_pairs = state["pairs"] / (steps * per_step)
+        else:
+            tile_pairs, probe_bin_pairs = mean_tile_pairs(gs, ws, cams, target_bufs, order, per_step, state, deg,
+                                                          tp)
         mean_pairs = tile_pairs
         slots_mean = 0.5 * (state["slots_start"] + state["slots"])
         tiles_touched = ws.download(gs.BUF_TILES, np.uint32, state["slots"])
@@ -723,7 +734,7 @@ def main():
             "densify_events": state["densified"], "slots_mean": slots_mean, "n_visible": n_visible,
             "kernels": kernel_table(events, kernel_steps, slots_mean, n_visible, mean_pairs, per_step, W * H, coeffs),
             "tuning": tuning, "tune_ms": tune_ms, "overflow_steps": ws.overflows(),
-            "kernel_timing": "eager pass after the timed run" if graph else "the timed run",
+            "kernel_timing": "eager pass after the timed run",
         }
         log(f"{mode}: {views_per_s:.1f} view-iterations/s ({1000 * elapsed / steps:.3f} ms per optimizer step)")
         results[mode]["heldout_psnr"] = heldout
@@ -783,7 +794,7 @@ def main():
                                        "workload": "20k gaussians SH0, 24 orbit views at 360x640, 0.8 L1 + 0.2 "
                                                    "D-SSIM, Adam 1.6e-4 / 5e-3, no densification"}
         log(f"cpu baseline: {cpu['value']} iter/s at the benchmark scene")
-    if args.depth_key == 16:
+    if args.depth_key == 16 and not args.no_extras:
         log("depth-key difference on one view")
         key_diff = depth_key_difference(gs, params, sh, cams[int(order[0])], target_bufs[int(order[0])], deg)
     else:
@@ -849,13 +860,13 @@ def main():
         "plain_float_key": None if "plain_float_key" not in results else {
             "iters_per_s": round(results["plain_float_key"]["views_per_s"], 3),
             "ms_per_step": round(results["plain_float_key"]["ms_per_step"], 4),
-            "heldout_psnr": round(results["plain_float_key"]["heldout_psnr"], 4),
+            "heldout_psnr": rnd(results["plain_float_key"]["heldout_psnr"]),
             "order": "(tile << 32) | float bits of z, ties by gaussian index"},
         "depth_key_difference": key_diff,
         "kernel_check": check,
         "quality_curve": quality,
-        "heldout_psnr": {"batched": round(r["heldout_psnr"], 4),
-                         "plain": None if plain is None else round(plain["heldout_psnr"], 4),
+        "heldout_psnr": {"batched": rnd(r["heldout_psnr"]),
+                         "plain": None if plain is None else rnd(plain["heldout_psnr"]),
                          "views": 8, "view_iterations": args.steps * args.batch + args.warmup * args.batch,
                          "note": "equal view-iterations: the plain run takes one Adam step per view, the batched "
                                  f"run one per {args.batch} views"},

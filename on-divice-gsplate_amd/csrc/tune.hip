@@ -60,14 +60,17 @@ int gs_hip_autotune(gs_hip_workspace* ws, const gs_hip_frame* frames, uint32_t c
   };
   // The bin size first: it changes how many pairs the sorts see, and so which
   // digit widths win.
+  // The backward with it: the stored-state backward runs with 16 px bins only.
   for (uint32_t shift : {0u, 1u, 2u}) {
     gs_hip_tuning c = cur;
     c.bin_shift = shift;
+    c.stored_state = 0u;
     consider(c);
   }
-  for (uint32_t stored : {0u, 1u}) {
+  {
     gs_hip_tuning c = cur;
-    c.stored_state = stored;
+    c.bin_shift = 0u;
+    c.stored_state = 1u;
     consider(c);
   }
   for (uint32_t groups : {0u, 1u, 2u, 4u, 8u}) {
