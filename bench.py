@@ -498,37 +498,46 @@ def depth_key_difference(gs, params, sh, cam, target_buf, deg):
             "grad2d_max_rel": float((np.abs(g16 - g32).max(axis=0) / scale).max())}
 
 
-def quality_curve(gs, cams, target_bufs, order, params, sh, held_cams, held_targets, deg, tp, flags, batch,
+def quality_curve(gs, cams, target_bufs, orders, params, sh, held_cams, held_targets, deg, tp, flags, batch,
                   densify_every, checkpoints):
     """Held-out PSNR of plain (one view per optimizer step) and batched
     (`batch` views per step; Adam's hyperparameters as given, and scaled for
-    the mean of `batch` views) training from the same start, over the same
-    view sequence and densification schedule, at each of `checkpoints`
-    view-iterations, with the training time (held-out renders excluded) to
-    reach it.  Default launch shapes, eager steps, the deterministic backward
-    (so that a rerun gives the same curve)."""
+    the mean of `batch` views) training from the same start with the same
+    densification schedule, at each of `checkpoints` view-iterations, with
+    the training time (held-out renders excluded) to reach it; the mean and
+    the range over the view sequences `orders` (a single run's curve moves by
+    up to a dB when any rounding changes: training is chaotic).  Default
+    launch shapes, eager steps, the deterministic backward (a rerun gives the
+    same curve)."""
     n = params.shape[0]
     w, h = cams[0].width, cams[0].height
-    curve = {"view_iterations": list(checkpoints)}
+    curve = {"view_iterations": list(checkpoints), "orders": len(orders)}
     for mode, per_step, scaling in (("plain", 1, 1), ("batched", batch, 1), ("batched_lr_scaled", batch, batch)):
-        ws = gs.Workspace(2 * n, deg, w, h)
-        ws.set_views(per_step)
-        ws.load(params, sh)
-        state = {"slots": n, "densified": 0, "pairs": 0, "views": 0, "step": 0,
-                 "flags": flags | gs.FLAG_DETERMINISTIC, "batch_scaling": scaling}
-        psnr, seconds, done, total = [], [], 0, 0.0
-        for c in checkpoints:
-            elapsed, _ = run(gs, ws, cams, target_bufs, order, per_step, c - done, 0, deg, tp, densify_every, state,
-                             timed=False)
-            total += elapsed
-            done = c
-            psnr.append(round(heldout_psnr(gs, ws, held_cams, held_targets, state["slots"], deg), 4))
-            seconds.append(round(total, 4))
-        curve[mode + "_db"] = psnr
-        curve[mode + "_s"] = seconds
-        curve[mode + "_gaussians_end"] = state["slots"]
-        ws.close()
-        log(f"quality: {mode} {psnr} dB at {list(checkpoints)} view-iterations ({seconds[-1]:.2f} s)")
+        runs_db, runs_s = [], []
+        for order in orders:
+            ws = gs.Workspace(2 * n, deg, w, h)
+            ws.set_views(per_step)
+            ws.load(params, sh)
+            state = {"slots": n, "densified": 0, "pairs": 0, "views": 0, "step": 0,
+                     "flags": flags | gs.FLAG_DETERMINISTIC, "batch_scaling": scaling}
+            psnr, seconds, done, total = [], [], 0, 0.0
+            for c in checkpoints:
+                elapsed, _ = run(gs, ws, cams, target_bufs, order, per_step, c - done, 0, deg, tp, densify_every,
+                                 state, timed=False)
+                total += elapsed
+                done = c
+                psnr.append(heldout_psnr(gs, ws, held_cams, held_targets, state["slots"], deg))
+                seconds.append(total)
+            ws.close()
+            runs_db.append(psnr)
+            runs_s.append(seconds)
+        db = np.asarray(runs_db)
+        curve[mode + "_db"] = [round(float(x), 3) for x in db.mean(axis=0)]
+        curve[mode + "_db_min"] = [round(float(x), 3) for x in db.min(axis=0)]
+        curve[mode + "_db_max"] = [round(float(x), 3) for x in db.max(axis=0)]
+        curve[mode + "_s"] = [round(float(x), 4) for x in np.asarray(runs_s).mean(axis=0)]
+        log(f"quality: {mode} {curve[mode + '_db']} dB (range {curve[mode + '_db_min']} .. {curve[mode + '_db_max']}) "
+            f"at {list(checkpoints)} view-iterations ({curve[mode + '_s'][-1]:.2f} s)")
     return curve
 
 
@@ -616,7 +625,7 @@ def main():
                          "or depth-key comparison): for profiler runs")
     ap.add_argument("--no-quality", action="store_true",
                     help="skip the held-out PSNR curve of plain vs batched training at equal view-iterations")
-    ap.add_argument("--quality-views", type=str, default="200,400,800,1600",
+    ap.add_argument("--quality-views", type=str, default="200,400,800,1600,3200",
                     help="view-iteration checkpoints of the quality curve")
     ap.add_argument("--batch-scaling", choices=("none", "sqrt"), default="none",
                     help="batched steps: Adam's hyperparameters as given (none) or scaled for the mean of the "
@@ -804,10 +813,12 @@ def main():
     quality = None
     if not args.no_quality:
         checkpoints = [int(x) for x in args.quality_views.split(",") if x]
-        quality = quality_curve(gs, cams, target_bufs, order, params, sh, held_cams, held_targets, deg, tp, key_flags,
+        orders = [order] + [np.random.default_rng(seed).permutation(args.views) for seed in (3004, 3005)]
+        quality = quality_curve(gs, cams, target_bufs, orders, params, sh, held_cams, held_targets, deg, tp, key_flags,
                                 args.batch, args.densify_every, checkpoints)
-        quality["note"] = ("held-out PSNR (8 views not trained on) after equal view-iterations, same start, view "
-                           f"order and densification every {args.densify_every} view-iterations; plain takes one Adam "
+        quality["note"] = ("held-out PSNR (8 views not trained on) after equal view-iterations, mean and range over "
+                           "3 view orders, same start and densification every "
+                           f"{args.densify_every} view-iterations; plain takes one Adam "
                            f"step per view, batched one per {args.batch} views on their mean gradient (lr_scaled: "
                            f"rates x sqrt({args.batch}), betas ^{args.batch}); seconds are training time to each "
                            "checkpoint (eager, default launch shape, deterministic backward)")
