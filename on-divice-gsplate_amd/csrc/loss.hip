@@ -72,9 +72,10 @@ __device__ __forceinline__ int halo_pixel(const gs_hip_frame& f, int e, int ox, 
   return in ? gy * int(f.cam.width) + gx : -1;
 }
 
-// Tile of block b among n: consecutive blocks go to the 8 XCDs in turn, so XCD x
-// gets the contiguous run of tiles [x q + min(x, r), ...) (q = n / 8, r = n % 8)
-// and neighbouring tiles' halos meet in one L2.
+// Tile of block b among n (a bijection on [0, n)): consecutive blocks go to
+// the 8 XCDs in turn, so XCD x gets the contiguous run of tiles
+// [x q + min(x, r), ...) (q = n / 8, r = n % 8) and neighbouring tiles' halos
+// meet in one L2.
 __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
   const uint32_t x = b & 7u, idx = b >> 3, q = n >> 3, r = n & 7u;
   return x * q + min(x, r) + idx;
@@ -133,8 +134,11 @@ __device__ void exposure_fit(const gsh::ViewDev& view, uint32_t tile, int pix) {
 // One tile (blockIdx.x) of one view (blockIdx.y).
 __global__ void __launch_bounds__(256) k_ssim(const gsh::ViewDev* __restrict__ views) {
   const gsh::ViewDev& view = views[blockIdx.y];
-  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
-  if (tile >= view.tile_count) return;  // a smaller view of the launch: the whole block leaves
+  // A smaller view of the launch leaves its extra blocks idle; its own tiles
+  // are dealt over all 8 XCDs (with the launch's width they would all fall
+  // on the first XCDs' runs).
+  if (blockIdx.x >= view.tile_count) return;  // the whole block leaves
+  const uint32_t tile = xcd_tile(blockIdx.x, view.tile_count);
   const gs_hip_frame& f = view.f;
   const LossArgs a = loss_args(view);
   __shared__ float s_x[3][kHalo * kStride];
@@ -241,8 +245,11 @@ __global__ void __launch_bounds__(256) k_ssim(const gsh::ViewDev* __restrict__ v
 
 __global__ void __launch_bounds__(256) k_loss_grad(const gsh::ViewDev* __restrict__ views) {
   const gsh::ViewDev& view = views[blockIdx.y];
-  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
-  if (tile >= view.tile_count) return;  // a smaller view of the launch: the whole block leaves
+  // A smaller view of the launch leaves its extra blocks idle; its own tiles
+  // are dealt over all 8 XCDs (with the launch's width they would all fall
+  // on the first XCDs' runs).
+  if (blockIdx.x >= view.tile_count) return;  // the whole block leaves
+  const uint32_t tile = xcd_tile(blockIdx.x, view.tile_count);
   const gs_hip_frame& f = view.f;
   const LossArgs a = loss_args(view);
   __shared__ float s_m[3][kHalo * kStride];
