@@ -39,19 +39,27 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// "synthetic[:N]" names the torus-knot demo scene of N splats (100000 when
+// no count is given); anything else is a .ply or .spz file.
+gs::SplatScene scene_from(const std::string& input) {
+  const std::string tag = "synthetic";
+  if (input.compare(0, tag.size(), tag) != 0) {
+    const std::vector<uint8_t> bytes = gs::read_file(input);
+    return gs::load_splat_file(bytes.data(), bytes.size());
+  }
+  const size_t colon = input.find(':');
+  const uint32_t n = colon == std::string::npos ? 100000u : uint32_t(std::strtoul(input.c_str() + colon + 1, nullptr, 10));
+  return gs::make_synthetic_scene(n);
+}
+
+// The scene and one line about it (the count, SH degree, axis convention,
+// antialiasing flag and load time, as the reference's viewer reports them).
 gs::SplatScene load_scene(const std::string& input) {
   const auto t0 = std::chrono::steady_clock::now();
-  gs::SplatScene scene;
-  if (input.rfind("synthetic", 0) == 0) {
-    const uint32_t n = input.size() > 10 ? uint32_t(std::atoi(input.c_str() + 10)) : 100000u;
-    scene = gs::make_synthetic_scene(n);
-  } else {
-    const std::vector<uint8_t> bytes = gs::read_file(input);
-    scene = gs::load_splat_file(bytes.data(), bytes.size());
-  }
-  std::printf("loaded %u splats, sh degree %d, %s, antialiased %d (%.0f ms)\n", scene.count, scene.sh_degree,
-              scene.convention == gs::AxisConvention::RDF ? "RDF" : "RUB", scene.antialiased ? 1 : 0,
-              ms_since(t0));
+  gs::SplatScene scene = scene_from(input);
+  const char* axes = scene.convention == gs::AxisConvention::RDF ? "RDF" : "RUB";
+  std::printf("loaded %u splats, sh degree %d, %s, antialiased %d (%.0f ms)\n", scene.count, scene.sh_degree, axes,
+              int(scene.antialiased), ms_since(t0));
   return scene;
 }
 
