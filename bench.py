@@ -541,6 +541,52 @@ def quality_curve(gs, cams, target_bufs, orders, params, sh, held_cams, held_tar
     return curve
 
 
+def small_scene_gpu(gs):
+    """The small-scene configuration in full on the GPU (the CPU baseline's
+    workload): 20k gaussians SH0, 24 orbit views at 360x640, 300 plain steps
+    (one view per optimizer step, in view order), 0.8 L1 + 0.2 D-SSIM, Adam
+    at 1.6e-4 / 5e-3, no densification; targets rendered on the GPU from the
+    seeded ground truth.  Returns its iterations/s and mean PSNR over the 24
+    views before and after."""
+    W, H, N, deg = 360, 640, 20_000, 0
+    cams = gs.orbit_cameras(24, W, H)
+    gt_p, gt_sh = gs.synthetic_scene(N, deg, 1001)
+    with gs.Workspace(N, deg, W, H) as gt:
+        gt.load(gt_p, gt_sh)
+        targets = []
+        for cam in cams:
+            img = gt.render(gs.make_frame(cam, N, deg))
+            img[..., 3] = 255
+            targets.append(img)
+    bufs = [gs.DeviceBuffer(t) for t in targets]
+    p0, s0 = gs.synthetic_scene(N, deg, 2002)
+    tp = gs.TrainParams()
+
+    def mean_psnr(ws):
+        out = []
+        for cam, t in zip(cams, targets):
+            ws.render(gs.make_frame(cam, N, deg))
+            img = ws.download(gs.BUF_IMAGE, np.float32).reshape(-1, 4)[:, :3].astype(np.float64)
+            out.append(-10.0 * math.log10(max(float(np.mean((img - t.reshape(-1, 4)[:, :3] / 255.0) ** 2)), 1e-12)))
+        return float(np.mean(out))
+
+    with gs.Workspace(N, deg, W, H) as ws:
+        ws.load(p0, s0)
+        before = mean_psnr(ws)
+        frames = [gs.make_frame(cams[s % 24], N, deg, step=s, tp=tp) for s in range(300)]
+        gs.synchronize()
+        t0 = time.perf_counter()
+        for s in range(300):
+            ws.set_target_device(bufs[s % 24].ptr)
+            ws.train_step(frames[s], want_losses=False)
+        gs.synchronize()
+        dt = time.perf_counter() - t0
+        after = mean_psnr(ws)
+    for b in bufs:
+        b.free()
+    return dt, before, after
+
+
 def kernel_check(gs, repeats=20):
     """The single-view forward + backward kernel check: 200k gaussians of the
     benchmark distributions at SH degree 3, one camera on the radius-3 orbit
@@ -822,7 +868,15 @@ def main():
                            f"step per view, batched one per {args.batch} views on their mean gradient (lr_scaled: "
                            f"rates x sqrt({args.batch}), betas ^{args.batch}); seconds are training time to each "
                            "checkpoint (eager, default launch shape, deterministic backward)")
+    small = None
     if not args.no_kernel_check:
+        log("small scene on the GPU: 20k SH0 360x640, 24 views, 300 plain steps")
+        ds, p0, p1 = small_scene_gpu(gs)
+        small = {"workload": "20k gaussians SH0, 24 orbit views at 360x640, 300 plain steps (one view per Adam "
+                             "step), 0.8 L1 + 0.2 D-SSIM, Adam 1.6e-4 / 5e-3, no densification; default launch shape",
+                 "seconds": round(ds, 4), "iters_per_s": round(300 / ds, 1), "psnr_start": round(p0, 4),
+                 "psnr_end": round(p1, 4)}
+        log(f"small scene on the GPU: {300 / ds:.0f} it/s, {p0:.2f} -> {p1:.2f} dB")
         log("kernel check: single view, 200k gaussians SH3, 1280x720")
         check = kernel_check(gs)
         log(f"kernel check: forward {check['forward']['mpix_per_s']} Mpix/s, backward "
@@ -875,6 +929,7 @@ def main():
             "order": "(tile << 32) | float bits of z, ties by gaussian index"},
         "depth_key_difference": key_diff,
         "kernel_check": check,
+        "small_scene": small,
         "quality_curve": quality,
         "heldout_psnr": {"batched": rnd(r["heldout_psnr"]),
                          "plain": None if plain is None else rnd(plain["heldout_psnr"]),
