@@ -102,6 +102,10 @@ int gs_hip_trainer_create(const char* dataset_dir, const gs_hip_trainer_options*
     t.sparse_adam = o.iterations >= 2000;
     t.views_per_step = std::max<uint32_t>(1, o.views_per_step);
     t.seed = o.seed;
+    // The launch shape and backward timed on the first training views (the
+    // session steps one view at a time by default, where the stored-state
+    // backward fills the GPU better than the re-walk).
+    t.autotune = true;
     gs::InitOptions init;
     init.max_gaussians = std::min<uint32_t>(t.max_gaussians / 2, 150000);
     init.subject_only = t.masked;
