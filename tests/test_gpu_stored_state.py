@@ -128,3 +128,37 @@ def test_deterministic_steps_take_the_rewalk():
     for (ai, al, ag), (bi, bl, bg) in zip(a_first, b_first):
         np.testing.assert_array_equal(ag.view(np.uint32), bg.view(np.uint32))
     np.testing.assert_array_equal(a_after.view(np.uint32), b_after.view(np.uint32))
+
+
+def test_stored_state_overflow_retry_matches_a_step_with_room():
+    """A step whose pairs overflow the buffers runs again with larger pair
+    buffers and with bucket states grown to match (the forward of the first
+    attempt wrote states for a smaller reserve): the result is that of a step
+    that had room from the start."""
+    case("stored-state backward, pair overflow: the first step outgrows its buffers")
+    n, deg, w, h = 6000, 0, 512, 384
+    params, _ = make_scene(n, deg, 73)
+    params = params.copy()
+    params[:, 4:7] += 1.5  # large splats: many tiles each
+    cam = make_camera(w, h, 1.2)
+    target = gs.DeviceBuffer(smooth_target(w, h, 73))
+    results = []
+    try:
+        for reserve in (None, 4_000_000):
+            with gs.Workspace(n, deg, w, h) as ws:
+                ws.set_tuning(stored_state=1)
+                if reserve:
+                    ws.reserve_pairs(reserve)
+                ws.load(params, None)
+                frame = gs.make_frame(cam, n, deg, tp=gs.TrainParams(lambda_dssim=0.2), step=1)
+                pairs, losses = ws.train_views([frame], [target.ptr])
+                results.append((pairs[0], losses[0], ws.overflows(),
+                                ws.download(gs.BUF_GRAD2D, np.float32, 9 * n).reshape(n, 9)))
+    finally:
+        target.free()
+    (p0, l0, o0, g0), (p1, l1, o1, g1) = results
+    print(f"[info] pairs {p0}, overflow steps {o0} / {o1}", flush=True)
+    assert p0 == p1 and o0 >= 1 and o1 == 0
+    np.testing.assert_allclose(l0, l1, rtol=1e-6)
+    scale = np.abs(g1).max(axis=0) + 1e-30
+    assert (np.abs(g0 - g1) / scale).max() <= 2e-5
