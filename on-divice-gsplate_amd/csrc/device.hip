@@ -259,7 +259,8 @@ uint32_t bins_along(uint32_t pixels, uint32_t shift) {
 }
 
 ViewDev make_view_dev(const View& v, const gs_hip_frame& f, uint32_t* totals, float* tile_loss, const void* target,
-                      bool counting_tile_sort, bool deterministic, uint32_t bin_shift, bool stored_state) {
+                      bool counting_tile_sort, bool deterministic, uint32_t bin_shift, bool stored_state,
+                      const uint32_t* step_overflow) {
   ViewDev d;
   std::memset(&d, 0, sizeof(d));
   d.f = f;
@@ -299,6 +300,7 @@ ViewDev make_view_dev(const View& v, const gs_hip_frame& f, uint32_t* totals, fl
   d.lim_y = 1.3f * std::max(f.cam.cy, H - f.cam.cy) / f.cam.fy;
   d.bucket_state = stored_state ? reinterpret_cast<float4*>(v.bucket_state) : nullptr;
   d.bucket_slots = stored_state ? v.bucket_slots : 0;
+  d.step_overflow = step_overflow;
   return d;
 }
 
@@ -467,7 +469,8 @@ int upload_entries(gs_hip_workspace* ws, const std::vector<TableEntry>& entries,
     const TableEntry& e = entries[i];
     host[i] = make_view_dev(*e.view, *e.frame, ws->totals_all + size_t(i) * 16,
                             ws->tile_loss_all + size_t(ws->tile_base[i]) * 4, e.target, shape.counting_tile_sort,
-                            shape.deterministic, shift, shape.stored_state);
+                            shape.deterministic, shift, shape.stored_state,
+                            ws->work_counters + kStepOverflowCounter);
   }
   SceneDev* hs = ws->host_scenes + size_t(s) * ws->table_scenes;
   std::copy(scenes.begin(), scenes.end(), hs);

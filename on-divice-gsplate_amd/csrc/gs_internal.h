@@ -142,7 +142,14 @@ struct ViewDev {
   // when the step re-walks); see bucket_slot.
   float4* bucket_state;
   uint64_t bucket_slots;
+  // The launch context's flag (k_work_order's): some view of the step found
+  // more pairs than its buffers hold.
+  const uint32_t* step_overflow;
 };
+
+// The launch context's work counter that k_work_order sets to the step's
+// pair-overflow flag ([0..1] raster work, [2..3] buckets).
+constexpr uint32_t kStepOverflowCounter = 4;
 
 // Stored-state backward: list positions per bucket (one wave's splats) and
 // the pixels of a bucket slot (a 16x16 tile, quadrant-major: pixel 64 q + l
@@ -303,7 +310,8 @@ struct gs_hip_workspace {
   gsh::ScanSeg* dev_scan_tmp = nullptr;    // [1]: host-count scans (densify)
   uint32_t* work = nullptr;           // rasterizer work items (view << 16 | tile), longest lists first
   uint32_t work_entries = 0;
-  uint32_t* work_counters = nullptr;  // [0] forward, [1] backward (persistent rasterizers), [2] buckets listed, [3] taken
+  uint32_t* work_counters = nullptr;  // [0] forward, [1] backward (persistent rasterizers), [2] buckets listed, [3] taken,
+                                      // [4] the step's pair overflow
   gsh::BucketEntry* bucket_list = nullptr;  // stored-state backward: the step's buckets, listed by the forward
   uint64_t bucket_list_entries = 0;
   uint32_t* totals_all = nullptr;     // [table_views][16]

@@ -71,6 +71,9 @@ __device__ __forceinline__ AdamArgs scene_args(const gsh::SceneDev& sc, const gs
 // runs the step again with larger buffers, trainer.cpp:819-836).
 __device__ __forceinline__ bool skip_update(const AdamArgs& a) {
   if ((a.views[0].f.flags & GS_HIP_FLAG_NO_UPDATE) != 0u) return true;
+  // The step's flag (set by k_work_order, which runs after the pair counts
+  // are known): one load instead of a dependent chain over every view.
+  if (a.views[0].step_overflow != nullptr) return *gmem(a.views[0].step_overflow) != 0u;
   for (uint32_t v = 0; v < a.step_count; ++v) {
     if (uint64_t(a.step_views[v].totals[0]) > a.step_views[v].pair_capacity) return true;
   }

@@ -585,9 +585,17 @@ __device__ __forceinline__ uint32_t quad_bucket(const ViewDev& view, uint32_t q)
 __global__ void __launch_bounds__(1024) k_work_order(const ViewDev* __restrict__ views, uint32_t count,
                                                      uint32_t* __restrict__ work, uint32_t* __restrict__ counters) {
   __shared__ uint32_t s_count[33];
+  __shared__ uint32_t s_over;
   if (threadIdx.x < 33) s_count[threadIdx.x] = 0u;
   if (threadIdx.x < 4) counters[threadIdx.x] = 0u;
+  if (threadIdx.x == 0) s_over = 0u;
   __syncthreads();
+  // Whether any view of the step found more pairs than its buffers hold: the
+  // optimizer then leaves every model alone (one flag instead of every view's
+  // count read by every optimizer thread).
+  for (uint32_t v = threadIdx.x; v < count; v += blockDim.x) {
+    if (uint64_t(views[v].totals[0]) > views[v].pair_capacity) atomicOr(&s_over, 1u);
+  }
   // Most quads fall into a few buckets, so the lanes of a wave that share
   // one add their count once (an LDS atomic from 64 lanes on one counter
   // would take 64 passes).
@@ -603,6 +611,7 @@ __global__ void __launch_bounds__(1024) k_work_order(const ViewDev* __restrict__
   }
   __syncthreads();
   if (threadIdx.x == 0) {
+    counters[gsh::kStepOverflowCounter] = s_over;
     uint32_t sum = 0;
     for (int b = 0; b < 33; ++b) {
       const uint32_t c = s_count[b];
