@@ -69,8 +69,7 @@ __device__ __forceinline__ AdamArgs scene_args(const gsh::SceneDev& sc, const gs
 // The step leaves the model alone when asked to, or when any view of the step
 // found more pairs than its buffers hold (its render was cut short; the host
 // runs the step again with larger buffers, trainer.cpp:819-836).
-__device__ __forceinline__ bool skip_update(const AdamArgs& a) {
-  if ((a.views[0].f.flags & GS_HIP_FLAG_NO_UPDATE) != 0u) return true;
+__device__ __forceinline__ bool step_overflowed(const AdamArgs& a) {
   // The step's flag (set by k_work_order, which runs after the pair counts
   // are known): one load instead of a dependent chain over every view.
   if (a.views[0].step_overflow != nullptr) return *gmem(a.views[0].step_overflow) != 0u;
@@ -78,6 +77,10 @@ __device__ __forceinline__ bool skip_update(const AdamArgs& a) {
     if (uint64_t(a.step_views[v].totals[0]) > a.step_views[v].pair_capacity) return true;
   }
   return false;
+}
+
+__device__ __forceinline__ bool skip_update(const AdamArgs& a) {
+  return (a.views[0].f.flags & GS_HIP_FLAG_NO_UPDATE) != 0u || step_overflowed(a);
 }
 
 // The gradient side of the optimizer (adjoint, SH gradients, Adam's step)
@@ -373,10 +376,7 @@ __global__ void __launch_bounds__(256) k_pose_grad(const gsh::SceneDev* __restri
                                                    const gsh::ViewDev* __restrict__ views, uint32_t count) {
   const AdamArgs a = scene_args(scenes[blockIdx.y], views, count);
   if (blockIdx.x * 256u >= a.capacity) return;  // past this scene's slots: the whole block
-  bool overflow = false;
-  for (uint32_t v = 0; v < a.step_count; ++v) {
-    overflow = overflow || uint64_t(a.step_views[v].totals[0]) > a.step_views[v].pair_capacity;
-  }
+  const bool overflow = step_overflowed(a);
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   bool live = !overflow && i < a.views[0].f.slots;
   float4 p0 = make_float4(0, 0, 0, 0), p1 = p0, q = p0;
