@@ -694,7 +694,7 @@ int gs_hip_workspace_create(gs_hip_workspace** out, uint32_t capacity, int sh_de
   A(alloc(&ws->m_sh, shn));
   A(alloc(&ws->v_sh, shn));
   A(alloc(&ws->stats, cap * 4));
-  if (ws->sh_coeffs > 0) A(alloc(&ws->dir_grad, cap * 3 * 4));
+  if (ws->sh_coeffs > 0) A(alloc(&ws->dir_grad, cap * uint64_t(gsh::kShGroups) * 4));
   if (tiles > gsh::kMaxTilesPerView) A(gsh::set_error(GS_HIP_E_ARGUMENT, "workspace_create: more than 65535 tiles"));
   A(gsh::alloc_tables(ws, GS_HIP_MAX_VIEWS, 1, uint32_t(tiles) * GS_HIP_MAX_VIEWS));
   if (rc == 0) {

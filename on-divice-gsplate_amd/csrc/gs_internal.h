@@ -25,6 +25,12 @@
 
 namespace gsh {
 
+// The SH optimizer's split of the SH rest coefficients: one wave per group
+// of kShGroup coefficients of 64 slots (its planes, colour gradients and
+// Adam moments in registers), kShGroups groups at degree 3.
+constexpr int kShGroup = 5;
+constexpr int kShGroups = (GS_HIP_MAX_SH_REST + kShGroup - 1) / kShGroup;
+
 // Thread-local last error; every C entry point returns its code.
 int set_error(int code, const std::string& message);
 int hip_error(hipError_t e, const char* what);
@@ -84,7 +90,7 @@ struct SceneDev {
   float* m_sh;
   float* v_sh;
   float4* stats;
-  float4* dir_grad;  // [3][capacity]: per group of 5 SH rest coefficients, its view-direction part of dL/dposition
+  float4* dir_grad;  // [kShGroups][capacity]: per group of kShGroup SH rest coefficients, its view-direction part of dL/dposition
   uint32_t capacity;
   uint32_t sh_coeffs;
   uint32_t first_view;
@@ -297,7 +303,7 @@ struct gs_hip_workspace {
   float* m_sh = nullptr;
   float* v_sh = nullptr;
   float* stats = nullptr;
-  float* dir_grad = nullptr;  // [3][capacity][4] (SH rest stored)
+  float* dir_grad = nullptr;  // [kShGroups][capacity][4] (SH rest stored)
   std::vector<gsh::View*> views;  // views[0] always exists (a batch's launch context borrows its scenes' views)
   bool borrowed_views = false;
   // device tables: ViewDev[table_views], SceneDev[table_scenes] and the batched sort / scan segments

@@ -38,7 +38,7 @@ struct AdamArgs {
   float* __restrict__ m_sh;
   float* __restrict__ v_sh;
   float4* __restrict__ stats;
-  float4* __restrict__ dir_grad;           // [3][capacity]
+  float4* __restrict__ dir_grad;           // [kShGroups][capacity]
   const gsh::ViewDev* __restrict__ views;  // the scene's views; views[0].f holds the optimizer's rates
   uint32_t capacity;
   uint32_t sh_coeffs;
@@ -114,8 +114,8 @@ __device__ __forceinline__ void add4(float4& a, float x, float y, float z, float
 }
 
 // SH rest coefficients per k_adam_sh thread, and the groups of 15.
-constexpr int kShGroup = 5;
-constexpr int kShGroups = (GS_HIP_MAX_SH_REST + kShGroup - 1) / kShGroup;
+using gsh::kShGroup;
+using gsh::kShGroups;
 
 // basis_{K+1}(direction) of SH rest coefficient K (0..14): the terms of
 // sh_basis (gs_common.h), one at a time.
@@ -422,6 +422,15 @@ __global__ void __launch_bounds__(256) k_pose_grad(const gsh::SceneDev* __restri
 // against dcol, carried to dL/dposition through the direction's
 // normalisation; the latter summed over the views into dir_grad[group] for
 // k_adam, then Adam on the group's coefficients at the views' mean gradient.
+// f(integral_constant<int, D>) for D in [D0, N).
+template <int D0, int N, class F>
+__device__ __forceinline__ void unroll_const(F& f) {
+  if constexpr (D0 < N) {
+    f(std::integral_constant<int, D0>{});
+    unroll_const<D0 + 1, N>(f);
+  }
+}
+
 template <int kGroup>
 __device__ __forceinline__ void sh_group(const AdamArgs& a, const gs_hip_frame& f, uint32_t i, int coeffs) {
   constexpr int k0 = kGroup * kShGroup;
@@ -483,11 +492,7 @@ __device__ __forceinline__ void sh_group(const AdamArgs& a, const gs_hip_frame& 
       ddy += w * gy;
       ddz += w * gz;
     };
-    term(std::integral_constant<int, 0>{});
-    term(std::integral_constant<int, 1>{});
-    term(std::integral_constant<int, 2>{});
-    term(std::integral_constant<int, 3>{});
-    term(std::integral_constant<int, 4>{});
+    unroll_const<0, kShGroup>(term);
     const float dd = x * ddx + y * ddy + z * ddz;
     dpx += (ddx - x * dd) * il;
     dpy += (ddy - y * dd) * il;
@@ -517,11 +522,22 @@ __device__ __forceinline__ void sh_group(const AdamArgs& a, const gs_hip_frame& 
   }
 }
 
-// A block is the three groups of 64 slots, one wave each: every wave reads
-// its coefficient planes coalesced, and the views' per-slot gradients, clamp
-// bits and tile counts that all three read come from the CU's cache after the
-// first.
+// A block is the groups of 64 slots, one wave each: every wave reads its
+// coefficient planes coalesced, and the views' per-slot gradients, clamp
+// bits and tile counts that all of them read come from the CU's cache after
+// the first.
 constexpr int kShBlock = 64 * kShGroups;
+
+template <int G = 0>
+__device__ __forceinline__ void sh_group_of(int group, const AdamArgs& a, const gs_hip_frame& f, uint32_t i, int coeffs) {
+  if constexpr (G < kShGroups) {
+    if (group == G) {
+      sh_group<G>(a, f, i, coeffs);
+    } else {
+      sh_group_of<G + 1>(group, a, f, i, coeffs);
+    }
+  }
+}
 
 __global__ void __launch_bounds__(kShBlock) k_adam_sh(const gsh::SceneDev* __restrict__ scenes,
                                                       const gsh::ViewDev* __restrict__ views, uint32_t count) {
@@ -532,9 +548,7 @@ __global__ void __launch_bounds__(kShBlock) k_adam_sh(const gsh::SceneDev* __res
   const int coeffs = min(sh_rest_coeffs(f.sh_degree), int(a.sh_coeffs));
   if (group * kShGroup >= coeffs || i >= f.slots || skip_update(a)) return;
   if (gload(a.params + i * 4u + 1u).w < 0.5f) return;  // free slot
-  if (group == 0) sh_group<0>(a, f, i, coeffs);
-  else if (group == 1) sh_group<1>(a, f, i, coeffs);
-  else sh_group<2>(a, f, i, coeffs);
+  sh_group_of(group, a, f, i, coeffs);
 }
 
 // ---- densification ----------------------------------------------------------
