@@ -387,6 +387,7 @@ int upload_entries(gs_hip_workspace* ws, const std::vector<TableEntry>& entries,
     ws->tile_base[i] = shape.total_tiles;
     shape.total_tiles += v.tile_grid;
     shape.work_items += quad_items(e.frame->tiles_x, e.frame->tiles_y);
+    shape.max_quads = std::max(shape.max_quads, quad_items(e.frame->tiles_x, e.frame->tiles_y) / 4u);
     shape.max_tiles = std::max(shape.max_tiles, v.tile_grid);
     shape.max_bins = std::max(shape.max_bins, bins_along(e.frame->cam.width, shift) *
                                                   bins_along(e.frame->cam.height, shift));
@@ -507,7 +508,7 @@ int alloc_tables(gs_hip_workspace* ws, uint32_t views, uint32_t scenes, uint32_t
   A(alloc(&ws->dev_tile_scan, views));
   A(alloc(&ws->dev_scan_tmp, 1));
   A(alloc(&ws->work, std::max<uint32_t>(ws->work_entries, 1)));
-  A(alloc(&ws->work_counters, 16));
+  A(alloc(&ws->work_counters, kWorkCounters));
   A(alloc(&ws->totals_all, size_t(16) * views));
   A(alloc(&ws->tile_loss_all, size_t(std::max<uint32_t>(tile_entries, 1)) * 4));
   auto pinned = [&](void** p, size_t bytes) {
@@ -526,7 +527,7 @@ int alloc_tables(gs_hip_workspace* ws, uint32_t views, uint32_t scenes, uint32_t
     }
   }
   if (rc == 0) rc = fill_u32(ws, ws->totals_all, size_t(16) * views, 0);
-  if (rc == 0) rc = fill_u32(ws, ws->work_counters, 16, 0);
+  if (rc == 0) rc = fill_u32(ws, ws->work_counters, kWorkCounters, 0);
   return rc;
 }
 

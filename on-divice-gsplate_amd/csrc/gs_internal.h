@@ -148,14 +148,19 @@ struct ViewDev {
   // when the step re-walks); see bucket_slot.
   float4* bucket_state;
   uint64_t bucket_slots;
-  // The launch context's flag (k_work_order's): some view of the step found
+  // The launch context's flag (k_work_clear's): some view of the step found
   // more pairs than its buffers hold.
   const uint32_t* step_overflow;
 };
 
-// The launch context's work counter that k_work_order sets to the step's
-// pair-overflow flag ([0..1] raster work, [2..3] buckets).
+// The launch context's work counters: [0..1] raster work, [2..3] stored-state
+// buckets, [4] the step's pair-overflow flag (set by k_work_clear once the
+// pair counts are known), then the work order's count and taken places of
+// each of its kWorkBuckets list-length buckets.
 constexpr uint32_t kStepOverflowCounter = 4;
+constexpr uint32_t kWorkBuckets = 33;
+constexpr uint32_t kWorkBucketCounters = 16;
+constexpr uint32_t kWorkCounters = kWorkBucketCounters + 2 * kWorkBuckets;
 
 // Stored-state backward: list positions per bucket (one wave's splats) and
 // the pixels of a bucket slot (a 16x16 tile, quadrant-major: pixel 64 q + l
@@ -216,6 +221,7 @@ struct StepShape {
   uint32_t max_tiles = 0;          // the largest tile grid (per-tile grids)
   uint32_t total_tiles = 0;        // every view's tiles (per-tile loss entries)
   uint32_t work_items = 0;         // rasterizer work items: 4 per 2x2 quad of tiles of every view
+  uint32_t max_quads = 0;          // the most 2x2 quads of a view
   uint32_t max_bins = 0;           // the most pair bins of a view
   uint32_t tile_key_bits = 0;      // tile sort key width (bins)
   bool counting_tile_sort = false; // one counting pass (tuning.tile_digit_bits == 0 and the tile grids fit)
@@ -316,8 +322,8 @@ struct gs_hip_workspace {
   gsh::ScanSeg* dev_scan_tmp = nullptr;    // [1]: host-count scans (densify)
   uint32_t* work = nullptr;           // rasterizer work items (view << 16 | tile), longest lists first
   uint32_t work_entries = 0;
-  uint32_t* work_counters = nullptr;  // [0] forward, [1] backward (persistent rasterizers), [2] buckets listed, [3] taken,
-                                      // [4] the step's pair overflow
+  uint32_t* work_counters = nullptr;  // kWorkCounters: [0] forward, [1] backward (persistent rasterizers), [2] buckets
+                                      // listed, [3] taken, [4] the step's pair overflow, [16..] the work order's buckets
   gsh::BucketEntry* bucket_list = nullptr;  // stored-state backward: the step's buckets, listed by the forward
   uint64_t bucket_list_entries = 0;
   uint32_t* totals_all = nullptr;     // [table_views][16]

@@ -563,11 +563,19 @@ __global__ void __launch_bounds__(256) k_ranges(const ViewDev* __restrict__ view
 // in 2x2 quads of neighbouring tiles (four consecutive items, kNoWork for a
 // member outside the grid), the quads bucketed by floor(log2(their longest
 // list)), longest first, empty quads last (longest-processing-time-first
-// scheduling within a factor of two, across all views).  The rasterizers
-// hand a quad's four items to workgroups on one XCD, so the records the
-// neighbours share (a whole bin's list with pair bins wider than a tile)
-// are fetched from HBM once into that XCD's L2.  One workgroup; also resets
-// the persistent rasterizers' work counters.
+// scheduling within a factor of two, across all views; the order within a
+// bucket is whatever the atomics give).  The rasterizers hand a quad's four
+// items to workgroups on one XCD, so the records the neighbours share (a
+// whole bin's list with pair bins wider than a tile) are fetched from HBM
+// once into that XCD's L2.  Three launches, a thread per quad in the last
+// two:
+//   k_work_clear: the persistent rasterizers' work counters, the step's
+//                 pair-overflow flag and the buckets' counts;
+//   k_work_count: each bucket's quads over the step (per-block LDS counts,
+//                 one global atomic per block and bucket);
+//   k_work_place: each block takes its run of each bucket (the bucket's
+//                 start, the exclusive prefix of the counts, plus the runs
+//                 taken before it) and writes its quads' items there.
 __device__ __forceinline__ uint32_t quad_bucket(const ViewDev& view, uint32_t q) {
   const uint32_t tx = view.f.tiles_x, ty = view.f.tiles_y, qx = gsh::quads_x(tx);
   const uint32_t x0 = (q % qx) * 2u, y0 = (q / qx) * 2u;
@@ -575,68 +583,90 @@ __device__ __forceinline__ uint32_t quad_bucket(const ViewDev& view, uint32_t q)
   for (uint32_t m = 0; m < 4u; ++m) {
     const uint32_t x = x0 + (m & 1u), y = y0 + (m >> 1);
     if (x < tx && y < ty) {
-      const uint2 r = view.ranges[gsh::bin_of_tile(view, y * tx + x)];
+      const uint2 r = gload(view.ranges + gsh::bin_of_tile(view, y * tx + x));
       longest = max(longest, r.y - r.x);
     }
   }
   return __clz(longest);
 }
 
-__global__ void __launch_bounds__(1024) k_work_order(const ViewDev* __restrict__ views, uint32_t count,
-                                                     uint32_t* __restrict__ work, uint32_t* __restrict__ counters) {
-  __shared__ uint32_t s_count[33];
-  __shared__ uint32_t s_over;
-  if (threadIdx.x < 33) s_count[threadIdx.x] = 0u;
+__global__ void __launch_bounds__(64) k_work_clear(const ViewDev* __restrict__ views, uint32_t count,
+                                                   uint32_t* __restrict__ counters) {
   if (threadIdx.x < 4) counters[threadIdx.x] = 0u;
-  if (threadIdx.x == 0) s_over = 0u;
-  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < 2u * gsh::kWorkBuckets; i += 64u) counters[gsh::kWorkBucketCounters + i] = 0u;
   // Whether any view of the step found more pairs than its buffers hold: the
   // optimizer then leaves every model alone (one flag instead of every view's
   // count read by every optimizer thread).
-  for (uint32_t v = threadIdx.x; v < count; v += blockDim.x) {
-    if (uint64_t(views[v].totals[0]) > views[v].pair_capacity) atomicOr(&s_over, 1u);
+  bool over = false;
+  for (uint32_t v = threadIdx.x; v < count; v += 64u) {
+    over = over || uint64_t(views[v].totals[0]) > views[v].pair_capacity;
   }
-  // Most quads fall into a few buckets, so the lanes of a wave that share
-  // one add their count once (an LDS atomic from 64 lanes on one counter
-  // would take 64 passes).
+  const uint64_t any = __ballot(over);
+  if (threadIdx.x == 0) counters[gsh::kStepOverflowCounter] = any != 0ull ? 1u : 0u;
+}
+
+// This thread's quad's bucket and its rank among the block's quads of that
+// bucket, counted in s_count (lanes of a wave that share a bucket add their
+// count once: an LDS atomic from 64 lanes on one counter would take 64
+// passes).
+__device__ __forceinline__ uint32_t count_quad(uint32_t b, uint32_t* s_count) {
+  const uint64_t peers = match_digit(b, 6, __ballot(true));
   const uint64_t below_me = lanemask_lt();
-  for (uint32_t v = 0; v < count; ++v) {
-    const ViewDev& view = views[v];
-    const uint32_t quads = gsh::quad_items(view.f.tiles_x, view.f.tiles_y) / 4u;
-    for (uint32_t q = threadIdx.x; q < quads; q += blockDim.x) {
-      const uint32_t b = quad_bucket(view, q);
-      const uint64_t peers = match_digit(b, 6, __ballot(true));
-      if ((peers & below_me) == 0ull) atomicAdd(&s_count[b], uint32_t(__popcll(peers)));
-    }
+  const int leader = __ffsll(static_cast<long long>(peers)) - 1;
+  uint32_t first = 0u;
+  if ((peers & below_me) == 0ull) first = atomicAdd(&s_count[b], uint32_t(__popcll(peers)));
+  first = __shfl(first, leader, 64);
+  return first + uint32_t(__popcll(peers & below_me));
+}
+
+__global__ void __launch_bounds__(256) k_work_count(const ViewDev* __restrict__ views, uint32_t* __restrict__ counters) {
+  __shared__ uint32_t s_count[gsh::kWorkBuckets];
+  const ViewDev& view = views[blockIdx.y];
+  const uint32_t quads = gsh::quad_items(view.f.tiles_x, view.f.tiles_y) / 4u;
+  if (blockIdx.x * 256u >= quads) return;  // the whole block
+  if (threadIdx.x < gsh::kWorkBuckets) s_count[threadIdx.x] = 0u;
+  __syncthreads();
+  const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+  if (q < quads) count_quad(quad_bucket(view, q), s_count);
+  __syncthreads();
+  if (threadIdx.x < gsh::kWorkBuckets && s_count[threadIdx.x] != 0u) {
+    atomicAdd(counters + gsh::kWorkBucketCounters + threadIdx.x, s_count[threadIdx.x]);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_work_place(const ViewDev* __restrict__ views, uint32_t* __restrict__ work,
+                                                    uint32_t* __restrict__ counters) {
+  __shared__ uint32_t s_count[gsh::kWorkBuckets];
+  __shared__ uint32_t s_base[gsh::kWorkBuckets];
+  const uint32_t v = blockIdx.y;
+  const ViewDev& view = views[v];
+  const uint32_t tx = view.f.tiles_x, ty = view.f.tiles_y, qx = gsh::quads_x(tx);
+  const uint32_t quads = gsh::quad_items(tx, ty) / 4u;
+  if (blockIdx.x * 256u >= quads) return;  // the whole block
+  if (threadIdx.x < gsh::kWorkBuckets) s_count[threadIdx.x] = 0u;
+  __syncthreads();
+  const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+  uint32_t b = 0u, rank = 0u;
+  if (q < quads) {
+    b = quad_bucket(view, q);
+    rank = count_quad(b, s_count);
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    counters[gsh::kStepOverflowCounter] = s_over;
-    uint32_t sum = 0;
-    for (int b = 0; b < 33; ++b) {
-      const uint32_t c = s_count[b];
-      s_count[b] = sum;
-      sum += c;
-    }
+  if (threadIdx.x < gsh::kWorkBuckets) {
+    const uint32_t k = threadIdx.x;
+    const GS_GLOBAL uint32_t* totals = gmem(counters) + gsh::kWorkBucketCounters;
+    uint32_t start = 0u;
+    for (uint32_t j = 0; j < k; ++j) start += totals[j];
+    const uint32_t mine = s_count[k];
+    s_base[k] = mine != 0u ? start + atomicAdd(counters + gsh::kWorkBucketCounters + gsh::kWorkBuckets + k, mine) : 0u;
   }
   __syncthreads();
-  for (uint32_t v = 0; v < count; ++v) {
-    const ViewDev& view = views[v];
-    const uint32_t tx = view.f.tiles_x, ty = view.f.tiles_y, qx = gsh::quads_x(tx);
-    const uint32_t quads = gsh::quad_items(tx, ty) / 4u;
-    for (uint32_t q = threadIdx.x; q < quads; q += blockDim.x) {
-      const uint32_t b = quad_bucket(view, q);
-      const uint64_t peers = match_digit(b, 6, __ballot(true));
-      const int leader = __ffsll(static_cast<long long>(peers)) - 1;
-      uint32_t first = 0u;
-      if ((peers & below_me) == 0ull) first = atomicAdd(&s_count[b], uint32_t(__popcll(peers)));
-      first = __shfl(first, leader, 64);
-      const uint32_t at = 4u * (first + uint32_t(__popcll(peers & below_me)));
-      const uint32_t x0 = (q % qx) * 2u, y0 = (q / qx) * 2u;
-      for (uint32_t m = 0; m < 4u; ++m) {
-        const uint32_t x = x0 + (m & 1u), y = y0 + (m >> 1);
-        work[at + m] = x < tx && y < ty ? (v << gsh::kTileBits) | (y * tx + x) : gsh::kNoWork;
-      }
+  if (q < quads) {
+    const uint32_t at = 4u * (s_base[b] + rank);
+    const uint32_t x0 = (q % qx) * 2u, y0 = (q / qx) * 2u;
+    for (uint32_t m = 0; m < 4u; ++m) {
+      const uint32_t x = x0 + (m & 1u), y = y0 + (m >> 1);
+      work[at + m] = x < tx && y < ty ? (v << gsh::kTileBits) | (y * tx + x) : gsh::kNoWork;
     }
   }
 }
@@ -900,8 +930,13 @@ int launch_tile_sort_and_ranges(gs_hip_workspace* ws, uint32_t count) {
     k_ranges<<<dim3(std::min<uint32_t>(div_up(cap, 256), 2048), count), 256, 0, ws->stream>>>(ws->dev_views);
     GS_CHECK_LAUNCH("k_ranges");
   }
-  k_work_order<<<1, 1024, 0, ws->stream>>>(ws->dev_views, count, ws->work, ws->work_counters);
-  GS_CHECK_LAUNCH("k_work_order");
+  const dim3 quad_grid(std::max<uint32_t>(div_up(ws->shape.max_quads, 256), 1), count);
+  k_work_clear<<<1, 64, 0, ws->stream>>>(ws->dev_views, count, ws->work_counters);
+  GS_CHECK_LAUNCH("k_work_clear");
+  k_work_count<<<quad_grid, 256, 0, ws->stream>>>(ws->dev_views, ws->work_counters);
+  GS_CHECK_LAUNCH("k_work_count");
+  k_work_place<<<quad_grid, 256, 0, ws->stream>>>(ws->dev_views, ws->work, ws->work_counters);
+  GS_CHECK_LAUNCH("k_work_place");
   timer_end(ws);
   return 0;
 }
