@@ -148,16 +148,17 @@ struct ViewDev {
   // when the step re-walks); see bucket_slot.
   float4* bucket_state;
   uint64_t bucket_slots;
-  // The launch context's flag (k_work_clear's): some view of the step found
+  // The launch context's flag (k_work_count's): some view of the step found
   // more pairs than its buffers hold.
   const uint32_t* step_overflow;
 };
 
 // The launch context's work counters: [0..1] raster work, [2..3] stored-state
-// buckets, [4] the step's pair-overflow flag (set by k_work_clear once the
-// pair counts are known), then the work order's count and taken places of
-// each of its kWorkBuckets list-length buckets.
+// buckets, [4] the step's pair-overflow flag (set by k_work_count once the
+// pair counts are known), [5] the work order's finished blocks, then its
+// count and taken places of each of its kWorkBuckets list-length buckets.
 constexpr uint32_t kStepOverflowCounter = 4;
+constexpr uint32_t kWorkDoneCounter = 5;
 constexpr uint32_t kWorkBuckets = 33;
 constexpr uint32_t kWorkBucketCounters = 16;
 constexpr uint32_t kWorkCounters = kWorkBucketCounters + 2 * kWorkBuckets;

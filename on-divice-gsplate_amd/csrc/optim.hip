@@ -70,7 +70,7 @@ __device__ __forceinline__ AdamArgs scene_args(const gsh::SceneDev& sc, const gs
 // found more pairs than its buffers hold (its render was cut short; the host
 // runs the step again with larger buffers, trainer.cpp:819-836).
 __device__ __forceinline__ bool step_overflowed(const AdamArgs& a) {
-  // The step's flag (set by k_work_clear, which runs after the pair counts
+  // The step's flag (set by k_work_count, which runs after the pair counts
   // are known): one load instead of a dependent chain over every view.
   if (a.views[0].step_overflow != nullptr) return *gmem(a.views[0].step_overflow) != 0u;
   for (uint32_t v = 0; v < a.step_count; ++v) {

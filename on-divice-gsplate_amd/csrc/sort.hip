@@ -567,15 +567,16 @@ __global__ void __launch_bounds__(256) k_ranges(const ViewDev* __restrict__ view
 // bucket is whatever the atomics give).  The rasterizers hand a quad's four
 // items to workgroups on one XCD, so the records the neighbours share (a
 // whole bin's list with pair bins wider than a tile) are fetched from HBM
-// once into that XCD's L2.  Three launches, a thread per quad in the last
-// two:
-//   k_work_clear: the persistent rasterizers' work counters, the step's
-//                 pair-overflow flag and the buckets' counts;
+// once into that XCD's L2.  Two launches, a thread per quad:
 //   k_work_count: each bucket's quads over the step (per-block LDS counts,
-//                 one global atomic per block and bucket);
+//                 one global atomic per block and bucket); its first block
+//                 also clears the persistent rasterizers' work counters and
+//                 sets the step's pair-overflow flag;
 //   k_work_place: each block takes its run of each bucket (the bucket's
 //                 start, the exclusive prefix of the counts, plus the runs
-//                 taken before it) and writes its quads' items there.
+//                 taken before it) and writes its quads' items there; the
+//                 last block to finish clears the buckets' counters for the
+//                 next step (they are zero from the workspace's creation on).
 __device__ __forceinline__ uint32_t quad_bucket(const ViewDev& view, uint32_t q) {
   const uint32_t tx = view.f.tiles_x, ty = view.f.tiles_y, qx = gsh::quads_x(tx);
   const uint32_t x0 = (q % qx) * 2u, y0 = (q / qx) * 2u;
@@ -590,13 +591,13 @@ __device__ __forceinline__ uint32_t quad_bucket(const ViewDev& view, uint32_t q)
   return __clz(longest);
 }
 
-__global__ void __launch_bounds__(64) k_work_clear(const ViewDev* __restrict__ views, uint32_t count,
-                                                   uint32_t* __restrict__ counters) {
+// The first block's first wave: the persistent rasterizers' work counters,
+// and whether any view of the step found more pairs than its buffers hold
+// (the optimizer then leaves every model alone: one flag instead of every
+// view's count read by every optimizer thread).
+__device__ __forceinline__ void clear_step_counters(const ViewDev* __restrict__ views, uint32_t count,
+                                                    uint32_t* __restrict__ counters) {
   if (threadIdx.x < 4) counters[threadIdx.x] = 0u;
-  for (uint32_t i = threadIdx.x; i < 2u * gsh::kWorkBuckets; i += 64u) counters[gsh::kWorkBucketCounters + i] = 0u;
-  // Whether any view of the step found more pairs than its buffers hold: the
-  // optimizer then leaves every model alone (one flag instead of every view's
-  // count read by every optimizer thread).
   bool over = false;
   for (uint32_t v = threadIdx.x; v < count; v += 64u) {
     over = over || uint64_t(views[v].totals[0]) > views[v].pair_capacity;
@@ -619,8 +620,10 @@ __device__ __forceinline__ uint32_t count_quad(uint32_t b, uint32_t* s_count) {
   return first + uint32_t(__popcll(peers & below_me));
 }
 
-__global__ void __launch_bounds__(256) k_work_count(const ViewDev* __restrict__ views, uint32_t* __restrict__ counters) {
+__global__ void __launch_bounds__(256) k_work_count(const ViewDev* __restrict__ views, uint32_t count,
+                                                    uint32_t* __restrict__ counters) {
   __shared__ uint32_t s_count[gsh::kWorkBuckets];
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 64) clear_step_counters(views, count, counters);
   const ViewDev& view = views[blockIdx.y];
   const uint32_t quads = gsh::quad_items(view.f.tiles_x, view.f.tiles_y) / 4u;
   if (blockIdx.x * 256u >= quads) return;  // the whole block
@@ -635,14 +638,15 @@ __global__ void __launch_bounds__(256) k_work_count(const ViewDev* __restrict__ 
 }
 
 __global__ void __launch_bounds__(256) k_work_place(const ViewDev* __restrict__ views, uint32_t* __restrict__ work,
-                                                    uint32_t* __restrict__ counters) {
+                                                    uint32_t items, uint32_t* __restrict__ counters) {
   __shared__ uint32_t s_count[gsh::kWorkBuckets];
   __shared__ uint32_t s_base[gsh::kWorkBuckets];
+  __shared__ uint32_t s_last;
   const uint32_t v = blockIdx.y;
   const ViewDev& view = views[v];
   const uint32_t tx = view.f.tiles_x, ty = view.f.tiles_y, qx = gsh::quads_x(tx);
   const uint32_t quads = gsh::quad_items(tx, ty) / 4u;
-  if (blockIdx.x * 256u >= quads) return;  // the whole block
+  // Blocks past a view's quads place nothing but still count as finished.
   if (threadIdx.x < gsh::kWorkBuckets) s_count[threadIdx.x] = 0u;
   __syncthreads();
   const uint32_t q = blockIdx.x * 256u + threadIdx.x;
@@ -661,13 +665,28 @@ __global__ void __launch_bounds__(256) k_work_place(const ViewDev* __restrict__ 
     s_base[k] = mine != 0u ? start + atomicAdd(counters + gsh::kWorkBucketCounters + gsh::kWorkBuckets + k, mine) : 0u;
   }
   __syncthreads();
-  if (q < quads) {
+  // (A place past the list could only come from counts left by a step that
+  // stopped between the two launches; it is dropped, not written.)
+  if (q < quads && 4u * (s_base[b] + rank) + 4u <= items) {
     const uint32_t at = 4u * (s_base[b] + rank);
     const uint32_t x0 = (q % qx) * 2u, y0 = (q / qx) * 2u;
     for (uint32_t m = 0; m < 4u; ++m) {
       const uint32_t x = x0 + (m & 1u), y = y0 + (m >> 1);
       work[at + m] = x < tx && y < ty ? (v << gsh::kTileBits) | (y * tx + x) : gsh::kNoWork;
     }
+  }
+  // Every block has read the counts and taken its runs before it counts
+  // itself finished, so the last one may clear them.
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const uint32_t done = atomicAdd(counters + gsh::kWorkDoneCounter, 1u);
+    s_last = done + 1u == gridDim.x * gridDim.y ? 1u : 0u;
+  }
+  __syncthreads();
+  if (s_last != 0u) {
+    __threadfence();
+    for (uint32_t i = threadIdx.x; i < 2u * gsh::kWorkBuckets; i += 256u) counters[gsh::kWorkBucketCounters + i] = 0u;
+    if (threadIdx.x == 0) counters[gsh::kWorkDoneCounter] = 0u;
   }
 }
 
@@ -931,11 +950,9 @@ int launch_tile_sort_and_ranges(gs_hip_workspace* ws, uint32_t count) {
     GS_CHECK_LAUNCH("k_ranges");
   }
   const dim3 quad_grid(std::max<uint32_t>(div_up(ws->shape.max_quads, 256), 1), count);
-  k_work_clear<<<1, 64, 0, ws->stream>>>(ws->dev_views, count, ws->work_counters);
-  GS_CHECK_LAUNCH("k_work_clear");
-  k_work_count<<<quad_grid, 256, 0, ws->stream>>>(ws->dev_views, ws->work_counters);
+  k_work_count<<<quad_grid, 256, 0, ws->stream>>>(ws->dev_views, count, ws->work_counters);
   GS_CHECK_LAUNCH("k_work_count");
-  k_work_place<<<quad_grid, 256, 0, ws->stream>>>(ws->dev_views, ws->work, ws->work_counters);
+  k_work_place<<<quad_grid, 256, 0, ws->stream>>>(ws->dev_views, ws->work, ws->shape.work_items, ws->work_counters);
   GS_CHECK_LAUNCH("k_work_place");
   timer_end(ws);
   return 0;
