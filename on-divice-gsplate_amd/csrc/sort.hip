@@ -71,7 +71,11 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t digit, int nbits, uint6
 // lanes on one counter would take 64 passes).
 __global__ void __launch_bounds__(kSortThreads) k_radix_upsweep(const SortSeg* __restrict__ segs, int shift,
                                                                 int nbits, int from_alt) {
-  __shared__ uint32_t h[kWaves][256];
+  // Four counters per digit and wave, one per lane mod 4 (rows of 257
+  // words: a digit's four counters sit in four banks), so lanes of one
+  // atomic instruction that share a digit mostly add to different words.
+  constexpr int kSplit = 4, kRow = 257;
+  __shared__ uint32_t h[kWaves][kSplit * kRow];
   const SortSeg& g = segs[blockIdx.y];
   if (blockIdx.x >= g.blocks) return;  // a smaller segment of the launch
   const uint32_t n = seg_count(g.count, g.count_max);
@@ -82,8 +86,9 @@ __global__ void __launch_bounds__(kSortThreads) k_radix_upsweep(const SortSeg* _
   }
   const GS_GLOBAL uint32_t* __restrict__ keys = gmem(from_alt ? g.keys_alt : g.keys);
   const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
-  for (int d = threadIdx.x; d < kWaves * 256; d += kSortThreads) (&h[0][0])[d] = 0u;
+  for (int d = threadIdx.x; d < kWaves * kSplit * kRow; d += kSortThreads) (&h[0][0])[d] = 0u;
   __syncthreads();
+  uint32_t* hw = &h[wave][(lane & (kSplit - 1)) * kRow];
   const uint32_t wbase = blockIdx.x * kSortItems + wave * (kSortItems / kWaves);
   uint32_t kv[kSortRounds];  // every round's key requested before any is counted
 #pragma unroll
@@ -99,15 +104,16 @@ __global__ void __launch_bounds__(kSortThreads) k_radix_upsweep(const SortSeg* _
     if (active == 0ull) continue;
     const uint32_t first = __builtin_amdgcn_readfirstlane(valid ? digit : 0xffffffffu);
     if (__ballot(valid && digit == first) == active) {
-      if (lane == __ffsll(static_cast<long long>(active)) - 1) atomicAdd(&h[wave][first], uint32_t(__popcll(active)));
+      if (lane == __ffsll(static_cast<long long>(active)) - 1) atomicAdd(&hw[first], uint32_t(__popcll(active)));
     } else if (valid) {
-      atomicAdd(&h[wave][digit], 1u);
+      atomicAdd(&hw[digit], 1u);
     }
   }
   __syncthreads();
   for (uint32_t d = threadIdx.x; d < radix; d += kSortThreads) {
     uint32_t s = 0;
-    for (int w = 0; w < kWaves; ++w) s += h[w][d];
+    for (int w = 0; w < kWaves; ++w)
+      for (int c = 0; c < kSplit; ++c) s += h[w][c * kRow + d];
     gmem(g.hist)[size_t(d) * g.blocks + blockIdx.x] = s;
   }
 }
